@@ -1,0 +1,282 @@
+// Periodic regularized E-field, direct sum, FP64 (gfx950).
+//
+// Replaces E_MQ_DirectSum::operator() (reference src/FieldStructure.cpp:
+// 106-156). With z = (x_i - x_j)/L wrapped to [-1/2, 1/2] and delta = eps/L
+// the reference term
+//     w_j * (-z L) * (1/sqrt(L^2 + 4 eps^2) - 1/sqrt(4 z^2 L^2 + 4 eps^2))
+// equals (w_j/2) * z * (1/sqrt(z^2 + delta^2) - C), C = 2/sqrt(1 + 4 delta^2),
+// which is what the kernel evaluates: positions are pre-scaled by 1/L and the
+// weights by 1/2 once per call, so a pair costs one subtract, a round-to-
+// nearest-even, a subtract, an FMA, a refined reciprocal square root and two
+// more FMA-class operations. C is computed on the device with the same
+// refined rsqrt, so z = +-1/2 gives exactly 0 like the reference's skip; z = 0
+// gives 0 through the factor z.
+//
+// Mapping to CDNA4: the source loop index is uniform across a wavefront, so
+// x_j and w_j are scalar loads into SGPRs feeding every lane's VALU operation
+// directly (no LDS traffic, no bank conflicts; the scalar cache serves the
+// whole wavefront). Each lane owns TPT targets for instruction-level
+// parallelism. When there are too few targets to fill 256 CUs the source
+// range is split across blockIdx.y and the per-split partials are summed by a
+// second kernel in split order, so results are deterministic and
+// independent of co-scheduled work. Accumulation: a plain FP64 sum over each
+// run of kSub sources, folded into a TwoSum (hi, lo) pair per target.
+#include "fs_internal.h"
+
+namespace fs {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSub = 256;
+
+// 1/sqrt(a), a > 0: hardware estimate (v_rsq_f64, relative error about
+// 2^-23) plus one third-order correction y0 (1 + e/2 + 3e^2/8), e = 1 - a y0^2,
+// which leaves a truncation error of order e^3 below FP64 rounding.
+__device__ __forceinline__ double rsq_f64(double a) {
+    const double y0 = __builtin_amdgcn_rsq(a);
+    const double t = a * y0;
+    const double e = __builtin_fma(-t, y0, 1.0);
+    const double r = y0 * e;
+    const double p = __builtin_fma(e, 0.375, 0.5);
+    return __builtin_fma(r, p, y0);
+}
+
+// s + lo += b, error-free (Knuth TwoSum).
+__device__ __forceinline__ void two_sum_acc(double& s, double& lo, double b) {
+    const double t = s + b;
+    const double bp = t - s;
+    const double ap = t - bp;
+    lo += (s - ap) + (b - bp);
+    s = t;
+}
+
+template <int TPT>
+__global__ __launch_bounds__(kBlock) void emq_direct_f64_kernel(
+    const double* __restrict__ tgt_s, int nt, const double* __restrict__ src_s,
+    const double* __restrict__ src_W, int ns, int chunk, double delta2,
+    double* __restrict__ out) {
+    const int split = blockIdx.y;
+    const int j0 = split * chunk;
+    const int j1 = min(ns, j0 + chunk);
+    const int i0 = blockIdx.x * (kBlock * TPT) + threadIdx.x;
+    double xi[TPT], hi[TPT], lo[TPT];
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        const int i = i0 + t * kBlock;
+        xi[t] = (i < nt) ? tgt_s[i] : 0.0;
+        hi[t] = 0.0;
+        lo[t] = 0.0;
+    }
+    const double C = rsq_f64(__builtin_fma(0.5, 0.5, delta2));
+    for (int jb = j0; jb < j1; jb += kSub) {
+        const int je = min(j1, jb + kSub);
+        double acc[TPT];
+#pragma unroll
+        for (int t = 0; t < TPT; ++t) acc[t] = 0.0;
+#pragma unroll 4
+        for (int j = jb; j < je; ++j) {
+            const double xj = src_s[j];
+            const double Wj = src_W[j];
+#pragma unroll
+            for (int t = 0; t < TPT; ++t) {
+                const double d = xi[t] - xj;
+                const double z = d - __builtin_rint(d);
+                const double y = rsq_f64(__builtin_fma(z, z, delta2));
+                acc[t] = __builtin_fma(Wj * z, y - C, acc[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < TPT; ++t) two_sum_acc(hi[t], lo[t], acc[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        const int i = i0 + t * kBlock;
+        if (i < nt) out[(size_t)split * nt + i] = hi[t] + lo[t];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void reduce_splits_kernel(
+    const double* __restrict__ part, int nt, int splits, double* __restrict__ es) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nt) return;
+    double s = 0.0, lo = 0.0;
+    for (int k = 0; k < splits; ++k) two_sum_acc(s, lo, part[(size_t)k * nt + i]);
+    es[i] = s + lo;
+}
+
+__global__ __launch_bounds__(kBlock) void scale_points_kernel(
+    const double* __restrict__ x, const double* __restrict__ w, int n, double invL,
+    double* __restrict__ xs, double* __restrict__ W) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    xs[i] = x[i] * invL;
+    if (w) W[i] = 0.5 * w[i];
+}
+
+__global__ __launch_bounds__(kBlock) void fill_kernel(double* __restrict__ a, int n, double v) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+struct Plan {
+    int tpt;
+    int splits;
+    int chunk;
+};
+
+Plan plan_direct(int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun) {
+    Plan p{};
+    // Aim for >= 16 wavefronts per CU (4 per SIMD) so FP64 latency is hidden.
+    const int64_t want_waves = int64_t(16) * num_cus;
+    if (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
+        tun.targets_per_thread == 4) {
+        p.tpt = tun.targets_per_thread;
+    } else {
+        p.tpt = (nt >= want_waves * 64 * 2) ? 2 : 1;
+    }
+    const int64_t waves = (nt + 64 * p.tpt - 1) / (64 * p.tpt);
+    int64_t splits = tun.source_splits > 0 ? tun.source_splits
+                                           : (want_waves + waves - 1) / std::max<int64_t>(waves, 1);
+    const int64_t max_splits = std::max<int64_t>(1, ns / kSub);
+    splits = std::min(std::max<int64_t>(splits, 1), max_splits);
+    splits = std::min<int64_t>(splits, 65535);
+    int64_t chunk = (ns + splits - 1) / splits;
+    chunk = (chunk + kSub - 1) / kSub * kSub;  // split boundaries on sub-sum boundaries
+    splits = std::max<int64_t>(1, (ns + chunk - 1) / chunk);
+    p.splits = (int)splits;
+    p.chunk = (int)chunk;
+    return p;
+}
+
+}  // namespace
+
+size_t field_scratch_bytes(int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun) {
+    const Plan p = plan_direct(nt, ns, num_cus, tun);
+    size_t b = 0;
+    b += align_up(sizeof(double) * nt, 256);             // scaled targets
+    b += align_up(sizeof(double) * ns, 256) * 2;         // scaled sources, weights
+    if (p.splits > 1) b += align_up(sizeof(double) * nt * p.splits, 256);
+    return b;
+}
+
+int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src, const double* d_w,
+                     int64_t ns, double L, double eps, double* d_es, void* scratch,
+                     int num_cus, const FieldTuning& tun, hipStream_t stream) {
+    if (nt < 0 || ns < 0 || nt > INT32_MAX || ns > INT32_MAX)
+        return fail(FS_E_INVALID, "direct sum: sizes nt=%lld ns=%lld out of range",
+                    (long long)nt, (long long)ns);
+    if (!(L > 0)) return fail(FS_E_INVALID, "direct sum: L must be > 0");
+    if (nt == 0) return FS_OK;
+    if (!d_tgt || !d_es || (ns > 0 && (!d_src || !d_w)))
+        return fail(FS_E_INVALID, "direct sum: null pointer");
+    const int nti = (int)nt, nsi = (int)ns;
+    if (ns == 0) {
+        fill_kernel<<<(nti + kBlock - 1) / kBlock, kBlock, 0, stream>>>(d_es, nti, 0.0);
+        FS_HIP_CHECK(hipGetLastError());
+        return FS_OK;
+    }
+    const Plan p = plan_direct(nt, ns, num_cus, tun);
+    char* s = static_cast<char*>(scratch);
+    double* tgt_s = reinterpret_cast<double*>(s);
+    s += align_up(sizeof(double) * nt, 256);
+    double* src_s = reinterpret_cast<double*>(s);
+    s += align_up(sizeof(double) * ns, 256);
+    double* src_W = reinterpret_cast<double*>(s);
+    s += align_up(sizeof(double) * ns, 256);
+    double* part = reinterpret_cast<double*>(s);
+
+    const double invL = 1.0 / L;
+    const double delta = eps / L;
+    const double delta2 = delta * delta;
+    scale_points_kernel<<<(nsi + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
+        d_src, d_w, nsi, invL, src_s, src_W);
+    const double* tgt_use = src_s;
+    if (d_tgt != d_src || nt != ns) {
+        scale_points_kernel<<<(nti + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
+            d_tgt, nullptr, nti, invL, tgt_s, nullptr);
+        tgt_use = tgt_s;
+    }
+    double* out = (p.splits > 1) ? part : d_es;
+    const dim3 grid((unsigned)((nt + (int64_t)kBlock * p.tpt - 1) / ((int64_t)kBlock * p.tpt)),
+                    (unsigned)p.splits);
+    switch (p.tpt) {
+        case 4:
+            emq_direct_f64_kernel<4><<<grid, kBlock, 0, stream>>>(tgt_use, nti, src_s, src_W,
+                                                                  nsi, p.chunk, delta2, out);
+            break;
+        case 2:
+            emq_direct_f64_kernel<2><<<grid, kBlock, 0, stream>>>(tgt_use, nti, src_s, src_W,
+                                                                  nsi, p.chunk, delta2, out);
+            break;
+        default:
+            emq_direct_f64_kernel<1><<<grid, kBlock, 0, stream>>>(tgt_use, nti, src_s, src_W,
+                                                                  nsi, p.chunk, delta2, out);
+            break;
+    }
+    FS_HIP_CHECK(hipGetLastError());
+    if (p.splits > 1) {
+        reduce_splits_kernel<<<(nti + kBlock - 1) / kBlock, kBlock, 0, stream>>>(part, nti,
+                                                                               p.splits, d_es);
+        FS_HIP_CHECK(hipGetLastError());
+    }
+    return FS_OK;
+}
+
+}  // namespace fs
+
+extern "C" {
+
+int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits) {
+    if (!ctx) return fs::fail(FS_E_INVALID, "null context");
+    if (targets_per_thread != 0 && targets_per_thread != 1 && targets_per_thread != 2 &&
+        targets_per_thread != 4)
+        return fs::fail(FS_E_INVALID, "targets_per_thread must be 0, 1, 2 or 4");
+    if (source_splits < 0) return fs::fail(FS_E_INVALID, "source_splits must be >= 0");
+    ctx->tuning.targets_per_thread = targets_per_thread;
+    ctx->tuning.source_splits = source_splits;
+    return FS_OK;
+}
+
+int fs_emq_periodic_f64_dev(fs_ctx* ctx, double* d_es, const double* d_targets, int64_t nt,
+                            const double* d_sources, const double* d_q_ws, int64_t ns,
+                            double L, double epsilon, void* stream) {
+    if (!ctx) return fs::fail(FS_E_INVALID, "null context");
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    const size_t need = fs::field_scratch_bytes(nt, ns, ctx->num_cus, ctx->tuning);
+    FS_TRY(ctx->scratch.ensure(need, st));
+    return fs::field_direct_f64(d_targets, nt, d_sources, d_q_ws, ns, L, epsilon, d_es,
+                                ctx->scratch.ptr, ctx->num_cus, ctx->tuning, st);
+}
+
+int fs_emq_periodic_f64(fs_ctx* ctx, double* es, const double* targets, int64_t nt,
+                        const double* sources, const double* q_ws, int64_t ns, double L,
+                        double epsilon) {
+    if (!ctx) return fs::fail(FS_E_INVALID, "null context");
+    if (nt < 0 || ns < 0) return fs::fail(FS_E_INVALID, "negative size");
+    if (nt == 0) return FS_OK;
+    if (!es || !targets || (ns > 0 && (!sources || !q_ws)))
+        return fs::fail(FS_E_INVALID, "null pointer");
+    FS_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t bt = fs::align_up(sizeof(double) * nt, 256);
+    const size_t bs = fs::align_up(sizeof(double) * ns, 256);
+    FS_TRY(ctx->io.ensure(2 * bt + 2 * bs, ctx->stream));
+    double* d_t = ctx->io.as<double>(0);
+    double* d_e = ctx->io.as<double>(bt);
+    double* d_s = ctx->io.as<double>(2 * bt);
+    double* d_w = ctx->io.as<double>(2 * bt + bs);
+    FS_HIP_CHECK(hipMemcpyAsync(d_t, targets, sizeof(double) * nt, hipMemcpyHostToDevice,
+                                ctx->stream));
+    if (ns > 0) {
+        FS_HIP_CHECK(hipMemcpyAsync(d_s, sources, sizeof(double) * ns, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        FS_HIP_CHECK(hipMemcpyAsync(d_w, q_ws, sizeof(double) * ns, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    }
+    FS_TRY(fs_emq_periodic_f64_dev(ctx, d_e, d_t, nt, d_s, d_w, ns, L, epsilon, ctx->stream));
+    FS_HIP_CHECK(hipMemcpyAsync(es, d_e, sizeof(double) * nt, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    FS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return FS_OK;
+}
+
+}  // extern "C"

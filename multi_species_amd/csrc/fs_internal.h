@@ -1,0 +1,69 @@
+// Internal declarations shared by the HIP translation units of
+// libfarrsight_hip.so. Nothing here is part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/farrsight_hip.h"
+
+namespace fs {
+
+// Records a message for fs_last_error() on this thread and returns `code`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define FS_HIP_CHECK(call)                                                        \
+    do {                                                                          \
+        hipError_t fs_e_ = (call);                                                \
+        if (fs_e_ != hipSuccess)                                                  \
+            return ::fs::fail(FS_E_HIP, "%s failed: %s (%s:%d)", #call,           \
+                              hipGetErrorString(fs_e_), __FILE__, __LINE__);      \
+    } while (0)
+
+#define FS_TRY(expr)                 \
+    do {                             \
+        int fs_rc_ = (expr);         \
+        if (fs_rc_ != FS_OK) return fs_rc_; \
+    } while (0)
+
+// Grow-only device allocation owned by a context. Growing first waits for the
+// owning stream so no enqueued kernel still reads the old block.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes, hipStream_t stream);
+    void release();
+    template <class T>
+    T* as(size_t byte_offset = 0) const {
+        return reinterpret_cast<T*>(static_cast<char*>(ptr) + byte_offset);
+    }
+};
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Field-solver launch state kept per context.
+struct FieldTuning {
+    int targets_per_thread = 0;  // 0 = automatic
+    int source_splits = 0;       // 0 = automatic
+};
+
+// Periodic direct sum on device arrays (FP64, compensated). `scratch` must
+// hold field_scratch_bytes(nt, ns, ...) bytes. Enqueued on `stream`.
+size_t field_scratch_bytes(int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun);
+int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src,
+                     const double* d_w, int64_t ns, double L, double eps,
+                     double* d_es, void* scratch, int num_cus,
+                     const FieldTuning& tun, hipStream_t stream);
+
+}  // namespace fs
+
+struct fs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 256;
+    fs::DevBuf scratch;   // kernel work space
+    fs::DevBuf io;        // staging for the host-pointer entry points
+    fs::FieldTuning tuning;
+};

@@ -1,0 +1,134 @@
+/* Test infrastructure only: C API of the FP64 CPU oracle (oracle/liboracle.so).
+ *
+ * The oracle restates the reference algorithm of FARRSIGHT (shujhan/
+ * multi_species) for the hot path: the periodic regularized E-field (direct
+ * sum and barycentric treecode), the panel mesh with adaptive refinement, the
+ * remesh interpolation, the reduced-x field utilities, RK4 and the step loop.
+ * Each function cites the reference file:line it follows. Only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it, as the
+ * checker; the product (multi_species_amd/) never links it.
+ */
+#ifndef FS_ORACLE_H
+#define FS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- field (part A) ---------------------------------------------------- */
+
+/* E_MQ_DirectSum::operator() (src/FieldStructure.cpp:106-156), same
+ * expression order, OpenMP over targets. */
+int orc_direct_sum(double* es, const double* targets, int64_t nt,
+                   const double* sources, const double* q_ws, int64_t ns,
+                   double L, double epsilon);
+
+/* The same sum evaluated in x87 extended precision (64-bit significand) with
+ * compensated accumulation: the "exact" value the FP64 sums are measured
+ * against. */
+int orc_direct_sum_ext(double* es, const double* targets, int64_t nt,
+                       const double* sources, const double* q_ws, int64_t ns,
+                       double L, double epsilon);
+
+/* E_MQ_Treecode::operator() (src/FieldStructure.cpp:230-1020): evaluates at
+ * the sources, writes es[0:ns]. */
+int orc_treecode(double* es, const double* sources, const double* q_ws,
+                 int64_t ns, double L, double epsilon, double mac, int degree,
+                 int max_source);
+
+/* Tree statistics of the last orc_treecode call on this thread:
+ * out[0]=node count, out[1]=leaf count, out[2]=far kernel evaluations,
+ * out[3]=near kernel evaluations. */
+void orc_treecode_last_stats(int64_t out[4]);
+
+/* ---- decks --------------------------------------------------------------- */
+
+typedef struct orc_species_params {
+    double q, m;
+    double p_min, p_max;
+    int initial_height, p_height, max_height;
+    int do_adaptively_refine;
+    int n_amr_epsilons;
+    double amr_epsilons[5];
+    int ics_type;               /* AMRSimulation_load.cpp:149-221 (+11 bump-on-tail) */
+    double normalized_wavenumber, amp, pth, pstr;
+    double beam_frac, beam_v, beam_vth;  /* ics_type 11 only */
+} orc_species_params;
+
+typedef struct orc_sim_params {
+    double x_min, x_max;
+    int bcs;                    /* 0 periodic, 1 open */
+    int quadrature;             /* 0 trap, 1 Simpson */
+    double greens_epsilon;
+    int use_treecode;
+    double mac;
+    int degree, max_source;
+    double dt;
+    int remesh_period;
+} orc_sim_params;
+
+/* ---- one species' mesh (part B) ------------------------------------------ */
+
+typedef struct orc_species orc_species;
+
+/* AMRStructure ctor (src/structure_files/AMRStructure.cpp:91-126): builds the
+ * initial mesh from f0. */
+orc_species* orc_species_create(const orc_sim_params* sim,
+                                const orc_species_params* sp);
+void orc_species_destroy(orc_species* s);
+int64_t orc_species_num_points(const orc_species* s);
+int64_t orc_species_num_panels(const orc_species* s);
+/* Copies out the point arrays (any pointer may be NULL). */
+void orc_species_get_points(const orc_species* s, double* xs, double* ps,
+                            double* fs, double* q_ws);
+/* Overwrites xs/ps (the advected positions, same count). */
+int orc_species_set_points(orc_species* s, const double* xs, const double* ps);
+/* Panel records, ORC_PANEL_INTS int32 per panel:
+ * [panel_ind, pt0..pt8, level, parent, which_child, left, top, right, bottom,
+ *  flags(bit0 left_bdry, bit1 right_bdry, bit2 needs_refinement,
+ *        bit3 refined_xp, bit4 refined_p), child_inds_start, 0] */
+#define ORC_PANEL_INTS 20
+void orc_species_get_panels(const orc_species* s, int32_t* rec);
+int64_t orc_species_num_leaves(const orc_species* s);
+void orc_species_get_leaves(const orc_species* s, int32_t* leaf_inds);
+double orc_species_f_beyond_boundary(const orc_species* s);
+/* AMRStructure::remesh (AMRStructure_amr.cpp:1156-1202). Returns 0, or -1 if
+ * the reference would have thrown ("too many shifts!"). */
+int orc_species_remesh(orc_species* s);
+/* Freeze the current mesh as the "old" mesh (AMRStructure_amr.cpp:1167-1175)
+ * and evaluate interpolate_from_mesh (AMRStructure_interp.cpp:1094-1143) at
+ * n points. */
+int orc_species_copy_to_old(orc_species* s);
+int orc_species_interp_from_mesh(orc_species* s, const double* x,
+                                 const double* p, int64_t n, double* f_out);
+/* interpolate_to_initial_xps (AMRStructure_interp.cpp:493-793) of the old
+ * mesh onto arbitrary base-grid points (n = nx*np); also returns the leaf
+ * each sorted point was assigned (0 = beyond boundary) in sorted order and
+ * the sort permutation. */
+int orc_species_interp_to_grid(orc_species* s, const double* xs,
+                               const double* ps, int nx, int np, double* f_out,
+                               int32_t* leaf_sorted, int32_t* sort_perm);
+
+/* ---- whole simulation -------------------------------------------------- */
+
+typedef struct orc_sim orc_sim;
+orc_sim* orc_sim_create(const orc_sim_params* sim,
+                        const orc_species_params* sp, int n_species);
+void orc_sim_destroy(orc_sim* s);
+/* AMRSimulation::step (src/simulation_files/AMRSimulation_step.cpp:5-99). */
+int orc_sim_step(orc_sim* s);
+int orc_sim_num_species(const orc_sim* s);
+orc_species* orc_sim_species(orc_sim* s, int i);
+/* Per-species E at the species' points (after the step's last evaluation). */
+void orc_sim_get_es(orc_sim* s, int i, double* es);
+/* Per-species diagnostics (standard_interface/diagnostic_plot.py:116-131):
+ * out[0]=charge, [1]=momentum, [2]=kinetic energy, [3]=electric energy. */
+void orc_sim_diagnostics(orc_sim* s, int i, double out[4]);
+int orc_sim_iter(const orc_sim* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

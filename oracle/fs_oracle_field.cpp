@@ -1,0 +1,314 @@
+// Test infrastructure only: FP64 CPU restatement of the reference's periodic
+// field solvers (src/FieldStructure.cpp). Checked against the reference's
+// own compiled solver (oracle/_ref) on seeded inputs, against closed forms and
+// against the extended-precision sum below (tests/test_oracle_field.py).
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fs_oracle_internal.h"
+
+namespace orc {
+
+// src/FieldStructure.cpp:106-156. Same association of every product and sum
+// as the reference expression, `round` for the periodic wrap, the three
+// DBL_MIN skips and a sequential inner sum per target.
+void direct_sum(double* es, const double* targets, int64_t nt,
+                const double* sources, const double* q_ws, int64_t ns,
+                double L, double epsilon) {
+    const double norm_const = 1 / std::sqrt(L * L + 4 * epsilon * epsilon);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nt; ++i) {
+        const double xi = targets[i];
+        double ei = 0.0;
+        for (int64_t j = 0; j < ns; ++j) {
+            double z = (xi - sources[j]) / L;
+            z = z - std::round(z);
+            if (std::fabs(z - 0.5) < DBL_MIN || std::fabs(z + 0.5) < DBL_MIN ||
+                std::fabs(z) < DBL_MIN)
+                continue;
+            ei += q_ws[j] * -1.0 * z * L *
+                  (norm_const - 1 / std::sqrt(4 * z * z * L * L +
+                                              4 * epsilon * epsilon));
+        }
+        es[i] = ei;
+    }
+}
+
+namespace {
+
+// Treecode node (src/FieldStructure.hpp:26-40).
+struct TNode {
+    size_t m0 = 0, m1 = 0;
+    double x0 = 0, x1 = 0, xc = 0;
+    std::vector<size_t> children;
+};
+
+thread_local int64_t g_tc_stats[4] = {0, 0, 0, 0};
+
+}  // namespace
+
+// src/FieldStructure.cpp:230-1050 without the heap bookkeeping (and without
+// its double frees). Evaluates at the sources.
+void treecode(double* es, const double* sources, const double* q_ws,
+              int64_t ns_, double L, double epsilon, double mac, int degree,
+              int max_source) {
+    const size_t n = (size_t)ns_;
+    std::vector<double> px(n), lambda(n);
+    for (size_t i = 0; i < n; ++i) {            // :240-247
+        px[i] = std::fmod(sources[i], L);
+        lambda[i] = q_ws[i];
+    }
+    const int P = degree, PP = P + 1;            // :249-256
+    const double sq_theta = mac * mac;
+    const double deno = L * L + 4 * epsilon * epsilon;
+    const size_t N0 = (size_t)max_source;
+
+    // minval/maxval with their paddings (:1032-1050)
+    double mn = px[0], mx = px[0];
+    for (size_t i = 1; i < n; ++i) {
+        if (mn > px[i]) mn = px[i];
+        if (mx < px[i]) mx = px[i];
+    }
+    mn -= 0.1234;
+    mx += 0.001;
+
+    std::vector<int> old_index(n);
+    for (size_t i = 0; i < n; ++i) old_index[i] = (int)i;
+
+    // build_tree_init (:506-527)
+    std::vector<TNode> tree(1);
+    tree[0].m0 = 0;
+    tree[0].m1 = n - 1;
+    tree[0].x0 = mn;
+    tree[0].x1 = mx;
+    tree[0].xc = 0.5 * (mn + mx);
+    std::vector<size_t> leaves;
+
+    // split_2 (:555-632): midpoint bisection, x <= mid goes to child 0, the
+    // address-table swaps amount to a stable partition of x, lambda and the
+    // original indices; empty children are dropped.
+    std::function<void(size_t)> build = [&](size_t node) {   // :530-545
+        const size_t cnt = tree[node].m1 - tree[node].m0 + 1;
+        if (cnt >= N0) {
+            const double a = tree[node].x0, b = tree[node].x1;
+            const double mid = 0.5 * (a + b);
+            const size_t s = tree[node].m0, e = tree[node].m1;
+            std::vector<size_t> side[2];
+            for (size_t k = s; k <= e; ++k) side[px[k] <= mid ? 0 : 1].push_back(k);
+            std::vector<double> nx, nl;
+            std::vector<int> no;
+            nx.reserve(cnt);
+            nl.reserve(cnt);
+            no.reserve(cnt);
+            for (int c = 0; c < 2; ++c)
+                for (size_t k : side[c]) {
+                    nx.push_back(px[k]);
+                    nl.push_back(lambda[k]);
+                    no.push_back(old_index[k]);
+                }
+            std::copy(nx.begin(), nx.end(), px.begin() + s);
+            std::copy(nl.begin(), nl.end(), lambda.begin() + s);
+            std::copy(no.begin(), no.end(), old_index.begin() + s);
+            size_t seq = s;
+            for (int c = 0; c < 2; ++c) {
+                if (side[c].empty()) continue;
+                TNode ch;
+                ch.x0 = (c == 0) ? a : mid;
+                ch.x1 = (c == 0) ? mid : b;
+                ch.xc = 0.5 * (ch.x0 + ch.x1);
+                ch.m0 = seq;
+                ch.m1 = seq + side[c].size() - 1;
+                seq += side[c].size();
+                tree[node].children.push_back(tree.size());
+                tree.push_back(ch);
+            }
+            const std::vector<size_t> kids = tree[node].children;
+            for (size_t k : kids) build(k);
+        } else {
+            leaves.push_back(node);
+        }
+    };
+    build(0);
+
+    // build_interaction_list (:663-702)
+    const size_t nleaf = leaves.size();
+    std::vector<std::vector<size_t>> far(nleaf), near(nleaf);
+    std::function<void(size_t, size_t)> lists = [&](size_t li, size_t node) {
+        const double xcb = tree[leaves[li]].xc, xcp = tree[node].xc;
+        const double x_vb = tree[leaves[li]].x1, x_vp = tree[node].x1;
+        const double r_dist = std::fabs(xcb - xcp);
+        const double dfbpx = std::fmin(r_dist, L - r_dist);
+        const double dfbx = x_vb - xcb, dfpx = x_vp - xcp;
+        const double sq_R = dfbpx * dfbpx, sq_rb = dfbx * dfbx, sq_rp = dfpx * dfpx;
+        if (tree[node].children.empty()) {
+            near[li].push_back(node);
+        } else if ((std::sqrt(sq_rb) + std::sqrt(sq_rp)) / std::sqrt(sq_R) <
+                   std::sqrt(sq_theta)) {
+            far[li].push_back(node);
+        } else {
+            for (size_t c : tree[node].children) lists(li, c);
+        }
+    };
+    for (size_t li = 0; li < nleaf; ++li) lists(li, 0);
+
+    // alloc_set_cluster_list (:775-866): Chebyshev points of the 2nd kind and
+    // barycentric moments for every node but the root.
+    const double pi = 3.14159265358979323846;
+    const double h = pi / P;
+    std::vector<double> t(PP), w1i(PP), dj(PP);
+    for (int i = 0; i < PP; ++i) t[i] = std::cos(i * h);
+    dj[0] = 0.5;
+    dj[P] = 0.5;
+    for (int j = 1; j < P; ++j) dj[j] = 1.0;
+    for (int j = 0; j < PP; ++j) w1i[j] = ((j % 2 == 0) ? 1 : -1) * dj[j];
+    const size_t nnode = tree.size();
+    std::vector<double> t1(nnode * PP, 0.0), mom(nnode * PP, 0.0);
+    std::vector<double> a1i(PP);
+    for (size_t node = 1; node < nnode; ++node) {
+        const double x1 = tree[node].x0, x2 = tree[node].x1;
+        double* tn = &t1[node * PP];
+        double* mn_ = &mom[node * PP];
+        for (int j = 0; j < PP; ++j) tn[j] = x1 + (t[j] + 1.0) * 0.5 * (x2 - x1);
+        for (size_t k = tree[node].m0; k <= tree[node].m1; ++k) {
+            const double x = px[k];
+            int exact = -1;
+            double SumA1 = 0.0;
+            for (int j = 0; j < PP; ++j) {
+                const double dx = x - tn[j];
+                if (std::fabs(dx) <= DBL_MIN) {
+                    exact = j;
+                } else {
+                    a1i[j] = w1i[j] / dx;
+                    SumA1 += a1i[j];
+                }
+            }
+            if (exact > -1) {
+                SumA1 = 1.0;
+                for (int j = 0; j < PP; ++j) a1i[j] = 0.0;
+                a1i[exact] = 1.0;
+            }
+            const double D = 1.0 / SumA1;
+            for (int i = 0; i < PP; ++i) {
+                const double s = a1i[i] * D;
+                mn_[i] += s * lambda[k];
+            }
+        }
+    }
+
+    // Call_BL (:904-961) then Call_Ds (:963-1020), accumulated per target in
+    // the reordered numbering.
+    std::vector<double> keps(n, 0.0);
+    int64_t far_evals = 0, near_evals = 0;
+    auto kern = [&](double s) {
+        return -1.0 * s * (1 - std::sqrt(deno / (4 * s * s * L * L +
+                                                 4 * epsilon * epsilon)));
+    };
+    for (size_t li = 0; li < nleaf; ++li) {
+        const TNode& lf = tree[leaves[li]];
+        for (size_t ii = lf.m0; ii <= lf.m1; ++ii) {
+            double tempx = 0.0;
+            const double p_x = px[ii];
+            for (size_t fi : far[li]) {
+                for (int kk = 0; kk < PP; ++kk) {
+                    double s = (p_x - t1[fi * PP + kk]) / L;
+                    s = s - std::round(s);
+                    if (std::fabs(s - 0.5) < DBL_MIN || std::fabs(s + 0.5) < DBL_MIN ||
+                        std::fabs(s) < DBL_MIN)
+                        continue;
+                    tempx += mom[fi * PP + kk] * -1.0 * s *
+                             (1 - std::sqrt(deno / (4 * s * s * L * L +
+                                                    4 * epsilon * epsilon)));
+                }
+                far_evals += PP;
+            }
+            keps[ii] += tempx;
+        }
+    }
+    for (size_t li = 0; li < nleaf; ++li) {
+        const TNode& lf = tree[leaves[li]];
+        for (size_t ii = lf.m0; ii <= lf.m1; ++ii) {
+            double tempx = 0.0;
+            for (size_t ni : near[li]) {
+                for (size_t jj = tree[ni].m0; jj <= tree[ni].m1; ++jj) {
+                    double s = (px[ii] - px[jj]) / L;
+                    s = s - std::round(s);
+                    if (std::fabs(s - 0.5) < DBL_MIN || std::fabs(s + 0.5) < DBL_MIN ||
+                        std::fabs(s) < DBL_MIN)
+                        continue;
+                    tempx += lambda[jj] * -1.0 * s *
+                             (1 - std::sqrt(deno / (4 * s * s * L * L +
+                                                    4 * epsilon * epsilon)));
+                }
+                near_evals += (int64_t)(tree[ni].m1 - tree[ni].m0 + 1);
+            }
+            keps[ii] += tempx;
+        }
+    }
+    (void)kern;
+    // un-permute (:480-486) and copy out (:278-280)
+    for (size_t i = 0; i < n; ++i) es[old_index[i]] = keps[i];
+    g_tc_stats[0] = (int64_t)nnode;
+    g_tc_stats[1] = (int64_t)nleaf;
+    g_tc_stats[2] = far_evals;
+    g_tc_stats[3] = near_evals;
+}
+
+}  // namespace orc
+
+extern "C" {
+
+int orc_direct_sum(double* es, const double* targets, int64_t nt,
+                   const double* sources, const double* q_ws, int64_t ns,
+                   double L, double epsilon) {
+    if (nt < 0 || ns < 0 || !(L > 0)) return -1;
+    orc::direct_sum(es, targets, nt, sources, q_ws, ns, L, epsilon);
+    return 0;
+}
+
+// Extended-precision evaluation of the same mathematical sum: every kernel
+// value in long double and a Neumaier-compensated long double accumulator.
+int orc_direct_sum_ext(double* es, const double* targets, int64_t nt,
+                       const double* sources, const double* q_ws, int64_t ns,
+                       double L, double epsilon) {
+    if (nt < 0 || ns < 0 || !(L > 0)) return -1;
+    const long double Ll = L, el = epsilon;
+    const long double c0 = 1.0L / sqrtl(Ll * Ll + 4.0L * el * el);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nt; ++i) {
+        long double s = 0.0L, c = 0.0L;
+        const long double xi = targets[i];
+        for (int64_t j = 0; j < ns; ++j) {
+            long double z = (xi - (long double)sources[j]) / Ll;
+            z = z - roundl(z);
+            if (z == 0.5L || z == -0.5L || z == 0.0L) continue;
+            const long double term =
+                (long double)q_ws[j] * -z * Ll *
+                (c0 - 1.0L / sqrtl(4.0L * z * z * Ll * Ll + 4.0L * el * el));
+            const long double tsum = s + term;
+            if (fabsl(s) >= fabsl(term))
+                c += (s - tsum) + term;
+            else
+                c += (term - tsum) + s;
+            s = tsum;
+        }
+        es[i] = (double)(s + c);
+    }
+    return 0;
+}
+
+int orc_treecode(double* es, const double* sources, const double* q_ws,
+                 int64_t ns, double L, double epsilon, double mac, int degree,
+                 int max_source) {
+    if (ns <= 0 || degree < 1 || max_source < 1 || !(L > 0)) return -1;
+    orc::treecode(es, sources, q_ws, ns, L, epsilon, mac, degree, max_source);
+    return 0;
+}
+
+void orc_treecode_last_stats(int64_t out[4]) {
+    for (int i = 0; i < 4; ++i) out[i] = orc::g_tc_stats[i];
+}
+
+}  // extern "C"

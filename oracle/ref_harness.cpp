@@ -1,0 +1,81 @@
+// Test infrastructure only: a C-ABI harness around the reference's own field
+// solvers (src/FieldStructure.cpp, compiled unmodified from its source tree by
+// oracle/Makefile into oracle/_ref/). Nothing in the product links this file;
+// tests/ and bench.py's CPU-baseline leg load it as the checker.
+//
+// Two details of the reference need handling here, not in the reference:
+//  * E_MQ_DirectSum / E_MQ_Treecode print a line to std::cout on every call
+//    (src/FieldStructure.cpp:116-119, :259-264, :417-420). The harness points
+//    std::cout at a null buffer for the duration of a call.
+//  * E_MQ_Treecode::operator() frees arrays that compute_RHS_BLTC already freed
+//    without nulling them (src/FieldStructure.cpp:466-470 and :498-499 vs
+//    :313-378), a double delete[] that aborts under glibc. The harness replaces
+//    the array form of operator delete with a switchable no-op while a treecode
+//    call runs (the library is linked -Bsymbolic so the reference's calls bind
+//    here). The per-call arrays are leaked, which is harmless for a checker.
+
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <new>
+#include <streambuf>
+#include <vector>
+
+#include "FieldStructure.hpp"
+
+namespace {
+bool g_leak_array_deletes = false;
+
+struct NullBuf : std::streambuf {
+    int overflow(int c) override { return c; }
+};
+
+struct QuietCout {
+    NullBuf nb;
+    std::streambuf* old;
+    QuietCout() : old(std::cout.rdbuf(&nb)) {}
+    ~QuietCout() { std::cout.rdbuf(old); }
+};
+}  // namespace
+
+void operator delete[](void* p) noexcept {
+    if (g_leak_array_deletes) return;
+    std::free(p);
+}
+void operator delete[](void* p, std::size_t) noexcept {
+    if (g_leak_array_deletes) return;
+    std::free(p);
+}
+
+extern "C" {
+
+// E_MQ_DirectSum::operator() (src/FieldStructure.cpp:106-156), run as-is.
+int fsref_direct_sum(double* es, const double* targets, int nt,
+                     const double* sources, const double* q_ws, int ns,
+                     double L, double epsilon) {
+    if (nt < 0 || ns < 0) return -1;
+    std::vector<double> t(targets, targets + nt), s(sources, sources + ns),
+        w(q_ws, q_ws + ns);
+    E_MQ_DirectSum field(L, epsilon);
+    QuietCout quiet;
+    field(es, t.data(), nt, s.data(), w.data(), ns);
+    return 0;
+}
+
+// E_MQ_Treecode::operator() (src/FieldStructure.cpp:230-382), run as-is.
+// Like the reference, it evaluates at the sources and writes es[0:ns].
+int fsref_treecode(double* es, const double* sources, const double* q_ws,
+                   int ns, double L, double epsilon, double mac, int degree,
+                   int max_source) {
+    if (ns <= 0) return -1;
+    std::vector<double> t(sources, sources + ns), s(sources, sources + ns),
+        w(q_ws, q_ws + ns);
+    E_MQ_Treecode field(L, epsilon, mac, degree, max_source, 2000, 0);
+    QuietCout quiet;
+    g_leak_array_deletes = true;
+    field(es, t.data(), ns, s.data(), w.data(), ns);
+    g_leak_array_deletes = false;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+"""GPU parity of the FP64 periodic direct sum (fs_emq_periodic_f64*).
+
+Bar: normwise max|dE| <= 1e-12 max|E| against the reference's own outputs
+(golden) and against the extended-precision oracle, plus pointwise
+|dE_i| <= 1e-12 max|E| everywhere. At the full 2^20-point size the check is
+sampled targets against the extended oracle and size-independent properties
+(exact oddness in w, linearity, periodic translation, determinism).
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+L0 = 4 * np.pi
+TOL = 1e-12
+
+
+def c2_deck(n, seed, eps_unused=None):
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(0, L0, n)
+    x = np.mod(u + 0.05 * np.sin(2 * np.pi / L0 * u), L0)
+    v = rng.uniform(-6, 6, n)
+    q = np.where(np.arange(n) < n // 2, -1.0, 1.0)
+    w = q * np.exp(-v * v / 2) / np.sqrt(2 * np.pi) * 1e-4
+    return x, w
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def test_matches_reference_golden(gpu_ctx, golden):
+    for name, c in golden.items():
+        if not name.startswith("ds_"):
+            continue
+        L, eps = c["params"]
+        e = gpu_ctx.emq_periodic_f64(c["t"], c["s"], c["w"], L, eps)
+        ex = oracle.direct_sum_ext(c["t"], c["s"], c["w"], L, eps)
+        print(name, "gpu-vs-ref", rel(e, c["es"]), "gpu-vs-ext", rel(e, ex),
+              "ref-vs-ext", rel(c["es"], ex))
+        assert rel(e, c["es"]) <= TOL, name
+        assert rel(e, ex) <= 1e-14, name
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 255, 257, 1000, 4097, 16770])
+def test_ragged_sizes(gpu_ctx, n):
+    x, w = c2_deck(n, seed=n)
+    e = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+    ex = oracle.direct_sum_ext(x, x, w, L0, 0.05)
+    if n == 1:
+        assert e[0] == 0.0
+    else:
+        assert rel(e, ex) <= 1e-14
+
+
+def test_empty_and_distinct_targets(gpu_ctx):
+    x, w = c2_deck(500, 3)
+    assert gpu_ctx.emq_periodic_f64(np.empty(0), x, w, L0, 0.05).size == 0
+    e0 = gpu_ctx.emq_periodic_f64(x[:10], np.empty(0), np.empty(0), L0, 0.05)
+    assert np.all(e0 == 0.0)
+    t = np.linspace(-L0, 2 * L0, 333)  # unwrapped targets, not the sources
+    e = gpu_ctx.emq_periodic_f64(t, x, w, L0, 0.05)
+    assert rel(e, oracle.direct_sum_ext(t, x, w, L0, 0.05)) <= 1e-14
+
+
+def test_coincident_points_give_exact_zero(gpu_ctx):
+    s = np.array([2.0])
+    e = gpu_ctx.emq_periodic_f64(np.array([2.0, 2.0 + L0 / 2, 2.0 - L0 / 2]), s,
+                                 np.array([1.0]), L0, 0.05)
+    assert np.all(e == 0.0)
+
+
+@pytest.mark.parametrize("tpt,splits", [(1, 1), (1, 7), (2, 1), (2, 3), (4, 1), (4, 16)])
+def test_launch_shapes_agree(gpu_ctx, tpt, splits):
+    x, w = c2_deck(20000, 4)
+    ex = oracle.direct_sum_ext(x, x, w, L0, 0.05)
+    gpu_ctx.set_tuning(tpt, splits)
+    try:
+        e1 = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+        e2 = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+    finally:
+        gpu_ctx.set_tuning(0, 0)
+    assert np.array_equal(e1, e2)  # deterministic
+    assert rel(e1, ex) <= 1e-14
+
+
+@pytest.mark.parametrize("eps", [0.01, 0.05])
+def test_full_size_c2(gpu_ctx, eps):
+    """N = 2^20 targets = sources: sampled targets against the extended oracle
+    over all sources, and size-independent properties of the whole field."""
+    n = 1 << 20
+    x, w = c2_deck(n, seed=0)
+    e = gpu_ctx.emq_periodic_f64(x, x, w, L0, eps)
+    rng = np.random.default_rng(1)
+    pick = np.sort(rng.choice(n, 96, replace=False))
+    ex = oracle.direct_sum_ext(x[pick], x, w, L0, eps)
+    scale = np.max(np.abs(e))
+    assert np.max(np.abs(e[pick] - ex)) <= TOL * scale
+    em = gpu_ctx.emq_periodic_f64(x, x, -w, L0, eps)
+    assert np.array_equal(em, -e)
+    e2 = gpu_ctx.emq_periodic_f64(x, x, 2.0 * w, L0, eps)
+    assert np.array_equal(e2, 2.0 * e)  # scaling by 2 is exact
+    eL = gpu_ctx.emq_periodic_f64(x + L0, x, w, L0, eps)
+    assert np.max(np.abs(eL - e)) <= TOL * scale
+
+
+def test_rsq_refinement_is_full_precision(gpu_ctx):
+    """Single-source fields sweep the kernel over z in [-1/2, 1/2]: every value
+    within a few ulp of the extended oracle (pins the refined v_rsq_f64)."""
+    z = np.linspace(-0.5, 0.5, 20001)
+    t = z * L0
+    for eps in (1e-3, 0.05, 1.0):
+        e = gpu_ctx.emq_periodic_f64(t, np.array([0.0]), np.array([1.0]), L0, eps)
+        ex = oracle.direct_sum_ext(t, np.array([0.0]), np.array([1.0]), L0, eps)
+        scale = np.max(np.abs(ex))
+        assert np.max(np.abs(e - ex)) <= 8e-16 * scale, eps
