@@ -93,6 +93,9 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--eps", type=float, default=0.05)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration "
+                         "for the HBM traffic figures) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -104,6 +107,15 @@ def main():
 
     import multi_species_amd as ms
 
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy << 17, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8}))
+        return
+
     n = args.n
     x, w = c2_inputs(n)
     dev = torch.device("cuda:0")
@@ -111,8 +123,11 @@ def main():
     tw = torch.from_numpy(w).to(dev)
     te = torch.empty(n, dtype=torch.float64, device=dev)
     ctx = ms.Context(0)
-    stream = torch.cuda.current_stream(dev)
+    # a dedicated default-priority stream: the library launches on it and the
+    # HIP events below are recorded on it
+    stream = torch.cuda.Stream(device=dev)
     sptr = stream.cuda_stream
+    assert sptr, "need a non-null stream handle"
 
     def step():
         ctx.emq_periodic_f64_dev(te.data_ptr(), tx.data_ptr(), n, tx.data_ptr(), tw.data_ptr(),

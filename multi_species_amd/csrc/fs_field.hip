@@ -4,13 +4,14 @@
 // 106-156). With z = (x_i - x_j)/L wrapped to [-1/2, 1/2] and delta = eps/L
 // the reference term
 //     w_j * (-z L) * (1/sqrt(L^2 + 4 eps^2) - 1/sqrt(4 z^2 L^2 + 4 eps^2))
-// equals (w_j/2) * z * (1/sqrt(z^2 + delta^2) - C), C = 2/sqrt(1 + 4 delta^2),
-// which is what the kernel evaluates: positions are pre-scaled by 1/L and the
-// weights by 1/2 once per call, so a pair costs one subtract, a round-to-
-// nearest-even, a subtract, an FMA, a refined reciprocal square root and two
-// more FMA-class operations. C is computed on the device with the same
-// refined rsqrt, so z = +-1/2 gives exactly 0 like the reference's skip; z = 0
-// gives 0 through the factor z.
+// equals (w_j/2) * z * (1/sqrt(z^2 + delta^2) - C), C = 2/sqrt(1 + 4 delta^2).
+// The kernel accumulates w_j z (1/sqrt(z^2 + delta^2) - C) and halves the sum
+// at the end (an exact scaling). Per pair: d = x_i - x_j (exact for nearby
+// points, like the reference), t = d / L as d * (1/L), z = t - rint(t), one
+// FMA for z^2 + delta^2, a refined reciprocal square root, and three more
+// FMA-class operations: 13 FP64 VALU operations and one v_rsq_f64. C is
+// computed on the device with the same refined rsqrt, so z = +-1/2 gives
+// exactly 0 like the reference's skip; z = 0 gives 0 through the factor z.
 //
 // Mapping to CDNA4: the source loop index is uniform across a wavefront, so
 // x_j and w_j are scalar loads into SGPRs feeding every lane's VALU operation
@@ -52,8 +53,8 @@ __device__ __forceinline__ void two_sum_acc(double& s, double& lo, double b) {
 
 template <int TPT>
 __global__ __launch_bounds__(kBlock) void emq_direct_f64_kernel(
-    const double* __restrict__ tgt_s, int nt, const double* __restrict__ src_s,
-    const double* __restrict__ src_W, int ns, int chunk, double delta2,
+    const double* __restrict__ tgt, int nt, const double* __restrict__ src,
+    const double* __restrict__ w, int ns, int chunk, double invL, double delta2,
     double* __restrict__ out) {
     const int split = blockIdx.y;
     const int j0 = split * chunk;
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void emq_direct_f64_kernel(
 #pragma unroll
     for (int t = 0; t < TPT; ++t) {
         const int i = i0 + t * kBlock;
-        xi[t] = (i < nt) ? tgt_s[i] : 0.0;
+        xi[t] = (i < nt) ? tgt[i] : 0.0;
         hi[t] = 0.0;
         lo[t] = 0.0;
     }
@@ -75,14 +76,14 @@ __global__ __launch_bounds__(kBlock) void emq_direct_f64_kernel(
         for (int t = 0; t < TPT; ++t) acc[t] = 0.0;
 #pragma unroll 4
         for (int j = jb; j < je; ++j) {
-            const double xj = src_s[j];
-            const double Wj = src_W[j];
+            const double xj = src[j];
+            const double wj = w[j];
 #pragma unroll
             for (int t = 0; t < TPT; ++t) {
-                const double d = xi[t] - xj;
-                const double z = d - __builtin_rint(d);
+                const double s = (xi[t] - xj) * invL;
+                const double z = s - __builtin_rint(s);
                 const double y = rsq_f64(__builtin_fma(z, z, delta2));
-                acc[t] = __builtin_fma(Wj * z, y - C, acc[t]);
+                acc[t] = __builtin_fma(wj * z, y - C, acc[t]);
             }
         }
 #pragma unroll
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void emq_direct_f64_kernel(
 #pragma unroll
     for (int t = 0; t < TPT; ++t) {
         const int i = i0 + t * kBlock;
-        if (i < nt) out[(size_t)split * nt + i] = hi[t] + lo[t];
+        if (i < nt) out[(size_t)split * nt + i] = 0.5 * (hi[t] + lo[t]);
     }
 }
 
@@ -102,15 +103,6 @@ __global__ __launch_bounds__(kBlock) void reduce_splits_kernel(
     double s = 0.0, lo = 0.0;
     for (int k = 0; k < splits; ++k) two_sum_acc(s, lo, part[(size_t)k * nt + i]);
     es[i] = s + lo;
-}
-
-__global__ __launch_bounds__(kBlock) void scale_points_kernel(
-    const double* __restrict__ x, const double* __restrict__ w, int n, double invL,
-    double* __restrict__ xs, double* __restrict__ W) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    xs[i] = x[i] * invL;
-    if (w) W[i] = 0.5 * w[i];
 }
 
 __global__ __launch_bounds__(kBlock) void fill_kernel(double* __restrict__ a, int n, double v) {
@@ -142,6 +134,7 @@ Plan plan_direct(int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun) {
     splits = std::min<int64_t>(splits, 65535);
     int64_t chunk = (ns + splits - 1) / splits;
     chunk = (chunk + kSub - 1) / kSub * kSub;  // split boundaries on sub-sum boundaries
+    chunk = std::max<int64_t>(chunk, kSub);   // ns == 0 still gives a valid plan
     splits = std::max<int64_t>(1, (ns + chunk - 1) / chunk);
     p.splits = (int)splits;
     p.chunk = (int)chunk;
@@ -152,11 +145,7 @@ Plan plan_direct(int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun) {
 
 size_t field_scratch_bytes(int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun) {
     const Plan p = plan_direct(nt, ns, num_cus, tun);
-    size_t b = 0;
-    b += align_up(sizeof(double) * nt, 256);             // scaled targets
-    b += align_up(sizeof(double) * ns, 256) * 2;         // scaled sources, weights
-    if (p.splits > 1) b += align_up(sizeof(double) * nt * p.splits, 256);
-    return b;
+    return p.splits > 1 ? align_up(sizeof(double) * nt * p.splits, 256) : 0;
 }
 
 int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src, const double* d_w,
@@ -176,41 +165,25 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src, const
         return FS_OK;
     }
     const Plan p = plan_direct(nt, ns, num_cus, tun);
-    char* s = static_cast<char*>(scratch);
-    double* tgt_s = reinterpret_cast<double*>(s);
-    s += align_up(sizeof(double) * nt, 256);
-    double* src_s = reinterpret_cast<double*>(s);
-    s += align_up(sizeof(double) * ns, 256);
-    double* src_W = reinterpret_cast<double*>(s);
-    s += align_up(sizeof(double) * ns, 256);
-    double* part = reinterpret_cast<double*>(s);
-
+    double* part = static_cast<double*>(scratch);
     const double invL = 1.0 / L;
     const double delta = eps / L;
     const double delta2 = delta * delta;
-    scale_points_kernel<<<(nsi + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
-        d_src, d_w, nsi, invL, src_s, src_W);
-    const double* tgt_use = src_s;
-    if (d_tgt != d_src || nt != ns) {
-        scale_points_kernel<<<(nti + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
-            d_tgt, nullptr, nti, invL, tgt_s, nullptr);
-        tgt_use = tgt_s;
-    }
     double* out = (p.splits > 1) ? part : d_es;
     const dim3 grid((unsigned)((nt + (int64_t)kBlock * p.tpt - 1) / ((int64_t)kBlock * p.tpt)),
                     (unsigned)p.splits);
     switch (p.tpt) {
         case 4:
-            emq_direct_f64_kernel<4><<<grid, kBlock, 0, stream>>>(tgt_use, nti, src_s, src_W,
-                                                                  nsi, p.chunk, delta2, out);
+            emq_direct_f64_kernel<4><<<grid, kBlock, 0, stream>>>(d_tgt, nti, d_src, d_w,
+                                                                  nsi, p.chunk, invL, delta2, out);
             break;
         case 2:
-            emq_direct_f64_kernel<2><<<grid, kBlock, 0, stream>>>(tgt_use, nti, src_s, src_W,
-                                                                  nsi, p.chunk, delta2, out);
+            emq_direct_f64_kernel<2><<<grid, kBlock, 0, stream>>>(d_tgt, nti, d_src, d_w,
+                                                                  nsi, p.chunk, invL, delta2, out);
             break;
         default:
-            emq_direct_f64_kernel<1><<<grid, kBlock, 0, stream>>>(tgt_use, nti, src_s, src_W,
-                                                                  nsi, p.chunk, delta2, out);
+            emq_direct_f64_kernel<1><<<grid, kBlock, 0, stream>>>(d_tgt, nti, d_src, d_w,
+                                                                  nsi, p.chunk, invL, delta2, out);
             break;
     }
     FS_HIP_CHECK(hipGetLastError());

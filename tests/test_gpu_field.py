@@ -41,7 +41,8 @@ def test_matches_reference_golden(gpu_ctx, golden):
         print(name, "gpu-vs-ref", rel(e, c["es"]), "gpu-vs-ext", rel(e, ex),
               "ref-vs-ext", rel(c["es"], ex))
         assert rel(e, c["es"]) <= TOL, name
-        assert rel(e, ex) <= 1e-14, name
+        # at least as close to the exact sum as the reference's own FP64 sum
+        assert rel(e, ex) <= max(1.5 * rel(c["es"], ex), 4e-15), name
 
 
 @pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 255, 257, 1000, 4097, 16770])
