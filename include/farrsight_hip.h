@@ -61,6 +61,104 @@ int fs_emq_periodic_f64_dev(fs_ctx* ctx, double* d_es, const double* d_targets,
  * in {1,2,4}; source_splits >= 1. Applies to later calls on this context. */
 int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits);
 
+/* ---- Part B: remesh interpolation -------------------------------------- */
+
+/* Panel records: FS_PANEL_INTS int32 per panel, the fields of struct Panel
+ * (inc/Panel.hpp:41-56):
+ *   [0] panel_ind, [1..9] point indices (Panel.hpp:57-75 order), [10] level,
+ *   [11] parent, [12] which_child, [13] left, [14] top, [15] right,
+ *   [16] bottom neighbour (-1: use the parent's, -2: boundary),
+ *   [17] flags (1 left boundary, 2 right boundary, 4 needs refinement,
+ *        8 refined in x and p, 16 refined in p only), [18] child_inds_start,
+ *   [19] unused. Panel 0 is the root. */
+#define FS_PANEL_INTS 20
+
+/* Replaces AMRStructure::interpolate_to_initial_xps
+ * (src/structure_files/AMRStructure_interp.cpp:493-793): f of the old mesh
+ * (panel records, advected old points old_xs/old_ps with values old_fs)
+ * interpolated onto the nx*np points new_xs/new_ps of a base grid (np rows
+ * of equal p with nx points each, any order). Periodic pre-shift, row sort,
+ * neighbour-walk leaf search and 9-point interpolant as the reference;
+ * points found beyond a p boundary get f_beyond. Host pointers. */
+int fs_remesh_interp_grid(fs_ctx* ctx, const int32_t* panels, int64_t n_panels,
+                          const double* old_xs, const double* old_ps, const double* old_fs,
+                          int64_t n_old, const double* new_xs, const double* new_ps, int nx,
+                          int np, double x_min, double x_max, int periodic, double f_beyond,
+                          double* f_out);
+
+/* Replaces AMRStructure::interpolate_from_mesh(x, p)
+ * (AMRStructure_interp.cpp:1094-1143) at n points: periodic pre-shift,
+ * root descent, single-point interpolant. Host pointers. */
+int fs_remesh_interp_points(fs_ctx* ctx, const int32_t* panels, int64_t n_panels,
+                            const double* old_xs, const double* old_ps, const double* old_fs,
+                            int64_t n_old, const double* x, const double* p, int64_t n,
+                            double x_min, double x_max, int periodic, double f_beyond,
+                            double* f_out);
+
+/* ---- The step loop: AMRSimulation on the device ------------------------- */
+
+/* One species of a deck (the per-species keys of the JSON deck,
+ * src/simulation_files/AMRSimulation_load.cpp:139-254). ics_type as the
+ * reference (1/2 Landau, 3 strong two-stream, 4 colder two-stream,
+ * 9/10 ion acoustic); 11 is a bump-on-tail (bulk Maxwellian pth centred at
+ * pstr plus a beam_frac beam at beam_v with width beam_vth), which the
+ * reference does not have. */
+typedef struct fs_species_params {
+    double q, m;
+    double p_min, p_max;
+    int initial_height, p_height, max_height;
+    int do_adaptively_refine;
+    int n_amr_epsilons;
+    double amr_epsilons[5];
+    int ics_type;
+    double normalized_wavenumber, amp, pth, pstr;
+    double beam_frac, beam_v, beam_vth;
+} fs_species_params;
+
+/* The deck's top-level keys (AMRSimulation_load.cpp:14-96). */
+typedef struct fs_sim_params {
+    double x_min, x_max;
+    int bcs;            /* 0 periodic, 1 open */
+    int quadrature;     /* 0 trapezoid, 1 Simpson */
+    double greens_epsilon;
+    int use_treecode;
+    double mac;
+    int degree, max_source;
+    double dt;
+    int remesh_period;
+} fs_sim_params;
+
+typedef struct fs_sim fs_sim;
+
+/* Host-only: the initial mesh of one species (AMRStructure ctor,
+ * src/structure_files/AMRStructure.cpp:91-126: base mesh, f0, adaptive
+ * refinement, quadrature weights). Sizes into n_points/n_panels; any output
+ * array may be NULL (call once with NULLs to size the buffers). */
+int fs_mesh_build(const fs_sim_params* sim, const fs_species_params* species,
+                  int64_t* n_points, int64_t* n_panels, int32_t* panels, double* xs,
+                  double* ps, double* fs, double* q_ws);
+
+/* AMRSimulation ctor (src/simulation_files/AMRSimulation.cpp:5-93): builds
+ * every species' mesh from f0 and evaluates E on the reduced grid. The
+ * simulation's state stays in device memory. */
+int fs_sim_create(fs_ctx* ctx, const fs_sim_params* sim, const fs_species_params* species,
+                  int n_species, fs_sim** out);
+int fs_sim_destroy(fs_sim* sim);
+/* AMRSimulation::step (AMRSimulation_step.cpp:5-99) without file output:
+ * one RK4 step (three field evaluations), then a remesh with the reduced-x
+ * field evaluation, or a full field evaluation. Synchronous. */
+int fs_sim_step(fs_sim* sim);
+int fs_sim_iter(fs_sim* sim, int* iter);
+int fs_sim_species_counts(fs_sim* sim, int species, int64_t* n_points, int64_t* n_panels);
+/* Copies one species' points (any pointer may be NULL); es is E at the points
+ * after the last field evaluation. */
+int fs_sim_get_points(fs_sim* sim, int species, double* xs, double* ps, double* fs,
+                      double* q_ws, double* es);
+int fs_sim_get_panels(fs_sim* sim, int species, int32_t* panels);
+/* out[0] charge, [1] momentum, [2] kinetic energy, [3] electric energy
+ * (standard_interface/diagnostic_plot.py:116-131). */
+int fs_sim_diagnostics(fs_sim* sim, int species, double out[4]);
+
 #ifdef __cplusplus
 }
 #endif

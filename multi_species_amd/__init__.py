@@ -46,6 +46,23 @@ def lib():
         L.fs_emq_periodic_f64.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
         L.fs_emq_periodic_f64_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d, _vp]
         L.fs_emq_set_tuning.argtypes = [_vp, _int, _int]
+        _ip = ctypes.POINTER(ctypes.c_int32)
+        L.fs_remesh_interp_grid.argtypes = [_vp, _ip, _i64, _dp, _dp, _dp, _i64, _dp, _dp, _int,
+                                            _int, _d, _d, _int, _d, _dp]
+        L.fs_remesh_interp_points.argtypes = [_vp, _ip, _i64, _dp, _dp, _dp, _i64, _dp, _dp,
+                                              _i64, _d, _d, _int, _d, _dp]
+        L.fs_mesh_build.argtypes = [ctypes.POINTER(SimParams), ctypes.POINTER(SpeciesParams),
+                                    ctypes.POINTER(_i64), ctypes.POINTER(_i64), _ip, _dp, _dp,
+                                    _dp, _dp]
+        L.fs_sim_create.argtypes = [_vp, ctypes.POINTER(SimParams), ctypes.POINTER(SpeciesParams),
+                                    _int, ctypes.POINTER(_vp)]
+        L.fs_sim_destroy.argtypes = [_vp]
+        L.fs_sim_step.argtypes = [_vp]
+        L.fs_sim_iter.argtypes = [_vp, ctypes.POINTER(_int)]
+        L.fs_sim_species_counts.argtypes = [_vp, _int, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]
+        L.fs_sim_get_points.argtypes = [_vp, _int, _dp, _dp, _dp, _dp, _dp]
+        L.fs_sim_get_panels.argtypes = [_vp, _int, _ip]
+        L.fs_sim_diagnostics.argtypes = [_vp, _int, _dp]
         _lib = L
     return _lib
 
@@ -112,6 +129,176 @@ class Context:
         _check(lib().fs_emq_periodic_f64_dev(self.h, es_ptr, t_ptr, nt, s_ptr, w_ptr, ns, L,
                                              eps, stream or None),
                "fs_emq_periodic_f64_dev")
+
+
+PANEL_INTS = 20
+
+
+class SpeciesParams(ctypes.Structure):
+    """fs_species_params (the per-species deck keys)."""
+    _fields_ = [
+        ("q", _d), ("m", _d), ("p_min", _d), ("p_max", _d),
+        ("initial_height", _int), ("p_height", _int), ("max_height", _int),
+        ("do_adaptively_refine", _int), ("n_amr_epsilons", _int), ("amr_epsilons", _d * 5),
+        ("ics_type", _int), ("normalized_wavenumber", _d), ("amp", _d), ("pth", _d),
+        ("pstr", _d), ("beam_frac", _d), ("beam_v", _d), ("beam_vth", _d),
+    ]
+
+
+class SimParams(ctypes.Structure):
+    """fs_sim_params (the deck's top-level keys)."""
+    _fields_ = [
+        ("x_min", _d), ("x_max", _d), ("bcs", _int), ("quadrature", _int),
+        ("greens_epsilon", _d), ("use_treecode", _int), ("mac", _d), ("degree", _int),
+        ("max_source", _int), ("dt", _d), ("remesh_period", _int),
+    ]
+
+
+def species_params(q=-1.0, m=1.0, p_min=-1.0, p_max=1.0, initial_height=6, p_height=0,
+                   max_height=None, do_adaptively_refine=False, amr_epsilons=(), ics_type=1,
+                   normalized_wavenumber=1.0, amp=0.0, pth=1.0, pstr=0.0, beam_frac=0.0,
+                   beam_v=0.0, beam_vth=1.0, **_ignored) -> SpeciesParams:
+    """A species entry with the reference deck's defaults (AMRSimulation_load.cpp:226-254)."""
+    sp = SpeciesParams()
+    sp.q, sp.m, sp.p_min, sp.p_max = q, m, p_min, p_max
+    sp.initial_height, sp.p_height = initial_height, p_height
+    sp.max_height = initial_height if max_height is None else max_height
+    sp.do_adaptively_refine = int(bool(do_adaptively_refine))
+    eps = list(amr_epsilons)[:5]
+    sp.n_amr_epsilons = len(eps)
+    for i, e in enumerate(eps):
+        sp.amr_epsilons[i] = e
+    sp.ics_type = ics_type
+    sp.normalized_wavenumber, sp.amp, sp.pth, sp.pstr = normalized_wavenumber, amp, pth, pstr
+    sp.beam_frac, sp.beam_v, sp.beam_vth = beam_frac, beam_v, beam_vth
+    return sp
+
+
+def sim_params(x_min=0.0, x_max=0.0, bcs=0, quadrature=1, greens_epsilon=0.2, use_treecode=0,
+               mac=-1.0, degree=-1, max_source=2000, dt=0.5, remesh_period=1,
+               **_ignored) -> SimParams:
+    """Top-level deck keys with the reference's defaults (AMRSimulation_load.cpp:14-59)."""
+    p = SimParams()
+    p.x_min, p.x_max, p.bcs, p.quadrature = x_min, x_max, bcs, quadrature
+    p.greens_epsilon, p.use_treecode, p.mac = greens_epsilon, use_treecode, mac
+    p.degree, p.max_source, p.dt, p.remesh_period = degree, max_source, dt, remesh_period
+    return p
+
+
+def mesh_build(sim: SimParams, sp: SpeciesParams):
+    """The initial mesh of one species, built by the library's host code."""
+    n, P = _i64(), _i64()
+    _check(lib().fs_mesh_build(ctypes.byref(sim), ctypes.byref(sp), ctypes.byref(n),
+                               ctypes.byref(P), None, None, None, None, None), "fs_mesh_build")
+    rec = np.empty((P.value, PANEL_INTS), dtype=np.int32)
+    xs, ps, fs, qw = (np.empty(n.value) for _ in range(4))
+    _check(lib().fs_mesh_build(ctypes.byref(sim), ctypes.byref(sp), None, None, _PI(rec),
+                               _P(xs), _P(ps), _P(fs), _P(qw)), "fs_mesh_build")
+    return rec, xs, ps, fs, qw
+
+
+class Simulation:
+    """AMRSimulation on the device (fs_sim)."""
+
+    def __init__(self, ctx: "Context", sim: SimParams, species):
+        self.ctx = ctx
+        self._sim = sim
+        arr = (SpeciesParams * len(species))(*species)
+        self._sp = arr
+        h = _vp()
+        _check(lib().fs_sim_create(ctx.h, ctypes.byref(sim), arr, len(species), ctypes.byref(h)),
+               "fs_sim_create")
+        self.h = h
+        self.n_species = len(species)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fs_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1):
+        for _ in range(n):
+            _check(lib().fs_sim_step(self.h), "fs_sim_step")
+
+    @property
+    def iter(self) -> int:
+        it = _int()
+        _check(lib().fs_sim_iter(self.h, ctypes.byref(it)), "fs_sim_iter")
+        return it.value
+
+    def counts(self, k: int):
+        n, P = _i64(), _i64()
+        _check(lib().fs_sim_species_counts(self.h, k, ctypes.byref(n), ctypes.byref(P)),
+               "fs_sim_species_counts")
+        return n.value, P.value
+
+    def points(self, k: int):
+        """(xs, ps, fs, q_ws, es) of species k."""
+        n, _ = self.counts(k)
+        arrs = [np.empty(n) for _ in range(5)]
+        _check(lib().fs_sim_get_points(self.h, k, *[_P(a) for a in arrs]), "fs_sim_get_points")
+        return tuple(arrs)
+
+    def panels(self, k: int) -> np.ndarray:
+        _, P = self.counts(k)
+        rec = np.empty((P, PANEL_INTS), dtype=np.int32)
+        _check(lib().fs_sim_get_panels(self.h, k, _PI(rec)), "fs_sim_get_panels")
+        return rec
+
+    def diagnostics(self, k: int) -> np.ndarray:
+        out = np.empty(4)
+        _check(lib().fs_sim_diagnostics(self.h, k, _P(out)), "fs_sim_diagnostics")
+        return out
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _PI(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _remesh_args(panels, old_xs, old_ps, old_fs):
+    rec = _i32(panels).reshape(-1, PANEL_INTS)
+    ox, op, of = _f64(old_xs), _f64(old_ps), _f64(old_fs)
+    if not (ox.size == op.size == of.size):
+        raise ValueError("old point arrays differ in length")
+    return rec, ox, op, of
+
+
+def remesh_interp_grid(ctx: "Context", panels, old_xs, old_ps, old_fs, new_xs, new_ps,
+                       nx: int, np_: int, x_min: float, x_max: float, periodic: bool,
+                       f_beyond: float) -> np.ndarray:
+    """AMRStructure::interpolate_to_initial_xps on the GPU (host arrays)."""
+    rec, ox, op, of = _remesh_args(panels, old_xs, old_ps, old_fs)
+    nxs, nps = _f64(new_xs), _f64(new_ps)
+    out = np.empty(nxs.size)
+    _check(lib().fs_remesh_interp_grid(ctx.h, _PI(rec), rec.shape[0], _P(ox), _P(op), _P(of),
+                                       ox.size, _P(nxs), _P(nps), nx, np_, x_min, x_max,
+                                       int(periodic), f_beyond, _P(out)),
+           "fs_remesh_interp_grid")
+    return out
+
+
+def remesh_interp_points(ctx: "Context", panels, old_xs, old_ps, old_fs, x, p,
+                         x_min: float, x_max: float, periodic: bool,
+                         f_beyond: float) -> np.ndarray:
+    """AMRStructure::interpolate_from_mesh at many points on the GPU."""
+    rec, ox, op, of = _remesh_args(panels, old_xs, old_ps, old_fs)
+    xx, pp = _f64(x), _f64(p)
+    out = np.empty(xx.size)
+    _check(lib().fs_remesh_interp_points(ctx.h, _PI(rec), rec.shape[0], _P(ox), _P(op), _P(of),
+                                         ox.size, _P(xx), _P(pp), xx.size, x_min, x_max,
+                                         int(periodic), f_beyond, _P(out)),
+           "fs_remesh_interp_points")
+    return out
 
 
 def declared_symbols(header: str = HEADER):

@@ -1,0 +1,667 @@
+// Remesh interpolation of f onto the new panel mesh (gfx950).
+//
+// Replaces the numerical core of AMRStructure::remesh (reference
+// src/structure_files/AMRStructure_amr.cpp:1156-1202): interpolate_to_initial_xps
+// (AMRStructure_interp.cpp:493-793) and interpolate_from_mesh
+// (AMRStructure_interp.cpp:1094-1143). Kernels:
+//   coef     one thread per old panel: the 9x9 monomial system of
+//            AMRStructure_interp.cpp:923-935 solved by LU with partial
+//            pivoting (per-thread matrix in LDS, lane-interleaved so the 64
+//            lanes of a wavefront hit 64 different banks); stores the 9
+//            coefficients and the panel centre.
+//   records  one thread per old panel: packs what one leaf-search step reads
+//            (advected corners, the neighbour to step to in each direction,
+//            boundary flags) into one 96-byte record, and the root-descent
+//            record (all 9 points and the children's neighbours) into 192
+//            bytes, so every search step is a single dependent load.
+//   shift    one thread per new point: the periodic pre-shift of
+//            AMRStructure_interp.cpp:429-491.
+//   rowsort  one workgroup per grid row: rank sort of the row by shifted x.
+//   column   one thread: root descent of the first sorted point and the
+//            sequential walk down the first column (:610-650).
+//   rows     one thread per row: the neighbour walk along the row (:696-722).
+//   eval     one thread per point: the interpolant (:943-962).
+//   points   one thread per point: shift, root descent, single-point
+//            interpolant (interpolate_from_mesh, for AMR refinement).
+// The leaf search keeps the reference's tie rules, step order and periodic
+// shifts; floating-point contraction is off in this file, so every product,
+// comparison and sum rounds exactly as in the reference's expressions (and
+// bit-for-bit like this repository's CPU oracle).
+#pragma clang fp contract(off)
+
+#include "fs_remesh.h"
+
+namespace fs {
+namespace remesh {
+
+// Panel record layout (int32 x kPanelInts): see fs_remesh.h.
+constexpr int kMaxWalkSteps = 256;
+constexpr int kMaxHistory = 32;
+constexpr int kMaxDescend = 256;
+
+__global__ __launch_bounds__(64) void coef_kernel(const int32_t* __restrict__ rec, int npanels,
+                                                  const double* __restrict__ ox,
+                                                  const double* __restrict__ op,
+                                                  const double* __restrict__ of,
+                                                  CoefRec* __restrict__ coef) {
+    __shared__ double lds[90 * 64];
+    const int lane = threadIdx.x;
+    const int pi = blockIdx.x * 64 + lane;
+    if (pi >= npanels) return;
+    const int32_t* r = rec + (size_t)pi * kPanelInts;
+    double* A = lds;  // element e of this lane at A[e * 64 + lane]
+#define AE(i, j) A[((i) * 9 + (j)) * 64 + lane]
+#define BE(i) A[(81 + (i)) * 64 + lane]
+    double px[9], pp[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int q = r[1 + i];
+        px[i] = ox[q];
+        pp[i] = op[q];
+        BE(i) = of[q];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const double dx = px[i] - px[4], dp = pp[i] - pp[4];
+        AE(i, 0) = 1;
+        AE(i, 1) = dx;
+        AE(i, 2) = dx * dp;
+        AE(i, 3) = dp;
+        AE(i, 4) = dx * dx;
+        AE(i, 5) = dx * dx * dp;
+        AE(i, 6) = dx * dx * dp * dp;
+        AE(i, 7) = dx * dp * dp;
+        AE(i, 8) = dp * dp;
+    }
+    int perm[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) perm[i] = i;
+    for (int k = 0; k < 9; ++k) {
+        int piv = k;
+        double best = fabs(AE(k, k));
+        for (int i = k + 1; i < 9; ++i) {
+            const double a = fabs(AE(i, k));
+            if (a > best) {
+                best = a;
+                piv = i;
+            }
+        }
+        if (piv != k) {
+            for (int j = 0; j < 9; ++j) {
+                const double t = AE(k, j);
+                AE(k, j) = AE(piv, j);
+                AE(piv, j) = t;
+            }
+            // perm lives in registers: swap through fully unrolled selects
+            int tk = 0, tp = 0;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                if (i == k) tk = perm[i];
+                if (i == piv) tp = perm[i];
+            }
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                if (i == k) perm[i] = tp;
+                if (i == piv) perm[i] = tk;
+            }
+        }
+        const double d = AE(k, k);
+        if (d != 0.0)
+            for (int i = k + 1; i < 9; ++i) AE(i, k) = AE(i, k) / d;
+        for (int i = k + 1; i < 9; ++i) {
+            const double l = AE(i, k);
+            for (int j = k + 1; j < 9; ++j) AE(i, j) = AE(i, j) - l * AE(k, j);
+        }
+    }
+    double y[9], c[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double b = 0.0;
+#pragma unroll
+        for (int m = 0; m < 9; ++m)
+            if (perm[i] == m) b = BE(m);
+        double s = b;
+#pragma unroll
+        for (int j = 0; j < i; ++j) s = s - AE(i, j) * y[j];
+        y[i] = s;
+    }
+#pragma unroll
+    for (int i = 8; i >= 0; --i) {
+        double s = y[i];
+#pragma unroll
+        for (int j = i + 1; j < 9; ++j) s = s - AE(i, j) * c[j];
+        c[i] = s / AE(i, i);
+    }
+#undef AE
+#undef BE
+    CoefRec out;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out.c[i] = c[i];
+    out.x4 = px[4];
+    out.p4 = pp[4];
+    coef[pi] = out;
+}
+
+__global__ __launch_bounds__(256) void records_kernel(const int32_t* __restrict__ rec,
+                                                      int npanels, const double* __restrict__ ox,
+                                                      const double* __restrict__ op,
+                                                      WalkRec* __restrict__ walk,
+                                                      DescRec* __restrict__ desc) {
+    const int pi = blockIdx.x * 256 + threadIdx.x;
+    if (pi >= npanels) return;
+    const int32_t* r = rec + (size_t)pi * kPanelInts;
+    const int flags = r[17];
+    const int nb[4] = {r[15], r[14], r[16], r[13]};  // right, top, bottom, left
+    const int parent = r[11];
+    WalkRec w;
+    w.xbl = ox[r[1 + 0]];
+    w.pbl = op[r[1 + 0]];
+    w.xtl = ox[r[1 + 2]];
+    w.ptl = op[r[1 + 2]];
+    w.xmid = ox[r[1 + 4]];
+    w.xbr = ox[r[1 + 6]];
+    w.pbr = op[r[1 + 6]];
+    w.xtr = ox[r[1 + 8]];
+    w.ptr = op[r[1 + 8]];
+    int wf = 0;
+    // child offset entered when stepping into a refined (x and p) neighbour:
+    // right -> child 0, top -> child 0, bottom -> child 1, left -> child 2
+    // (AMRStructure_interp.cpp:91-92, :116-117, :141-142, :176-177)
+    const int enter[4] = {0, 0, 1, 2};
+    // parent's neighbour slot for the -1 ("use parent's") case
+    const int pslot[4] = {15, 14, 16, 13};
+    for (int d = 0; d < 4; ++d) {
+        int nxt = -3;
+        if (nb[d] == -2) {
+            wf |= 1 << d;
+        } else if (nb[d] == -1) {
+            nxt = (parent >= 0) ? rec[(size_t)parent * kPanelInts + pslot[d]] : -3;
+        } else {
+            const int32_t* q = rec + (size_t)nb[d] * kPanelInts;
+            nxt = (q[17] & kFlagRefinedXP) ? q[18] + enter[d] : nb[d];
+        }
+        w.next[d] = nxt;
+    }
+    if (flags & kFlagLeftBdry) wf |= kWalkLeftBdry;
+    if (flags & kFlagRightBdry) wf |= kWalkRightBdry;
+    w.flags = wf;
+    w.pad[0] = w.pad[1] = w.pad[2] = 0;
+    walk[pi] = w;
+
+    DescRec dr;
+    for (int i = 0; i < 9; ++i) {
+        dr.x[i] = ox[r[1 + i]];
+        dr.p[i] = op[r[1 + i]];
+    }
+    const int cs = r[18];
+    dr.child_start = cs;
+    int df = 0;
+    if (flags & kFlagRefinedXP) df |= kDescRefinedXP;
+    if (flags & kFlagRefinedP) df |= kDescRefinedP;
+    if (flags & kFlagLeftBdry) df |= kDescLeftBdry;
+    if (flags & kFlagRightBdry) df |= kDescRightBdry;
+    for (int d = 0; d < 4; ++d)
+        if (nb[d] == -2) df |= kDescNbrBoundary << d;
+    dr.flags = df;
+    for (int i = 0; i < 8; ++i) dr.cn[i] = 0;
+    if (flags & (kFlagRefinedXP | kFlagRefinedP)) {
+        const bool ponly = (flags & kFlagRefinedP) != 0;
+        const int32_t* c0 = rec + (size_t)cs * kPanelInts;
+        const int32_t* c1 = rec + (size_t)(cs + 1) * kPanelInts;
+        const int32_t* c2 = rec + (size_t)(ponly ? cs : cs + 2) * kPanelInts;
+        const int32_t* c3 = rec + (size_t)(ponly ? cs + 1 : cs + 3) * kPanelInts;
+        dr.cn[0] = c1[14];  // child 1 top
+        dr.cn[1] = c1[13];  // child 1 left
+        dr.cn[2] = c3[14];  // child 3 top
+        dr.cn[3] = c3[15];  // child 3 right
+        dr.cn[4] = c0[16];  // child 0 bottom
+        dr.cn[5] = c0[13];  // child 0 left
+        dr.cn[6] = c2[16];  // child 2 bottom
+        dr.cn[7] = c2[15];  // child 2 right
+    }
+    desc[pi] = dr;
+}
+
+// shift_xs for one point (AMRStructure_interp.cpp:447-490). Returns false
+// when the reference would throw "too many shifts!".
+__device__ bool shift_one(double& xt, double p, const RootCorners& rc, double Lx) {
+    bool left_ok = (rc.xtl - rc.xbl) * (p - rc.pbl) <= (rc.ptl - rc.pbl) * (xt - rc.xbl);
+    int cnt = 0;
+    while (!left_ok) {
+        xt += Lx;
+        left_ok = (rc.xtl - rc.xbl) * (p - rc.pbl) <= (rc.ptl - rc.pbl) * (xt - rc.xbl);
+        if (++cnt > 10) return false;
+    }
+    bool right_ok = (rc.xtr - rc.xbr) * (p - rc.pbr) > (rc.ptr - rc.pbr) * (xt - (rc.xbr));
+    cnt = 0;
+    while (!right_ok) {
+        xt -= Lx;
+        right_ok = (rc.xtr - rc.xbr) * (p - rc.pbr) > (rc.ptr - rc.pbr) * (xt - (rc.xbr));
+        if (++cnt > 10) return false;
+    }
+    return true;
+}
+
+// find_leaf_containing_point_from_neighbor (AMRStructure_interp.cpp:12-230)
+// as a loop over precomputed walk records. Returns the leaf (0 when the walk
+// reaches the root sentinel); sets *err on a runaway walk.
+__device__ int walk_one(const WalkRec* __restrict__ W, double& tx, double tp, bool& beyond,
+                        int start, double Lx, bool periodic, int& err) {
+    int hist[kMaxHistory];
+    int nh = 1;
+    hist[0] = start;
+    int cur = start;
+    for (int it = 0; it < kMaxWalkSteps; ++it) {
+        if (cur == 0) return 0;
+        if (cur < 0) {
+            err = 1;
+            return 0;
+        }
+        const WalkRec w = W[cur];
+        if (tx - w.xmid >= Lx / 2 && periodic) tx -= Lx;
+        if (tx - w.xmid < -Lx / 2 && periodic) tx += Lx;
+        const bool r = (w.xtr - w.xbr) * (tp - w.pbr) > (w.ptr - w.pbr) * (tx - (w.xbr));
+        const bool l = (w.xtl - w.xbl) * (tp - w.pbl) <= (w.ptl - w.pbl) * (tx - w.xbl);
+        const bool t = (w.xtr - w.xtl) * (tp - w.ptl) < (w.ptr - w.ptl) * (tx - w.xtl);
+        const bool b = (w.xbr - w.xbl) * (tp - w.pbl) >= (w.pbr - w.pbl) * (tx - w.xbl);
+        const int f = w.flags;
+        int nxt = cur;
+        if (!r && !(f & kWalkBoundaryRight)) {
+            if ((f & kWalkRightBdry) && periodic) tx -= Lx;
+            nxt = w.next[0];
+        } else if (!t && !(f & kWalkBoundaryTop)) {
+            nxt = w.next[1];
+        } else if (!b && !(f & kWalkBoundaryBottom)) {
+            nxt = w.next[2];
+        } else if (!l && !(f & kWalkBoundaryLeft)) {
+            if ((f & kWalkLeftBdry) && periodic) tx += Lx;
+            nxt = w.next[3];
+        }
+        if ((!r && (f & kWalkBoundaryRight)) || (!l && (f & kWalkBoundaryLeft)) ||
+            (!t && (f & kWalkBoundaryTop)) || (!b && (f & kWalkBoundaryBottom)))
+            beyond = true;
+        for (int k = 0; k < nh; ++k)
+            if (hist[k] == nxt) return nxt;
+        if (nh == kMaxHistory) {
+            err = 1;
+            return nxt;
+        }
+        hist[nh++] = nxt;
+        cur = nxt;
+    }
+    err = 1;
+    return cur;
+}
+
+// find_leaf_containing_xp_recursively (AMRStructure_interp.cpp:232-426).
+__device__ int descend_one(const DescRec* __restrict__ D, double& x, double p, bool& beyond,
+                           int pi, double Lx, bool periodic, int& err) {
+    for (int it = 0; it < kMaxDescend; ++it) {
+        if (pi < 0) {
+            err = 1;
+            return 0;
+        }
+        const DescRec& d = D[pi];
+        const int f = d.flags;
+        const double* X = d.x;
+        const double* P = d.p;
+        if (!(f & (kDescRefinedXP | kDescRefinedP))) {
+            const double x_bl = X[0], p_bl = P[0], x_tl = X[2], p_tl = P[2];
+            const double x_br = X[6], p_br = P[6], x_tr = X[8], p_tr = P[8];
+            const bool r = (x_tr - x_br) * (p - p_br) > (p_tr - p_br) * (x - (x_br));
+            const bool l = (x_tl - x_bl) * (p - p_bl) <= (p_tl - p_bl) * (x - x_bl);
+            const bool t = (x_tr - x_tl) * (p - p_tl) < (p_tr - p_tl) * (x - x_tl);
+            const bool b = (x_br - x_bl) * (p - p_bl) >= (p_br - p_bl) * (x - x_bl);
+            if ((!l && (f & (kDescNbrBoundary << 3))) || (!r && (f & (kDescNbrBoundary << 0))) ||
+                (!t && (f & (kDescNbrBoundary << 1))) || (!b && (f & (kDescNbrBoundary << 2))))
+                beyond = true;
+            return pi;
+        }
+        const int cs = d.child_start;
+        const bool ponly = (f & kDescRefinedP) != 0;
+        const double x_bl = X[0], p_bl = P[0], x_ml = X[1], p_ml = P[1];
+        const double x_tl = X[2], p_tl = P[2], x_bm = X[3], p_bm = P[3];
+        const double x_mm = X[4], p_mm = P[4], x_tm = X[5], p_tm = P[5];
+        const double x_br = X[6], p_br = P[6], x_mr = X[7], p_mr = P[7];
+        const double x_tr = X[8], p_tr = P[8];
+        const bool b1 = (x_mm - x_ml) * (p - p_ml) >= (p_mm - p_ml) * (x - x_ml);
+        const bool r1 = (x_tm - x_mm) * (p - p_mm) >= (p_tm - p_mm) * (x - x_mm);
+        const bool b3 = (x_mr - x_mm) * (p - p_mm) >= (p_mr - p_mm) * (x - x_mm);
+        int sub;
+        if (b1 && r1) {
+            const bool t1 = (x_tm - x_tl) * (p - p_tl) <= (p_tm - p_tl) * (x - x_tl);
+            if (t1 || d.cn[0] < 0) {
+                const bool l1 = (x_tl - x_ml) * (p - p_ml) <= (p_tl - p_ml) * (x - x_ml);
+                if (l1 || d.cn[1] < 0) {
+                    sub = cs + 1;
+                } else {
+                    sub = d.cn[1];
+                    if ((f & kDescLeftBdry) && periodic) x += Lx;
+                }
+            } else {
+                sub = d.cn[0];
+            }
+        } else if (b3 && !r1) {
+            const bool t3 = (x_tr - x_tm) * (p - p_tm) <= (p_tr - p_tm) * (x - x_tm);
+            if (t3 || d.cn[2] < 0) {
+                const bool r3 = (x_tr - x_mr) * (p - p_mr) <= (p_tr - p_mr) * (x - x_mr);
+                if (!r3 || d.cn[3] < 0) {
+                    sub = ponly ? cs + 1 : cs + 3;
+                } else {
+                    sub = d.cn[3];
+                    if ((f & kDescRightBdry) && periodic) x -= Lx;
+                }
+            } else {
+                sub = d.cn[2];
+            }
+        } else {
+            const bool r0 = (x_mm - x_bm) * (p - p_bm) >= (p_mm - p_bm) * (x - x_bm);
+            if (r0) {
+                const bool b0 = (x_bm - x_bl) * (p - p_bl) >= (p_bm - p_bl) * (x - x_bl);
+                if (b0 || d.cn[4] < 0) {
+                    const bool l0 = (x_ml - x_bl) * (p - p_bl) <= (p_ml - p_bl) * (x - x_bl);
+                    if (l0 || d.cn[5] < 0) {
+                        sub = cs;
+                    } else {
+                        sub = d.cn[5];
+                        if ((f & kDescLeftBdry) && periodic) x += Lx;
+                    }
+                } else {
+                    sub = d.cn[4];
+                }
+            } else {
+                const bool b2 = (x_br - x_bm) * (p - p_bm) >= (p_br - p_bm) * (x - x_bm);
+                if (b2 || d.cn[6] < 0) {
+                    const bool r2 = (x_mr - x_br) * (p - p_br) <= (p_mr - p_br) * (x - x_br);
+                    if (!r2 || d.cn[7] < 0) {
+                        sub = ponly ? cs : cs + 2;
+                    } else {
+                        sub = d.cn[7];
+                        if ((f & kDescRightBdry) && periodic) x -= Lx;
+                    }
+                } else {
+                    sub = d.cn[6];
+                }
+            }
+        }
+        pi = sub;
+    }
+    err = 1;
+    return 0;
+}
+
+// The advected corners of the root panel are old points 0, 2, 6 and 8
+// (AMRStructure_interp.cpp:436-439); read on the device so no host round trip
+// is needed between the step and the remesh.
+__device__ __forceinline__ RootCorners load_corners(const double* __restrict__ ox,
+                                                    const double* __restrict__ op) {
+    RootCorners rc;
+    rc.xbl = ox[0];
+    rc.pbl = op[0];
+    rc.xtl = ox[2];
+    rc.ptl = op[2];
+    rc.xbr = ox[6];
+    rc.pbr = op[6];
+    rc.xtr = ox[8];
+    rc.ptr = op[8];
+    return rc;
+}
+
+__global__ __launch_bounds__(256) void shift_kernel(const double* __restrict__ x,
+                                                    const double* __restrict__ p, int n,
+                                                    const double* __restrict__ ox,
+                                                    const double* __restrict__ op, double Lx,
+                                                    int periodic, double* __restrict__ sx,
+                                                    int* __restrict__ status) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const RootCorners rc = load_corners(ox, op);
+    double xt = x[i];
+    if (periodic && !shift_one(xt, p[i], rc, Lx)) atomicOr(status, kStatusTooManyShifts);
+    sx[i] = xt;
+}
+
+// One workgroup per row: each thread ranks its point by (shifted x, index)
+// against the whole row and writes it to its sorted slot.
+__global__ __launch_bounds__(256) void rowsort_kernel(const int32_t* __restrict__ row_pts,
+                                                      int nx, const double* __restrict__ sx,
+                                                      const double* __restrict__ p,
+                                                      double* __restrict__ sorted_x,
+                                                      double* __restrict__ sorted_p,
+                                                      int32_t* __restrict__ sorted_idx) {
+    extern __shared__ double keys[];
+    const int row = blockIdx.x;
+    const int32_t* rp = row_pts + (size_t)row * nx;
+    for (int k = threadIdx.x; k < nx; k += blockDim.x) keys[k] = sx[rp[k]];
+    __syncthreads();
+    for (int k = threadIdx.x; k < nx; k += blockDim.x) {
+        const double kk = keys[k];
+        int rank = 0;
+        for (int m = 0; m < nx; ++m) {
+            const double km = keys[m];
+            rank += (km < kk) || (km == kk && m < k);
+        }
+        const int32_t idx = rp[k];
+        const size_t o = (size_t)row * nx + rank;
+        sorted_x[o] = kk;
+        sorted_p[o] = p[idx];
+        sorted_idx[o] = idx;
+    }
+}
+
+__global__ void column_kernel(const WalkRec* __restrict__ W, const DescRec* __restrict__ D,
+                              int nx, int np, double Lx, int periodic,
+                              double* __restrict__ sorted_x, const double* __restrict__ sorted_p,
+                              int32_t* __restrict__ leaf_of, int32_t* __restrict__ first_leaf,
+                              int* __restrict__ status) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int err = 0;
+    bool beyond = false;
+    double x0 = sorted_x[0];
+    int leaf = descend_one(D, x0, sorted_p[0], beyond, 0, Lx, periodic != 0, err);
+    sorted_x[0] = x0;
+    for (int r = 0; r < np; ++r) {
+        beyond = false;
+        const size_t k = (size_t)r * nx;
+        double tx = sorted_x[k];
+        leaf = walk_one(W, tx, sorted_p[k], beyond, leaf, Lx, periodic != 0, err);
+        sorted_x[k] = tx;
+        first_leaf[r] = leaf;
+        leaf_of[k] = beyond ? 0 : leaf;
+    }
+    if (err) atomicOr(status, kStatusSearchFailed);
+}
+
+__global__ __launch_bounds__(64) void rows_kernel(const WalkRec* __restrict__ W, int nx, int np,
+                                                  double Lx, int periodic,
+                                                  double* __restrict__ sorted_x,
+                                                  const double* __restrict__ sorted_p,
+                                                  int32_t* __restrict__ leaf_of,
+                                                  const int32_t* __restrict__ first_leaf,
+                                                  int* __restrict__ status) {
+    const int r = blockIdx.x * 64 + threadIdx.x;
+    if (r >= np) return;
+    int err = 0;
+    int lc = first_leaf[r];
+    size_t k = (size_t)r * nx;
+    for (int c = 1; c < nx; ++c) {
+        bool beyond = false;
+        ++k;
+        double tx = sorted_x[k];
+        lc = walk_one(W, tx, sorted_p[k], beyond, lc, Lx, periodic != 0, err);
+        sorted_x[k] = tx;
+        leaf_of[k] = beyond ? 0 : lc;
+    }
+    if (err) atomicOr(status, kStatusSearchFailed);
+}
+
+// The batched evaluation of AMRStructure_interp.cpp:943-962: row of
+// monomials times coefficients, summed in order.
+__global__ __launch_bounds__(256) void eval_grid_kernel(const CoefRec* __restrict__ coef, int n,
+                                                        const double* __restrict__ sorted_x,
+                                                        const double* __restrict__ sorted_p,
+                                                        const int32_t* __restrict__ leaf_of,
+                                                        const int32_t* __restrict__ sorted_idx,
+                                                        double f_beyond, double* __restrict__ f) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int leaf = leaf_of[k];
+    double v;
+    if (leaf == 0) {
+        v = f_beyond;
+    } else {
+        const CoefRec& c = coef[leaf];
+        const double dx = sorted_x[k] - c.x4, dp = sorted_p[k] - c.p4;
+        const double dxsq = dx * dx, dpsq = dp * dp;
+        const double m[9] = {1, dx, dx * dp, dp, dxsq, dxsq * dp, dxsq * dpsq, dx * dpsq, dpsq};
+        v = 0.0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) v = v + m[j] * c.c[j];
+    }
+    f[sorted_idx[k]] = v;
+}
+
+// interpolate_from_mesh (AMRStructure_interp.cpp:1094-1143) at n points.
+__global__ __launch_bounds__(256) void points_kernel(const DescRec* __restrict__ D,
+                                                     const CoefRec* __restrict__ coef,
+                                                     const double* __restrict__ x,
+                                                     const double* __restrict__ p, int n,
+                                                     const double* __restrict__ ox,
+                                                     const double* __restrict__ op, double Lx,
+                                                     int periodic, double f_beyond,
+                                                     double* __restrict__ f,
+                                                     int* __restrict__ status) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const RootCorners rc = load_corners(ox, op);
+    double xt = x[i];
+    const double pt = p[i];
+    if (periodic && !shift_one(xt, pt, rc, Lx)) {
+        atomicOr(status, kStatusTooManyShifts);
+        f[i] = 0.0;
+        return;
+    }
+    bool beyond = false;
+    int err = 0;
+    int leaf = descend_one(D, xt, pt, beyond, 0, Lx, periodic != 0, err);
+    if (err) atomicOr(status, kStatusSearchFailed);
+    if (beyond) leaf = 0;
+    double v;
+    if (leaf == 0) {
+        v = f_beyond;
+    } else {
+        const CoefRec& c = coef[leaf];
+        const double* cc = c.c;
+        const double dx = xt - c.x4, dp = pt - c.p4;
+        v = cc[0] + cc[1] * dx + cc[2] * dx * dp + cc[3] * dp + cc[4] * dx * dx +
+            cc[5] * dx * dx * dp + cc[6] * dx * dx * dp * dp + cc[7] * dx * dp * dp +
+            cc[8] * dp * dp;
+    }
+    f[i] = v;
+}
+
+__global__ void zero_status_kernel(int* status) {
+    if (threadIdx.x == 0) *status = 0;
+}
+
+// ---------------------------------------------------------------- host side
+
+size_t OldMesh::bytes(int npanels) {
+    return align_up(sizeof(CoefRec) * npanels, 256) + align_up(sizeof(WalkRec) * npanels, 256) +
+           align_up(sizeof(DescRec) * npanels, 256);
+}
+
+void OldMesh::carve(void* base, int np_) {
+    char* s = static_cast<char*>(base);
+    npanels = np_;
+    coef = reinterpret_cast<CoefRec*>(s);
+    s += align_up(sizeof(CoefRec) * np_, 256);
+    walk = reinterpret_cast<WalkRec*>(s);
+    s += align_up(sizeof(WalkRec) * np_, 256);
+    desc = reinterpret_cast<DescRec*>(s);
+}
+
+int prepare_old_mesh(const int32_t* d_rec, int npanels, const double* d_ox, const double* d_op,
+                     const double* d_of, OldMesh& m, hipStream_t stream) {
+    if (npanels <= 0) return fail(FS_E_INVALID, "remesh: empty old mesh");
+    m.ox = d_ox;
+    m.op = d_op;
+    coef_kernel<<<(npanels + 63) / 64, 64, 0, stream>>>(d_rec, npanels, d_ox, d_op, d_of,
+                                                        m.coef);
+    FS_HIP_CHECK(hipGetLastError());
+    records_kernel<<<(npanels + 255) / 256, 256, 0, stream>>>(d_rec, npanels, d_ox, d_op,
+                                                              m.walk, m.desc);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+size_t GridWork::bytes(int n, int np) {
+    return align_up(sizeof(double) * n, 256) * 3 + align_up(sizeof(int32_t) * n, 256) * 2 +
+           align_up(sizeof(int32_t) * np, 256) + 256;
+}
+
+void GridWork::carve(void* base, int n, int np) {
+    char* s = static_cast<char*>(base);
+    shifted = reinterpret_cast<double*>(s);
+    s += align_up(sizeof(double) * n, 256);
+    sorted_x = reinterpret_cast<double*>(s);
+    s += align_up(sizeof(double) * n, 256);
+    sorted_p = reinterpret_cast<double*>(s);
+    s += align_up(sizeof(double) * n, 256);
+    sorted_idx = reinterpret_cast<int32_t*>(s);
+    s += align_up(sizeof(int32_t) * n, 256);
+    leaf_of = reinterpret_cast<int32_t*>(s);
+    s += align_up(sizeof(int32_t) * n, 256);
+    first_leaf = reinterpret_cast<int32_t*>(s);
+    s += align_up(sizeof(int32_t) * np, 256);
+    status = reinterpret_cast<int*>(s);
+}
+
+int interp_to_grid(const OldMesh& m, const double* d_x,
+                   const double* d_p, const int32_t* d_row_pts, int nx, int np, double Lx,
+                   bool periodic, double f_beyond, double* d_f, GridWork& w,
+                   hipStream_t stream) {
+    const int n = nx * np;
+    if (nx <= 0 || np <= 0) return fail(FS_E_INVALID, "remesh: empty grid");
+    zero_status_kernel<<<1, 64, 0, stream>>>(w.status);
+    shift_kernel<<<(n + 255) / 256, 256, 0, stream>>>(d_x, d_p, n, m.ox, m.op, Lx,
+                                                      periodic ? 1 : 0, w.shifted, w.status);
+    const int tpb = nx >= 256 ? 256 : ((nx + 63) / 64) * 64;
+    rowsort_kernel<<<np, tpb, sizeof(double) * nx, stream>>>(d_row_pts, nx, w.shifted, d_p,
+                                                              w.sorted_x, w.sorted_p,
+                                                              w.sorted_idx);
+    column_kernel<<<1, 64, 0, stream>>>(m.walk, m.desc, nx, np, Lx, periodic ? 1 : 0,
+                                        w.sorted_x, w.sorted_p, w.leaf_of, w.first_leaf,
+                                        w.status);
+    rows_kernel<<<(np + 63) / 64, 64, 0, stream>>>(m.walk, nx, np, Lx, periodic ? 1 : 0,
+                                                   w.sorted_x, w.sorted_p, w.leaf_of,
+                                                   w.first_leaf, w.status);
+    eval_grid_kernel<<<(n + 255) / 256, 256, 0, stream>>>(m.coef, n, w.sorted_x, w.sorted_p,
+                                                          w.leaf_of, w.sorted_idx, f_beyond,
+                                                          d_f);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int interp_points(const OldMesh& m, const double* d_x, const double* d_p,
+                  int n, double Lx, bool periodic, double f_beyond, double* d_f, int* d_status,
+                  hipStream_t stream) {
+    if (n <= 0) return FS_OK;
+    zero_status_kernel<<<1, 64, 0, stream>>>(d_status);
+    points_kernel<<<(n + 255) / 256, 256, 0, stream>>>(m.desc, m.coef, d_x, d_p, n, m.ox, m.op,
+                                                       Lx, periodic ? 1 : 0, f_beyond, d_f,
+                                                       d_status);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int status_to_error(int status) {
+    if (status & kStatusTooManyShifts)
+        return fail(FS_E_RANGE, "remesh: a point is more than 10 periods away (too many shifts!)");
+    if (status & kStatusSearchFailed)
+        return fail(FS_E_INTERNAL, "remesh: leaf search did not terminate");
+    return FS_OK;
+}
+
+}  // namespace remesh
+}  // namespace fs

@@ -1,0 +1,732 @@
+// The FARRSIGHT step loop on the device: AMRSimulation (reference
+// src/simulation_files/AMRSimulation*.cpp) with every species' state
+// resident in HBM.
+//
+// Layout: the species are concatenated (AMRSimulation::gather order) in
+// one set of arrays xs, ps, q_ws, es, fs; species s owns [off_s, off_s+n_s).
+// A step enqueues, on the context's stream and without host round trips for
+// non-adaptive decks:
+//   rk4 stage 1 | field | stage 2 | field | stage 3 | field | final update
+//   (AMRSimulation_step.cpp:122-263; contraction off, so every update rounds
+//   as the reference's expression), then either
+//   remesh per species (old-mesh coefficients and search records, grid
+//   interpolation, reset of x/p to the base grid, q_ws = base weight * f) and
+//   the reduced-x field (segment sums in point order, direct sum over the
+//   unique x of all species, scatter back; AMRSimulation_field.cpp:5-162,
+//   AMRStructure.cpp:147-196), or a full field evaluation.
+// Adaptive decks rebuild the refined tree on the host after the grid
+// interpolation, with every refinement pass' new-point f evaluated by one
+// device launch (interpolate_from_mesh), then re-upload.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "fs_mesh.h"
+#include "fs_remesh.h"
+
+namespace fs {
+namespace sim {
+
+constexpr int kMaxSpecies = 8;
+constexpr int kBlock = 256;
+
+struct SpeciesTable {
+    int n;
+    int off[kMaxSpecies + 1];
+    double qm[kMaxSpecies];
+};
+
+__device__ __forceinline__ double qm_of(const SpeciesTable& T, int i) {
+    int s = 0;
+    while (s + 1 < T.n && i >= T.off[s + 1]) ++s;
+    return T.qm[s];
+}
+
+// k1 -> k2 (AMRSimulation_step.cpp:148-164): ef1 = es q/m, p2 = p + dt/2 ef1,
+// x_k = x + dt/2 p1 (p is the velocity).
+__global__ __launch_bounds__(kBlock) void rk4_stage1(SpeciesTable T, int n,
+                                                     const double* __restrict__ xs,
+                                                     const double* __restrict__ ps,
+                                                     const double* __restrict__ es, double h,
+                                                     double* __restrict__ p2,
+                                                     double* __restrict__ xk) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double ef1 = es[i] * qm_of(T, i);
+    p2[i] = ps[i] + h * ef1;
+    xk[i] = xs[i] + h * ps[i];
+}
+
+// k2 -> k3 (:179-189)
+__global__ __launch_bounds__(kBlock) void rk4_stage2(SpeciesTable T, int n,
+                                                     const double* __restrict__ xs,
+                                                     const double* __restrict__ ps,
+                                                     const double* __restrict__ e2, double h,
+                                                     const double* __restrict__ p2,
+                                                     double* __restrict__ p3,
+                                                     double* __restrict__ xk) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double ef2 = e2[i] * qm_of(T, i);
+    p3[i] = ps[i] + h * ef2;
+    xk[i] = xs[i] + h * p2[i];
+}
+
+// k3 -> k4 (:212-220)
+__global__ __launch_bounds__(kBlock) void rk4_stage3(SpeciesTable T, int n,
+                                                     const double* __restrict__ xs,
+                                                     const double* __restrict__ ps,
+                                                     const double* __restrict__ e3, double dt,
+                                                     const double* __restrict__ p3,
+                                                     double* __restrict__ p4,
+                                                     double* __restrict__ xk) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double ef3 = e3[i] * qm_of(T, i);
+    p4[i] = ps[i] + dt * ef3;
+    xk[i] = xs[i] + dt * p3[i];
+}
+
+// u_{n+1} = u_n + dt/6 (k1 + 2 k2 + 2 k3 + k4) (:236-248)
+__global__ __launch_bounds__(kBlock) void rk4_final(
+    SpeciesTable T, int n, double* __restrict__ xs, double* __restrict__ ps,
+    const double* __restrict__ es, const double* __restrict__ e2, const double* __restrict__ e3,
+    const double* __restrict__ e4, const double* __restrict__ p2, const double* __restrict__ p3,
+    const double* __restrict__ p4, double dt6) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double qm = qm_of(T, i);
+    const double p1 = ps[i];
+    const double ef1 = es[i] * qm, ef2 = e2[i] * qm, ef3 = e3[i] * qm, ef4 = e4[i] * qm;
+    xs[i] = xs[i] + dt6 * (p1 + 2 * p2[i] + 2 * p3[i] + p4[i]);
+    ps[i] = p1 + dt6 * (ef1 + 2 * ef2 + 2 * ef3 + ef4);
+}
+
+// After a remesh: points back on the base grid, f interpolated, q_ws = base
+// weight * f (AMRStructure_amr.cpp:1096-1099).
+__global__ __launch_bounds__(kBlock) void reset_points(int n, const double* __restrict__ gx,
+                                                       const double* __restrict__ gp,
+                                                       const double* __restrict__ fnew,
+                                                       const double* __restrict__ bw,
+                                                       double* __restrict__ xs,
+                                                       double* __restrict__ ps,
+                                                       double* __restrict__ fs,
+                                                       double* __restrict__ qw) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double f = fnew[i];
+    xs[i] = gx[i];
+    ps[i] = gp[i];
+    fs[i] = f;
+    qw[i] = bw[i] * f;
+}
+
+// reduced_ws[u] += q_ws[i] over the points of each unique x, in point order
+// (AMRStructure.cpp:177-180).
+__global__ __launch_bounds__(kBlock) void reduced_sum(int U, const int32_t* __restrict__ offsets,
+                                                      const int32_t* __restrict__ members,
+                                                      const double* __restrict__ qw,
+                                                      double* __restrict__ rw) {
+    const int u = blockIdx.x * kBlock + threadIdx.x;
+    if (u >= U) return;
+    double s = 0.0;
+    for (int k = offsets[u]; k < offsets[u + 1]; ++k) s += qw[members[k]];
+    rw[u] = s;
+}
+
+// es[i] = reduced_es[unique(i)] (AMRStructure.cpp:191-195).
+__global__ __launch_bounds__(kBlock) void scatter_reduced(int n,
+                                                          const int32_t* __restrict__ unique_of,
+                                                          const double* __restrict__ re,
+                                                          double* __restrict__ es) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    es[i] = re[unique_of[i]];
+}
+
+inline unsigned blocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+__global__ void zero_ints(int* a, int n) {
+    if ((int)threadIdx.x < n) a[threadIdx.x] = 0;
+}
+
+// f0 of the deck (src/structure_files/initial_distributions.cpp:12-152 via
+// AMRSimulation_load.cpp:149-221), same expression order.
+double initial_f(const fs_species_params& sp, double kx, double x, double p) {
+    const double pth = sp.pth, pstr = sp.pstr, amp = sp.amp;
+    auto maxw = [](double v, double th, double c) {
+        return 1.0 / std::sqrt(2.0 * M_PI) / th * std::exp(-(v - c) * (v - c) / 2 / th / th);
+    };
+    switch (sp.ics_type) {
+        case 3:
+            return 1.0 / std::sqrt(2.0 * M_PI) / pth *
+                   std::exp(-(p - 0.0) * (p - 0.0) / 2 / pth / pth) *
+                   (1 + amp * std::cos(kx * x)) * p * p;
+        case 4:
+            return .5 * (maxw(p - pstr, pth, 0.0) + maxw(p + pstr, pth, 0.0)) *
+                   (1 + amp * std::cos(kx * x));
+        case 11: {
+            const double nb = sp.beam_frac;
+            return ((1.0 - nb) * maxw(p, pth, pstr) + nb * maxw(p, sp.beam_vth, sp.beam_v)) *
+                   (1 + amp * std::cos(kx * x));
+        }
+        default:
+            return 1.0 / std::sqrt(2.0 * M_PI) / pth *
+                   std::exp(-(p - pstr) * (p - pstr) / 2 / pth / pth) *
+                   (1 + amp * std::cos(kx * x));
+    }
+}
+
+struct Species {
+    fs_species_params prm{};
+    double q = -1, qm = -1, m = 1, kx = 0;
+    mesh::Spec spec;
+    bool adaptive = false;
+    mesh::PanelTree base;  // the uniform base mesh (structure and grid)
+    mesh::PanelTree tree;  // the current mesh
+    int nx = 0, np = 0;
+    std::vector<int32_t> rows;
+    std::vector<double> base_w;
+    mesh::ReducedMap red;
+    double f_beyond = 0;
+
+    DevBuf d_rec, d_grid, d_bw, d_fnew, d_old, d_work;
+    double *gx = nullptr, *gp = nullptr;
+    int32_t* rows_d = nullptr;
+    remesh::OldMesh old;
+    remesh::GridWork work;
+
+    int n() const { return (int)tree.xs.size(); }
+    int npanels() const { return tree.size(); }
+};
+
+}  // namespace sim
+}  // namespace fs
+
+using fs::DevBuf;
+namespace msh = fs::mesh;
+namespace rm = fs::remesh;
+
+struct fs_sim {
+    fs_ctx* ctx = nullptr;
+    fs_sim_params prm{};
+    std::vector<std::unique_ptr<fs::sim::Species>> sp;
+    fs::sim::SpeciesTable T{};
+    int N = 0;
+    int U = 0;
+    int iter = 0;
+    double t = 0;
+    bool any_adaptive = false;
+    // concatenated state: xs, ps, qw, es, fs; RK4 work: xk, p2, p3, p4, e2, e3, e4
+    DevBuf state, rk, red, fscratch, status;
+    double *xs = nullptr, *ps = nullptr, *qw = nullptr, *es = nullptr, *fs_ = nullptr;
+    double *xk = nullptr, *p2 = nullptr, *p3 = nullptr, *p4 = nullptr, *e2 = nullptr,
+           *e3 = nullptr, *e4 = nullptr;
+    double *rx = nullptr, *rw = nullptr, *re = nullptr;
+    int32_t *r_off = nullptr, *r_mem = nullptr, *r_uof = nullptr;
+    int* d_status = nullptr;
+
+    double Lx() const { return prm.x_max - prm.x_min; }
+    hipStream_t stream() const { return ctx->stream; }
+};
+
+namespace fs {
+namespace sim {
+namespace {
+
+using Sim = ::fs_sim;
+
+int field_eval(Sim& S, double* out, const double* x, const double* w, int n) {
+    if (S.prm.use_treecode > 0)
+        return fail(FS_E_INVALID, "use_treecode: the device treecode is not part of this build");
+    const size_t need = field_scratch_bytes(n, n, S.ctx->num_cus, S.ctx->tuning);
+    FS_TRY(S.fscratch.ensure(need, S.stream()));
+    return field_direct_f64(x, n, x, w, n, S.Lx(), S.prm.greens_epsilon, out, S.fscratch.ptr,
+                            S.ctx->num_cus, S.ctx->tuning, S.stream());
+}
+
+// Per-species device copies of what a remesh reads (grid, rows, base weights,
+// panel records) and its work space.
+int upload_species(Sim& S, Species& s, bool grid_too) {
+    hipStream_t st = S.stream();
+    const int n = s.n(), nb = s.nx * s.np, P = s.npanels();
+    FS_TRY(s.d_rec.ensure(sizeof(int32_t) * s.tree.rec.size(), st));
+    FS_HIP_CHECK(hipMemcpyAsync(s.d_rec.ptr, s.tree.rec.data(), sizeof(int32_t) * s.tree.rec.size(),
+                                hipMemcpyHostToDevice, st));
+    FS_TRY(s.d_bw.ensure(sizeof(double) * n, st));
+    FS_HIP_CHECK(hipMemcpyAsync(s.d_bw.ptr, s.base_w.data(), sizeof(double) * n,
+                                hipMemcpyHostToDevice, st));
+    if (grid_too) {
+        const size_t bg = align_up(sizeof(double) * nb, 256);
+        FS_TRY(s.d_grid.ensure(2 * bg + sizeof(int32_t) * nb, st));
+        s.gx = s.d_grid.as<double>(0);
+        s.gp = s.d_grid.as<double>(bg);
+        s.rows_d = s.d_grid.as<int32_t>(2 * bg);
+        FS_HIP_CHECK(hipMemcpyAsync(s.gx, s.base.xs.data(), sizeof(double) * nb,
+                                    hipMemcpyHostToDevice, st));
+        FS_HIP_CHECK(hipMemcpyAsync(s.gp, s.base.ps.data(), sizeof(double) * nb,
+                                    hipMemcpyHostToDevice, st));
+        FS_HIP_CHECK(hipMemcpyAsync(s.rows_d, s.rows.data(), sizeof(int32_t) * nb,
+                                    hipMemcpyHostToDevice, st));
+    }
+    FS_TRY(s.d_fnew.ensure(sizeof(double) * std::max(n, nb), st));
+    FS_TRY(s.d_old.ensure(rm::OldMesh::bytes(P), st));
+    s.old.carve(s.d_old.ptr, P);
+    FS_TRY(s.d_work.ensure(rm::GridWork::bytes(nb, s.np), st));
+    s.work.carve(s.d_work.ptr, nb, s.np);
+    return FS_OK;
+}
+
+// (Re)allocates the concatenated arrays for the current species sizes and
+// uploads every species' host state (x, p, f, q_ws; es of each species when
+// given) plus the global reduced-x map.
+int layout(Sim& S, const std::vector<std::vector<double>>* es_host) {
+    hipStream_t st = S.stream();
+    int N = 0;
+    S.T.n = (int)S.sp.size();
+    for (size_t k = 0; k < S.sp.size(); ++k) {
+        S.T.off[k] = N;
+        S.T.qm[k] = S.sp[k]->qm;
+        N += S.sp[k]->n();
+    }
+    S.T.off[S.sp.size()] = N;
+    S.N = N;
+    const size_t bN = align_up(sizeof(double) * N, 256);
+    FS_TRY(S.state.ensure(5 * bN, st));
+    S.xs = S.state.as<double>(0);
+    S.ps = S.state.as<double>(bN);
+    S.qw = S.state.as<double>(2 * bN);
+    S.es = S.state.as<double>(3 * bN);
+    S.fs_ = S.state.as<double>(4 * bN);
+    FS_TRY(S.rk.ensure(7 * bN, st));
+    double* r = S.rk.as<double>(0);
+    S.xk = r;
+    S.p2 = S.rk.as<double>(bN);
+    S.p3 = S.rk.as<double>(2 * bN);
+    S.p4 = S.rk.as<double>(3 * bN);
+    S.e2 = S.rk.as<double>(4 * bN);
+    S.e3 = S.rk.as<double>(5 * bN);
+    S.e4 = S.rk.as<double>(6 * bN);
+    // global reduced map: species' unique x concatenated (duplicates across
+    // species kept, AMRSimulation_field.cpp:9-13)
+    std::vector<double> rxh;
+    std::vector<int32_t> off(1, 0), mem, uof(N);
+    int ubase = 0;
+    for (size_t k = 0; k < S.sp.size(); ++k) {
+        const Species& s = *S.sp[k];
+        const int o = S.T.off[k];
+        rxh.insert(rxh.end(), s.red.ux.begin(), s.red.ux.end());
+        for (size_t u = 0; u < s.red.ux.size(); ++u) {
+            for (int m = s.red.offsets[u]; m < s.red.offsets[u + 1]; ++m)
+                mem.push_back(o + s.red.members[m]);
+            off.push_back((int32_t)mem.size());
+        }
+        for (int i = 0; i < s.n(); ++i) uof[o + i] = ubase + s.red.unique_of[i];
+        ubase += (int)s.red.ux.size();
+    }
+    S.U = ubase;
+    const size_t bU = align_up(sizeof(double) * S.U, 256);
+    const size_t bo = align_up(sizeof(int32_t) * (S.U + 1), 256);
+    const size_t bm = align_up(sizeof(int32_t) * N, 256);
+    FS_TRY(S.red.ensure(3 * bU + bo + 2 * bm, st));
+    S.rx = S.red.as<double>(0);
+    S.rw = S.red.as<double>(bU);
+    S.re = S.red.as<double>(2 * bU);
+    S.r_off = S.red.as<int32_t>(3 * bU);
+    S.r_mem = S.red.as<int32_t>(3 * bU + bo);
+    S.r_uof = S.red.as<int32_t>(3 * bU + bo + bm);
+    FS_HIP_CHECK(hipMemcpyAsync(S.rx, rxh.data(), sizeof(double) * S.U, hipMemcpyHostToDevice, st));
+    FS_HIP_CHECK(hipMemcpyAsync(S.r_off, off.data(), sizeof(int32_t) * (S.U + 1),
+                                hipMemcpyHostToDevice, st));
+    FS_HIP_CHECK(hipMemcpyAsync(S.r_mem, mem.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
+    FS_HIP_CHECK(hipMemcpyAsync(S.r_uof, uof.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
+    for (size_t k = 0; k < S.sp.size(); ++k) {
+        Species& s = *S.sp[k];
+        const int o = S.T.off[k], n = s.n();
+        std::vector<double> qw(n);
+        for (int i = 0; i < n; ++i) qw[i] = s.base_w[i] * s.tree.fs[i];
+        FS_HIP_CHECK(hipMemcpyAsync(S.xs + o, s.tree.xs.data(), sizeof(double) * n,
+                                    hipMemcpyHostToDevice, st));
+        FS_HIP_CHECK(hipMemcpyAsync(S.ps + o, s.tree.ps.data(), sizeof(double) * n,
+                                    hipMemcpyHostToDevice, st));
+        FS_HIP_CHECK(hipMemcpyAsync(S.fs_ + o, s.tree.fs.data(), sizeof(double) * n,
+                                    hipMemcpyHostToDevice, st));
+        FS_HIP_CHECK(hipMemcpyAsync(S.qw + o, qw.data(), sizeof(double) * n,
+                                    hipMemcpyHostToDevice, st));
+        if (es_host)
+            FS_HIP_CHECK(hipMemcpyAsync(S.es + o, (*es_host)[k].data(), sizeof(double) * n,
+                                        hipMemcpyHostToDevice, st));
+        FS_TRY(upload_species(S, s, true));
+    }
+    FS_TRY(S.status.ensure(sizeof(int) * kMaxSpecies, st));
+    S.d_status = S.status.as<int>();
+    zero_ints<<<1, 64, 0, st>>>(S.d_status, kMaxSpecies);
+    FS_HIP_CHECK(hipGetLastError());
+    // the host copies above must outlive the async copies
+    FS_HIP_CHECK(hipStreamSynchronize(st));
+    return FS_OK;
+}
+
+// evaluate_field_uniform_grid (AMRSimulation_field.cpp:5-162)
+int field_reduced(Sim& S) {
+    hipStream_t st = S.stream();
+    reduced_sum<<<blocks(S.U), kBlock, 0, st>>>(S.U, S.r_off, S.r_mem, S.qw, S.rw);
+    FS_HIP_CHECK(hipGetLastError());
+    FS_TRY(field_eval(S, S.re, S.rx, S.rw, S.U));
+    scatter_reduced<<<blocks(S.N), kBlock, 0, st>>>(S.N, S.r_uof, S.re, S.es);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+// rk4_step (AMRSimulation_step.cpp:122-263)
+int rk4(Sim& S) {
+    hipStream_t st = S.stream();
+    const double dt = S.prm.dt, h = 0.5 * dt;
+    const int N = S.N;
+    rk4_stage1<<<blocks(N), kBlock, 0, st>>>(S.T, N, S.xs, S.ps, S.es, h, S.p2, S.xk);
+    FS_TRY(field_eval(S, S.e2, S.xk, S.qw, N));
+    rk4_stage2<<<blocks(N), kBlock, 0, st>>>(S.T, N, S.xs, S.ps, S.e2, h, S.p2, S.p3, S.xk);
+    FS_TRY(field_eval(S, S.e3, S.xk, S.qw, N));
+    rk4_stage3<<<blocks(N), kBlock, 0, st>>>(S.T, N, S.xs, S.ps, S.e3, dt, S.p3, S.p4, S.xk);
+    FS_TRY(field_eval(S, S.e4, S.xk, S.qw, N));
+    rk4_final<<<blocks(N), kBlock, 0, st>>>(S.T, N, S.xs, S.ps, S.es, S.e2, S.e3, S.e4, S.p2,
+                                            S.p3, S.p4, dt / 6.0);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int read_status(Sim& S) {
+    int h[kMaxSpecies] = {0};
+    const int n = (int)S.sp.size();
+    FS_HIP_CHECK(hipMemcpyAsync(h, S.d_status, sizeof(int) * n, hipMemcpyDeviceToHost,
+                                S.stream()));
+    FS_HIP_CHECK(hipStreamSynchronize(S.stream()));
+    int all = 0;
+    for (int k = 0; k < n; ++k) all |= h[k];
+    return rm::status_to_error(all);
+}
+
+__global__ void copy_status(const int* src, int* dst) {
+    if (threadIdx.x == 0) *dst = *src;
+}
+
+// Grid interpolation of one species from its current (advected) mesh; f
+// onto s.d_fnew. Status into S.d_status[k].
+int interp_species_grid(Sim& S, int k) {
+    Species& s = *S.sp[k];
+    hipStream_t st = S.stream();
+    const int o = S.T.off[k];
+    FS_TRY(rm::prepare_old_mesh(s.d_rec.as<int32_t>(), s.npanels(), S.xs + o, S.ps + o,
+                                S.fs_ + o, s.old, st));
+    FS_TRY(rm::interp_to_grid(s.old, s.gx, s.gp, s.rows_d, s.nx, s.np, S.Lx(),
+                              S.prm.bcs == 0, s.f_beyond, s.d_fnew.as<double>(), s.work, st));
+    copy_status<<<1, 64, 0, st>>>(s.work.status, S.d_status + k);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+// AMRSimulation::remesh (AMRSimulation_remesh.cpp:3-48), device-only path.
+int remesh_static(Sim& S) {
+    hipStream_t st = S.stream();
+    for (size_t k = 0; k < S.sp.size(); ++k) {
+        Species& s = *S.sp[k];
+        const int o = S.T.off[k];
+        FS_TRY(interp_species_grid(S, (int)k));
+        reset_points<<<blocks(s.n()), kBlock, 0, st>>>(s.n(), s.gx, s.gp, s.d_fnew.as<double>(),
+                                                       s.d_bw.as<double>(), S.xs + o, S.ps + o,
+                                                       S.fs_ + o, S.qw + o);
+        FS_HIP_CHECK(hipGetLastError());
+    }
+    return field_reduced(S);
+}
+
+// Remesh with adaptive refinement: base-grid f on the device, refinement on
+// the host with batched device interpolation of new points.
+int remesh_adaptive(Sim& S) {
+    hipStream_t st = S.stream();
+    std::vector<std::vector<double>> es_unused;
+    for (size_t k = 0; k < S.sp.size(); ++k) {
+        Species& s = *S.sp[k];
+        const int o = S.T.off[k];
+        FS_TRY(interp_species_grid(S, (int)k));
+        const int nb = s.nx * s.np;
+        mesh::PanelTree t = s.base;
+        FS_HIP_CHECK(hipMemcpyAsync(t.fs.data(), s.d_fnew.ptr, sizeof(double) * nb,
+                                    hipMemcpyDeviceToHost, st));
+        FS_TRY(read_status(S));
+        if (s.adaptive) {
+            int rc = FS_OK;
+            DevBuf io;
+            auto eval = [&](const double* x, const double* p, size_t n, double* f) {
+                if (rc != FS_OK || n == 0) return;
+                const size_t b = align_up(sizeof(double) * n, 256);
+                if ((rc = io.ensure(3 * b, st)) != FS_OK) return;
+                double* dx = io.as<double>(0);
+                double* dp = io.as<double>(b);
+                double* df = io.as<double>(2 * b);
+                if (hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+                    hipMemcpyAsync(dp, p, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess) {
+                    rc = fail(FS_E_HIP, "adaptive remesh: upload failed");
+                    return;
+                }
+                if ((rc = rm::interp_points(s.old, dx, dp, (int)n, S.Lx(), S.prm.bcs == 0,
+                                            s.f_beyond, df, S.d_status + k, st)) != FS_OK)
+                    return;
+                if (hipMemcpyAsync(f, df, sizeof(double) * n, hipMemcpyDeviceToHost, st) != hipSuccess) {
+                    rc = fail(FS_E_HIP, "adaptive remesh: download failed");
+                    return;
+                }
+                rc = read_status(S);
+            };
+            t.refine_adaptively(s.spec, eval);
+            io.release();
+            FS_TRY(rc);
+        }
+        s.tree = std::move(t);
+        std::vector<int32_t> leaves;
+        s.tree.leaf_weights(s.q, S.prm.quadrature == 1, s.base_w, leaves);
+        s.red = mesh::reduced_map(s.tree.xs);
+        (void)o;
+    }
+    FS_TRY(layout(S, nullptr));
+    return field_reduced(S);
+}
+
+int build_species(Sim& S, Species& s) {
+    const fs_species_params& p = s.prm;
+    s.q = p.q;
+    s.qm = p.q / p.m;
+    s.m = p.m;
+    s.kx = 2.0 * M_PI / S.Lx() * p.normalized_wavenumber;
+    s.spec.x_min = S.prm.x_min;
+    s.spec.x_max = S.prm.x_max;
+    s.spec.p_min = p.p_min;
+    s.spec.p_max = p.p_max;
+    s.spec.initial_height = p.initial_height;
+    s.spec.p_height = p.p_height;
+    s.spec.max_height = p.max_height;
+    s.spec.periodic = S.prm.bcs == 0;
+    s.spec.amr_eps.assign(p.amr_epsilons, p.amr_epsilons + std::max(0, std::min(5, p.n_amr_epsilons)));
+    s.adaptive = p.do_adaptively_refine != 0 && !s.spec.amr_eps.empty();
+    s.nx = (1 << (p.initial_height + 1)) + 1;
+    s.np = (1 << (p.initial_height + p.p_height + 1)) + 1;
+    try {
+        s.base.build_base(s.spec);
+    } catch (...) {
+        return fail(FS_E_INVALID, "species: initial_height must be >= 1");
+    }
+    if ((int)s.base.xs.size() != s.nx * s.np)
+        return fail(FS_E_INTERNAL, "species: base grid has %zu points, expected %d",
+                    s.base.xs.size(), s.nx * s.np);
+    s.rows = mesh::grid_rows(s.base.xs, s.base.ps, s.nx, s.np);
+    // generate_mesh(f0, adaptive, initial) (AMRStructure_amr.cpp:841-969)
+    s.tree = s.base;
+    for (size_t i = 0; i < s.tree.xs.size(); ++i)
+        s.tree.fs[i] = initial_f(p, s.kx, s.tree.xs[i], s.tree.ps[i]);
+    if (s.adaptive) {
+        auto f0 = [&](const double* x, const double* pp, size_t n, double* f) {
+            for (size_t i = 0; i < n; ++i) f[i] = initial_f(p, s.kx, x[i], pp[i]);
+        };
+        s.tree.refine_adaptively(s.spec, f0);
+    }
+    std::vector<int32_t> leaves;
+    s.tree.leaf_weights(s.q, S.prm.quadrature == 1, s.base_w, leaves);
+    s.f_beyond = *std::min_element(s.tree.fs.begin(), s.tree.fs.end());  // AMRStructure.cpp:124
+    s.red = mesh::reduced_map(s.tree.xs);
+    return FS_OK;
+}
+
+int download_species(Sim& S, int k, double* xs, double* ps, double* fs, double* qw, double* es) {
+    hipStream_t st = S.stream();
+    const int o = S.T.off[k], n = S.sp[k]->n();
+    double* dst[5] = {xs, ps, fs, qw, es};
+    const double* src[5] = {S.xs, S.ps, S.fs_, S.qw, S.es};
+    for (int a = 0; a < 5; ++a)
+        if (dst[a])
+            FS_HIP_CHECK(hipMemcpyAsync(dst[a], src[a] + o, sizeof(double) * n,
+                                        hipMemcpyDeviceToHost, st));
+    FS_HIP_CHECK(hipStreamSynchronize(st));
+    return FS_OK;
+}
+
+}  // namespace
+}  // namespace sim
+}  // namespace fs
+
+using namespace fs;
+
+extern "C" {
+
+int fs_sim_create(fs_ctx* ctx, const fs_sim_params* prm, const fs_species_params* species,
+                  int n_species, fs_sim** out) {
+    if (!ctx || !prm || !species || !out) return fail(FS_E_INVALID, "fs_sim_create: null argument");
+    *out = nullptr;
+    if (n_species < 1 || n_species > sim::kMaxSpecies)
+        return fail(FS_E_INVALID, "fs_sim_create: 1..%d species", sim::kMaxSpecies);
+    if (!(prm->x_max > prm->x_min) || !(prm->dt > 0) || prm->remesh_period < 1)
+        return fail(FS_E_INVALID, "fs_sim_create: bad domain, dt or remesh_period");
+    if (prm->bcs != 0) return fail(FS_E_INVALID, "fs_sim_create: only periodic decks (bcs 0)");
+    for (int k = 0; k < n_species; ++k) {
+        const fs_species_params& p = species[k];
+        if (p.initial_height < 1 || p.initial_height > 12 || p.p_height < 0 ||
+            p.initial_height + p.p_height > 13 || !(p.p_max > p.p_min) || p.m == 0.0)
+            return fail(FS_E_INVALID, "fs_sim_create: species %d has bad mesh parameters", k);
+    }
+    FS_HIP_CHECK(hipSetDevice(ctx->device));
+    auto S = std::make_unique<fs_sim>();
+    S->ctx = ctx;
+    S->prm = *prm;
+    for (int k = 0; k < n_species; ++k) {
+        auto s = std::make_unique<sim::Species>();
+        s->prm = species[k];
+        FS_TRY(sim::build_species(*S, *s));
+        S->any_adaptive = S->any_adaptive || s->adaptive;
+        S->sp.push_back(std::move(s));
+    }
+    FS_TRY(sim::layout(*S, nullptr));
+    FS_TRY(sim::field_reduced(*S));  // AMRSimulation.cpp:84-89
+    FS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = S.release();
+    return FS_OK;
+}
+
+int fs_mesh_build(const fs_sim_params* prm, const fs_species_params* species,
+                  int64_t* n_points, int64_t* n_panels, int32_t* panels, double* xs, double* ps,
+                  double* fs, double* q_ws) {
+    if (!prm || !species) return fail(FS_E_INVALID, "fs_mesh_build: null argument");
+    if (!(prm->x_max > prm->x_min)) return fail(FS_E_INVALID, "fs_mesh_build: x_max <= x_min");
+    const fs_species_params& p = *species;
+    if (p.initial_height < 1 || p.initial_height > 12 || p.p_height < 0 ||
+        p.initial_height + p.p_height > 13 || !(p.p_max > p.p_min) || p.m == 0.0)
+        return fail(FS_E_INVALID, "fs_mesh_build: bad mesh parameters");
+    fs_sim S;
+    S.prm = *prm;
+    sim::Species s;
+    s.prm = p;
+    FS_TRY(sim::build_species(S, s));
+    if (n_points) *n_points = s.n();
+    if (n_panels) *n_panels = s.npanels();
+    if (panels) std::copy(s.tree.rec.begin(), s.tree.rec.end(), panels);
+    if (xs) std::copy(s.tree.xs.begin(), s.tree.xs.end(), xs);
+    if (ps) std::copy(s.tree.ps.begin(), s.tree.ps.end(), ps);
+    if (fs) std::copy(s.tree.fs.begin(), s.tree.fs.end(), fs);
+    if (q_ws)
+        for (int i = 0; i < s.n(); ++i) q_ws[i] = s.base_w[i] * s.tree.fs[i];
+    return FS_OK;
+}
+
+int fs_sim_destroy(fs_sim* S) {
+    if (!S) return FS_OK;
+    (void)hipStreamSynchronize(S->ctx->stream);
+    for (auto& s : S->sp) {
+        s->d_rec.release();
+        s->d_grid.release();
+        s->d_bw.release();
+        s->d_fnew.release();
+        s->d_old.release();
+        s->d_work.release();
+    }
+    S->state.release();
+    S->rk.release();
+    S->red.release();
+    S->fscratch.release();
+    S->status.release();
+    delete S;
+    return FS_OK;
+}
+
+int fs_sim_step(fs_sim* S) {
+    if (!S) return fail(FS_E_INVALID, "null simulation");
+    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
+    FS_TRY(sim::rk4(*S));
+    S->iter += 1;
+    S->t += S->prm.dt;
+    if (S->iter % S->prm.remesh_period == 0) {
+        if (S->any_adaptive) {
+            FS_TRY(sim::remesh_adaptive(*S));
+        } else {
+            FS_TRY(sim::remesh_static(*S));
+        }
+        return sim::read_status(*S);
+    }
+    FS_TRY(sim::field_eval(*S, S->es, S->xs, S->qw, S->N));
+    FS_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+    return FS_OK;
+}
+
+int fs_sim_iter(fs_sim* S, int* iter) {
+    if (!S || !iter) return fail(FS_E_INVALID, "null argument");
+    *iter = S->iter;
+    return FS_OK;
+}
+
+int fs_sim_species_counts(fs_sim* S, int k, int64_t* n_points, int64_t* n_panels) {
+    if (!S || k < 0 || k >= (int)S->sp.size()) return fail(FS_E_INVALID, "bad species index");
+    if (n_points) *n_points = S->sp[k]->n();
+    if (n_panels) *n_panels = S->sp[k]->npanels();
+    return FS_OK;
+}
+
+int fs_sim_get_points(fs_sim* S, int k, double* xs, double* ps, double* fs, double* q_ws,
+                      double* es) {
+    if (!S || k < 0 || k >= (int)S->sp.size()) return fail(FS_E_INVALID, "bad species index");
+    return sim::download_species(*S, k, xs, ps, fs, q_ws, es);
+}
+
+int fs_sim_get_panels(fs_sim* S, int k, int32_t* panels) {
+    if (!S || !panels || k < 0 || k >= (int)S->sp.size())
+        return fail(FS_E_INVALID, "bad species index");
+    const auto& r = S->sp[k]->tree.rec;
+    std::copy(r.begin(), r.end(), panels);
+    return FS_OK;
+}
+
+int fs_sim_diagnostics(fs_sim* S, int k, double out[4]) {
+    if (!S || !out || k < 0 || k >= (int)S->sp.size())
+        return fail(FS_E_INVALID, "bad species index");
+    const sim::Species& s = *S->sp[k];
+    const int n = s.n();
+    std::vector<double> xs(n), ps(n), qw(n), es(n);
+    FS_TRY(sim::download_species(*S, k, xs.data(), ps.data(), nullptr, qw.data(), es.data()));
+    double charge = 0, mom = 0, kin = 0;
+    for (int i = 0; i < n; ++i) {
+        charge += qw[i];
+        mom += ps[i] * qw[i];
+        kin += ps[i] * ps[i] * qw[i];
+    }
+    out[0] = charge;
+    out[1] = 1.0 / s.q * mom;
+    out[2] = 1.0 / 2.0 / s.qm * kin;
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return xs[a] < xs[b]; });
+    std::vector<double> ux, ue;
+    const double L = S->Lx();
+    for (int j = 0; j < n; ++j) {
+        const double x = xs[idx[j]];
+        if (j > 0 && x == xs[idx[j - 1]]) continue;
+        if (x >= 0 && x < L) {
+            ux.push_back(x);
+            ue.push_back(es[idx[j]]);
+        }
+    }
+    const size_t m = ux.size();
+    double pot = 0;
+    if (m >= 3) {
+        for (size_t j = 0; j < m; ++j) {
+            const double d = (j == 0) ? .5 * (ux[1] + L - ux[m - 1])
+                             : (j == m - 1) ? .5 * (ux[0] + L - ux[m - 2])
+                                            : .5 * (ux[j + 1] - ux[j - 1]);
+            pot += d * ue[j] * ue[j];
+        }
+        pot *= 0.5;
+    }
+    out[3] = pot;
+    return FS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+"""The device-resident step loop (fs_sim) against the oracle's step loop.
+
+Bars (per species, after every step): panel tree bit-exact, points back on
+the grid bit-exact after a remesh, E normwise within 1e-12 of the oracle,
+f within 1e-12, charge/momentum/energy/electric energy within 1e-10
+(relative). Decks: two-species weak Landau damping (ions m = 1836) with
+remesh every step and every second step, Simpson quadrature, and an
+adaptively refined two-stream deck.
+"""
+import numpy as np
+import pytest
+
+import multi_species_amd as ms
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+L0 = 4 * np.pi
+
+
+def to_ms(deck: oracle.Deck):
+    sim = ms.sim_params(x_min=deck.x_min, x_max=deck.x_max, bcs=deck.bcs,
+                        quadrature=deck.quadrature, greens_epsilon=deck.greens_epsilon,
+                        use_treecode=deck.use_treecode, mac=deck.mac, degree=deck.degree,
+                        max_source=deck.max_source, dt=deck.dt,
+                        remesh_period=deck.remesh_period)
+    sps = [ms.species_params(**{k: getattr(s, k) for k in (
+        "q", "m", "p_min", "p_max", "initial_height", "p_height", "max_height",
+        "do_adaptively_refine", "amr_epsilons", "ics_type", "normalized_wavenumber", "amp",
+        "pth", "pstr", "beam_frac", "beam_v", "beam_vth")}) for s in deck.species]
+    return sim, sps
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def compare(gsim, osim, nsp, step, tol_e=1e-12, tol_f=1e-12, tol_d=1e-10, expect_grid=True):
+    worst = 0.0
+    for k in range(nsp):
+        xs, ps, fs, qw, es = gsim.points(k)
+        o = osim.species(k)
+        oxs, ops, ofs, oqw = o.points()
+        oes = osim.es(k)
+        assert np.array_equal(gsim.panels(k), o.panels()), f"step {step} species {k}: tree"
+        if expect_grid:
+            assert np.array_equal(xs, oxs) and np.array_equal(ps, ops), f"step {step}: grid"
+        else:
+            assert rel(xs, oxs) < 1e-13 and rel(ps, ops) < 1e-12
+        ef, ee = rel(fs, ofs), rel(es, oes)
+        assert ef <= tol_f, f"step {step} species {k}: f {ef}"
+        assert ee <= tol_e, f"step {step} species {k}: E {ee}"
+        gd, od = gsim.diagnostics(k), osim.diagnostics(k)
+        dd = np.abs(gd - od) / np.maximum(np.abs(od), 1e-300)
+        # momentum sits at round-off (~1e-16); compare it absolutely against the charge scale
+        dd[1] = abs(gd[1] - od[1]) / max(abs(od[0]), 1.0)
+        assert np.all(dd <= tol_d), f"step {step} species {k}: diagnostics {dd}"
+        worst = max(worst, ee, ef)
+    return worst
+
+
+@pytest.mark.parametrize("remesh_period,quad", [(1, 0), (2, 0), (1, 1)])
+def test_landau_two_species_matches_oracle(gpu_ctx, remesh_period, quad):
+    deck = oracle.landau_deck(initial_height=4, p_height=1, quadrature=quad)
+    deck.remesh_period = remesh_period
+    osim = oracle.OracleSim(deck)
+    gsim = ms.Simulation(gpu_ctx, *to_ms(deck))
+    compare(gsim, osim, 2, 0)
+    worst = 0.0
+    for step in range(1, 9):
+        osim.step()
+        gsim.step()
+        assert gsim.iter == osim.iter == step
+        worst = max(worst, compare(gsim, osim, 2, step,
+                                   expect_grid=(step % remesh_period == 0)))
+    print(f"remesh_period={remesh_period} quad={quad}: worst rel E/f diff {worst:.2e}")
+    gsim.close()
+
+
+def test_adaptive_two_stream_matches_oracle(gpu_ctx):
+    e = oracle.Species(name="electrons", q=-1.0, m=1.0, p_min=-6.0, p_max=6.0,
+                       initial_height=4, p_height=1, max_height=6,
+                       do_adaptively_refine=True, amr_epsilons=[2e-3], ics_type=4,
+                       normalized_wavenumber=1.0, amp=1e-3, pth=0.5, pstr=2.4)
+    i = oracle.Species(name="ions", q=1.0, m=100.0, p_min=-1.0, p_max=1.0, initial_height=4,
+                       p_height=1, ics_type=1, amp=0.0, pth=0.1)
+    deck = oracle.Deck(x_min=0.0, x_max=10 * np.pi, quadrature=0, greens_epsilon=0.05,
+                       dt=0.05, remesh_period=1, species=[e, i])
+    osim = oracle.OracleSim(deck)
+    gsim = ms.Simulation(gpu_ctx, *to_ms(deck))
+    n0 = gsim.counts(0)
+    assert n0 == (osim.species(0).n, osim.species(0).panels().shape[0])
+    assert n0[0] > 33 * 65  # refined beyond the base grid
+    compare(gsim, osim, 2, 0)
+    for step in range(1, 5):
+        osim.step()
+        gsim.step()
+        compare(gsim, osim, 2, step)
+    gsim.close()
