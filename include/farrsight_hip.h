@@ -57,6 +57,17 @@ int fs_emq_periodic_f64_dev(fs_ctx* ctx, double* d_es, const double* d_targets,
                             int64_t nt, const double* d_sources, const double* d_q_ws,
                             int64_t ns, double L, double epsilon, void* stream);
 
+/* Replaces E_MQ_Treecode::operator() (src/FieldStructure.cpp:230-1050):
+ * the periodic barycentric-Lagrange treecode (bisection tree with leaves of
+ * fewer than max_source points, MAC theta = mac with the periodic distance,
+ * Chebyshev degree `degree` <= 15), evaluated at the sources like the
+ * reference (es[0:ns]). Tree built on the host, moments and far/near sums on
+ * the device. evals (may be NULL) receives {far, near} kernel-evaluation
+ * counts. Host pointers, synchronous. */
+int fs_emq_treecode_f64(fs_ctx* ctx, double* es, const double* sources, const double* q_ws,
+                        int64_t ns, double L, double epsilon, double mac, int degree,
+                        int max_source, int64_t* evals);
+
 /* Tuning knobs of the direct-sum kernel (0 = automatic). targets_per_thread
  * in {1,2,4}; source_splits >= 1. Applies to later calls on this context. */
 int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits);

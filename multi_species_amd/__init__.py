@@ -46,6 +46,8 @@ def lib():
         L.fs_emq_periodic_f64.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
         L.fs_emq_periodic_f64_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d, _vp]
         L.fs_emq_set_tuning.argtypes = [_vp, _int, _int]
+        L.fs_emq_treecode_f64.argtypes = [_vp, _dp, _dp, _dp, _i64, _d, _d, _d, _int, _int,
+                                          ctypes.POINTER(_i64)]
         _ip = ctypes.POINTER(ctypes.c_int32)
         L.fs_remesh_interp_grid.argtypes = [_vp, _ip, _i64, _dp, _dp, _dp, _i64, _dp, _dp, _int,
                                             _int, _d, _d, _int, _d, _dp]
@@ -122,6 +124,18 @@ class Context:
         _check(lib().fs_emq_periodic_f64(self.h, _P(es), _P(t), t.size, _P(s), _P(w),
                                          s.size, L, eps), "fs_emq_periodic_f64")
         return es
+
+    def emq_treecode_f64(self, sources, q_ws, L: float, eps: float, mac: float, degree: int,
+                         max_source: int, return_evals: bool = False):
+        """E_MQ_Treecode::operator() on host arrays: E at the sources."""
+        s, w = _f64(sources), _f64(q_ws)
+        if s.size != w.size:
+            raise ValueError("sources and q_ws differ in length")
+        es = np.empty(s.size)
+        ev = (_i64 * 2)()
+        _check(lib().fs_emq_treecode_f64(self.h, _P(es), _P(s), _P(w), s.size, L, eps, mac,
+                                         degree, max_source, ev), "fs_emq_treecode_f64")
+        return (es, (ev[0], ev[1])) if return_evals else es
 
     def emq_periodic_f64_dev(self, es_ptr: int, t_ptr: int, nt: int, s_ptr: int, w_ptr: int,
                              ns: int, L: float, eps: float, stream: Optional[int] = None):

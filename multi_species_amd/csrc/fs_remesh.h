@@ -21,6 +21,9 @@ enum : int {
     kFlagRefinedP = 16,
 };
 
+// Largest grid row / column handled by one workgroup of the search kernels.
+constexpr int kMaxLine = 1024;
+
 // Interpolant of one old panel: coefficients of
 // (1, dx, dx dp, dp, dx^2, dx^2 dp, dx^2 dp^2, dx dp^2, dp^2) about point 4.
 struct CoefRec {
@@ -69,41 +72,64 @@ struct RootCorners {
 
 enum : int { kStatusTooManyShifts = 1, kStatusSearchFailed = 2 };
 
-struct OldMesh {
-    CoefRec* coef = nullptr;
-    WalkRec* walk = nullptr;
-    DescRec* desc = nullptr;
-    const double* ox = nullptr;  // advected old points (root corners are 0, 2, 6, 8)
-    const double* op = nullptr;
-    int npanels = 0;
-    static size_t bytes(int npanels);
-    void carve(void* base, int npanels);
+// One interpolation job: an old (advected) mesh and the points to
+// interpolate onto. Grid jobs: np rows of nx points listed by `rows`; point
+// jobs: n = nx points x, p (np = 1, rows unused). Every kernel of this module
+// takes an array of jobs and works on job blockIdx.y, so all species (and
+// all decks of a batch) share one launch per kernel.
+struct Job {
+    // old mesh
+    const int32_t* rec;
+    int npanels;
+    const double* ox;
+    const double* op;
+    const double* of;
+    CoefRec* coef;
+    WalkRec* walk;
+    DescRec* desc;
+    // target points
+    const double* x;
+    const double* p;
+    const int32_t* rows;
+    int nx, np;
+    double Lx;
+    int periodic;
+    double f_beyond;
+    // work
+    double* shifted;
+    double* sorted_x;
+    double* sorted_p;
+    int32_t* sorted_idx;
+    int32_t* leaf_of;
+    int32_t* first_leaf;
+    // output
+    double* f;
+    int* status;
 };
 
-struct GridWork {
-    double* shifted = nullptr;
-    double* sorted_x = nullptr;
-    double* sorted_p = nullptr;
-    int32_t* sorted_idx = nullptr;
-    int32_t* leaf_of = nullptr;
-    int32_t* first_leaf = nullptr;
-    int* status = nullptr;
-    static size_t bytes(int n, int np);
-    void carve(void* base, int n, int np);
+// Sizes of the per-job work arrays.
+size_t mesh_bytes(int npanels);
+size_t grid_bytes(int n, int np);
+// Points a job's mesh and grid work arrays into the given device blocks.
+void carve_mesh(Job& j, void* base, int npanels);
+void carve_grid(Job& j, void* base, int n, int np);
+
+struct Launch {
+    int njobs = 0;
+    int max_panels = 0;
+    int max_points = 0;  // max nx*np over jobs
+    int max_rows = 0;    // max np
+    int max_nx = 0;
 };
 
-// Builds coefficient, walk and descent records of an old mesh.
-int prepare_old_mesh(const int32_t* d_rec, int npanels, const double* d_ox, const double* d_op,
-                     const double* d_of, OldMesh& m, hipStream_t stream);
-// interpolate_to_initial_xps: d_row_pts lists the np rows (ascending p) of nx
-// point indices each. Writes d_f[0:nx*np]; w.status gets error bits.
-int interp_to_grid(const OldMesh& m, const double* d_x,
-                   const double* d_p, const int32_t* d_row_pts, int nx, int np, double Lx,
-                   bool periodic, double f_beyond, double* d_f, GridWork& w, hipStream_t stream);
-// interpolate_from_mesh at n points.
-int interp_points(const OldMesh& m, const double* d_x, const double* d_p,
-                  int n, double Lx, bool periodic, double f_beyond, double* d_f, int* d_status,
-                  hipStream_t stream);
+// Coefficient, walk and descent records of every job's old mesh.
+int prepare_old_meshes(const Job* d_jobs, const Launch& l, hipStream_t stream);
+// interpolate_to_initial_xps for every (grid) job.
+int interp_grids(const Job* d_jobs, const Launch& l, hipStream_t stream);
+// interpolate_from_mesh for every (point) job.
+int interp_points(const Job* d_jobs, const Launch& l, hipStream_t stream);
+// Clears the status word of every job.
+int clear_status(const Job* d_jobs, const Launch& l, hipStream_t stream);
 // Maps status bits to an FS_E_* code (and the error text).
 int status_to_error(int status);
 

@@ -17,9 +17,16 @@
 //   shift    one thread per new point: the periodic pre-shift of
 //            AMRStructure_interp.cpp:429-491.
 //   rowsort  one workgroup per grid row: rank sort of the row by shifted x.
-//   column   one thread: root descent of the first sorted point and the
-//            sequential walk down the first column (:610-650).
-//   rows     one thread per row: the neighbour walk along the row (:696-722).
+//   column   one workgroup: the walk down the first column (:610-650).
+//   rows     one workgroup per row: the walk along the row (:696-722).
+//            The reference walks each line sequentially, every point
+//            starting from its predecessor's leaf. Here every point first
+//            gets a guess (root descent, then a walk), then walks again from
+//            its predecessor's guess, all in parallel; where every guess up
+//            to a point equals the leaf the walk from the predecessor's guess
+//            returned, that walk started from the true predecessor leaf and
+//            is exactly the reference's. From the first mismatch on the line
+//            is walked sequentially as in the reference.
 //   eval     one thread per point: the interpolant (:943-962).
 //   points   one thread per point: shift, root descent, single-point
 //            interpolant (interpolate_from_mesh, for AMR refinement).
@@ -35,19 +42,23 @@ namespace fs {
 namespace remesh {
 
 // Panel record layout (int32 x kPanelInts): see fs_remesh.h.
+constexpr int kLineThreads = 256;
+constexpr int kLinePerThread = kMaxLine / kLineThreads;
 constexpr int kMaxWalkSteps = 256;
 constexpr int kMaxHistory = 32;
 constexpr int kMaxDescend = 256;
 
-__global__ __launch_bounds__(64) void coef_kernel(const int32_t* __restrict__ rec, int npanels,
-                                                  const double* __restrict__ ox,
-                                                  const double* __restrict__ op,
-                                                  const double* __restrict__ of,
-                                                  CoefRec* __restrict__ coef) {
+__global__ __launch_bounds__(64) void coef_kernel(const Job* __restrict__ jobs) {
     __shared__ double lds[90 * 64];
+    const Job& J = jobs[blockIdx.y];
     const int lane = threadIdx.x;
     const int pi = blockIdx.x * 64 + lane;
-    if (pi >= npanels) return;
+    if (pi >= J.npanels) return;
+    const int32_t* __restrict__ rec = J.rec;
+    const double* __restrict__ ox = J.ox;
+    const double* __restrict__ op = J.op;
+    const double* __restrict__ of = J.of;
+    CoefRec* __restrict__ coef = J.coef;
     const int32_t* r = rec + (size_t)pi * kPanelInts;
     double* A = lds;  // element e of this lane at A[e * 64 + lane]
 #define AE(i, j) A[((i) * 9 + (j)) * 64 + lane]
@@ -142,13 +153,15 @@ __global__ __launch_bounds__(64) void coef_kernel(const int32_t* __restrict__ re
     coef[pi] = out;
 }
 
-__global__ __launch_bounds__(256) void records_kernel(const int32_t* __restrict__ rec,
-                                                      int npanels, const double* __restrict__ ox,
-                                                      const double* __restrict__ op,
-                                                      WalkRec* __restrict__ walk,
-                                                      DescRec* __restrict__ desc) {
+__global__ __launch_bounds__(256) void records_kernel(const Job* __restrict__ jobs) {
+    const Job& J = jobs[blockIdx.y];
     const int pi = blockIdx.x * 256 + threadIdx.x;
-    if (pi >= npanels) return;
+    if (pi >= J.npanels) return;
+    const int32_t* __restrict__ rec = J.rec;
+    const double* __restrict__ ox = J.ox;
+    const double* __restrict__ op = J.op;
+    WalkRec* __restrict__ walk = J.walk;
+    DescRec* __restrict__ desc = J.desc;
     const int32_t* r = rec + (size_t)pi * kPanelInts;
     const int flags = r[17];
     const int nb[4] = {r[15], r[14], r[16], r[13]};  // right, top, bottom, left
@@ -407,32 +420,28 @@ __device__ __forceinline__ RootCorners load_corners(const double* __restrict__ o
     return rc;
 }
 
-__global__ __launch_bounds__(256) void shift_kernel(const double* __restrict__ x,
-                                                    const double* __restrict__ p, int n,
-                                                    const double* __restrict__ ox,
-                                                    const double* __restrict__ op, double Lx,
-                                                    int periodic, double* __restrict__ sx,
-                                                    int* __restrict__ status) {
+
+__global__ __launch_bounds__(256) void shift_kernel(const Job* __restrict__ jobs) {
+    const Job& J = jobs[blockIdx.y];
+    const int n = J.nx * J.np;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const RootCorners rc = load_corners(ox, op);
-    double xt = x[i];
-    if (periodic && !shift_one(xt, p[i], rc, Lx)) atomicOr(status, kStatusTooManyShifts);
-    sx[i] = xt;
+    double xt = J.x[i];
+    if (J.periodic && !shift_one(xt, J.p[i], load_corners(J.ox, J.op), J.Lx))
+        atomicOr(J.status, kStatusTooManyShifts);
+    J.shifted[i] = xt;
 }
 
-// One workgroup per row: each thread ranks its point by (shifted x, index)
-// against the whole row and writes it to its sorted slot.
-__global__ __launch_bounds__(256) void rowsort_kernel(const int32_t* __restrict__ row_pts,
-                                                      int nx, const double* __restrict__ sx,
-                                                      const double* __restrict__ p,
-                                                      double* __restrict__ sorted_x,
-                                                      double* __restrict__ sorted_p,
-                                                      int32_t* __restrict__ sorted_idx) {
-    extern __shared__ double keys[];
+// One workgroup per row: each point's rank by (shifted x, position in the
+// row list) is its sorted slot.
+__global__ __launch_bounds__(256) void rowsort_kernel(const Job* __restrict__ jobs) {
+    __shared__ double keys[kMaxLine];
+    const Job& J = jobs[blockIdx.y];
     const int row = blockIdx.x;
-    const int32_t* rp = row_pts + (size_t)row * nx;
-    for (int k = threadIdx.x; k < nx; k += blockDim.x) keys[k] = sx[rp[k]];
+    if (row >= J.np) return;
+    const int nx = J.nx;
+    const int32_t* rp = J.rows + (size_t)row * nx;
+    for (int k = threadIdx.x; k < nx; k += blockDim.x) keys[k] = J.shifted[rp[k]];
     __syncthreads();
     for (int k = threadIdx.x; k < nx; k += blockDim.x) {
         const double kk = keys[k];
@@ -443,214 +452,288 @@ __global__ __launch_bounds__(256) void rowsort_kernel(const int32_t* __restrict_
         }
         const int32_t idx = rp[k];
         const size_t o = (size_t)row * nx + rank;
-        sorted_x[o] = kk;
-        sorted_p[o] = p[idx];
-        sorted_idx[o] = idx;
+        J.sorted_x[o] = kk;
+        J.sorted_p[o] = J.p[idx];
+        J.sorted_idx[o] = idx;
     }
 }
 
-__global__ void column_kernel(const WalkRec* __restrict__ W, const DescRec* __restrict__ D,
-                              int nx, int np, double Lx, int periodic,
-                              double* __restrict__ sorted_x, const double* __restrict__ sorted_p,
-                              int32_t* __restrict__ leaf_of, int32_t* __restrict__ first_leaf,
-                              int* __restrict__ status) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The walk along one line of points, each starting from its predecessor's
+// leaf (see the file comment). Line element k is sorted point base + k*stride.
+// first_start: the exact leaf the walk of element 0 starts from; element 0's
+// x may already have been moved by the root descent (x0_override).
+// Results: leaf per element (walk result, also the next start), beyond flag,
+// the element's final x.
+struct LineOut {
+    int leaf[kLinePerThread];
+    double tx[kLinePerThread];
+    bool beyond[kLinePerThread];
+    bool failed[kLinePerThread];
+};
+
+__device__ void walk_line(const Job& J, size_t base, size_t stride, int len, int first_start,
+                          bool element0_exact_from_start, int* s_guess, int* s_leaf, int* s_first,
+                          LineOut& out, int& err) {
+    const bool periodic = J.periodic != 0;
+    const int tid = threadIdx.x;
+    // A: a guess per element from a root descent and a walk (a failed guess
+    // only costs the sequential fallback, so its error flag is dropped);
+    // element 0 of the first column walks exactly from its given start
+    int guess_err = 0;
+#pragma unroll
+    for (int q = 0; q < kLinePerThread; ++q) {
+        const int k = tid + q * kLineThreads;
+        if (k >= len) continue;
+        double x = J.sorted_x[base + (size_t)k * stride];
+        const double p = J.sorted_p[base + (size_t)k * stride];
+        bool bb = false;
+        if (k == 0 && element0_exact_from_start) {
+            s_guess[0] = walk_one(J.walk, x, p, bb, first_start, J.Lx, periodic, err);
+            continue;
+        }
+        const int d = descend_one(J.desc, x, p, bb, 0, J.Lx, periodic, guess_err);
+        bb = false;
+        s_guess[k] = walk_one(J.walk, x, p, bb, d, J.Lx, periodic, guess_err);
+    }
+    __syncthreads();
+    // B: walk every element from its predecessor's guess (element 0: from the
+    // exact start)
+#pragma unroll
+    for (int q = 0; q < kLinePerThread; ++q) {
+        const int k = tid + q * kLineThreads;
+        if (k >= len) continue;
+        const int start = (k == 0) ? first_start : s_guess[k - 1];
+        double tx = J.sorted_x[base + (size_t)k * stride];
+        bool bb = false;
+        int werr = 0;
+        const int lf = walk_one(J.walk, tx, J.sorted_p[base + (size_t)k * stride], bb, start,
+                                J.Lx, periodic, werr);
+        out.failed[q] = werr != 0;
+        out.leaf[q] = lf;
+        out.tx[q] = tx;
+        out.beyond[q] = bb;
+        s_leaf[k] = lf;
+    }
+    __syncthreads();
+    // C: first element whose exact walk disagrees with the guess its
+    // successor started from
+    if (tid == 0) *s_first = len;
+    __syncthreads();
+    for (int k = tid; k < len - 1; k += kLineThreads)
+        if (s_leaf[k] != s_guess[k]) atomicMin(s_first, k);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kLineThreads) void column_kernel(const Job* __restrict__ jobs) {
+    __shared__ int s_guess[kMaxLine], s_leaf[kMaxLine], s_first;
+    const Job& J = jobs[blockIdx.y];
+    const bool periodic = J.periodic != 0;
+    const int np = J.np, nx = J.nx;
     int err = 0;
-    bool beyond = false;
-    double x0 = sorted_x[0];
-    int leaf = descend_one(D, x0, sorted_p[0], beyond, 0, Lx, periodic != 0, err);
-    sorted_x[0] = x0;
-    for (int r = 0; r < np; ++r) {
-        beyond = false;
+    // the exact start: root descent of sorted point 0 (it may move x0)
+    __shared__ int s_start;
+    if (threadIdx.x == 0) {
+        double x0 = J.sorted_x[0];
+        bool bb = false;
+        s_start = descend_one(J.desc, x0, J.sorted_p[0], bb, 0, J.Lx, periodic, err);
+        J.sorted_x[0] = x0;
+    }
+    __syncthreads();
+    LineOut out;
+    walk_line(J, 0, (size_t)nx, np, s_start, true, s_guess, s_leaf, &s_first, out, err);
+    const int m = s_first;  // rows 0..m exact
+#pragma unroll
+    for (int q = 0; q < kLinePerThread; ++q) {
+        const int r = threadIdx.x + q * kLineThreads;
+        if (r >= np || r > m) continue;
         const size_t k = (size_t)r * nx;
-        double tx = sorted_x[k];
-        leaf = walk_one(W, tx, sorted_p[k], beyond, leaf, Lx, periodic != 0, err);
-        sorted_x[k] = tx;
-        first_leaf[r] = leaf;
-        leaf_of[k] = beyond ? 0 : leaf;
+        if (out.failed[q]) err = 1;
+        J.sorted_x[k] = out.tx[q];
+        J.first_leaf[r] = out.leaf[q];
+        J.leaf_of[k] = out.beyond[q] ? 0 : out.leaf[q];
     }
-    if (err) atomicOr(status, kStatusSearchFailed);
+    __syncthreads();
+    if (threadIdx.x == 0 && m < np - 1) {
+        int leaf = s_leaf[m];
+        for (int r = m + 1; r < np; ++r) {
+            const size_t k = (size_t)r * nx;
+            double tx = J.sorted_x[k];
+            bool bb = false;
+            leaf = walk_one(J.walk, tx, J.sorted_p[k], bb, leaf, J.Lx, periodic, err);
+            J.sorted_x[k] = tx;
+            J.first_leaf[r] = leaf;
+            J.leaf_of[k] = bb ? 0 : leaf;
+        }
+    }
+    if (err) atomicOr(J.status, kStatusSearchFailed);
 }
 
-__global__ __launch_bounds__(64) void rows_kernel(const WalkRec* __restrict__ W, int nx, int np,
-                                                  double Lx, int periodic,
-                                                  double* __restrict__ sorted_x,
-                                                  const double* __restrict__ sorted_p,
-                                                  int32_t* __restrict__ leaf_of,
-                                                  const int32_t* __restrict__ first_leaf,
-                                                  int* __restrict__ status) {
-    const int r = blockIdx.x * 64 + threadIdx.x;
-    if (r >= np) return;
+__global__ __launch_bounds__(kLineThreads) void rows_kernel(const Job* __restrict__ jobs) {
+    __shared__ int s_guess[kMaxLine], s_leaf[kMaxLine], s_first;
+    const Job& J = jobs[blockIdx.y];
+    const int r = blockIdx.x;
+    if (r >= J.np) return;
+    const bool periodic = J.periodic != 0;
+    const int nx = J.nx;
     int err = 0;
-    int lc = first_leaf[r];
-    size_t k = (size_t)r * nx;
-    for (int c = 1; c < nx; ++c) {
-        bool beyond = false;
-        ++k;
-        double tx = sorted_x[k];
-        lc = walk_one(W, tx, sorted_p[k], beyond, lc, Lx, periodic != 0, err);
-        sorted_x[k] = tx;
-        leaf_of[k] = beyond ? 0 : lc;
+    // elements 1..nx-1 of the row, starting from the first column's leaf
+    const size_t base = (size_t)r * nx + 1;
+    LineOut out;
+    walk_line(J, base, 1, nx - 1, J.first_leaf[r], false, s_guess, s_leaf, &s_first, out, err);
+    const int m = s_first;
+#pragma unroll
+    for (int q = 0; q < kLinePerThread; ++q) {
+        const int k = threadIdx.x + q * kLineThreads;
+        if (k >= nx - 1 || k > m) continue;
+        if (out.failed[q]) err = 1;
+        J.sorted_x[base + k] = out.tx[q];
+        J.leaf_of[base + k] = out.beyond[q] ? 0 : out.leaf[q];
     }
-    if (err) atomicOr(status, kStatusSearchFailed);
+    __syncthreads();
+    if (threadIdx.x == 0 && m < nx - 2) {
+        int leaf = s_leaf[m];
+        for (int k = m + 1; k < nx - 1; ++k) {
+            double tx = J.sorted_x[base + k];
+            bool bb = false;
+            leaf = walk_one(J.walk, tx, J.sorted_p[base + k], bb, leaf, J.Lx, periodic, err);
+            J.sorted_x[base + k] = tx;
+            J.leaf_of[base + k] = bb ? 0 : leaf;
+        }
+    }
+    if (err) atomicOr(J.status, kStatusSearchFailed);
 }
 
 // The batched evaluation of AMRStructure_interp.cpp:943-962: row of
 // monomials times coefficients, summed in order.
-__global__ __launch_bounds__(256) void eval_grid_kernel(const CoefRec* __restrict__ coef, int n,
-                                                        const double* __restrict__ sorted_x,
-                                                        const double* __restrict__ sorted_p,
-                                                        const int32_t* __restrict__ leaf_of,
-                                                        const int32_t* __restrict__ sorted_idx,
-                                                        double f_beyond, double* __restrict__ f) {
+__global__ __launch_bounds__(256) void eval_grid_kernel(const Job* __restrict__ jobs) {
+    const Job& J = jobs[blockIdx.y];
+    const int n = J.nx * J.np;
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
-    const int leaf = leaf_of[k];
+    const int leaf = J.leaf_of[k];
     double v;
     if (leaf == 0) {
-        v = f_beyond;
+        v = J.f_beyond;
     } else {
-        const CoefRec& c = coef[leaf];
-        const double dx = sorted_x[k] - c.x4, dp = sorted_p[k] - c.p4;
+        const CoefRec& c = J.coef[leaf];
+        const double dx = J.sorted_x[k] - c.x4, dp = J.sorted_p[k] - c.p4;
         const double dxsq = dx * dx, dpsq = dp * dp;
         const double m[9] = {1, dx, dx * dp, dp, dxsq, dxsq * dp, dxsq * dpsq, dx * dpsq, dpsq};
         v = 0.0;
 #pragma unroll
         for (int j = 0; j < 9; ++j) v = v + m[j] * c.c[j];
     }
-    f[sorted_idx[k]] = v;
+    J.f[J.sorted_idx[k]] = v;
 }
 
-// interpolate_from_mesh (AMRStructure_interp.cpp:1094-1143) at n points.
-__global__ __launch_bounds__(256) void points_kernel(const DescRec* __restrict__ D,
-                                                     const CoefRec* __restrict__ coef,
-                                                     const double* __restrict__ x,
-                                                     const double* __restrict__ p, int n,
-                                                     const double* __restrict__ ox,
-                                                     const double* __restrict__ op, double Lx,
-                                                     int periodic, double f_beyond,
-                                                     double* __restrict__ f,
-                                                     int* __restrict__ status) {
+// interpolate_from_mesh (AMRStructure_interp.cpp:1094-1143) at n = nx points.
+__global__ __launch_bounds__(256) void points_kernel(const Job* __restrict__ jobs) {
+    const Job& J = jobs[blockIdx.y];
+    const int n = J.nx;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const RootCorners rc = load_corners(ox, op);
-    double xt = x[i];
-    const double pt = p[i];
-    if (periodic && !shift_one(xt, pt, rc, Lx)) {
-        atomicOr(status, kStatusTooManyShifts);
-        f[i] = 0.0;
+    double xt = J.x[i];
+    const double pt = J.p[i];
+    if (J.periodic && !shift_one(xt, pt, load_corners(J.ox, J.op), J.Lx)) {
+        atomicOr(J.status, kStatusTooManyShifts);
+        J.f[i] = 0.0;
         return;
     }
     bool beyond = false;
     int err = 0;
-    int leaf = descend_one(D, xt, pt, beyond, 0, Lx, periodic != 0, err);
-    if (err) atomicOr(status, kStatusSearchFailed);
+    int leaf = descend_one(J.desc, xt, pt, beyond, 0, J.Lx, J.periodic != 0, err);
+    if (err) atomicOr(J.status, kStatusSearchFailed);
     if (beyond) leaf = 0;
     double v;
     if (leaf == 0) {
-        v = f_beyond;
+        v = J.f_beyond;
     } else {
-        const CoefRec& c = coef[leaf];
+        const CoefRec& c = J.coef[leaf];
         const double* cc = c.c;
         const double dx = xt - c.x4, dp = pt - c.p4;
         v = cc[0] + cc[1] * dx + cc[2] * dx * dp + cc[3] * dp + cc[4] * dx * dx +
             cc[5] * dx * dx * dp + cc[6] * dx * dx * dp * dp + cc[7] * dx * dp * dp +
             cc[8] * dp * dp;
     }
-    f[i] = v;
+    J.f[i] = v;
 }
 
-__global__ void zero_status_kernel(int* status) {
-    if (threadIdx.x == 0) *status = 0;
+__global__ void clear_status_kernel(const Job* __restrict__ jobs, int njobs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < njobs) *jobs[j].status = 0;
 }
 
 // ---------------------------------------------------------------- host side
 
-size_t OldMesh::bytes(int npanels) {
+size_t mesh_bytes(int npanels) {
     return align_up(sizeof(CoefRec) * npanels, 256) + align_up(sizeof(WalkRec) * npanels, 256) +
            align_up(sizeof(DescRec) * npanels, 256);
 }
 
-void OldMesh::carve(void* base, int np_) {
+void carve_mesh(Job& j, void* base, int np_) {
     char* s = static_cast<char*>(base);
-    npanels = np_;
-    coef = reinterpret_cast<CoefRec*>(s);
+    j.coef = reinterpret_cast<CoefRec*>(s);
     s += align_up(sizeof(CoefRec) * np_, 256);
-    walk = reinterpret_cast<WalkRec*>(s);
+    j.walk = reinterpret_cast<WalkRec*>(s);
     s += align_up(sizeof(WalkRec) * np_, 256);
-    desc = reinterpret_cast<DescRec*>(s);
+    j.desc = reinterpret_cast<DescRec*>(s);
 }
 
-int prepare_old_mesh(const int32_t* d_rec, int npanels, const double* d_ox, const double* d_op,
-                     const double* d_of, OldMesh& m, hipStream_t stream) {
-    if (npanels <= 0) return fail(FS_E_INVALID, "remesh: empty old mesh");
-    m.ox = d_ox;
-    m.op = d_op;
-    coef_kernel<<<(npanels + 63) / 64, 64, 0, stream>>>(d_rec, npanels, d_ox, d_op, d_of,
-                                                        m.coef);
-    FS_HIP_CHECK(hipGetLastError());
-    records_kernel<<<(npanels + 255) / 256, 256, 0, stream>>>(d_rec, npanels, d_ox, d_op,
-                                                              m.walk, m.desc);
-    FS_HIP_CHECK(hipGetLastError());
-    return FS_OK;
-}
-
-size_t GridWork::bytes(int n, int np) {
+size_t grid_bytes(int n, int np) {
     return align_up(sizeof(double) * n, 256) * 3 + align_up(sizeof(int32_t) * n, 256) * 2 +
            align_up(sizeof(int32_t) * np, 256) + 256;
 }
 
-void GridWork::carve(void* base, int n, int np) {
+void carve_grid(Job& j, void* base, int n, int np) {
     char* s = static_cast<char*>(base);
-    shifted = reinterpret_cast<double*>(s);
+    j.shifted = reinterpret_cast<double*>(s);
     s += align_up(sizeof(double) * n, 256);
-    sorted_x = reinterpret_cast<double*>(s);
+    j.sorted_x = reinterpret_cast<double*>(s);
     s += align_up(sizeof(double) * n, 256);
-    sorted_p = reinterpret_cast<double*>(s);
+    j.sorted_p = reinterpret_cast<double*>(s);
     s += align_up(sizeof(double) * n, 256);
-    sorted_idx = reinterpret_cast<int32_t*>(s);
+    j.sorted_idx = reinterpret_cast<int32_t*>(s);
     s += align_up(sizeof(int32_t) * n, 256);
-    leaf_of = reinterpret_cast<int32_t*>(s);
+    j.leaf_of = reinterpret_cast<int32_t*>(s);
     s += align_up(sizeof(int32_t) * n, 256);
-    first_leaf = reinterpret_cast<int32_t*>(s);
+    j.first_leaf = reinterpret_cast<int32_t*>(s);
     s += align_up(sizeof(int32_t) * np, 256);
-    status = reinterpret_cast<int*>(s);
+    j.status = reinterpret_cast<int*>(s);
 }
 
-int interp_to_grid(const OldMesh& m, const double* d_x,
-                   const double* d_p, const int32_t* d_row_pts, int nx, int np, double Lx,
-                   bool periodic, double f_beyond, double* d_f, GridWork& w,
-                   hipStream_t stream) {
-    const int n = nx * np;
-    if (nx <= 0 || np <= 0) return fail(FS_E_INVALID, "remesh: empty grid");
-    zero_status_kernel<<<1, 64, 0, stream>>>(w.status);
-    shift_kernel<<<(n + 255) / 256, 256, 0, stream>>>(d_x, d_p, n, m.ox, m.op, Lx,
-                                                      periodic ? 1 : 0, w.shifted, w.status);
-    const int tpb = nx >= 256 ? 256 : ((nx + 63) / 64) * 64;
-    rowsort_kernel<<<np, tpb, sizeof(double) * nx, stream>>>(d_row_pts, nx, w.shifted, d_p,
-                                                              w.sorted_x, w.sorted_p,
-                                                              w.sorted_idx);
-    column_kernel<<<1, 64, 0, stream>>>(m.walk, m.desc, nx, np, Lx, periodic ? 1 : 0,
-                                        w.sorted_x, w.sorted_p, w.leaf_of, w.first_leaf,
-                                        w.status);
-    rows_kernel<<<(np + 63) / 64, 64, 0, stream>>>(m.walk, nx, np, Lx, periodic ? 1 : 0,
-                                                   w.sorted_x, w.sorted_p, w.leaf_of,
-                                                   w.first_leaf, w.status);
-    eval_grid_kernel<<<(n + 255) / 256, 256, 0, stream>>>(m.coef, n, w.sorted_x, w.sorted_p,
-                                                          w.leaf_of, w.sorted_idx, f_beyond,
-                                                          d_f);
+static unsigned cdiv(int a, int b) { return (unsigned)((a + b - 1) / b); }
+
+int clear_status(const Job* d_jobs, const Launch& l, hipStream_t stream) {
+    clear_status_kernel<<<cdiv(l.njobs, 64), 64, 0, stream>>>(d_jobs, l.njobs);
     FS_HIP_CHECK(hipGetLastError());
     return FS_OK;
 }
 
-int interp_points(const OldMesh& m, const double* d_x, const double* d_p,
-                  int n, double Lx, bool periodic, double f_beyond, double* d_f, int* d_status,
-                  hipStream_t stream) {
-    if (n <= 0) return FS_OK;
-    zero_status_kernel<<<1, 64, 0, stream>>>(d_status);
-    points_kernel<<<(n + 255) / 256, 256, 0, stream>>>(m.desc, m.coef, d_x, d_p, n, m.ox, m.op,
-                                                       Lx, periodic ? 1 : 0, f_beyond, d_f,
-                                                       d_status);
+int prepare_old_meshes(const Job* d_jobs, const Launch& l, hipStream_t stream) {
+    if (l.njobs <= 0 || l.max_panels <= 0) return fail(FS_E_INVALID, "remesh: empty old mesh");
+    coef_kernel<<<dim3(cdiv(l.max_panels, 64), l.njobs), 64, 0, stream>>>(d_jobs);
+    FS_HIP_CHECK(hipGetLastError());
+    records_kernel<<<dim3(cdiv(l.max_panels, 256), l.njobs), 256, 0, stream>>>(d_jobs);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int interp_grids(const Job* d_jobs, const Launch& l, hipStream_t stream) {
+    if (l.max_nx > kMaxLine || l.max_rows > kMaxLine)
+        return fail(FS_E_INVALID, "remesh: grid lines longer than %d points", kMaxLine);
+    const unsigned nj = (unsigned)l.njobs;
+    shift_kernel<<<dim3(cdiv(l.max_points, 256), nj), 256, 0, stream>>>(d_jobs);
+    rowsort_kernel<<<dim3((unsigned)l.max_rows, nj), 256, 0, stream>>>(d_jobs);
+    column_kernel<<<dim3(1, nj), kLineThreads, 0, stream>>>(d_jobs);
+    rows_kernel<<<dim3((unsigned)l.max_rows, nj), kLineThreads, 0, stream>>>(d_jobs);
+    eval_grid_kernel<<<dim3(cdiv(l.max_points, 256), nj), 256, 0, stream>>>(d_jobs);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int interp_points(const Job* d_jobs, const Launch& l, hipStream_t stream) {
+    if (l.max_points <= 0) return FS_OK;
+    points_kernel<<<dim3(cdiv(l.max_points, 256), (unsigned)l.njobs), 256, 0, stream>>>(d_jobs);
     FS_HIP_CHECK(hipGetLastError());
     return FS_OK;
 }

@@ -75,6 +75,36 @@ int read_status(fs_ctx* ctx, const int* d_status) {
     return remesh::status_to_error(status);
 }
 
+int run_job(fs_ctx* ctx, remesh::Job& j, int npanels, int n, int np, bool grid,
+            double* f_host) {
+    // job descriptor and its work arrays live in the context's scratch
+    const size_t bmesh = remesh::mesh_bytes(npanels);
+    const size_t bwork = remesh::grid_bytes(n, np);
+    const size_t bjob = align_up(sizeof(remesh::Job), 256);
+    FS_TRY(ctx->scratch.ensure(bmesh + bwork + bjob, ctx->stream));
+    remesh::carve_mesh(j, ctx->scratch.ptr, npanels);
+    remesh::carve_grid(j, ctx->scratch.as<char>(bmesh), n, np);
+    remesh::Job* d_job = ctx->scratch.as<remesh::Job>(bmesh + bwork);
+    FS_HIP_CHECK(hipMemcpyAsync(d_job, &j, sizeof(remesh::Job), hipMemcpyHostToDevice,
+                                ctx->stream));
+    remesh::Launch l;
+    l.njobs = 1;
+    l.max_panels = npanels;
+    l.max_points = grid ? n : j.nx;
+    l.max_rows = np;
+    l.max_nx = j.nx;
+    FS_TRY(remesh::clear_status(d_job, l, ctx->stream));
+    FS_TRY(remesh::prepare_old_meshes(d_job, l, ctx->stream));
+    if (grid) {
+        FS_TRY(remesh::interp_grids(d_job, l, ctx->stream));
+    } else {
+        FS_TRY(remesh::interp_points(d_job, l, ctx->stream));
+    }
+    FS_HIP_CHECK(hipMemcpyAsync(f_host, j.f, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    return read_status(ctx, j.status);
+}
+
 }  // namespace
 }  // namespace fs
 
@@ -92,6 +122,9 @@ int fs_remesh_interp_grid(fs_ctx* ctx, const int32_t* panels, int64_t n_panels,
     if (nx < 1 || np < 1 || (int64_t)nx * np > INT32_MAX || n_panels > INT32_MAX ||
         n_old > INT32_MAX)
         return fail(FS_E_INVALID, "fs_remesh_interp_grid: sizes out of range");
+    if (nx > remesh::kMaxLine || np > remesh::kMaxLine)
+        return fail(FS_E_INVALID, "fs_remesh_interp_grid: at most %d points per row and column",
+                    remesh::kMaxLine);
     if (!(x_max > x_min)) return fail(FS_E_INVALID, "fs_remesh_interp_grid: x_max <= x_min");
     FS_TRY(check_panels(panels, n_panels, n_old));
     const int n = nx * np;
@@ -114,9 +147,15 @@ int fs_remesh_interp_grid(fs_ctx* ctx, const int32_t* panels, int64_t n_panels,
     char* extra = nullptr;
     FS_TRY(stage_old_mesh(ctx, panels, n_panels, old_xs, old_ps, old_fs, n_old,
                           3 * bn + brow, st, extra));
+    remesh::Job j{};
+    j.rec = st.rec;
+    j.npanels = (int)n_panels;
+    j.ox = st.ox;
+    j.op = st.op;
+    j.of = st.of;
     double* d_x = reinterpret_cast<double*>(extra);
     double* d_p = reinterpret_cast<double*>(extra + bn);
-    double* d_f = reinterpret_cast<double*>(extra + 2 * bn);
+    j.f = reinterpret_cast<double*>(extra + 2 * bn);
     int32_t* d_rows = reinterpret_cast<int32_t*>(extra + 3 * bn);
     FS_HIP_CHECK(hipMemcpyAsync(d_x, new_xs, sizeof(double) * n, hipMemcpyHostToDevice,
                                 ctx->stream));
@@ -124,19 +163,15 @@ int fs_remesh_interp_grid(fs_ctx* ctx, const int32_t* panels, int64_t n_panels,
                                 ctx->stream));
     FS_HIP_CHECK(hipMemcpyAsync(d_rows, order.data(), sizeof(int32_t) * n,
                                 hipMemcpyHostToDevice, ctx->stream));
-    const size_t bmesh = remesh::OldMesh::bytes((int)n_panels);
-    const size_t bwork = remesh::GridWork::bytes(n, np);
-    FS_TRY(ctx->scratch.ensure(bmesh + bwork, ctx->stream));
-    remesh::OldMesh m;
-    m.carve(ctx->scratch.ptr, (int)n_panels);
-    remesh::GridWork w;
-    w.carve(ctx->scratch.as<char>(bmesh), n, np);
-    FS_TRY(remesh::prepare_old_mesh(st.rec, (int)n_panels, st.ox, st.op, st.of, m, ctx->stream));
-    FS_TRY(remesh::interp_to_grid(m, d_x, d_p, d_rows, nx, np,
-                                  x_max - x_min, periodic != 0, f_beyond, d_f, w, ctx->stream));
-    FS_HIP_CHECK(hipMemcpyAsync(f_out, d_f, sizeof(double) * n, hipMemcpyDeviceToHost,
-                                ctx->stream));
-    return read_status(ctx, w.status);
+    j.x = d_x;
+    j.p = d_p;
+    j.rows = d_rows;
+    j.nx = nx;
+    j.np = np;
+    j.Lx = x_max - x_min;
+    j.periodic = periodic != 0;
+    j.f_beyond = f_beyond;
+    return run_job(ctx, j, (int)n_panels, n, np, true, f_out);
 }
 
 int fs_remesh_interp_points(fs_ctx* ctx, const int32_t* panels, int64_t n_panels,
@@ -155,25 +190,27 @@ int fs_remesh_interp_points(fs_ctx* ctx, const int32_t* panels, int64_t n_panels
     const size_t bn = align_up(sizeof(double) * n, 256);
     Staged st;
     char* extra = nullptr;
-    FS_TRY(stage_old_mesh(ctx, panels, n_panels, old_xs, old_ps, old_fs, n_old, 3 * bn + 256,
-                          st, extra));
+    FS_TRY(stage_old_mesh(ctx, panels, n_panels, old_xs, old_ps, old_fs, n_old, 3 * bn, st,
+                          extra));
+    remesh::Job j{};
+    j.rec = st.rec;
+    j.npanels = (int)n_panels;
+    j.ox = st.ox;
+    j.op = st.op;
+    j.of = st.of;
     double* d_x = reinterpret_cast<double*>(extra);
     double* d_p = reinterpret_cast<double*>(extra + bn);
-    double* d_f = reinterpret_cast<double*>(extra + 2 * bn);
-    int* d_status = reinterpret_cast<int*>(extra + 3 * bn);
+    j.f = reinterpret_cast<double*>(extra + 2 * bn);
     FS_HIP_CHECK(hipMemcpyAsync(d_x, x, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     FS_HIP_CHECK(hipMemcpyAsync(d_p, p, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
-    const size_t bmesh = remesh::OldMesh::bytes((int)n_panels);
-    FS_TRY(ctx->scratch.ensure(bmesh, ctx->stream));
-    remesh::OldMesh m;
-    m.carve(ctx->scratch.ptr, (int)n_panels);
-    FS_TRY(remesh::prepare_old_mesh(st.rec, (int)n_panels, st.ox, st.op, st.of, m, ctx->stream));
-    FS_TRY(remesh::interp_points(m, d_x, d_p, (int)n,
-                                 x_max - x_min, periodic != 0, f_beyond, d_f, d_status,
-                                 ctx->stream));
-    FS_HIP_CHECK(hipMemcpyAsync(f_out, d_f, sizeof(double) * n, hipMemcpyDeviceToHost,
-                                ctx->stream));
-    return read_status(ctx, d_status);
+    j.x = d_x;
+    j.p = d_p;
+    j.nx = (int)n;
+    j.np = 1;
+    j.Lx = x_max - x_min;
+    j.periodic = periodic != 0;
+    j.f_beyond = f_beyond;
+    return run_job(ctx, j, (int)n_panels, (int)n, 1, false, f_out);
 }
 
 }  // extern "C"

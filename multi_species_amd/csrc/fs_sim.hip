@@ -125,17 +125,54 @@ __global__ __launch_bounds__(kBlock) void reset_points(int n, const double* __re
     qw[i] = bw[i] * f;
 }
 
+// The same for every species of the deck in one launch.
+struct ResetJob {
+    int n;
+    const double* gx;
+    const double* gp;
+    const double* fnew;
+    const double* bw;
+    double* xs;
+    double* ps;
+    double* fs;
+    double* qw;
+};
+
+__global__ __launch_bounds__(kBlock) void reset_points_jobs(const ResetJob* __restrict__ jobs) {
+    const ResetJob& J = jobs[blockIdx.y];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= J.n) return;
+    const double f = J.fnew[i];
+    J.xs[i] = J.gx[i];
+    J.ps[i] = J.gp[i];
+    J.fs[i] = f;
+    J.qw[i] = J.bw[i] * f;
+}
+
 // reduced_ws[u] += q_ws[i] over the points of each unique x, in point order
 // (AMRStructure.cpp:177-180).
-__global__ __launch_bounds__(kBlock) void reduced_sum(int U, const int32_t* __restrict__ offsets,
-                                                      const int32_t* __restrict__ members,
-                                                      const double* __restrict__ qw,
-                                                      double* __restrict__ rw) {
-    const int u = blockIdx.x * kBlock + threadIdx.x;
+// One wavefront per unique x: the lanes fetch 64 members at a time, lane 0
+// adds them in point order.
+__global__ __launch_bounds__(64) void reduced_sum(int U, const int32_t* __restrict__ offsets,
+                                                  const int32_t* __restrict__ members,
+                                                  const double* __restrict__ qw,
+                                                  double* __restrict__ rw) {
+    __shared__ double vals[64];
+    const int u = blockIdx.x;
     if (u >= U) return;
+    const int k0 = offsets[u], k1 = offsets[u + 1];
     double s = 0.0;
-    for (int k = offsets[u]; k < offsets[u + 1]; ++k) s += qw[members[k]];
-    rw[u] = s;
+    for (int c = k0; c < k1; c += 64) {
+        const int k = c + (int)threadIdx.x;
+        vals[threadIdx.x] = (k < k1) ? qw[members[k]] : 0.0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int m = min(64, k1 - c);
+            for (int t = 0; t < m; ++t) s += vals[t];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rw[u] = s;
 }
 
 // es[i] = reduced_es[unique(i)] (AMRStructure.cpp:191-195).
@@ -197,8 +234,7 @@ struct Species {
     DevBuf d_rec, d_grid, d_bw, d_fnew, d_old, d_work;
     double *gx = nullptr, *gp = nullptr;
     int32_t* rows_d = nullptr;
-    remesh::OldMesh old;
-    remesh::GridWork work;
+    remesh::Job job{};  // this species' grid interpolation (device pointers)
 
     int n() const { return (int)tree.xs.size(); }
     int npanels() const { return tree.size(); }
@@ -222,7 +258,10 @@ struct fs_sim {
     double t = 0;
     bool any_adaptive = false;
     // concatenated state: xs, ps, qw, es, fs; RK4 work: xk, p2, p3, p4, e2, e3, e4
-    DevBuf state, rk, red, fscratch, status;
+    DevBuf state, rk, red, fscratch, status, jobs;
+    fs::remesh::Job* d_jobs = nullptr;    // grid jobs, one per species
+    fs::sim::ResetJob* d_reset = nullptr; // reset descriptors, one per species
+    fs::remesh::Launch launch;
     double *xs = nullptr, *ps = nullptr, *qw = nullptr, *es = nullptr, *fs_ = nullptr;
     double *xk = nullptr, *p2 = nullptr, *p3 = nullptr, *p4 = nullptr, *e2 = nullptr,
            *e3 = nullptr, *e4 = nullptr;
@@ -241,8 +280,15 @@ namespace {
 using Sim = ::fs_sim;
 
 int field_eval(Sim& S, double* out, const double* x, const double* w, int n) {
-    if (S.prm.use_treecode > 0)
-        return fail(FS_E_INVALID, "use_treecode: the device treecode is not part of this build");
+    if (S.prm.use_treecode > 0) {
+        // E_MQ_Treecode evaluates at the sources; the tree is built on the host
+        std::vector<double> hx(n);
+        FS_HIP_CHECK(hipMemcpyAsync(hx.data(), x, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                    S.stream()));
+        FS_HIP_CHECK(hipStreamSynchronize(S.stream()));
+        return treecode_f64(hx.data(), x, w, n, S.Lx(), S.prm.greens_epsilon, S.prm.mac,
+                            S.prm.degree, S.prm.max_source, out, S.fscratch, S.stream(), nullptr);
+    }
     const size_t need = field_scratch_bytes(n, n, S.ctx->num_cus, S.ctx->tuning);
     FS_TRY(S.fscratch.ensure(need, S.stream()));
     return field_direct_f64(x, n, x, w, n, S.Lx(), S.prm.greens_epsilon, out, S.fscratch.ptr,
@@ -274,10 +320,23 @@ int upload_species(Sim& S, Species& s, bool grid_too) {
                                     hipMemcpyHostToDevice, st));
     }
     FS_TRY(s.d_fnew.ensure(sizeof(double) * std::max(n, nb), st));
-    FS_TRY(s.d_old.ensure(rm::OldMesh::bytes(P), st));
-    s.old.carve(s.d_old.ptr, P);
-    FS_TRY(s.d_work.ensure(rm::GridWork::bytes(nb, s.np), st));
-    s.work.carve(s.d_work.ptr, nb, s.np);
+    FS_TRY(s.d_old.ensure(rm::mesh_bytes(P), st));
+    FS_TRY(s.d_work.ensure(rm::grid_bytes(nb, s.np), st));
+    rm::Job& j = s.job;
+    j = rm::Job{};
+    rm::carve_mesh(j, s.d_old.ptr, P);
+    rm::carve_grid(j, s.d_work.ptr, nb, s.np);
+    j.rec = s.d_rec.as<int32_t>();
+    j.npanels = P;
+    j.x = s.gx;
+    j.p = s.gp;
+    j.rows = s.rows_d;
+    j.nx = s.nx;
+    j.np = s.np;
+    j.Lx = S.Lx();
+    j.periodic = S.prm.bcs == 0;
+    j.f_beyond = s.f_beyond;
+    j.f = s.d_fnew.as<double>();
     return FS_OK;
 }
 
@@ -366,6 +425,36 @@ int layout(Sim& S, const std::vector<std::vector<double>>* es_host) {
     S.d_status = S.status.as<int>();
     zero_ints<<<1, 64, 0, st>>>(S.d_status, kMaxSpecies);
     FS_HIP_CHECK(hipGetLastError());
+    // job descriptors: the old mesh of species k is its own slice of the state
+    const int nsp = (int)S.sp.size();
+    std::vector<rm::Job> jobs(nsp);
+    std::vector<ResetJob> resets(nsp);
+    S.launch = rm::Launch{};
+    S.launch.njobs = nsp;
+    for (int k = 0; k < nsp; ++k) {
+        Species& s = *S.sp[k];
+        const int o = S.T.off[k];
+        rm::Job& j = s.job;
+        j.ox = S.xs + o;
+        j.op = S.ps + o;
+        j.of = S.fs_ + o;
+        j.status = S.d_status + k;
+        jobs[k] = j;
+        resets[k] = ResetJob{s.nx * s.np, s.gx, s.gp, j.f, s.d_bw.as<double>(), S.xs + o,
+                             S.ps + o, S.fs_ + o, S.qw + o};
+        S.launch.max_panels = std::max(S.launch.max_panels, s.npanels());
+        S.launch.max_points = std::max(S.launch.max_points, s.nx * s.np);
+        S.launch.max_rows = std::max(S.launch.max_rows, s.np);
+        S.launch.max_nx = std::max(S.launch.max_nx, s.nx);
+    }
+    const size_t bj = align_up(sizeof(rm::Job) * nsp, 256);
+    FS_TRY(S.jobs.ensure(bj + sizeof(ResetJob) * nsp, st));
+    S.d_jobs = S.jobs.as<rm::Job>();
+    S.d_reset = S.jobs.as<ResetJob>(bj);
+    FS_HIP_CHECK(hipMemcpyAsync(S.d_jobs, jobs.data(), sizeof(rm::Job) * nsp,
+                                hipMemcpyHostToDevice, st));
+    FS_HIP_CHECK(hipMemcpyAsync(S.d_reset, resets.data(), sizeof(ResetJob) * nsp,
+                                hipMemcpyHostToDevice, st));
     // the host copies above must outlive the async copies
     FS_HIP_CHECK(hipStreamSynchronize(st));
     return FS_OK;
@@ -374,7 +463,7 @@ int layout(Sim& S, const std::vector<std::vector<double>>* es_host) {
 // evaluate_field_uniform_grid (AMRSimulation_field.cpp:5-162)
 int field_reduced(Sim& S) {
     hipStream_t st = S.stream();
-    reduced_sum<<<blocks(S.U), kBlock, 0, st>>>(S.U, S.r_off, S.r_mem, S.qw, S.rw);
+    reduced_sum<<<(unsigned)S.U, 64, 0, st>>>(S.U, S.r_off, S.r_mem, S.qw, S.rw);
     FS_HIP_CHECK(hipGetLastError());
     FS_TRY(field_eval(S, S.re, S.rx, S.rw, S.U));
     scatter_reduced<<<blocks(S.N), kBlock, 0, st>>>(S.N, S.r_uof, S.re, S.es);
@@ -410,88 +499,83 @@ int read_status(Sim& S) {
     return rm::status_to_error(all);
 }
 
-__global__ void copy_status(const int* src, int* dst) {
-    if (threadIdx.x == 0) *dst = *src;
-}
-
-// Grid interpolation of one species from its current (advected) mesh; f
-// onto s.d_fnew. Status into S.d_status[k].
-int interp_species_grid(Sim& S, int k) {
-    Species& s = *S.sp[k];
-    hipStream_t st = S.stream();
-    const int o = S.T.off[k];
-    FS_TRY(rm::prepare_old_mesh(s.d_rec.as<int32_t>(), s.npanels(), S.xs + o, S.ps + o,
-                                S.fs_ + o, s.old, st));
-    FS_TRY(rm::interp_to_grid(s.old, s.gx, s.gp, s.rows_d, s.nx, s.np, S.Lx(),
-                              S.prm.bcs == 0, s.f_beyond, s.d_fnew.as<double>(), s.work, st));
-    copy_status<<<1, 64, 0, st>>>(s.work.status, S.d_status + k);
-    FS_HIP_CHECK(hipGetLastError());
-    return FS_OK;
-}
-
-// AMRSimulation::remesh (AMRSimulation_remesh.cpp:3-48), device-only path.
+// AMRSimulation::remesh (AMRSimulation_remesh.cpp:3-48), device-only path:
+// every species' grid interpolation in one launch per kernel, then the
+// points reset to the grid and the reduced-x field.
 int remesh_static(Sim& S) {
     hipStream_t st = S.stream();
-    for (size_t k = 0; k < S.sp.size(); ++k) {
-        Species& s = *S.sp[k];
-        const int o = S.T.off[k];
-        FS_TRY(interp_species_grid(S, (int)k));
-        reset_points<<<blocks(s.n()), kBlock, 0, st>>>(s.n(), s.gx, s.gp, s.d_fnew.as<double>(),
-                                                       s.d_bw.as<double>(), S.xs + o, S.ps + o,
-                                                       S.fs_ + o, S.qw + o);
-        FS_HIP_CHECK(hipGetLastError());
-    }
+    FS_TRY(rm::clear_status(S.d_jobs, S.launch, st));
+    FS_TRY(rm::prepare_old_meshes(S.d_jobs, S.launch, st));
+    FS_TRY(rm::interp_grids(S.d_jobs, S.launch, st));
+    reset_points_jobs<<<dim3(blocks(S.launch.max_points), (unsigned)S.sp.size()), kBlock, 0, st>>>(
+        S.d_reset);
+    FS_HIP_CHECK(hipGetLastError());
     return field_reduced(S);
 }
 
 // Remesh with adaptive refinement: base-grid f on the device, refinement on
-// the host with batched device interpolation of new points.
+// the host with batched device interpolation of each pass's new points.
 int remesh_adaptive(Sim& S) {
     hipStream_t st = S.stream();
-    std::vector<std::vector<double>> es_unused;
+    FS_TRY(rm::clear_status(S.d_jobs, S.launch, st));
+    FS_TRY(rm::prepare_old_meshes(S.d_jobs, S.launch, st));
+    FS_TRY(rm::interp_grids(S.d_jobs, S.launch, st));
+    FS_TRY(read_status(S));
+    DevBuf io;
     for (size_t k = 0; k < S.sp.size(); ++k) {
         Species& s = *S.sp[k];
-        const int o = S.T.off[k];
-        FS_TRY(interp_species_grid(S, (int)k));
         const int nb = s.nx * s.np;
         mesh::PanelTree t = s.base;
         FS_HIP_CHECK(hipMemcpyAsync(t.fs.data(), s.d_fnew.ptr, sizeof(double) * nb,
                                     hipMemcpyDeviceToHost, st));
-        FS_TRY(read_status(S));
+        FS_HIP_CHECK(hipStreamSynchronize(st));
         if (s.adaptive) {
             int rc = FS_OK;
-            DevBuf io;
+            // a point job on this species' old mesh (its records are already built)
             auto eval = [&](const double* x, const double* p, size_t n, double* f) {
                 if (rc != FS_OK || n == 0) return;
                 const size_t b = align_up(sizeof(double) * n, 256);
-                if ((rc = io.ensure(3 * b, st)) != FS_OK) return;
-                double* dx = io.as<double>(0);
-                double* dp = io.as<double>(b);
-                double* df = io.as<double>(2 * b);
-                if (hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
-                    hipMemcpyAsync(dp, p, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess) {
+                const size_t bj = align_up(sizeof(rm::Job), 256);
+                if ((rc = io.ensure(3 * b + bj, st)) != FS_OK) return;
+                rm::Job j = s.job;
+                j.x = io.as<double>(0);
+                j.p = io.as<double>(b);
+                j.f = io.as<double>(2 * b);
+                j.nx = (int)n;
+                j.np = 1;
+                rm::Job* dj = io.as<rm::Job>(3 * b);
+                rm::Launch l;
+                l.njobs = 1;
+                l.max_points = (int)n;
+                if (hipMemcpyAsync((void*)j.x, x, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+                    hipMemcpyAsync((void*)j.p, p, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+                    hipMemcpyAsync(dj, &j, sizeof(rm::Job), hipMemcpyHostToDevice, st) != hipSuccess) {
                     rc = fail(FS_E_HIP, "adaptive remesh: upload failed");
                     return;
                 }
-                if ((rc = rm::interp_points(s.old, dx, dp, (int)n, S.Lx(), S.prm.bcs == 0,
-                                            s.f_beyond, df, S.d_status + k, st)) != FS_OK)
-                    return;
-                if (hipMemcpyAsync(f, df, sizeof(double) * n, hipMemcpyDeviceToHost, st) != hipSuccess) {
+                if ((rc = rm::clear_status(dj, l, st)) != FS_OK) return;
+                if ((rc = rm::interp_points(dj, l, st)) != FS_OK) return;
+                if (hipMemcpyAsync(f, j.f, sizeof(double) * n, hipMemcpyDeviceToHost, st) != hipSuccess) {
                     rc = fail(FS_E_HIP, "adaptive remesh: download failed");
                     return;
                 }
-                rc = read_status(S);
+                int status = 0;
+                if (hipMemcpyAsync(&status, j.status, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                    hipStreamSynchronize(st) != hipSuccess) {
+                    rc = fail(FS_E_HIP, "adaptive remesh: sync failed");
+                    return;
+                }
+                rc = rm::status_to_error(status);
             };
             t.refine_adaptively(s.spec, eval);
-            io.release();
             FS_TRY(rc);
         }
         s.tree = std::move(t);
         std::vector<int32_t> leaves;
         s.tree.leaf_weights(s.q, S.prm.quadrature == 1, s.base_w, leaves);
         s.red = mesh::reduced_map(s.tree.xs);
-        (void)o;
     }
+    io.release();
     FS_TRY(layout(S, nullptr));
     return field_reduced(S);
 }

@@ -97,3 +97,16 @@ def test_adaptive_two_stream_matches_oracle(gpu_ctx):
         gsim.step()
         compare(gsim, osim, 2, step)
     gsim.close()
+
+
+def test_treecode_deck_matches_oracle(gpu_ctx):
+    deck = oracle.landau_deck(initial_height=4, p_height=1)
+    deck.use_treecode, deck.mac, deck.degree, deck.max_source = 1, 0.7, 8, 200
+    osim = oracle.OracleSim(deck)
+    gsim = ms.Simulation(gpu_ctx, *to_ms(deck))
+    compare(gsim, osim, 2, 0)
+    for step in range(1, 4):
+        osim.step()
+        gsim.step()
+        compare(gsim, osim, 2, step)
+    gsim.close()
