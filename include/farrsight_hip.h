@@ -68,9 +68,28 @@ int fs_emq_treecode_f64(fs_ctx* ctx, double* es, const double* sources, const do
                         int64_t ns, double L, double epsilon, double mac, int degree,
                         int max_source, int64_t* evals);
 
-/* Tuning knobs of the direct-sum kernel (0 = automatic). targets_per_thread
+/* Tuning knobs of the direct-sum kernels (0 = automatic). targets_per_thread
  * in {1,2,4}; source_splits >= 1. Applies to later calls on this context. */
 int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits);
+
+/* Mixed-precision form of fs_emq_periodic_f64 (no counterpart in the
+ * reference; the FP64 entry points stay the drop-in). Positions are wrapped
+ * in FP64 and differenced exactly in 64-bit fixed point; the step and
+ * neutralizing part of the kernel is summed in FP64 from sorted prefix sums,
+ * the regularization remainder in FP32 away from z = 0 and in FP64 within
+ * B = max(2^-6, 16 d, 4000 d^2) of it (d = eps/L; every pair in FP64 when
+ * B >= 1/4), with compensated FP64 accumulation per target.
+ * Relative error max_i|E_i - E_i^FP64| / max_i|E_i^FP64| below 1e-8 on
+ * the tested decks. Requires epsilon > 0. When d_targets == d_sources and
+ * nt == ns the sort is shared. Host pointers, synchronous. */
+int fs_emq_periodic_mixed(fs_ctx* ctx, double* es, const double* targets, int64_t nt,
+                          const double* sources, const double* q_ws, int64_t ns,
+                          double L, double epsilon);
+
+/* Same on device-resident arrays, enqueued on `stream`. */
+int fs_emq_periodic_mixed_dev(fs_ctx* ctx, double* d_es, const double* d_targets,
+                              int64_t nt, const double* d_sources, const double* d_q_ws,
+                              int64_t ns, double L, double epsilon, void* stream);
 
 /* ---- Part B: remesh interpolation -------------------------------------- */
 
@@ -169,6 +188,41 @@ int fs_sim_get_panels(fs_sim* sim, int species, int32_t* panels);
 /* out[0] charge, [1] momentum, [2] kinetic energy, [3] electric energy
  * (standard_interface/diagnostic_plot.py:116-131). */
 int fs_sim_diagnostics(fs_sim* sim, int species, double out[4]);
+
+/* ---- Batches of independent decks (no counterpart in the reference) -----
+ * One fs_sim may hold many decks; each deck's field sees only its own
+ * charges. All decks step together on one stream: every field evaluation of
+ * a step is one batched launch over all decks, every remesh kernel one
+ * launch over all species, and the whole static step is captured once in a
+ * HIP graph and replayed. Species are numbered over the batch, deck by deck
+ * (fs_sim_deck_info gives each deck's first species). A deck's results do
+ * not depend on which other decks share the batch. */
+enum { FS_PRECISION_F64 = 0, FS_PRECISION_MIXED = 1 };
+
+typedef struct fs_deck {
+    const fs_sim_params* sim;
+    const fs_species_params* species;
+    int n_species;
+} fs_deck;
+
+/* Creates a batch. Decks share remesh_period; precision FS_PRECISION_F64
+ * (the reference's method) or FS_PRECISION_MIXED (fs_emq_periodic_mixed's
+ * method for every field evaluation). fs_sim_create is a batch of one deck
+ * in FP64. */
+int fs_batch_create(fs_ctx* ctx, const fs_deck* decks, int n_decks, int precision,
+                    fs_sim** out);
+/* Continuous batching: a deck joins (built on the host, uploaded; the others
+ * keep their device state) or leaves between steps. Joining needs
+ * remesh_period 1 for the batch and the deck. */
+int fs_sim_add_deck(fs_sim* sim, const fs_deck* deck, int* index);
+int fs_sim_remove_deck(fs_sim* sim, int deck);
+int fs_sim_num_decks(fs_sim* sim, int* n_decks);
+/* Any output may be NULL. iter counts the steps the deck has taken. */
+int fs_sim_deck_info(fs_sim* sim, int deck, int* first_species, int* n_species, int* iter,
+                     int64_t* n_points);
+/* precision as above; use_graph 0 runs every step eagerly (adaptive and
+ * treecode decks always do). */
+int fs_sim_set_options(fs_sim* sim, int precision, int use_graph);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 from typing import Optional
 
 import numpy as np
@@ -46,6 +47,9 @@ def lib():
         L.fs_emq_periodic_f64.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
         L.fs_emq_periodic_f64_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d, _vp]
         L.fs_emq_set_tuning.argtypes = [_vp, _int, _int]
+        L.fs_emq_periodic_mixed.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
+        L.fs_emq_periodic_mixed_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d,
+                                                _vp]
         L.fs_emq_treecode_f64.argtypes = [_vp, _dp, _dp, _dp, _i64, _d, _d, _d, _int, _int,
                                           ctypes.POINTER(_i64)]
         _ip = ctypes.POINTER(ctypes.c_int32)
@@ -65,6 +69,13 @@ def lib():
         L.fs_sim_get_points.argtypes = [_vp, _int, _dp, _dp, _dp, _dp, _dp]
         L.fs_sim_get_panels.argtypes = [_vp, _int, _ip]
         L.fs_sim_diagnostics.argtypes = [_vp, _int, _dp]
+        L.fs_batch_create.argtypes = [_vp, ctypes.POINTER(Deck), _int, _int, ctypes.POINTER(_vp)]
+        L.fs_sim_add_deck.argtypes = [_vp, ctypes.POINTER(Deck), ctypes.POINTER(_int)]
+        L.fs_sim_remove_deck.argtypes = [_vp, _int]
+        L.fs_sim_num_decks.argtypes = [_vp, ctypes.POINTER(_int)]
+        L.fs_sim_deck_info.argtypes = [_vp, _int, ctypes.POINTER(_int), ctypes.POINTER(_int),
+                                       ctypes.POINTER(_int), ctypes.POINTER(_i64)]
+        L.fs_sim_set_options.argtypes = [_vp, _int, _int]
         _lib = L
     return _lib
 
@@ -98,6 +109,9 @@ class Context:
             self.h = None
 
     def __del__(self):
+        # at interpreter exit the HIP runtime may already be gone
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -124,6 +138,26 @@ class Context:
         _check(lib().fs_emq_periodic_f64(self.h, _P(es), _P(t), t.size, _P(s), _P(w),
                                          s.size, L, eps), "fs_emq_periodic_f64")
         return es
+
+    def emq_periodic_mixed(self, targets, sources, q_ws, L: float, eps: float,
+                           same: bool = False) -> np.ndarray:
+        """Mixed-precision direct sum on host arrays (synchronous). same=True
+        evaluates at the sources (targets ignored) with one shared sort."""
+        s, w = _f64(sources), _f64(q_ws)
+        t = s if same else _f64(targets)
+        if s.size != w.size:
+            raise ValueError("sources and q_ws differ in length")
+        es = np.empty(t.size)
+        _check(lib().fs_emq_periodic_mixed(self.h, _P(es), _P(t), t.size, _P(s), _P(w),
+                                           s.size, L, eps), "fs_emq_periodic_mixed")
+        return es
+
+    def emq_periodic_mixed_dev(self, es_ptr: int, t_ptr: int, nt: int, s_ptr: int, w_ptr: int,
+                               ns: int, L: float, eps: float, stream: Optional[int] = None):
+        """Mixed-precision direct sum on device pointers, enqueued on `stream`."""
+        _check(lib().fs_emq_periodic_mixed_dev(self.h, es_ptr, t_ptr, nt, s_ptr, w_ptr, ns, L,
+                                               eps, stream or None),
+               "fs_emq_periodic_mixed_dev")
 
     def emq_treecode_f64(self, sources, q_ws, L: float, eps: float, mac: float, degree: int,
                          max_source: int, return_evals: bool = False):
@@ -166,6 +200,22 @@ class SimParams(ctypes.Structure):
         ("greens_epsilon", _d), ("use_treecode", _int), ("mac", _d), ("degree", _int),
         ("max_source", _int), ("dt", _d), ("remesh_period", _int),
     ]
+
+
+class Deck(ctypes.Structure):
+    """fs_deck: one deck of a batch."""
+    _fields_ = [("sim", ctypes.POINTER(SimParams)), ("species", ctypes.POINTER(SpeciesParams)),
+                ("n_species", _int)]
+
+
+PRECISION_F64 = 0
+PRECISION_MIXED = 1
+
+
+def _deck(sim: SimParams, species):
+    arr = (SpeciesParams * len(species))(*species)
+    d = Deck(ctypes.pointer(sim), ctypes.cast(arr, ctypes.POINTER(SpeciesParams)), len(species))
+    return d, arr
 
 
 def species_params(q=-1.0, m=1.0, p_min=-1.0, p_max=1.0, initial_height=6, p_height=0,
@@ -212,18 +262,69 @@ def mesh_build(sim: SimParams, sp: SpeciesParams):
 
 
 class Simulation:
-    """AMRSimulation on the device (fs_sim)."""
+    """AMRSimulation on the device (fs_sim): one deck, or a batch of decks.
 
-    def __init__(self, ctx: "Context", sim: SimParams, species):
+    Simulation(ctx, sim, species) is one deck (fs_sim_create);
+    Simulation.batch(ctx, [(sim, species), ...], precision) a batch."""
+
+    def __init__(self, ctx: "Context", sim: SimParams = None, species=None, *, _handle=None,
+                 _keep=None):
         self.ctx = ctx
+        if _handle is not None:
+            self.h = _handle
+            self._keep = _keep
+            return
         self._sim = sim
         arr = (SpeciesParams * len(species))(*species)
-        self._sp = arr
+        self._keep = [sim, arr]
         h = _vp()
         _check(lib().fs_sim_create(ctx.h, ctypes.byref(sim), arr, len(species), ctypes.byref(h)),
                "fs_sim_create")
         self.h = h
-        self.n_species = len(species)
+
+    @classmethod
+    def batch(cls, ctx: "Context", decks, precision: int = PRECISION_F64) -> "Simulation":
+        keep, ds = [], []
+        for sim, species in decks:
+            d, arr = _deck(sim, species)
+            keep += [sim, arr]
+            ds.append(d)
+        darr = (Deck * len(ds))(*ds)
+        h = _vp()
+        _check(lib().fs_batch_create(ctx.h, darr, len(ds), precision, ctypes.byref(h)),
+               "fs_batch_create")
+        return cls(ctx, _handle=h, _keep=keep)
+
+    def add_deck(self, sim: SimParams, species) -> int:
+        d, arr = _deck(sim, species)
+        idx = _int()
+        _check(lib().fs_sim_add_deck(self.h, ctypes.byref(d), ctypes.byref(idx)),
+               "fs_sim_add_deck")
+        return idx.value
+
+    def remove_deck(self, deck: int):
+        _check(lib().fs_sim_remove_deck(self.h, deck), "fs_sim_remove_deck")
+
+    @property
+    def n_decks(self) -> int:
+        n = _int()
+        _check(lib().fs_sim_num_decks(self.h, ctypes.byref(n)), "fs_sim_num_decks")
+        return n.value
+
+    def deck_info(self, deck: int):
+        """(first_species, n_species, iter, n_points) of a deck."""
+        a, b, c, n = _int(), _int(), _int(), _i64()
+        _check(lib().fs_sim_deck_info(self.h, deck, ctypes.byref(a), ctypes.byref(b),
+                                      ctypes.byref(c), ctypes.byref(n)), "fs_sim_deck_info")
+        return a.value, b.value, c.value, n.value
+
+    def set_options(self, precision: int = PRECISION_F64, use_graph: bool = True):
+        _check(lib().fs_sim_set_options(self.h, precision, int(bool(use_graph))),
+               "fs_sim_set_options")
+
+    @property
+    def n_species(self) -> int:
+        return sum(self.deck_info(d)[1] for d in range(self.n_decks))
 
     def close(self):
         if getattr(self, "h", None):
@@ -231,6 +332,9 @@ class Simulation:
             self.h = None
 
     def __del__(self):
+        # at interpreter exit the HIP runtime may already be gone
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

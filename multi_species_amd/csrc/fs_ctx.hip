@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstring>
 
+#include "fs_field_batch.h"
 #include "fs_internal.h"
 
 namespace fs {
@@ -76,6 +77,12 @@ int fs_ctx_destroy(fs_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
     ctx->io.release();
+    ctx->aux.release();
+    if (ctx->api_batch) {
+        ctx->api_batch->release();
+        delete ctx->api_batch;
+        ctx->api_batch = nullptr;
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return FS_OK;

@@ -1,0 +1,96 @@
+// Batched periodic direct sum over independent point segments (decks).
+// Internal to libfarrsight_hip.
+#pragma once
+
+#include <vector>
+
+#include "fs_internal.h"
+
+namespace fs {
+
+enum { kPrecisionF64 = 0, kPrecisionMixed = 1 };
+
+// One segment of the concatenated point arrays. A deck is one segment that
+// is its own source; the single-call API with distinct targets uses two
+// segments (the sources, not evaluated, and the targets, sourced from the
+// first).
+struct BatchSeg {
+    int off = 0, n = 0;  // points [off, off + n)
+    int src = 0;         // segment whose points are the sources
+    bool eval = true;    // evaluate the field at this segment's points
+    double L = 1.0, eps = 0.0;
+};
+
+// Per-segment launch parameters (device copy).
+struct SegDev {
+    int off, n;
+    int src_off, src_n;
+    int splits, chunk;
+    int eval, all_band;
+    int64_t part_base;
+    double invL, delta2, C;
+    // mixed only
+    uint64_t band;
+    double d2s, wscale;
+    float b1, b2;
+    int pw_base;
+    int src;  // index of the source segment
+};
+
+struct WorkItem {
+    int seg;
+    int tb, te;   // targets [tb, te) (concatenated index; sorted position when mixed)
+    int sb, se;   // sources [sb, se)
+    int split;
+};
+
+struct ScanBlock {
+    int seg;
+    int b, e;         // sorted positions [b, e)
+    int first;        // index of the segment's first scan block
+    int count;        // scan blocks of the segment
+};
+
+// Plan and device tables of one batched field evaluation; the tables live
+// in the object and are rebuilt by plan(). run() only launches kernels on
+// the given stream (no allocation, no memset, no host synchronization), so
+// a caller may capture it in a HIP graph.
+class FieldBatch {
+public:
+    int plan(const std::vector<BatchSeg>& segs, int n_total, int precision, int num_cus,
+             const FieldTuning& tun, hipStream_t stream);
+    // x, w: concatenated positions and weights; es: field at every point of
+    // the evaluated segments (others untouched).
+    int run(const double* x, const double* w, double* es, hipStream_t stream);
+    void release();
+    int precision() const { return precision_; }
+    int tpt() const { return tpt_; }
+    int items() const { return (int)items_.size(); }
+    int64_t pairs() const { return pairs_; }
+
+private:
+    int precision_ = kPrecisionF64;
+    int n_total_ = 0, nseg_ = 0, tpt_ = 2, grid_ = 0;
+    int64_t pairs_ = 0;
+    std::vector<SegDev> segs_;
+    std::vector<WorkItem> items_;
+    std::vector<ScanBlock> scan_;
+    DevBuf tables_, work_;
+    // device views
+    SegDev* d_segs_ = nullptr;
+    WorkItem* d_items_ = nullptr;
+    ScanBlock* d_scan_ = nullptr;
+    int32_t* d_seg_of_ = nullptr;
+    int* d_counter_ = nullptr;
+    // work space offsets (bytes into work_)
+    size_t o_part_ = 0, o_key_ = 0, o_skey_ = 0, o_perm_ = 0, o_sort_ = 0, o_hs_ = 0, o_ws_ = 0,
+           o_wf_ = 0, o_pw_ = 0, o_bsum_ = 0, o_bsa_ = 0, o_tot_ = 0;
+};
+
+// Stable radix sort by (segment of the original index, top 32 key bits)
+// (fs_radix.hip). kin must not alias kout.
+size_t radix_sort_bytes(int n);
+int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, uint64_t* kout,
+                   int32_t* iout, void* tmp, hipStream_t stream);
+
+}  // namespace fs

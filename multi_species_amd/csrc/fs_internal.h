@@ -11,6 +11,8 @@
 
 namespace fs {
 
+class FieldBatch;
+
 // Records a message for fs_last_error() on this thread and returns `code`.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 
@@ -71,5 +73,7 @@ struct fs_ctx {
     int num_cus = 256;
     fs::DevBuf scratch;   // kernel work space
     fs::DevBuf io;        // staging for the host-pointer entry points
+    fs::DevBuf aux;       // concatenated points of the mixed entry points
     fs::FieldTuning tuning;
+    fs::FieldBatch* api_batch = nullptr;  // plan of the last mixed entry-point call
 };

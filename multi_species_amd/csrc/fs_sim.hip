@@ -1,22 +1,30 @@
 // The FARRSIGHT step loop on the device: AMRSimulation (reference
 // src/simulation_files/AMRSimulation*.cpp) with every species' state
-// resident in HBM.
+// resident in HBM, for one deck or a batch of independent decks.
 //
-// Layout: the species are concatenated (AMRSimulation::gather order) in
-// one set of arrays xs, ps, q_ws, es, fs; species s owns [off_s, off_s+n_s).
-// A step enqueues, on the context's stream and without host round trips for
+// Layout: the species of all decks are concatenated (deck by deck, each in
+// AMRSimulation::gather order) in one set of arrays xs, ps, q_ws, es, fs;
+// species s owns [off_s, off_s+n_s), deck d the union of its species. Each
+// point carries its species index, which gives q/m and the deck's time step
+// to the RK4 kernels. The field is a batched direct sum (fs_field_batch.hip)
+// with one segment per deck, so every deck sees only its own charges. A step
+// enqueues, on the context's stream and without host round trips for
 // non-adaptive decks:
 //   rk4 stage 1 | field | stage 2 | field | stage 3 | field | final update
 //   (AMRSimulation_step.cpp:122-263; contraction off, so every update rounds
 //   as the reference's expression), then either
-//   remesh per species (old-mesh coefficients and search records, grid
-//   interpolation, reset of x/p to the base grid, q_ws = base weight * f) and
-//   the reduced-x field (segment sums in point order, direct sum over the
-//   unique x of all species, scatter back; AMRSimulation_field.cpp:5-162,
-//   AMRStructure.cpp:147-196), or a full field evaluation.
+//   remesh of every species of every deck (old-mesh coefficients and search
+//   records, grid interpolation, reset of x/p to the base grid, q_ws = base
+//   weight * f) and the reduced-x field (segment sums in point order, direct
+//   sum over each deck's unique x, scatter back; AMRSimulation_field.cpp:
+//   5-162, AMRStructure.cpp:147-196), or a full field evaluation.
+// That sequence is captured once in a HIP graph and replayed every step
+// (the plans it uses are built at layout time; nothing in it allocates,
+// clears with memset or synchronizes).
 // Adaptive decks rebuild the refined tree on the host after the grid
 // interpolation, with every refinement pass' new-point f evaluated by one
-// device launch (interpolate_from_mesh), then re-upload.
+// device launch (interpolate_from_mesh), then re-upload; treecode decks build
+// their tree on the host. Both run eagerly, without the graph.
 #pragma clang fp contract(off)
 
 #include <algorithm>
@@ -25,81 +33,82 @@
 #include <memory>
 #include <vector>
 
+#include "fs_field_batch.h"
 #include "fs_mesh.h"
 #include "fs_remesh.h"
 
 namespace fs {
 namespace sim {
 
-constexpr int kMaxSpecies = 8;
 constexpr int kBlock = 256;
 
-struct SpeciesTable {
-    int n;
-    int off[kMaxSpecies + 1];
-    double qm[kMaxSpecies];
+// Per species: q/m and the deck's time step (h = dt/2, dt, dt/6 rounded as the
+// reference's host expressions).
+struct SpeciesInfo {
+    double qm, h, dt, dt6;
 };
-
-__device__ __forceinline__ double qm_of(const SpeciesTable& T, int i) {
-    int s = 0;
-    while (s + 1 < T.n && i >= T.off[s + 1]) ++s;
-    return T.qm[s];
-}
 
 // k1 -> k2 (AMRSimulation_step.cpp:148-164): ef1 = es q/m, p2 = p + dt/2 ef1,
 // x_k = x + dt/2 p1 (p is the velocity).
-__global__ __launch_bounds__(kBlock) void rk4_stage1(SpeciesTable T, int n,
+__global__ __launch_bounds__(kBlock) void rk4_stage1(int n, const int32_t* __restrict__ sid,
+                                                     const SpeciesInfo* __restrict__ si,
                                                      const double* __restrict__ xs,
                                                      const double* __restrict__ ps,
-                                                     const double* __restrict__ es, double h,
+                                                     const double* __restrict__ es,
                                                      double* __restrict__ p2,
                                                      double* __restrict__ xk) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const double ef1 = es[i] * qm_of(T, i);
-    p2[i] = ps[i] + h * ef1;
-    xk[i] = xs[i] + h * ps[i];
+    const SpeciesInfo I = si[sid[i]];
+    const double ef1 = es[i] * I.qm;
+    p2[i] = ps[i] + I.h * ef1;
+    xk[i] = xs[i] + I.h * ps[i];
 }
 
 // k2 -> k3 (:179-189)
-__global__ __launch_bounds__(kBlock) void rk4_stage2(SpeciesTable T, int n,
+__global__ __launch_bounds__(kBlock) void rk4_stage2(int n, const int32_t* __restrict__ sid,
+                                                     const SpeciesInfo* __restrict__ si,
                                                      const double* __restrict__ xs,
                                                      const double* __restrict__ ps,
-                                                     const double* __restrict__ e2, double h,
+                                                     const double* __restrict__ e2,
                                                      const double* __restrict__ p2,
                                                      double* __restrict__ p3,
                                                      double* __restrict__ xk) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const double ef2 = e2[i] * qm_of(T, i);
-    p3[i] = ps[i] + h * ef2;
-    xk[i] = xs[i] + h * p2[i];
+    const SpeciesInfo I = si[sid[i]];
+    const double ef2 = e2[i] * I.qm;
+    p3[i] = ps[i] + I.h * ef2;
+    xk[i] = xs[i] + I.h * p2[i];
 }
 
 // k3 -> k4 (:212-220)
-__global__ __launch_bounds__(kBlock) void rk4_stage3(SpeciesTable T, int n,
+__global__ __launch_bounds__(kBlock) void rk4_stage3(int n, const int32_t* __restrict__ sid,
+                                                     const SpeciesInfo* __restrict__ si,
                                                      const double* __restrict__ xs,
                                                      const double* __restrict__ ps,
-                                                     const double* __restrict__ e3, double dt,
+                                                     const double* __restrict__ e3,
                                                      const double* __restrict__ p3,
                                                      double* __restrict__ p4,
                                                      double* __restrict__ xk) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const double ef3 = e3[i] * qm_of(T, i);
-    p4[i] = ps[i] + dt * ef3;
-    xk[i] = xs[i] + dt * p3[i];
+    const SpeciesInfo I = si[sid[i]];
+    const double ef3 = e3[i] * I.qm;
+    p4[i] = ps[i] + I.dt * ef3;
+    xk[i] = xs[i] + I.dt * p3[i];
 }
 
 // u_{n+1} = u_n + dt/6 (k1 + 2 k2 + 2 k3 + k4) (:236-248)
 __global__ __launch_bounds__(kBlock) void rk4_final(
-    SpeciesTable T, int n, double* __restrict__ xs, double* __restrict__ ps,
-    const double* __restrict__ es, const double* __restrict__ e2, const double* __restrict__ e3,
-    const double* __restrict__ e4, const double* __restrict__ p2, const double* __restrict__ p3,
-    const double* __restrict__ p4, double dt6) {
+    int n, const int32_t* __restrict__ sid, const SpeciesInfo* __restrict__ si,
+    double* __restrict__ xs, double* __restrict__ ps, const double* __restrict__ es,
+    const double* __restrict__ e2, const double* __restrict__ e3, const double* __restrict__ e4,
+    const double* __restrict__ p2, const double* __restrict__ p3, const double* __restrict__ p4) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const double qm = qm_of(T, i);
+    const SpeciesInfo I = si[sid[i]];
+    const double qm = I.qm, dt6 = I.dt6;
     const double p1 = ps[i];
     const double ef1 = es[i] * qm, ef2 = e2[i] * qm, ef3 = e3[i] * qm, ef4 = e4[i] * qm;
     xs[i] = xs[i] + dt6 * (p1 + 2 * p2[i] + 2 * p3[i] + p4[i]);
@@ -107,25 +116,7 @@ __global__ __launch_bounds__(kBlock) void rk4_final(
 }
 
 // After a remesh: points back on the base grid, f interpolated, q_ws = base
-// weight * f (AMRStructure_amr.cpp:1096-1099).
-__global__ __launch_bounds__(kBlock) void reset_points(int n, const double* __restrict__ gx,
-                                                       const double* __restrict__ gp,
-                                                       const double* __restrict__ fnew,
-                                                       const double* __restrict__ bw,
-                                                       double* __restrict__ xs,
-                                                       double* __restrict__ ps,
-                                                       double* __restrict__ fs,
-                                                       double* __restrict__ qw) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const double f = fnew[i];
-    xs[i] = gx[i];
-    ps[i] = gp[i];
-    fs[i] = f;
-    qw[i] = bw[i] * f;
-}
-
-// The same for every species of the deck in one launch.
+// weight * f (AMRStructure_amr.cpp:1096-1099), every species in one launch.
 struct ResetJob {
     int n;
     const double* gx;
@@ -185,11 +176,12 @@ __global__ __launch_bounds__(kBlock) void scatter_reduced(int n,
     es[i] = re[unique_of[i]];
 }
 
-inline unsigned blocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
-
-__global__ void zero_ints(int* a, int n) {
-    if ((int)threadIdx.x < n) a[threadIdx.x] = 0;
+__global__ __launch_bounds__(kBlock) void zero_ints(int* a, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = 0;
 }
+
+inline unsigned blocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // f0 of the deck (src/structure_files/initial_distributions.cpp:12-152 via
 // AMRSimulation_load.cpp:149-221), same expression order.
@@ -220,6 +212,7 @@ double initial_f(const fs_species_params& sp, double kx, double x, double p) {
 
 struct Species {
     fs_species_params prm{};
+    int deck = 0;
     double q = -1, qm = -1, m = 1, kx = 0;
     mesh::Spec spec;
     bool adaptive = false;
@@ -230,6 +223,7 @@ struct Species {
     std::vector<double> base_w;
     mesh::ReducedMap red;
     double f_beyond = 0;
+    int off = -1;          // offset in the concatenated arrays (-1: not resident)
 
     DevBuf d_rec, d_grid, d_bw, d_fnew, d_old, d_work;
     double *gx = nullptr, *gp = nullptr;
@@ -238,6 +232,25 @@ struct Species {
 
     int n() const { return (int)tree.xs.size(); }
     int npanels() const { return tree.size(); }
+    void release() {
+        d_rec.release();
+        d_grid.release();
+        d_bw.release();
+        d_fnew.release();
+        d_old.release();
+        d_work.release();
+    }
+};
+
+struct Deck {
+    fs_sim_params prm{};
+    std::vector<std::unique_ptr<Species>> sp;
+    int off = 0, n = 0;    // points
+    int uoff = 0, u = 0;   // reduced points
+    int iter = 0;
+    double t = 0;
+    bool adaptive = false;
+    double Lx() const { return prm.x_max - prm.x_min; }
 };
 
 }  // namespace sim
@@ -249,27 +262,31 @@ namespace rm = fs::remesh;
 
 struct fs_sim {
     fs_ctx* ctx = nullptr;
-    fs_sim_params prm{};
-    std::vector<std::unique_ptr<fs::sim::Species>> sp;
-    fs::sim::SpeciesTable T{};
+    std::vector<std::unique_ptr<fs::sim::Deck>> decks;
+    std::vector<fs::sim::Species*> sp;  // all species, deck by deck
     int N = 0;
     int U = 0;
     int iter = 0;
-    double t = 0;
-    bool any_adaptive = false;
+    int remesh_period = 1;
+    int precision = FS_PRECISION_F64;
+    bool use_graph = true;
+    bool any_adaptive = false, any_treecode = false;
     // concatenated state: xs, ps, qw, es, fs; RK4 work: xk, p2, p3, p4, e2, e3, e4
-    DevBuf state, rk, red, fscratch, status, jobs;
-    fs::remesh::Job* d_jobs = nullptr;    // grid jobs, one per species
+    DevBuf state, rk, red, status, jobs, pinfo;
+    fs::FieldBatch ffull, fred;
+    rm::Job* d_jobs = nullptr;            // grid jobs, one per species
     fs::sim::ResetJob* d_reset = nullptr; // reset descriptors, one per species
-    fs::remesh::Launch launch;
+    rm::Launch launch;
     double *xs = nullptr, *ps = nullptr, *qw = nullptr, *es = nullptr, *fs_ = nullptr;
     double *xk = nullptr, *p2 = nullptr, *p3 = nullptr, *p4 = nullptr, *e2 = nullptr,
            *e3 = nullptr, *e4 = nullptr;
     double *rx = nullptr, *rw = nullptr, *re = nullptr;
     int32_t *r_off = nullptr, *r_mem = nullptr, *r_uof = nullptr;
+    int32_t* sid = nullptr;
+    fs::sim::SpeciesInfo* sinfo = nullptr;
     int* d_status = nullptr;
+    hipGraphExec_t g_remesh = nullptr, g_plain = nullptr;
 
-    double Lx() const { return prm.x_max - prm.x_min; }
     hipStream_t stream() const { return ctx->stream; }
 };
 
@@ -279,25 +296,103 @@ namespace {
 
 using Sim = ::fs_sim;
 
-int field_eval(Sim& S, double* out, const double* x, const double* w, int n) {
-    if (S.prm.use_treecode > 0) {
-        // E_MQ_Treecode evaluates at the sources; the tree is built on the host
-        std::vector<double> hx(n);
-        FS_HIP_CHECK(hipMemcpyAsync(hx.data(), x, sizeof(double) * n, hipMemcpyDeviceToHost,
-                                    S.stream()));
-        FS_HIP_CHECK(hipStreamSynchronize(S.stream()));
-        return treecode_f64(hx.data(), x, w, n, S.Lx(), S.prm.greens_epsilon, S.prm.mac,
-                            S.prm.degree, S.prm.max_source, out, S.fscratch, S.stream(), nullptr);
+void drop_graphs(Sim& S) {
+    if (S.g_remesh) (void)hipGraphExecDestroy(S.g_remesh);
+    if (S.g_plain) (void)hipGraphExecDestroy(S.g_plain);
+    S.g_remesh = S.g_plain = nullptr;
+}
+
+// E_MQ_Treecode on one deck's slice: the tree is built on the host.
+int treecode_deck(Sim& S, const Deck& D, double* out, const double* x, const double* w, int n) {
+    std::vector<double> hx(n);
+    FS_HIP_CHECK(hipMemcpyAsync(hx.data(), x, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                S.stream()));
+    FS_HIP_CHECK(hipStreamSynchronize(S.stream()));
+    DevBuf work;
+    const int rc = treecode_f64(hx.data(), x, w, n, D.Lx(), D.prm.greens_epsilon, D.prm.mac,
+                                D.prm.degree, D.prm.max_source, out, work, S.stream(), nullptr);
+    (void)hipStreamSynchronize(S.stream());
+    work.release();
+    return rc;
+}
+
+// E at every point: the batched direct sum over all direct-sum decks, then
+// the treecode decks one by one.
+int field_full(Sim& S, double* out, const double* x) {
+    FS_TRY(S.ffull.run(x, S.qw, out, S.stream()));
+    if (S.any_treecode)
+        for (auto& D : S.decks)
+            if (D->prm.use_treecode > 0)
+                FS_TRY(treecode_deck(S, *D, out + D->off, x + D->off, S.qw + D->off, D->n));
+    return FS_OK;
+}
+
+// evaluate_field_uniform_grid (AMRSimulation_field.cpp:5-162)
+int field_reduced(Sim& S) {
+    hipStream_t st = S.stream();
+    if (S.U > 0) {
+        reduced_sum<<<(unsigned)S.U, 64, 0, st>>>(S.U, S.r_off, S.r_mem, S.qw, S.rw);
+        FS_HIP_CHECK(hipGetLastError());
     }
-    const size_t need = field_scratch_bytes(n, n, S.ctx->num_cus, S.ctx->tuning);
-    FS_TRY(S.fscratch.ensure(need, S.stream()));
-    return field_direct_f64(x, n, x, w, n, S.Lx(), S.prm.greens_epsilon, out, S.fscratch.ptr,
-                            S.ctx->num_cus, S.ctx->tuning, S.stream());
+    FS_TRY(S.fred.run(S.rx, S.rw, S.re, st));
+    if (S.any_treecode)
+        for (auto& D : S.decks)
+            if (D->prm.use_treecode > 0)
+                FS_TRY(treecode_deck(S, *D, S.re + D->uoff, S.rx + D->uoff, S.rw + D->uoff, D->u));
+    scatter_reduced<<<blocks(S.N), kBlock, 0, st>>>(S.N, S.r_uof, S.re, S.es);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+// rk4_step (AMRSimulation_step.cpp:122-263)
+int rk4(Sim& S) {
+    hipStream_t st = S.stream();
+    const int N = S.N;
+    if (N == 0) return FS_OK;
+    rk4_stage1<<<blocks(N), kBlock, 0, st>>>(N, S.sid, S.sinfo, S.xs, S.ps, S.es, S.p2, S.xk);
+    FS_TRY(field_full(S, S.e2, S.xk));
+    rk4_stage2<<<blocks(N), kBlock, 0, st>>>(N, S.sid, S.sinfo, S.xs, S.ps, S.e2, S.p2, S.p3,
+                                             S.xk);
+    FS_TRY(field_full(S, S.e3, S.xk));
+    rk4_stage3<<<blocks(N), kBlock, 0, st>>>(N, S.sid, S.sinfo, S.xs, S.ps, S.e3, S.p3, S.p4,
+                                             S.xk);
+    FS_TRY(field_full(S, S.e4, S.xk));
+    rk4_final<<<blocks(N), kBlock, 0, st>>>(N, S.sid, S.sinfo, S.xs, S.ps, S.es, S.e2, S.e3,
+                                            S.e4, S.p2, S.p3, S.p4);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int read_status(Sim& S) {
+    const int n = (int)S.sp.size();
+    if (n == 0) return FS_OK;
+    std::vector<int> h(n, 0);
+    FS_HIP_CHECK(hipMemcpyAsync(h.data(), S.d_status, sizeof(int) * n, hipMemcpyDeviceToHost,
+                                S.stream()));
+    FS_HIP_CHECK(hipStreamSynchronize(S.stream()));
+    int all = 0;
+    for (int k = 0; k < n; ++k) all |= h[k];
+    return rm::status_to_error(all);
+}
+
+// AMRSimulation::remesh (AMRSimulation_remesh.cpp:3-48), device-only path:
+// every species' grid interpolation in one launch per kernel, then the
+// points reset to the grid and the reduced-x field.
+int remesh_static(Sim& S) {
+    hipStream_t st = S.stream();
+    if (S.sp.empty()) return FS_OK;
+    FS_TRY(rm::clear_status(S.d_jobs, S.launch, st));
+    FS_TRY(rm::prepare_old_meshes(S.d_jobs, S.launch, st));
+    FS_TRY(rm::interp_grids(S.d_jobs, S.launch, st));
+    reset_points_jobs<<<dim3(blocks(S.launch.max_points), (unsigned)S.sp.size()), kBlock, 0, st>>>(
+        S.d_reset);
+    FS_HIP_CHECK(hipGetLastError());
+    return field_reduced(S);
 }
 
 // Per-species device copies of what a remesh reads (grid, rows, base weights,
 // panel records) and its work space.
-int upload_species(Sim& S, Species& s, bool grid_too) {
+int upload_species(Sim& S, const Deck& D, Species& s, bool grid_too) {
     hipStream_t st = S.stream();
     const int n = s.n(), nb = s.nx * s.np, P = s.npanels();
     FS_TRY(s.d_rec.ensure(sizeof(int32_t) * s.tree.rec.size(), st));
@@ -333,64 +428,133 @@ int upload_species(Sim& S, Species& s, bool grid_too) {
     j.rows = s.rows_d;
     j.nx = s.nx;
     j.np = s.np;
-    j.Lx = S.Lx();
-    j.periodic = S.prm.bcs == 0;
+    j.Lx = D.Lx();
+    j.periodic = D.prm.bcs == 0;
     j.f_beyond = s.f_beyond;
     j.f = s.d_fnew.as<double>();
     return FS_OK;
 }
 
-// (Re)allocates the concatenated arrays for the current species sizes and
-// uploads every species' host state (x, p, f, q_ws; es of each species when
-// given) plus the global reduced-x map.
-int layout(Sim& S, const std::vector<std::vector<double>>* es_host) {
+// (Re)builds the concatenated arrays for the current decks. A species whose
+// state is resident (off >= 0 and keep_resident) is copied device to device
+// from the previous arrays; every other species is uploaded from its host
+// mesh (es included when es_host is given). Plans the batched fields and
+// drops the captured graphs.
+int layout(Sim& S, bool keep_resident) {
     hipStream_t st = S.stream();
+    drop_graphs(S);
+    S.sp.clear();
+    S.any_adaptive = S.any_treecode = false;
     int N = 0;
-    S.T.n = (int)S.sp.size();
-    for (size_t k = 0; k < S.sp.size(); ++k) {
-        S.T.off[k] = N;
-        S.T.qm[k] = S.sp[k]->qm;
-        N += S.sp[k]->n();
+    for (auto& D : S.decks) {
+        D->off = N;
+        for (auto& s : D->sp) {
+            S.sp.push_back(s.get());
+            N += s->n();
+        }
+        D->n = N - D->off;
+        S.any_adaptive = S.any_adaptive || D->adaptive;
+        S.any_treecode = S.any_treecode || D->prm.use_treecode > 0;
     }
-    S.T.off[S.sp.size()] = N;
     S.N = N;
-    const size_t bN = align_up(sizeof(double) * N, 256);
-    FS_TRY(S.state.ensure(5 * bN, st));
-    S.xs = S.state.as<double>(0);
-    S.ps = S.state.as<double>(bN);
-    S.qw = S.state.as<double>(2 * bN);
-    S.es = S.state.as<double>(3 * bN);
-    S.fs_ = S.state.as<double>(4 * bN);
+    const int nsp = (int)S.sp.size();
+    const size_t bN = align_up(sizeof(double) * std::max(N, 1), 256);
+    DevBuf state;
+    FS_TRY(state.ensure(5 * bN, st));
+    double* xs = state.as<double>(0);
+    double* ps = state.as<double>(bN);
+    double* qw = state.as<double>(2 * bN);
+    double* es = state.as<double>(3 * bN);
+    double* fsv = state.as<double>(4 * bN);
+    int o = 0;
+    std::vector<std::vector<double>> host_qw(nsp);
+    for (int k = 0; k < nsp; ++k) {
+        Species& s = *S.sp[k];
+        const int n = s.n();
+        if (keep_resident && s.off >= 0) {
+            const double* src[5] = {S.xs, S.ps, S.qw, S.es, S.fs_};
+            double* dst[5] = {xs, ps, qw, es, fsv};
+            for (int a = 0; a < 5; ++a)
+                FS_HIP_CHECK(hipMemcpyAsync(dst[a] + o, src[a] + s.off, sizeof(double) * n,
+                                            hipMemcpyDeviceToDevice, st));
+        } else {
+            std::vector<double>& q = host_qw[k];
+            q.resize(n);
+            for (int i = 0; i < n; ++i) q[i] = s.base_w[i] * s.tree.fs[i];
+            FS_HIP_CHECK(hipMemcpyAsync(xs + o, s.tree.xs.data(), sizeof(double) * n,
+                                        hipMemcpyHostToDevice, st));
+            FS_HIP_CHECK(hipMemcpyAsync(ps + o, s.tree.ps.data(), sizeof(double) * n,
+                                        hipMemcpyHostToDevice, st));
+            FS_HIP_CHECK(hipMemcpyAsync(fsv + o, s.tree.fs.data(), sizeof(double) * n,
+                                        hipMemcpyHostToDevice, st));
+            FS_HIP_CHECK(hipMemcpyAsync(qw + o, q.data(), sizeof(double) * n,
+                                        hipMemcpyHostToDevice, st));
+        }
+        o += n;
+    }
+    // the previous arrays must outlive the copies from them
+    FS_HIP_CHECK(hipStreamSynchronize(st));
+    std::swap(S.state, state);
+    state.release();
+    S.xs = xs;
+    S.ps = ps;
+    S.qw = qw;
+    S.es = es;
+    S.fs_ = fsv;
+    o = 0;
+    for (int k = 0; k < nsp; ++k) {
+        S.sp[k]->off = o;
+        o += S.sp[k]->n();
+    }
     FS_TRY(S.rk.ensure(7 * bN, st));
-    double* r = S.rk.as<double>(0);
-    S.xk = r;
+    S.xk = S.rk.as<double>(0);
     S.p2 = S.rk.as<double>(bN);
     S.p3 = S.rk.as<double>(2 * bN);
     S.p4 = S.rk.as<double>(3 * bN);
     S.e2 = S.rk.as<double>(4 * bN);
     S.e3 = S.rk.as<double>(5 * bN);
     S.e4 = S.rk.as<double>(6 * bN);
-    // global reduced map: species' unique x concatenated (duplicates across
-    // species kept, AMRSimulation_field.cpp:9-13)
-    std::vector<double> rxh;
-    std::vector<int32_t> off(1, 0), mem, uof(N);
-    int ubase = 0;
-    for (size_t k = 0; k < S.sp.size(); ++k) {
+    // per-point species index and per-species constants
+    std::vector<int32_t> sidh(std::max(N, 1), 0);
+    std::vector<SpeciesInfo> info(std::max(nsp, 1));
+    for (int k = 0; k < nsp; ++k) {
         const Species& s = *S.sp[k];
-        const int o = S.T.off[k];
-        rxh.insert(rxh.end(), s.red.ux.begin(), s.red.ux.end());
-        for (size_t u = 0; u < s.red.ux.size(); ++u) {
-            for (int m = s.red.offsets[u]; m < s.red.offsets[u + 1]; ++m)
-                mem.push_back(o + s.red.members[m]);
-            off.push_back((int32_t)mem.size());
+        for (int i = 0; i < s.n(); ++i) sidh[s.off + i] = k;
+        const double dt = S.decks[s.deck]->prm.dt;
+        info[k] = SpeciesInfo{s.qm, 0.5 * dt, dt, dt / 6.0};
+    }
+    const size_t bsid = align_up(sizeof(int32_t) * sidh.size(), 256);
+    FS_TRY(S.pinfo.ensure(bsid + sizeof(SpeciesInfo) * info.size(), st));
+    S.sid = S.pinfo.as<int32_t>(0);
+    S.sinfo = S.pinfo.as<SpeciesInfo>(bsid);
+    FS_HIP_CHECK(hipMemcpyAsync(S.sid, sidh.data(), sizeof(int32_t) * sidh.size(),
+                                hipMemcpyHostToDevice, st));
+    FS_HIP_CHECK(hipMemcpyAsync(S.sinfo, info.data(), sizeof(SpeciesInfo) * info.size(),
+                                hipMemcpyHostToDevice, st));
+    // reduced map: each deck's species' unique x concatenated (duplicates
+    // across species kept, AMRSimulation_field.cpp:9-13)
+    std::vector<double> rxh;
+    std::vector<int32_t> off(1, 0), mem, uof(std::max(N, 1));
+    int ubase = 0;
+    for (auto& D : S.decks) {
+        D->uoff = ubase;
+        for (auto& sp : D->sp) {
+            const Species& s = *sp;
+            rxh.insert(rxh.end(), s.red.ux.begin(), s.red.ux.end());
+            for (size_t u = 0; u < s.red.ux.size(); ++u) {
+                for (int m = s.red.offsets[u]; m < s.red.offsets[u + 1]; ++m)
+                    mem.push_back(s.off + s.red.members[m]);
+                off.push_back((int32_t)mem.size());
+            }
+            for (int i = 0; i < s.n(); ++i) uof[s.off + i] = ubase + s.red.unique_of[i];
+            ubase += (int)s.red.ux.size();
         }
-        for (int i = 0; i < s.n(); ++i) uof[o + i] = ubase + s.red.unique_of[i];
-        ubase += (int)s.red.ux.size();
+        D->u = ubase - D->uoff;
     }
     S.U = ubase;
-    const size_t bU = align_up(sizeof(double) * S.U, 256);
+    const size_t bU = align_up(sizeof(double) * std::max(S.U, 1), 256);
     const size_t bo = align_up(sizeof(int32_t) * (S.U + 1), 256);
-    const size_t bm = align_up(sizeof(int32_t) * N, 256);
+    const size_t bm = align_up(sizeof(int32_t) * std::max(N, 1), 256);
     FS_TRY(S.red.ensure(3 * bU + bo + 2 * bm, st));
     S.rx = S.red.as<double>(0);
     S.rw = S.red.as<double>(bU);
@@ -398,119 +562,76 @@ int layout(Sim& S, const std::vector<std::vector<double>>* es_host) {
     S.r_off = S.red.as<int32_t>(3 * bU);
     S.r_mem = S.red.as<int32_t>(3 * bU + bo);
     S.r_uof = S.red.as<int32_t>(3 * bU + bo + bm);
-    FS_HIP_CHECK(hipMemcpyAsync(S.rx, rxh.data(), sizeof(double) * S.U, hipMemcpyHostToDevice, st));
+    if (S.U > 0)
+        FS_HIP_CHECK(hipMemcpyAsync(S.rx, rxh.data(), sizeof(double) * S.U, hipMemcpyHostToDevice, st));
     FS_HIP_CHECK(hipMemcpyAsync(S.r_off, off.data(), sizeof(int32_t) * (S.U + 1),
                                 hipMemcpyHostToDevice, st));
-    FS_HIP_CHECK(hipMemcpyAsync(S.r_mem, mem.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
-    FS_HIP_CHECK(hipMemcpyAsync(S.r_uof, uof.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
-    for (size_t k = 0; k < S.sp.size(); ++k) {
-        Species& s = *S.sp[k];
-        const int o = S.T.off[k], n = s.n();
-        std::vector<double> qw(n);
-        for (int i = 0; i < n; ++i) qw[i] = s.base_w[i] * s.tree.fs[i];
-        FS_HIP_CHECK(hipMemcpyAsync(S.xs + o, s.tree.xs.data(), sizeof(double) * n,
-                                    hipMemcpyHostToDevice, st));
-        FS_HIP_CHECK(hipMemcpyAsync(S.ps + o, s.tree.ps.data(), sizeof(double) * n,
-                                    hipMemcpyHostToDevice, st));
-        FS_HIP_CHECK(hipMemcpyAsync(S.fs_ + o, s.tree.fs.data(), sizeof(double) * n,
-                                    hipMemcpyHostToDevice, st));
-        FS_HIP_CHECK(hipMemcpyAsync(S.qw + o, qw.data(), sizeof(double) * n,
-                                    hipMemcpyHostToDevice, st));
-        if (es_host)
-            FS_HIP_CHECK(hipMemcpyAsync(S.es + o, (*es_host)[k].data(), sizeof(double) * n,
-                                        hipMemcpyHostToDevice, st));
-        FS_TRY(upload_species(S, s, true));
+    if (N > 0) {
+        FS_HIP_CHECK(hipMemcpyAsync(S.r_mem, mem.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
+        FS_HIP_CHECK(hipMemcpyAsync(S.r_uof, uof.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
     }
-    FS_TRY(S.status.ensure(sizeof(int) * kMaxSpecies, st));
+    for (auto& D : S.decks)
+        for (auto& sp : D->sp) FS_TRY(upload_species(S, *D, *sp, sp->d_grid.ptr == nullptr || !keep_resident));
+    FS_TRY(S.status.ensure(sizeof(int) * std::max(nsp, 1), st));
     S.d_status = S.status.as<int>();
-    zero_ints<<<1, 64, 0, st>>>(S.d_status, kMaxSpecies);
+    zero_ints<<<blocks(std::max(nsp, 1)), kBlock, 0, st>>>(S.d_status, std::max(nsp, 1));
     FS_HIP_CHECK(hipGetLastError());
     // job descriptors: the old mesh of species k is its own slice of the state
-    const int nsp = (int)S.sp.size();
-    std::vector<rm::Job> jobs(nsp);
-    std::vector<ResetJob> resets(nsp);
+    std::vector<rm::Job> jobs(std::max(nsp, 1));
+    std::vector<ResetJob> resets(std::max(nsp, 1));
     S.launch = rm::Launch{};
     S.launch.njobs = nsp;
     for (int k = 0; k < nsp; ++k) {
         Species& s = *S.sp[k];
-        const int o = S.T.off[k];
+        const int o2 = s.off;
         rm::Job& j = s.job;
-        j.ox = S.xs + o;
-        j.op = S.ps + o;
-        j.of = S.fs_ + o;
+        j.ox = S.xs + o2;
+        j.op = S.ps + o2;
+        j.of = S.fs_ + o2;
         j.status = S.d_status + k;
         jobs[k] = j;
-        resets[k] = ResetJob{s.nx * s.np, s.gx, s.gp, j.f, s.d_bw.as<double>(), S.xs + o,
-                             S.ps + o, S.fs_ + o, S.qw + o};
+        resets[k] = ResetJob{s.nx * s.np, s.gx, s.gp, j.f, s.d_bw.as<double>(), S.xs + o2,
+                             S.ps + o2, S.fs_ + o2, S.qw + o2};
         S.launch.max_panels = std::max(S.launch.max_panels, s.npanels());
         S.launch.max_points = std::max(S.launch.max_points, s.nx * s.np);
         S.launch.max_rows = std::max(S.launch.max_rows, s.np);
         S.launch.max_nx = std::max(S.launch.max_nx, s.nx);
     }
-    const size_t bj = align_up(sizeof(rm::Job) * nsp, 256);
-    FS_TRY(S.jobs.ensure(bj + sizeof(ResetJob) * nsp, st));
+    const size_t bj = align_up(sizeof(rm::Job) * jobs.size(), 256);
+    FS_TRY(S.jobs.ensure(bj + sizeof(ResetJob) * resets.size(), st));
     S.d_jobs = S.jobs.as<rm::Job>();
     S.d_reset = S.jobs.as<ResetJob>(bj);
-    FS_HIP_CHECK(hipMemcpyAsync(S.d_jobs, jobs.data(), sizeof(rm::Job) * nsp,
+    FS_HIP_CHECK(hipMemcpyAsync(S.d_jobs, jobs.data(), sizeof(rm::Job) * jobs.size(),
                                 hipMemcpyHostToDevice, st));
-    FS_HIP_CHECK(hipMemcpyAsync(S.d_reset, resets.data(), sizeof(ResetJob) * nsp,
+    FS_HIP_CHECK(hipMemcpyAsync(S.d_reset, resets.data(), sizeof(ResetJob) * resets.size(),
                                 hipMemcpyHostToDevice, st));
+    // batched fields: one segment per deck (treecode decks not evaluated here)
+    std::vector<BatchSeg> full, redsg;
+    for (size_t d = 0; d < S.decks.size(); ++d) {
+        const Deck& D = *S.decks[d];
+        BatchSeg a;
+        a.off = D.off;
+        a.n = D.n;
+        a.src = (int)d;
+        a.eval = D.prm.use_treecode <= 0;
+        a.L = D.Lx();
+        a.eps = D.prm.greens_epsilon;
+        full.push_back(a);
+        a.off = D.uoff;
+        a.n = D.u;
+        redsg.push_back(a);
+    }
+    if (full.empty()) {
+        BatchSeg none;
+        none.eval = false;
+        full.push_back(none);
+        redsg.push_back(none);
+    }
+    FS_TRY(S.ffull.plan(full, N, S.precision, S.ctx->num_cus, S.ctx->tuning, st));
+    FS_TRY(S.fred.plan(redsg, S.U, S.precision, S.ctx->num_cus, S.ctx->tuning, st));
     // the host copies above must outlive the async copies
     FS_HIP_CHECK(hipStreamSynchronize(st));
     return FS_OK;
-}
-
-// evaluate_field_uniform_grid (AMRSimulation_field.cpp:5-162)
-int field_reduced(Sim& S) {
-    hipStream_t st = S.stream();
-    reduced_sum<<<(unsigned)S.U, 64, 0, st>>>(S.U, S.r_off, S.r_mem, S.qw, S.rw);
-    FS_HIP_CHECK(hipGetLastError());
-    FS_TRY(field_eval(S, S.re, S.rx, S.rw, S.U));
-    scatter_reduced<<<blocks(S.N), kBlock, 0, st>>>(S.N, S.r_uof, S.re, S.es);
-    FS_HIP_CHECK(hipGetLastError());
-    return FS_OK;
-}
-
-// rk4_step (AMRSimulation_step.cpp:122-263)
-int rk4(Sim& S) {
-    hipStream_t st = S.stream();
-    const double dt = S.prm.dt, h = 0.5 * dt;
-    const int N = S.N;
-    rk4_stage1<<<blocks(N), kBlock, 0, st>>>(S.T, N, S.xs, S.ps, S.es, h, S.p2, S.xk);
-    FS_TRY(field_eval(S, S.e2, S.xk, S.qw, N));
-    rk4_stage2<<<blocks(N), kBlock, 0, st>>>(S.T, N, S.xs, S.ps, S.e2, h, S.p2, S.p3, S.xk);
-    FS_TRY(field_eval(S, S.e3, S.xk, S.qw, N));
-    rk4_stage3<<<blocks(N), kBlock, 0, st>>>(S.T, N, S.xs, S.ps, S.e3, dt, S.p3, S.p4, S.xk);
-    FS_TRY(field_eval(S, S.e4, S.xk, S.qw, N));
-    rk4_final<<<blocks(N), kBlock, 0, st>>>(S.T, N, S.xs, S.ps, S.es, S.e2, S.e3, S.e4, S.p2,
-                                            S.p3, S.p4, dt / 6.0);
-    FS_HIP_CHECK(hipGetLastError());
-    return FS_OK;
-}
-
-int read_status(Sim& S) {
-    int h[kMaxSpecies] = {0};
-    const int n = (int)S.sp.size();
-    FS_HIP_CHECK(hipMemcpyAsync(h, S.d_status, sizeof(int) * n, hipMemcpyDeviceToHost,
-                                S.stream()));
-    FS_HIP_CHECK(hipStreamSynchronize(S.stream()));
-    int all = 0;
-    for (int k = 0; k < n; ++k) all |= h[k];
-    return rm::status_to_error(all);
-}
-
-// AMRSimulation::remesh (AMRSimulation_remesh.cpp:3-48), device-only path:
-// every species' grid interpolation in one launch per kernel, then the
-// points reset to the grid and the reduced-x field.
-int remesh_static(Sim& S) {
-    hipStream_t st = S.stream();
-    FS_TRY(rm::clear_status(S.d_jobs, S.launch, st));
-    FS_TRY(rm::prepare_old_meshes(S.d_jobs, S.launch, st));
-    FS_TRY(rm::interp_grids(S.d_jobs, S.launch, st));
-    reset_points_jobs<<<dim3(blocks(S.launch.max_points), (unsigned)S.sp.size()), kBlock, 0, st>>>(
-        S.d_reset);
-    FS_HIP_CHECK(hipGetLastError());
-    return field_reduced(S);
 }
 
 // Remesh with adaptive refinement: base-grid f on the device, refinement on
@@ -524,6 +645,7 @@ int remesh_adaptive(Sim& S) {
     DevBuf io;
     for (size_t k = 0; k < S.sp.size(); ++k) {
         Species& s = *S.sp[k];
+        const Deck& D = *S.decks[s.deck];
         const int nb = s.nx * s.np;
         mesh::PanelTree t = s.base;
         FS_HIP_CHECK(hipMemcpyAsync(t.fs.data(), s.d_fnew.ptr, sizeof(double) * nb,
@@ -572,28 +694,29 @@ int remesh_adaptive(Sim& S) {
         }
         s.tree = std::move(t);
         std::vector<int32_t> leaves;
-        s.tree.leaf_weights(s.q, S.prm.quadrature == 1, s.base_w, leaves);
+        s.tree.leaf_weights(s.q, D.prm.quadrature == 1, s.base_w, leaves);
         s.red = mesh::reduced_map(s.tree.xs);
     }
     io.release();
-    FS_TRY(layout(S, nullptr));
+    FS_TRY(layout(S, false));
     return field_reduced(S);
 }
 
-int build_species(Sim& S, Species& s) {
+int build_species(const fs_sim_params& P, Species& s) {
     const fs_species_params& p = s.prm;
+    const double Lx = P.x_max - P.x_min;
     s.q = p.q;
     s.qm = p.q / p.m;
     s.m = p.m;
-    s.kx = 2.0 * M_PI / S.Lx() * p.normalized_wavenumber;
-    s.spec.x_min = S.prm.x_min;
-    s.spec.x_max = S.prm.x_max;
+    s.kx = 2.0 * M_PI / Lx * p.normalized_wavenumber;
+    s.spec.x_min = P.x_min;
+    s.spec.x_max = P.x_max;
     s.spec.p_min = p.p_min;
     s.spec.p_max = p.p_max;
     s.spec.initial_height = p.initial_height;
     s.spec.p_height = p.p_height;
     s.spec.max_height = p.max_height;
-    s.spec.periodic = S.prm.bcs == 0;
+    s.spec.periodic = P.bcs == 0;
     s.spec.amr_eps.assign(p.amr_epsilons, p.amr_epsilons + std::max(0, std::min(5, p.n_amr_epsilons)));
     s.adaptive = p.do_adaptively_refine != 0 && !s.spec.amr_eps.empty();
     s.nx = (1 << (p.initial_height + 1)) + 1;
@@ -618,15 +741,52 @@ int build_species(Sim& S, Species& s) {
         s.tree.refine_adaptively(s.spec, f0);
     }
     std::vector<int32_t> leaves;
-    s.tree.leaf_weights(s.q, S.prm.quadrature == 1, s.base_w, leaves);
+    s.tree.leaf_weights(s.q, P.quadrature == 1, s.base_w, leaves);
     s.f_beyond = *std::min_element(s.tree.fs.begin(), s.tree.fs.end());  // AMRStructure.cpp:124
     s.red = mesh::reduced_map(s.tree.xs);
     return FS_OK;
 }
 
+int check_deck(const fs_sim_params* prm, const fs_species_params* species, int n_species) {
+    if (!prm || !species) return fail(FS_E_INVALID, "deck: null argument");
+    if (n_species < 1 || n_species > 64) return fail(FS_E_INVALID, "deck: 1..64 species");
+    if (!(prm->x_max > prm->x_min) || !(prm->dt > 0) || prm->remesh_period < 1)
+        return fail(FS_E_INVALID, "deck: bad domain, dt or remesh_period");
+    if (prm->bcs != 0) return fail(FS_E_INVALID, "deck: only periodic decks (bcs 0)");
+    for (int k = 0; k < n_species; ++k) {
+        const fs_species_params& p = species[k];
+        if (p.initial_height < 1 || p.initial_height > 12 || p.p_height < 0 ||
+            p.initial_height + p.p_height > 13 || !(p.p_max > p.p_min) || p.m == 0.0)
+            return fail(FS_E_INVALID, "deck: species %d has bad mesh parameters", k);
+    }
+    return FS_OK;
+}
+
+// Builds a deck's species meshes on the host.
+int build_deck(const fs_sim_params* prm, const fs_species_params* species, int n_species,
+               std::unique_ptr<Deck>& out) {
+    FS_TRY(check_deck(prm, species, n_species));
+    auto D = std::make_unique<Deck>();
+    D->prm = *prm;
+    for (int k = 0; k < n_species; ++k) {
+        auto s = std::make_unique<Species>();
+        s->prm = species[k];
+        FS_TRY(build_species(D->prm, *s));
+        D->adaptive = D->adaptive || s->adaptive;
+        D->sp.push_back(std::move(s));
+    }
+    out = std::move(D);
+    return FS_OK;
+}
+
+void renumber(Sim& S) {
+    for (size_t d = 0; d < S.decks.size(); ++d)
+        for (auto& s : S.decks[d]->sp) s->deck = (int)d;
+}
+
 int download_species(Sim& S, int k, double* xs, double* ps, double* fs, double* qw, double* es) {
     hipStream_t st = S.stream();
-    const int o = S.T.off[k], n = S.sp[k]->n();
+    const int o = S.sp[k]->off, n = S.sp[k]->n();
     double* dst[5] = {xs, ps, fs, qw, es};
     const double* src[5] = {S.xs, S.ps, S.fs_, S.qw, S.es};
     for (int a = 0; a < 5; ++a)
@@ -634,6 +794,34 @@ int download_species(Sim& S, int k, double* xs, double* ps, double* fs, double* 
             FS_HIP_CHECK(hipMemcpyAsync(dst[a], src[a] + o, sizeof(double) * n,
                                         hipMemcpyDeviceToHost, st));
     FS_HIP_CHECK(hipStreamSynchronize(st));
+    return FS_OK;
+}
+
+// The device part of one step (everything but the adaptive host work).
+int step_body(Sim& S, bool remesh_now) {
+    FS_TRY(rk4(S));
+    if (remesh_now) return remesh_static(S);
+    return field_full(S, S.es, S.xs);
+}
+
+int capture(Sim& S, bool remesh_now, hipGraphExec_t& exec) {
+    hipStream_t st = S.stream();
+    FS_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = step_body(S, remesh_now);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc != FS_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess)
+        return fail(FS_E_HIP, "step capture failed: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        exec = nullptr;
+        return fail(FS_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+    }
     return FS_OK;
 }
 
@@ -645,36 +833,153 @@ using namespace fs;
 
 extern "C" {
 
-int fs_sim_create(fs_ctx* ctx, const fs_sim_params* prm, const fs_species_params* species,
-                  int n_species, fs_sim** out) {
-    if (!ctx || !prm || !species || !out) return fail(FS_E_INVALID, "fs_sim_create: null argument");
+int fs_batch_create(fs_ctx* ctx, const fs_deck* decks, int n_decks, int precision,
+                    fs_sim** out) {
+    if (!ctx || !decks || !out) return fail(FS_E_INVALID, "fs_batch_create: null argument");
     *out = nullptr;
-    if (n_species < 1 || n_species > sim::kMaxSpecies)
-        return fail(FS_E_INVALID, "fs_sim_create: 1..%d species", sim::kMaxSpecies);
-    if (!(prm->x_max > prm->x_min) || !(prm->dt > 0) || prm->remesh_period < 1)
-        return fail(FS_E_INVALID, "fs_sim_create: bad domain, dt or remesh_period");
-    if (prm->bcs != 0) return fail(FS_E_INVALID, "fs_sim_create: only periodic decks (bcs 0)");
-    for (int k = 0; k < n_species; ++k) {
-        const fs_species_params& p = species[k];
-        if (p.initial_height < 1 || p.initial_height > 12 || p.p_height < 0 ||
-            p.initial_height + p.p_height > 13 || !(p.p_max > p.p_min) || p.m == 0.0)
-            return fail(FS_E_INVALID, "fs_sim_create: species %d has bad mesh parameters", k);
-    }
+    if (n_decks < 1) return fail(FS_E_INVALID, "fs_batch_create: no decks");
+    if (precision != FS_PRECISION_F64 && precision != FS_PRECISION_MIXED)
+        return fail(FS_E_INVALID, "fs_batch_create: unknown precision %d", precision);
+    for (int d = 1; d < n_decks; ++d)
+        if (!decks[d].sim || !decks[0].sim ||
+            decks[d].sim->remesh_period != decks[0].sim->remesh_period)
+            return fail(FS_E_INVALID, "fs_batch_create: decks of a batch share remesh_period");
     FS_HIP_CHECK(hipSetDevice(ctx->device));
     auto S = std::make_unique<fs_sim>();
     S->ctx = ctx;
-    S->prm = *prm;
-    for (int k = 0; k < n_species; ++k) {
-        auto s = std::make_unique<sim::Species>();
-        s->prm = species[k];
-        FS_TRY(sim::build_species(*S, *s));
-        S->any_adaptive = S->any_adaptive || s->adaptive;
-        S->sp.push_back(std::move(s));
+    S->precision = precision;
+    for (int d = 0; d < n_decks; ++d) {
+        std::unique_ptr<sim::Deck> D;
+        FS_TRY(sim::build_deck(decks[d].sim, decks[d].species, decks[d].n_species, D));
+        S->decks.push_back(std::move(D));
     }
-    FS_TRY(sim::layout(*S, nullptr));
+    S->remesh_period = S->decks[0]->prm.remesh_period;
+    sim::renumber(*S);
+    FS_TRY(sim::layout(*S, false));
     FS_TRY(sim::field_reduced(*S));  // AMRSimulation.cpp:84-89
     FS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     *out = S.release();
+    return FS_OK;
+}
+
+int fs_sim_create(fs_ctx* ctx, const fs_sim_params* prm, const fs_species_params* species,
+                  int n_species, fs_sim** out) {
+    if (!ctx || !prm || !species || !out) return fail(FS_E_INVALID, "fs_sim_create: null argument");
+    fs_deck d{prm, species, n_species};
+    return fs_batch_create(ctx, &d, 1, FS_PRECISION_F64, out);
+}
+
+int fs_sim_add_deck(fs_sim* S, const fs_deck* deck, int* index) {
+    if (!S || !deck) return fail(FS_E_INVALID, "fs_sim_add_deck: null argument");
+    if (!deck->sim) return fail(FS_E_INVALID, "fs_sim_add_deck: null deck");
+    if (S->remesh_period != 1 || deck->sim->remesh_period != 1)
+        return fail(FS_E_INVALID, "fs_sim_add_deck: decks joining a running batch remesh every step");
+    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
+    std::unique_ptr<sim::Deck> D;
+    FS_TRY(sim::build_deck(deck->sim, deck->species, deck->n_species, D));
+    S->decks.push_back(std::move(D));
+    sim::renumber(*S);
+    // resident decks keep their state; the field of the reduced points is
+    // recomputed for all (bitwise the same for the resident ones)
+    FS_TRY(sim::layout(*S, true));
+    FS_TRY(sim::field_reduced(*S));
+    FS_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+    if (index) *index = (int)S->decks.size() - 1;
+    return FS_OK;
+}
+
+int fs_sim_remove_deck(fs_sim* S, int deck) {
+    if (!S || deck < 0 || deck >= (int)S->decks.size())
+        return fail(FS_E_INVALID, "fs_sim_remove_deck: bad deck index");
+    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
+    FS_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+    for (auto& s : S->decks[deck]->sp) s->release();
+    S->decks.erase(S->decks.begin() + deck);
+    sim::renumber(*S);
+    return sim::layout(*S, true);
+}
+
+int fs_sim_num_decks(fs_sim* S, int* n) {
+    if (!S || !n) return fail(FS_E_INVALID, "null argument");
+    *n = (int)S->decks.size();
+    return FS_OK;
+}
+
+int fs_sim_deck_info(fs_sim* S, int deck, int* first_species, int* n_species, int* iter,
+                     int64_t* n_points) {
+    if (!S || deck < 0 || deck >= (int)S->decks.size())
+        return fail(FS_E_INVALID, "bad deck index");
+    const sim::Deck& D = *S->decks[deck];
+    int first = 0;
+    for (int d = 0; d < deck; ++d) first += (int)S->decks[d]->sp.size();
+    if (first_species) *first_species = first;
+    if (n_species) *n_species = (int)D.sp.size();
+    if (iter) *iter = D.iter;
+    if (n_points) *n_points = D.n;
+    return FS_OK;
+}
+
+int fs_sim_set_options(fs_sim* S, int precision, int use_graph) {
+    if (!S) return fail(FS_E_INVALID, "null simulation");
+    if (precision != FS_PRECISION_F64 && precision != FS_PRECISION_MIXED)
+        return fail(FS_E_INVALID, "unknown precision %d", precision);
+    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
+    S->use_graph = use_graph != 0;
+    if (precision != S->precision) {
+        S->precision = precision;
+        FS_TRY(sim::layout(*S, true));
+    } else {
+        sim::drop_graphs(*S);
+    }
+    return FS_OK;
+}
+
+int fs_sim_destroy(fs_sim* S) {
+    if (!S) return FS_OK;
+    (void)hipStreamSynchronize(S->ctx->stream);
+    sim::drop_graphs(*S);
+    for (auto& D : S->decks)
+        for (auto& s : D->sp) s->release();
+    S->state.release();
+    S->rk.release();
+    S->red.release();
+    S->status.release();
+    S->jobs.release();
+    S->pinfo.release();
+    S->ffull.release();
+    S->fred.release();
+    delete S;
+    return FS_OK;
+}
+
+int fs_sim_step(fs_sim* S) {
+    if (!S) return fail(FS_E_INVALID, "null simulation");
+    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
+    const bool remesh_now = ((S->iter + 1) % S->remesh_period) == 0;
+    const bool eager = S->any_adaptive || S->any_treecode || !S->use_graph;
+    if (eager) {
+        FS_TRY(sim::rk4(*S));
+        if (remesh_now) {
+            if (S->any_adaptive) {
+                FS_TRY(sim::remesh_adaptive(*S));
+            } else {
+                FS_TRY(sim::remesh_static(*S));
+            }
+        } else {
+            FS_TRY(sim::field_full(*S, S->es, S->xs));
+        }
+    } else {
+        hipGraphExec_t& g = remesh_now ? S->g_remesh : S->g_plain;
+        if (!g) FS_TRY(sim::capture(*S, remesh_now, g));
+        FS_HIP_CHECK(hipGraphLaunch(g, S->ctx->stream));
+    }
+    S->iter += 1;
+    for (auto& D : S->decks) {
+        D->iter += 1;
+        D->t += D->prm.dt;
+    }
+    if (remesh_now) return sim::read_status(*S);
+    FS_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
     return FS_OK;
 }
 
@@ -687,11 +992,9 @@ int fs_mesh_build(const fs_sim_params* prm, const fs_species_params* species,
     if (p.initial_height < 1 || p.initial_height > 12 || p.p_height < 0 ||
         p.initial_height + p.p_height > 13 || !(p.p_max > p.p_min) || p.m == 0.0)
         return fail(FS_E_INVALID, "fs_mesh_build: bad mesh parameters");
-    fs_sim S;
-    S.prm = *prm;
     sim::Species s;
     s.prm = p;
-    FS_TRY(sim::build_species(S, s));
+    FS_TRY(sim::build_species(*prm, s));
     if (n_points) *n_points = s.n();
     if (n_panels) *n_panels = s.npanels();
     if (panels) std::copy(s.tree.rec.begin(), s.tree.rec.end(), panels);
@@ -700,45 +1003,6 @@ int fs_mesh_build(const fs_sim_params* prm, const fs_species_params* species,
     if (fs) std::copy(s.tree.fs.begin(), s.tree.fs.end(), fs);
     if (q_ws)
         for (int i = 0; i < s.n(); ++i) q_ws[i] = s.base_w[i] * s.tree.fs[i];
-    return FS_OK;
-}
-
-int fs_sim_destroy(fs_sim* S) {
-    if (!S) return FS_OK;
-    (void)hipStreamSynchronize(S->ctx->stream);
-    for (auto& s : S->sp) {
-        s->d_rec.release();
-        s->d_grid.release();
-        s->d_bw.release();
-        s->d_fnew.release();
-        s->d_old.release();
-        s->d_work.release();
-    }
-    S->state.release();
-    S->rk.release();
-    S->red.release();
-    S->fscratch.release();
-    S->status.release();
-    delete S;
-    return FS_OK;
-}
-
-int fs_sim_step(fs_sim* S) {
-    if (!S) return fail(FS_E_INVALID, "null simulation");
-    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
-    FS_TRY(sim::rk4(*S));
-    S->iter += 1;
-    S->t += S->prm.dt;
-    if (S->iter % S->prm.remesh_period == 0) {
-        if (S->any_adaptive) {
-            FS_TRY(sim::remesh_adaptive(*S));
-        } else {
-            FS_TRY(sim::remesh_static(*S));
-        }
-        return sim::read_status(*S);
-    }
-    FS_TRY(sim::field_eval(*S, S->es, S->xs, S->qw, S->N));
-    FS_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
     return FS_OK;
 }
 
@@ -789,7 +1053,7 @@ int fs_sim_diagnostics(fs_sim* S, int k, double out[4]) {
     for (int i = 0; i < n; ++i) idx[i] = i;
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return xs[a] < xs[b]; });
     std::vector<double> ux, ue;
-    const double L = S->Lx();
+    const double L = S->decks[s.deck]->Lx();
     for (int j = 0; j < n; ++j) {
         const double x = xs[idx[j]];
         if (j > 0 && x == xs[idx[j - 1]]) continue;

@@ -1,0 +1,143 @@
+"""Batches of independent decks in one fs_sim (fs_batch_create).
+
+Bars:
+* a deck's state in a batch is bitwise that of the same deck run alone, in
+  FP64 and in mixed precision, for decks of different sizes, domains, time
+  steps and eps;
+* the captured step graph: every replay bitwise equal to the eager step, with
+  host reads and eager work between replays;
+* the mixed batch against the FP64 runs of the same decks: E within 1e-8
+  (relative, normwise) after the first step;
+* continuous batching: a deck that joins a running batch evolves bitwise as
+  it does alone; removing a deck leaves the others' states unchanged.
+"""
+import numpy as np
+import pytest
+
+import multi_species_amd as ms
+
+pytestmark = pytest.mark.gpu
+
+
+def landau(h, ph, L=4 * np.pi, dt=0.1, eps=0.05, amp=0.01, k=1.0, mass=1836.0):
+    sim = ms.sim_params(x_min=0.0, x_max=L, bcs=0, quadrature=0, greens_epsilon=eps,
+                        use_treecode=0, dt=dt, remesh_period=1)
+    e = ms.species_params(q=-1.0, m=1.0, p_min=-6.0, p_max=6.0, initial_height=h, p_height=ph,
+                          ics_type=1, normalized_wavenumber=k, amp=amp, pth=1.0)
+    i = ms.species_params(q=1.0, m=mass, p_min=-6.0, p_max=6.0, initial_height=h, p_height=ph,
+                          ics_type=1, normalized_wavenumber=k, amp=0.0, pth=1.0 / np.sqrt(mass))
+    return sim, [e, i]
+
+
+def two_stream(h, ph):
+    sim = ms.sim_params(x_min=0.0, x_max=10 * np.pi, bcs=0, quadrature=1, greens_epsilon=0.1,
+                        use_treecode=0, dt=0.05, remesh_period=1)
+    e = ms.species_params(q=-1.0, m=1.0, p_min=-6.0, p_max=6.0, initial_height=h, p_height=ph,
+                          ics_type=4, normalized_wavenumber=1.0, amp=1e-3, pth=0.5, pstr=2.4)
+    i = ms.species_params(q=1.0, m=100.0, p_min=-2.0, p_max=2.0, initial_height=h, p_height=ph,
+                          ics_type=1, normalized_wavenumber=1.0, amp=0.0, pth=0.1)
+    return sim, [e, i]
+
+
+DECKS = [landau(3, 1), landau(4, 1, dt=0.05, eps=0.08, amp=0.3, k=1.5, mass=25.0),
+         two_stream(4, 0), landau(5, 1)]
+
+
+def state(sim, first, nsp):
+    return [np.concatenate(sim.points(first + k)) for k in range(nsp)]
+
+
+def run_alone(ctx, deck, steps, precision):
+    s = ms.Simulation.batch(ctx, [deck], precision)
+    s.step(steps)
+    out = state(s, 0, len(deck[1]))
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("precision", [ms.PRECISION_F64, ms.PRECISION_MIXED])
+def test_batch_equals_alone(gpu_ctx, precision):
+    steps = 3
+    b = ms.Simulation.batch(gpu_ctx, DECKS, precision)
+    assert b.n_decks == len(DECKS)
+    b.step(steps)
+    for d, deck in enumerate(DECKS):
+        first, nsp, it, _ = b.deck_info(d)
+        assert it == steps
+        alone = run_alone(gpu_ctx, deck, steps, precision)
+        mixed_in = state(b, first, nsp)
+        for k in range(nsp):
+            assert np.array_equal(mixed_in[k], alone[k]), (precision, d, k)
+    b.close()
+
+
+def test_graph_replays_match_eager(gpu_ctx):
+    g = ms.Simulation.batch(gpu_ctx, DECKS[:3], ms.PRECISION_MIXED)
+    e = ms.Simulation.batch(gpu_ctx, DECKS[:3], ms.PRECISION_MIXED)
+    e.set_options(ms.PRECISION_MIXED, use_graph=False)
+    for step in range(4):
+        g.step()
+        e.step()
+        # eager device work and host reads between replays
+        _ = gpu_ctx.emq_periodic_f64(np.linspace(0, 1, 100), np.linspace(0, 1, 50),
+                                     np.ones(50), 1.0, 0.1)
+        for k in range(6):
+            assert np.array_equal(np.concatenate(g.points(k)), np.concatenate(e.points(k))), \
+                (step, k)
+    g.close()
+    e.close()
+
+
+def test_mixed_field_within_1e8_of_f64(gpu_ctx):
+    f = ms.Simulation.batch(gpu_ctx, DECKS, ms.PRECISION_F64)
+    m = ms.Simulation.batch(gpu_ctx, DECKS, ms.PRECISION_MIXED)
+    # initial reduced-grid fields, then after one step
+    for stage in range(2):
+        for d in range(len(DECKS)):
+            first, nsp, _, _ = f.deck_info(d)
+            ef = np.concatenate([f.points(first + k)[4] for k in range(nsp)])
+            em = np.concatenate([m.points(first + k)[4] for k in range(nsp)])
+            r = np.max(np.abs(em - ef)) / np.max(np.abs(ef))
+            print(f"stage {stage} deck {d}: mixed vs FP64 E rel {r:.2e}")
+            assert r <= 1e-8
+        f.step()
+        m.step()
+    f.close()
+    m.close()
+
+
+def test_continuous_batching(gpu_ctx):
+    b = ms.Simulation.batch(gpu_ctx, DECKS[:2], ms.PRECISION_F64)
+    b.step(2)
+    j = b.add_deck(*DECKS[2])
+    assert j == 2 and b.n_decks == 3
+    b.step(3)
+    first, nsp, it, _ = b.deck_info(2)
+    assert it == 3
+    joined = state(b, first, nsp)
+    alone = run_alone(gpu_ctx, DECKS[2], 3, ms.PRECISION_F64)
+    for k in range(nsp):
+        assert np.array_equal(joined[k], alone[k])
+    # deck 0 has taken 5 steps; remove deck 1 and check deck 0 and 2 continue unchanged
+    b.remove_deck(1)
+    assert b.n_decks == 2
+    b.step(1)
+    d0 = state(b, 0, 2)
+    a0 = run_alone(gpu_ctx, DECKS[0], 6, ms.PRECISION_F64)
+    for k in range(2):
+        assert np.array_equal(d0[k], a0[k])
+    first, nsp, it, _ = b.deck_info(1)
+    assert it == 4
+    d2 = state(b, first, nsp)
+    a2 = run_alone(gpu_ctx, DECKS[2], 4, ms.PRECISION_F64)
+    for k in range(nsp):
+        assert np.array_equal(d2[k], a2[k])
+    b.close()
+
+
+def test_batch_rejects_mixed_periods(gpu_ctx):
+    s1, sp1 = landau(3, 1)
+    s2, sp2 = landau(3, 1)
+    s2.remesh_period = 2
+    with pytest.raises(ms.FarrsightError):
+        ms.Simulation.batch(gpu_ctx, [(s1, sp1), (s2, sp2)])
