@@ -93,10 +93,15 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--eps", type=float, default=0.05)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["field", "c1"], default="field",
+    ap.add_argument("--workload", choices=["field", "c1", "c4"], default="field",
                     help="field: N=2^20 direct-sum evaluation (default); c1: the two-species "
                          "weak-Landau deck, 65x129 points per species, RK4 dt=0.1 with a remesh "
-                         "every step (a step = one RK4 step + remesh)")
+                         "every step (a step = one RK4 step + remesh); c4: 64 independent "
+                         "two-species Landau decks (16x32 to 128x256 panels) stepped as one "
+                         "batch in mixed precision, against the same decks one by one in FP64")
+    ap.add_argument("--precision", choices=["f64", "mixed"], default="f64",
+                    help="c1: field precision of the step loop")
+    ap.add_argument("--decks", type=int, default=64, help="c4: decks in the batch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration "
                          "for the HBM traffic figures) and exit")
@@ -122,6 +127,8 @@ def main():
 
     if args.workload == "c1":
         return bench_c1(args)
+    if args.workload == "c4":
+        return bench_c4(args)
 
     n = args.n
     x, w = c2_inputs(n)
@@ -214,6 +221,7 @@ def main():
         "gpu_ms": mixed_ms,
         "speedup_vs_f64": kernel_ms / mixed_ms,
         "rel_err_vs_f64": err,
+        "tuned_shape": dict(zip(("targets_per_thread", "wgs_per_cu"), ctx.mixed_shape())),
         "precision": "FP64 wrap + 64-bit fixed-point displacement; FP64 step/neutralizing "
                      "part from sorted prefix sums and FP64 band |z| < max(2^-6, 16 d, 4000 d^2), d = eps/L; "
                      "FP32 series elsewhere; compensated FP64 accumulation",
@@ -243,7 +251,8 @@ def bench_c1(args):
 
     ctx = ms.Context(0)
     sim_p, sps = c1_deck()
-    sim = ms.Simulation(ctx, sim_p, sps)
+    prec = ms.PRECISION_MIXED if args.precision == "mixed" else ms.PRECISION_F64
+    sim = ms.Simulation.batch(ctx, [(sim_p, sps)], prec)
     npts = sum(sim.counts(k)[0] for k in range(2))
     for _ in range(args.warmup):
         sim.step()
@@ -262,12 +271,12 @@ def bench_c1(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "f64" if prec == ms.PRECISION_F64 else "f64+f32", "data": "synthetic",
         "config": {"workload": "two-species weak Landau damping deck (electrons q=-1 m=1 "
                                "amp 0.01 k=0.5; ions q=1 m=1836 vth=1/sqrt(1836)), 32x64 "
                                "quadratic panels per species (65x129 points), eps=0.05, "
                                "direct sum, RK4 dt=0.1, remesh every step",
-                   "points": npts},
+                   "points": npts, "precision": args.precision},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
     }
     if not args.no_cpu_baseline:
@@ -290,6 +299,97 @@ def bench_c1(args):
     ctx.close()
     print(json.dumps(result))
 
+
+def c4_decks(n: int, seed: int = 4):
+    """Independent two-species Landau decks: amplitude log-uniform in [0.01, 0.5],
+    k uniform in [0.3, 0.6] (one wavelength per box), ion mass ratio in {25, 100, 1836},
+    base panels 16x32 to 128x256 (initial_height 4..7, p_height 1), eps 0.05, dt 0.1."""
+    import multi_species_amd as ms
+
+    rng = np.random.default_rng(seed)
+    decks = []
+    for _ in range(n):
+        amp = float(np.exp(rng.uniform(np.log(0.01), np.log(0.5))))
+        k = float(rng.uniform(0.3, 0.6))
+        mass = float(rng.choice([25.0, 100.0, 1836.0]))
+        h = int(rng.integers(4, 8))
+        sim = ms.sim_params(x_min=0.0, x_max=2 * np.pi / k, bcs=0, quadrature=0,
+                            greens_epsilon=0.05, use_treecode=0, dt=0.1, remesh_period=1)
+        e = ms.species_params(q=-1.0, m=1.0, p_min=-6.0, p_max=6.0, initial_height=h, p_height=1,
+                              ics_type=1, normalized_wavenumber=1.0, amp=amp, pth=1.0)
+        i = ms.species_params(q=1.0, m=mass, p_min=-6.0, p_max=6.0, initial_height=h, p_height=1,
+                              ics_type=1, normalized_wavenumber=1.0, amp=0.0,
+                              pth=1.0 / np.sqrt(mass))
+        decks.append((sim, [e, i]))
+    return decks
+
+
+def bench_c4(args):
+    """64 decks as one batch (mixed precision, graph-captured step) against the same decks
+    stepped one by one in plain FP64."""
+    import torch  # noqa: F401  (device init through the library)
+
+    import multi_species_amd as ms
+
+    ctx = ms.Context(0)
+    decks = c4_decks(args.decks)
+    b = ms.Simulation.batch(ctx, decks, ms.PRECISION_MIXED)
+    npts = sum(b.deck_info(d)[3] for d in range(b.n_decks))
+    for _ in range(args.warmup):
+        b.step()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        b.step()
+        lat.append(time.perf_counter() - t1)
+    wall = time.perf_counter() - t0
+    value = npts * args.steps / wall
+    # E error of the mixed batch against FP64 runs of the same decks, after the
+    # same number of steps (state drift included)
+    f = ms.Simulation.batch(ctx, decks, ms.PRECISION_F64)
+    f.step(args.warmup + args.steps)
+    err = 0.0
+    for d in range(b.n_decks):
+        first, nsp, _, _ = b.deck_info(d)
+        em = np.concatenate([b.points(first + k)[4] for k in range(nsp)])
+        ef = np.concatenate([f.points(first + k)[4] for k in range(nsp)])
+        err = max(err, float(np.max(np.abs(em - ef)) / np.max(np.abs(ef))))
+    f.close()
+    b.close()
+    # plain FP64, one deck at a time
+    t_plain = 0.0
+    for sim_p, sps in decks:
+        s = ms.Simulation.batch(ctx, [(sim_p, sps)], ms.PRECISION_F64)
+        s.step()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            s.step()
+        t_plain += time.perf_counter() - t1
+        s.close()
+    plain = npts * args.steps / t_plain
+    lat = np.array(lat) * 1e3
+    result = {
+        "metric": "batched RK4 point-steps/s",
+        "value": value,
+        "unit": "point-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": f"{args.decks} independent two-species Landau decks in one batch "
+                               "(amp log-uniform [0.01,0.5], k uniform [0.3,0.6], ion mass "
+                               "ratio in {25,100,1836}, 16x32..128x256 panels per species, "
+                               "eps 0.05, dt 0.1, remesh every step), mixed precision, "
+                               "graph-captured step",
+                   "decks": args.decks, "points": int(npts)},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "plain_f64_one_by_one": {"value": plain, "unit": "point-steps/s",
+                                 "speedup_of_batched_mixed": value / plain},
+        "rel_e_err_vs_f64": err,
+    }
+    ctx.close()
+    print(json.dumps(result))
 
 if __name__ == "__main__":
     main()

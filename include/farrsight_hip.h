@@ -86,6 +86,10 @@ int fs_emq_periodic_mixed(fs_ctx* ctx, double* es, const double* targets, int64_
                           const double* sources, const double* q_ws, int64_t ns,
                           double L, double epsilon);
 
+/* Launch shape (targets per thread, persistent workgroups per CU) that the
+ * last mixed call used, as chosen by its autotuner. Informational. */
+int fs_emq_mixed_shape(fs_ctx* ctx, int* targets_per_thread, int* wgs_per_cu);
+
 /* Same on device-resident arrays, enqueued on `stream`. */
 int fs_emq_periodic_mixed_dev(fs_ctx* ctx, double* d_es, const double* d_targets,
                               int64_t nt, const double* d_sources, const double* d_q_ws,

@@ -48,6 +48,7 @@ def lib():
         L.fs_emq_periodic_f64_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d, _vp]
         L.fs_emq_set_tuning.argtypes = [_vp, _int, _int]
         L.fs_emq_periodic_mixed.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
+        L.fs_emq_mixed_shape.argtypes = [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int)]
         L.fs_emq_periodic_mixed_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d,
                                                 _vp]
         L.fs_emq_treecode_f64.argtypes = [_vp, _dp, _dp, _dp, _i64, _d, _d, _d, _int, _int,
@@ -151,6 +152,13 @@ class Context:
         _check(lib().fs_emq_periodic_mixed(self.h, _P(es), _P(t), t.size, _P(s), _P(w),
                                            s.size, L, eps), "fs_emq_periodic_mixed")
         return es
+
+    def mixed_shape(self):
+        """(targets per thread, workgroups per CU) of the last mixed call."""
+        t, g = _int(), _int()
+        _check(lib().fs_emq_mixed_shape(self.h, ctypes.byref(t), ctypes.byref(g)),
+               "fs_emq_mixed_shape")
+        return t.value, g.value
 
     def emq_periodic_mixed_dev(self, es_ptr: int, t_ptr: int, nt: int, s_ptr: int, w_ptr: int,
                                ns: int, L: float, eps: float, stream: Optional[int] = None):

@@ -31,10 +31,18 @@ struct SegDev {
     double invL, delta2, C;
     // mixed only
     uint64_t band;
-    double d2s, wscale;
-    float b1, b2;
+    double d2s;       // delta^2 2^128 (squared units of the fixed point)
+    double b1z, b2z;  // series coefficients -3/4 and 5/8 in powers of delta^2 2^64
     int pw_base;
-    int src;  // index of the source segment
+    int src;          // index of the source segment
+    int is_src;       // some segment takes its sources from this one
+    int pad;
+};
+
+// Far-field record of one sorted source (FP32 path of the mixed kernel).
+struct FarSrc {
+    uint32_t h;             // high word of the position
+    float w, wb1, wb2;      // -w delta^2/2 and its products with b1z, b2z
 };
 
 struct WorkItem {
@@ -59,23 +67,37 @@ class FieldBatch {
 public:
     int plan(const std::vector<BatchSeg>& segs, int n_total, int precision, int num_cus,
              const FieldTuning& tun, hipStream_t stream);
+    // Launch shape of the pair kernel: targets per thread (1, 2, 4) and
+    // persistent workgroups per CU. Does not change any result.
+    int set_shape(int tpt, int wgs_per_cu, hipStream_t stream);
+    // Times every shape on this batch's data, keeps the fastest (es is used
+    // as scratch). Synchronizes the stream.
+    int autotune(const double* x, const double* w, double* es, hipStream_t stream,
+                 int* best_tpt, int* best_wgs);
     // x, w: concatenated positions and weights; es: field at every point of
     // the evaluated segments (others untouched).
     int run(const double* x, const double* w, double* es, hipStream_t stream);
     void release();
     int precision() const { return precision_; }
     int tpt() const { return tpt_; }
+    int wgs_per_cu() const { return wgs_per_cu_; }
     int items() const { return (int)items_.size(); }
     int64_t pairs() const { return pairs_; }
 
 private:
+    void build_items(int tpt, std::vector<WorkItem>& items) const;
+    int upload_items(const std::vector<WorkItem>& items, hipStream_t stream);
+    int launch_pairs(int nit, int grid, hipStream_t stream);
+
     int precision_ = kPrecisionF64;
-    int n_total_ = 0, nseg_ = 0, tpt_ = 2, grid_ = 0;
+    int n_total_ = 0, nseg_ = 0, tpt_ = 2, grid_ = 0, wgs_per_cu_ = 8, num_cus_ = 256;
+    const double* x_ = nullptr;  // inputs of the current run (FP64 pair kernel)
+    const double* w_ = nullptr;
     int64_t pairs_ = 0;
     std::vector<SegDev> segs_;
     std::vector<WorkItem> items_;
     std::vector<ScanBlock> scan_;
-    DevBuf tables_, work_;
+    DevBuf tables_, work_, items_buf_;
     // device views
     SegDev* d_segs_ = nullptr;
     WorkItem* d_items_ = nullptr;
@@ -83,8 +105,8 @@ private:
     int32_t* d_seg_of_ = nullptr;
     int* d_counter_ = nullptr;
     // work space offsets (bytes into work_)
-    size_t o_part_ = 0, o_key_ = 0, o_skey_ = 0, o_perm_ = 0, o_sort_ = 0, o_hs_ = 0, o_ws_ = 0,
-           o_wf_ = 0, o_pw_ = 0, o_bsum_ = 0, o_bsa_ = 0, o_tot_ = 0;
+    size_t o_part_ = 0, o_key_ = 0, o_skey_ = 0, o_perm_ = 0, o_sort_ = 0, o_as_ = 0,
+           o_asm1_ = 0, o_ws_ = 0, o_far_ = 0, o_pw_ = 0, o_bsum_ = 0, o_bsa_ = 0, o_tot_ = 0;
 };
 
 // Stable radix sort by (segment of the original index, top 32 key bits)

@@ -13,9 +13,10 @@
 // bitwise the same alone and in any mix.
 //
 // Two precisions:
-//  * FP64: per pair and summation order exactly those of fs_field.hip
-//    (emq_direct_f64_kernel): d = x_i - x_j, z = d/L - rint, refined rsqrt,
-//    TwoSum over 256-source runs, halved at the end.
+//  * FP64: per pair exactly fs_field.hip's emq_direct_f64_kernel: d = x_i - x_j,
+//    z = d/L - rint, refined rsqrt, TwoSum over 256-source runs of each chunk,
+//    halved at the end (the chunking, and so the last bits of the sums, can
+//    differ from the single-call kernel's).
 //  * mixed: the decomposition of fs_field_mixed.hip's comment (FP64 step and
 //    neutralizing part from sorted prefix sums, FP64 near band, FP32
 //    far-field series, compensated FP64 accumulation), with positions sorted
@@ -43,7 +44,8 @@ constexpr int kTile = 32;          // mixed: sources per tile
 constexpr int kScanItems = 2048;   // mixed: sorted positions per scan block
 constexpr uint64_t kHalf = 0x8000000000000000ull;
 constexpr uint64_t kAntiMargin = uint64_t(1) << 40;
-constexpr uint64_t kLow32 = 0xffffffffull;
+constexpr int kSortShift = 40;  // keys are sorted by bits 40..63 (fs_radix.hip)
+constexpr uint64_t kLowSort = (uint64_t(1) << kSortShift) - 1;
 
 __device__ __forceinline__ int pull_item(int* counter) {
     __shared__ int s_item;
@@ -57,7 +59,7 @@ __device__ __forceinline__ int pull_item(int* counter) {
 // ---- FP64 ------------------------------------------------------------------
 
 template <int TPT>
-__global__ __launch_bounds__(kBlock) void batch_f64_kernel(const WorkItem* __restrict__ items,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void batch_f64_kernel(const WorkItem* __restrict__ items,
                                                            int nitems,
                                                            const SegDev* __restrict__ segs,
                                                            const double* __restrict__ x,
@@ -144,50 +146,70 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(int n_total, const int32_t
     key[g] = (uint64_t)__builtin_ldexp(a, 64);
 }
 
-// Sorted weights, FP32 far-field weights, high words and per-block sums.
-// Coincident sources (equal keys: the species of a deck share the x lattice
-// of the remesh grid) have the same far-field kernel value, so within each
-// 32-source tile (the unit that takes either the FP64 or the FP32 path) their
-// FP32 far-field weight is their FP64 sum, held by the first of them (the
-// others get 0): the FP32 terms then carry the net charge, and their rounding
-// errors scale with it instead of with each species' charge.
+// Per sorted source: FP64 position a = U 2^-64 and a - 1 (both exact; the
+// near band reads the second across the periodic seam), FP64 weight, and the
+// far-field record. Coincident sources (equal keys: the species of a deck
+// share the x lattice of the remesh grid) have the same kernel value, so
+// within each 32-source tile (the unit that takes one path) their far-field
+// weight is their FP64 sum, held by the first of them (the others get 0):
+// the FP32 terms then carry the net charge, and their rounding errors scale
+// with it instead of with each species' charge. One thread per position
+// (weights and records only for source segments).
 __global__ __launch_bounds__(kBlock) void gather_kernel(
-    const ScanBlock* __restrict__ blocks, const SegDev* __restrict__ segs,
+    int n_total, const int32_t* __restrict__ seg_of, const SegDev* __restrict__ segs,
     const uint64_t* __restrict__ us, const int32_t* __restrict__ perm,
-    const double* __restrict__ w, uint32_t* __restrict__ hs, double* __restrict__ ws,
-    float* __restrict__ wf, double* __restrict__ bsum, double* __restrict__ bsa) {
+    const double* __restrict__ w, double* __restrict__ as, double* __restrict__ asm1,
+    double* __restrict__ ws, FarSrc* __restrict__ far) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_total) return;
+    const SegDev& S = segs[seg_of[i]];
+    const uint64_t u = us[i];
+    const double a = __builtin_ldexp((double)u, -64);
+    as[i] = a;  // targets read it too
+    asm1[i] = a - 1.0;
+    if (!S.is_src) return;
+    ws[i] = w[perm[i]];
+    // tiles start at multiples of kTile from the segment start
+    const int t0 = S.off + ((i - S.off) & ~(kTile - 1));
+    const int t1 = min(t0 + kTile, S.off + S.n);
+    bool first = true;
+    for (int j = t0; j < i; ++j)
+        if (us[j] == u) {
+            first = false;
+            break;
+        }
+    double net = 0.0;
+    if (first)
+        for (int j = i; j < t1; ++j)
+            if (us[j] == u) net += w[perm[j]];
+    // g w = -(w delta^2/2) s/z^2 (1 - 3q/4 + 5q^2/8), z = zs 2^-32, q = delta^2/z^2:
+    // with w'' = -w delta^2/2 the FP32 term is sq (w'' + q (w'' b1 + q w'' b2)),
+    // sq = s/zs^2 (zs units), q = |sq|, and the tile sum is scaled by 2^64.
+    const double w2 = -0.5 * net * S.delta2;
+    FarSrc f;
+    f.h = (uint32_t)(u >> 32);
+    f.w = (float)w2;
+    f.wb1 = (float)(w2 * S.b1z);
+    f.wb2 = (float)(w2 * S.b2z);
+    far[i] = f;
+}
+
+// Per scan block of a source segment: sum of w and of w a.
+__global__ __launch_bounds__(kBlock) void blocksum_kernel(const ScanBlock* __restrict__ blocks,
+                                                          const double* __restrict__ ws,
+                                                          const double* __restrict__ as,
+                                                          double* __restrict__ bsum,
+                                                          double* __restrict__ bsa) {
     __shared__ double sw[kBlock], swa[kBlock];
     const ScanBlock B = blocks[blockIdx.x];
-    const SegDev& S = segs[B.seg];
-    const double wscale = S.wscale;
-    const int s_lo = S.off, s_hi = S.off + S.n;  // the tiles never cross these
     constexpr int kPer = kScanItems / kBlock;
-    const int base = B.b + (int)threadIdx.x * kPer;
     double s = 0.0, sa = 0.0;
     for (int k = 0; k < kPer; ++k) {
-        const int i = base + k;
+        const int i = B.b + k * kBlock + (int)threadIdx.x;
         if (i < B.e) {
-            const uint64_t u = us[i];
-            const uint32_t top = (uint32_t)(u >> 32);
-            const double wi = w[perm[i]];
-            hs[i] = top;
-            ws[i] = wi;
+            const double wi = ws[i];
             s += wi;
-            sa += wi * __builtin_ldexp((double)u, -64);
-            // tiles start at multiples of kTile from the segment start
-            const int t0 = s_lo + ((i - s_lo) & ~(kTile - 1));
-            const int t1 = min(t0 + kTile, s_hi);
-            bool first = true;
-            for (int j = t0; j < i; ++j)
-                if (us[j] == u) {
-                    first = false;
-                    break;
-                }
-            double net = 0.0;
-            if (first)
-                for (int j = i; j < t1; ++j)
-                    if (us[j] == u) net += w[perm[j]];
-            wf[i] = (float)(net * wscale);
+            sa += wi * as[i];
         }
     }
     sw[threadIdx.x] = s;
@@ -263,46 +285,55 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(
     }
 }
 
+// Pair kernel. Per (target block, 32-source tile), one of three paths, the
+// choice uniform over the workgroup:
+//  * near tile (some displacement within B of 0): FP64 from the FP64
+//    positions, z = a_i - a_j with the periodic seam handled per tile (the
+//    targets' a - 1 and the sources' a - 1 are exact where they are used);
+//  * antipodal tile (some displacement within 2^-24 of 1/2) or a tile whose
+//    displacements span half a period: FP64 from the exact 64-bit difference;
+//  * far tile: the FP32 series from the high words.
 template <int TPT>
-__global__ __launch_bounds__(kBlock) void batch_mixed_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void batch_mixed_kernel(
     const WorkItem* __restrict__ items, int nitems, const SegDev* __restrict__ segs,
-    const uint64_t* __restrict__ us, const uint32_t* __restrict__ hs,
-    const double* __restrict__ ws, const float* __restrict__ wf, double* __restrict__ part,
-    int* __restrict__ counter) {
+    const uint64_t* __restrict__ us, const double* __restrict__ as,
+    const double* __restrict__ asm1, const double* __restrict__ ws,
+    const FarSrc* __restrict__ far, double* __restrict__ part, int* __restrict__ counter) {
     for (;;) {
         const int it = pull_item(counter);
         if (it >= nitems) break;
         const WorkItem W = items[it];
         const SegDev S = segs[W.seg];
-        // target positions: sorted by their top words only, so widen the
-        // block's range by one unit of 2^32 on each side
-        const uint64_t uf = us[W.tb] & ~kLow32, ul = us[W.te - 1] | kLow32;
+        const double d2 = S.delta2;
+        // positions are sorted by their top bits only: widen the block's range
+        // by one unit of the sort on each side
+        const uint64_t uf = us[W.tb] & ~kLowSort, ul = us[W.te - 1] | kLowSort;
         const uint64_t tspan = ul - uf;
         uint64_t ui[TPT];
         uint32_t hi[TPT];
-        double sh[TPT], sl[TPT];
+        double ai[TPT], sh[TPT], sl[TPT];
 #pragma unroll
         for (int t = 0; t < TPT; ++t) {
             const int i = min(W.tb + (int)threadIdx.x + t * kBlock, W.te - 1);
             ui[t] = us[i];
             hi[t] = (uint32_t)(ui[t] >> 32);
+            ai[t] = as[i];
             sh[t] = 0.0;
             sl[t] = 0.0;
         }
         for (int jb = W.sb; jb < W.se; jb += kTile) {
             const int je = min(W.se, jb + kTile);
-            const uint64_t vf = us[jb] & ~kLow32, vl = us[je - 1] | kLow32;
+            const uint64_t vf = us[jb] & ~kLowSort, vl = us[je - 1] | kLowSort;
             const uint64_t vspan = vl - vf;
             const bool wide = tspan >= kHalf || vspan >= kHalf || tspan + vspan >= kHalf;
             const uint64_t len = tspan + vspan;
             const uint64_t lo = uf - vl;
-            const bool band = S.all_band || wide || cyc_meets(lo + S.band, len, 2 * S.band) ||
-                              cyc_meets(lo - kHalf + kAntiMargin, len, 2 * kAntiMargin);
-            if (band) {
+            const bool near = S.all_band || cyc_meets(lo + S.band, len, 2 * S.band);
+            const bool anti = cyc_meets(lo - kHalf + kAntiMargin, len, 2 * kAntiMargin);
+            if (wide || anti) {
                 double acc[TPT];
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) acc[t] = 0.0;
-#pragma unroll 2
                 for (int j = jb; j < je; ++j) {
                     const uint64_t uj = us[j];
                     const double wj = ws[j];
@@ -318,26 +349,82 @@ __global__ __launch_bounds__(kBlock) void batch_mixed_kernel(
                 }
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], acc[t]);
+            } else if (near) {
+                // no displacement of this tile reaches 1/2, so each pair's
+                // wrap is the tile's: targets above the sources by more than
+                // half a period are seen one period down, and the reverse
+                const bool t_down = uf > vl && uf - vl > kHalf;
+                const bool s_down = vf > ul && vf - ul > kHalf;
+                const double* __restrict__ src = s_down ? asm1 : as;
+                double acc[TPT], at[TPT];
+#pragma unroll
+                for (int t = 0; t < TPT; ++t) {
+                    acc[t] = 0.0;
+                    at[t] = t_down ? ai[t] - 1.0 : ai[t];
+                }
+#pragma unroll 2
+                for (int j = jb; j < je; ++j) {
+                    const double aj = src[j];
+                    const double wj = ws[j];
+#pragma unroll
+                    for (int t = 0; t < TPT; ++t) {
+                        const double z = at[t] - aj;
+                        const double y = rsq_f64(__builtin_fma(z, z, d2));
+                        const double s = __builtin_copysign(1.0, z);
+                        acc[t] = __builtin_fma(wj, __builtin_fma(z, y, -s), acc[t]);
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], acc[t]);
             } else {
                 float acc[TPT];
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) acc[t] = 0.0f;
-#pragma unroll 8
-                for (int j = jb; j < je; ++j) {
-                    const uint32_t hj = hs[j];
-                    const float wj = wf[j];
+                if (je - jb == kTile) {
+                    // whole tile: records in groups of 4 (one scalar load each),
+                    // the next group's load issued before this group's math
+                    FarSrc cur[4], nxt[4];
 #pragma unroll
-                    for (int t = 0; t < TPT; ++t) {
-                        const float zs = (float)(int32_t)(hi[t] - hj);
-                        const float r = __builtin_amdgcn_rcpf(zs);
-                        const float sq = r * __builtin_fabsf(r);  // s(z) / zs^2
-                        const float q = __builtin_fabsf(sq);
-                        const float p = __builtin_fmaf(q, __builtin_fmaf(q, S.b2, S.b1), 1.0f);
-                        acc[t] = __builtin_fmaf(wj * sq, p, acc[t]);
+                    for (int k = 0; k < 4; ++k) cur[k] = far[jb + k];
+#pragma unroll
+                    for (int g = 0; g < kTile / 4; ++g) {
+                        if (g + 1 < kTile / 4) {
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) nxt[k] = far[jb + 4 * (g + 1) + k];
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const FarSrc f = cur[k];
+#pragma unroll
+                            for (int t = 0; t < TPT; ++t) {
+                                const float zs = (float)(int32_t)(hi[t] - f.h);
+                                const float r = __builtin_amdgcn_rcpf(zs);
+                                const float sq = r * __builtin_fabsf(r);
+                                const float q = __builtin_fabsf(sq);
+                                const float p = __builtin_fmaf(q, __builtin_fmaf(q, f.wb2, f.wb1), f.w);
+                                acc[t] = __builtin_fmaf(sq, p, acc[t]);
+                            }
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) cur[k] = nxt[k];
+                    }
+                } else {
+                    for (int j = jb; j < je; ++j) {
+                        const FarSrc f = far[j];
+#pragma unroll
+                        for (int t = 0; t < TPT; ++t) {
+                            const float zs = (float)(int32_t)(hi[t] - f.h);
+                            const float r = __builtin_amdgcn_rcpf(zs);
+                            const float sq = r * __builtin_fabsf(r);  // s(z) / zs^2
+                            const float q = __builtin_fabsf(sq);
+                            const float p = __builtin_fmaf(q, __builtin_fmaf(q, f.wb2, f.wb1), f.w);
+                            acc[t] = __builtin_fmaf(sq, p, acc[t]);
+                        }
                     }
                 }
 #pragma unroll
-                for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], (double)acc[t]);
+                for (int t = 0; t < TPT; ++t)
+                    two_sum_acc(sh[t], sl[t], __builtin_ldexp((double)acc[t], 64));
             }
         }
 #pragma unroll
@@ -349,21 +436,21 @@ __global__ __launch_bounds__(kBlock) void batch_mixed_kernel(
 }
 
 // Sum of the sorted weights of [off, off+n) whose key is <= key. The keys
-// are sorted by their top words; within a run of equal top words the full
-// keys are compared one by one.
+// are sorted by their top bits; within a run of equal top bits the full keys
+// are compared one by one.
 __device__ double w_le(const uint64_t* __restrict__ us, const double* __restrict__ ws,
                        const double* __restrict__ pw, int off, int n, int pwb, uint64_t key) {
-    const uint32_t kt = (uint32_t)(key >> 32);
+    const uint64_t kt = key >> kSortShift;
     int lo = 0, hi = n;  // first index with top >= kt
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if ((uint32_t)(us[off + mid] >> 32) < kt)
+        if ((us[off + mid] >> kSortShift) < kt)
             lo = mid + 1;
         else
             hi = mid;
     }
     double s = pw[pwb + lo];
-    for (int j = lo; j < n && (uint32_t)(us[off + j] >> 32) == kt; ++j)
+    for (int j = lo; j < n && (us[off + j] >> kSortShift) == kt; ++j)
         if (us[off + j] <= key) s += ws[off + j];
     return s;
 }
@@ -409,11 +496,13 @@ __global__ void zero_int_kernel(int* p) { *p = 0; }
 
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-// Source chunking of one deck: the FP64 rule of fs_field.hip (plan_direct)
-// and its mixed counterpart. Depends only on the deck's sizes and the device.
+// Source chunking of one deck: enough (target block, source chunk) items for
+// about 16 wavefronts per SIMD when the deck runs alone (a persistent kernel
+// with up to 8 resident workgroups per CU then always has work). Depends only
+// on the deck's sizes and the device, never on the rest of the batch.
 void chunking(int precision, int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun,
               int& splits, int& chunk) {
-    const int64_t want_waves = int64_t(16) * num_cus;
+    const int64_t want_waves = int64_t(64) * num_cus;
     const int unit = precision == kPrecisionF64 ? kSub : kTile;
     const int tpt_auto = (nt >= want_waves * 64 * 2) ? 2 : 1;
     const int64_t waves = (nt + 64 * tpt_auto - 1) / (64 * tpt_auto);
@@ -435,6 +524,7 @@ void chunking(int precision, int64_t nt, int64_t ns, int num_cus, const FieldTun
 void FieldBatch::release() {
     tables_.release();
     work_.release();
+    items_buf_.release();
     segs_.clear();
     items_.clear();
     scan_.clear();
@@ -494,20 +584,17 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
             S.band = S.all_band ? (uint64_t(1) << 62) : (uint64_t)std::ldexp(B, 64);
             S.d2s = d2 * two64 * two64;
             const double kq2 = d2 * two64;
-            S.b1 = S.all_band ? 0.0f : (float)(-0.75 * kq2);
-            S.b2 = S.all_band ? 0.0f : (float)(0.625 * kq2 * kq2);
+            S.b1z = S.all_band ? 0.0 : -0.75 * kq2;
+            S.b2z = S.all_band ? 0.0 : 0.625 * kq2 * kq2;
         }
     }
     if (precision == kPrecisionMixed) {
-        // the far-field weight scale belongs to the source segment (set by
-        // the segment that evaluates against it)
-        for (int d = 0; d < nseg_; ++d)
-            if (segs[d].eval) segs_[segs[d].src].wscale = -0.5 * segs_[d].delta2 * 18446744073709551616.0;
         // scan blocks of every segment that serves as a source
         std::vector<char> is_src(nseg_, 0);
         for (int d = 0; d < nseg_; ++d)
             if (segs[d].eval) is_src[segs[d].src] = 1;
         for (int d = 0; d < nseg_; ++d) {
+            segs_[d].is_src = is_src[d];
             if (!is_src[d]) continue;
             const SegDev& S = segs_[d];
             const int first = (int)scan_.size();
@@ -518,37 +605,17 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
                                           first, cnt});
         }
     }
-    // work items, largest first
-    const int tblock = kBlock * tpt_;
-    for (int d = 0; d < nseg_; ++d) {
-        const SegDev& S = segs_[d];
-        if (!S.eval || S.n == 0 || S.src_n == 0) continue;
-        for (int tb = S.off; tb < S.off + S.n; tb += tblock)
-            for (int s = 0; s < S.splits; ++s) {
-                const int sb = S.src_off + s * S.chunk;
-                const int se = std::min(S.src_off + S.src_n, sb + S.chunk);
-                items_.push_back(WorkItem{d, tb, std::min(S.off + S.n, tb + tblock), sb, se, s});
-            }
-    }
-    std::stable_sort(items_.begin(), items_.end(), [](const WorkItem& a, const WorkItem& b) {
-        return (int64_t)(a.te - a.tb) * (a.se - a.sb) > (int64_t)(b.te - b.tb) * (b.se - b.sb);
-    });
     // device tables
     const size_t b_segs = align_up(sizeof(SegDev) * nseg_, 256);
-    const size_t b_items = align_up(sizeof(WorkItem) * std::max<size_t>(items_.size(), 1), 256);
     const size_t b_scan = align_up(sizeof(ScanBlock) * std::max<size_t>(scan_.size(), 1), 256);
     const size_t b_segof = align_up(sizeof(int32_t) * seg_of.size(), 256);
-    FS_TRY(tables_.ensure(b_segs + b_items + b_scan + b_segof + 256, stream));
+    FS_TRY(tables_.ensure(b_segs + b_scan + b_segof + 256, stream));
     d_segs_ = tables_.as<SegDev>(0);
-    d_items_ = tables_.as<WorkItem>(b_segs);
-    d_scan_ = tables_.as<ScanBlock>(b_segs + b_items);
-    d_seg_of_ = tables_.as<int32_t>(b_segs + b_items + b_scan);
-    d_counter_ = tables_.as<int>(b_segs + b_items + b_scan + b_segof);
+    d_scan_ = tables_.as<ScanBlock>(b_segs);
+    d_seg_of_ = tables_.as<int32_t>(b_segs + b_scan);
+    d_counter_ = tables_.as<int>(b_segs + b_scan + b_segof);
     FS_HIP_CHECK(hipMemcpyAsync(d_segs_, segs_.data(), sizeof(SegDev) * nseg_,
                                 hipMemcpyHostToDevice, stream));
-    if (!items_.empty())
-        FS_HIP_CHECK(hipMemcpyAsync(d_items_, items_.data(), sizeof(WorkItem) * items_.size(),
-                                    hipMemcpyHostToDevice, stream));
     if (!scan_.empty())
         FS_HIP_CHECK(hipMemcpyAsync(d_scan_, scan_.data(), sizeof(ScanBlock) * scan_.size(),
                                     hipMemcpyHostToDevice, stream));
@@ -570,19 +637,155 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         o_skey_ = take(8 * n);
         o_perm_ = take(4 * n);
         o_sort_ = take(radix_sort_bytes(n_total));
-        o_hs_ = take(4 * n);
+        o_as_ = take(8 * n);
+        o_asm1_ = take(8 * n);
         o_ws_ = take(8 * n);
-        o_wf_ = take(4 * n);
+        o_far_ = take(sizeof(FarSrc) * n);
         o_pw_ = take(8 * (n + nseg_));
         o_bsum_ = take(8 * std::max<size_t>(scan_.size(), 1));
         o_bsa_ = take(8 * std::max<size_t>(scan_.size(), 1));
         o_tot_ = take(8 * (size_t)nseg_);
     }
     FS_TRY(work_.ensure(o, stream));
-    // persistent grid: every item can start at once up to 8 workgroups per CU
-    grid_ = (int)std::min<size_t>(items_.size(), (size_t)8 * num_cus);
+    num_cus_ = num_cus;
     // the host tables must outlive the copies
     FS_HIP_CHECK(hipStreamSynchronize(stream));
+    return set_shape(tpt_, wgs_per_cu_, stream);
+}
+
+// Work items for blocks of 256*tpt targets, largest first, and the
+// persistent grid (wgs_per_cu workgroups per CU, at most one per item).
+void FieldBatch::build_items(int tpt, std::vector<WorkItem>& items) const {
+    items.clear();
+    const int tblock = kBlock * tpt;
+    for (int d = 0; d < nseg_; ++d) {
+        const SegDev& S = segs_[d];
+        if (!S.eval || S.n == 0 || S.src_n == 0) continue;
+        for (int tb = S.off; tb < S.off + S.n; tb += tblock)
+            for (int s = 0; s < S.splits; ++s) {
+                const int sb = S.src_off + s * S.chunk;
+                const int se = std::min(S.src_off + S.src_n, sb + S.chunk);
+                items.push_back(WorkItem{d, tb, std::min(S.off + S.n, tb + tblock), sb, se, s});
+            }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const WorkItem& a, const WorkItem& b) {
+        return (int64_t)(a.te - a.tb) * (a.se - a.sb) > (int64_t)(b.te - b.tb) * (b.se - b.sb);
+    });
+}
+
+int FieldBatch::upload_items(const std::vector<WorkItem>& items, hipStream_t stream) {
+    FS_TRY(items_buf_.ensure(sizeof(WorkItem) * std::max<size_t>(items.size(), 1), stream));
+    d_items_ = items_buf_.as<WorkItem>();
+    if (!items.empty())
+        FS_HIP_CHECK(hipMemcpyAsync(d_items_, items.data(), sizeof(WorkItem) * items.size(),
+                                    hipMemcpyHostToDevice, stream));
+    FS_HIP_CHECK(hipStreamSynchronize(stream));
+    return FS_OK;
+}
+
+int FieldBatch::set_shape(int tpt, int wgs_per_cu, hipStream_t stream) {
+    if (tpt != 1 && tpt != 2 && tpt != 4) return fail(FS_E_INVALID, "field batch: tpt %d", tpt);
+    tpt_ = tpt;
+    wgs_per_cu_ = std::max(1, wgs_per_cu);
+    build_items(tpt_, items_);
+    FS_TRY(upload_items(items_, stream));
+    grid_ = (int)std::min<size_t>(items_.size(), (size_t)wgs_per_cu_ * num_cus_);
+    return FS_OK;
+}
+
+int FieldBatch::launch_pairs(int nit, int grid, hipStream_t stream) {
+    if (nit <= 0 || grid <= 0) return FS_OK;
+    char* wk = static_cast<char*>(work_.ptr);
+    double* part = reinterpret_cast<double*>(wk + o_part_);
+    if (precision_ == kPrecisionF64) {
+        const double* x = x_;
+        const double* w = w_;
+        switch (tpt_) {
+            case 1: batch_f64_kernel<1><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, x, w, part, d_counter_); break;
+            case 4: batch_f64_kernel<4><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, x, w, part, d_counter_); break;
+            default: batch_f64_kernel<2><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, x, w, part, d_counter_); break;
+        }
+    } else {
+        const uint64_t* skey = reinterpret_cast<const uint64_t*>(wk + o_skey_);
+        const double* as = reinterpret_cast<const double*>(wk + o_as_);
+        const double* asm1 = reinterpret_cast<const double*>(wk + o_asm1_);
+        const double* ws = reinterpret_cast<const double*>(wk + o_ws_);
+        const FarSrc* far = reinterpret_cast<const FarSrc*>(wk + o_far_);
+        switch (tpt_) {
+            case 1: batch_mixed_kernel<1><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, part, d_counter_); break;
+            case 4: batch_mixed_kernel<4><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, part, d_counter_); break;
+            default: batch_mixed_kernel<2><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, part, d_counter_); break;
+        }
+    }
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+// Times the pair kernel for every (tpt, workgroups per CU) on this batch's
+// data and keeps the fastest. A trial runs all the launch's items with their
+// source ranges shortened to about 2e10 pairs in total. The shape changes neither the items' summation order
+// nor any result. es receives a full evaluation (it is scratch here).
+int FieldBatch::autotune(const double* x, const double* w, double* es, hipStream_t stream,
+                         int* best_tpt, int* best_wgs) {
+    FS_TRY(run(x, w, es, stream));  // sorted arrays and plan tables in place
+    static const int tpts[3] = {1, 2, 4};
+    static const int wgss[3] = {2, 4, 8};
+    hipEvent_t e0, e1;
+    FS_HIP_CHECK(hipEventCreate(&e0));
+    FS_HIP_CHECK(hipEventCreate(&e1));
+    double best_rate = -1.0;
+    int bt = tpt_, bw = wgs_per_cu_;
+    std::vector<WorkItem> trial;
+    int rc = FS_OK;
+    for (int a = 0; a < 3 && rc == FS_OK; ++a) {
+        // every item of the real launch (so the grid fills as it will), each
+        // cut to a source count that keeps the trial near 2e10 pairs
+        build_items(tpts[a], trial);
+        int64_t targets = 0;
+        for (const WorkItem& it : trial) targets += it.te - it.tb;
+        const int64_t cut = std::max<int64_t>(512, (int64_t)2e10 / std::max<int64_t>(targets, 1));
+        // the windows are spread over each chunk (a window at the chunk start
+        // would hold the near-band sources of most target blocks)
+        int64_t pairs = 0;
+        uint64_t hsh = 0x9e3779b97f4a7c15ull;
+        for (WorkItem& it : trial) {
+            const int64_t len = it.se - it.sb;
+            if (len > cut) {
+                hsh = hsh * 6364136223846793005ull + 1442695040888963407ull;
+                const int64_t slots = (len - cut) / kTile + 1;
+                it.sb += (int)((int64_t)((hsh >> 33) % (uint64_t)slots) * kTile);
+                it.se = (int)(it.sb + cut);
+            }
+            pairs += (int64_t)(it.te - it.tb) * (it.se - it.sb);
+        }
+        tpt_ = tpts[a];
+        if ((rc = upload_items(trial, stream)) != FS_OK) break;
+        for (int b = 0; b < 3 && rc == FS_OK; ++b) {
+            const int grid = (int)std::min<size_t>(trial.size(), (size_t)wgss[b] * num_cus_);
+            float ms = 0.f;
+            for (int rep = 0; rep < 2 && rc == FS_OK; ++rep) {
+                zero_int_kernel<<<1, 1, 0, stream>>>(d_counter_);
+                if (hipEventRecord(e0, stream) != hipSuccess) rc = fail(FS_E_HIP, "autotune: event");
+                if (rc == FS_OK) rc = launch_pairs((int)trial.size(), grid, stream);
+                if (rc == FS_OK && hipEventRecord(e1, stream) != hipSuccess) rc = fail(FS_E_HIP, "autotune: event");
+                if (rc == FS_OK && hipEventSynchronize(e1) != hipSuccess) rc = fail(FS_E_HIP, "autotune: sync");
+                if (rc == FS_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+            }
+            const double rate = ms > 0 ? pairs / (double)ms : 0.0;
+            if (rc == FS_OK && rate > best_rate) {
+                best_rate = rate;
+                bt = tpts[a];
+                bw = wgss[b];
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    zero_int_kernel<<<1, 1, 0, stream>>>(d_counter_);
+    FS_TRY(rc);
+    FS_TRY(set_shape(bt, bw, stream));
+    if (best_tpt) *best_tpt = bt;
+    if (best_wgs) *best_wgs = bw;
     return FS_OK;
 }
 
@@ -591,24 +794,10 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
     char* wk = static_cast<char*>(work_.ptr);
     double* part = reinterpret_cast<double*>(wk + o_part_);
     const int nit = (int)items_.size();
+    x_ = x;
+    w_ = w;
     if (precision_ == kPrecisionF64) {
-        if (nit > 0) {
-            switch (tpt_) {
-                case 1:
-                    batch_f64_kernel<1><<<grid_, kBlock, 0, stream>>>(d_items_, nit, d_segs_, x, w,
-                                                                       part, d_counter_);
-                    break;
-                case 4:
-                    batch_f64_kernel<4><<<grid_, kBlock, 0, stream>>>(d_items_, nit, d_segs_, x, w,
-                                                                       part, d_counter_);
-                    break;
-                default:
-                    batch_f64_kernel<2><<<grid_, kBlock, 0, stream>>>(d_items_, nit, d_segs_, x, w,
-                                                                       part, d_counter_);
-                    break;
-            }
-            FS_HIP_CHECK(hipGetLastError());
-        }
+        FS_TRY(launch_pairs(nit, grid_, stream));
         finalize_f64_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(n_total_, d_seg_of_, d_segs_,
                                                                       part, es, d_counter_);
         FS_HIP_CHECK(hipGetLastError());
@@ -617,9 +806,10 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
     uint64_t* key = reinterpret_cast<uint64_t*>(wk + o_key_);
     uint64_t* skey = reinterpret_cast<uint64_t*>(wk + o_skey_);
     int32_t* perm = reinterpret_cast<int32_t*>(wk + o_perm_);
-    uint32_t* hs = reinterpret_cast<uint32_t*>(wk + o_hs_);
+    double* as = reinterpret_cast<double*>(wk + o_as_);
+    double* asm1 = reinterpret_cast<double*>(wk + o_asm1_);
     double* ws = reinterpret_cast<double*>(wk + o_ws_);
-    float* wf = reinterpret_cast<float*>(wk + o_wf_);
+    FarSrc* far = reinterpret_cast<FarSrc*>(wk + o_far_);
     double* pw = reinterpret_cast<double*>(wk + o_pw_);
     double* bsum = reinterpret_cast<double*>(wk + o_bsum_);
     double* bsa = reinterpret_cast<double*>(wk + o_bsa_);
@@ -627,31 +817,17 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
     prep_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(n_total_, d_seg_of_, d_segs_, x, key);
     FS_HIP_CHECK(hipGetLastError());
     FS_TRY(radix_sort_seg(key, n_total_, d_seg_of_, nseg_, skey, perm, wk + o_sort_, stream));
+    gather_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(n_total_, d_seg_of_, d_segs_, skey,
+                                                            perm, w, as, asm1, ws, far);
+    FS_HIP_CHECK(hipGetLastError());
     if (!scan_.empty()) {
-        gather_kernel<<<(unsigned)scan_.size(), kBlock, 0, stream>>>(d_scan_, d_segs_, skey, perm,
-                                                                     w, hs, ws, wf, bsum, bsa);
+        blocksum_kernel<<<(unsigned)scan_.size(), kBlock, 0, stream>>>(d_scan_, ws, as, bsum, bsa);
         FS_HIP_CHECK(hipGetLastError());
         scan_kernel<<<(unsigned)scan_.size(), kBlock, 0, stream>>>(d_scan_, d_segs_, ws, bsum, bsa,
                                                                    pw, tot);
         FS_HIP_CHECK(hipGetLastError());
     }
-    if (nit > 0) {
-        switch (tpt_) {
-            case 1:
-                batch_mixed_kernel<1><<<grid_, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, hs,
-                                                                     ws, wf, part, d_counter_);
-                break;
-            case 4:
-                batch_mixed_kernel<4><<<grid_, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, hs,
-                                                                     ws, wf, part, d_counter_);
-                break;
-            default:
-                batch_mixed_kernel<2><<<grid_, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, hs,
-                                                                     ws, wf, part, d_counter_);
-                break;
-        }
-        FS_HIP_CHECK(hipGetLastError());
-    }
+    FS_TRY(launch_pairs(nit, grid_, stream));
     finalize_mixed_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
         n_total_, d_seg_of_, d_segs_, skey, perm, ws, pw, tot, part, es, d_counter_);
     FS_HIP_CHECK(hipGetLastError());

@@ -55,6 +55,21 @@ FieldBatch& api_batch(fs_ctx* ctx) {
     return *ctx->api_batch;
 }
 
+// Launch shape for this problem class (sizes by powers of two): autotuned on
+// the first call of a class, then reused. es is written by the tuning run.
+int shape(fs_ctx* ctx, FieldBatch& fb, int64_t nt, int64_t ns, bool same, const double* x,
+          const double* w, double* es, hipStream_t st) {
+    if (ctx->tuning.targets_per_thread != 0) return FS_OK;  // explicit tuning wins
+    auto lg = [](int64_t v) { uint64_t b = 0; while ((int64_t(1) << b) < v && b < 62) ++b; return b; };
+    const uint64_t key = (lg(nt) << 8) | (lg(ns) << 1) | (same ? 1 : 0);
+    auto it = ctx->shape_cache.find(key);
+    if (it != ctx->shape_cache.end()) return fb.set_shape(it->second.first, it->second.second, st);
+    int t = 2, g = 8;
+    FS_TRY(fb.autotune(x, w, es, st, &t, &g));
+    ctx->shape_cache[key] = std::make_pair(t, g);
+    return FS_OK;
+}
+
 }  // namespace
 }  // namespace fs
 
@@ -86,6 +101,7 @@ int fs_emq_periodic_mixed_dev(fs_ctx* ctx, double* d_es, const double* d_targets
         a.eps = epsilon;
         segs.push_back(a);
         FS_TRY(fb.plan(segs, (int)nt, fs::kPrecisionMixed, ctx->num_cus, ctx->tuning, st));
+        FS_TRY(fs::shape(ctx, fb, nt, ns, true, d_sources, d_q_ws, d_es, st));
         return fb.run(d_sources, d_q_ws, d_es, st);
     }
     // sources then targets in one array; the targets' segment is sourced
@@ -114,8 +130,17 @@ int fs_emq_periodic_mixed_dev(fs_ctx* ctx, double* d_es, const double* d_targets
         FS_HIP_CHECK(hipMemcpyAsync(x, d_sources, sizeof(double) * ns, hipMemcpyDeviceToDevice, st));
     FS_HIP_CHECK(hipMemcpyAsync(x + ns, d_targets, sizeof(double) * nt, hipMemcpyDeviceToDevice, st));
     FS_TRY(fb.plan(segs, n, fs::kPrecisionMixed, ctx->num_cus, ctx->tuning, st));
+    FS_TRY(fs::shape(ctx, fb, nt, ns, false, x, d_q_ws, e, st));
     FS_TRY(fb.run(x, d_q_ws, e, st));
     FS_HIP_CHECK(hipMemcpyAsync(d_es, e + ns, sizeof(double) * nt, hipMemcpyDeviceToDevice, st));
+    return FS_OK;
+}
+
+int fs_emq_mixed_shape(fs_ctx* ctx, int* targets_per_thread, int* wgs_per_cu) {
+    if (!ctx) return fs::fail(FS_E_INVALID, "null context");
+    fs::FieldBatch& fb = fs::api_batch(ctx);
+    if (targets_per_thread) *targets_per_thread = fb.tpt();
+    if (wgs_per_cu) *wgs_per_cu = fb.wgs_per_cu();
     return FS_OK;
 }
 

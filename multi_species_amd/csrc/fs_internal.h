@@ -6,6 +6,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
+#include <utility>
 
 #include "../../include/farrsight_hip.h"
 
@@ -76,4 +78,7 @@ struct fs_ctx {
     fs::DevBuf aux;       // concatenated points of the mixed entry points
     fs::FieldTuning tuning;
     fs::FieldBatch* api_batch = nullptr;  // plan of the last mixed entry-point call
+    // autotuned launch shapes (tpt, workgroups per CU) of the mixed entry
+    // points, by problem class
+    std::map<uint64_t, std::pair<int, int>> shape_cache;
 };

@@ -1,10 +1,10 @@
-// Stable LSD radix sort of points by (segment, top 32 bits of a 64-bit key).
+// Stable LSD radix sort of points by (segment, top 24 bits of a 64-bit key).
 //
 // The mixed-precision field sorts positions (64-bit fixed point) within each
-// deck of a batch. Ordering by the top 32 bits is enough for its uses (range
-// tests widen by one unit of 2^32; the prefix-sum cut points compare full
-// keys within a run of equal top words), and every pass is stable, so equal
-// keys keep index order and the result is deterministic. A pass is three
+// deck of a batch. Ordering by the top 24 bits (2^-24 of the period) is
+// enough for its uses (range tests widen by one such unit; the prefix-sum cut
+// points compare full keys within a run of equal top bits), and every pass is
+// stable, so equal keys keep index order and the result is deterministic. A pass is three
 // kernels: per-block digit histograms, one exclusive scan over
 // (digit, block), and a stable scatter (wavefront ballots rank equal digits,
 // per-wave counts in LDS order the waves). Nothing here uses memset or host
@@ -112,7 +112,8 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(
     }
 }
 
-int passes_for(int nseg) { return 4 + (nseg > 1 ? 1 : 0) + (nseg > 256 ? 1 : 0); }
+constexpr int kKeyPasses = 3;  // key bits 40..63
+int passes_for(int nseg) { return kKeyPasses + (nseg > 1 ? 1 : 0) + (nseg > 256 ? 1 : 0); }
 
 }  // namespace
 
@@ -143,11 +144,11 @@ int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, 
         uint64_t* kdst = to_out ? kout : kb;
         int32_t* idst = to_out ? iout : ib;
         Digit dg{0, nullptr, 0};
-        if (q < 4) {
-            dg.shift = 32 + 8 * q;
+        if (q < kKeyPasses) {
+            dg.shift = 64 - 8 * (kKeyPasses - q);
         } else {
             dg.seg_of = seg_of;
-            dg.seg_shift = 8 * (q - 4);
+            dg.seg_shift = 8 * (q - kKeyPasses);
         }
         radix_hist<<<nb, kBlock, 0, stream>>>(ksrc, isrc, n, dg, nb, hist);
         FS_HIP_CHECK(hipGetLastError());

@@ -270,6 +270,8 @@ struct fs_sim {
     int remesh_period = 1;
     int precision = FS_PRECISION_F64;
     bool use_graph = true;
+    bool tune_pending = true;  // autotune the field launch shape at the next layout
+    int tuned_tpt = 2, tuned_wgs = 8;
     bool any_adaptive = false, any_treecode = false;
     // concatenated state: xs, ps, qw, es, fs; RK4 work: xk, p2, p3, p4, e2, e3, e4
     DevBuf state, rk, red, status, jobs, pinfo;
@@ -629,6 +631,16 @@ int layout(Sim& S, bool keep_resident) {
     }
     FS_TRY(S.ffull.plan(full, N, S.precision, S.ctx->num_cus, S.ctx->tuning, st));
     FS_TRY(S.fred.plan(redsg, S.U, S.precision, S.ctx->num_cus, S.ctx->tuning, st));
+    FS_TRY(S.fred.set_shape(1, 8, st));
+    // launch shape of the full field: timed on this batch when its membership
+    // or precision changed, reused when an adaptive remesh only resized it
+    FS_HIP_CHECK(hipStreamSynchronize(st));
+    if (S.tune_pending && N > 0 && S.ctx->tuning.targets_per_thread == 0) {
+        FS_TRY(S.ffull.autotune(S.xs, S.qw, S.e2, st, &S.tuned_tpt, &S.tuned_wgs));
+        S.tune_pending = false;
+    } else if (S.ctx->tuning.targets_per_thread == 0) {
+        FS_TRY(S.ffull.set_shape(S.tuned_tpt, S.tuned_wgs, st));
+    }
     // the host copies above must outlive the async copies
     FS_HIP_CHECK(hipStreamSynchronize(st));
     return FS_OK;
@@ -879,6 +891,7 @@ int fs_sim_add_deck(fs_sim* S, const fs_deck* deck, int* index) {
     FS_TRY(sim::build_deck(deck->sim, deck->species, deck->n_species, D));
     S->decks.push_back(std::move(D));
     sim::renumber(*S);
+    S->tune_pending = true;
     // resident decks keep their state; the field of the reduced points is
     // recomputed for all (bitwise the same for the resident ones)
     FS_TRY(sim::layout(*S, true));
@@ -896,6 +909,7 @@ int fs_sim_remove_deck(fs_sim* S, int deck) {
     for (auto& s : S->decks[deck]->sp) s->release();
     S->decks.erase(S->decks.begin() + deck);
     sim::renumber(*S);
+    S->tune_pending = true;
     return sim::layout(*S, true);
 }
 
@@ -927,6 +941,7 @@ int fs_sim_set_options(fs_sim* S, int precision, int use_graph) {
     S->use_graph = use_graph != 0;
     if (precision != S->precision) {
         S->precision = precision;
+        S->tune_pending = true;
         FS_TRY(sim::layout(*S, true));
     } else {
         sim::drop_graphs(*S);
