@@ -79,12 +79,21 @@ int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits);
  * the regularization remainder in FP32 away from z = 0 and in FP64 within
  * B = max(2^-6, 16 d, 4000 d^2) of it (d = eps/L; every pair in FP64 when
  * B >= 1/4), with compensated FP64 accumulation per target.
- * Relative error max_i|E_i - E_i^FP64| / max_i|E_i^FP64| below 1e-8 on
- * the tested decks. Requires epsilon > 0. When d_targets == d_sources and
- * nt == ns the sort is shared. Host pointers, synchronous. */
+ * Relative error max_i|E_i - E_i^FP64| / max_i|E_i^FP64|: 1e-10 to 1e-8 at
+ * that band on the tested decks; with fs_emq_set_mixed_target the band is
+ * widened until the FP32 error bound holds the given target. Requires
+ * epsilon > 0. When targets == sources and nt == ns the sort is shared.
+ * Host pointers, synchronous. */
 int fs_emq_periodic_mixed(fs_ctx* ctx, double* es, const double* targets, int64_t nt,
                           const double* sources, const double* q_ws, int64_t ns,
                           double L, double epsilon);
+
+/* Error target of the mixed entry points on this context (no counterpart in
+ * the reference). 0 (the default): the minimum band B above, one pass. > 0:
+ * a first pass gives max|E|; when the bound 2 u W d^2 / B on the FP32 terms'
+ * error (u = 2^-24, W = sum |q_w|) exceeds rel_err max|E|, the pair sums are
+ * redone with B widened to meet it. At most 1e-3. */
+int fs_emq_set_mixed_target(fs_ctx* ctx, double rel_err);
 
 /* Launch shape (targets per thread, persistent workgroups per CU) that the
  * last mixed call used, as chosen by its autotuner. Informational. */
@@ -227,6 +236,58 @@ int fs_sim_deck_info(fs_sim* sim, int deck, int* first_species, int* n_species, 
 /* precision as above; use_graph 0 runs every step eagerly (adaptive and
  * treecode decks always do). */
 int fs_sim_set_options(fs_sim* sim, int precision, int use_graph);
+
+/* ---- Serving loop: continuous batching (no counterpart in the reference) --
+ * A server owns a context's device and an initially empty batch. Requests
+ * (a deck and a step count) may be submitted from any thread at any time;
+ * a builder thread makes the deck's meshes on the host, and a worker thread
+ * admits built decks between steps in arrival order while a slot is free and
+ * the estimated device footprint of the resident decks stays within
+ * hbm_budget_bytes, steps all resident decks together, and retires finished
+ * ones with their final state copied to the host. While a server runs, its
+ * context must not be used by other calls. Decks must remesh every step. */
+enum {
+    FS_REQ_QUEUED = 0,
+    FS_REQ_BUILDING = 1,
+    FS_REQ_READY = 2,    /* built, waiting for a slot or device memory */
+    FS_REQ_ACTIVE = 3,   /* resident, stepping */
+    FS_REQ_DONE = 4,
+    FS_REQ_FAILED = -1
+};
+
+typedef struct fs_server_params {
+    double hbm_budget_bytes;
+    int max_active;      /* resident decks at most (0 = no limit) */
+    int precision;       /* FS_PRECISION_F64 or FS_PRECISION_MIXED */
+    int use_graph;
+} fs_server_params;
+
+typedef struct fs_server_stats {
+    int64_t submitted, completed, failed, active, queued;
+    int64_t steps;           /* batch steps taken */
+    int64_t point_steps;     /* sum over steps of the resident points */
+    double busy_s;           /* time spent in steps */
+    double span_s;           /* first admission to last retirement */
+    double point_steps_per_s;  /* point_steps / span_s */
+    double step_p50_ms, step_p99_ms;  /* per-step latency of the resident decks */
+    int64_t hbm_in_use;      /* estimated bytes of the resident decks */
+} fs_server_stats;
+
+typedef struct fs_server fs_server;
+
+int fs_server_create(fs_ctx* ctx, const fs_server_params* params, fs_server** out);
+int fs_server_submit(fs_server* server, const fs_deck* deck, int n_steps, int64_t* request_id);
+int fs_server_status(fs_server* server, int64_t request_id, int* state, int* steps_done);
+/* Blocks until the request is done or failed (timeout_s <= 0: no limit). */
+int fs_server_wait(fs_server* server, int64_t request_id, double timeout_s);
+/* Final state of one species of a finished request (any output may be NULL;
+ * n_points first). diag as fs_sim_diagnostics. */
+int fs_server_result(fs_server* server, int64_t request_id, int species, int64_t* n_points,
+                     double* xs, double* ps, double* fs, double* q_ws, double* es,
+                     double diag[4]);
+int fs_server_get_stats(fs_server* server, fs_server_stats* out);
+/* Stops the threads; requests not finished are dropped. */
+int fs_server_destroy(fs_server* server);
 
 #ifdef __cplusplus
 }

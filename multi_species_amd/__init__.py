@@ -49,6 +49,7 @@ def lib():
         L.fs_emq_set_tuning.argtypes = [_vp, _int, _int]
         L.fs_emq_periodic_mixed.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
         L.fs_emq_mixed_shape.argtypes = [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int)]
+        L.fs_emq_set_mixed_target.argtypes = [_vp, _d]
         L.fs_emq_periodic_mixed_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d,
                                                 _vp]
         L.fs_emq_treecode_f64.argtypes = [_vp, _dp, _dp, _dp, _i64, _d, _d, _d, _int, _int,
@@ -77,6 +78,14 @@ def lib():
         L.fs_sim_deck_info.argtypes = [_vp, _int, ctypes.POINTER(_int), ctypes.POINTER(_int),
                                        ctypes.POINTER(_int), ctypes.POINTER(_i64)]
         L.fs_sim_set_options.argtypes = [_vp, _int, _int]
+        L.fs_server_create.argtypes = [_vp, ctypes.POINTER(ServerParams), ctypes.POINTER(_vp)]
+        L.fs_server_submit.argtypes = [_vp, ctypes.POINTER(Deck), _int, ctypes.POINTER(_i64)]
+        L.fs_server_status.argtypes = [_vp, _i64, ctypes.POINTER(_int), ctypes.POINTER(_int)]
+        L.fs_server_wait.argtypes = [_vp, _i64, _d]
+        L.fs_server_result.argtypes = [_vp, _i64, _int, ctypes.POINTER(_i64), _dp, _dp, _dp,
+                                       _dp, _dp, _dp]
+        L.fs_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
+        L.fs_server_destroy.argtypes = [_vp]
         _lib = L
     return _lib
 
@@ -153,6 +162,12 @@ class Context:
                                            s.size, L, eps), "fs_emq_periodic_mixed")
         return es
 
+    def set_mixed_target(self, rel_err: float):
+        """Error target of the mixed calls on this context: 0 = minimum FP64
+        band, one pass; > 0 = the band is widened after a first pass until the
+        FP32 error bound holds rel_err * max|E|."""
+        _check(lib().fs_emq_set_mixed_target(self.h, rel_err), "fs_emq_set_mixed_target")
+
     def mixed_shape(self):
         """(targets per thread, workgroups per CU) of the last mixed call."""
         t, g = _int(), _int()
@@ -218,6 +233,22 @@ class Deck(ctypes.Structure):
 
 PRECISION_F64 = 0
 PRECISION_MIXED = 1
+
+REQ_QUEUED, REQ_BUILDING, REQ_READY, REQ_ACTIVE, REQ_DONE, REQ_FAILED = 0, 1, 2, 3, 4, -1
+
+
+class ServerParams(ctypes.Structure):
+    """fs_server_params."""
+    _fields_ = [("hbm_budget_bytes", _d), ("max_active", _int), ("precision", _int),
+                ("use_graph", _int)]
+
+
+class ServerStats(ctypes.Structure):
+    """fs_server_stats."""
+    _fields_ = [("submitted", _i64), ("completed", _i64), ("failed", _i64), ("active", _i64),
+                ("queued", _i64), ("steps", _i64), ("point_steps", _i64), ("busy_s", _d),
+                ("span_s", _d), ("point_steps_per_s", _d), ("step_p50_ms", _d),
+                ("step_p99_ms", _d), ("hbm_in_use", _i64)]
 
 
 def _deck(sim: SimParams, species):
@@ -434,3 +465,65 @@ def declared_symbols(header: str = HEADER):
     text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(fs_[a-z0-9_]+)\s*\(", text)))
+
+
+class Server:
+    """Serving loop with continuous batching (fs_server). The context must not
+    be used by other calls while the server runs."""
+
+    def __init__(self, ctx: "Context", hbm_budget_bytes: float = 200e9, max_active: int = 0,
+                 precision: int = PRECISION_MIXED, use_graph: bool = True):
+        self.ctx = ctx
+        p = ServerParams(float(hbm_budget_bytes), int(max_active), int(precision),
+                         int(bool(use_graph)))
+        h = _vp()
+        _check(lib().fs_server_create(ctx.h, ctypes.byref(p), ctypes.byref(h)),
+               "fs_server_create")
+        self.h = h
+        self.n_species = {}
+
+    def submit(self, sim: "SimParams", species, n_steps: int) -> int:
+        d, arr = _deck(sim, species)
+        rid = _i64()
+        _check(lib().fs_server_submit(self.h, ctypes.byref(d), int(n_steps), ctypes.byref(rid)),
+               "fs_server_submit")
+        self.n_species[rid.value] = len(species)
+        return rid.value
+
+    def status(self, rid: int):
+        st, done = _int(), _int()
+        _check(lib().fs_server_status(self.h, rid, ctypes.byref(st), ctypes.byref(done)),
+               "fs_server_status")
+        return st.value, done.value
+
+    def wait(self, rid: int, timeout_s: float = 0.0):
+        _check(lib().fs_server_wait(self.h, rid, float(timeout_s)), "fs_server_wait")
+
+    def result(self, rid: int, k: int):
+        """(xs, ps, fs, q_ws, es, diagnostics) of species k of a finished request."""
+        n = _i64()
+        _check(lib().fs_server_result(self.h, rid, k, ctypes.byref(n), None, None, None, None,
+                                      None, None), "fs_server_result")
+        arrs = [np.empty(n.value) for _ in range(5)]
+        diag = np.empty(4)
+        _check(lib().fs_server_result(self.h, rid, k, None, *[_P(a) for a in arrs], _P(diag)),
+               "fs_server_result")
+        return (*arrs, diag)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib().fs_server_get_stats(self.h, ctypes.byref(st)), "fs_server_get_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fs_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -10,6 +10,10 @@ namespace fs {
 
 enum { kPrecisionF64 = 0, kPrecisionMixed = 1 };
 
+// Governor state per segment: max |E| and the error at each of 8 candidate
+// FP64 bands (mixed precision).
+constexpr int kGovVals = 9;
+
 // One segment of the concatenated point arrays. A deck is one segment that
 // is its own source; the single-call API with distinct targets uses two
 // segments (the sources, not evaluated, and the targets, sourced from the
@@ -36,7 +40,8 @@ struct SegDev {
     int pw_base;
     int src;          // index of the source segment
     int is_src;       // some segment takes its sources from this one
-    int pad;
+    int all_band0;    // all_band of the minimum band (no governor)
+    uint64_t band0;   // the minimum band B0 * 2^64
 };
 
 // Far-field record of one sorted source (FP32 path of the mixed kernel).
@@ -67,6 +72,25 @@ class FieldBatch {
 public:
     int plan(const std::vector<BatchSeg>& segs, int n_total, int precision, int num_cus,
              const FieldTuning& tun, hipStream_t stream);
+    // Accuracy governor (mixed precision; off after plan()). The state of a
+    // segment is kGovVals doubles (bit patterns): max |E| of a run and the E
+    // error that each candidate FP64 band would have given, measured on
+    // probed targets. Before each run the narrowest band whose error was
+    // within target * max|E| is taken from src (every pair in FP64 while a
+    // segment has no state); reset clears src after reading it. With record
+    // set, each run writes its own state to gov(). target <= 0: 4e-9.
+    void govern(const uint64_t* src, bool reset, bool record, double target = 0);
+    uint64_t* gov() const;
+    // A run whose result meets the governor's target without a previous
+    // evaluation: a recorded run at the minimum band, then, when its measured
+    // error exceeds the target, the pair sums again with the band the
+    // governor picks. Synchronizes the stream.
+    int run_checked(const double* x, const double* w, double* es, double target,
+                    hipStream_t stream);
+    // Governor state of every segment (nseg * kGovVals), for carrying it
+    // across re-plans.
+    int get_gov(std::vector<uint64_t>& out, hipStream_t stream) const;
+    int set_gov(const std::vector<uint64_t>& v, hipStream_t stream);
     // Launch shape of the pair kernel: targets per thread (1, 2, 4) and
     // persistent workgroups per CU. Does not change any result.
     int set_shape(int tpt, int wgs_per_cu, hipStream_t stream);
@@ -90,6 +114,9 @@ private:
     int launch_pairs(int nit, int grid, hipStream_t stream);
 
     int precision_ = kPrecisionF64;
+    const uint64_t* gov_src_ = nullptr;
+    bool gov_reset_ = false, gov_record_ = false;
+    double gov_target_ = 4e-9;
     int n_total_ = 0, nseg_ = 0, tpt_ = 2, grid_ = 0, wgs_per_cu_ = 8, num_cus_ = 256;
     const double* x_ = nullptr;  // inputs of the current run (FP64 pair kernel)
     const double* w_ = nullptr;
@@ -106,7 +133,8 @@ private:
     int* d_counter_ = nullptr;
     // work space offsets (bytes into work_)
     size_t o_part_ = 0, o_key_ = 0, o_skey_ = 0, o_perm_ = 0, o_sort_ = 0, o_as_ = 0,
-           o_asm1_ = 0, o_ws_ = 0, o_far_ = 0, o_pw_ = 0, o_bsum_ = 0, o_bsa_ = 0, o_tot_ = 0;
+           o_asm1_ = 0, o_ws_ = 0, o_far_ = 0, o_pw_ = 0, o_bsum_ = 0, o_bsa_ = 0, o_tot_ = 0,
+           o_gov_ = 0;
 };
 
 // Stable radix sort by (segment of the original index, top 32 key bits)

@@ -26,6 +26,7 @@
 // the caller can capture it in a HIP graph.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 #include "fs_device.h"
@@ -46,6 +47,10 @@ constexpr uint64_t kHalf = 0x8000000000000000ull;
 constexpr uint64_t kAntiMargin = uint64_t(1) << 40;
 constexpr int kSortShift = 40;  // keys are sorted by bits 40..63 (fs_radix.hip)
 constexpr uint64_t kLowSort = (uint64_t(1) << kSortShift) - 1;
+constexpr int kLadder = kGovVals - 1;  // governor: candidate bands
+constexpr int kLadderB0 = 2;           // governor: the candidate equal to the minimum band
+constexpr int kProbeMax = 256;         // governor: probed targets per segment
+constexpr double kGovTarget = 4e-9;    // governor: default error target (relative to max |E|)
 
 __device__ __forceinline__ int pull_item(int* counter) {
     __shared__ int s_item;
@@ -227,6 +232,7 @@ __global__ __launch_bounds__(kBlock) void blocksum_kernel(const ScanBlock* __res
         bsa[blockIdx.x] = swa[0];
     }
 }
+
 
 // Exclusive prefix sums of each source segment's sorted weights into
 // pw[pw_base + 0 .. n], and totals[seg] = sum w a. Fixed order throughout.
@@ -435,6 +441,133 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     }
 }
 
+// ---- accuracy governor ----------------------------------------------------
+// The FP32 far-field error of a segment is measured, not bounded: a probe
+// takes up to 256 evenly spaced targets of the segment and, for each of the
+// ladder bands B_q = B0 2^((q-2)/2) (q = 0..7, at least 2^-6), sums over the
+// source tiles that are far from the target at B_q the difference between
+// the tile's FP32 sum, computed exactly as the pair kernel computes it, and
+// the exact FP64 regularization remainder of the same pairs. Per segment the
+// state holds max |E| of the run (finalize) and, per ladder band, the max
+// over the probed targets of |that difference| / 2 (the E error the band
+// would have given). The next run takes the narrowest band whose error was
+// within target * max|E|, or every pair in FP64 when none was (or when no
+// state exists yet). A tile that is far for the pair kernel's whole target
+// block is far for each of its targets, so the probe counts every FP32 tile
+// of the kernel (and a few more).
+
+__device__ __forceinline__ uint64_t ladder_band(const SegDev& S, int q) {
+    double f = __builtin_ldexp(1.0, (q >> 1) - 1);  // 2^((q - 2)/2)
+    if (q & 1) f *= 1.4142135623730951;
+    double b = __builtin_ldexp((double)S.band0, -64) * f;
+    b = b > 0.015625 ? b : 0.015625;
+    return b >= 0.25 ? 0 : (uint64_t)__builtin_ldexp(b, 64);  // 0: all pairs FP64
+}
+
+__device__ __forceinline__ int probe_samples(int n) {
+    const int s = n / 64;
+    return min(n, min(kProbeMax, max(16, s)));
+}
+
+// grid (kProbeMax, nseg): one workgroup per probed target.
+__global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict__ segs,
+                                                       const uint64_t* __restrict__ us,
+                                                       const double* __restrict__ ws,
+                                                       const FarSrc* __restrict__ far,
+                                                       uint64_t* __restrict__ gov) {
+    __shared__ double red[kLadder][kBlock];
+    const int d = blockIdx.y;
+    const SegDev S = segs[d];
+    if (!S.eval || S.all_band0 || S.n == 0 || S.src_n == 0) return;
+    const int ns = probe_samples(S.n);
+    if ((int)blockIdx.x >= ns) return;
+    const int k = S.off + (int)(((int64_t)(2 * blockIdx.x + 1) * S.n) / (2 * ns));
+    const uint64_t ut = us[k];
+    const uint32_t ht = (uint32_t)(ut >> 32);
+    const uint64_t uf = ut & ~kLowSort, ul = ut | kLowSort;
+    const uint64_t tspan = ul - uf;
+    uint64_t band[kLadder];
+#pragma unroll
+    for (int q = 0; q < kLadder; ++q) band[q] = ladder_band(S, q);
+    double err[kLadder];
+#pragma unroll
+    for (int q = 0; q < kLadder; ++q) err[q] = 0.0;
+    const SegDev R = segs[S.src];
+    const double d2 = S.delta2;
+    const int ntile = (R.n + kTile - 1) / kTile;
+    for (int m = threadIdx.x; m < ntile; m += kBlock) {
+        const int jb = R.off + m * kTile, je = min(jb + kTile, R.off + R.n);
+        const uint64_t vf = us[jb] & ~kLowSort, vl = us[je - 1] | kLowSort;
+        const uint64_t vspan = vl - vf;
+        if (tspan >= kHalf || vspan >= kHalf || tspan + vspan >= kHalf) continue;
+        const uint64_t len = tspan + vspan;
+        const uint64_t lo = uf - vl;
+        if (cyc_meets(lo - kHalf + kAntiMargin, len, 2 * kAntiMargin)) continue;
+        int kf = 0;
+        while (kf < kLadder && band[kf] != 0 && !cyc_meets(lo + band[kf], len, 2 * band[kf])) ++kf;
+        if (kf == 0) continue;
+        float acc = 0.0f;
+        double ex = 0.0;
+        for (int j = jb; j < je; ++j) {
+            const FarSrc f = far[j];
+            const float zs = (float)(int32_t)(ht - f.h);
+            const float r = __builtin_amdgcn_rcpf(zs);
+            const float sq = r * __builtin_fabsf(r);
+            const float qq = __builtin_fabsf(sq);
+            const float p = __builtin_fmaf(qq, __builtin_fmaf(qq, f.wb2, f.wb1), f.w);
+            acc = __builtin_fmaf(sq, p, acc);
+            // z/sqrt(z^2 + d^2) - s(z) = -d^2 s / (r (r + |z|)), r = sqrt(z^2 + d^2)
+            const double z = __builtin_ldexp((double)(int64_t)(ut - us[j]), -64);
+            const double rz = __builtin_sqrt(__builtin_fma(z, z, d2));
+            ex += ws[j] * (-d2 * __builtin_copysign(1.0, z) / (rz * (rz + __builtin_fabs(z))));
+        }
+        const double e = 0.5 * (__builtin_ldexp((double)acc, 64) - ex);
+        for (int q = 0; q < kf; ++q) err[q] += e;
+    }
+#pragma unroll
+    for (int q = 0; q < kLadder; ++q) red[q][threadIdx.x] = err[q];
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int q = 0; q < kLadder; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < kLadder) {
+        const double e = __builtin_fabs(red[threadIdx.x][0]);
+        atomicMax((unsigned long long*)&gov[(int64_t)d * kGovVals + 1 + threadIdx.x],
+                  (unsigned long long)__double_as_longlong(e));
+    }
+}
+
+// Band of every evaluated segment from the governor state src (see above);
+// reset clears the state after reading it.
+__global__ void governor_kernel(int nseg, SegDev* __restrict__ segs, uint64_t* __restrict__ src,
+                                int reset, double target) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nseg) return;
+    SegDev& S = segs[d];
+    uint64_t* g = src + (int64_t)d * kGovVals;
+    const double em = __longlong_as_double((long long)g[0]);
+    double e[kLadder];
+    for (int q = 0; q < kLadder; ++q) e[q] = __longlong_as_double((long long)g[1 + q]);
+    if (reset)
+        for (int q = 0; q < kGovVals; ++q) g[q] = 0;
+    if (!S.eval || S.all_band0) return;
+    uint64_t band = 0;
+    if (em > 0.0)
+        for (int q = 0; q < kLadder; ++q) {
+            const uint64_t b = ladder_band(S, q);
+            if (b == 0) break;
+            if (e[q] <= target * em) {
+                band = b;
+                break;
+            }
+        }
+    S.all_band = band == 0 ? 1 : 0;
+    S.band = band == 0 ? (uint64_t(1) << 62) : band;
+}
+
 // Sum of the sorted weights of [off, off+n) whose key is <= key. The keys
 // are sorted by their top bits; within a run of equal top bits the full keys
 // are compared one by one.
@@ -455,27 +588,21 @@ __device__ double w_le(const uint64_t* __restrict__ us, const double* __restrict
     return s;
 }
 
-// E = (S + G)/2 at every sorted position of the evaluated segments,
-// scattered to the original index. With P(<= u) the weight of the sources at
-// or left of u, W the total and A = sum w a:
-//   sum_j w_j s_ij = 2 (P(<= u_i) - P(<= u_i - 1/2) - P(<= u_i + 1/2)) + W
-//   sum_j w_j z_ij = a_i W - A - P(<= u_i - 1/2) + (W - P(<= u_i + 1/2)),
-// where P(<= u_i - 1/2) = 0 for u_i < 1/2 and P(<= u_i + 1/2) = W for u_i >= 1/2.
-__global__ __launch_bounds__(kBlock) void finalize_mixed_kernel(
-    int n_total, const int32_t* __restrict__ seg_of, const SegDev* __restrict__ segs,
-    const uint64_t* __restrict__ us, const int32_t* __restrict__ perm,
-    const double* __restrict__ ws, const double* __restrict__ pw,
-    const double* __restrict__ totals, const double* __restrict__ part,
-    double* __restrict__ es, int* __restrict__ counter) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k == 0) *counter = 0;
-    if (k >= n_total) return;
-    const int d = seg_of[k];
+// E at sorted position k (segment d), written to es at the original index;
+// returns it (0 for points of segments that are not evaluated).
+__device__ __forceinline__ double field_at(int k, int d, const SegDev* __restrict__ segs,
+                                           const uint64_t* __restrict__ us,
+                                           const int32_t* __restrict__ perm,
+                                           const double* __restrict__ ws,
+                                           const double* __restrict__ pw,
+                                           const double* __restrict__ totals,
+                                           const double* __restrict__ part,
+                                           double* __restrict__ es) {
     const SegDev& S = segs[d];
-    if (!S.eval) return;
+    if (!S.eval) return 0.0;
     if (S.src_n == 0) {
         es[perm[k]] = 0.0;
-        return;
+        return 0.0;
     }
     double g = 0.0, gl = 0.0;
     for (int s = 0; s < S.splits; ++s)
@@ -489,10 +616,51 @@ __global__ __launch_bounds__(kBlock) void finalize_mixed_kernel(
     const double sum_s = 2.0 * ((P0 - Pm) - Pp) + W;
     const double ai = __builtin_ldexp((double)u, -64);
     const double sum_z = ((ai * W - A) - Pm) + (W - Pp);
-    es[perm[k]] = 0.5 * ((sum_s - S.C * sum_z) + (g + gl));
+    const double e = 0.5 * ((sum_s - S.C * sum_z) + (g + gl));
+    es[perm[k]] = e;
+    return e;
+}
+
+// E = (S + G)/2 at every sorted position of the evaluated segments,
+// scattered to the original index. With P(<= u) the weight of the sources at
+// or left of u, W the total and A = sum w a:
+//   sum_j w_j s_ij = 2 (P(<= u_i) - P(<= u_i - 1/2) - P(<= u_i + 1/2)) + W
+//   sum_j w_j z_ij = a_i W - A - P(<= u_i - 1/2) + (W - P(<= u_i + 1/2)),
+// where P(<= u_i - 1/2) = 0 for u_i < 1/2 and P(<= u_i + 1/2) = W for u_i >= 1/2.
+__global__ __launch_bounds__(kBlock) void finalize_mixed_kernel(
+    int n_total, const int32_t* __restrict__ seg_of, const SegDev* __restrict__ segs,
+    const uint64_t* __restrict__ us, const int32_t* __restrict__ perm,
+    const double* __restrict__ ws, const double* __restrict__ pw,
+    const double* __restrict__ totals, const double* __restrict__ part,
+    double* __restrict__ es, int* __restrict__ counter, uint64_t* __restrict__ emax) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k == 0) *counter = 0;
+    const int d = seg_of[min(k, n_total - 1)];
+    const double e = (k < n_total) ? field_at(k, d, segs, us, perm, ws, pw, totals, part, es) : 0.0;
+    if (emax) {
+        // max |E| per segment (among non-negative doubles the bit patterns
+        // order as the values): lanes of the wave's first segment reduce in
+        // registers, one atomic per wave; lanes of other segments add theirs
+        const uint64_t eb = (uint64_t)__double_as_longlong(__builtin_fabs(e));
+        const int d0 = __shfl(d, 0);
+        uint64_t r = (d == d0) ? eb : 0;
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint64_t v = __shfl_xor(r, o);
+            r = v > r ? v : r;
+        }
+        if ((threadIdx.x & 63) == 0 && r)
+            atomicMax((unsigned long long*)&emax[(int64_t)d0 * kGovVals], (unsigned long long)r);
+        if (d != d0 && eb)
+            atomicMax((unsigned long long*)&emax[(int64_t)d * kGovVals], (unsigned long long)eb);
+    }
 }
 
 __global__ void zero_int_kernel(int* p) { *p = 0; }
+
+__global__ void zero_u64_kernel(uint64_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
 
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
@@ -536,6 +704,8 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         return fail(FS_E_INVALID, "field batch: unknown precision %d", precision);
     if (segs.empty() || n_total < 0) return fail(FS_E_INVALID, "field batch: no segments");
     precision_ = precision;
+    gov_src_ = nullptr;
+    gov_reset_ = gov_record_ = false;
     n_total_ = n_total;
     nseg_ = (int)segs.size();
     tpt_ = (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
@@ -582,6 +752,8 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
             const double B = std::max({std::ldexp(1.0, -6), 16.0 * delta, 4000.0 * d2});
             S.all_band = (B >= 0.25) ? 1 : 0;
             S.band = S.all_band ? (uint64_t(1) << 62) : (uint64_t)std::ldexp(B, 64);
+            S.all_band0 = S.all_band;
+            S.band0 = S.band;
             S.d2s = d2 * two64 * two64;
             const double kq2 = d2 * two64;
             S.b1z = S.all_band ? 0.0 : -0.75 * kq2;
@@ -645,8 +817,13 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         o_bsum_ = take(8 * std::max<size_t>(scan_.size(), 1));
         o_bsa_ = take(8 * std::max<size_t>(scan_.size(), 1));
         o_tot_ = take(8 * (size_t)nseg_);
+        o_gov_ = take(8 * (size_t)nseg_ * kGovVals);
     }
     FS_TRY(work_.ensure(o, stream));
+    if (precision == kPrecisionMixed) {
+        zero_u64_kernel<<<nblocks(nseg_ * kGovVals), kBlock, 0, stream>>>(gov(), nseg_ * kGovVals);
+        FS_HIP_CHECK(hipGetLastError());
+    }
     num_cus_ = num_cus;
     // the host tables must outlive the copies
     FS_HIP_CHECK(hipStreamSynchronize(stream));
@@ -723,11 +900,20 @@ int FieldBatch::launch_pairs(int nit, int grid, hipStream_t stream) {
 
 // Times the pair kernel for every (tpt, workgroups per CU) on this batch's
 // data and keeps the fastest. A trial runs all the launch's items with their
-// source ranges shortened to about 2e10 pairs in total. The shape changes neither the items' summation order
-// nor any result. es receives a full evaluation (it is scratch here).
+// source ranges shortened to about 2e10 pairs in total. The shape changes
+// neither the items' summation order nor any result. es receives a full
+// evaluation (it is scratch here); the governor state is left as it was.
 int FieldBatch::autotune(const double* x, const double* w, double* es, hipStream_t stream,
                          int* best_tpt, int* best_wgs) {
-    FS_TRY(run(x, w, es, stream));  // sorted arrays and plan tables in place
+    const uint64_t* gsrc = gov_src_;
+    const bool greset = gov_reset_, grecord = gov_record_;
+    gov_src_ = nullptr;
+    gov_reset_ = gov_record_ = false;
+    const int rc0 = run(x, w, es, stream);  // sorted arrays and plan tables in place
+    gov_src_ = gsrc;
+    gov_reset_ = greset;
+    gov_record_ = grecord;
+    FS_TRY(rc0);
     static const int tpts[3] = {1, 2, 4};
     static const int wgss[3] = {2, 4, 8};
     hipEvent_t e0, e1;
@@ -789,6 +975,80 @@ int FieldBatch::autotune(const double* x, const double* w, double* es, hipStream
     return FS_OK;
 }
 
+void FieldBatch::govern(const uint64_t* src, bool reset, bool record, double target) {
+    const bool on = precision_ == kPrecisionMixed;
+    gov_src_ = on ? src : nullptr;
+    gov_reset_ = on && reset;
+    gov_record_ = on && record;
+    gov_target_ = target > 0 ? target : kGovTarget;
+}
+
+// A recorded run at the minimum band; when its measured error exceeds the
+// target, the governor's band and the pair sums and finalize again (the
+// sorted arrays and prefix sums are those of the first run).
+int FieldBatch::run_checked(const double* x, const double* w, double* es, double target,
+                            hipStream_t stream) {
+    if (n_total_ == 0) return FS_OK;
+    if (precision_ != kPrecisionMixed) return run(x, w, es, stream);
+    auto body = [&]() -> int {
+        govern(nullptr, false, true, target);
+        zero_u64_kernel<<<nblocks(nseg_ * kGovVals), kBlock, 0, stream>>>(gov(), nseg_ * kGovVals);
+        FS_HIP_CHECK(hipGetLastError());
+        FS_TRY(run(x, w, es, stream));
+        std::vector<uint64_t> st;
+        FS_TRY(get_gov(st, stream));
+        bool again = false;
+        for (int d = 0; d < nseg_; ++d) {
+            if (!segs_[d].eval || segs_[d].all_band0 || segs_[d].n == 0) continue;
+            double em, e0;
+            std::memcpy(&em, &st[(size_t)d * kGovVals], 8);
+            std::memcpy(&e0, &st[(size_t)d * kGovVals + 1 + kLadderB0], 8);
+            again = again || !(e0 <= gov_target_ * em);
+        }
+        if (!again) return FS_OK;
+        governor_kernel<<<nblocks(nseg_), kBlock, 0, stream>>>(nseg_, d_segs_, gov(), 1,
+                                                              gov_target_);
+        FS_HIP_CHECK(hipGetLastError());
+        char* wk = static_cast<char*>(work_.ptr);
+        FS_TRY(launch_pairs((int)items_.size(), grid_, stream));
+        finalize_mixed_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
+            n_total_, d_seg_of_, d_segs_, reinterpret_cast<uint64_t*>(wk + o_skey_),
+            reinterpret_cast<int32_t*>(wk + o_perm_), reinterpret_cast<double*>(wk + o_ws_),
+            reinterpret_cast<double*>(wk + o_pw_), reinterpret_cast<double*>(wk + o_tot_),
+            reinterpret_cast<double*>(wk + o_part_), es, d_counter_, nullptr);
+        FS_HIP_CHECK(hipGetLastError());
+        return FS_OK;
+    };
+    const int rc = body();
+    govern(nullptr, false, false);
+    return rc;
+}
+
+uint64_t* FieldBatch::gov() const {
+    if (precision_ != kPrecisionMixed || !work_.ptr) return nullptr;
+    return reinterpret_cast<uint64_t*>(static_cast<char*>(work_.ptr) + o_gov_);
+}
+
+int FieldBatch::get_gov(std::vector<uint64_t>& out, hipStream_t stream) const {
+    out.assign((size_t)nseg_ * kGovVals, 0);
+    if (!gov() || nseg_ == 0) return FS_OK;
+    FS_HIP_CHECK(hipMemcpyAsync(out.data(), gov(), sizeof(uint64_t) * out.size(),
+                                hipMemcpyDeviceToHost, stream));
+    FS_HIP_CHECK(hipStreamSynchronize(stream));
+    return FS_OK;
+}
+
+int FieldBatch::set_gov(const std::vector<uint64_t>& v, hipStream_t stream) {
+    if (!gov() || nseg_ == 0) return FS_OK;
+    if (v.size() != (size_t)nseg_ * kGovVals)
+        return fail(FS_E_INVALID, "field batch: governor state of %zu values for %d segments",
+                    v.size(), nseg_);
+    FS_HIP_CHECK(hipMemcpyAsync(gov(), v.data(), sizeof(uint64_t) * v.size(),
+                                hipMemcpyHostToDevice, stream));
+    FS_HIP_CHECK(hipStreamSynchronize(stream));
+    return FS_OK;
+}
+
 int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t stream) {
     if (n_total_ == 0) return FS_OK;
     char* wk = static_cast<char*>(work_.ptr);
@@ -827,9 +1087,19 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
                                                                    pw, tot);
         FS_HIP_CHECK(hipGetLastError());
     }
+    if (gov_src_) {
+        governor_kernel<<<nblocks(nseg_), kBlock, 0, stream>>>(
+            nseg_, d_segs_, const_cast<uint64_t*>(gov_src_), gov_reset_ ? 1 : 0, gov_target_);
+        FS_HIP_CHECK(hipGetLastError());
+    }
     FS_TRY(launch_pairs(nit, grid_, stream));
+    if (gov_record_) {
+        probe_kernel<<<dim3(kProbeMax, nseg_), kBlock, 0, stream>>>(d_segs_, skey, ws, far, gov());
+        FS_HIP_CHECK(hipGetLastError());
+    }
     finalize_mixed_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
-        n_total_, d_seg_of_, d_segs_, skey, perm, ws, pw, tot, part, es, d_counter_);
+        n_total_, d_seg_of_, d_segs_, skey, perm, ws, pw, tot, part, es, d_counter_,
+        gov_record_ ? gov() : nullptr);
     FS_HIP_CHECK(hipGetLastError());
     return FS_OK;
 }

@@ -78,6 +78,7 @@ struct fs_ctx {
     fs::DevBuf aux;       // concatenated points of the mixed entry points
     fs::FieldTuning tuning;
     fs::FieldBatch* api_batch = nullptr;  // plan of the last mixed entry-point call
+    double mixed_target = 0;              // > 0: mixed entry points hold this error bound
     // autotuned launch shapes (tpt, workgroups per CU) of the mixed entry
     // points, by problem class
     std::map<uint64_t, std::pair<int, int>> shape_cache;

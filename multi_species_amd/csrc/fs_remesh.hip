@@ -44,8 +44,8 @@ namespace remesh {
 // Panel record layout (int32 x kPanelInts): see fs_remesh.h.
 constexpr int kLineThreads = 256;
 constexpr int kLinePerThread = kMaxLine / kLineThreads;
-constexpr int kMaxWalkSteps = 256;
-constexpr int kMaxHistory = 32;
+constexpr int kMaxWalkSteps = 4096;  // a walk's steps (the reference's has no bound)
+constexpr int kMaxHistory = 32;      // visited panels kept in registers
 constexpr int kMaxDescend = 256;
 
 __global__ __launch_bounds__(64) void coef_kernel(const Job* __restrict__ jobs) {
@@ -257,9 +257,14 @@ __device__ bool shift_one(double& xt, double p, const RootCorners& rc, double Lx
 
 // find_leaf_containing_point_from_neighbor (AMRStructure_interp.cpp:12-230)
 // as a loop over precomputed walk records. Returns the leaf (0 when the walk
-// reaches the root sentinel); sets *err on a runaway walk.
+// reaches the root sentinel). The walk stops at the first panel it visits a
+// second time, as the reference's history set makes it: the first
+// kMaxHistory visited panels are kept in registers, later ones in ext (LDS,
+// kMaxWalkSteps entries) when given. err = 1 on a runaway walk; err = 2 when
+// the history outgrew the registers and no ext was given (the caller redoes
+// that walk with one).
 __device__ int walk_one(const WalkRec* __restrict__ W, double& tx, double tp, bool& beyond,
-                        int start, double Lx, bool periodic, int& err) {
+                        int start, double Lx, bool periodic, int& err, int* ext) {
     int hist[kMaxHistory];
     int nh = 1;
     hist[0] = start;
@@ -293,13 +298,21 @@ __device__ int walk_one(const WalkRec* __restrict__ W, double& tx, double tp, bo
         if ((!r && (f & kWalkBoundaryRight)) || (!l && (f & kWalkBoundaryLeft)) ||
             (!t && (f & kWalkBoundaryTop)) || (!b && (f & kWalkBoundaryBottom)))
             beyond = true;
-        for (int k = 0; k < nh; ++k)
+        const int nreg = nh < kMaxHistory ? nh : kMaxHistory;
+        for (int k = 0; k < nreg; ++k)
             if (hist[k] == nxt) return nxt;
-        if (nh == kMaxHistory) {
-            err = 1;
+        if (nh > kMaxHistory)
+            for (int k = 0; k < nh - kMaxHistory; ++k)
+                if (ext[k] == nxt) return nxt;
+        if (nh < kMaxHistory) {
+            hist[nh] = nxt;
+        } else if (ext) {
+            ext[nh - kMaxHistory] = nxt;
+        } else {
+            err = 2;
             return nxt;
         }
-        hist[nh++] = nxt;
+        ++nh;
         cur = nxt;
     }
     err = 1;
@@ -473,7 +486,7 @@ struct LineOut {
 
 __device__ void walk_line(const Job& J, size_t base, size_t stride, int len, int first_start,
                           bool element0_exact_from_start, int* s_guess, int* s_leaf, int* s_first,
-                          LineOut& out, int& err) {
+                          int* s_fail, int* ext, LineOut& out, int& err) {
     const bool periodic = J.periodic != 0;
     const int tid = threadIdx.x;
     // A: a guess per element from a root descent and a walk (a failed guess
@@ -488,12 +501,12 @@ __device__ void walk_line(const Job& J, size_t base, size_t stride, int len, int
         const double p = J.sorted_p[base + (size_t)k * stride];
         bool bb = false;
         if (k == 0 && element0_exact_from_start) {
-            s_guess[0] = walk_one(J.walk, x, p, bb, first_start, J.Lx, periodic, err);
+            s_guess[0] = walk_one(J.walk, x, p, bb, first_start, J.Lx, periodic, err, ext);
             continue;
         }
         const int d = descend_one(J.desc, x, p, bb, 0, J.Lx, periodic, guess_err);
         bb = false;
-        s_guess[k] = walk_one(J.walk, x, p, bb, d, J.Lx, periodic, guess_err);
+        s_guess[k] = walk_one(J.walk, x, p, bb, d, J.Lx, periodic, guess_err, nullptr);
     }
     __syncthreads();
     // B: walk every element from its predecessor's guess (element 0: from the
@@ -507,7 +520,7 @@ __device__ void walk_line(const Job& J, size_t base, size_t stride, int len, int
         bool bb = false;
         int werr = 0;
         const int lf = walk_one(J.walk, tx, J.sorted_p[base + (size_t)k * stride], bb, start,
-                                J.Lx, periodic, werr);
+                                J.Lx, periodic, werr, nullptr);
         out.failed[q] = werr != 0;
         out.leaf[q] = lf;
         out.tx[q] = tx;
@@ -515,17 +528,30 @@ __device__ void walk_line(const Job& J, size_t base, size_t stride, int len, int
         s_leaf[k] = lf;
     }
     __syncthreads();
-    // C: first element whose exact walk disagrees with the guess its
-    // successor started from
-    if (tid == 0) *s_first = len;
+    // C: elements 0..m are exact, m = the first element whose exact walk
+    // disagrees with the guess its successor started from, or the element
+    // before the first walk that could not finish here (it is redone with the
+    // full history); -1 when element 0 itself must be redone
+    if (tid == 0) {
+        *s_first = len;
+        *s_fail = len;
+    }
     __syncthreads();
     for (int k = tid; k < len - 1; k += kLineThreads)
         if (s_leaf[k] != s_guess[k]) atomicMin(s_first, k);
+#pragma unroll
+    for (int q = 0; q < kLinePerThread; ++q) {
+        const int k = tid + q * kLineThreads;
+        if (k < len && out.failed[q]) atomicMin(s_fail, k);
+    }
+    __syncthreads();
+    if (tid == 0 && *s_fail < len) *s_first = min(*s_first, *s_fail - 1);
     __syncthreads();
 }
 
 __global__ __launch_bounds__(kLineThreads) void column_kernel(const Job* __restrict__ jobs) {
-    __shared__ int s_guess[kMaxLine], s_leaf[kMaxLine], s_first;
+    __shared__ int s_guess[kMaxLine], s_leaf[kMaxLine], s_first, s_fail;
+    __shared__ int ext[kMaxWalkSteps];
     const Job& J = jobs[blockIdx.y];
     const bool periodic = J.periodic != 0;
     const int np = J.np, nx = J.nx;
@@ -540,26 +566,26 @@ __global__ __launch_bounds__(kLineThreads) void column_kernel(const Job* __restr
     }
     __syncthreads();
     LineOut out;
-    walk_line(J, 0, (size_t)nx, np, s_start, true, s_guess, s_leaf, &s_first, out, err);
+    walk_line(J, 0, (size_t)nx, np, s_start, true, s_guess, s_leaf, &s_first, &s_fail, ext, out,
+              err);
     const int m = s_first;  // rows 0..m exact
 #pragma unroll
     for (int q = 0; q < kLinePerThread; ++q) {
         const int r = threadIdx.x + q * kLineThreads;
         if (r >= np || r > m) continue;
         const size_t k = (size_t)r * nx;
-        if (out.failed[q]) err = 1;
         J.sorted_x[k] = out.tx[q];
         J.first_leaf[r] = out.leaf[q];
         J.leaf_of[k] = out.beyond[q] ? 0 : out.leaf[q];
     }
     __syncthreads();
     if (threadIdx.x == 0 && m < np - 1) {
-        int leaf = s_leaf[m];
+        int leaf = m >= 0 ? s_leaf[m] : s_start;
         for (int r = m + 1; r < np; ++r) {
             const size_t k = (size_t)r * nx;
             double tx = J.sorted_x[k];
             bool bb = false;
-            leaf = walk_one(J.walk, tx, J.sorted_p[k], bb, leaf, J.Lx, periodic, err);
+            leaf = walk_one(J.walk, tx, J.sorted_p[k], bb, leaf, J.Lx, periodic, err, ext);
             J.sorted_x[k] = tx;
             J.first_leaf[r] = leaf;
             J.leaf_of[k] = bb ? 0 : leaf;
@@ -569,7 +595,8 @@ __global__ __launch_bounds__(kLineThreads) void column_kernel(const Job* __restr
 }
 
 __global__ __launch_bounds__(kLineThreads) void rows_kernel(const Job* __restrict__ jobs) {
-    __shared__ int s_guess[kMaxLine], s_leaf[kMaxLine], s_first;
+    __shared__ int s_guess[kMaxLine], s_leaf[kMaxLine], s_first, s_fail;
+    __shared__ int ext[kMaxWalkSteps];
     const Job& J = jobs[blockIdx.y];
     const int r = blockIdx.x;
     if (r >= J.np) return;
@@ -579,23 +606,23 @@ __global__ __launch_bounds__(kLineThreads) void rows_kernel(const Job* __restric
     // elements 1..nx-1 of the row, starting from the first column's leaf
     const size_t base = (size_t)r * nx + 1;
     LineOut out;
-    walk_line(J, base, 1, nx - 1, J.first_leaf[r], false, s_guess, s_leaf, &s_first, out, err);
+    const int start = J.first_leaf[r];
+    walk_line(J, base, 1, nx - 1, start, false, s_guess, s_leaf, &s_first, &s_fail, ext, out, err);
     const int m = s_first;
 #pragma unroll
     for (int q = 0; q < kLinePerThread; ++q) {
         const int k = threadIdx.x + q * kLineThreads;
         if (k >= nx - 1 || k > m) continue;
-        if (out.failed[q]) err = 1;
         J.sorted_x[base + k] = out.tx[q];
         J.leaf_of[base + k] = out.beyond[q] ? 0 : out.leaf[q];
     }
     __syncthreads();
     if (threadIdx.x == 0 && m < nx - 2) {
-        int leaf = s_leaf[m];
+        int leaf = m >= 0 ? s_leaf[m] : start;
         for (int k = m + 1; k < nx - 1; ++k) {
             double tx = J.sorted_x[base + k];
             bool bb = false;
-            leaf = walk_one(J.walk, tx, J.sorted_p[base + k], bb, leaf, J.Lx, periodic, err);
+            leaf = walk_one(J.walk, tx, J.sorted_p[base + k], bb, leaf, J.Lx, periodic, err, ext);
             J.sorted_x[base + k] = tx;
             J.leaf_of[base + k] = bb ? 0 : leaf;
         }

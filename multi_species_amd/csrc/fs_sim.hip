@@ -28,14 +28,19 @@
 #pragma clang fp contract(off)
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "fs_field_batch.h"
 #include "fs_mesh.h"
 #include "fs_remesh.h"
+#include "fs_sim_internal.h"
 
 namespace fs {
 namespace sim {
@@ -242,7 +247,15 @@ struct Species {
     }
 };
 
+inline uint64_t next_uid() {
+    static std::atomic<uint64_t> n{1};
+    return n.fetch_add(1);
+}
+
 struct Deck {
+    uint64_t uid = next_uid();
+    // mixed-precision governor state of its full field (empty: none yet)
+    std::vector<uint64_t> gov;
     fs_sim_params prm{};
     std::vector<std::unique_ptr<Species>> sp;
     int off = 0, n = 0;    // points
@@ -288,6 +301,7 @@ struct fs_sim {
     fs::sim::SpeciesInfo* sinfo = nullptr;
     int* d_status = nullptr;
     hipGraphExec_t g_remesh = nullptr, g_plain = nullptr;
+    std::vector<uint64_t> planned;  // deck uids in the segment order of the field plans
 
     hipStream_t stream() const { return ctx->stream; }
 };
@@ -445,6 +459,15 @@ int upload_species(Sim& S, const Deck& D, Species& s, bool grid_too) {
 int layout(Sim& S, bool keep_resident) {
     hipStream_t st = S.stream();
     drop_graphs(S);
+    {
+        // each deck keeps its governor state across the re-plan
+        std::vector<uint64_t> g;
+        FS_TRY(S.ffull.get_gov(g, st));
+        for (size_t i = 0; i < S.planned.size() && (i + 1) * fs::kGovVals <= g.size(); ++i)
+            for (auto& D : S.decks)
+                if (D->uid == S.planned[i])
+                    D->gov.assign(g.begin() + i * fs::kGovVals, g.begin() + (i + 1) * fs::kGovVals);
+    }
     S.sp.clear();
     S.any_adaptive = S.any_treecode = false;
     int N = 0;
@@ -632,6 +655,25 @@ int layout(Sim& S, bool keep_resident) {
     FS_TRY(S.ffull.plan(full, N, S.precision, S.ctx->num_cus, S.ctx->tuning, st));
     FS_TRY(S.fred.plan(redsg, S.U, S.precision, S.ctx->num_cus, S.ctx->tuning, st));
     FS_TRY(S.fred.set_shape(1, 8, st));
+    // Mixed precision: the full field sets each deck's FP64 band from the
+    // error its previous evaluation measured and records its own; the
+    // reduced field reads the full field's latest state without changing
+    // it, so evaluating it again (decks joining a running batch) reproduces
+    // it bit for bit. A deck's state is its own: it does not depend on the
+    // other decks of the batch.
+    S.planned.clear();
+    {
+        std::vector<uint64_t> g(full.size() * fs::kGovVals, 0);
+        for (size_t d = 0; d < S.decks.size(); ++d) {
+            S.planned.push_back(S.decks[d]->uid);
+            if (S.decks[d]->gov.size() == (size_t)fs::kGovVals)
+                std::copy(S.decks[d]->gov.begin(), S.decks[d]->gov.end(),
+                          g.begin() + d * fs::kGovVals);
+        }
+        FS_TRY(S.ffull.set_gov(g, st));
+    }
+    S.ffull.govern(S.ffull.gov(), true, true);
+    S.fred.govern(S.ffull.gov(), false, false);
     // launch shape of the full field: timed on this batch when its membership
     // or precision changed, reused when an adaptive remesh only resized it
     FS_HIP_CHECK(hipStreamSynchronize(st));
@@ -646,71 +688,109 @@ int layout(Sim& S, bool keep_resident) {
     return FS_OK;
 }
 
-// Remesh with adaptive refinement: base-grid f on the device, refinement on
-// the host with batched device interpolation of each pass's new points.
+// Remesh with adaptive refinement: grid interpolation of every species on
+// the device; species without refinement are reset to their grid there and
+// stay resident; each refining species rebuilds its tree on a host thread of
+// its own (the trees are independent), evaluating each refinement pass' new
+// points with one device launch (interpolate_from_mesh; device calls from the
+// threads take turns on the stream), and is uploaded again.
 int remesh_adaptive(Sim& S) {
     hipStream_t st = S.stream();
     FS_TRY(rm::clear_status(S.d_jobs, S.launch, st));
     FS_TRY(rm::prepare_old_meshes(S.d_jobs, S.launch, st));
     FS_TRY(rm::interp_grids(S.d_jobs, S.launch, st));
+    // every species back on its grid (the refining ones are replaced below)
+    reset_points_jobs<<<dim3(blocks(S.launch.max_points), (unsigned)S.sp.size()), kBlock, 0, st>>>(
+        S.d_reset);
+    FS_HIP_CHECK(hipGetLastError());
     FS_TRY(read_status(S));
-    DevBuf io;
-    for (size_t k = 0; k < S.sp.size(); ++k) {
-        Species& s = *S.sp[k];
-        const Deck& D = *S.decks[s.deck];
-        const int nb = s.nx * s.np;
-        mesh::PanelTree t = s.base;
-        FS_HIP_CHECK(hipMemcpyAsync(t.fs.data(), s.d_fnew.ptr, sizeof(double) * nb,
-                                    hipMemcpyDeviceToHost, st));
-        FS_HIP_CHECK(hipStreamSynchronize(st));
-        if (s.adaptive) {
-            int rc = FS_OK;
-            // a point job on this species' old mesh (its records are already built)
-            auto eval = [&](const double* x, const double* p, size_t n, double* f) {
-                if (rc != FS_OK || n == 0) return;
-                const size_t b = align_up(sizeof(double) * n, 256);
-                const size_t bj = align_up(sizeof(rm::Job), 256);
-                if ((rc = io.ensure(3 * b + bj, st)) != FS_OK) return;
-                rm::Job j = s.job;
-                j.x = io.as<double>(0);
-                j.p = io.as<double>(b);
-                j.f = io.as<double>(2 * b);
-                j.nx = (int)n;
-                j.np = 1;
-                rm::Job* dj = io.as<rm::Job>(3 * b);
-                rm::Launch l;
-                l.njobs = 1;
-                l.max_points = (int)n;
-                if (hipMemcpyAsync((void*)j.x, x, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
-                    hipMemcpyAsync((void*)j.p, p, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
-                    hipMemcpyAsync(dj, &j, sizeof(rm::Job), hipMemcpyHostToDevice, st) != hipSuccess) {
-                    rc = fail(FS_E_HIP, "adaptive remesh: upload failed");
-                    return;
-                }
-                if ((rc = rm::clear_status(dj, l, st)) != FS_OK) return;
-                if ((rc = rm::interp_points(dj, l, st)) != FS_OK) return;
-                if (hipMemcpyAsync(f, j.f, sizeof(double) * n, hipMemcpyDeviceToHost, st) != hipSuccess) {
-                    rc = fail(FS_E_HIP, "adaptive remesh: download failed");
-                    return;
-                }
-                int status = 0;
-                if (hipMemcpyAsync(&status, j.status, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-                    hipStreamSynchronize(st) != hipSuccess) {
-                    rc = fail(FS_E_HIP, "adaptive remesh: sync failed");
-                    return;
-                }
-                rc = rm::status_to_error(status);
-            };
-            t.refine_adaptively(s.spec, eval);
-            FS_TRY(rc);
+    std::vector<Species*> refine;
+    for (Species* s : S.sp)
+        if (s->adaptive) refine.push_back(s);
+    std::vector<mesh::PanelTree> trees(refine.size());
+    for (size_t q = 0; q < refine.size(); ++q) {
+        Species& s = *refine[q];
+        trees[q] = s.base;
+        FS_HIP_CHECK(hipMemcpyAsync(trees[q].fs.data(), s.d_fnew.ptr,
+                                    sizeof(double) * s.nx * s.np, hipMemcpyDeviceToHost, st));
+    }
+    FS_HIP_CHECK(hipStreamSynchronize(st));
+    std::mutex dev_mu;
+    std::vector<int> rcs(refine.size(), FS_OK);
+    std::vector<std::string> errs(refine.size());
+    auto work = [&](size_t q) {
+        Species& s = *refine[q];
+        DevBuf io;
+        int& rc = rcs[q];
+        // a point job on this species' old mesh (its records are already built)
+        auto eval = [&](const double* x, const double* p, size_t n, double* f) {
+            if (rc != FS_OK || n == 0) return;
+            std::lock_guard<std::mutex> lk(dev_mu);
+            if (hipSetDevice(S.ctx->device) != hipSuccess) {
+                rc = fail(FS_E_HIP, "adaptive remesh: set device failed");
+                return;
+            }
+            const size_t b = align_up(sizeof(double) * n, 256);
+            const size_t bj = align_up(sizeof(rm::Job), 256);
+            if ((rc = io.ensure(3 * b + bj, st)) != FS_OK) return;
+            rm::Job j = s.job;
+            j.x = io.as<double>(0);
+            j.p = io.as<double>(b);
+            j.f = io.as<double>(2 * b);
+            j.nx = (int)n;
+            j.np = 1;
+            rm::Job* dj = io.as<rm::Job>(3 * b);
+            rm::Launch l;
+            l.njobs = 1;
+            l.max_points = (int)n;
+            if (hipMemcpyAsync((void*)j.x, x, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync((void*)j.p, p, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(dj, &j, sizeof(rm::Job), hipMemcpyHostToDevice, st) != hipSuccess) {
+                rc = fail(FS_E_HIP, "adaptive remesh: upload failed");
+                return;
+            }
+            if ((rc = rm::clear_status(dj, l, st)) != FS_OK) return;
+            if ((rc = rm::interp_points(dj, l, st)) != FS_OK) return;
+            if (hipMemcpyAsync(f, j.f, sizeof(double) * n, hipMemcpyDeviceToHost, st) != hipSuccess) {
+                rc = fail(FS_E_HIP, "adaptive remesh: download failed");
+                return;
+            }
+            int status = 0;
+            if (hipMemcpyAsync(&status, j.status, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess) {
+                rc = fail(FS_E_HIP, "adaptive remesh: sync failed");
+                return;
+            }
+            rc = rm::status_to_error(status);
+        };
+        trees[q].refine_adaptively(s.spec, eval);
+        {
+            std::lock_guard<std::mutex> lk(dev_mu);
+            (void)hipStreamSynchronize(st);
+            io.release();
         }
-        s.tree = std::move(t);
+        if (rc != FS_OK) errs[q] = fs_last_error();
+    };
+    if (refine.size() == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> th;
+        for (size_t q = 0; q < refine.size(); ++q) th.emplace_back(work, q);
+        for (auto& t : th) t.join();
+    }
+    for (size_t q = 0; q < refine.size(); ++q)
+        if (rcs[q] != FS_OK) return fail(rcs[q], "%s", errs[q].c_str());
+    FS_HIP_CHECK(hipSetDevice(S.ctx->device));
+    for (size_t q = 0; q < refine.size(); ++q) {
+        Species& s = *refine[q];
+        const Deck& D = *S.decks[s.deck];
+        s.tree = std::move(trees[q]);
         std::vector<int32_t> leaves;
         s.tree.leaf_weights(s.q, D.prm.quadrature == 1, s.base_w, leaves);
         s.red = mesh::reduced_map(s.tree.xs);
+        s.off = -1;  // upload from the host tree
     }
-    io.release();
-    FS_TRY(layout(S, false));
+    FS_TRY(layout(S, true));
     return field_reduced(S);
 }
 
@@ -770,6 +850,11 @@ int check_deck(const fs_sim_params* prm, const fs_species_params* species, int n
         if (p.initial_height < 1 || p.initial_height > 12 || p.p_height < 0 ||
             p.initial_height + p.p_height > 13 || !(p.p_max > p.p_min) || p.m == 0.0)
             return fail(FS_E_INVALID, "deck: species %d has bad mesh parameters", k);
+        // the remesh searches hold one grid row or column per workgroup
+        if ((1 << (p.initial_height + p.p_height + 1)) + 1 > rm::kMaxLine)
+            return fail(FS_E_INVALID,
+                        "deck: species %d has grid lines longer than %d points "
+                        "(initial_height + p_height <= 8)", k, rm::kMaxLine);
     }
     return FS_OK;
 }
@@ -838,6 +923,120 @@ int capture(Sim& S, bool remesh_now, hipGraphExec_t& exec) {
 }
 
 }  // namespace
+
+// ---- batch operations for the serving loop --------------------------------
+
+int build_deck_host(const fs_deck* deck, Deck** out) {
+    if (!deck || !out) return fail(FS_E_INVALID, "deck: null argument");
+    *out = nullptr;
+    std::unique_ptr<Deck> D;
+    FS_TRY(build_deck(deck->sim, deck->species, deck->n_species, D));
+    *out = D.release();
+    return FS_OK;
+}
+
+void free_deck(Deck* d) {
+    if (!d) return;
+    for (auto& s : d->sp) s->release();
+    delete d;
+}
+
+int64_t deck_points(const Deck* d) {
+    int64_t n = 0;
+    for (const auto& s : d->sp) n += s->n();
+    return n;
+}
+
+size_t deck_footprint(const Deck* d, int precision) {
+    // per point: state 5 + RK4 7 doubles, reduced map and species index,
+    // field partials (up to 16 splits) and, in mixed precision, the sorted
+    // copies; per panel: record, coefficient, walk and descent records
+    size_t b = 0;
+    for (const auto& s : d->sp) {
+        const size_t n = (size_t)s->n(), P = (size_t)s->npanels(), g = (size_t)(s->nx * s->np);
+        b += n * (12 * 8 + 3 * 4 + 16 * 8 + (precision == FS_PRECISION_MIXED ? 96 : 0));
+        b += P * (80 + 96 + 104 + 184);
+        b += g * (4 * 8 + 4) + rm::grid_bytes((int)g, s->np);
+    }
+    return b + (size_t(1) << 20);
+}
+
+int create_empty(fs_ctx* ctx, int precision, fs_sim** out) {
+    if (!ctx || !out) return fail(FS_E_INVALID, "null argument");
+    if (precision != FS_PRECISION_F64 && precision != FS_PRECISION_MIXED)
+        return fail(FS_E_INVALID, "unknown precision %d", precision);
+    FS_HIP_CHECK(hipSetDevice(ctx->device));
+    auto S = std::make_unique<fs_sim>();
+    S->ctx = ctx;
+    S->precision = precision;
+    S->remesh_period = 1;
+    FS_TRY(layout(*S, true));
+    *out = S.release();
+    return FS_OK;
+}
+
+int attach_decks(fs_sim* S, std::vector<Deck*>& decks, std::vector<int>* indices) {
+    if (decks.empty()) return FS_OK;
+    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
+    for (Deck*& d : decks) {
+        if (d->prm.remesh_period != 1) {
+            for (Deck*& e : decks) free_deck(e), e = nullptr;
+            decks.clear();
+            return fail(FS_E_INVALID, "decks joining a running batch remesh every step");
+        }
+        S->decks.emplace_back(d);
+        if (indices) indices->push_back((int)S->decks.size() - 1);
+        d = nullptr;
+    }
+    decks.clear();
+    renumber(*S);
+    S->tune_pending = true;
+    FS_TRY(layout(*S, true));
+    FS_TRY(field_reduced(*S));
+    FS_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+    return FS_OK;
+}
+
+int detach_decks(fs_sim* S, const std::vector<int>& idx,
+                 std::vector<std::vector<SpeciesState>>& out) {
+    out.assign(idx.size(), {});
+    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
+    std::vector<char> drop(S->decks.size(), 0);
+    for (size_t q = 0; q < idx.size(); ++q) {
+        const int d = idx[q];
+        if (d < 0 || d >= (int)S->decks.size()) return fail(FS_E_INVALID, "bad deck index");
+        drop[d] = 1;
+        int first = 0;
+        for (int e = 0; e < d; ++e) first += (int)S->decks[e]->sp.size();
+        for (size_t k = 0; k < S->decks[d]->sp.size(); ++k) {
+            const int g = first + (int)k;
+            SpeciesState st;
+            const int n = S->sp[g]->n();
+            st.xs.resize(n);
+            st.ps.resize(n);
+            st.fs.resize(n);
+            st.qw.resize(n);
+            st.es.resize(n);
+            FS_TRY(download_species(*S, g, st.xs.data(), st.ps.data(), st.fs.data(),
+                                    st.qw.data(), st.es.data()));
+            FS_TRY(fs_sim_diagnostics(S, g, st.diag));
+            out[q].push_back(std::move(st));
+        }
+    }
+    std::vector<std::unique_ptr<Deck>> keep;
+    for (size_t d = 0; d < S->decks.size(); ++d) {
+        if (drop[d]) {
+            for (auto& s : S->decks[d]->sp) s->release();
+        } else {
+            keep.push_back(std::move(S->decks[d]));
+        }
+    }
+    S->decks = std::move(keep);
+    renumber(*S);
+    S->tune_pending = true;
+    return layout(*S, true);
+}
+
 }  // namespace sim
 }  // namespace fs
 

@@ -8,6 +8,8 @@ Bars:
   host reads and eager work between replays;
 * the mixed batch against the FP64 runs of the same decks: E within 1e-8
   (relative, normwise) after the first step;
+* the governed step loop: the served E of weak decks within 1e-8 of the FP64
+  field of the same points, step after step;
 * continuous batching: a deck that joins a running batch evolves bitwise as
   it does alone; removing a deck leaves the others' states unchanged.
 """
@@ -106,8 +108,31 @@ def test_mixed_field_within_1e8_of_f64(gpu_ctx):
     m.close()
 
 
-def test_continuous_batching(gpu_ctx):
-    b = ms.Simulation.batch(gpu_ctx, DECKS[:2], ms.PRECISION_F64)
+def test_governed_step_loop_error(gpu_ctx):
+    """The mixed step loop widens each deck's FP64 band from its previous
+    max|E|: weak decks (max|E| small against the weights) keep the served E
+    within 1e-8 of the FP64 field of the same points, step after step."""
+    decks = [landau(5, 1, amp=1e-3, k=0.5), landau(4, 1, amp=1e-2, k=0.5), two_stream(4, 0)]
+    m = ms.Simulation.batch(gpu_ctx, decks, ms.PRECISION_MIXED)
+    for step in range(4):
+        m.step()
+        for d, (sim, sps) in enumerate(decks):
+            first, nsp, _, _ = m.deck_info(d)
+            pts = [m.points(first + k) for k in range(nsp)]
+            x = np.concatenate([p[0] for p in pts])
+            w = np.concatenate([p[3] for p in pts])
+            es = np.concatenate([p[4] for p in pts])
+            ef = gpu_ctx.emq_periodic_f64(x - sim.x_min, x - sim.x_min, w,
+                                          sim.x_max - sim.x_min, sim.greens_epsilon)
+            r = np.max(np.abs(es - ef)) / np.max(np.abs(ef))
+            print(f"step {step} deck {d}: served E vs FP64 of the same points {r:.2e}")
+            assert r <= 1e-8, (step, d)
+    m.close()
+
+
+@pytest.mark.parametrize("precision", [ms.PRECISION_F64, ms.PRECISION_MIXED])
+def test_continuous_batching(gpu_ctx, precision):
+    b = ms.Simulation.batch(gpu_ctx, DECKS[:2], precision)
     b.step(2)
     j = b.add_deck(*DECKS[2])
     assert j == 2 and b.n_decks == 3
@@ -115,7 +140,7 @@ def test_continuous_batching(gpu_ctx):
     first, nsp, it, _ = b.deck_info(2)
     assert it == 3
     joined = state(b, first, nsp)
-    alone = run_alone(gpu_ctx, DECKS[2], 3, ms.PRECISION_F64)
+    alone = run_alone(gpu_ctx, DECKS[2], 3, precision)
     for k in range(nsp):
         assert np.array_equal(joined[k], alone[k])
     # deck 0 has taken 5 steps; remove deck 1 and check deck 0 and 2 continue unchanged
@@ -123,13 +148,13 @@ def test_continuous_batching(gpu_ctx):
     assert b.n_decks == 2
     b.step(1)
     d0 = state(b, 0, 2)
-    a0 = run_alone(gpu_ctx, DECKS[0], 6, ms.PRECISION_F64)
+    a0 = run_alone(gpu_ctx, DECKS[0], 6, precision)
     for k in range(2):
         assert np.array_equal(d0[k], a0[k])
     first, nsp, it, _ = b.deck_info(1)
     assert it == 4
     d2 = state(b, first, nsp)
-    a2 = run_alone(gpu_ctx, DECKS[2], 4, ms.PRECISION_F64)
+    a2 = run_alone(gpu_ctx, DECKS[2], 4, precision)
     for k in range(nsp):
         assert np.array_equal(d2[k], a2[k])
     b.close()
