@@ -76,9 +76,11 @@ int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits);
  * reference; the FP64 entry points stay the drop-in). Positions are wrapped
  * in FP64 and differenced exactly in 64-bit fixed point; the step and
  * neutralizing part of the kernel is summed in FP64 from sorted prefix sums,
- * the regularization remainder in FP32 away from z = 0 and in FP64 within
- * B = max(2^-6, 16 d, 4000 d^2) of it (d = eps/L; every pair in FP64 when
- * B >= 1/4), with compensated FP64 accumulation per target.
+ * the regularization remainder in FP64 within B = max(2^-6, 16 d, 4000 d^2)
+ * of z = 0 (d = eps/L; every pair in FP64 when B >= 1/4) and in FP32 beyond,
+ * where each 32-source tile that is compact against its distance is summed
+ * from an 8-moment expansion about its centre and the others pair by pair;
+ * compensated FP64 accumulation per target.
  * Relative error max_i|E_i - E_i^FP64| / max_i|E_i^FP64|: 1e-10 to 1e-8 at
  * that band on the tested decks; with fs_emq_set_mixed_target the band is
  * widened until the FP32 error bound holds the given target. Requires
@@ -89,10 +91,11 @@ int fs_emq_periodic_mixed(fs_ctx* ctx, double* es, const double* targets, int64_
                           double L, double epsilon);
 
 /* Error target of the mixed entry points on this context (no counterpart in
- * the reference). 0 (the default): the minimum band B above, one pass. > 0:
- * a first pass gives max|E|; when the bound 2 u W d^2 / B on the FP32 terms'
- * error (u = 2^-24, W = sum |q_w|) exceeds rel_err max|E|, the pair sums are
- * redone with B widened to meet it. At most 1e-3. */
+ * the reference). 0 (the default): the minimum band B above. > 0: before
+ * its pair sums, each call measures on up to 256 probed targets the E error
+ * that each of 8 candidate bands (B/2 to 4 sqrt(2) B) would give against the
+ * exact FP64 field, and takes the narrowest one within rel_err * max|E|
+ * (every pair in FP64 when none is). At most 1e-3. */
 int fs_emq_set_mixed_target(fs_ctx* ctx, double rel_err);
 
 /* Launch shape (targets per thread, persistent workgroups per CU) that the
@@ -236,6 +239,15 @@ int fs_sim_deck_info(fs_sim* sim, int deck, int* first_species, int* n_species, 
 /* precision as above; use_graph 0 runs every step eagerly (adaptive and
  * treecode decks always do). */
 int fs_sim_set_options(fs_sim* sim, int precision, int use_graph);
+/* Mixed precision: what the accuracy governor measured in the deck's last
+ * full-field evaluation (every mixed evaluation measures its own error
+ * before its pair sums: up to 256 probed targets against the exact FP64
+ * field). max_e = max|E| over the probed targets; errs[q] = the largest E
+ * error there when pairs beyond bands[q] (fraction of the period; >= 1/4
+ * means every pair in FP64) take the FP32 path, q = 0..7. The evaluation
+ * took the narrowest band with errs[q] <= 4e-9 max_e. Any output may be
+ * NULL (errs, bands: 8 doubles). */
+int fs_sim_field_governor(fs_sim* sim, int deck, double* max_e, double* errs, double* bands);
 
 /* ---- Serving loop: continuous batching (no counterpart in the reference) --
  * A server owns a context's device and an initially empty batch. Requests

@@ -69,6 +69,7 @@ def lib():
         L.fs_sim_iter.argtypes = [_vp, ctypes.POINTER(_int)]
         L.fs_sim_species_counts.argtypes = [_vp, _int, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]
         L.fs_sim_get_points.argtypes = [_vp, _int, _dp, _dp, _dp, _dp, _dp]
+        L.fs_sim_field_governor.argtypes = [_vp, _int, _dp, _dp, _dp]
         L.fs_sim_get_panels.argtypes = [_vp, _int, _ip]
         L.fs_sim_diagnostics.argtypes = [_vp, _int, _dp]
         L.fs_batch_create.argtypes = [_vp, ctypes.POINTER(Deck), _int, _int, ctypes.POINTER(_vp)]
@@ -394,6 +395,15 @@ class Simulation:
         _check(lib().fs_sim_species_counts(self.h, k, ctypes.byref(n), ctypes.byref(P)),
                "fs_sim_species_counts")
         return n.value, P.value
+
+    def field_governor(self, deck: int):
+        """(max|E|, errors[8], bands[8]) of the mixed-precision accuracy governor
+        for one deck (its last full-field evaluation)."""
+        em = np.zeros(1)
+        errs, bands = np.zeros(8), np.zeros(8)
+        _check(lib().fs_sim_field_governor(self.h, deck, _P(em), _P(errs), _P(bands)),
+               "fs_sim_field_governor")
+        return float(em[0]), errs, bands
 
     def points(self, k: int):
         """(xs, ps, fs, q_ws, es) of species k."""

@@ -14,6 +14,12 @@ enum { kPrecisionF64 = 0, kPrecisionMixed = 1 };
 // FP64 bands (mixed precision).
 constexpr int kGovVals = 9;
 
+// Minimum FP64 band B0 of the mixed method for delta = eps/L (as a fraction
+// of the period), and the governor's candidate band q = 0..7,
+// B0 2^((q-2)/2), at least 2^-6 (>= 1/4: every pair in FP64).
+double min_band(double delta);
+double governor_band(double delta, int q);
+
 // One segment of the concatenated point arrays. A deck is one segment that
 // is its own source; the single-call API with distinct targets uses two
 // segments (the sources, not evaluated, and the targets, sourced from the
@@ -42,6 +48,20 @@ struct SegDev {
     int is_src;       // some segment takes its sources from this one
     int all_band0;    // all_band of the minimum band (no governor)
     uint64_t band0;   // the minimum band B0 * 2^64
+    uint64_t xband;   // far tiles whose sources lie within xband of their centre expand
+    int tile_base;    // index of this segment's first source tile (source segments)
+    int src_tile_base;
+    int log2h;        // expansion scale H = 2^log2h, the power of 2 at or below B0
+};
+
+// Far-field expansion of one 32-source tile (mixed precision): with Z the
+// displacement from the tile's centre to a target and rho = H/Z, the tile's
+// regularization remainder at the target is rho |rho| sum_m c[m] rho^m.
+struct TileRec {
+    uint32_t hc;      // high word of the centre
+    uint32_t pad;
+    uint64_t emax;    // largest |centre - source| (units of 2^-64)
+    float c[12];
 };
 
 // Far-field record of one sorted source (FP32 path of the mixed kernel).
@@ -72,25 +92,14 @@ class FieldBatch {
 public:
     int plan(const std::vector<BatchSeg>& segs, int n_total, int precision, int num_cus,
              const FieldTuning& tun, hipStream_t stream);
-    // Accuracy governor (mixed precision; off after plan()). The state of a
-    // segment is kGovVals doubles (bit patterns): max |E| of a run and the E
-    // error that each candidate FP64 band would have given, measured on
-    // probed targets. Before each run the narrowest band whose error was
-    // within target * max|E| is taken from src (every pair in FP64 while a
-    // segment has no state); reset clears src after reading it. With record
-    // set, each run writes its own state to gov(). target <= 0: 4e-9.
-    void govern(const uint64_t* src, bool reset, bool record, double target = 0);
-    uint64_t* gov() const;
-    // A run whose result meets the governor's target without a previous
-    // evaluation: a recorded run at the minimum band, then, when its measured
-    // error exceeds the target, the pair sums again with the band the
-    // governor picks. Synchronizes the stream.
-    int run_checked(const double* x, const double* w, double* es, double target,
-                    hipStream_t stream);
-    // Governor state of every segment (nseg * kGovVals), for carrying it
-    // across re-plans.
-    int get_gov(std::vector<uint64_t>& out, hipStream_t stream) const;
-    int set_gov(const std::vector<uint64_t>& v, hipStream_t stream);
+    // Accuracy governor (mixed precision; off after plan()). When on, each
+    // run first measures on its own inputs the E error that each candidate
+    // FP64 band would give (probed targets against the exact FP64 field) and
+    // takes the narrowest band within target * max|E| (target <= 0: 4e-9).
+    void govern(bool on, double target = 0);
+    // The last governed run's measurements (nseg * kGovVals values: per
+    // segment max|E| and the error at each candidate band, as doubles).
+    int get_gov(std::vector<double>& out, hipStream_t stream) const;
     // Launch shape of the pair kernel: targets per thread (1, 2, 4) and
     // persistent workgroups per CU. Does not change any result.
     int set_shape(int tpt, int wgs_per_cu, hipStream_t stream);
@@ -114,9 +123,9 @@ private:
     int launch_pairs(int nit, int grid, hipStream_t stream);
 
     int precision_ = kPrecisionF64;
-    const uint64_t* gov_src_ = nullptr;
-    bool gov_reset_ = false, gov_record_ = false;
+    bool governed_ = false;
     double gov_target_ = 4e-9;
+    uint64_t* gov() const;
     int n_total_ = 0, nseg_ = 0, tpt_ = 2, grid_ = 0, wgs_per_cu_ = 8, num_cus_ = 256;
     const double* x_ = nullptr;  // inputs of the current run (FP64 pair kernel)
     const double* w_ = nullptr;
@@ -134,7 +143,8 @@ private:
     // work space offsets (bytes into work_)
     size_t o_part_ = 0, o_key_ = 0, o_skey_ = 0, o_perm_ = 0, o_sort_ = 0, o_as_ = 0,
            o_asm1_ = 0, o_ws_ = 0, o_far_ = 0, o_pw_ = 0, o_bsum_ = 0, o_bsa_ = 0, o_tot_ = 0,
-           o_gov_ = 0;
+           o_gov_ = 0, o_tiles_ = 0;
+    int ntiles_ = 0;
 };
 
 // Stable radix sort by (segment of the original index, top 32 key bits)

@@ -47,8 +47,10 @@ constexpr uint64_t kHalf = 0x8000000000000000ull;
 constexpr uint64_t kAntiMargin = uint64_t(1) << 40;
 constexpr int kSortShift = 40;  // keys are sorted by bits 40..63 (fs_radix.hip)
 constexpr uint64_t kLowSort = (uint64_t(1) << kSortShift) - 1;
+constexpr int kXShift = 4;             // tiles expand when their sources lie within band/16
+constexpr int kMoments = 8;            // moments of a tile's expansion
+constexpr int kCoef = kMoments + 4;    // its polynomial coefficients (TileRec::c)
 constexpr int kLadder = kGovVals - 1;  // governor: candidate bands
-constexpr int kLadderB0 = 2;           // governor: the candidate equal to the minimum band
 constexpr int kProbeMax = 256;         // governor: probed targets per segment
 constexpr double kGovTarget = 4e-9;    // governor: default error target (relative to max |E|)
 
@@ -199,6 +201,87 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
     far[i] = f;
 }
 
+// Far-field expansion of each 32-source tile. The regularization remainder
+// beyond the band is G(z) = s(z) (a1 z^-2 + a2 z^-4 + a3 z^-6) with
+// a1 = -d^2/2, a2 = 3d^4/8, a3 = -5d^6/16 (the binomial series of
+// z/sqrt(z^2+d^2) - s(z) in d^2/z^2). With z_j = Z + e_j, e_j = c - x_j for
+// the tile centre c and Z = t - c, (Z + e)^-n = Z^-n sum_k C(n+k-1, k)(-e/Z)^k,
+// so the tile sums to s(Z) sum_n rho^n sum_{2i+k=n} a_i H^-2i C(2i+k-1, k)
+// (-1)^k M_k with rho = H/Z and the moments M_k = sum_j w_j (e_j/H)^k,
+// k < kMoments. The centre is on the 2^-32 grid, so Z is the difference of
+// high words, as in the pair path. Coefficients are summed in FP64 and
+// rounded once to FP32.
+__global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t* __restrict__ seg_of,
+                                                      const SegDev* __restrict__ segs,
+                                                      const uint64_t* __restrict__ us,
+                                                      const double* __restrict__ ws,
+                                                      TileRec* __restrict__ tiles) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_total) return;
+    const SegDev& S = segs[seg_of[i]];
+    if (!S.is_src || ((i - S.off) & (kTile - 1)) != 0) return;
+    const int je = min(i + kTile, S.off + S.n);
+    uint64_t umin = us[i], umax = us[i];
+    for (int j = i + 1; j < je; ++j) {
+        const uint64_t u = us[j];
+        umin = u < umin ? u : umin;
+        umax = u > umax ? u : umax;
+    }
+    const uint64_t mid = umin + ((umax - umin) >> 1);
+    const uint32_t hc = (uint32_t)(mid >> 32);
+    const uint64_t uc = (uint64_t)hc << 32;
+    uint64_t emax = 0;
+    double M[kMoments];
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k) M[k] = 0.0;
+    for (int j = i; j < je; ++j) {
+        const int64_t e = (int64_t)(uc - us[j]);
+        const uint64_t ae = (uint64_t)(e < 0 ? -e : e);
+        emax = ae > emax ? ae : emax;
+        const double eh = __builtin_ldexp((double)e, -64 - S.log2h);
+        double pw = ws[j];
+#pragma unroll
+        for (int k = 0; k < kMoments; ++k) {
+            M[k] += pw;
+            pw *= eh;
+        }
+    }
+    const double dh2 = __builtin_ldexp(S.delta2, -2 * S.log2h);  // (d/H)^2
+    const double a[3] = {-0.5 * dh2, 0.375 * dh2 * dh2, -0.3125 * dh2 * dh2 * dh2};
+    TileRec T;
+    T.hc = hc;
+    T.pad = 0;
+    T.emax = emax;
+#pragma unroll
+    for (int m = 0; m < kCoef; ++m) {
+        // n = m + 2 = 2i + k (i = 1..3)
+        double c = 0.0;
+#pragma unroll
+        for (int ii = 1; ii <= 3; ++ii) {
+            const int k = m + 2 - 2 * ii;
+            if (k < 0 || k >= kMoments) continue;
+            // C(2i+k-1, k)
+            double b = 1.0;
+            for (int q = 1; q <= k; ++q) b = b * (2 * ii - 1 + q) / q;
+            c += a[ii - 1] * b * ((k & 1) ? -M[k] : M[k]);
+        }
+        T.c[m] = (float)c;
+    }
+    tiles[S.tile_base + (i - S.off) / kTile] = T;
+}
+
+// The FP32 value of one expanded tile at a target (high word ht): the pair
+// kernel and the probe evaluate it with the same operations.
+__device__ __forceinline__ float tile_value(const TileRec& T, uint32_t ht, int hexp) {
+    const float zs = (float)(int32_t)(ht - T.hc);
+    const float rho = __builtin_amdgcn_rcpf(__builtin_ldexpf(zs, hexp));
+    const float sq = rho * __builtin_fabsf(rho);
+    float p = T.c[kCoef - 1];
+#pragma unroll
+    for (int m = kCoef - 2; m >= 0; --m) p = __builtin_fmaf(p, rho, T.c[m]);
+    return sq * p;
+}
+
 // Per scan block of a source segment: sum of w and of w a.
 __global__ __launch_bounds__(kBlock) void blocksum_kernel(const ScanBlock* __restrict__ blocks,
                                                           const double* __restrict__ ws,
@@ -304,20 +387,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const WorkItem* __restrict__ items, int nitems, const SegDev* __restrict__ segs,
     const uint64_t* __restrict__ us, const double* __restrict__ as,
     const double* __restrict__ asm1, const double* __restrict__ ws,
-    const FarSrc* __restrict__ far, double* __restrict__ part, int* __restrict__ counter) {
+    const FarSrc* __restrict__ far, const TileRec* __restrict__ tiles, double* __restrict__ part,
+    int* __restrict__ counter) {
     for (;;) {
         const int it = pull_item(counter);
         if (it >= nitems) break;
         const WorkItem W = items[it];
         const SegDev S = segs[W.seg];
         const double d2 = S.delta2;
+        const int hexp = -32 - S.log2h;
         // positions are sorted by their top bits only: widen the block's range
         // by one unit of the sort on each side
         const uint64_t uf = us[W.tb] & ~kLowSort, ul = us[W.te - 1] | kLowSort;
         const uint64_t tspan = ul - uf;
         uint64_t ui[TPT];
         uint32_t hi[TPT];
-        double ai[TPT], sh[TPT], sl[TPT];
+        double ai[TPT], sh[TPT], sl[TPT], gx[TPT];
 #pragma unroll
         for (int t = 0; t < TPT; ++t) {
             const int i = min(W.tb + (int)threadIdx.x + t * kBlock, W.te - 1);
@@ -326,6 +411,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
             ai[t] = as[i];
             sh[t] = 0.0;
             sl[t] = 0.0;
+            gx[t] = 0.0;
         }
         for (int jb = W.sb; jb < W.se; jb += kTile) {
             const int je = min(W.se, jb + kTile);
@@ -382,6 +468,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                 }
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], acc[t]);
+            } else if (tiles[S.src_tile_base + (jb - S.src_off) / kTile].emax <= S.xband) {
+                // compact far tile: its expansion about the centre
+                const TileRec T = tiles[S.src_tile_base + (jb - S.src_off) / kTile];
+#pragma unroll
+                for (int t = 0; t < TPT; ++t) gx[t] += (double)tile_value(T, hi[t], hexp);
             } else {
                 float acc[TPT];
 #pragma unroll
@@ -435,6 +526,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
         }
 #pragma unroll
         for (int t = 0; t < TPT; ++t) {
+            two_sum_acc(sh[t], sl[t], gx[t]);
             const int i = W.tb + (int)threadIdx.x + t * kBlock;
             if (i < W.te) part[S.part_base + (int64_t)W.split * S.n + (i - S.off)] = sh[t] + sl[t];
         }
@@ -442,19 +534,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
 }
 
 // ---- accuracy governor ----------------------------------------------------
-// The FP32 far-field error of a segment is measured, not bounded: a probe
-// takes up to 256 evenly spaced targets of the segment and, for each of the
-// ladder bands B_q = B0 2^((q-2)/2) (q = 0..7, at least 2^-6), sums over the
-// source tiles that are far from the target at B_q the difference between
-// the tile's FP32 sum, computed exactly as the pair kernel computes it, and
-// the exact FP64 regularization remainder of the same pairs. Per segment the
-// state holds max |E| of the run (finalize) and, per ladder band, the max
-// over the probed targets of |that difference| / 2 (the E error the band
-// would have given). The next run takes the narrowest band whose error was
-// within target * max|E|, or every pair in FP64 when none was (or when no
-// state exists yet). A tile that is far for the pair kernel's whole target
+// Before the pair kernel of a governed run, the FP32 far-field error of each
+// evaluated segment is measured on the run's own inputs: a probe takes up to
+// 256 evenly spaced targets of the segment and sums, for each target,
+//  * the exact FP64 field (every source, the reference kernel on the exact
+//    64-bit displacement), whose largest magnitude over the probed targets
+//    stands for max|E|;
+//  * for each of the ladder bands B_q = B0 2^((q-2)/2) (q = 0..7, at least
+//    2^-6), over the source tiles that are far from the target at B_q, the
+//    difference between the tile's FP32 value, computed exactly as the pair
+//    kernel computes it (tile expansion or FP32 pair sum), and the exact FP64
+//    regularization remainder of the same pairs.
+// The governor then gives each segment the narrowest band whose largest
+// error (over the probed targets) is within target * max|E|, or every pair in
+// FP64 when none is. A tile that is far for the pair kernel's whole target
 // block is far for each of its targets, so the probe counts every FP32 tile
-// of the kernel (and a few more).
+// of the kernel (and a few more). The band is a function of the run's inputs
+// alone: evaluating the same points again gives the same field, bit for bit.
 
 __device__ __forceinline__ uint64_t ladder_band(const SegDev& S, int q) {
     double f = __builtin_ldexp(1.0, (q >> 1) - 1);  // 2^((q - 2)/2)
@@ -469,13 +565,15 @@ __device__ __forceinline__ int probe_samples(int n) {
     return min(n, min(kProbeMax, max(16, s)));
 }
 
-// grid (kProbeMax, nseg): one workgroup per probed target.
+// grid (kProbeMax, nseg): one workgroup per probed target. gov (zeroed
+// before) receives per segment max|E| and the error at each ladder band.
 __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict__ segs,
                                                        const uint64_t* __restrict__ us,
                                                        const double* __restrict__ ws,
                                                        const FarSrc* __restrict__ far,
+                                                       const TileRec* __restrict__ tiles,
                                                        uint64_t* __restrict__ gov) {
-    __shared__ double red[kLadder][kBlock];
+    __shared__ double red[kGovVals][kBlock];
     const int d = blockIdx.y;
     const SegDev S = segs[d];
     if (!S.eval || S.all_band0 || S.n == 0 || S.src_n == 0) return;
@@ -492,11 +590,23 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
     double err[kLadder];
 #pragma unroll
     for (int q = 0; q < kLadder; ++q) err[q] = 0.0;
+    double efield = 0.0;
     const SegDev R = segs[S.src];
     const double d2 = S.delta2;
     const int ntile = (R.n + kTile - 1) / kTile;
     for (int m = threadIdx.x; m < ntile; m += kBlock) {
         const int jb = R.off + m * kTile, je = min(jb + kTile, R.off + R.n);
+        // the exact field of the tile's sources: K(z) = (z/sqrt(z^2+d^2) - C z)/2,
+        // and its regularization remainder g(z) = z/sqrt(z^2+d^2) - s(z)
+        // = -d^2 s / (r (r + |z|)), r = sqrt(z^2 + d^2)
+        double ek = 0.0, ex = 0.0;
+        for (int j = jb; j < je; ++j) {
+            const double z = __builtin_ldexp((double)(int64_t)(ut - us[j]), -64);
+            const double rz = __builtin_sqrt(__builtin_fma(z, z, d2));
+            ek += ws[j] * (z / rz - S.C * z);
+            ex += ws[j] * (-d2 * __builtin_copysign(1.0, z) / (rz * (rz + __builtin_fabs(z))));
+        }
+        efield += 0.5 * ek;
         const uint64_t vf = us[jb] & ~kLowSort, vl = us[je - 1] | kLowSort;
         const uint64_t vspan = vl - vf;
         if (tspan >= kHalf || vspan >= kHalf || tspan + vspan >= kHalf) continue;
@@ -506,66 +616,63 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
         int kf = 0;
         while (kf < kLadder && band[kf] != 0 && !cyc_meets(lo + band[kf], len, 2 * band[kf])) ++kf;
         if (kf == 0) continue;
+        // the tile expands at the bands q >= qx and takes the pair path below
+        const TileRec T = tiles[R.tile_base + m];
+        int qx = 0;
+        while (qx < kf && T.emax > (band[qx] >> kXShift)) ++qx;
         float acc = 0.0f;
-        double ex = 0.0;
-        for (int j = jb; j < je; ++j) {
-            const FarSrc f = far[j];
-            const float zs = (float)(int32_t)(ht - f.h);
-            const float r = __builtin_amdgcn_rcpf(zs);
-            const float sq = r * __builtin_fabsf(r);
-            const float qq = __builtin_fabsf(sq);
-            const float p = __builtin_fmaf(qq, __builtin_fmaf(qq, f.wb2, f.wb1), f.w);
-            acc = __builtin_fmaf(sq, p, acc);
-            // z/sqrt(z^2 + d^2) - s(z) = -d^2 s / (r (r + |z|)), r = sqrt(z^2 + d^2)
-            const double z = __builtin_ldexp((double)(int64_t)(ut - us[j]), -64);
-            const double rz = __builtin_sqrt(__builtin_fma(z, z, d2));
-            ex += ws[j] * (-d2 * __builtin_copysign(1.0, z) / (rz * (rz + __builtin_fabs(z))));
-        }
-        const double e = 0.5 * (__builtin_ldexp((double)acc, 64) - ex);
-        for (int q = 0; q < kf; ++q) err[q] += e;
+        if (qx > 0)
+            for (int j = jb; j < je; ++j) {
+                const FarSrc f = far[j];
+                const float zs = (float)(int32_t)(ht - f.h);
+                const float r = __builtin_amdgcn_rcpf(zs);
+                const float sq = r * __builtin_fabsf(r);
+                const float qq = __builtin_fabsf(sq);
+                const float p = __builtin_fmaf(qq, __builtin_fmaf(qq, f.wb2, f.wb1), f.w);
+                acc = __builtin_fmaf(sq, p, acc);
+            }
+        const double ep = 0.5 * (__builtin_ldexp((double)acc, 64) - ex);
+        const double ee = (qx < kf) ? 0.5 * ((double)tile_value(T, ht, -32 - S.log2h) - ex) : 0.0;
+        for (int q = 0; q < kf; ++q) err[q] += (q < qx) ? ep : ee;
     }
 #pragma unroll
-    for (int q = 0; q < kLadder; ++q) red[q][threadIdx.x] = err[q];
+    for (int q = 0; q < kLadder; ++q) red[1 + q][threadIdx.x] = err[q];
+    red[0][threadIdx.x] = efield;
     __syncthreads();
     for (int h = kBlock / 2; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h)
 #pragma unroll
-            for (int q = 0; q < kLadder; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+            for (int q = 0; q < kGovVals; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
         __syncthreads();
     }
-    if ((int)threadIdx.x < kLadder) {
+    if ((int)threadIdx.x < kGovVals) {
         const double e = __builtin_fabs(red[threadIdx.x][0]);
-        atomicMax((unsigned long long*)&gov[(int64_t)d * kGovVals + 1 + threadIdx.x],
+        atomicMax((unsigned long long*)&gov[(int64_t)d * kGovVals + threadIdx.x],
                   (unsigned long long)__double_as_longlong(e));
     }
 }
 
-// Band of every evaluated segment from the governor state src (see above);
-// reset clears the state after reading it.
-__global__ void governor_kernel(int nseg, SegDev* __restrict__ segs, uint64_t* __restrict__ src,
-                                int reset, double target) {
+// Band of every evaluated segment from the probe's measurements (above).
+__global__ void governor_kernel(int nseg, SegDev* __restrict__ segs,
+                                const uint64_t* __restrict__ gov, double target) {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= nseg) return;
     SegDev& S = segs[d];
-    uint64_t* g = src + (int64_t)d * kGovVals;
-    const double em = __longlong_as_double((long long)g[0]);
-    double e[kLadder];
-    for (int q = 0; q < kLadder; ++q) e[q] = __longlong_as_double((long long)g[1 + q]);
-    if (reset)
-        for (int q = 0; q < kGovVals; ++q) g[q] = 0;
     if (!S.eval || S.all_band0) return;
+    const uint64_t* g = gov + (int64_t)d * kGovVals;
+    const double em = __longlong_as_double((long long)g[0]);
     uint64_t band = 0;
-    if (em > 0.0)
-        for (int q = 0; q < kLadder; ++q) {
-            const uint64_t b = ladder_band(S, q);
-            if (b == 0) break;
-            if (e[q] <= target * em) {
-                band = b;
-                break;
-            }
+    for (int q = 0; q < kLadder; ++q) {
+        const uint64_t b = ladder_band(S, q);
+        if (b == 0) break;
+        if (__longlong_as_double((long long)g[1 + q]) <= target * em) {
+            band = b;
+            break;
         }
+    }
     S.all_band = band == 0 ? 1 : 0;
     S.band = band == 0 ? (uint64_t(1) << 62) : band;
+    S.xband = S.band >> kXShift;
 }
 
 // Sum of the sorted weights of [off, off+n) whose key is <= key. The keys
@@ -632,27 +739,11 @@ __global__ __launch_bounds__(kBlock) void finalize_mixed_kernel(
     const uint64_t* __restrict__ us, const int32_t* __restrict__ perm,
     const double* __restrict__ ws, const double* __restrict__ pw,
     const double* __restrict__ totals, const double* __restrict__ part,
-    double* __restrict__ es, int* __restrict__ counter, uint64_t* __restrict__ emax) {
+    double* __restrict__ es, int* __restrict__ counter) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k == 0) *counter = 0;
-    const int d = seg_of[min(k, n_total - 1)];
-    const double e = (k < n_total) ? field_at(k, d, segs, us, perm, ws, pw, totals, part, es) : 0.0;
-    if (emax) {
-        // max |E| per segment (among non-negative doubles the bit patterns
-        // order as the values): lanes of the wave's first segment reduce in
-        // registers, one atomic per wave; lanes of other segments add theirs
-        const uint64_t eb = (uint64_t)__double_as_longlong(__builtin_fabs(e));
-        const int d0 = __shfl(d, 0);
-        uint64_t r = (d == d0) ? eb : 0;
-        for (int o = 32; o > 0; o >>= 1) {
-            const uint64_t v = __shfl_xor(r, o);
-            r = v > r ? v : r;
-        }
-        if ((threadIdx.x & 63) == 0 && r)
-            atomicMax((unsigned long long*)&emax[(int64_t)d0 * kGovVals], (unsigned long long)r);
-        if (d != d0 && eb)
-            atomicMax((unsigned long long*)&emax[(int64_t)d * kGovVals], (unsigned long long)eb);
-    }
+    if (k >= n_total) return;
+    (void)field_at(k, seg_of[k], segs, us, perm, ws, pw, totals, part, es);
 }
 
 __global__ void zero_int_kernel(int* p) { *p = 0; }
@@ -689,6 +780,16 @@ void chunking(int precision, int64_t nt, int64_t ns, int num_cus, const FieldTun
 
 }  // namespace
 
+double min_band(double delta) {
+    return std::max({std::ldexp(1.0, -6), 16.0 * delta, 4000.0 * delta * delta});
+}
+
+double governor_band(double delta, int q) {
+    double f = std::ldexp(1.0, (q >> 1) - 1);
+    if (q & 1) f *= 1.4142135623730951;
+    return std::max(min_band(delta) * f, 0.015625);
+}
+
 void FieldBatch::release() {
     tables_.release();
     work_.release();
@@ -704,8 +805,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         return fail(FS_E_INVALID, "field batch: unknown precision %d", precision);
     if (segs.empty() || n_total < 0) return fail(FS_E_INVALID, "field batch: no segments");
     precision_ = precision;
-    gov_src_ = nullptr;
-    gov_reset_ = gov_record_ = false;
+    governed_ = false;
     n_total_ = n_total;
     nseg_ = (int)segs.size();
     tpt_ = (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
@@ -749,11 +849,13 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         if (precision == kPrecisionMixed) {
             const double two64 = 18446744073709551616.0;
             const double d2 = S.delta2;
-            const double B = std::max({std::ldexp(1.0, -6), 16.0 * delta, 4000.0 * d2});
+            const double B = min_band(delta);
             S.all_band = (B >= 0.25) ? 1 : 0;
             S.band = S.all_band ? (uint64_t(1) << 62) : (uint64_t)std::ldexp(B, 64);
             S.all_band0 = S.all_band;
             S.band0 = S.band;
+            S.xband = S.band >> kXShift;
+            S.log2h = (int)std::floor(std::log2(B));
             S.d2s = d2 * two64 * two64;
             const double kq2 = d2 * two64;
             S.b1z = S.all_band ? 0.0 : -0.75 * kq2;
@@ -765,8 +867,14 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         std::vector<char> is_src(nseg_, 0);
         for (int d = 0; d < nseg_; ++d)
             if (segs[d].eval) is_src[segs[d].src] = 1;
+        ntiles_ = 0;
         for (int d = 0; d < nseg_; ++d) {
             segs_[d].is_src = is_src[d];
+            segs_[d].tile_base = ntiles_;
+            if (is_src[d]) ntiles_ += (segs_[d].n + kTile - 1) / kTile;
+        }
+        for (int d = 0; d < nseg_; ++d) segs_[d].src_tile_base = segs_[segs_[d].src].tile_base;
+        for (int d = 0; d < nseg_; ++d) {
             if (!is_src[d]) continue;
             const SegDev& S = segs_[d];
             const int first = (int)scan_.size();
@@ -818,6 +926,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         o_bsa_ = take(8 * std::max<size_t>(scan_.size(), 1));
         o_tot_ = take(8 * (size_t)nseg_);
         o_gov_ = take(8 * (size_t)nseg_ * kGovVals);
+        o_tiles_ = take(sizeof(TileRec) * (size_t)std::max(ntiles_, 1));
     }
     FS_TRY(work_.ensure(o, stream));
     if (precision == kPrecisionMixed) {
@@ -888,10 +997,11 @@ int FieldBatch::launch_pairs(int nit, int grid, hipStream_t stream) {
         const double* asm1 = reinterpret_cast<const double*>(wk + o_asm1_);
         const double* ws = reinterpret_cast<const double*>(wk + o_ws_);
         const FarSrc* far = reinterpret_cast<const FarSrc*>(wk + o_far_);
+        const TileRec* tiles = reinterpret_cast<const TileRec*>(wk + o_tiles_);
         switch (tpt_) {
-            case 1: batch_mixed_kernel<1><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, part, d_counter_); break;
-            case 4: batch_mixed_kernel<4><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, part, d_counter_); break;
-            default: batch_mixed_kernel<2><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, part, d_counter_); break;
+            case 1: batch_mixed_kernel<1><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, part, d_counter_); break;
+            case 4: batch_mixed_kernel<4><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, part, d_counter_); break;
+            default: batch_mixed_kernel<2><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, part, d_counter_); break;
         }
     }
     FS_HIP_CHECK(hipGetLastError());
@@ -902,17 +1012,13 @@ int FieldBatch::launch_pairs(int nit, int grid, hipStream_t stream) {
 // data and keeps the fastest. A trial runs all the launch's items with their
 // source ranges shortened to about 2e10 pairs in total. The shape changes
 // neither the items' summation order nor any result. es receives a full
-// evaluation (it is scratch here); the governor state is left as it was.
+// evaluation (it is scratch here) at the minimum band.
 int FieldBatch::autotune(const double* x, const double* w, double* es, hipStream_t stream,
                          int* best_tpt, int* best_wgs) {
-    const uint64_t* gsrc = gov_src_;
-    const bool greset = gov_reset_, grecord = gov_record_;
-    gov_src_ = nullptr;
-    gov_reset_ = gov_record_ = false;
+    const bool governed = governed_;
+    governed_ = false;
     const int rc0 = run(x, w, es, stream);  // sorted arrays and plan tables in place
-    gov_src_ = gsrc;
-    gov_reset_ = greset;
-    gov_record_ = grecord;
+    governed_ = governed;
     FS_TRY(rc0);
     static const int tpts[3] = {1, 2, 4};
     static const int wgss[3] = {2, 4, 8};
@@ -975,53 +1081,9 @@ int FieldBatch::autotune(const double* x, const double* w, double* es, hipStream
     return FS_OK;
 }
 
-void FieldBatch::govern(const uint64_t* src, bool reset, bool record, double target) {
-    const bool on = precision_ == kPrecisionMixed;
-    gov_src_ = on ? src : nullptr;
-    gov_reset_ = on && reset;
-    gov_record_ = on && record;
+void FieldBatch::govern(bool on, double target) {
+    governed_ = on && precision_ == kPrecisionMixed;
     gov_target_ = target > 0 ? target : kGovTarget;
-}
-
-// A recorded run at the minimum band; when its measured error exceeds the
-// target, the governor's band and the pair sums and finalize again (the
-// sorted arrays and prefix sums are those of the first run).
-int FieldBatch::run_checked(const double* x, const double* w, double* es, double target,
-                            hipStream_t stream) {
-    if (n_total_ == 0) return FS_OK;
-    if (precision_ != kPrecisionMixed) return run(x, w, es, stream);
-    auto body = [&]() -> int {
-        govern(nullptr, false, true, target);
-        zero_u64_kernel<<<nblocks(nseg_ * kGovVals), kBlock, 0, stream>>>(gov(), nseg_ * kGovVals);
-        FS_HIP_CHECK(hipGetLastError());
-        FS_TRY(run(x, w, es, stream));
-        std::vector<uint64_t> st;
-        FS_TRY(get_gov(st, stream));
-        bool again = false;
-        for (int d = 0; d < nseg_; ++d) {
-            if (!segs_[d].eval || segs_[d].all_band0 || segs_[d].n == 0) continue;
-            double em, e0;
-            std::memcpy(&em, &st[(size_t)d * kGovVals], 8);
-            std::memcpy(&e0, &st[(size_t)d * kGovVals + 1 + kLadderB0], 8);
-            again = again || !(e0 <= gov_target_ * em);
-        }
-        if (!again) return FS_OK;
-        governor_kernel<<<nblocks(nseg_), kBlock, 0, stream>>>(nseg_, d_segs_, gov(), 1,
-                                                              gov_target_);
-        FS_HIP_CHECK(hipGetLastError());
-        char* wk = static_cast<char*>(work_.ptr);
-        FS_TRY(launch_pairs((int)items_.size(), grid_, stream));
-        finalize_mixed_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
-            n_total_, d_seg_of_, d_segs_, reinterpret_cast<uint64_t*>(wk + o_skey_),
-            reinterpret_cast<int32_t*>(wk + o_perm_), reinterpret_cast<double*>(wk + o_ws_),
-            reinterpret_cast<double*>(wk + o_pw_), reinterpret_cast<double*>(wk + o_tot_),
-            reinterpret_cast<double*>(wk + o_part_), es, d_counter_, nullptr);
-        FS_HIP_CHECK(hipGetLastError());
-        return FS_OK;
-    };
-    const int rc = body();
-    govern(nullptr, false, false);
-    return rc;
 }
 
 uint64_t* FieldBatch::gov() const {
@@ -1029,22 +1091,11 @@ uint64_t* FieldBatch::gov() const {
     return reinterpret_cast<uint64_t*>(static_cast<char*>(work_.ptr) + o_gov_);
 }
 
-int FieldBatch::get_gov(std::vector<uint64_t>& out, hipStream_t stream) const {
-    out.assign((size_t)nseg_ * kGovVals, 0);
+int FieldBatch::get_gov(std::vector<double>& out, hipStream_t stream) const {
+    out.assign((size_t)nseg_ * kGovVals, 0.0);
     if (!gov() || nseg_ == 0) return FS_OK;
-    FS_HIP_CHECK(hipMemcpyAsync(out.data(), gov(), sizeof(uint64_t) * out.size(),
+    FS_HIP_CHECK(hipMemcpyAsync(out.data(), gov(), sizeof(double) * out.size(),
                                 hipMemcpyDeviceToHost, stream));
-    FS_HIP_CHECK(hipStreamSynchronize(stream));
-    return FS_OK;
-}
-
-int FieldBatch::set_gov(const std::vector<uint64_t>& v, hipStream_t stream) {
-    if (!gov() || nseg_ == 0) return FS_OK;
-    if (v.size() != (size_t)nseg_ * kGovVals)
-        return fail(FS_E_INVALID, "field batch: governor state of %zu values for %d segments",
-                    v.size(), nseg_);
-    FS_HIP_CHECK(hipMemcpyAsync(gov(), v.data(), sizeof(uint64_t) * v.size(),
-                                hipMemcpyHostToDevice, stream));
     FS_HIP_CHECK(hipStreamSynchronize(stream));
     return FS_OK;
 }
@@ -1080,6 +1131,10 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
     gather_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(n_total_, d_seg_of_, d_segs_, skey,
                                                             perm, w, as, asm1, ws, far);
     FS_HIP_CHECK(hipGetLastError());
+    TileRec* tiles = reinterpret_cast<TileRec*>(wk + o_tiles_);
+    tile_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(n_total_, d_seg_of_, d_segs_, skey, ws,
+                                                          tiles);
+    FS_HIP_CHECK(hipGetLastError());
     if (!scan_.empty()) {
         blocksum_kernel<<<(unsigned)scan_.size(), kBlock, 0, stream>>>(d_scan_, ws, as, bsum, bsa);
         FS_HIP_CHECK(hipGetLastError());
@@ -1087,19 +1142,18 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
                                                                    pw, tot);
         FS_HIP_CHECK(hipGetLastError());
     }
-    if (gov_src_) {
-        governor_kernel<<<nblocks(nseg_), kBlock, 0, stream>>>(
-            nseg_, d_segs_, const_cast<uint64_t*>(gov_src_), gov_reset_ ? 1 : 0, gov_target_);
+    if (governed_) {
+        zero_u64_kernel<<<nblocks(nseg_ * kGovVals), kBlock, 0, stream>>>(gov(), nseg_ * kGovVals);
+        FS_HIP_CHECK(hipGetLastError());
+        probe_kernel<<<dim3(kProbeMax, nseg_), kBlock, 0, stream>>>(d_segs_, skey, ws, far, tiles,
+                                                                   gov());
+        FS_HIP_CHECK(hipGetLastError());
+        governor_kernel<<<nblocks(nseg_), kBlock, 0, stream>>>(nseg_, d_segs_, gov(), gov_target_);
         FS_HIP_CHECK(hipGetLastError());
     }
     FS_TRY(launch_pairs(nit, grid_, stream));
-    if (gov_record_) {
-        probe_kernel<<<dim3(kProbeMax, nseg_), kBlock, 0, stream>>>(d_segs_, skey, ws, far, gov());
-        FS_HIP_CHECK(hipGetLastError());
-    }
     finalize_mixed_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
-        n_total_, d_seg_of_, d_segs_, skey, perm, ws, pw, tot, part, es, d_counter_,
-        gov_record_ ? gov() : nullptr);
+        n_total_, d_seg_of_, d_segs_, skey, perm, ws, pw, tot, part, es, d_counter_);
     FS_HIP_CHECK(hipGetLastError());
     return FS_OK;
 }

@@ -102,8 +102,7 @@ int fs_emq_periodic_mixed_dev(fs_ctx* ctx, double* d_es, const double* d_targets
         segs.push_back(a);
         FS_TRY(fb.plan(segs, (int)nt, fs::kPrecisionMixed, ctx->num_cus, ctx->tuning, st));
         FS_TRY(fs::shape(ctx, fb, nt, ns, true, d_sources, d_q_ws, d_es, st));
-        if (ctx->mixed_target > 0)
-            return fb.run_checked(d_sources, d_q_ws, d_es, ctx->mixed_target, st);
+        fb.govern(ctx->mixed_target > 0, ctx->mixed_target);
         return fb.run(d_sources, d_q_ws, d_es, st);
     }
     // sources then targets in one array; the targets' segment is sourced
@@ -133,10 +132,8 @@ int fs_emq_periodic_mixed_dev(fs_ctx* ctx, double* d_es, const double* d_targets
     FS_HIP_CHECK(hipMemcpyAsync(x + ns, d_targets, sizeof(double) * nt, hipMemcpyDeviceToDevice, st));
     FS_TRY(fb.plan(segs, n, fs::kPrecisionMixed, ctx->num_cus, ctx->tuning, st));
     FS_TRY(fs::shape(ctx, fb, nt, ns, false, x, d_q_ws, e, st));
-    if (ctx->mixed_target > 0)
-        FS_TRY(fb.run_checked(x, d_q_ws, e, ctx->mixed_target, st));
-    else
-        FS_TRY(fb.run(x, d_q_ws, e, st));
+    fb.govern(ctx->mixed_target > 0, ctx->mixed_target);
+    FS_TRY(fb.run(x, d_q_ws, e, st));
     FS_HIP_CHECK(hipMemcpyAsync(d_es, e + ns, sizeof(double) * nt, hipMemcpyDeviceToDevice, st));
     return FS_OK;
 }

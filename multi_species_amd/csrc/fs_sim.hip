@@ -28,7 +28,6 @@
 #pragma clang fp contract(off)
 
 #include <algorithm>
-#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -247,15 +246,7 @@ struct Species {
     }
 };
 
-inline uint64_t next_uid() {
-    static std::atomic<uint64_t> n{1};
-    return n.fetch_add(1);
-}
-
 struct Deck {
-    uint64_t uid = next_uid();
-    // mixed-precision governor state of its full field (empty: none yet)
-    std::vector<uint64_t> gov;
     fs_sim_params prm{};
     std::vector<std::unique_ptr<Species>> sp;
     int off = 0, n = 0;    // points
@@ -301,7 +292,6 @@ struct fs_sim {
     fs::sim::SpeciesInfo* sinfo = nullptr;
     int* d_status = nullptr;
     hipGraphExec_t g_remesh = nullptr, g_plain = nullptr;
-    std::vector<uint64_t> planned;  // deck uids in the segment order of the field plans
 
     hipStream_t stream() const { return ctx->stream; }
 };
@@ -459,15 +449,6 @@ int upload_species(Sim& S, const Deck& D, Species& s, bool grid_too) {
 int layout(Sim& S, bool keep_resident) {
     hipStream_t st = S.stream();
     drop_graphs(S);
-    {
-        // each deck keeps its governor state across the re-plan
-        std::vector<uint64_t> g;
-        FS_TRY(S.ffull.get_gov(g, st));
-        for (size_t i = 0; i < S.planned.size() && (i + 1) * fs::kGovVals <= g.size(); ++i)
-            for (auto& D : S.decks)
-                if (D->uid == S.planned[i])
-                    D->gov.assign(g.begin() + i * fs::kGovVals, g.begin() + (i + 1) * fs::kGovVals);
-    }
     S.sp.clear();
     S.any_adaptive = S.any_treecode = false;
     int N = 0;
@@ -655,25 +636,10 @@ int layout(Sim& S, bool keep_resident) {
     FS_TRY(S.ffull.plan(full, N, S.precision, S.ctx->num_cus, S.ctx->tuning, st));
     FS_TRY(S.fred.plan(redsg, S.U, S.precision, S.ctx->num_cus, S.ctx->tuning, st));
     FS_TRY(S.fred.set_shape(1, 8, st));
-    // Mixed precision: the full field sets each deck's FP64 band from the
-    // error its previous evaluation measured and records its own; the
-    // reduced field reads the full field's latest state without changing
-    // it, so evaluating it again (decks joining a running batch) reproduces
-    // it bit for bit. A deck's state is its own: it does not depend on the
-    // other decks of the batch.
-    S.planned.clear();
-    {
-        std::vector<uint64_t> g(full.size() * fs::kGovVals, 0);
-        for (size_t d = 0; d < S.decks.size(); ++d) {
-            S.planned.push_back(S.decks[d]->uid);
-            if (S.decks[d]->gov.size() == (size_t)fs::kGovVals)
-                std::copy(S.decks[d]->gov.begin(), S.decks[d]->gov.end(),
-                          g.begin() + d * fs::kGovVals);
-        }
-        FS_TRY(S.ffull.set_gov(g, st));
-    }
-    S.ffull.govern(S.ffull.gov(), true, true);
-    S.fred.govern(S.ffull.gov(), false, false);
+    // mixed precision: every evaluation measures its own FP32 error first
+    // and sets each deck's FP64 band from it
+    S.ffull.govern(true);
+    S.fred.govern(true);
     // launch shape of the full field: timed on this batch when its membership
     // or precision changed, reused when an adaptive remesh only resized it
     FS_HIP_CHECK(hipStreamSynchronize(st));
@@ -1237,6 +1203,23 @@ int fs_sim_get_points(fs_sim* S, int k, double* xs, double* ps, double* fs, doub
                       double* es) {
     if (!S || k < 0 || k >= (int)S->sp.size()) return fail(FS_E_INVALID, "bad species index");
     return sim::download_species(*S, k, xs, ps, fs, q_ws, es);
+}
+
+int fs_sim_field_governor(fs_sim* S, int deck, double* max_e, double* errs, double* bands) {
+    if (!S || deck < 0 || deck >= (int)S->decks.size()) return fail(FS_E_INVALID, "bad deck index");
+    std::vector<double> g;
+    FS_TRY(S->ffull.get_gov(g, S->stream()));
+    double v[fs::kGovVals] = {0};
+    if ((size_t)(deck + 1) * fs::kGovVals <= g.size())
+        std::memcpy(v, &g[(size_t)deck * fs::kGovVals], sizeof(v));
+    if (max_e) *max_e = v[0];
+    if (errs) std::memcpy(errs, v + 1, sizeof(double) * (fs::kGovVals - 1));
+    if (bands) {
+        const sim::Deck& D = *S->decks[deck];
+        for (int q = 0; q < fs::kGovVals - 1; ++q)
+            bands[q] = fs::governor_band(D.prm.greens_epsilon / D.Lx(), q);
+    }
+    return FS_OK;
 }
 
 int fs_sim_get_panels(fs_sim* S, int k, int32_t* panels) {

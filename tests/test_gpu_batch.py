@@ -8,8 +8,9 @@ Bars:
   host reads and eager work between replays;
 * the mixed batch against the FP64 runs of the same decks: E within 1e-8
   (relative, normwise) after the first step;
-* the governed step loop: the served E of weak decks within 1e-8 of the FP64
-  field of the same points, step after step;
+* the governed step loop: E of weak decks within 1e-8 of the FP64 field of
+  the same points, step after step, at moved points (full-field evaluation)
+  and after remesh;
 * continuous batching: a deck that joins a running batch evolves bitwise as
   it does alone; removing a deck leaves the others' states unchanged.
 """
@@ -109,12 +110,17 @@ def test_mixed_field_within_1e8_of_f64(gpu_ctx):
 
 
 def test_governed_step_loop_error(gpu_ctx):
-    """The mixed step loop widens each deck's FP64 band from its previous
-    max|E|: weak decks (max|E| small against the weights) keep the served E
-    within 1e-8 of the FP64 field of the same points, step after step."""
+    """The mixed step loop sets each deck's FP64 band from the error its
+    previous full-field evaluation measured. With remesh every second step,
+    E after an odd step is the loop's own full-field evaluation at moved
+    points (the positions an RK stage sees), after an even step the remesh
+    field: both within 1e-8 of the FP64 field of the same points, for weak
+    decks (max|E| small against the charges) and step after step."""
     decks = [landau(5, 1, amp=1e-3, k=0.5), landau(4, 1, amp=1e-2, k=0.5), two_stream(4, 0)]
+    for sim, _ in decks:
+        sim.remesh_period = 2
     m = ms.Simulation.batch(gpu_ctx, decks, ms.PRECISION_MIXED)
-    for step in range(4):
+    for step in range(6):
         m.step()
         for d, (sim, sps) in enumerate(decks):
             first, nsp, _, _ = m.deck_info(d)
@@ -125,7 +131,10 @@ def test_governed_step_loop_error(gpu_ctx):
             ef = gpu_ctx.emq_periodic_f64(x - sim.x_min, x - sim.x_min, w,
                                           sim.x_max - sim.x_min, sim.greens_epsilon)
             r = np.max(np.abs(es - ef)) / np.max(np.abs(ef))
-            print(f"step {step} deck {d}: served E vs FP64 of the same points {r:.2e}")
+            em, errs, bands = m.field_governor(d)
+            print(f"step {step + 1} deck {d}: E vs FP64 of the same points {r:.2e}; governor "
+                  f"max|E| {em:.3e} (FP64 {np.max(np.abs(ef)):.3e}), rel errors "
+                  + " ".join(f"{b:.4f}:{e / max(em, 1e-300):.1e}" for b, e in zip(bands, errs)))
             assert r <= 1e-8, (step, d)
     m.close()
 

@@ -163,9 +163,10 @@ def test_full_size_sampled(gpu_ctx):
 @pytest.mark.parametrize("amp", [1e-3, 1e-2])
 def test_error_target(gpu_ctx, amp):
     """Error target of the mixed calls: weak perturbations (max|E| small
-    against the weights) are where the minimum band's FP32 bound is loosest.
-    With the target the result stays within 1e-8 and no worse than one pass;
-    target 0 restores the one-pass result bit for bit."""
+    against the weights) are where the FP32 far field's error is largest
+    against the field. With the target (the band chosen from the call's own
+    probed error) the result stays within 1e-8; target 0 restores the
+    minimum-band result bit for bit."""
     import multi_species_amd as msa
     x, w = lattice_deck(0.3, 25.0, amp=amp)
     ex = oracle.direct_sum_ext(x, x, w, L0, 0.05)
@@ -178,9 +179,8 @@ def test_error_target(gpu_ctx, amp):
         gpu_ctx.set_mixed_target(0.0)
     e3 = gpu_ctx.emq_periodic_mixed(x, x, w, L0, 0.05, same=True)
     r0, r1, r2 = rel(e0, ex), rel(e1, ex), rel(e2, ex)
-    print(f"amp {amp}: one pass {r0:.2e}, target 4e-9: {r1:.2e} / {r2:.2e}")
+    print(f"amp {amp}: minimum band {r0:.2e}, target 4e-9: {r1:.2e} / {r2:.2e}")
     assert r1 <= TOL and r2 <= TOL
-    assert r1 <= r0 * 1.01 + 1e-12
     assert np.array_equal(e3, e0)
     for bad in (-1.0, 1.0):
         with pytest.raises(msa.FarrsightError):
