@@ -51,7 +51,17 @@ struct SegDev {
     uint64_t xband;   // far tiles whose sources lie within xband of their centre expand
     int tile_base;    // index of this segment's first source tile (source segments)
     int src_tile_base;
+    int group_base;   // index of this segment's first group of 4 tiles
+    int src_group_base;
     int log2h;        // expansion scale H = 2^log2h, the power of 2 at or below B0
+};
+
+// Span of 4 consecutive tiles (128 sorted sources) and their largest emax:
+// a group that is far and compact is summed without per-tile tests.
+struct GroupRec {
+    uint64_t vf, vl;  // first and last key, widened by the unsorted low bits
+    uint64_t emax;
+    uint64_t pad;
 };
 
 // Far-field expansion of one 32-source tile (mixed precision): with Z the
@@ -143,8 +153,8 @@ private:
     // work space offsets (bytes into work_)
     size_t o_part_ = 0, o_key_ = 0, o_skey_ = 0, o_perm_ = 0, o_sort_ = 0, o_as_ = 0,
            o_asm1_ = 0, o_ws_ = 0, o_far_ = 0, o_pw_ = 0, o_bsum_ = 0, o_bsa_ = 0, o_tot_ = 0,
-           o_gov_ = 0, o_tiles_ = 0;
-    int ntiles_ = 0;
+           o_gov_ = 0, o_tiles_ = 0, o_groups_ = 0;
+    int ntiles_ = 0, ngroups_ = 0;
 };
 
 // Stable radix sort by (segment of the original index, top 32 key bits)

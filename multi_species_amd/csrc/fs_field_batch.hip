@@ -47,6 +47,7 @@ constexpr uint64_t kHalf = 0x8000000000000000ull;
 constexpr uint64_t kAntiMargin = uint64_t(1) << 40;
 constexpr int kSortShift = 40;  // keys are sorted by bits 40..63 (fs_radix.hip)
 constexpr uint64_t kLowSort = (uint64_t(1) << kSortShift) - 1;
+constexpr int kGroupSrc = 4 * kTile;   // mixed: sources per group (the chunk unit)
 constexpr int kXShift = 4;             // tiles expand when their sources lie within band/16
 constexpr int kMoments = 8;            // moments of a tile's expansion
 constexpr int kCoef = kMoments + 4;    // its polynomial coefficients (TileRec::c)
@@ -270,6 +271,29 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t
     tiles[S.tile_base + (i - S.off) / kTile] = T;
 }
 
+// Span and largest tile emax of each group of 4 tiles (after tile_kernel).
+__global__ __launch_bounds__(kBlock) void group_kernel(int n_total, const int32_t* __restrict__ seg_of,
+                                                       const SegDev* __restrict__ segs,
+                                                       const uint64_t* __restrict__ us,
+                                                       const TileRec* __restrict__ tiles,
+                                                       GroupRec* __restrict__ groups) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_total) return;
+    const SegDev& S = segs[seg_of[i]];
+    if (!S.is_src || ((i - S.off) % kGroupSrc) != 0) return;
+    const int ge = min(i + kGroupSrc, S.off + S.n);
+    GroupRec G;
+    G.vf = us[i] & ~kLowSort;
+    G.vl = us[ge - 1] | kLowSort;
+    G.emax = 0;
+    G.pad = 0;
+    for (int t = (i - S.off) / kTile; t * kTile < ge - S.off; ++t) {
+        const uint64_t e = tiles[S.tile_base + t].emax;
+        G.emax = e > G.emax ? e : G.emax;
+    }
+    groups[S.group_base + (i - S.off) / kGroupSrc] = G;
+}
+
 // The FP32 value of one expanded tile at a target (high word ht): the pair
 // kernel and the probe evaluate it with the same operations.
 __device__ __forceinline__ float tile_value(const TileRec& T, uint32_t ht, int hexp) {
@@ -387,8 +411,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const WorkItem* __restrict__ items, int nitems, const SegDev* __restrict__ segs,
     const uint64_t* __restrict__ us, const double* __restrict__ as,
     const double* __restrict__ asm1, const double* __restrict__ ws,
-    const FarSrc* __restrict__ far, const TileRec* __restrict__ tiles, double* __restrict__ part,
-    int* __restrict__ counter) {
+    const FarSrc* __restrict__ far, const TileRec* __restrict__ tiles,
+    const GroupRec* __restrict__ groups, double* __restrict__ part, int* __restrict__ counter) {
     for (;;) {
         const int it = pull_item(counter);
         if (it >= nitems) break;
@@ -413,7 +437,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
             sl[t] = 0.0;
             gx[t] = 0.0;
         }
-        for (int jb = W.sb; jb < W.se; jb += kTile) {
+        for (int gb = W.sb; gb < W.se; gb += kGroupSrc) {
+          const int ge = min(W.se, gb + kGroupSrc);
+          if (!S.all_band && tspan < kHalf) {
+            // a group far from the whole target block, not antipodal, and of
+            // compact tiles: every tile takes its expansion
+            const GroupRec G = groups[S.src_group_base + (gb - S.src_off) / kGroupSrc];
+            const uint64_t gspan = G.vl - G.vf;
+            const uint64_t glen = tspan + gspan, glo = uf - G.vl;
+            if (gspan < kHalf && glen < kHalf && G.emax <= S.xband &&
+                !cyc_meets(glo + S.band, glen, 2 * S.band) &&
+                !cyc_meets(glo - kHalf + kAntiMargin, glen, 2 * kAntiMargin)) {
+                for (int jb = gb; jb < ge; jb += kTile) {
+                    const TileRec T = tiles[S.src_tile_base + (jb - S.src_off) / kTile];
+#pragma unroll
+                    for (int t = 0; t < TPT; ++t) gx[t] += (double)tile_value(T, hi[t], hexp);
+                }
+                continue;
+            }
+          }
+          for (int jb = gb; jb < ge; jb += kTile) {
             const int je = min(W.se, jb + kTile);
             const uint64_t vf = us[jb] & ~kLowSort, vl = us[je - 1] | kLowSort;
             const uint64_t vspan = vl - vf;
@@ -523,6 +566,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                 for (int t = 0; t < TPT; ++t)
                     two_sum_acc(sh[t], sl[t], __builtin_ldexp((double)acc[t], 64));
             }
+          }
         }
 #pragma unroll
         for (int t = 0; t < TPT; ++t) {
@@ -596,15 +640,16 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
     const int ntile = (R.n + kTile - 1) / kTile;
     for (int m = threadIdx.x; m < ntile; m += kBlock) {
         const int jb = R.off + m * kTile, je = min(jb + kTile, R.off + R.n);
-        // the exact field of the tile's sources: K(z) = (z/sqrt(z^2+d^2) - C z)/2,
-        // and its regularization remainder g(z) = z/sqrt(z^2+d^2) - s(z)
-        // = -d^2 s / (r (r + |z|)), r = sqrt(z^2 + d^2)
+        // the exact field of the tile's sources, K(z) = (z/sqrt(z^2+d^2) - C z)/2,
+        // and its regularization remainder g(z) = z/sqrt(z^2+d^2) - s(z) (for
+        // |z| >= 2^-6 the cancellation costs ~1e-16 d^-2 z^2 relative, far
+        // below the FP32 errors being measured)
         double ek = 0.0, ex = 0.0;
         for (int j = jb; j < je; ++j) {
             const double z = __builtin_ldexp((double)(int64_t)(ut - us[j]), -64);
-            const double rz = __builtin_sqrt(__builtin_fma(z, z, d2));
-            ek += ws[j] * (z / rz - S.C * z);
-            ex += ws[j] * (-d2 * __builtin_copysign(1.0, z) / (rz * (rz + __builtin_fabs(z))));
+            const double tz = z * rsq_f64(__builtin_fma(z, z, d2));
+            ek += ws[j] * (tz - S.C * z);
+            ex += ws[j] * (tz - __builtin_copysign(1.0, z));
         }
         efield += 0.5 * ek;
         const uint64_t vf = us[jb] & ~kLowSort, vl = us[je - 1] | kLowSort;
@@ -762,7 +807,7 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 void chunking(int precision, int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun,
               int& splits, int& chunk) {
     const int64_t want_waves = int64_t(64) * num_cus;
-    const int unit = precision == kPrecisionF64 ? kSub : kTile;
+    const int unit = precision == kPrecisionF64 ? kSub : kGroupSrc;
     const int tpt_auto = (nt >= want_waves * 64 * 2) ? 2 : 1;
     const int64_t waves = (nt + 64 * tpt_auto - 1) / (64 * tpt_auto);
     int64_t sp = tun.source_splits > 0 ? tun.source_splits
@@ -867,13 +912,20 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         std::vector<char> is_src(nseg_, 0);
         for (int d = 0; d < nseg_; ++d)
             if (segs[d].eval) is_src[segs[d].src] = 1;
-        ntiles_ = 0;
+        ntiles_ = ngroups_ = 0;
         for (int d = 0; d < nseg_; ++d) {
             segs_[d].is_src = is_src[d];
             segs_[d].tile_base = ntiles_;
-            if (is_src[d]) ntiles_ += (segs_[d].n + kTile - 1) / kTile;
+            segs_[d].group_base = ngroups_;
+            if (is_src[d]) {
+                ntiles_ += (segs_[d].n + kTile - 1) / kTile;
+                ngroups_ += (segs_[d].n + kGroupSrc - 1) / kGroupSrc;
+            }
         }
-        for (int d = 0; d < nseg_; ++d) segs_[d].src_tile_base = segs_[segs_[d].src].tile_base;
+        for (int d = 0; d < nseg_; ++d) {
+            segs_[d].src_tile_base = segs_[segs_[d].src].tile_base;
+            segs_[d].src_group_base = segs_[segs_[d].src].group_base;
+        }
         for (int d = 0; d < nseg_; ++d) {
             if (!is_src[d]) continue;
             const SegDev& S = segs_[d];
@@ -927,6 +979,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         o_tot_ = take(8 * (size_t)nseg_);
         o_gov_ = take(8 * (size_t)nseg_ * kGovVals);
         o_tiles_ = take(sizeof(TileRec) * (size_t)std::max(ntiles_, 1));
+        o_groups_ = take(sizeof(GroupRec) * (size_t)std::max(ngroups_, 1));
     }
     FS_TRY(work_.ensure(o, stream));
     if (precision == kPrecisionMixed) {
@@ -998,10 +1051,11 @@ int FieldBatch::launch_pairs(int nit, int grid, hipStream_t stream) {
         const double* ws = reinterpret_cast<const double*>(wk + o_ws_);
         const FarSrc* far = reinterpret_cast<const FarSrc*>(wk + o_far_);
         const TileRec* tiles = reinterpret_cast<const TileRec*>(wk + o_tiles_);
+        const GroupRec* groups = reinterpret_cast<const GroupRec*>(wk + o_groups_);
         switch (tpt_) {
-            case 1: batch_mixed_kernel<1><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, part, d_counter_); break;
-            case 4: batch_mixed_kernel<4><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, part, d_counter_); break;
-            default: batch_mixed_kernel<2><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, part, d_counter_); break;
+            case 1: batch_mixed_kernel<1><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, part, d_counter_); break;
+            case 4: batch_mixed_kernel<4><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, part, d_counter_); break;
+            default: batch_mixed_kernel<2><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, part, d_counter_); break;
         }
     }
     FS_HIP_CHECK(hipGetLastError());
@@ -1134,6 +1188,9 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
     TileRec* tiles = reinterpret_cast<TileRec*>(wk + o_tiles_);
     tile_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(n_total_, d_seg_of_, d_segs_, skey, ws,
                                                           tiles);
+    FS_HIP_CHECK(hipGetLastError());
+    group_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
+        n_total_, d_seg_of_, d_segs_, skey, tiles, reinterpret_cast<GroupRec*>(wk + o_groups_));
     FS_HIP_CHECK(hipGetLastError());
     if (!scan_.empty()) {
         blocksum_kernel<<<(unsigned)scan_.size(), kBlock, 0, stream>>>(d_scan_, ws, as, bsum, bsa);
