@@ -93,10 +93,11 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--eps", type=float, default=0.05)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["field", "c1", "c4", "c5"], default="field",
+    ap.add_argument("--workload", choices=["field", "c1", "c3", "c4", "c5"], default="field",
                     help="field: N=2^20 direct-sum evaluation (default); c1: the two-species "
                          "weak-Landau deck, 65x129 points per species, RK4 dt=0.1 with a remesh "
-                         "every step (a step = one RK4 step + remesh); c4: 64 independent "
+                         "every step (a step = one RK4 step + remesh); c3: adaptive two-stream "
+                         "with ions and the treecode; c4: 64 independent "
                          "two-species Landau decks (16x32 to 128x256 panels) stepped as one "
                          "batch in mixed precision, against the same decks one by one in FP64; "
                          "c5: Poisson arrivals of decks served with continuous batching")
@@ -130,6 +131,8 @@ def main():
 
     if args.workload == "c1":
         return bench_c1(args)
+    if args.workload == "c3":
+        return bench_c3(args)
     if args.workload == "c4":
         return bench_c4(args)
     if args.workload == "c5":
@@ -321,6 +324,76 @@ def bench_c1(args):
     sim.close()
     ctx.close()
     print(json.dumps(result))
+
+
+def c3_deck(treecode: bool = True):
+    """Adaptive two-stream with ions: electron beams at v = +-2.4 (vth 0.5) with
+    amplitude 1e-3 cos(0.2x) on [0, 10pi), ions of mass ratio 100 (vth 0.1); base
+    64x128 panels (129x257 points) per species, electrons refined up to three levels
+    beyond initial_height 6 (max_height 9) where a panel's f varies by more than 1e-3
+    (290,837 electron points at the start); the treecode with theta 0.7, degree 8,
+    leaves under 2000 points; dt 0.05."""
+    import multi_species_amd as ms
+
+    sim = ms.sim_params(x_min=0.0, x_max=10 * np.pi, bcs=0, quadrature=0, greens_epsilon=0.05,
+                        use_treecode=1 if treecode else 0, mac=0.7, degree=8, max_source=2000,
+                        dt=0.05, remesh_period=1)
+    e = ms.species_params(q=-1.0, m=1.0, p_min=-6.0, p_max=6.0, initial_height=6, p_height=1,
+                          max_height=9, do_adaptively_refine=True, amr_epsilons=[1e-3],
+                          ics_type=4, normalized_wavenumber=1.0, amp=1e-3, pth=0.5, pstr=2.4)
+    i = ms.species_params(q=1.0, m=100.0, p_min=-6.0, p_max=6.0, initial_height=6, p_height=1,
+                          ics_type=1, normalized_wavenumber=1.0, amp=0.0, pth=0.1)
+    return sim, [e, i]
+
+
+def bench_c3(args):
+    """The adaptive two-stream deck stepped with the treecode field (host tree,
+    device moments and sums) and adaptive remesh; at the end the treecode E of the
+    final state against the FP64 direct sum of the same points."""
+    import multi_species_amd as ms
+
+    ctx = ms.Context(0)
+    s = ms.Simulation(ctx, *c3_deck())
+    counts0 = [s.counts(k)[0] for k in range(2)]
+    for _ in range(args.warmup):
+        s.step()
+    lat, pts = [], 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        n = sum(s.counts(k)[0] for k in range(2))
+        t1 = time.perf_counter()
+        s.step()
+        lat.append(time.perf_counter() - t1)
+        pts += n
+    wall = time.perf_counter() - t0
+    counts = [s.counts(k)[0] for k in range(2)]
+    sim_p = c3_deck()[0]
+    L = sim_p.x_max - sim_p.x_min
+    P = [s.points(k) for k in range(2)]
+    x = np.concatenate([p[0] for p in P])
+    w = np.concatenate([p[3] for p in P])
+    es = np.concatenate([p[4] for p in P])
+    ef = ctx.emq_periodic_f64(x, x, w, L, sim_p.greens_epsilon)
+    err = float(np.max(np.abs(es - ef)) / np.max(np.abs(ef)))
+    s.close()
+    ctx.close()
+    lat = np.array(lat) * 1e3
+    print(json.dumps({
+        "metric": "RK4 point-steps/s",
+        "value": pts / sum(lat) * 1e3,
+        "unit": "point-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": float(np.mean(lat)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "adaptive two-stream with ions (beams +-2.4, vth 0.5, amp 1e-3, "
+                               "L=10pi; ions m=100), 129x257 base points per species, electrons "
+                               "refined up to 3 levels on panel variation > 1e-3, treecode "
+                               "theta 0.7 degree 8 max_source 2000, dt 0.05, remesh every step",
+                   "points_start": counts0, "points_end": counts},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "treecode_rel_e_err_vs_direct_f64": err,
+    }))
 
 
 def c4_decks(n: int, seed: int = 4):

@@ -110,3 +110,28 @@ def test_treecode_deck_matches_oracle(gpu_ctx):
         gsim.step()
         compare(gsim, osim, 2, step)
     gsim.close()
+
+
+def test_adaptive_two_stream_treecode_matches_oracle(gpu_ctx):
+    """The shape of the adaptive two-stream workload at reduced size: beams at
+    +-2.4 (vth 0.5) with 1e-3 cos(0.2x) on [0, 10pi), ions of mass ratio 100,
+    refinement on the panel variation, and the treecode (theta 0.7, degree 8)
+    for every field evaluation."""
+    e = oracle.Species(name="electrons", q=-1.0, m=1.0, p_min=-6.0, p_max=6.0,
+                       initial_height=4, p_height=1, max_height=6,
+                       do_adaptively_refine=True, amr_epsilons=[1e-3], ics_type=4,
+                       normalized_wavenumber=1.0, amp=1e-3, pth=0.5, pstr=2.4)
+    i = oracle.Species(name="ions", q=1.0, m=100.0, p_min=-6.0, p_max=6.0, initial_height=4,
+                       p_height=1, ics_type=1, amp=0.0, pth=0.1)
+    deck = oracle.Deck(x_min=0.0, x_max=10 * np.pi, quadrature=0, greens_epsilon=0.05,
+                       use_treecode=1, mac=0.7, degree=8, max_source=300,
+                       dt=0.05, remesh_period=1, species=[e, i])
+    osim = oracle.OracleSim(deck)
+    gsim = ms.Simulation(gpu_ctx, *to_ms(deck))
+    assert gsim.counts(0)[0] > 33 * 65
+    compare(gsim, osim, 2, 0)
+    for step in range(1, 4):
+        osim.step()
+        gsim.step()
+        compare(gsim, osim, 2, step)
+    gsim.close()
