@@ -74,6 +74,15 @@ struct TileRec {
     float c[12];
 };
 
+// The accuracy probe's FP64 reference for a compact far tile: the same
+// expansion carried to 7 terms of the series in d^2/z^2 and 12 moments (so
+// that it sees the truncations of the FP32 expansion as well as its
+// rounding), and the tile's moments sum w and sum w e/H.
+struct TileRec64 {
+    double c[24];
+    double m0, m1;
+};
+
 // Far-field record of one sorted source (FP32 path of the mixed kernel).
 struct FarSrc {
     uint32_t h;             // high word of the position
@@ -153,7 +162,7 @@ private:
     // work space offsets (bytes into work_)
     size_t o_part_ = 0, o_key_ = 0, o_skey_ = 0, o_perm_ = 0, o_sort_ = 0, o_as_ = 0,
            o_asm1_ = 0, o_ws_ = 0, o_far_ = 0, o_pw_ = 0, o_bsum_ = 0, o_bsa_ = 0, o_tot_ = 0,
-           o_gov_ = 0, o_tiles_ = 0, o_groups_ = 0;
+           o_gov_ = 0, o_tiles_ = 0, o_groups_ = 0, o_tiles64_ = 0;
     int ntiles_ = 0, ngroups_ = 0;
 };
 

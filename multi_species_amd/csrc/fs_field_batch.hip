@@ -51,6 +51,9 @@ constexpr int kGroupSrc = 4 * kTile;   // mixed: sources per group (the chunk un
 constexpr int kXShift = 4;             // tiles expand when their sources lie within band/16
 constexpr int kMoments = 8;            // moments of a tile's expansion
 constexpr int kCoef = kMoments + 4;    // its polynomial coefficients (TileRec::c)
+constexpr int kMoments64 = 12;         // the probe's FP64 reference: moments,
+constexpr int kTerms64 = 7;            // terms of the series in d^2/z^2,
+constexpr int kCoef64 = 2 * kTerms64 + kMoments64 - 2;  // and coefficients (TileRec64::c)
 constexpr int kLadder = kGovVals - 1;  // governor: candidate bands
 constexpr int kProbeMax = 256;         // governor: probed targets per segment
 constexpr double kGovTarget = 4e-9;    // governor: default error target (relative to max |E|)
@@ -216,7 +219,8 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t
                                                       const SegDev* __restrict__ segs,
                                                       const uint64_t* __restrict__ us,
                                                       const double* __restrict__ ws,
-                                                      TileRec* __restrict__ tiles) {
+                                                      TileRec* __restrict__ tiles,
+                                                      TileRec64* __restrict__ tiles64) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n_total) return;
     const SegDev& S = segs[seg_of[i]];
@@ -232,9 +236,9 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t
     const uint32_t hc = (uint32_t)(mid >> 32);
     const uint64_t uc = (uint64_t)hc << 32;
     uint64_t emax = 0;
-    double M[kMoments];
+    double M[kMoments64];
 #pragma unroll
-    for (int k = 0; k < kMoments; ++k) M[k] = 0.0;
+    for (int k = 0; k < kMoments64; ++k) M[k] = 0.0;
     for (int j = i; j < je; ++j) {
         const int64_t e = (int64_t)(uc - us[j]);
         const uint64_t ae = (uint64_t)(e < 0 ? -e : e);
@@ -242,33 +246,46 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t
         const double eh = __builtin_ldexp((double)e, -64 - S.log2h);
         double pw = ws[j];
 #pragma unroll
-        for (int k = 0; k < kMoments; ++k) {
+        for (int k = 0; k < kMoments64; ++k) {
             M[k] += pw;
             pw *= eh;
         }
     }
+    // a_i (d/H)^2i: the binomial series (1+q)^-1/2 - 1 = sum_i C(-1/2, i) q^i
     const double dh2 = __builtin_ldexp(S.delta2, -2 * S.log2h);  // (d/H)^2
-    const double a[3] = {-0.5 * dh2, 0.375 * dh2 * dh2, -0.3125 * dh2 * dh2 * dh2};
+    double a[kTerms64];
+    {
+        double c = 1.0, p = 1.0;
+        for (int ii = 1; ii <= kTerms64; ++ii) {
+            c = -c * (2.0 * ii - 1.0) / (2.0 * ii);  // C(-1/2, i), exact (dyadic)
+            p *= dh2;
+            a[ii - 1] = c * p;
+        }
+    }
     TileRec T;
+    TileRec64 T64;
     T.hc = hc;
     T.pad = 0;
     T.emax = emax;
-#pragma unroll
-    for (int m = 0; m < kCoef; ++m) {
-        // n = m + 2 = 2i + k (i = 1..3)
-        double c = 0.0;
-#pragma unroll
-        for (int ii = 1; ii <= 3; ++ii) {
+    T64.m0 = M[0];
+    T64.m1 = M[1];
+    // coefficient of rho^(m+2): n = m + 2 = 2i + k
+    for (int m = 0; m < kCoef64; ++m) {
+        double c32 = 0.0, c64 = 0.0;
+        for (int ii = 1; ii <= kTerms64; ++ii) {
             const int k = m + 2 - 2 * ii;
-            if (k < 0 || k >= kMoments) continue;
-            // C(2i+k-1, k)
-            double b = 1.0;
+            if (k < 0 || k >= kMoments64) continue;
+            double b = 1.0;  // C(2i+k-1, k)
             for (int q = 1; q <= k; ++q) b = b * (2 * ii - 1 + q) / q;
-            c += a[ii - 1] * b * ((k & 1) ? -M[k] : M[k]);
+            const double t = a[ii - 1] * b * ((k & 1) ? -M[k] : M[k]);
+            c64 += t;
+            if (ii <= 3 && k < kMoments) c32 += t;
         }
-        T.c[m] = (float)c;
+        if (m < kCoef) T.c[m] = (float)c32;
+        T64.c[m] = c64;
     }
     tiles[S.tile_base + (i - S.off) / kTile] = T;
+    tiles64[S.tile_base + (i - S.off) / kTile] = T64;
 }
 
 // Span and largest tile emax of each group of 4 tiles (after tile_kernel).
@@ -616,6 +633,7 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
                                                        const double* __restrict__ ws,
                                                        const FarSrc* __restrict__ far,
                                                        const TileRec* __restrict__ tiles,
+                                                       const TileRec64* __restrict__ tiles64,
                                                        uint64_t* __restrict__ gov) {
     __shared__ double red[kGovVals][kBlock];
     const int d = blockIdx.y;
@@ -638,33 +656,53 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
     const SegDev R = segs[S.src];
     const double d2 = S.delta2;
     const int ntile = (R.n + kTile - 1) / kTile;
+    const double H = __builtin_ldexp(1.0, S.log2h);
     for (int m = threadIdx.x; m < ntile; m += kBlock) {
         const int jb = R.off + m * kTile, je = min(jb + kTile, R.off + R.n);
-        // the exact field of the tile's sources, K(z) = (z/sqrt(z^2+d^2) - C z)/2,
-        // and its regularization remainder g(z) = z/sqrt(z^2+d^2) - s(z) (for
-        // |z| >= 2^-6 the cancellation costs ~1e-16 d^-2 z^2 relative, far
-        // below the FP32 errors being measured)
-        double ek = 0.0, ex = 0.0;
-        for (int j = jb; j < je; ++j) {
-            const double z = __builtin_ldexp((double)(int64_t)(ut - us[j]), -64);
-            const double tz = z * rsq_f64(__builtin_fma(z, z, d2));
-            ek += ws[j] * (tz - S.C * z);
-            ex += ws[j] * (tz - __builtin_copysign(1.0, z));
-        }
-        efield += 0.5 * ek;
         const uint64_t vf = us[jb] & ~kLowSort, vl = us[je - 1] | kLowSort;
         const uint64_t vspan = vl - vf;
-        if (tspan >= kHalf || vspan >= kHalf || tspan + vspan >= kHalf) continue;
+        const bool wide = tspan >= kHalf || vspan >= kHalf || tspan + vspan >= kHalf;
         const uint64_t len = tspan + vspan;
         const uint64_t lo = uf - vl;
-        if (cyc_meets(lo - kHalf + kAntiMargin, len, 2 * kAntiMargin)) continue;
+        const bool anti = !wide && cyc_meets(lo - kHalf + kAntiMargin, len, 2 * kAntiMargin);
         int kf = 0;
-        while (kf < kLadder && band[kf] != 0 && !cyc_meets(lo + band[kf], len, 2 * band[kf])) ++kf;
-        if (kf == 0) continue;
-        // the tile expands at the bands q >= qx and takes the pair path below
+        if (!wide && !anti)
+            while (kf < kLadder && band[kf] != 0 && !cyc_meets(lo + band[kf], len, 2 * band[kf]))
+                ++kf;
         const TileRec T = tiles[R.tile_base + m];
+        // the tile expands at the bands q >= qx and takes the pair path below
         int qx = 0;
         while (qx < kf && T.emax > (band[qx] >> kXShift)) ++qx;
+        // the exact field of the tile's sources, K(z) = (z/sqrt(z^2+d^2) - C z)/2
+        // = ((s - C z) + g)/2, and its regularization remainder g: for a
+        // compact far tile from the FP64 reference expansion (7 series terms,
+        // 12 moments: truncation below 1e-12 of the remainder at |z| >= 8d
+        // and e/|Z| <= 1/15, so the FP32 expansion's own truncations show in
+        // the measured error) and moments, otherwise pair by pair (for
+        // |z| >= 2^-6 the cancellation in z/sqrt(z^2+d^2) - s costs
+        // ~1e-16 d^-2 z^2 relative, far below the FP32 errors being measured)
+        double ek, ex;
+        if (kf > 0 && qx == 0) {
+            const TileRec64 T64 = tiles64[R.tile_base + m];
+            const double Z = __builtin_ldexp((double)(int64_t)(ut - ((uint64_t)T.hc << 32)), -64);
+            const double rho = H / Z;
+            double p = T64.c[kCoef64 - 1];
+#pragma unroll
+            for (int c = kCoef64 - 2; c >= 0; --c) p = __builtin_fma(p, rho, T64.c[c]);
+            ex = rho * __builtin_fabs(rho) * p;
+            ek = __builtin_copysign(1.0, Z) * T64.m0 - S.C * (Z * T64.m0 + H * T64.m1) + ex;
+        } else {
+            ek = 0.0;
+            ex = 0.0;
+            for (int j = jb; j < je; ++j) {
+                const double z = __builtin_ldexp((double)(int64_t)(ut - us[j]), -64);
+                const double tz = z * rsq_f64(__builtin_fma(z, z, d2));
+                ek += ws[j] * (tz - S.C * z);
+                ex += ws[j] * (tz - __builtin_copysign(1.0, z));
+            }
+        }
+        efield += 0.5 * ek;
+        if (kf == 0) continue;
         float acc = 0.0f;
         if (qx > 0)
             for (int j = jb; j < je; ++j) {
@@ -980,6 +1018,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         o_gov_ = take(8 * (size_t)nseg_ * kGovVals);
         o_tiles_ = take(sizeof(TileRec) * (size_t)std::max(ntiles_, 1));
         o_groups_ = take(sizeof(GroupRec) * (size_t)std::max(ngroups_, 1));
+        o_tiles64_ = take(sizeof(TileRec64) * (size_t)std::max(ntiles_, 1));
     }
     FS_TRY(work_.ensure(o, stream));
     if (precision == kPrecisionMixed) {
@@ -1186,8 +1225,8 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
                                                             perm, w, as, asm1, ws, far);
     FS_HIP_CHECK(hipGetLastError());
     TileRec* tiles = reinterpret_cast<TileRec*>(wk + o_tiles_);
-    tile_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(n_total_, d_seg_of_, d_segs_, skey, ws,
-                                                          tiles);
+    tile_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
+        n_total_, d_seg_of_, d_segs_, skey, ws, tiles, reinterpret_cast<TileRec64*>(wk + o_tiles64_));
     FS_HIP_CHECK(hipGetLastError());
     group_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
         n_total_, d_seg_of_, d_segs_, skey, tiles, reinterpret_cast<GroupRec*>(wk + o_groups_));
@@ -1202,8 +1241,8 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
     if (governed_) {
         zero_u64_kernel<<<nblocks(nseg_ * kGovVals), kBlock, 0, stream>>>(gov(), nseg_ * kGovVals);
         FS_HIP_CHECK(hipGetLastError());
-        probe_kernel<<<dim3(kProbeMax, nseg_), kBlock, 0, stream>>>(d_segs_, skey, ws, far, tiles,
-                                                                   gov());
+        probe_kernel<<<dim3(kProbeMax, nseg_), kBlock, 0, stream>>>(
+            d_segs_, skey, ws, far, tiles, reinterpret_cast<TileRec64*>(wk + o_tiles64_), gov());
         FS_HIP_CHECK(hipGetLastError());
         governor_kernel<<<nblocks(nseg_), kBlock, 0, stream>>>(nseg_, d_segs_, gov(), gov_target_);
         FS_HIP_CHECK(hipGetLastError());
