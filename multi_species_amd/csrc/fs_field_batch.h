@@ -49,6 +49,7 @@ struct SegDev {
     int all_band0;    // all_band of the minimum band (no governor)
     uint64_t band0;   // the minimum band B0 * 2^64
     uint64_t xband;   // far tiles whose sources lie within xband of their centre expand
+    uint64_t nxband;  // near tiles expand within d/16 of their centre (FP64)
     int tile_base;    // index of this segment's first source tile (source segments)
     int src_tile_base;
     int group_base;   // index of this segment's first group of 4 tiles
@@ -79,8 +80,16 @@ struct TileRec {
 // that it sees the truncations of the FP32 expansion as well as its
 // rounding), and the tile's moments sum w and sum w e/H.
 struct TileRec64 {
-    double c[24];
+    double c[25];
     double m0, m1;
+};
+
+// Moments sum_j w_j e_j^k (k = 0..12, e = centre - source) of one tile: the
+// FP64 near-band expansion of a compact tile on one side of its targets,
+// sum_j w_j z_j / sqrt(z_j^2 + d^2) = sum_k P_k(-Z/r) r^-(k+1) (Z M_k + M_k+1)
+// with r = sqrt(Z^2 + d^2) (the Legendre generating function).
+struct TileMom {
+    double m[13];
 };
 
 // Far-field record of one sorted source (FP32 path of the mixed kernel).
@@ -162,7 +171,7 @@ private:
     // work space offsets (bytes into work_)
     size_t o_part_ = 0, o_key_ = 0, o_skey_ = 0, o_perm_ = 0, o_sort_ = 0, o_as_ = 0,
            o_asm1_ = 0, o_ws_ = 0, o_far_ = 0, o_pw_ = 0, o_bsum_ = 0, o_bsa_ = 0, o_tot_ = 0,
-           o_gov_ = 0, o_tiles_ = 0, o_groups_ = 0, o_tiles64_ = 0;
+           o_gov_ = 0, o_tiles_ = 0, o_groups_ = 0, o_tiles64_ = 0, o_moms_ = 0;
     int ntiles_ = 0, ngroups_ = 0;
 };
 

@@ -51,9 +51,10 @@ constexpr int kGroupSrc = 4 * kTile;   // mixed: sources per group (the chunk un
 constexpr int kXShift = 4;             // tiles expand when their sources lie within band/16
 constexpr int kMoments = 8;            // moments of a tile's expansion
 constexpr int kCoef = kMoments + 4;    // its polynomial coefficients (TileRec::c)
-constexpr int kMoments64 = 12;         // the probe's FP64 reference: moments,
+constexpr int kMoments64 = 13;         // the probe's FP64 reference: moments,
 constexpr int kTerms64 = 7;            // terms of the series in d^2/z^2,
 constexpr int kCoef64 = 2 * kTerms64 + kMoments64 - 2;  // and coefficients (TileRec64::c)
+constexpr int kNearTerms = 12;         // near-band FP64 expansion: Legendre terms
 constexpr int kLadder = kGovVals - 1;  // governor: candidate bands
 constexpr int kProbeMax = 256;         // governor: probed targets per segment
 constexpr double kGovTarget = 4e-9;    // governor: default error target (relative to max |E|)
@@ -220,7 +221,8 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t
                                                       const uint64_t* __restrict__ us,
                                                       const double* __restrict__ ws,
                                                       TileRec* __restrict__ tiles,
-                                                      TileRec64* __restrict__ tiles64) {
+                                                      TileRec64* __restrict__ tiles64,
+                                                      TileMom* __restrict__ moms) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n_total) return;
     const SegDev& S = segs[seg_of[i]];
@@ -286,6 +288,9 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t
     }
     tiles[S.tile_base + (i - S.off) / kTile] = T;
     tiles64[S.tile_base + (i - S.off) / kTile] = T64;
+    TileMom N;
+    for (int k = 0; k < kMoments64; ++k) N.m[k] = __builtin_ldexp(M[k], k * S.log2h);
+    moms[S.tile_base + (i - S.off) / kTile] = N;
 }
 
 // Span and largest tile emax of each group of 4 tiles (after tile_kernel).
@@ -429,7 +434,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const uint64_t* __restrict__ us, const double* __restrict__ as,
     const double* __restrict__ asm1, const double* __restrict__ ws,
     const FarSrc* __restrict__ far, const TileRec* __restrict__ tiles,
-    const GroupRec* __restrict__ groups, double* __restrict__ part, int* __restrict__ counter) {
+    const GroupRec* __restrict__ groups, const TileMom* __restrict__ moms,
+    double* __restrict__ part, int* __restrict__ counter) {
     for (;;) {
         const int it = pull_item(counter);
         if (it >= nitems) break;
@@ -501,6 +507,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                 }
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], acc[t]);
+            } else if (near && tiles[S.src_tile_base + (jb - S.src_off) / kTile].emax <= S.nxband &&
+                       !cyc_meets(lo, len, 1)) {
+                // compact near tile on one side of the whole target block: its
+                // FP64 expansion about the centre (Legendre generating function;
+                // truncation below 16^-12 of sum |w|), and s = the side
+                const int tix = S.src_tile_base + (jb - S.src_off) / kTile;
+                const uint64_t uc = (uint64_t)tiles[tix].hc << 32;
+                const TileMom N = moms[tix];
+                const double s = (lo < kHalf) ? 1.0 : -1.0;
+#pragma unroll
+                for (int t = 0; t < TPT; ++t) {
+                    const double Z = __builtin_ldexp((double)(int64_t)(ui[t] - uc), -64);
+                    const double ri = rsq_f64(__builtin_fma(Z, Z, d2));
+                    const double x = -Z * ri;
+                    double pk = x, pkm1 = 1.0, rp = ri;
+                    double acc = rp * __builtin_fma(Z, N.m[0], N.m[1]);
+#pragma unroll
+                    for (int k = 1; k < kNearTerms; ++k) {
+                        rp *= ri;
+                        acc = __builtin_fma(pk * rp, __builtin_fma(Z, N.m[k], N.m[k + 1]), acc);
+                        const double pk1 = ((2 * k + 1) * x * pk - k * pkm1) * (1.0 / (k + 1));
+                        pkm1 = pk;
+                        pk = pk1;
+                    }
+                    two_sum_acc(sh[t], sl[t], acc - s * N.m[0]);
+                }
             } else if (near) {
                 // no displacement of this tile reaches 1/2, so each pair's
                 // wrap is the tile's: targets above the sources by more than
@@ -939,6 +971,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
             S.band0 = S.band;
             S.xband = S.band >> kXShift;
             S.log2h = (int)std::floor(std::log2(B));
+            S.nxband = (uint64_t)std::ldexp(delta, 64 - 4);
             S.d2s = d2 * two64 * two64;
             const double kq2 = d2 * two64;
             S.b1z = S.all_band ? 0.0 : -0.75 * kq2;
@@ -1019,6 +1052,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         o_tiles_ = take(sizeof(TileRec) * (size_t)std::max(ntiles_, 1));
         o_groups_ = take(sizeof(GroupRec) * (size_t)std::max(ngroups_, 1));
         o_tiles64_ = take(sizeof(TileRec64) * (size_t)std::max(ntiles_, 1));
+        o_moms_ = take(sizeof(TileMom) * (size_t)std::max(ntiles_, 1));
     }
     FS_TRY(work_.ensure(o, stream));
     if (precision == kPrecisionMixed) {
@@ -1091,10 +1125,11 @@ int FieldBatch::launch_pairs(int nit, int grid, hipStream_t stream) {
         const FarSrc* far = reinterpret_cast<const FarSrc*>(wk + o_far_);
         const TileRec* tiles = reinterpret_cast<const TileRec*>(wk + o_tiles_);
         const GroupRec* groups = reinterpret_cast<const GroupRec*>(wk + o_groups_);
+        const TileMom* moms = reinterpret_cast<const TileMom*>(wk + o_moms_);
         switch (tpt_) {
-            case 1: batch_mixed_kernel<1><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, part, d_counter_); break;
-            case 4: batch_mixed_kernel<4><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, part, d_counter_); break;
-            default: batch_mixed_kernel<2><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, part, d_counter_); break;
+            case 1: batch_mixed_kernel<1><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, part, d_counter_); break;
+            case 4: batch_mixed_kernel<4><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, part, d_counter_); break;
+            default: batch_mixed_kernel<2><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, part, d_counter_); break;
         }
     }
     FS_HIP_CHECK(hipGetLastError());
@@ -1226,7 +1261,8 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
     FS_HIP_CHECK(hipGetLastError());
     TileRec* tiles = reinterpret_cast<TileRec*>(wk + o_tiles_);
     tile_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
-        n_total_, d_seg_of_, d_segs_, skey, ws, tiles, reinterpret_cast<TileRec64*>(wk + o_tiles64_));
+        n_total_, d_seg_of_, d_segs_, skey, ws, tiles, reinterpret_cast<TileRec64*>(wk + o_tiles64_),
+        reinterpret_cast<TileMom*>(wk + o_moms_));
     FS_HIP_CHECK(hipGetLastError());
     group_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
         n_total_, d_seg_of_, d_segs_, skey, tiles, reinterpret_cast<GroupRec*>(wk + o_groups_));
