@@ -588,6 +588,20 @@ def bench_c5(args):
         errs_min.append(rel(ctx.emq_periodic_mixed(x, x, w, L, 0.05, same=True)))
         ctx.set_mixed_target(4e-9)
         errs_chk.append(rel(ctx.emq_periodic_mixed(x, x, w, L, 0.05, same=True)))
+    ctx.set_mixed_target(0.0)
+    # the same requests in plain FP64, one deck at a time (3 timed steps per deck
+    # after one untimed: the rate at which a one-deck-at-a-time FP64 loop runs them)
+    plain_pts, plain_s = 0, 0.0
+    for _, _, deck in reqs:
+        s = ms.Simulation.batch(ctx, [deck], ms.PRECISION_F64)
+        n = sum(s.counts(k)[0] for k in range(len(deck[1])))
+        s.step()
+        t1 = time.perf_counter()
+        s.step(3)
+        plain_s += (time.perf_counter() - t1) / 3
+        plain_pts += n
+        s.close()
+    plain = plain_pts / plain_s
     ctx.close()
     result = {
         "metric": "served RK4 point-steps/s",
@@ -608,6 +622,9 @@ def bench_c5(args):
                                       "point_steps", "busy_s", "span_s")},
         "wall_s": wall,
         "rel_e_err_vs_f64": max(errs) if errs else None,
+        "plain_f64_one_by_one": {"value": plain, "unit": "point-steps/s",
+                                 "speedup_of_served_mixed": st["point_steps_per_s"] / plain,
+                                 "sample": "3 timed steps of each request's deck in plain FP64"},
         "entry_point_rel_e_err_vs_f64": {"min_band": max(errs_min) if errs_min else None,
                                          "target_4e-9": max(errs_chk) if errs_chk else None},
     }

@@ -206,6 +206,17 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
     far[i] = f;
 }
 
+// C(2i+k-1, k), i = 1..7, k = 0..12: the coefficients of (Z + e)^-2i in (-e/Z)^k.
+__device__ constexpr double kBinom[7][13] = {
+    {1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0, 8.0, 9.0, 10.0, 11.0, 12.0, 13.0},
+    {1.0, 4.0, 10.0, 20.0, 35.0, 56.0, 84.0, 120.0, 165.0, 220.0, 286.0, 364.0, 455.0},
+    {1.0, 6.0, 21.0, 56.0, 126.0, 252.0, 462.0, 792.0, 1287.0, 2002.0, 3003.0, 4368.0, 6188.0},
+    {1.0, 8.0, 36.0, 120.0, 330.0, 792.0, 1716.0, 3432.0, 6435.0, 11440.0, 19448.0, 31824.0, 50388.0},
+    {1.0, 10.0, 55.0, 220.0, 715.0, 2002.0, 5005.0, 11440.0, 24310.0, 48620.0, 92378.0, 167960.0, 293930.0},
+    {1.0, 12.0, 78.0, 364.0, 1365.0, 4368.0, 12376.0, 31824.0, 75582.0, 167960.0, 352716.0, 705432.0, 1352078.0},
+    {1.0, 14.0, 105.0, 560.0, 2380.0, 8568.0, 27132.0, 77520.0, 203490.0, 497420.0, 1144066.0, 2496144.0, 5200300.0},
+};
+
 // Far-field expansion of each 32-source tile. The regularization remainder
 // beyond the band is G(z) = s(z) (a1 z^-2 + a2 z^-4 + a3 z^-6) with
 // a1 = -d^2/2, a2 = 3d^4/8, a3 = -5d^6/16 (the binomial series of
@@ -277,9 +288,7 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t
         for (int ii = 1; ii <= kTerms64; ++ii) {
             const int k = m + 2 - 2 * ii;
             if (k < 0 || k >= kMoments64) continue;
-            double b = 1.0;  // C(2i+k-1, k)
-            for (int q = 1; q <= k; ++q) b = b * (2 * ii - 1 + q) / q;
-            const double t = a[ii - 1] * b * ((k & 1) ? -M[k] : M[k]);
+            const double t = a[ii - 1] * kBinom[ii - 1][k] * ((k & 1) ? -M[k] : M[k]);
             c64 += t;
             if (ii <= 3 && k < kMoments) c32 += t;
         }
