@@ -227,20 +227,11 @@ __device__ constexpr double kBinom[7][13] = {
 // k < kMoments. The centre is on the 2^-32 grid, so Z is the difference of
 // high words, as in the pair path. Coefficients are summed in FP64 and
 // rounded once to FP32.
-__global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t* __restrict__ seg_of,
-                                                      const SegDev* __restrict__ segs,
-                                                      const uint64_t* __restrict__ us,
-                                                      const double* __restrict__ ws,
-                                                      TileRec* __restrict__ tiles,
-                                                      TileRec64* __restrict__ tiles64,
-                                                      TileMom* __restrict__ moms) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n_total) return;
-    const SegDev& S = segs[seg_of[i]];
-    if (!S.is_src || ((i - S.off) & (kTile - 1)) != 0) return;
-    const int je = min(i + kTile, S.off + S.n);
-    uint64_t umin = us[i], umax = us[i];
-    for (int j = i + 1; j < je; ++j) {
+__device__ void expand_range(const uint64_t* __restrict__ us, const double* __restrict__ ws,
+                             int b, int e, const SegDev& S, TileRec& T, TileMom& N,
+                             TileRec64* T64) {
+    uint64_t umin = us[b], umax = us[b];
+    for (int j = b + 1; j < e; ++j) {
         const uint64_t u = us[j];
         umin = u < umin ? u : umin;
         umax = u > umax ? u : umax;
@@ -252,11 +243,11 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t
     double M[kMoments64];
 #pragma unroll
     for (int k = 0; k < kMoments64; ++k) M[k] = 0.0;
-    for (int j = i; j < je; ++j) {
-        const int64_t e = (int64_t)(uc - us[j]);
-        const uint64_t ae = (uint64_t)(e < 0 ? -e : e);
-        emax = ae > emax ? ae : emax;
-        const double eh = __builtin_ldexp((double)e, -64 - S.log2h);
+    for (int j = b; j < e; ++j) {
+        const int64_t d = (int64_t)(uc - us[j]);
+        const uint64_t ad = (uint64_t)(d < 0 ? -d : d);
+        emax = ad > emax ? ad : emax;
+        const double eh = __builtin_ldexp((double)d, -64 - S.log2h);
         double pw = ws[j];
 #pragma unroll
         for (int k = 0; k < kMoments64; ++k) {
@@ -275,13 +266,13 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t
             a[ii - 1] = c * p;
         }
     }
-    TileRec T;
-    TileRec64 T64;
     T.hc = hc;
     T.pad = 0;
     T.emax = emax;
-    T64.m0 = M[0];
-    T64.m1 = M[1];
+    if (T64) {
+        T64->m0 = M[0];
+        T64->m1 = M[1];
+    }
     // coefficient of rho^(m+2): n = m + 2 = 2i + k
     for (int m = 0; m < kCoef64; ++m) {
         double c32 = 0.0, c64 = 0.0;
@@ -293,21 +284,42 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t
             if (ii <= 3 && k < kMoments) c32 += t;
         }
         if (m < kCoef) T.c[m] = (float)c32;
-        T64.c[m] = c64;
+        if (T64) T64->c[m] = c64;
     }
-    tiles[S.tile_base + (i - S.off) / kTile] = T;
-    tiles64[S.tile_base + (i - S.off) / kTile] = T64;
-    TileMom N;
     for (int k = 0; k < kMoments64; ++k) N.m[k] = __builtin_ldexp(M[k], k * S.log2h);
-    moms[S.tile_base + (i - S.off) / kTile] = N;
 }
 
-// Span and largest tile emax of each group of 4 tiles (after tile_kernel).
+__global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t* __restrict__ seg_of,
+                                                      const SegDev* __restrict__ segs,
+                                                      const uint64_t* __restrict__ us,
+                                                      const double* __restrict__ ws,
+                                                      TileRec* __restrict__ tiles,
+                                                      TileRec64* __restrict__ tiles64,
+                                                      TileMom* __restrict__ moms) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_total) return;
+    const SegDev& S = segs[seg_of[i]];
+    if (!S.is_src || ((i - S.off) & (kTile - 1)) != 0) return;
+    const int t = S.tile_base + (i - S.off) / kTile;
+    TileRec T;
+    TileMom N;
+    TileRec64 T64;
+    expand_range(us, ws, i, min(i + kTile, S.off + S.n), S, T, N, &T64);
+    tiles[t] = T;
+    tiles64[t] = T64;
+    moms[t] = N;
+}
+
+// Span and largest tile emax of each group of 4 tiles (after tile_kernel),
+// and the group's own expansions.
 __global__ __launch_bounds__(kBlock) void group_kernel(int n_total, const int32_t* __restrict__ seg_of,
                                                        const SegDev* __restrict__ segs,
                                                        const uint64_t* __restrict__ us,
+                                                       const double* __restrict__ ws,
                                                        const TileRec* __restrict__ tiles,
-                                                       GroupRec* __restrict__ groups) {
+                                                       GroupRec* __restrict__ groups,
+                                                       TileRec* __restrict__ gexp,
+                                                       TileMom* __restrict__ gmoms) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n_total) return;
     const SegDev& S = segs[seg_of[i]];
@@ -322,7 +334,14 @@ __global__ __launch_bounds__(kBlock) void group_kernel(int n_total, const int32_
         const uint64_t e = tiles[S.tile_base + t].emax;
         G.emax = e > G.emax ? e : G.emax;
     }
-    groups[S.group_base + (i - S.off) / kGroupSrc] = G;
+    const int g = S.group_base + (i - S.off) / kGroupSrc;
+    groups[g] = G;
+    // the group's own expansions about its centre (FP32 far, FP64 near)
+    TileRec T;
+    TileMom N;
+    expand_range(us, ws, i, ge, S, T, N, nullptr);
+    gexp[g] = T;
+    gmoms[g] = N;
 }
 
 // The FP32 value of one expanded tile at a target (high word ht): the pair
@@ -335,6 +354,27 @@ __device__ __forceinline__ float tile_value(const TileRec& T, uint32_t ht, int h
 #pragma unroll
     for (int m = kCoef - 2; m >= 0; --m) p = __builtin_fmaf(p, rho, T.c[m]);
     return sq * p;
+}
+
+// sum_j w_j g(z_j) of a compact range (moments N about the centre uc) on the
+// side s of a target with key ut, in FP64: sum_j w_j z_j/sqrt(z_j^2 + d^2)
+// from the Legendre generating function, minus s sum_j w_j.
+__device__ __forceinline__ double near_value(const TileMom& N, uint64_t uc, uint64_t ut, double d2,
+                                             double s) {
+    const double Z = __builtin_ldexp((double)(int64_t)(ut - uc), -64);
+    const double ri = rsq_f64(__builtin_fma(Z, Z, d2));
+    const double x = -Z * ri;
+    double pk = x, pkm1 = 1.0, rp = ri;
+    double acc = rp * __builtin_fma(Z, N.m[0], N.m[1]);
+#pragma unroll
+    for (int k = 1; k < kNearTerms; ++k) {
+        rp *= ri;
+        acc = __builtin_fma(pk * rp, __builtin_fma(Z, N.m[k], N.m[k + 1]), acc);
+        const double pk1 = ((2 * k + 1) * x * pk - k * pkm1) * (1.0 / (k + 1));
+        pkm1 = pk;
+        pk = pk1;
+    }
+    return acc - s * N.m[0];
 }
 
 // Per scan block of a source segment: sum of w and of w a.
@@ -444,6 +484,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const double* __restrict__ asm1, const double* __restrict__ ws,
     const FarSrc* __restrict__ far, const TileRec* __restrict__ tiles,
     const GroupRec* __restrict__ groups, const TileMom* __restrict__ moms,
+    const TileRec* __restrict__ gexp, const TileMom* __restrict__ gmoms,
     double* __restrict__ part, int* __restrict__ counter) {
     for (;;) {
         const int it = pull_item(counter);
@@ -471,21 +512,42 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
         }
         for (int gb = W.sb; gb < W.se; gb += kGroupSrc) {
           const int ge = min(W.se, gb + kGroupSrc);
-          if (!S.all_band && tspan < kHalf) {
-            // a group far from the whole target block, not antipodal, and of
-            // compact tiles: every tile takes its expansion
-            const GroupRec G = groups[S.src_group_base + (gb - S.src_off) / kGroupSrc];
+          if (tspan < kHalf) {
+            const int gix = S.src_group_base + (gb - S.src_off) / kGroupSrc;
+            const GroupRec G = groups[gix];
             const uint64_t gspan = G.vl - G.vf;
             const uint64_t glen = tspan + gspan, glo = uf - G.vl;
-            if (gspan < kHalf && glen < kHalf && G.emax <= S.xband &&
-                !cyc_meets(glo + S.band, glen, 2 * S.band) &&
+            if (gspan < kHalf && glen < kHalf &&
                 !cyc_meets(glo - kHalf + kAntiMargin, glen, 2 * kAntiMargin)) {
-                for (int jb = gb; jb < ge; jb += kTile) {
-                    const TileRec T = tiles[S.src_tile_base + (jb - S.src_off) / kTile];
+                const uint64_t gemax = gexp[gix].emax;
+                if (!S.all_band && !cyc_meets(glo + S.band, glen, 2 * S.band)) {
+                    // a group far from the whole target block: its own FP32
+                    // expansion when compact, else its tiles' when they are
+                    if (gemax <= S.xband) {
+                        const TileRec T = gexp[gix];
 #pragma unroll
-                    for (int t = 0; t < TPT; ++t) gx[t] += (double)tile_value(T, hi[t], hexp);
+                        for (int t = 0; t < TPT; ++t) gx[t] += (double)tile_value(T, hi[t], hexp);
+                        continue;
+                    }
+                    if (G.emax <= S.xband) {
+                        for (int jb = gb; jb < ge; jb += kTile) {
+                            const TileRec T = tiles[S.src_tile_base + (jb - S.src_off) / kTile];
+#pragma unroll
+                            for (int t = 0; t < TPT; ++t) gx[t] += (double)tile_value(T, hi[t], hexp);
+                        }
+                        continue;
+                    }
+                } else if (gemax <= S.nxband && !cyc_meets(glo, glen, 1)) {
+                    // a compact group on one side of the whole target block,
+                    // within reach of the band: its FP64 expansion
+                    const uint64_t uc = (uint64_t)gexp[gix].hc << 32;
+                    const TileMom N = gmoms[gix];
+                    const double s = (glo < kHalf) ? 1.0 : -1.0;
+#pragma unroll
+                    for (int t = 0; t < TPT; ++t)
+                        two_sum_acc(sh[t], sl[t], near_value(N, uc, ui[t], d2, s));
+                    continue;
                 }
-                continue;
             }
           }
           for (int jb = gb; jb < ge; jb += kTile) {
@@ -526,22 +588,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                 const TileMom N = moms[tix];
                 const double s = (lo < kHalf) ? 1.0 : -1.0;
 #pragma unroll
-                for (int t = 0; t < TPT; ++t) {
-                    const double Z = __builtin_ldexp((double)(int64_t)(ui[t] - uc), -64);
-                    const double ri = rsq_f64(__builtin_fma(Z, Z, d2));
-                    const double x = -Z * ri;
-                    double pk = x, pkm1 = 1.0, rp = ri;
-                    double acc = rp * __builtin_fma(Z, N.m[0], N.m[1]);
-#pragma unroll
-                    for (int k = 1; k < kNearTerms; ++k) {
-                        rp *= ri;
-                        acc = __builtin_fma(pk * rp, __builtin_fma(Z, N.m[k], N.m[k + 1]), acc);
-                        const double pk1 = ((2 * k + 1) * x * pk - k * pkm1) * (1.0 / (k + 1));
-                        pkm1 = pk;
-                        pk = pk1;
-                    }
-                    two_sum_acc(sh[t], sl[t], acc - s * N.m[0]);
-                }
+                for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], near_value(N, uc, ui[t], d2, s));
             } else if (near) {
                 // no displacement of this tile reaches 1/2, so each pair's
                 // wrap is the tile's: targets above the sources by more than
@@ -675,6 +722,8 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
                                                        const FarSrc* __restrict__ far,
                                                        const TileRec* __restrict__ tiles,
                                                        const TileRec64* __restrict__ tiles64,
+                                                       const GroupRec* __restrict__ groups,
+                                                       const TileRec* __restrict__ gexp,
                                                        uint64_t* __restrict__ gov) {
     __shared__ double red[kGovVals][kBlock];
     const int d = blockIdx.y;
@@ -697,67 +746,91 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
     const SegDev R = segs[S.src];
     const double d2 = S.delta2;
     const int ntile = (R.n + kTile - 1) / kTile;
+    const int ngroup = (R.n + kGroupSrc - 1) / kGroupSrc;
     const double H = __builtin_ldexp(1.0, S.log2h);
-    for (int m = threadIdx.x; m < ntile; m += kBlock) {
-        const int jb = R.off + m * kTile, je = min(jb + kTile, R.off + R.n);
-        const uint64_t vf = us[jb] & ~kLowSort, vl = us[je - 1] | kLowSort;
-        const uint64_t vspan = vl - vf;
-        const bool wide = tspan >= kHalf || vspan >= kHalf || tspan + vspan >= kHalf;
-        const uint64_t len = tspan + vspan;
-        const uint64_t lo = uf - vl;
-        const bool anti = !wide && cyc_meets(lo - kHalf + kAntiMargin, len, 2 * kAntiMargin);
-        int kf = 0;
-        if (!wide && !anti)
-            while (kf < kLadder && band[kf] != 0 && !cyc_meets(lo + band[kf], len, 2 * band[kf]))
-                ++kf;
-        const TileRec T = tiles[R.tile_base + m];
-        // the tile expands at the bands q >= qx and takes the pair path below
-        int qx = 0;
-        while (qx < kf && T.emax > (band[qx] >> kXShift)) ++qx;
-        // the exact field of the tile's sources, K(z) = (z/sqrt(z^2+d^2) - C z)/2
-        // = ((s - C z) + g)/2, and its regularization remainder g: for a
-        // compact far tile from the FP64 reference expansion (7 series terms,
-        // 12 moments: truncation below 1e-12 of the remainder at |z| >= 8d
-        // and e/|Z| <= 1/15, so the FP32 expansion's own truncations show in
-        // the measured error) and moments, otherwise pair by pair (for
-        // |z| >= 2^-6 the cancellation in z/sqrt(z^2+d^2) - s costs
-        // ~1e-16 d^-2 z^2 relative, far below the FP32 errors being measured)
-        double ek, ex;
-        if (kf > 0 && qx == 0) {
-            const TileRec64 T64 = tiles64[R.tile_base + m];
-            const double Z = __builtin_ldexp((double)(int64_t)(ut - ((uint64_t)T.hc << 32)), -64);
-            const double rho = H / Z;
-            double p = T64.c[kCoef64 - 1];
+    const int hexp = -32 - S.log2h;
+    for (int g = threadIdx.x; g < ngroup; g += kBlock) {
+        // the group as one: far at the bands q < kfg, its own expansion
+        // compact at q >= qg (the pair kernel takes it when the group is far
+        // from its whole target block)
+        const GroupRec G = groups[R.group_base + g];
+        const TileRec GE = gexp[R.group_base + g];
+        const uint64_t gspan = G.vl - G.vf;
+        const uint64_t glen = tspan + gspan, glo = uf - G.vl;
+        int kfg = 0;
+        if (gspan < kHalf && glen < kHalf && !cyc_meets(glo - kHalf + kAntiMargin, glen, 2 * kAntiMargin))
+            while (kfg < kLadder && band[kfg] != 0 && !cyc_meets(glo + band[kfg], glen, 2 * band[kfg]))
+                ++kfg;
+        int qg = 0;
+        while (qg < kfg && GE.emax > (band[qg] >> kXShift)) ++qg;
+        double et[kLadder];
 #pragma unroll
-            for (int c = kCoef64 - 2; c >= 0; --c) p = __builtin_fma(p, rho, T64.c[c]);
-            ex = rho * __builtin_fabs(rho) * p;
-            ek = __builtin_copysign(1.0, Z) * T64.m0 - S.C * (Z * T64.m0 + H * T64.m1) + ex;
-        } else {
-            ek = 0.0;
-            ex = 0.0;
-            for (int j = jb; j < je; ++j) {
-                const double z = __builtin_ldexp((double)(int64_t)(ut - us[j]), -64);
-                const double tz = z * rsq_f64(__builtin_fma(z, z, d2));
-                ek += ws[j] * (tz - S.C * z);
-                ex += ws[j] * (tz - __builtin_copysign(1.0, z));
+        for (int q = 0; q < kLadder; ++q) et[q] = 0.0;
+        double exg = 0.0;
+        for (int m = g * (kGroupSrc / kTile); m < min(ntile, (g + 1) * (kGroupSrc / kTile)); ++m) {
+            const int jb = R.off + m * kTile, je = min(jb + kTile, R.off + R.n);
+            const uint64_t vf = us[jb] & ~kLowSort, vl = us[je - 1] | kLowSort;
+            const uint64_t vspan = vl - vf;
+            const bool wide = tspan >= kHalf || vspan >= kHalf || tspan + vspan >= kHalf;
+            const uint64_t len = tspan + vspan;
+            const uint64_t lo = uf - vl;
+            const bool anti = !wide && cyc_meets(lo - kHalf + kAntiMargin, len, 2 * kAntiMargin);
+            int kf = 0;
+            if (!wide && !anti)
+                while (kf < kLadder && band[kf] != 0 && !cyc_meets(lo + band[kf], len, 2 * band[kf]))
+                    ++kf;
+            const TileRec T = tiles[R.tile_base + m];
+            // the tile expands at the bands q >= qx and takes the pair path below
+            int qx = 0;
+            while (qx < kf && T.emax > (band[qx] >> kXShift)) ++qx;
+            // the exact field of the tile's sources, K(z) = (z/sqrt(z^2+d^2) - C z)/2
+            // = ((s - C z) + g)/2, and its regularization remainder g: for a
+            // compact far tile from the FP64 reference expansion (7 series terms,
+            // 13 moments: truncation below 1e-12 of the remainder at |z| >= 8d
+            // and e/|Z| <= 1/15, so the FP32 expansion's own truncations show in
+            // the measured error) and moments, otherwise pair by pair (for
+            // |z| >= 2^-6 the cancellation in z/sqrt(z^2+d^2) - s costs
+            // ~1e-16 d^-2 z^2 relative, far below the FP32 errors being measured)
+            double ek, ex;
+            if (kf > 0 && qx == 0) {
+                const TileRec64 T64 = tiles64[R.tile_base + m];
+                const double Z = __builtin_ldexp((double)(int64_t)(ut - ((uint64_t)T.hc << 32)), -64);
+                const double rho = H / Z;
+                double p = T64.c[kCoef64 - 1];
+#pragma unroll
+                for (int c = kCoef64 - 2; c >= 0; --c) p = __builtin_fma(p, rho, T64.c[c]);
+                ex = rho * __builtin_fabs(rho) * p;
+                ek = __builtin_copysign(1.0, Z) * T64.m0 - S.C * (Z * T64.m0 + H * T64.m1) + ex;
+            } else {
+                ek = 0.0;
+                ex = 0.0;
+                for (int j = jb; j < je; ++j) {
+                    const double z = __builtin_ldexp((double)(int64_t)(ut - us[j]), -64);
+                    const double tz = z * rsq_f64(__builtin_fma(z, z, d2));
+                    ek += ws[j] * (tz - S.C * z);
+                    ex += ws[j] * (tz - __builtin_copysign(1.0, z));
+                }
             }
+            efield += 0.5 * ek;
+            exg += ex;
+            if (kf == 0) continue;
+            float acc = 0.0f;
+            if (qx > 0)
+                for (int j = jb; j < je; ++j) {
+                    const FarSrc f = far[j];
+                    const float zs = (float)(int32_t)(ht - f.h);
+                    const float r = __builtin_amdgcn_rcpf(zs);
+                    const float sq = r * __builtin_fabsf(r);
+                    const float qq = __builtin_fabsf(sq);
+                    const float p = __builtin_fmaf(qq, __builtin_fmaf(qq, f.wb2, f.wb1), f.w);
+                    acc = __builtin_fmaf(sq, p, acc);
+                }
+            const double ep = 0.5 * (__builtin_ldexp((double)acc, 64) - ex);
+            const double ee = (qx < kf) ? 0.5 * ((double)tile_value(T, ht, hexp) - ex) : 0.0;
+            for (int q = 0; q < kf; ++q) et[q] += (q < qx) ? ep : ee;
         }
-        efield += 0.5 * ek;
-        if (kf == 0) continue;
-        float acc = 0.0f;
-        if (qx > 0)
-            for (int j = jb; j < je; ++j) {
-                const FarSrc f = far[j];
-                const float zs = (float)(int32_t)(ht - f.h);
-                const float r = __builtin_amdgcn_rcpf(zs);
-                const float sq = r * __builtin_fabsf(r);
-                const float qq = __builtin_fabsf(sq);
-                const float p = __builtin_fmaf(qq, __builtin_fmaf(qq, f.wb2, f.wb1), f.w);
-                acc = __builtin_fmaf(sq, p, acc);
-            }
-        const double ep = 0.5 * (__builtin_ldexp((double)acc, 64) - ex);
-        const double ee = (qx < kf) ? 0.5 * ((double)tile_value(T, ht, -32 - S.log2h) - ex) : 0.0;
-        for (int q = 0; q < kf; ++q) err[q] += (q < qx) ? ep : ee;
+        const double eg = (qg < kfg) ? 0.5 * ((double)tile_value(GE, ht, hexp) - exg) : 0.0;
+        for (int q = 0; q < kLadder; ++q) err[q] += (q >= qg && q < kfg) ? eg : et[q];
     }
 #pragma unroll
     for (int q = 0; q < kLadder; ++q) red[1 + q][threadIdx.x] = err[q];
@@ -1062,6 +1135,8 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         o_groups_ = take(sizeof(GroupRec) * (size_t)std::max(ngroups_, 1));
         o_tiles64_ = take(sizeof(TileRec64) * (size_t)std::max(ntiles_, 1));
         o_moms_ = take(sizeof(TileMom) * (size_t)std::max(ntiles_, 1));
+        o_gexp_ = take(sizeof(TileRec) * (size_t)std::max(ngroups_, 1));
+        o_gmoms_ = take(sizeof(TileMom) * (size_t)std::max(ngroups_, 1));
     }
     FS_TRY(work_.ensure(o, stream));
     if (precision == kPrecisionMixed) {
@@ -1135,10 +1210,12 @@ int FieldBatch::launch_pairs(int nit, int grid, hipStream_t stream) {
         const TileRec* tiles = reinterpret_cast<const TileRec*>(wk + o_tiles_);
         const GroupRec* groups = reinterpret_cast<const GroupRec*>(wk + o_groups_);
         const TileMom* moms = reinterpret_cast<const TileMom*>(wk + o_moms_);
+        const TileRec* gexp = reinterpret_cast<const TileRec*>(wk + o_gexp_);
+        const TileMom* gmoms = reinterpret_cast<const TileMom*>(wk + o_gmoms_);
         switch (tpt_) {
-            case 1: batch_mixed_kernel<1><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, part, d_counter_); break;
-            case 4: batch_mixed_kernel<4><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, part, d_counter_); break;
-            default: batch_mixed_kernel<2><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, part, d_counter_); break;
+            case 1: batch_mixed_kernel<1><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, gexp, gmoms, part, d_counter_); break;
+            case 4: batch_mixed_kernel<4><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, gexp, gmoms, part, d_counter_); break;
+            default: batch_mixed_kernel<2><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, gexp, gmoms, part, d_counter_); break;
         }
     }
     FS_HIP_CHECK(hipGetLastError());
@@ -1274,7 +1351,8 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
         reinterpret_cast<TileMom*>(wk + o_moms_));
     FS_HIP_CHECK(hipGetLastError());
     group_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
-        n_total_, d_seg_of_, d_segs_, skey, tiles, reinterpret_cast<GroupRec*>(wk + o_groups_));
+        n_total_, d_seg_of_, d_segs_, skey, ws, tiles, reinterpret_cast<GroupRec*>(wk + o_groups_),
+        reinterpret_cast<TileRec*>(wk + o_gexp_), reinterpret_cast<TileMom*>(wk + o_gmoms_));
     FS_HIP_CHECK(hipGetLastError());
     if (!scan_.empty()) {
         blocksum_kernel<<<(unsigned)scan_.size(), kBlock, 0, stream>>>(d_scan_, ws, as, bsum, bsa);
@@ -1287,7 +1365,9 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
         zero_u64_kernel<<<nblocks(nseg_ * kGovVals), kBlock, 0, stream>>>(gov(), nseg_ * kGovVals);
         FS_HIP_CHECK(hipGetLastError());
         probe_kernel<<<dim3(kProbeMax, nseg_), kBlock, 0, stream>>>(
-            d_segs_, skey, ws, far, tiles, reinterpret_cast<TileRec64*>(wk + o_tiles64_), gov());
+            d_segs_, skey, ws, far, tiles, reinterpret_cast<TileRec64*>(wk + o_tiles64_),
+            reinterpret_cast<GroupRec*>(wk + o_groups_), reinterpret_cast<TileRec*>(wk + o_gexp_),
+            gov());
         FS_HIP_CHECK(hipGetLastError());
         governor_kernel<<<nblocks(nseg_), kBlock, 0, stream>>>(nseg_, d_segs_, gov(), gov_target_);
         FS_HIP_CHECK(hipGetLastError());
