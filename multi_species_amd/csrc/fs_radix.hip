@@ -4,9 +4,9 @@
 // deck of a batch. Ordering by the top 24 bits (2^-24 of the period) is
 // enough for its uses (range tests widen by one such unit; the prefix-sum cut
 // points compare full keys within a run of equal top bits), and every pass is
-// stable, so equal keys keep index order and the result is deterministic. A pass is three
-// kernels: per-block digit histograms, one exclusive scan over
-// (digit, block), and a stable scatter (wavefront ballots rank equal digits,
+// stable, so equal keys keep index order and the result is deterministic. A pass is
+// per-block digit histograms, an exclusive scan over (digit, block) (three
+// small kernels), and a stable scatter (wavefront ballots rank equal digits,
 // per-wave counts in LDS order the waves). Nothing here uses memset or host
 // synchronization, so the sort can be captured in a HIP graph.
 #include "fs_internal.h"
@@ -43,7 +43,65 @@ __global__ __launch_bounds__(kBlock) void radix_hist(const uint64_t* __restrict_
     hist[threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of m ints in place, one workgroup.
+// The exclusive scan of the (digit, block) counts: block sums of 4096
+// counts, one workgroup's scan of those sums, then every block's own scan
+// (all loads and stores coalesced).
+constexpr int kScanItems = kBlock * 16;
+
+__global__ __launch_bounds__(kBlock) void scan_reduce(const int* __restrict__ a, int m,
+                                                      int* __restrict__ partial) {
+    __shared__ int red[kBlock];
+    const int b0 = blockIdx.x * kScanItems;
+    int s = 0;
+    for (int r = 0; r < kScanItems / kBlock; ++r) {
+        const int i = b0 + r * kBlock + (int)threadIdx.x;
+        if (i < m) s += a[i];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kBlock) void scan_apply(int* __restrict__ a, int m,
+                                                     const int* __restrict__ partial) {
+    constexpr int kPer = kScanItems / kBlock;
+    __shared__ int buf[kScanItems];
+    __shared__ int tot[kBlock];
+    const int b0 = blockIdx.x * kScanItems;
+    for (int r = 0; r < kPer; ++r) {
+        const int i = b0 + r * kBlock + (int)threadIdx.x;
+        buf[r * kBlock + threadIdx.x] = (i < m) ? a[i] : 0;
+    }
+    __syncthreads();
+    // each thread's kPer consecutive counts, then the threads' totals
+    int s = 0;
+    for (int k = 0; k < kPer; ++k) s += buf[threadIdx.x * kPer + k];
+    tot[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 1; h < kBlock; h <<= 1) {
+        const int add = ((int)threadIdx.x >= h) ? tot[threadIdx.x - h] : 0;
+        __syncthreads();
+        tot[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int run = partial[blockIdx.x] + tot[threadIdx.x] - s;
+    for (int k = 0; k < kPer; ++k) {
+        const int v = buf[threadIdx.x * kPer + k];
+        buf[threadIdx.x * kPer + k] = run;
+        run += v;
+    }
+    __syncthreads();
+    for (int r = 0; r < kPer; ++r) {
+        const int i = b0 + r * kBlock + (int)threadIdx.x;
+        if (i < m) a[i] = buf[r * kBlock + threadIdx.x];
+    }
+}
+
+// Exclusive scan of m ints in place, one workgroup (the block sums).
 __global__ __launch_bounds__(kScanThreads) void radix_scan(int* __restrict__ a, int m) {
     __shared__ int part[kScanThreads];
     const int per = (m + kScanThreads - 1) / kScanThreads;
@@ -120,8 +178,9 @@ int passes_for(int nseg) { return kKeyPasses + (nseg > 1 ? 1 : 0) + (nseg > 256 
 size_t radix_sort_bytes(int n) {
     if (n <= 0) return 0;
     const size_t nb = (size_t)(n + kItems - 1) / kItems;
+    const size_t ns = (256 * nb + kScanItems - 1) / kScanItems;
     return align_up(8 * (size_t)n, 256) + align_up(4 * (size_t)n, 256) +
-           align_up(4 * 256 * nb, 256);
+           align_up(4 * 256 * nb, 256) + align_up(4 * ns, 256);
 }
 
 int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, uint64_t* kout,
@@ -135,6 +194,10 @@ int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, 
     int32_t* ib = reinterpret_cast<int32_t*>(p);
     p += align_up(4 * (size_t)n, 256);
     int* hist = reinterpret_cast<int*>(p);
+    p += align_up(4 * 256 * (size_t)nb, 256);
+    int* partial = reinterpret_cast<int*>(p);
+    const int m = 256 * nb;
+    const int ns = (m + kScanItems - 1) / kScanItems;
     const int passes = passes_for(nseg);
     // ping-pong so that the last pass writes kout/iout
     const uint64_t* ksrc = kin;
@@ -152,7 +215,11 @@ int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, 
         }
         radix_hist<<<nb, kBlock, 0, stream>>>(ksrc, isrc, n, dg, nb, hist);
         FS_HIP_CHECK(hipGetLastError());
-        radix_scan<<<1, kScanThreads, 0, stream>>>(hist, 256 * nb);
+        scan_reduce<<<ns, kBlock, 0, stream>>>(hist, m, partial);
+        FS_HIP_CHECK(hipGetLastError());
+        radix_scan<<<1, kScanThreads, 0, stream>>>(partial, ns);
+        FS_HIP_CHECK(hipGetLastError());
+        scan_apply<<<ns, kBlock, 0, stream>>>(hist, m, partial);
         FS_HIP_CHECK(hipGetLastError());
         radix_scatter<<<nb, kBlock, 0, stream>>>(ksrc, isrc, n, dg, nb, hist, kdst, idst);
         FS_HIP_CHECK(hipGetLastError());
