@@ -187,6 +187,25 @@ def test_error_target(gpu_ctx, amp):
             gpu_ctx.set_mixed_target(bad)
 
 
+@pytest.mark.parametrize("eps", [0.2, 0.5])
+@pytest.mark.parametrize("kind", ["random", "lattice"])
+def test_all_band_expansions_fp64(gpu_ctx, eps, kind):
+    """When the minimum band reaches a quarter period every pair is FP64: the
+    step part from prefix sums, compact near tiles and groups from their FP64
+    Legendre expansions (truncation below 16^-12 of their weight), the rest
+    pair by pair. The result then agrees with the extended-precision sum to
+    FP64 accuracy, not just 1e-8."""
+    if kind == "random":
+        x, w = random_deck(30000, 11)
+    else:
+        x, w = lattice_deck(0.3, 25.0)
+    e = gpu_ctx.emq_periodic_mixed(x, x, w, L0, eps, same=True)
+    ex = oracle.direct_sum_ext(x, x, w, L0, eps)
+    r = rel(e, ex)
+    print(f"all-band {kind} eps {eps}: rel {r:.2e}")
+    assert r <= 1e-11
+
+
 def test_rejects_bad_eps(gpu_ctx):
     import multi_species_amd as msa
     x, w = random_deck(10, 1)
