@@ -18,9 +18,10 @@
 //    halved at the end (the chunking, and so the last bits of the sums, can
 //    differ from the single-call kernel's).
 //  * mixed: the decomposition of fs_field_mixed.hip's comment (FP64 step and
-//    neutralizing part from sorted prefix sums, FP64 near band, FP32
-//    far-field series, compensated FP64 accumulation), with positions sorted
-//    per deck by the graph-safe radix sort of fs_radix.hip.
+//    neutralizing part from sorted prefix sums; group and tile expansions,
+//    FP32 beyond the band and FP64 within it; pairs where no expansion
+//    applies; compensated FP64 accumulation; the accuracy governor), with
+//    positions sorted per deck by the graph-safe radix sort of fs_radix.hip.
 //
 // run() launches kernels only (no allocation, memset or synchronization), so
 // the caller can capture it in a HIP graph.

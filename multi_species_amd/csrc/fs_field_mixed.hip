@@ -10,37 +10,50 @@
 //    the neutralizing background, is piecewise linear in the positions: with
 //    the sources sorted by position it is a handful of prefix sums per target
 //    (FP64, O(N log N) in total).
-//  * G_i = sum_j w_j g(z_ij) carries the O(N^2) work. g is O(1) only within a
-//    few delta of z = 0 and decays like delta^2/(2 z^2) beyond, so
-//      - pairs with |z| < B = max(2^-6, 16 delta, 4000 delta^2) (and the
-//        pairs within 2^-24 of z = -1/2, where the sign flips) are summed in
-//        FP64; the delta^2 term keeps the FP32 remainder's share of the
-//        error flat as eps grows, and B >= 1/4 sends every pair to FP64;
-//      - every other pair is summed in FP32 from the cancellation-free series
-//        g = -s(z) (q/2)(1 - 3q/4 + 5q^2/8), q = delta^2/z^2 <= 1/256, whose
-//        next term is below FP32 rounding;
-//      - each 32-source tile's FP32 sum is folded into an FP64 TwoSum (hi, lo)
-//        pair per target (compensated FP64 accumulation).
+//  * G_i = sum_j w_j g(z_ij) carries the rest. g is O(1) only within a few
+//    delta of z = 0 and decays like delta^2/(2 z^2) beyond. The sorted sources
+//    form 32-source tiles and 128-source groups, and each (block of targets,
+//    group or tile) takes one of:
+//      - beyond the band B = max(2^-6, 16 delta, 4000 delta^2) (B >= 1/4 sends
+//        everything to FP64), a group or tile whose sources lie within B/16
+//        of its centre: the FP32 expansion of g's series
+//        -s (q/2)(1 - 3q/4 + 5q^2/8), q = delta^2/z^2, in the distance from
+//        the centre, from 8 FP64 moments rounded once (one reciprocal and 12
+//        FMAs per target for up to 128 sources);
+//      - beyond the band otherwise: the series pair by pair in FP32;
+//      - a group or tile within delta/16 of its centre on one side of all the
+//        targets: the FP64 Legendre expansion sum_j w_j z_j/sqrt(z_j^2 +
+//        delta^2) = sum_k P_k(-Z/r) r^-(k+1) (Z M_k + M_k+1), r = sqrt(Z^2 +
+//        delta^2), from 13 moments, minus s sum_j w_j;
+//      - otherwise (the target's own neighbourhood, spread-out tiles, tiles
+//        within 2^-24 of the antipode where the sign flips) pair by pair in
+//        FP64;
+//    and per target the pieces go into an FP64 TwoSum (hi, lo) pair.
+//  * With an error target (fs_emq_set_mixed_target) a probe first measures
+//    on up to 256 targets the error each of 8 candidate bands would give
+//    against the exact FP64 field, and the band is the narrowest within the
+//    target.
 // Positions are wrapped in FP64 (a = x/L - floor(x/L)) and turned into 64-bit
 // fixed point U = a 2^64, so every displacement U_i - U_j (mod 2^64) is the
-// exact periodic difference of the wrapped positions: the FP64 band uses it
-// whole, the FP32 path its high word (|z| >= B, so the high word keeps 2^-26
-// relative precision). The FP32 path per pair: one integer subtract, one
-// conversion, v_rcp_f32, five multiply/FMA operations.
+// exact periodic difference of the wrapped positions: the FP64 paths use it
+// whole, the FP32 paths its high word (|z| >= B, so the high word keeps 2^-26
+// relative precision).
 //
 // The FP64 method stays the drop-in (fs_field.hip); this one is measured
 // against it: relative error max_i |E_i - E_i^FP64| / max_i |E_i^FP64| below
 // 1e-8 on the decks the tests run (lattice decks, where rounding errors add
-// coherently, are the hardest case).
+// coherently, are the hardest case), and at FP64 accuracy when every pair is
+// in the band.
 //
 // Work on the device, in order (kernels in fs_field_batch.hip, shared with
 // the batched decks): wrap + fixed point (prep), radix sort of the sources
 // (and of the targets when they differ, as a second segment), gather of the
-// sorted weights with per-block sums, a deterministic prefix scan, the
-// persistent pair kernel over (target block, source chunk) items, and a
-// finalize kernel that evaluates S_i, adds the chunk partials in order and
-// scatters E back to the caller's order. Every reduction runs in a fixed
-// order, so results do not depend on scheduling.
+// sorted weights, tile and group expansions, a deterministic prefix scan,
+// the probe and band choice when governed, the persistent pair kernel over
+// (target block, source chunk) items, and a finalize kernel that evaluates
+// S_i, adds the chunk partials in order and scatters E back to the caller's
+// order. Every reduction runs in a fixed order, so results do not depend on
+// scheduling.
 //
 // This file holds the C ABI of the single-call form.
 #include <vector>
