@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--precision", choices=["f64", "mixed"], default="f64",
                     help="c1: field precision of the step loop")
     ap.add_argument("--decks", type=int, default=64, help="c4: decks in the batch")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="c4: only the batched steps (no FP64 comparison runs, no error checks; "
+                         "for profiling the batched step alone)")
     ap.add_argument("--requests", type=int, default=12, help="c5: requests served")
     ap.add_argument("--req-steps", type=int, default=50, help="c5: steps per request")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -443,6 +446,27 @@ def bench_c4(args):
         lat.append(time.perf_counter() - t1)
     wall = time.perf_counter() - t0
     value = npts * args.steps / wall
+    # algorithmic bytes of one remesh of every species (target x, p in and f out;
+    # old x, p, f; nine point ids and child/flag words per old panel)
+    remesh_bytes = 0
+    for k in range(sum(b.deck_info(d)[1] for d in range(b.n_decks))):
+        n_k, p_k = b.counts(k)
+        remesh_bytes += 24 * n_k + 24 * n_k + 48 * p_k
+    if args.no_compare:
+        b.close()
+        ctx.close()
+        lat = np.array(lat) * 1e3
+        print(json.dumps({
+            "metric": "batched RK4 point-steps/s", "value": value, "unit": "point-steps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+            "config": {"workload": f"{args.decks} Landau decks, batched step only",
+                       "decks": args.decks, "points": int(npts)},
+            "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                           "p99": float(np.percentile(lat, 99))},
+            "remesh_algorithmic_bytes_per_step": remesh_bytes}))
+        return
     # E error of the mixed batch against the FP64 field of the same points,
     # every deck: the served E at the end (remesh field), and the full-field
     # evaluation at moved points (the positions RK stages see) after three
@@ -510,6 +534,7 @@ def bench_c4(args):
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "plain_f64_one_by_one": {"value": plain, "unit": "point-steps/s",
                                  "speedup_of_batched_mixed": value / plain},
+        "remesh_algorithmic_bytes_per_step": remesh_bytes,
         "rel_e_err_vs_f64": err,
         "stage_field_rel_e_err_vs_f64": err_stage,
         "rel_e_diff_vs_f64_run": drift,
