@@ -181,5 +181,8 @@ private:
 size_t radix_sort_bytes(int n);
 int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, uint64_t* kout,
                    int32_t* iout, void* tmp, hipStream_t stream);
+// Stable sort by all 64 key bits (one segment).
+int radix_sort_full(const uint64_t* kin, int n, uint64_t* kout, int32_t* iout, void* tmp,
+                    hipStream_t stream);
 
 }  // namespace fs

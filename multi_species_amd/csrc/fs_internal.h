@@ -61,11 +61,11 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src,
                      double* d_es, void* scratch, int num_cus,
                      const FieldTuning& tun, hipStream_t stream);
 
-// Treecode on device arrays (h_x: a host copy of the positions for the tree
-// build). Synchronizes `stream` before returning.
-int treecode_f64(const double* h_x, const double* d_x, const double* d_w, int n, double L,
-                 double eps, double mac, int degree, int max_source, double* d_es,
-                 DevBuf& work, hipStream_t stream, int64_t* evals);
+// Treecode on device arrays (the points are sorted on the device and the
+// tree built on the host). Synchronizes `stream` before returning.
+int treecode_f64(const double* d_x, const double* d_w, int n, double L, double eps, double mac,
+                 int degree, int max_source, double* d_es, DevBuf& work, hipStream_t stream,
+                 int64_t* evals);
 
 }  // namespace fs
 

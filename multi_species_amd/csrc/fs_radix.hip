@@ -171,7 +171,9 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(
 }
 
 constexpr int kKeyPasses = 3;  // key bits 40..63
-int passes_for(int nseg) { return kKeyPasses + (nseg > 1 ? 1 : 0) + (nseg > 256 ? 1 : 0); }
+int passes_for(int key_passes, int nseg) {
+    return key_passes + (nseg > 1 ? 1 : 0) + (nseg > 256 ? 1 : 0);
+}
 
 }  // namespace
 
@@ -183,8 +185,11 @@ size_t radix_sort_bytes(int n) {
            align_up(4 * 256 * nb, 256) + align_up(4 * ns, 256);
 }
 
-int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, uint64_t* kout,
-                   int32_t* iout, void* tmp, hipStream_t stream) {
+namespace {
+
+// Sorts by (segment, top 8*key_passes key bits).
+int radix_sort_bits(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, int key_passes,
+                    uint64_t* kout, int32_t* iout, void* tmp, hipStream_t stream) {
     if (n <= 0) return FS_OK;
     if (nseg < 1 || nseg > 65536) return fail(FS_E_INVALID, "radix sort: 1..65536 segments");
     const int nb = (n + kItems - 1) / kItems;
@@ -198,7 +203,7 @@ int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, 
     int* partial = reinterpret_cast<int*>(p);
     const int m = 256 * nb;
     const int ns = (m + kScanItems - 1) / kScanItems;
-    const int passes = passes_for(nseg);
+    const int passes = passes_for(key_passes, nseg);
     // ping-pong so that the last pass writes kout/iout
     const uint64_t* ksrc = kin;
     const int32_t* isrc = nullptr;  // identity on the first pass
@@ -207,11 +212,11 @@ int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, 
         uint64_t* kdst = to_out ? kout : kb;
         int32_t* idst = to_out ? iout : ib;
         Digit dg{0, nullptr, 0};
-        if (q < kKeyPasses) {
-            dg.shift = 64 - 8 * (kKeyPasses - q);
+        if (q < key_passes) {
+            dg.shift = 64 - 8 * (key_passes - q);
         } else {
             dg.seg_of = seg_of;
-            dg.seg_shift = 8 * (q - kKeyPasses);
+            dg.seg_shift = 8 * (q - key_passes);
         }
         radix_hist<<<nb, kBlock, 0, stream>>>(ksrc, isrc, n, dg, nb, hist);
         FS_HIP_CHECK(hipGetLastError());
@@ -227,6 +232,18 @@ int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, 
         isrc = idst;
     }
     return FS_OK;
+}
+
+}  // namespace
+
+int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, uint64_t* kout,
+                   int32_t* iout, void* tmp, hipStream_t stream) {
+    return radix_sort_bits(kin, n, seg_of, nseg, kKeyPasses, kout, iout, tmp, stream);
+}
+
+int radix_sort_full(const uint64_t* kin, int n, uint64_t* kout, int32_t* iout, void* tmp,
+                    hipStream_t stream) {
+    return radix_sort_bits(kin, n, nullptr, 1, 8, kout, iout, tmp, stream);
 }
 
 }  // namespace fs

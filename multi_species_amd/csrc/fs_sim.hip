@@ -279,6 +279,7 @@ struct fs_sim {
     bool any_adaptive = false, any_treecode = false;
     // concatenated state: xs, ps, qw, es, fs; RK4 work: xk, p2, p3, p4, e2, e3, e4
     DevBuf state, rk, red, status, jobs, pinfo;
+    DevBuf tree_work;  // treecode work space (grows to the largest treecode deck)
     fs::FieldBatch ffull, fred;
     rm::Job* d_jobs = nullptr;            // grid jobs, one per species
     fs::sim::ResetJob* d_reset = nullptr; // reset descriptors, one per species
@@ -308,18 +309,10 @@ void drop_graphs(Sim& S) {
     S.g_remesh = S.g_plain = nullptr;
 }
 
-// E_MQ_Treecode on one deck's slice: the tree is built on the host.
+// E_MQ_Treecode on one deck's slice (points sorted on the device, tree on the host).
 int treecode_deck(Sim& S, const Deck& D, double* out, const double* x, const double* w, int n) {
-    std::vector<double> hx(n);
-    FS_HIP_CHECK(hipMemcpyAsync(hx.data(), x, sizeof(double) * n, hipMemcpyDeviceToHost,
-                                S.stream()));
-    FS_HIP_CHECK(hipStreamSynchronize(S.stream()));
-    DevBuf work;
-    const int rc = treecode_f64(hx.data(), x, w, n, D.Lx(), D.prm.greens_epsilon, D.prm.mac,
-                                D.prm.degree, D.prm.max_source, out, work, S.stream(), nullptr);
-    (void)hipStreamSynchronize(S.stream());
-    work.release();
-    return rc;
+    return treecode_f64(x, w, n, D.Lx(), D.prm.greens_epsilon, D.prm.mac, D.prm.degree,
+                        D.prm.max_source, out, S.tree_work, S.stream(), nullptr);
 }
 
 // E at every point: the batched direct sum over all direct-sum decks, then
@@ -1128,6 +1121,7 @@ int fs_sim_destroy(fs_sim* S) {
     S->pinfo.release();
     S->ffull.release();
     S->fred.release();
+    S->tree_work.release();
     delete S;
     return FS_OK;
 }

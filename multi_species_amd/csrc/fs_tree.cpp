@@ -29,14 +29,16 @@ bool build(const double* px, int n, double L, double mac, int degree, int max_so
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return px[a] < px[b]; });
     std::vector<double> sx(n);
     for (int k = 0; k < n; ++k) sx[k] = px[order[k]];
+    return build_sorted(order, sx, L, mac, degree, max_source, t);
+}
+
+bool build_sorted(const std::vector<int32_t>& order, const std::vector<double>& sx, double L,
+                  double mac, int degree, int max_source, Tree& t) {
+    t = Tree();
+    const int n = (int)sx.size();
+    if (n <= 0) return true;
     // minval/maxval with their paddings (src/FieldStructure.cpp:1032-1050)
-    double mn = px[0], mx = px[0];
-    for (int i = 1; i < n; ++i) {
-        if (mn > px[i]) mn = px[i];
-        if (mx < px[i]) mx = px[i];
-    }
-    mn -= 0.1234;
-    mx += 0.001;
+    double mn = sx[0] - 0.1234, mx = sx[n - 1] + 0.001;
     // nodes over sorted ranges [s0, s1)
     struct Pending { int node, s0, s1, depth; };
     std::vector<int> sorted_lo, sorted_hi;
@@ -71,16 +73,21 @@ bool build(const double* px, int n, double L, double mac, int degree, int max_so
     };
     rec(0, 0);
     if (!ok) return false;
-    // tree order: leaves in depth-first order, each leaf's points in input order
-    t.perm.clear();
-    t.perm.reserve(n);
+    // tree order: leaves in depth-first order, each leaf's points in input
+    // order (one pass over the input indices fills every leaf's slots)
     std::vector<int> leaf_start(t.nodes.size(), -1), leaf_end(t.nodes.size(), -1);
+    std::vector<int32_t> leaf_of(n);
+    int pos = 0;
     for (int leaf : t.leaves) {
-        std::vector<int32_t> members(order.begin() + sorted_lo[leaf], order.begin() + sorted_hi[leaf]);
-        std::sort(members.begin(), members.end());
-        leaf_start[leaf] = (int)t.perm.size();
-        t.perm.insert(t.perm.end(), members.begin(), members.end());
-        leaf_end[leaf] = (int)t.perm.size() - 1;
+        leaf_start[leaf] = pos;
+        for (int k = sorted_lo[leaf]; k < sorted_hi[leaf]; ++k) leaf_of[order[k]] = leaf;
+        pos += sorted_hi[leaf] - sorted_lo[leaf];
+        leaf_end[leaf] = pos - 1;
+    }
+    t.perm.assign(n, 0);
+    {
+        std::vector<int> fill(leaf_start);
+        for (int i = 0; i < n; ++i) t.perm[fill[leaf_of[i]]++] = i;
     }
     // member ranges of internal nodes: from their first to their last leaf
     std::function<void(int)> ranges = [&](int node) {

@@ -30,5 +30,10 @@ struct Tree {
 // (more than max_source identical coordinates).
 bool build(const double* px, int n, double L, double mac, int degree, int max_source, Tree& t);
 
+// The same from the points already in stable increasing order: order[k] is
+// the input index of the k-th smallest, sx[k] its coordinate.
+bool build_sorted(const std::vector<int32_t>& order, const std::vector<double>& sx, double L,
+                  double mac, int degree, int max_source, Tree& t);
+
 }  // namespace tree
 }  // namespace fs

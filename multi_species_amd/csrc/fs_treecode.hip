@@ -20,8 +20,10 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
+#include "fs_field_batch.h"
 #include "fs_internal.h"
 #include "fs_tree.h"
 
@@ -152,24 +154,64 @@ __global__ __launch_bounds__(256) void gather_kernel(const double* __restrict__ 
     dst[k] = do_fmod ? fmod(v, L) : v;
 }
 
+// Order-preserving 64-bit image of fmod(x, L) (-0 taken as +0): the bit
+// pattern with the sign bit set for positive values, inverted for negative.
+__global__ __launch_bounds__(256) void tree_key_kernel(const double* __restrict__ x, int n,
+                                                       double L, uint64_t* __restrict__ key) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    double v = fmod(x[k], L);
+    if (v == 0.0) v = 0.0;
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    key[k] = (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+double key_value(uint64_t key) {
+    uint64_t b = (key >> 63) ? (key ^ 0x8000000000000000ull) : ~key;
+    double v;
+    std::memcpy(&v, &b, 8);
+    return v;
+}
+
 }  // namespace
 
-// Device treecode: d_x, d_w (input order) -> d_es (input order). The tree is
-// built on the host from a host copy of fmod(x, L); everything else runs on
-// `stream`. evals (optional) receives the far and near kernel-evaluation
+// Device treecode: d_x, d_w (input order) -> d_es (input order). The points
+// are sorted by fmod(x, L) on the device (a stable sort, so equal
+// coordinates keep input order, as the reference's stable partitions do);
+// the tree is built on the host from the sorted order; everything else runs
+// on `stream`. evals (optional) receives the far and near kernel-evaluation
 // counts.
-int treecode_f64(const double* h_x, const double* d_x, const double* d_w, int n, double L,
-                 double eps, double mac, int degree, int max_source, double* d_es,
-                 DevBuf& work, hipStream_t stream, int64_t* evals) {
+int treecode_f64(const double* d_x, const double* d_w, int n, double L, double eps, double mac,
+                 int degree, int max_source, double* d_es, DevBuf& work, hipStream_t stream,
+                 int64_t* evals) {
     if (n <= 0) return FS_OK;
     if (degree < 1 || degree + 1 > kMaxPP)
         return fail(FS_E_INVALID, "treecode: degree must be in [1, %d]", kMaxPP - 1);
     if (max_source < 1) return fail(FS_E_INVALID, "treecode: max_source must be >= 1");
     if (!(L > 0)) return fail(FS_E_INVALID, "treecode: L must be > 0");
-    std::vector<double> px(n);
-    for (int i = 0; i < n; ++i) px[i] = std::fmod(h_x[i], L);  // FieldStructure.cpp:240-247
+    // the points in increasing fmod(x, L) (FieldStructure.cpp:240-247), stable
+    std::vector<int32_t> order(n);
+    std::vector<double> sx(n);
+    {
+        const size_t b_k = align_up(sizeof(uint64_t) * n, 256), b_i = align_up(sizeof(int32_t) * n, 256);
+        FS_TRY(work.ensure(2 * b_k + b_i + radix_sort_bytes(n), stream));
+        char* p = work.as<char>();
+        uint64_t* key = reinterpret_cast<uint64_t*>(p);
+        uint64_t* skey = reinterpret_cast<uint64_t*>(p + b_k);
+        int32_t* ord = reinterpret_cast<int32_t*>(p + 2 * b_k);
+        tree_key_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(d_x, n, L, key);
+        FS_HIP_CHECK(hipGetLastError());
+        FS_TRY(radix_sort_full(key, n, skey, ord, p + 2 * b_k + b_i, stream));
+        std::vector<uint64_t> hk(n);
+        FS_HIP_CHECK(hipMemcpyAsync(hk.data(), skey, sizeof(uint64_t) * n, hipMemcpyDeviceToHost,
+                                    stream));
+        FS_HIP_CHECK(hipMemcpyAsync(order.data(), ord, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+                                    stream));
+        FS_HIP_CHECK(hipStreamSynchronize(stream));
+        for (int k = 0; k < n; ++k) sx[k] = key_value(hk[k]);
+    }
     tree::Tree t;
-    if (!tree::build(px.data(), n, L, mac, degree, max_source, t))
+    if (!tree::build_sorted(order, sx, L, mac, degree, max_source, t))
         return fail(FS_E_RANGE, "treecode: more than max_source coincident points");
     const int PP = degree + 1;
     const int nn = (int)t.nodes.size();
@@ -262,7 +304,7 @@ int fs_emq_treecode_f64(fs_ctx* ctx, double* es, const double* sources, const do
     double* d_e = ctx->io.as<double>(2 * b);
     FS_HIP_CHECK(hipMemcpyAsync(d_x, sources, sizeof(double) * ns, hipMemcpyHostToDevice, ctx->stream));
     FS_HIP_CHECK(hipMemcpyAsync(d_w, q_ws, sizeof(double) * ns, hipMemcpyHostToDevice, ctx->stream));
-    FS_TRY(fs::treecode_f64(sources, d_x, d_w, (int)ns, L, epsilon, mac, degree, max_source, d_e,
+    FS_TRY(fs::treecode_f64(d_x, d_w, (int)ns, L, epsilon, mac, degree, max_source, d_e,
                             ctx->scratch, ctx->stream, evals));
     FS_HIP_CHECK(hipMemcpyAsync(es, d_e, sizeof(double) * ns, hipMemcpyDeviceToHost, ctx->stream));
     FS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
