@@ -165,8 +165,9 @@ class Context:
 
     def set_mixed_target(self, rel_err: float):
         """Error target of the mixed calls on this context: 0 = minimum FP64
-        band, one pass; > 0 = the band is widened after a first pass until the
-        FP32 error bound holds rel_err * max|E|."""
+        band, no probe; > 0 = each call first measures, at up to 256 targets
+        per segment against the exact FP64 field, the error of each candidate
+        band and takes the narrowest one within rel_err * max|E|."""
         _check(lib().fs_emq_set_mixed_target(self.h, rel_err), "fs_emq_set_mixed_target")
 
     def mixed_shape(self):

@@ -128,8 +128,9 @@ public:
     // The last governed run's measurements (nseg * kGovVals values: per
     // segment max|E| and the error at each candidate band, as doubles).
     int get_gov(std::vector<double>& out, hipStream_t stream) const;
-    // Launch shape of the pair kernel: targets per thread (1, 2, 4) and
-    // persistent workgroups per CU. Does not change any result.
+    // Launch shape of the pair kernel: targets per thread (1, 2, 4; always 2
+    // in mixed precision) and persistent workgroups per CU. Does not change
+    // any result.
     int set_shape(int tpt, int wgs_per_cu, hipStream_t stream);
     // Times every shape on this batch's data, keeps the fastest (es is used
     // as scratch). Synchronizes the stream.
@@ -176,7 +177,7 @@ private:
     int ntiles_ = 0, ngroups_ = 0;
 };
 
-// Stable radix sort by (segment of the original index, top 32 key bits)
+// Stable radix sort by (segment of the original index, top 24 key bits)
 // (fs_radix.hip). kin must not alias kout.
 size_t radix_sort_bytes(int n);
 int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, uint64_t* kout,

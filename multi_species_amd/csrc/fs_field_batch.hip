@@ -4,7 +4,7 @@
 // The field of each deck is the reference's E_MQ_DirectSum
 // (src/FieldStructure.cpp:106-156) over that deck's points. The work of all
 // decks is cut into items of (deck, block of 256*TPT targets, chunk of
-// sources); a persistent kernel sized to the device pulls items from an
+// sources; TPT is 2 in mixed precision); a persistent kernel sized to the device pulls items from an
 // atomic counter, largest first, so decks of very different sizes (2k to
 // 500k points) share the chip without a launch per deck. Every item writes
 // its own partial sum; a finalize kernel adds the partials of each target in
@@ -49,6 +49,10 @@ constexpr uint64_t kAntiMargin = uint64_t(1) << 40;
 constexpr int kSortShift = 40;  // keys are sorted by bits 40..63 (fs_radix.hip)
 constexpr uint64_t kLowSort = (uint64_t(1) << kSortShift) - 1;
 constexpr int kGroupSrc = 4 * kTile;   // mixed: sources per group (the chunk unit)
+// mixed: targets per thread, fixed, since the kernel chooses each source
+// group's path for a whole work item (kBlock * kMixedTpt sorted targets): a
+// tuned value would make results depend on timing
+constexpr int kMixedTpt = 2;
 constexpr int kXShift = 4;             // tiles expand when their sources lie within band/16
 constexpr int kMoments = 8;            // moments of a tile's expansion
 constexpr int kCoef = kMoments + 4;    // its polynomial coefficients (TileRec::c)
@@ -1006,8 +1010,12 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     governed_ = false;
     n_total_ = n_total;
     nseg_ = (int)segs.size();
-    tpt_ = (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
-            tun.targets_per_thread == 4)
+    // mixed precision: one target per thread always, so target blocks are
+    // 256 targets from the segment start whatever the launch shape (the
+    // near/far choices are made per target block)
+    tpt_ = precision == kPrecisionMixed ? kMixedTpt
+           : (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
+              tun.targets_per_thread == 4)
                ? tun.targets_per_thread
                : 2;
     segs_.assign(nseg_, SegDev{});
@@ -1182,7 +1190,7 @@ int FieldBatch::upload_items(const std::vector<WorkItem>& items, hipStream_t str
 
 int FieldBatch::set_shape(int tpt, int wgs_per_cu, hipStream_t stream) {
     if (tpt != 1 && tpt != 2 && tpt != 4) return fail(FS_E_INVALID, "field batch: tpt %d", tpt);
-    tpt_ = tpt;
+    tpt_ = precision_ == kPrecisionMixed ? kMixedTpt : tpt;
     wgs_per_cu_ = std::max(1, wgs_per_cu);
     build_items(tpt_, items_);
     FS_TRY(upload_items(items_, stream));
@@ -1213,11 +1221,9 @@ int FieldBatch::launch_pairs(int nit, int grid, hipStream_t stream) {
         const TileMom* moms = reinterpret_cast<const TileMom*>(wk + o_moms_);
         const TileRec* gexp = reinterpret_cast<const TileRec*>(wk + o_gexp_);
         const TileMom* gmoms = reinterpret_cast<const TileMom*>(wk + o_gmoms_);
-        switch (tpt_) {
-            case 1: batch_mixed_kernel<1><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, gexp, gmoms, part, d_counter_); break;
-            case 4: batch_mixed_kernel<4><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, gexp, gmoms, part, d_counter_); break;
-            default: batch_mixed_kernel<2><<<grid, kBlock, 0, stream>>>(d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, gexp, gmoms, part, d_counter_); break;
-        }
+        batch_mixed_kernel<kMixedTpt><<<grid, kBlock, 0, stream>>>(
+            d_items_, nit, d_segs_, skey, as, asm1, ws, far, tiles, groups, moms, gexp, gmoms, part,
+            d_counter_);
     }
     FS_HIP_CHECK(hipGetLastError());
     return FS_OK;
@@ -1244,7 +1250,9 @@ int FieldBatch::autotune(const double* x, const double* w, double* es, hipStream
     int bt = tpt_, bw = wgs_per_cu_;
     std::vector<WorkItem> trial;
     int rc = FS_OK;
-    for (int a = 0; a < 3 && rc == FS_OK; ++a) {
+    // mixed precision: tpt fixed (its near/far choices are made per item)
+    const bool mixed = precision_ == kPrecisionMixed;
+    for (int a = mixed ? 1 : 0; a < (mixed ? 2 : 3) && rc == FS_OK; ++a) {
         // every item of the real launch (so the grid fills as it will), each
         // cut to a source count that keeps the trial near 2e10 pairs
         build_items(tpts[a], trial);

@@ -49,10 +49,8 @@ constexpr int kMaxHistory = 32;      // visited panels kept in registers
 constexpr int kMaxDescend = 256;
 
 __global__ __launch_bounds__(64) void coef_kernel(const Job* __restrict__ jobs) {
-    __shared__ double lds[90 * 64];
     const Job& J = jobs[blockIdx.y];
-    const int lane = threadIdx.x;
-    const int pi = blockIdx.x * 64 + lane;
+    const int pi = blockIdx.x * 64 + threadIdx.x;
     if (pi >= J.npanels) return;
     const int32_t* __restrict__ rec = J.rec;
     const double* __restrict__ ox = J.ox;
@@ -60,68 +58,69 @@ __global__ __launch_bounds__(64) void coef_kernel(const Job* __restrict__ jobs) 
     const double* __restrict__ of = J.of;
     CoefRec* __restrict__ coef = J.coef;
     const int32_t* r = rec + (size_t)pi * kPanelInts;
-    double* A = lds;  // element e of this lane at A[e * 64 + lane]
-#define AE(i, j) A[((i) * 9 + (j)) * 64 + lane]
-#define BE(i) A[(81 + (i)) * 64 + lane]
-    double px[9], pp[9];
+    // the 9x9 system in registers: every index below is static (loops fully
+    // unrolled), the pivot row swap a select per candidate row
+    double px[9], pp[9], bb[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const int q = r[1 + i];
         px[i] = ox[q];
         pp[i] = op[q];
-        BE(i) = of[q];
+        bb[i] = of[q];
     }
+    double a[9][9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const double dx = px[i] - px[4], dp = pp[i] - pp[4];
-        AE(i, 0) = 1;
-        AE(i, 1) = dx;
-        AE(i, 2) = dx * dp;
-        AE(i, 3) = dp;
-        AE(i, 4) = dx * dx;
-        AE(i, 5) = dx * dx * dp;
-        AE(i, 6) = dx * dx * dp * dp;
-        AE(i, 7) = dx * dp * dp;
-        AE(i, 8) = dp * dp;
+        a[i][0] = 1;
+        a[i][1] = dx;
+        a[i][2] = dx * dp;
+        a[i][3] = dp;
+        a[i][4] = dx * dx;
+        a[i][5] = dx * dx * dp;
+        a[i][6] = dx * dx * dp * dp;
+        a[i][7] = dx * dp * dp;
+        a[i][8] = dp * dp;
     }
     int perm[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) perm[i] = i;
+#pragma unroll
     for (int k = 0; k < 9; ++k) {
+        // partial pivoting: the first row of largest |a[i][k]|, i >= k
         int piv = k;
-        double best = fabs(AE(k, k));
+        double best = fabs(a[k][k]);
+#pragma unroll
         for (int i = k + 1; i < 9; ++i) {
-            const double a = fabs(AE(i, k));
-            if (a > best) {
-                best = a;
+            const double v = fabs(a[i][k]);
+            if (v > best) {
+                best = v;
                 piv = i;
             }
         }
-        if (piv != k) {
-            for (int j = 0; j < 9; ++j) {
-                const double t = AE(k, j);
-                AE(k, j) = AE(piv, j);
-                AE(piv, j) = t;
-            }
-            // perm lives in registers: swap through fully unrolled selects
-            int tk = 0, tp = 0;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                if (i == k) tk = perm[i];
-                if (i == piv) tp = perm[i];
-            }
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                if (i == k) perm[i] = tp;
-                if (i == piv) perm[i] = tk;
-            }
-        }
-        const double d = AE(k, k);
-        if (d != 0.0)
-            for (int i = k + 1; i < 9; ++i) AE(i, k) = AE(i, k) / d;
         for (int i = k + 1; i < 9; ++i) {
-            const double l = AE(i, k);
-            for (int j = k + 1; j < 9; ++j) AE(i, j) = AE(i, j) - l * AE(k, j);
+            const bool sw = (i == piv);
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const double t = a[k][j];
+                a[k][j] = sw ? a[i][j] : t;
+                a[i][j] = sw ? t : a[i][j];
+            }
+            const int tp = perm[k];
+            perm[k] = sw ? perm[i] : tp;
+            perm[i] = sw ? tp : perm[i];
+        }
+        const double d = a[k][k];
+        if (d != 0.0) {
+#pragma unroll
+            for (int i = k + 1; i < 9; ++i) a[i][k] = a[i][k] / d;
+        }
+#pragma unroll
+        for (int i = k + 1; i < 9; ++i) {
+            const double l = a[i][k];
+#pragma unroll
+            for (int j = k + 1; j < 9; ++j) a[i][j] = a[i][j] - l * a[k][j];
         }
     }
     double y[9], c[9];
@@ -130,21 +129,19 @@ __global__ __launch_bounds__(64) void coef_kernel(const Job* __restrict__ jobs) 
         double b = 0.0;
 #pragma unroll
         for (int m = 0; m < 9; ++m)
-            if (perm[i] == m) b = BE(m);
+            if (perm[i] == m) b = bb[m];
         double s = b;
 #pragma unroll
-        for (int j = 0; j < i; ++j) s = s - AE(i, j) * y[j];
+        for (int j = 0; j < i; ++j) s = s - a[i][j] * y[j];
         y[i] = s;
     }
 #pragma unroll
     for (int i = 8; i >= 0; --i) {
         double s = y[i];
 #pragma unroll
-        for (int j = i + 1; j < 9; ++j) s = s - AE(i, j) * c[j];
-        c[i] = s / AE(i, i);
+        for (int j = i + 1; j < 9; ++j) s = s - a[i][j] * c[j];
+        c[i] = s / a[i][i];
     }
-#undef AE
-#undef BE
     CoefRec out;
 #pragma unroll
     for (int i = 0; i < 9; ++i) out.c[i] = c[i];

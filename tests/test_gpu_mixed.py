@@ -144,6 +144,13 @@ def test_deterministic_and_tuning(gpu_ctx):
             gpu_ctx.set_tuning(tpt, splits)
             e = gpu_ctx.emq_periodic_mixed(x, x, w, L0, 0.05, same=True)
             assert rel(e, ex) <= TOL, (tpt, splits)
+        # targets per thread is fixed in mixed precision: asking for another
+        # changes no bit
+        gpu_ctx.set_tuning(1, 3)
+        e13 = gpu_ctx.emq_periodic_mixed(x, x, w, L0, 0.05, same=True)
+        assert gpu_ctx.mixed_shape()[0] == 2
+        gpu_ctx.set_tuning(4, 3)
+        assert np.array_equal(gpu_ctx.emq_periodic_mixed(x, x, w, L0, 0.05, same=True), e13)
     finally:
         gpu_ctx.set_tuning(0, 0)
 
