@@ -6,11 +6,14 @@ sources, x uniform in [0, 4 pi) plus a 0.05 sin perturbation, weights
 q f(v) dA with f Maxwellian in v ~ U[-6, 6], dA = 1e-4, half electron- and
 half ion-signed, eps = 0.05. One step = one full direct-sum evaluation
 (N^2 pair interactions) through fs_emq_periodic_f64_dev on device-resident
-inputs. Prints one JSON line.
+inputs; with targets = sources the library evaluates each pair once and
+gives it to both points. Prints one JSON line.
 
 The roofline block uses HIP events (torch.cuda.Event on the stream the
 library launches on) around the timed launches; the kernel is VALU-bound
-(no MFMA: not a contraction), so it is priced at 12 FLOP per pair against
+(no MFMA: not a contraction), so it is priced at 12 FLOP per kernel
+evaluation with one accumulation (14 with the second accumulation of the
+pair-once form) times the evaluations the library reports, against
 MI355X's FP64 vector peak. The CPU baseline times the reference's own
 compiled E_MQ_DirectSum (oracle/_ref) on a bounded target sample on this
 host's cores.
@@ -177,7 +180,11 @@ def main():
     kernel_ms = ev0.elapsed_time(ev1) / args.steps
     pairs = float(n) * float(n)
     value = pairs * args.steps / wall
-    achieved_tflops = FLOP_PER_PAIR * pairs / (kernel_ms * 1e-3) / 1e12
+    # targets = sources: the library evaluates each pair once (the kernel is
+    # odd) and gives it to both points, two accumulations per evaluation
+    evals = float(ctx.f64_pair_evals())
+    flop_per_eval = FLOP_PER_PAIR + (2 if evals < pairs else 0)
+    achieved_tflops = flop_per_eval * evals / (kernel_ms * 1e-3) / 1e12
     traffic = read_traffic()
 
     result = {
@@ -207,7 +214,11 @@ def main():
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
             "traffic": traffic,
             "kernel_ms": kernel_ms,
-            "flop_per_pair": FLOP_PER_PAIR,
+            "kernel_evals_per_launch": evals,
+            "flop_per_eval": flop_per_eval,
+            # 12 FLOP per interaction: the rate a kernel evaluating all N^2
+            # ordered pairs would need for the same interactions/s
+            "interaction_equiv_tflops": FLOP_PER_PAIR * pairs / (kernel_ms * 1e-3) / 1e12,
         },
     }
     # the mixed-precision form on the same resident deck, against the FP64 result

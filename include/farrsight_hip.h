@@ -74,6 +74,20 @@ int fs_emq_treecode_f64(fs_ctx* ctx, double* es, const double* sources, const do
  * later calls on this context. */
 int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits);
 
+/* FP64 direct sum with the targets being the sources (the same array, or the
+ * same host pointer, and nt == ns): the kernel is odd, so each pair can be
+ * evaluated once and given to both points. mode 0 (default) = from 16384
+ * points up, 1 = never, 2 = at every size. No counterpart in the reference
+ * (its E_MQ_DirectSum evaluates every ordered pair); the result agrees with
+ * the full sum to rounding. Applies to later calls on this context. */
+int fs_emq_set_symmetric(fs_ctx* ctx, int mode);
+
+/* Kernel evaluations (pairs, padding of the last block or tile included) of
+ * the last FP64 direct sum launched on this context: nt * ns for the full
+ * sum, about n^2 / 2 for the pair-once form. No counterpart in the reference
+ * (for throughput accounting). */
+int fs_emq_f64_pair_evals(fs_ctx* ctx, int64_t* evals);
+
 /* Mixed-precision form of fs_emq_periodic_f64 (no counterpart in the
  * reference; the FP64 entry points stay the drop-in). Positions are wrapped
  * in FP64 and differenced exactly in 64-bit fixed point; the step and

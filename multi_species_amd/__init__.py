@@ -47,6 +47,8 @@ def lib():
         L.fs_emq_periodic_f64.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
         L.fs_emq_periodic_f64_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d, _vp]
         L.fs_emq_set_tuning.argtypes = [_vp, _int, _int]
+        L.fs_emq_set_symmetric.argtypes = [_vp, _int]
+        L.fs_emq_f64_pair_evals.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64)]
         L.fs_emq_periodic_mixed.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
         L.fs_emq_mixed_shape.argtypes = [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int)]
         L.fs_emq_set_mixed_target.argtypes = [_vp, _d]
@@ -119,9 +121,10 @@ class Context:
             lib().fs_ctx_destroy(self.h)
             self.h = None
 
-    def __del__(self):
-        # at interpreter exit the HIP runtime may already be gone
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the HIP runtime may already be gone (and this
+        # module's globals with it: the check is bound at definition)
+        if _finalizing():
             return
         try:
             self.close()
@@ -134,6 +137,17 @@ class Context:
 
     def synchronize(self):
         _check(lib().fs_ctx_synchronize(self.h), "fs_ctx_synchronize")
+
+    def f64_pair_evals(self) -> int:
+        """Kernel evaluations of the last FP64 direct sum on this context."""
+        v = ctypes.c_int64()
+        _check(lib().fs_emq_f64_pair_evals(self.h, ctypes.byref(v)), "fs_emq_f64_pair_evals")
+        return v.value
+
+    def set_symmetric(self, mode: int = 0):
+        """Pair-once FP64 sum when targets are the sources: 0 = from 16384
+        points up (default), 1 = never, 2 = always."""
+        _check(lib().fs_emq_set_symmetric(self.h, mode), "fs_emq_set_symmetric")
 
     def set_tuning(self, targets_per_thread: int = 0, source_splits: int = 0):
         _check(lib().fs_emq_set_tuning(self.h, targets_per_thread, source_splits),
@@ -372,9 +386,10 @@ class Simulation:
             lib().fs_sim_destroy(self.h)
             self.h = None
 
-    def __del__(self):
-        # at interpreter exit the HIP runtime may already be gone
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the HIP runtime may already be gone (and this
+        # module's globals with it: the check is bound at definition)
+        if _finalizing():
             return
         try:
             self.close()
@@ -531,8 +546,8 @@ class Server:
             lib().fs_server_destroy(self.h)
             self.h = None
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
             return
         try:
             self.close()

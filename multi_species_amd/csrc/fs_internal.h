@@ -51,15 +51,21 @@ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 struct FieldTuning {
     int targets_per_thread = 0;  // 0 = automatic
     int source_splits = 0;       // 0 = automatic
+    // FP64 direct sum with targets = sources: 0 = each pair once (K odd)
+    // from a size up, 1 = never, 2 = at every size
+    int symmetric = 0;
 };
 
 // Periodic direct sum on device arrays (FP64, compensated). `scratch` must
-// hold field_scratch_bytes(nt, ns, ...) bytes. Enqueued on `stream`.
-size_t field_scratch_bytes(int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun);
+// hold field_scratch_bytes(nt, ns, same, ...) bytes, same = the targets are
+// the sources (one array, nt == ns). Enqueued on `stream`.
+size_t field_scratch_bytes(int64_t nt, int64_t ns, bool same, int num_cus,
+                           const FieldTuning& tun);
 int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src,
                      const double* d_w, int64_t ns, double L, double eps,
                      double* d_es, void* scratch, int num_cus,
-                     const FieldTuning& tun, hipStream_t stream);
+                     const FieldTuning& tun, hipStream_t stream,
+                     int64_t* pair_evals = nullptr);
 
 // Treecode on device arrays (the points are sorted on the device and the
 // tree built on the host). Synchronizes `stream` before returning.
@@ -79,6 +85,7 @@ struct fs_ctx {
     fs::FieldTuning tuning;
     fs::FieldBatch* api_batch = nullptr;  // plan of the last mixed entry-point call
     double mixed_target = 0;              // > 0: mixed entry points hold this error bound
+    int64_t f64_pair_evals = 0;           // kernel evaluations of the last FP64 direct sum
     // autotuned launch shapes (tpt, workgroups per CU) of the mixed entry
     // points, by problem class
     std::map<uint64_t, std::pair<int, int>> shape_cache;

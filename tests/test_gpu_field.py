@@ -87,10 +87,59 @@ def test_launch_shapes_agree(gpu_ctx, tpt, splits):
     assert rel(e1, ex) <= 1e-14
 
 
+@pytest.mark.parametrize("n,tpt", [(1, 1), (2, 1), (63, 1), (64, 2), (65, 4), (511, 1),
+                                   (512, 1), (513, 1), (1025, 1), (1536, 2), (2049, 1),
+                                   (4097, 4), (16770, 0), (40000, 0)])
+def test_pair_once_ragged(gpu_ctx, n, tpt):
+    """Targets = sources, each pair evaluated once and given to both points:
+    ragged sizes around the 64-point tile and the row block (512, 1024, 2048
+    points), odd and even block counts, against the extended oracle."""
+    x, w = c2_deck(n, seed=n + 7)
+    gpu_ctx.set_symmetric(2)
+    gpu_ctx.set_tuning(tpt, 0)
+    try:
+        e = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+        e2 = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+        em = gpu_ctx.emq_periodic_f64(x, x, -w, L0, 0.05)
+    finally:
+        gpu_ctx.set_symmetric(0)
+        gpu_ctx.set_tuning(0, 0)
+    assert np.array_equal(e, e2)
+    assert np.array_equal(em, -e)
+    ex = oracle.direct_sum_ext(x, x, w, L0, 0.05)
+    if n == 1:
+        assert e[0] == 0.0
+    else:
+        assert rel(e, ex) <= 1e-14, rel(e, ex)
+
+
+@pytest.mark.parametrize("splits", [0, 1, 3, 1000])
+def test_pair_once_matches_full_sum(gpu_ctx, splits):
+    """The pair-once form against the full sum of every ordered pair (same
+    device arrays), for several chunkings of the source blocks."""
+    n = 70001
+    x, w = c2_deck(n, seed=11)
+    gpu_ctx.set_symmetric(1)
+    try:
+        full = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+    finally:
+        gpu_ctx.set_symmetric(0)
+    gpu_ctx.set_tuning(0, splits)
+    try:
+        e = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+    finally:
+        gpu_ctx.set_tuning(0, 0)
+    assert rel(e, full) <= 1e-14, rel(e, full)
+    pick = np.arange(0, n, 997)
+    ex = oracle.direct_sum_ext(x[pick], x, w, L0, 0.05)
+    assert np.max(np.abs(e[pick] - ex)) <= 1e-14 * np.max(np.abs(ex))
+
+
 @pytest.mark.parametrize("eps", [0.01, 0.05])
 def test_full_size_c2(gpu_ctx, eps):
-    """N = 2^20 targets = sources: sampled targets against the extended oracle
-    over all sources, and size-independent properties of the whole field."""
+    """N = 2^20 targets = sources (the pair-once form): sampled targets
+    against the extended oracle over all sources, and size-independent
+    properties of the whole field."""
     n = 1 << 20
     x, w = c2_deck(n, seed=0)
     e = gpu_ctx.emq_periodic_f64(x, x, w, L0, eps)
