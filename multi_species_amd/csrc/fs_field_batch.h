@@ -84,12 +84,15 @@ struct TileRec64 {
     double m0, m1;
 };
 
-// Moments sum_j w_j e_j^k (k = 0..12, e = centre - source) of one tile: the
+// Moments M_k = sum_j w_j e_j^k (e = centre - source) of one tile, for the
 // FP64 near-band expansion of a compact tile on one side of its targets,
 // sum_j w_j z_j / sqrt(z_j^2 + d^2) = sum_k P_k(-Z/r) r^-(k+1) (Z M_k + M_k+1)
-// with r = sqrt(Z^2 + d^2) (the Legendre generating function).
+// with r = sqrt(Z^2 + d^2) (the Legendre generating function), k < 12,
+// stored scaled by gamma_k = C(2k, k) / 2^k: a_k = gamma_k M_k and
+// b_k = gamma_k M_k+1.
 struct TileMom {
-    double m[13];
+    double a[12];
+    double b[12];
 };
 
 // Far-field record of one sorted source (FP32 path of the mixed kernel).

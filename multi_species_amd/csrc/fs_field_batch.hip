@@ -291,7 +291,14 @@ __device__ void expand_range(const uint64_t* __restrict__ us, const double* __re
         if (m < kCoef) T.c[m] = (float)c32;
         if (T64) T64->c[m] = c64;
     }
-    for (int k = 0; k < kMoments64; ++k) N.m[k] = __builtin_ldexp(M[k], k * S.log2h);
+    // the near expansion's moments in units of the period, scaled by
+    // gamma_k = C(2k, k) / 2^k (exact)
+    double gk = 1.0;
+    for (int k = 0; k < kNearTerms; ++k) {
+        N.a[k] = gk * __builtin_ldexp(M[k], k * S.log2h);
+        N.b[k] = gk * __builtin_ldexp(M[k + 1], (k + 1) * S.log2h);
+        gk = gk * (2.0 * k + 1.0) / (k + 1.0);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t* __restrict__ seg_of,
@@ -361,25 +368,41 @@ __device__ __forceinline__ float tile_value(const TileRec& T, uint32_t ht, int h
     return sq * p;
 }
 
-// sum_j w_j g(z_j) of a compact range (moments N about the centre uc) on the
-// side s of a target with key ut, in FP64: sum_j w_j z_j/sqrt(z_j^2 + d^2)
-// from the Legendre generating function, minus s sum_j w_j.
+// A zero the compiler cannot see through, held in a VGPR: added to a uniform
+// index it makes a load a vector load, so the moments of a near expansion
+// sit in VGPRs (an FP64 FMA may take two of them; in SGPRs they would need a
+// move each and crowd out the kernel's other scalar state).
+__device__ __forceinline__ int vgpr_zero() {
+    int z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
+
+// sum_j w_j g(z_j) of a compact range (scaled moments N about the centre uc)
+// on the side s of a target with key ut, in FP64: sum_j w_j z_j/sqrt(z_j^2 +
+// d^2) from the Legendre generating function, minus s sum_j w_j. With
+// r = sqrt(Z^2 + d^2), x = -Z/r and V_k = P_k(x) r^-(k+1) / gamma_k, the sum is
+// sum_k V_k (Z a_k + b_k), and V obeys V_k+1 = A V_k - c_k B V_k-1 with
+// A = -Z/r^2, B = 1/r^2, c_k = k^2/(4k^2 - 1): three operations per term
+// for the recurrence, two for the sum.
 __device__ __forceinline__ double near_value(const TileMom& N, uint64_t uc, uint64_t ut, double d2,
                                              double s) {
     const double Z = __builtin_ldexp((double)(int64_t)(ut - uc), -64);
     const double ri = rsq_f64(__builtin_fma(Z, Z, d2));
-    const double x = -Z * ri;
-    double pk = x, pkm1 = 1.0, rp = ri;
-    double acc = rp * __builtin_fma(Z, N.m[0], N.m[1]);
+    const double B = ri * ri;
+    const double A = -Z * B;
+    double vm = ri, v = A * ri;
+    double acc = vm * __builtin_fma(Z, N.a[0], N.b[0]);
+    acc = __builtin_fma(v, __builtin_fma(Z, N.a[1], N.b[1]), acc);
 #pragma unroll
-    for (int k = 1; k < kNearTerms; ++k) {
-        rp *= ri;
-        acc = __builtin_fma(pk * rp, __builtin_fma(Z, N.m[k], N.m[k + 1]), acc);
-        const double pk1 = ((2 * k + 1) * x * pk - k * pkm1) * (1.0 / (k + 1));
-        pkm1 = pk;
-        pk = pk1;
+    for (int k = 1; k + 1 < kNearTerms; ++k) {
+        const double ck = (double)(k * k) / (double)(4 * k * k - 1);
+        const double vn = __builtin_fma(-ck, B * vm, A * v);
+        vm = v;
+        v = vn;
+        acc = __builtin_fma(v, __builtin_fma(Z, N.a[k + 1], N.b[k + 1]), acc);
     }
-    return acc - s * N.m[0];
+    return acc - s * N.a[0];
 }
 
 // Per scan block of a source segment: sum of w and of w a.
@@ -483,7 +506,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(
 //    displacements span half a period: FP64 from the exact 64-bit difference;
 //  * far tile: the FP32 series from the high words.
 template <int TPT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void batch_mixed_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_mixed_kernel(
     const WorkItem* __restrict__ items, int nitems, const SegDev* __restrict__ segs,
     const uint64_t* __restrict__ us, const double* __restrict__ as,
     const double* __restrict__ asm1, const double* __restrict__ ws,
@@ -546,7 +569,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                     // a compact group on one side of the whole target block,
                     // within reach of the band: its FP64 expansion
                     const uint64_t uc = (uint64_t)gexp[gix].hc << 32;
-                    const TileMom N = gmoms[gix];
+                    const TileMom N = gmoms[gix + vgpr_zero()];
                     const double s = (glo < kHalf) ? 1.0 : -1.0;
 #pragma unroll
                     for (int t = 0; t < TPT; ++t)
@@ -590,7 +613,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                 // truncation below 16^-12 of sum |w|), and s = the side
                 const int tix = S.src_tile_base + (jb - S.src_off) / kTile;
                 const uint64_t uc = (uint64_t)tiles[tix].hc << 32;
-                const TileMom N = moms[tix];
+                const TileMom N = moms[tix + vgpr_zero()];
                 const double s = (lo < kHalf) ? 1.0 : -1.0;
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], near_value(N, uc, ui[t], d2, s));
@@ -631,17 +654,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) acc[t] = 0.0f;
                 if (je - jb == kTile) {
-                    // whole tile: records in groups of 4 (one scalar load each),
-                    // the next group's load issued before this group's math
-                    FarSrc cur[4], nxt[4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) cur[k] = far[jb + k];
+                    // whole tile: records in groups of 4 (one scalar load each;
+                    // a second buffer in flight would crowd the SGPRs)
+                    FarSrc cur[4];
 #pragma unroll
                     for (int g = 0; g < kTile / 4; ++g) {
-                        if (g + 1 < kTile / 4) {
 #pragma unroll
-                            for (int k = 0; k < 4; ++k) nxt[k] = far[jb + 4 * (g + 1) + k];
-                        }
+                        for (int k = 0; k < 4; ++k) cur[k] = far[jb + 4 * g + k];
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
                             const FarSrc f = cur[k];
@@ -655,8 +674,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                                 acc[t] = __builtin_fmaf(sq, p, acc[t]);
                             }
                         }
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) cur[k] = nxt[k];
                     }
                 } else {
                     for (int j = jb; j < je; ++j) {
