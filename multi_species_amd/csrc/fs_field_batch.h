@@ -49,7 +49,6 @@ struct SegDev {
     int all_band0;    // all_band of the minimum band (no governor)
     uint64_t band0;   // the minimum band B0 * 2^64
     uint64_t xband;   // far tiles whose sources lie within xband of their centre expand
-    uint64_t nxband;  // near tiles expand within d/16 of their centre (FP64)
     int tile_base;    // index of this segment's first source tile (source segments)
     int src_tile_base;
     int group_base;   // index of this segment's first group of 4 tiles

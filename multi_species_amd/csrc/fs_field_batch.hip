@@ -378,6 +378,18 @@ __device__ __forceinline__ int vgpr_zero() {
     return z;
 }
 
+// The near expansion of a range within emax of its centre, whose
+// displacements to every target of a block lie on one side, at least gap
+// away (both in units of 2^-64 of the period): every target is at r =
+// sqrt(Z^2 + d^2) >= sqrt(gap^2 + d^2) from the centre, and 12 Legendre terms
+// leave a truncation below (emax/r)^12, at most 16^-12 of sum |w| when
+// 16 emax <= sqrt(gap^2 + d^2).
+__device__ __forceinline__ bool near_compact(uint64_t emax, uint64_t gap, double d2) {
+    const double e = __builtin_ldexp((double)emax, -60);
+    const double g = __builtin_ldexp((double)gap, -64);
+    return e * e <= __builtin_fma(g, g, d2);
+}
+
 // sum_j w_j g(z_j) of a compact range (scaled moments N about the centre uc)
 // on the side s of a target with key ut, in FP64: sum_j w_j z_j/sqrt(z_j^2 +
 // d^2) from the Legendre generating function, minus s sum_j w_j. With
@@ -565,9 +577,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         }
                         continue;
                     }
-                } else if (gemax <= S.nxband && !cyc_meets(glo, glen, 1)) {
-                    // a compact group on one side of the whole target block,
-                    // within reach of the band: its FP64 expansion
+                } else if (!cyc_meets(glo, glen, 1) &&
+                           near_compact(gemax, glo < kHalf ? glo : 0 - (glo + glen), d2)) {
+                    // a group on one side of the whole target block, within
+                    // reach of the band and compact for its distance: its
+                    // FP64 expansion
                     const uint64_t uc = (uint64_t)gexp[gix].hc << 32;
                     const TileMom N = gmoms[gix + vgpr_zero()];
                     const double s = (glo < kHalf) ? 1.0 : -1.0;
@@ -606,11 +620,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 }
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], acc[t]);
-            } else if (near && tiles[S.src_tile_base + (jb - S.src_off) / kTile].emax <= S.nxband &&
-                       !cyc_meets(lo, len, 1)) {
-                // compact near tile on one side of the whole target block: its
-                // FP64 expansion about the centre (Legendre generating function;
-                // truncation below 16^-12 of sum |w|), and s = the side
+            } else if (near && !cyc_meets(lo, len, 1) &&
+                       near_compact(tiles[S.src_tile_base + (jb - S.src_off) / kTile].emax,
+                                    lo < kHalf ? lo : 0 - (lo + len), d2)) {
+                // near tile on one side of the whole target block, compact for
+                // its distance: its FP64 expansion about the centre (Legendre
+                // generating function; truncation below 16^-12 of sum |w|), and
+                // s = the side
                 const int tix = S.src_tile_base + (jb - S.src_off) / kTile;
                 const uint64_t uc = (uint64_t)tiles[tix].hc << 32;
                 const TileMom N = moms[tix + vgpr_zero()];
@@ -1079,7 +1095,6 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
             S.band0 = S.band;
             S.xband = S.band >> kXShift;
             S.log2h = (int)std::floor(std::log2(B));
-            S.nxband = (uint64_t)std::ldexp(delta, 64 - 4);
             S.d2s = d2 * two64 * two64;
             const double kq2 = d2 * two64;
             S.b1z = S.all_band ? 0.0 : -0.75 * kq2;

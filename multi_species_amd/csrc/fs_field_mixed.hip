@@ -21,10 +21,12 @@
 //        the centre, from 8 FP64 moments rounded once (one reciprocal and 12
 //        FMAs per target for up to 128 sources);
 //      - beyond the band otherwise: the series pair by pair in FP32;
-//      - a group or tile within delta/16 of its centre on one side of all the
-//        targets: the FP64 Legendre expansion sum_j w_j z_j/sqrt(z_j^2 +
-//        delta^2) = sum_k P_k(-Z/r) r^-(k+1) (Z M_k + M_k+1), r = sqrt(Z^2 +
-//        delta^2), from 13 moments, minus s sum_j w_j;
+//      - a group or tile on one side of all the targets, at least gap away,
+//        whose sources lie within sqrt(gap^2 + delta^2)/16 of its centre:
+//        the FP64 Legendre expansion sum_j w_j z_j/sqrt(z_j^2 + delta^2) =
+//        sum_k P_k(-Z/r) r^-(k+1) (Z M_k + M_k+1), r = sqrt(Z^2 + delta^2),
+//        12 terms from 13 moments (truncation below 16^-12), minus
+//        s sum_j w_j;
 //      - otherwise (the target's own neighbourhood, spread-out tiles, tiles
 //        within 2^-24 of the antipode where the sign flips) pair by pair in
 //        FP64;
