@@ -54,10 +54,16 @@ struct SegDev {
     int group_base;   // index of this segment's first group of 4 tiles
     int src_group_base;
     int log2h;        // expansion scale H = 2^log2h, the power of 2 at or below B0
+    // far-field units of 512 and 2048 sources: index of this segment's first
+    // record in the group arrays (after every group), and its source's
+    int unit_base[2];
+    int src_unit_base[2];
 };
 
 // Span of 4 consecutive tiles (128 sorted sources) and their largest emax:
-// a group that is far and compact is summed without per-tile tests.
+// a group that is far and compact is summed without per-tile tests. The
+// same record serves the far-field units of 512 and 2048 sources, with the
+// unit's own emax (largest distance from its expansion's centre).
 struct GroupRec {
     uint64_t vf, vl;  // first and last key, widened by the unsorted low bits
     uint64_t emax;
@@ -176,7 +182,8 @@ private:
            o_asm1_ = 0, o_ws_ = 0, o_far_ = 0, o_pw_ = 0, o_bsum_ = 0, o_bsa_ = 0, o_tot_ = 0,
            o_gov_ = 0, o_tiles_ = 0, o_groups_ = 0, o_tiles64_ = 0, o_moms_ = 0, o_gexp_ = 0,
            o_gmoms_ = 0;
-    int ntiles_ = 0, ngroups_ = 0;
+    int ntiles_ = 0, ngroups_ = 0, nrecs_ = 0;
+    int lstart_[2] = {0, 0}, nunits_[2] = {0, 0};  // far-field unit records per level
 };
 
 // Stable radix sort by (segment of the original index, top 24 key bits)

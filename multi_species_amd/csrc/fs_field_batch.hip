@@ -48,7 +48,13 @@ constexpr uint64_t kHalf = 0x8000000000000000ull;
 constexpr uint64_t kAntiMargin = uint64_t(1) << 40;
 constexpr int kSortShift = 40;  // keys are sorted by bits 40..63 (fs_radix.hip)
 constexpr uint64_t kLowSort = (uint64_t(1) << kSortShift) - 1;
-constexpr int kGroupSrc = 4 * kTile;   // mixed: sources per group (the chunk unit)
+constexpr int kGroupSrc = 4 * kTile;   // mixed: sources per group
+// mixed: far-field units above the group (512 and 2048 sorted sources), each
+// with its own span and FP32 expansion; a source chunk is a whole number of
+// the largest, so every unit lies inside one chunk
+constexpr int kLevels = 2;
+__device__ __host__ constexpr int unit_src(int level) { return kGroupSrc << (2 * (level + 1)); }
+constexpr int kTopSrc = kGroupSrc << (2 * kLevels);  // the mixed chunk unit
 // mixed: targets per thread, fixed, since the kernel chooses each source
 // group's path for a whole work item (kBlock * kMixedTpt sorted targets): a
 // tuned value would make results depend on timing
@@ -232,6 +238,31 @@ __device__ constexpr double kBinom[7][13] = {
 // k < kMoments. The centre is on the 2^-32 grid, so Z is the difference of
 // high words, as in the pair path. Coefficients are summed in FP64 and
 // rounded once to FP32.
+// The centre of a range of keys (its midpoint, on the 2^-32 grid).
+__device__ __forceinline__ uint32_t range_centre(uint64_t umin, uint64_t umax) {
+    const uint64_t mid = umin + ((umax - umin) >> 1);
+    return (uint32_t)(mid >> 32);
+}
+
+// Adds source (key u, weight w) to the moments M_k = sum w (e/H)^k about the
+// centre uc and to the largest distance emax.
+__device__ __forceinline__ void add_moments(double* M, uint64_t& emax, uint64_t uc, uint64_t u,
+                                            double w, int log2h) {
+    const int64_t d = (int64_t)(uc - u);
+    const uint64_t ad = (uint64_t)(d < 0 ? -d : d);
+    emax = ad > emax ? ad : emax;
+    const double eh = __builtin_ldexp((double)d, -64 - log2h);
+    double pw = w;
+#pragma unroll
+    for (int k = 0; k < kMoments64; ++k) {
+        M[k] += pw;
+        pw *= eh;
+    }
+}
+
+__device__ void expand_coeffs(const double* M, uint32_t hc, uint64_t emax, const SegDev& S,
+                              TileRec& T, TileMom* N, TileRec64* T64);
+
 __device__ void expand_range(const uint64_t* __restrict__ us, const double* __restrict__ ws,
                              int b, int e, const SegDev& S, TileRec& T, TileMom& N,
                              TileRec64* T64) {
@@ -241,25 +272,20 @@ __device__ void expand_range(const uint64_t* __restrict__ us, const double* __re
         umin = u < umin ? u : umin;
         umax = u > umax ? u : umax;
     }
-    const uint64_t mid = umin + ((umax - umin) >> 1);
-    const uint32_t hc = (uint32_t)(mid >> 32);
+    const uint32_t hc = range_centre(umin, umax);
     const uint64_t uc = (uint64_t)hc << 32;
     uint64_t emax = 0;
     double M[kMoments64];
 #pragma unroll
     for (int k = 0; k < kMoments64; ++k) M[k] = 0.0;
-    for (int j = b; j < e; ++j) {
-        const int64_t d = (int64_t)(uc - us[j]);
-        const uint64_t ad = (uint64_t)(d < 0 ? -d : d);
-        emax = ad > emax ? ad : emax;
-        const double eh = __builtin_ldexp((double)d, -64 - S.log2h);
-        double pw = ws[j];
-#pragma unroll
-        for (int k = 0; k < kMoments64; ++k) {
-            M[k] += pw;
-            pw *= eh;
-        }
-    }
+    for (int j = b; j < e; ++j) add_moments(M, emax, uc, us[j], ws[j], S.log2h);
+    expand_coeffs(M, hc, emax, S, T, &N, T64);
+}
+
+// Coefficients of the FP32 far expansion (and, when asked, the FP64 near
+// moments and the probe's FP64 reference) from the moments about the centre.
+__device__ void expand_coeffs(const double* M, uint32_t hc, uint64_t emax, const SegDev& S,
+                              TileRec& T, TileMom* N, TileRec64* T64) {
     // a_i (d/H)^2i: the binomial series (1+q)^-1/2 - 1 = sum_i C(-1/2, i) q^i
     const double dh2 = __builtin_ldexp(S.delta2, -2 * S.log2h);  // (d/H)^2
     double a[kTerms64];
@@ -291,12 +317,13 @@ __device__ void expand_range(const uint64_t* __restrict__ us, const double* __re
         if (m < kCoef) T.c[m] = (float)c32;
         if (T64) T64->c[m] = c64;
     }
+    if (!N) return;
     // the near expansion's moments in units of the period, scaled by
     // gamma_k = C(2k, k) / 2^k (exact)
     double gk = 1.0;
     for (int k = 0; k < kNearTerms; ++k) {
-        N.a[k] = gk * __builtin_ldexp(M[k], k * S.log2h);
-        N.b[k] = gk * __builtin_ldexp(M[k + 1], (k + 1) * S.log2h);
+        N->a[k] = gk * __builtin_ldexp(M[k], k * S.log2h);
+        N->b[k] = gk * __builtin_ldexp(M[k + 1], (k + 1) * S.log2h);
         gk = gk * (2.0 * k + 1.0) / (k + 1.0);
     }
 }
@@ -354,6 +381,85 @@ __global__ __launch_bounds__(kBlock) void group_kernel(int n_total, const int32_
     expand_range(us, ws, i, ge, S, T, N, nullptr);
     gexp[g] = T;
     gmoms[g] = N;
+}
+
+// Span and FP32 expansion of each far-field unit of one level (one
+// workgroup per unit): the keys' extremes, then the 13 moments and the
+// largest distance about the centre, summed per thread over a fixed stride
+// and reduced in a fixed order. The record's emax is the unit's own.
+__global__ __launch_bounds__(kBlock) void unit_kernel(int level, int lstart, int nseg,
+                                                      const SegDev* __restrict__ segs,
+                                                      const uint64_t* __restrict__ us,
+                                                      const double* __restrict__ ws,
+                                                      GroupRec* __restrict__ groups,
+                                                      TileRec* __restrict__ gexp) {
+    __shared__ uint64_t smin[kBlock], smax[kBlock];
+    __shared__ double sm[kMoments64][kBlock];
+    const int ua = lstart + (int)blockIdx.x;  // record index (after the groups)
+    const int usrc = unit_src(level);
+    int d = -1;
+    for (int k = 0; k < nseg; ++k) {
+        const SegDev& Q = segs[k];
+        const int nu = (Q.n + usrc - 1) / usrc;
+        if (Q.is_src && ua >= Q.unit_base[level] && ua < Q.unit_base[level] + nu) d = k;
+    }
+    if (d < 0) return;
+    const SegDev S = segs[d];
+    const int b = S.off + (ua - S.unit_base[level]) * usrc;
+    const int e = min(b + usrc, S.off + S.n);
+    uint64_t umin = ~uint64_t(0), umax = 0;
+    for (int j = b + (int)threadIdx.x; j < e; j += kBlock) {
+        const uint64_t u = us[j];
+        umin = u < umin ? u : umin;
+        umax = u > umax ? u : umax;
+    }
+    smin[threadIdx.x] = umin;
+    smax[threadIdx.x] = umax;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            const uint64_t a0 = smin[threadIdx.x + h], a1 = smax[threadIdx.x + h];
+            smin[threadIdx.x] = a0 < smin[threadIdx.x] ? a0 : smin[threadIdx.x];
+            smax[threadIdx.x] = a1 > smax[threadIdx.x] ? a1 : smax[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    const uint32_t hc = range_centre(smin[0], smax[0]);
+    const uint64_t uc = (uint64_t)hc << 32;
+    double M[kMoments64];
+#pragma unroll
+    for (int k = 0; k < kMoments64; ++k) M[k] = 0.0;
+    uint64_t emax = 0;
+    for (int j = b + (int)threadIdx.x; j < e; j += kBlock)
+        add_moments(M, emax, uc, us[j], ws[j], S.log2h);
+    __syncthreads();  // smin/smax[0] read by every thread above
+#pragma unroll
+    for (int k = 0; k < kMoments64; ++k) sm[k][threadIdx.x] = M[k];
+    smax[threadIdx.x] = emax;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+#pragma unroll
+            for (int k = 0; k < kMoments64; ++k) sm[k][threadIdx.x] += sm[k][threadIdx.x + h];
+            const uint64_t a1 = smax[threadIdx.x + h];
+            smax[threadIdx.x] = a1 > smax[threadIdx.x] ? a1 : smax[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < kMoments64; ++k) M[k] = sm[k][0];
+        TileRec T;
+        expand_coeffs(M, hc, smax[0], S, T, nullptr, nullptr);
+        const int u = ua;
+        GroupRec G;
+        G.vf = us[b] & ~kLowSort;
+        G.vl = us[e - 1] | kLowSort;
+        G.emax = T.emax;
+        G.pad = 0;
+        groups[u] = G;
+        gexp[u] = T;
+    }
 }
 
 // The FP32 value of one expanded tile at a target (high word ht): the pair
@@ -550,10 +656,39 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             sl[t] = 0.0;
             gx[t] = 0.0;
         }
-        for (int gb = W.sb; gb < W.se; gb += kGroupSrc) {
+        for (int gb = W.sb; gb < W.se;) {
+          // the largest far-field unit (2048 or 512 sources) starting here
+          // that is far from the whole target block and compact: one FP32
+          // expansion for all its sources
+          if (tspan < kHalf && !S.all_band) {
+            int step = 0;
+#pragma unroll
+            for (int L = kLevels - 1; L >= 0; --L) {
+              const int rel = gb - S.src_off;
+              if (step != 0 || (rel & (unit_src(L) - 1)) != 0) continue;
+              const int uix = S.src_unit_base[L] + rel / unit_src(L);
+              const GroupRec U = groups[uix];
+              const uint64_t uspan = U.vl - U.vf;
+              const uint64_t ulen = tspan + uspan, ulo = uf - U.vl;
+              if (uspan < kHalf && ulen < kHalf && U.emax <= S.xband &&
+                  !cyc_meets(ulo - kHalf + kAntiMargin, ulen, 2 * kAntiMargin) &&
+                  !cyc_meets(ulo + S.band, ulen, 2 * S.band)) {
+                const TileRec T = gexp[uix];
+#pragma unroll
+                for (int t = 0; t < TPT; ++t) gx[t] += (double)tile_value(T, hi[t], hexp);
+                step = unit_src(L);
+              }
+            }
+            if (step != 0) {
+              gb = min(W.se, gb + step);
+              continue;
+            }
+          }
+          const int gcur = gb;
           const int ge = min(W.se, gb + kGroupSrc);
+          gb = ge;
           if (tspan < kHalf) {
-            const int gix = S.src_group_base + (gb - S.src_off) / kGroupSrc;
+            const int gix = S.src_group_base + (gcur - S.src_off) / kGroupSrc;
             const GroupRec G = groups[gix];
             const uint64_t gspan = G.vl - G.vf;
             const uint64_t glen = tspan + gspan, glo = uf - G.vl;
@@ -570,7 +705,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         continue;
                     }
                     if (G.emax <= S.xband) {
-                        for (int jb = gb; jb < ge; jb += kTile) {
+                        for (int jb = gcur; jb < ge; jb += kTile) {
                             const TileRec T = tiles[S.src_tile_base + (jb - S.src_off) / kTile];
 #pragma unroll
                             for (int t = 0; t < TPT; ++t) gx[t] += (double)tile_value(T, hi[t], hexp);
@@ -592,7 +727,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 }
             }
           }
-          for (int jb = gb; jb < ge; jb += kTile) {
+          for (int jb = gcur; jb < ge; jb += kTile) {
             const int je = min(W.se, jb + kTile);
             const uint64_t vf = us[jb] & ~kLowSort, vl = us[je - 1] | kLowSort;
             const uint64_t vspan = vl - vf;
@@ -801,6 +936,30 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
                 ++kfg;
         int qg = 0;
         while (qg < kfg && GE.emax > (band[qg] >> kXShift)) ++qg;
+        // the far-field units that contain the group: far at the bands
+        // q < kfu, compact at q >= qu; the pair kernel takes the largest
+        // such unit, and the unit's value is counted once (with its first
+        // group), against the exact remainders of all its groups
+        int kfu[kLevels], qu[kLevels];
+        double vu[kLevels];
+#pragma unroll
+        for (int L = 0; L < kLevels; ++L) {
+            const int gpu = unit_src(L) / kGroupSrc;
+            const int uix = R.unit_base[L] + g / gpu;
+            const GroupRec U = groups[uix];
+            const uint64_t uspan = U.vl - U.vf;
+            const uint64_t ulen = tspan + uspan, ulo = uf - U.vl;
+            kfu[L] = 0;
+            if (tspan < kHalf && uspan < kHalf && ulen < kHalf &&
+                !cyc_meets(ulo - kHalf + kAntiMargin, ulen, 2 * kAntiMargin))
+                while (kfu[L] < kLadder && band[kfu[L]] != 0 &&
+                       !cyc_meets(ulo + band[kfu[L]], ulen, 2 * band[kfu[L]]))
+                    ++kfu[L];
+            qu[L] = 0;
+            while (qu[L] < kfu[L] && U.emax > (band[qu[L]] >> kXShift)) ++qu[L];
+            vu[L] = (g % gpu == 0 && qu[L] < kfu[L]) ? (double)tile_value(gexp[uix], ht, hexp)
+                                                      : 0.0;
+        }
         double et[kLadder];
 #pragma unroll
         for (int q = 0; q < kLadder; ++q) et[q] = 0.0;
@@ -868,7 +1027,13 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
             for (int q = 0; q < kf; ++q) et[q] += (q < qx) ? ep : ee;
         }
         const double eg = (qg < kfg) ? 0.5 * ((double)tile_value(GE, ht, hexp) - exg) : 0.0;
-        for (int q = 0; q < kLadder; ++q) err[q] += (q >= qg && q < kfg) ? eg : et[q];
+        for (int q = 0; q < kLadder; ++q) {
+            double v = (q >= qg && q < kfg) ? eg : et[q];
+#pragma unroll
+            for (int L = 0; L < kLevels; ++L)
+                if (q >= qu[L] && q < kfu[L]) v = 0.5 * (vu[L] - exg);
+            err[q] += v;
+        }
     }
 #pragma unroll
     for (int q = 0; q < kLadder; ++q) red[1 + q][threadIdx.x] = err[q];
@@ -997,13 +1162,13 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 void chunking(int precision, int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun,
               int& splits, int& chunk) {
     const int64_t want_waves = int64_t(64) * num_cus;
-    const int unit = precision == kPrecisionF64 ? kSub : kGroupSrc;
+    const int unit = precision == kPrecisionF64 ? kSub : kTopSrc;
     const int tpt_auto = (nt >= want_waves * 64 * 2) ? 2 : 1;
     const int64_t waves = (nt + 64 * tpt_auto - 1) / (64 * tpt_auto);
     int64_t sp = tun.source_splits > 0 ? tun.source_splits
                                        : (want_waves + waves - 1) / std::max<int64_t>(waves, 1);
     const int64_t max_splits =
-        std::max<int64_t>(1, ns / (precision == kPrecisionF64 ? kSub : 8 * kTile));
+        std::max<int64_t>(1, ns / (precision == kPrecisionF64 ? kSub : kTopSrc));
     sp = std::min(std::max<int64_t>(sp, 1), max_splits);
     sp = std::min<int64_t>(sp, 65535);
     int64_t ch = (ns + sp - 1) / sp;
@@ -1043,9 +1208,10 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     governed_ = false;
     n_total_ = n_total;
     nseg_ = (int)segs.size();
-    // mixed precision: one target per thread always, so target blocks are
-    // 256 targets from the segment start whatever the launch shape (the
-    // near/far choices are made per target block)
+    for (int L = 0; L < kLevels; ++L) lstart_[L] = nunits_[L] = 0;
+    // mixed precision: a fixed number of targets per thread, so target
+    // blocks are kBlock * kMixedTpt targets from the segment start whatever
+    // the launch shape (the near/far choices are made per target block)
     tpt_ = precision == kPrecisionMixed ? kMixedTpt
            : (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
               tun.targets_per_thread == 4)
@@ -1106,7 +1272,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         std::vector<char> is_src(nseg_, 0);
         for (int d = 0; d < nseg_; ++d)
             if (segs[d].eval) is_src[segs[d].src] = 1;
-        ntiles_ = ngroups_ = 0;
+        ntiles_ = ngroups_ = nrecs_ = 0;
         for (int d = 0; d < nseg_; ++d) {
             segs_[d].is_src = is_src[d];
             segs_[d].tile_base = ntiles_;
@@ -1116,9 +1282,22 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
                 ngroups_ += (segs_[d].n + kGroupSrc - 1) / kGroupSrc;
             }
         }
+        // far-field units: records after every group, level by level
+        int nrec = ngroups_;
+        for (int L = 0; L < kLevels; ++L) {
+            lstart_[L] = nrec;
+            for (int d = 0; d < nseg_; ++d) {
+                segs_[d].unit_base[L] = nrec;
+                if (is_src[d]) nrec += (segs_[d].n + unit_src(L) - 1) / unit_src(L);
+            }
+            nunits_[L] = nrec - lstart_[L];
+        }
+        nrecs_ = nrec;
         for (int d = 0; d < nseg_; ++d) {
             segs_[d].src_tile_base = segs_[segs_[d].src].tile_base;
             segs_[d].src_group_base = segs_[segs_[d].src].group_base;
+            for (int L = 0; L < kLevels; ++L)
+                segs_[d].src_unit_base[L] = segs_[segs_[d].src].unit_base[L];
         }
         for (int d = 0; d < nseg_; ++d) {
             if (!is_src[d]) continue;
@@ -1173,10 +1352,10 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         o_tot_ = take(8 * (size_t)nseg_);
         o_gov_ = take(8 * (size_t)nseg_ * kGovVals);
         o_tiles_ = take(sizeof(TileRec) * (size_t)std::max(ntiles_, 1));
-        o_groups_ = take(sizeof(GroupRec) * (size_t)std::max(ngroups_, 1));
+        o_groups_ = take(sizeof(GroupRec) * (size_t)std::max(nrecs_, 1));
         o_tiles64_ = take(sizeof(TileRec64) * (size_t)std::max(ntiles_, 1));
         o_moms_ = take(sizeof(TileMom) * (size_t)std::max(ntiles_, 1));
-        o_gexp_ = take(sizeof(TileRec) * (size_t)std::max(ngroups_, 1));
+        o_gexp_ = take(sizeof(TileRec) * (size_t)std::max(nrecs_, 1));
         o_gmoms_ = take(sizeof(TileMom) * (size_t)std::max(ngroups_, 1));
     }
     FS_TRY(work_.ensure(o, stream));
@@ -1395,6 +1574,13 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
         n_total_, d_seg_of_, d_segs_, skey, ws, tiles, reinterpret_cast<GroupRec*>(wk + o_groups_),
         reinterpret_cast<TileRec*>(wk + o_gexp_), reinterpret_cast<TileMom*>(wk + o_gmoms_));
     FS_HIP_CHECK(hipGetLastError());
+    for (int L = 0; L < kLevels; ++L) {
+        if (nunits_[L] == 0) continue;
+        unit_kernel<<<nunits_[L], kBlock, 0, stream>>>(
+            L, lstart_[L], nseg_, d_segs_, skey, ws, reinterpret_cast<GroupRec*>(wk + o_groups_),
+            reinterpret_cast<TileRec*>(wk + o_gexp_));
+        FS_HIP_CHECK(hipGetLastError());
+    }
     if (!scan_.empty()) {
         blocksum_kernel<<<(unsigned)scan_.size(), kBlock, 0, stream>>>(d_scan_, ws, as, bsum, bsa);
         FS_HIP_CHECK(hipGetLastError());
