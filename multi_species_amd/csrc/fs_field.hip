@@ -23,6 +23,7 @@
 // independent of co-scheduled work. Accumulation: a plain FP64 sum over each
 // run of kSub sources, folded into a TwoSum (hi, lo) pair per target.
 #include "fs_internal.h"
+#include "fs_sym.h"
 
 namespace fs {
 namespace {
@@ -110,151 +111,21 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(double* __restrict__ a, in
     if (i < n) a[i] = v;
 }
 
-// ---- targets = sources: each pair once --------------------------------
-//
-// K(d) = z (1/sqrt(z^2 + delta^2) - C) is odd in d (rint rounds half to even,
-// which is symmetric), so the pair (i, j) gives w_j K to target i and
-// -w_i K to target j: half the kernel evaluations of the full sum. Points are
-// cut into row blocks of kSymWaves * 64 * T points. A workgroup takes one
-// row block's targets (T per lane) and a chunk of source blocks at cyclic
-// block distances d = 1..H_b from it (H = (nb - 1)/2, plus the distance nb/2
-// for the first half of the blocks when nb is even: every unordered pair of
-// blocks exactly once); a block's pairs with itself are one more item,
-// summed in full without the symmetry. Within a 64-source tile each lane
-// holds one source (x_j, w_j and its accumulator) and the three move one
-// lane on per step (DPP wavefront rotation), so after 64 steps every lane
-// has met every source of the tile and each source's accumulator is back in
-// its own lane; the workgroup's 8 wavefronts add their accumulators of a
-// tile through LDS in a fixed order. Targets keep their sums in registers
-// (plain FP64 over a tile, TwoSum across tiles). Every item writes its row
-// partials and its sources' column partials to their own slots, and a
-// finalize kernel adds each point's partials in a fixed order, so results
-// are deterministic and independent of scheduling.
-constexpr int kSymWaves = 8;
-constexpr int kSymThreads = 64 * kSymWaves;
-
-struct SymPlan {
-    int n;       // points
-    int t;       // targets per lane (1, 2, 4)
-    int rb;      // points per row block, kSymThreads * t
-    int nb;      // row blocks
-    int h;       // (nb - 1) / 2
-    int even;    // nb even: blocks b < nb/2 also take the distance nb/2
-    int hmax;    // largest distance
-    int chb;     // source blocks per item
-    int nch;     // items per row block
-    int nsym;    // nb * nch symmetric items, then nb self items
-    int nitems;
-};
-
-__device__ __host__ __forceinline__ int sym_hb(const SymPlan& P, int b) {
-    return P.h + ((P.even && b < P.nb / 2) ? 1 : 0);
-}
-
-// one lane on across the wavefront (DPP wave_ror:1), 64-bit
-__device__ __forceinline__ double wave_rotate(double v) {
-    const long long u = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)u, 0x13C, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), 0x13C, 0xF, 0xF, false);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
-}
+// ---- targets = sources: each pair once (fs_sym.h) ----------------------
 
 template <int T>
-__global__ __launch_bounds__(kSymThreads) void emq_sym_f64_kernel(
-    const double* __restrict__ x, const double* __restrict__ w, SymPlan P, double invL,
+__global__ __launch_bounds__(sym::kThreads) void emq_sym_f64_kernel(
+    const double* __restrict__ x, const double* __restrict__ w, sym::Plan P, double invL,
     double delta2, double* __restrict__ rowpart, double* __restrict__ colpart) {
-    __shared__ double red[2][kSymWaves][64];
-    const int id = blockIdx.x;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int rb = kSymThreads * T;
-    int b, d0, d1;
-    const bool self = id >= P.nsym;
-    if (!self) {
-        b = id / P.nch;
-        const int c = id - b * P.nch;
-        d0 = 1 + c * P.chb;
-        d1 = min(sym_hb(P, b), (c + 1) * P.chb) + 1;
-    } else {
-        b = id - P.nsym;
-        d0 = 0;
-        d1 = 1;
-    }
-    const int n = P.n;
-    const int p0 = b * rb + wave * 64 * T + lane;
-    double xi[T], nwi[T], acc[T], hi[T], lo[T];
-#pragma unroll
-    for (int t = 0; t < T; ++t) {
-        const int p = p0 + t * 64;
-        xi[t] = p < n ? x[p] : 0.0;
-        nwi[t] = p < n ? -w[p] : 0.0;
-        acc[t] = 0.0;
-        hi[t] = 0.0;
-        lo[t] = 0.0;
-    }
-    const double C = rsq_f64(__builtin_fma(0.5, 0.5, delta2));
-    int tile = 0;
-    for (int d = d0; d < d1; ++d) {
-        const int B = b + d < P.nb ? b + d : b + d - P.nb;
-        for (int jt = 0; jt < rb / 64; ++jt, ++tile) {
-            const int q = B * rb + jt * 64 + lane;
-            double xj = q < n ? x[q] : 0.0;
-            double wj = q < n ? w[q] : 0.0;
-            double accj = 0.0;
-#pragma unroll 2
-            for (int s = 0; s < 64; ++s) {
-#pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    const double u = (xi[t] - xj) * invL;
-                    const double z = u - __builtin_rint(u);
-                    const double y = rsq_f64(__builtin_fma(z, z, delta2));
-                    const double k = z * (y - C);
-                    acc[t] = __builtin_fma(wj, k, acc[t]);
-                    accj = __builtin_fma(nwi[t], k, accj);
-                }
-                xj = wave_rotate(xj);
-                wj = wave_rotate(wj);
-                accj = wave_rotate(accj);
-            }
-#pragma unroll
-            for (int t = 0; t < T; ++t) {
-                two_sum_acc(hi[t], lo[t], acc[t]);
-                acc[t] = 0.0;
-            }
-            if (!self) {
-                red[tile & 1][wave][lane] = accj;
-                __syncthreads();
-                if (wave == (tile & (kSymWaves - 1))) {
-                    double sum = red[tile & 1][0][lane];
-#pragma unroll
-                    for (int v = 1; v < kSymWaves; ++v) sum += red[tile & 1][v][lane];
-                    colpart[((size_t)id * P.chb + (d - d0)) * rb + jt * 64 + lane] = sum;
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < T; ++t) rowpart[(size_t)id * rb + wave * 64 * T + t * 64 + lane] = hi[t] + lo[t];
+    __shared__ double red[2][sym::kWaves][64];
+    sym::item<T>(x, w, P, blockIdx.x, invL, delta2, rowpart, colpart, red);
 }
 
-// E at each point: its self item's row partial, its row block's items in
-// chunk order, then the column partials of the items that reached its block,
-// by increasing distance; TwoSum throughout, halved at the end.
 __global__ __launch_bounds__(kBlock) void sym_finalize_kernel(
-    SymPlan P, const double* __restrict__ rowpart, const double* __restrict__ colpart,
+    sym::Plan P, const double* __restrict__ rowpart, const double* __restrict__ colpart,
     double* __restrict__ es) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= P.n) return;
-    const int b = p / P.rb, loc = p - b * P.rb;
-    double s = 0.0, lo = 0.0;
-    two_sum_acc(s, lo, rowpart[(size_t)(P.nsym + b) * P.rb + loc]);
-    for (int c = 0; c < P.nch; ++c) two_sum_acc(s, lo, rowpart[(size_t)(b * P.nch + c) * P.rb + loc]);
-    for (int d = 1; d <= P.hmax; ++d) {
-        const int b2 = b - d >= 0 ? b - d : b - d + P.nb;
-        if (d > sym_hb(P, b2)) continue;
-        const int id2 = b2 * P.nch + (d - 1) / P.chb;
-        two_sum_acc(s, lo, colpart[((size_t)id2 * P.chb + (d - 1) % P.chb) * P.rb + loc]);
-    }
-    es[p] = 0.5 * (s + lo);
+    if (p < P.n) es[p] = sym::point(P, p, rowpart, colpart);
 }
 
 struct Plan {
@@ -288,55 +159,23 @@ Plan plan_direct(int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun) {
     return p;
 }
 
-// Points from which the pair-once form is the default when targets are the
-// sources.
-constexpr int64_t kSymMinPoints = 16384;
-
 bool use_sym(int64_t nt, int64_t ns, bool same, const FieldTuning& tun) {
     if (!same || nt != ns || nt == 0 || tun.symmetric == 1) return false;
-    return tun.symmetric == 2 || nt >= kSymMinPoints;
+    return tun.symmetric == 2 || nt >= sym::kMinPoints;
 }
 
-SymPlan plan_sym(int64_t n, int num_cus, const FieldTuning& tun) {
-    SymPlan P{};
-    P.n = (int)n;
-    if (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
-        tun.targets_per_thread == 4)
-        P.t = tun.targets_per_thread;
-    else
-        P.t = n >= (int64_t(1) << 18) ? 4 : n >= (int64_t(1) << 16) ? 2 : 1;
-    P.rb = kSymThreads * P.t;
-    P.nb = (int)((n + P.rb - 1) / P.rb);
-    P.h = (P.nb - 1) / 2;
-    P.even = (P.nb % 2 == 0) ? 1 : 0;
-    P.hmax = P.h + P.even;
-    // about 32 items per CU (two resident workgroups per CU, 16 rounds) so
-    // the last round's imbalance stays small
-    const int64_t pairs = (int64_t)P.nb * P.h + (P.even ? P.nb / 2 : 0);
-    const int64_t want = int64_t(32) * num_cus;
-    int64_t chb = tun.source_splits > 0 ? (P.hmax + tun.source_splits - 1) / tun.source_splits
-                                        : (pairs + want - 1) / want;
-    chb = std::max<int64_t>(1, std::min<int64_t>(chb, std::max(P.hmax, 1)));
-    P.chb = (int)chb;
-    P.nch = P.hmax > 0 ? (P.hmax + P.chb - 1) / P.chb : 0;
-    P.nsym = P.nb * P.nch;
-    P.nitems = P.nsym + P.nb;
-    return P;
+size_t sym_rowpart_bytes(const sym::Plan& P) {
+    return align_up(sizeof(double) * (size_t)sym::rowpart_elems(P), 256);
 }
-
-size_t sym_rowpart_bytes(const SymPlan& P) {
-    return align_up(sizeof(double) * (size_t)P.nitems * P.rb, 256);
+size_t sym_colpart_bytes(const sym::Plan& P) {
+    return align_up(sizeof(double) * (size_t)sym::colpart_elems(P), 256);
 }
-size_t sym_colpart_bytes(const SymPlan& P) {
-    return align_up(sizeof(double) * (size_t)P.nsym * P.chb * P.rb, 256);
-}
-
 }  // namespace
 
 size_t field_scratch_bytes(int64_t nt, int64_t ns, bool same, int num_cus,
                            const FieldTuning& tun) {
     if (use_sym(nt, ns, same, tun)) {
-        const SymPlan P = plan_sym(nt, num_cus, tun);
+        const sym::Plan P = sym::plan(nt, num_cus, tun);
         return sym_rowpart_bytes(P) + sym_colpart_bytes(P);
     }
     const Plan p = plan_direct(nt, ns, num_cus, tun);
@@ -365,21 +204,21 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src, const
     const double delta = eps / L;
     const double delta2 = delta * delta;
     if (use_sym(nt, ns, d_tgt == d_src, tun)) {
-        const SymPlan P = plan_sym(nt, num_cus, tun);
+        const sym::Plan P = sym::plan(nt, num_cus, tun);
         double* rowpart = static_cast<double*>(scratch);
         double* colpart = reinterpret_cast<double*>(static_cast<char*>(scratch) +
                                                     sym_rowpart_bytes(P));
         switch (P.t) {
             case 4:
-                emq_sym_f64_kernel<4><<<P.nitems, kSymThreads, 0, stream>>>(
+                emq_sym_f64_kernel<4><<<P.nitems, sym::kThreads, 0, stream>>>(
                     d_src, d_w, P, invL, delta2, rowpart, colpart);
                 break;
             case 2:
-                emq_sym_f64_kernel<2><<<P.nitems, kSymThreads, 0, stream>>>(
+                emq_sym_f64_kernel<2><<<P.nitems, sym::kThreads, 0, stream>>>(
                     d_src, d_w, P, invL, delta2, rowpart, colpart);
                 break;
             default:
-                emq_sym_f64_kernel<1><<<P.nitems, kSymThreads, 0, stream>>>(
+                emq_sym_f64_kernel<1><<<P.nitems, sym::kThreads, 0, stream>>>(
                     d_src, d_w, P, invL, delta2, rowpart, colpart);
                 break;
         }
@@ -387,11 +226,7 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src, const
         sym_finalize_kernel<<<(nti + kBlock - 1) / kBlock, kBlock, 0, stream>>>(P, rowpart,
                                                                               colpart, d_es);
         FS_HIP_CHECK(hipGetLastError());
-        if (pair_evals) {
-            // block pairs at distance >= 1 once, each block with itself in full
-            const int64_t bp = (int64_t)P.nb * P.h + (P.even ? P.nb / 2 : 0) + P.nb;
-            *pair_evals = bp * P.rb * P.rb;
-        }
+        if (pair_evals) *pair_evals = sym::evals(P);
         return FS_OK;
     }
     const Plan p = plan_direct(nt, ns, num_cus, tun);

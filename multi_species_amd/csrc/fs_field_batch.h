@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "fs_internal.h"
+#include "fs_sym.h"
 
 namespace fs {
 
@@ -33,6 +34,7 @@ struct BatchSeg {
 
 // Per-segment launch parameters (device copy).
 struct SegDev {
+    int sym;          // FP64: index of the segment's pair-once plan, or -1
     int off, n;
     int src_off, src_n;
     int splits, chunk;
@@ -120,6 +122,15 @@ struct ScanBlock {
     int count;        // scan blocks of the segment
 };
 
+// Pair-once plan of one FP64 segment (its own source; fs_sym.h).
+struct SymSeg {
+    sym::Plan P;
+    int seg, off;
+    int item_base;          // first workgroup of the segment in its launch
+    int64_t rbase, cbase;   // row and column partials (doubles into the work space)
+    double invL, delta2;
+};
+
 // Plan and device tables of one batched field evaluation; the tables live
 // in the object and are rebuilt by plan(). run() only launches kernels on
 // the given stream (no allocation, no memset, no host synchronization), so
@@ -153,6 +164,9 @@ public:
     int wgs_per_cu() const { return wgs_per_cu_; }
     int items() const { return (int)items_.size(); }
     int64_t pairs() const { return pairs_; }
+    // FP64 kernel evaluations of one run (the pair-once segments evaluate
+    // each pair once)
+    int64_t evals() const { return evals_; }
 
 private:
     void build_items(int tpt, std::vector<WorkItem>& items) const;
@@ -166,7 +180,13 @@ private:
     int n_total_ = 0, nseg_ = 0, tpt_ = 2, grid_ = 0, wgs_per_cu_ = 8, num_cus_ = 256;
     const double* x_ = nullptr;  // inputs of the current run (FP64 pair kernel)
     const double* w_ = nullptr;
-    int64_t pairs_ = 0;
+    int64_t pairs_ = 0, evals_ = 0;
+    // FP64 segments that are their own source and large enough: each pair
+    // once (fs_sym.h), one launch per targets-per-lane value
+    std::vector<SymSeg> symsegs_;
+    int sym_first_[3] = {0, 0, 0}, sym_count_[3] = {0, 0, 0}, sym_items_[3] = {0, 0, 0};
+    SymSeg* d_symsegs_ = nullptr;
+    size_t o_sympart_ = 0;
     std::vector<SegDev> segs_;
     std::vector<WorkItem> items_;
     std::vector<ScanBlock> scan_;

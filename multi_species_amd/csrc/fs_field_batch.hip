@@ -133,9 +133,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     }
 }
 
+// Pair-once segments (fs_sym.h) of one targets-per-lane value: workgroup
+// blockIdx.x belongs to the last table entry whose item_base is not above it.
+template <int T>
+__global__ __launch_bounds__(sym::kThreads) void batch_sym_kernel(
+    const SymSeg* __restrict__ tab, int first, int count, const double* __restrict__ x,
+    const double* __restrict__ w, double* __restrict__ sympart) {
+    __shared__ double red[2][sym::kWaves][64];
+    int lo = first, hi = first + count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].item_base <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const SymSeg E = tab[lo];
+    sym::item<T>(x + E.off, w + E.off, E.P, (int)blockIdx.x - E.item_base, E.invL, E.delta2,
+                 sympart + E.rbase, sympart + E.cbase, red);
+}
+
 __global__ __launch_bounds__(kBlock) void finalize_f64_kernel(
     int n_total, const int32_t* __restrict__ seg_of, const SegDev* __restrict__ segs,
-    const double* __restrict__ part, double* __restrict__ es, int* __restrict__ counter) {
+    const double* __restrict__ part, const SymSeg* __restrict__ symsegs,
+    const double* __restrict__ sympart, double* __restrict__ es, int* __restrict__ counter) {
     const int g = blockIdx.x * kBlock + threadIdx.x;
     if (g == 0) *counter = 0;  // the pair kernel has finished: ready for the next run
     if (g >= n_total) return;
@@ -143,6 +162,11 @@ __global__ __launch_bounds__(kBlock) void finalize_f64_kernel(
     if (!S.eval) return;
     if (S.src_n == 0) {
         es[g] = 0.0;
+        return;
+    }
+    if (S.sym >= 0) {
+        const SymSeg& E = symsegs[S.sym];
+        es[g] = sym::point(E.P, g - S.off, sympart + E.rbase, sympart + E.cbase);
         return;
     }
     double s = 0.0, lo = 0.0;
@@ -1220,7 +1244,8 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     segs_.assign(nseg_, SegDev{});
     items_.clear();
     scan_.clear();
-    pairs_ = 0;
+    pairs_ = evals_ = 0;
+    symsegs_.clear();
     std::vector<int32_t> seg_of(std::max(n_total, 1), 0);
     int64_t part_total = 0;
     for (int d = 0; d < nseg_; ++d) {
@@ -1245,11 +1270,27 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         S.pw_base = b.off + d;
         S.splits = 1;
         S.chunk = std::max(1, S.src_n);
-        if (b.eval && b.n > 0) {
+        S.sym = -1;
+        const bool pair_once = precision == kPrecisionF64 && b.eval && b.src == d && b.n > 0 &&
+                               tun.symmetric != 1 &&
+                               (tun.symmetric == 2 || b.n >= sym::kMinPoints);
+        if (pair_once) {
+            SymSeg E{};
+            E.P = sym::plan(b.n, num_cus, tun);
+            E.seg = d;
+            E.off = b.off;
+            E.invL = S.invL;
+            E.delta2 = S.delta2;
+            S.sym = (int)symsegs_.size();  // renumbered below, grouped by targets per lane
+            symsegs_.push_back(E);
+            pairs_ += (int64_t)b.n * b.n;
+            evals_ += sym::evals(E.P);
+        } else if (b.eval && b.n > 0) {
             chunking(precision, b.n, S.src_n, num_cus, tun, S.splits, S.chunk);
             S.part_base = part_total;
             part_total += (int64_t)S.splits * b.n;
             pairs_ += (int64_t)b.n * S.src_n;
+            evals_ += (int64_t)b.n * S.src_n;
         }
         if (precision == kPrecisionMixed) {
             const double two64 = 18446744073709551616.0;
@@ -1266,6 +1307,31 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
             S.b1z = S.all_band ? 0.0 : -0.75 * kq2;
             S.b2z = S.all_band ? 0.0 : 0.625 * kq2 * kq2;
         }
+    }
+    // pair-once segments grouped by targets per lane (one launch each), with
+    // their workgroups and partial slots
+    int64_t sym_elems = 0;
+    {
+        std::vector<SymSeg> grouped;
+        for (int k = 0; k < 3; ++k) {
+            const int t = 1 << k;
+            sym_first_[k] = (int)grouped.size();
+            sym_items_[k] = 0;
+            for (const SymSeg& E0 : symsegs_) {
+                if (E0.P.t != t) continue;
+                SymSeg E = E0;
+                E.item_base = sym_items_[k];
+                sym_items_[k] += E.P.nitems;
+                E.rbase = sym_elems;
+                sym_elems += sym::rowpart_elems(E.P);
+                E.cbase = sym_elems;
+                sym_elems += sym::colpart_elems(E.P);
+                segs_[E.seg].sym = (int)grouped.size();
+                grouped.push_back(E);
+            }
+            sym_count_[k] = (int)grouped.size() - sym_first_[k];
+        }
+        symsegs_ = grouped;
     }
     if (precision == kPrecisionMixed) {
         // scan blocks of every segment that serves as a source
@@ -1314,11 +1380,16 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     const size_t b_segs = align_up(sizeof(SegDev) * nseg_, 256);
     const size_t b_scan = align_up(sizeof(ScanBlock) * std::max<size_t>(scan_.size(), 1), 256);
     const size_t b_segof = align_up(sizeof(int32_t) * seg_of.size(), 256);
-    FS_TRY(tables_.ensure(b_segs + b_scan + b_segof + 256, stream));
+    const size_t b_sym = align_up(sizeof(SymSeg) * std::max<size_t>(symsegs_.size(), 1), 256);
+    FS_TRY(tables_.ensure(b_segs + b_scan + b_segof + b_sym + 256, stream));
     d_segs_ = tables_.as<SegDev>(0);
     d_scan_ = tables_.as<ScanBlock>(b_segs);
     d_seg_of_ = tables_.as<int32_t>(b_segs + b_scan);
-    d_counter_ = tables_.as<int>(b_segs + b_scan + b_segof);
+    d_symsegs_ = tables_.as<SymSeg>(b_segs + b_scan + b_segof);
+    d_counter_ = tables_.as<int>(b_segs + b_scan + b_segof + b_sym);
+    if (!symsegs_.empty())
+        FS_HIP_CHECK(hipMemcpyAsync(d_symsegs_, symsegs_.data(), sizeof(SymSeg) * symsegs_.size(),
+                                    hipMemcpyHostToDevice, stream));
     FS_HIP_CHECK(hipMemcpyAsync(d_segs_, segs_.data(), sizeof(SegDev) * nseg_,
                                 hipMemcpyHostToDevice, stream));
     if (!scan_.empty())
@@ -1337,6 +1408,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     };
     const size_t n = (size_t)std::max(n_total, 1);
     o_part_ = take(8 * (size_t)std::max<int64_t>(part_total, 1));
+    o_sympart_ = take(8 * (size_t)std::max<int64_t>(sym_elems, 1));
     if (precision == kPrecisionMixed) {
         o_key_ = take(8 * n);
         o_skey_ = take(8 * n);
@@ -1376,7 +1448,7 @@ void FieldBatch::build_items(int tpt, std::vector<WorkItem>& items) const {
     const int tblock = kBlock * tpt;
     for (int d = 0; d < nseg_; ++d) {
         const SegDev& S = segs_[d];
-        if (!S.eval || S.n == 0 || S.src_n == 0) continue;
+        if (!S.eval || S.n == 0 || S.src_n == 0 || S.sym >= 0) continue;
         for (int tb = S.off; tb < S.off + S.n; tb += tblock)
             for (int s = 0; s < S.splits; ++s) {
                 const int sb = S.src_off + s * S.chunk;
@@ -1543,8 +1615,19 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
     w_ = w;
     if (precision_ == kPrecisionF64) {
         FS_TRY(launch_pairs(nit, grid_, stream));
-        finalize_f64_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(n_total_, d_seg_of_, d_segs_,
-                                                                      part, es, d_counter_);
+        double* sympart = reinterpret_cast<double*>(wk + o_sympart_);
+        if (sym_items_[0] > 0)
+            batch_sym_kernel<1><<<sym_items_[0], sym::kThreads, 0, stream>>>(
+                d_symsegs_, sym_first_[0], sym_count_[0], x, w, sympart);
+        if (sym_items_[1] > 0)
+            batch_sym_kernel<2><<<sym_items_[1], sym::kThreads, 0, stream>>>(
+                d_symsegs_, sym_first_[1], sym_count_[1], x, w, sympart);
+        if (sym_items_[2] > 0)
+            batch_sym_kernel<4><<<sym_items_[2], sym::kThreads, 0, stream>>>(
+                d_symsegs_, sym_first_[2], sym_count_[2], x, w, sympart);
+        FS_HIP_CHECK(hipGetLastError());
+        finalize_f64_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
+            n_total_, d_seg_of_, d_segs_, part, d_symsegs_, sympart, es, d_counter_);
         FS_HIP_CHECK(hipGetLastError());
         return FS_OK;
     }
