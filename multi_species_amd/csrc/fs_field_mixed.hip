@@ -12,14 +12,16 @@
 //    (FP64, O(N log N) in total).
 //  * G_i = sum_j w_j g(z_ij) carries the rest. g is O(1) only within a few
 //    delta of z = 0 and decays like delta^2/(2 z^2) beyond. The sorted sources
-//    form 32-source tiles and 128-source groups, and each (block of targets,
-//    group or tile) takes one of:
+//    form 32-source tiles, 128-source groups and far-field units of 512 and
+//    2048 sources, and each (block of targets, unit, group or tile) takes
+//    one of:
 //      - beyond the band B = max(2^-6, 16 delta, 4000 delta^2) (B >= 1/4 sends
-//        everything to FP64), a group or tile whose sources lie within B/16
-//        of its centre: the FP32 expansion of g's series
-//        -s (q/2)(1 - 3q/4 + 5q^2/8), q = delta^2/z^2, in the distance from
-//        the centre, from 8 FP64 moments rounded once (one reciprocal and 12
-//        FMAs per target for up to 128 sources);
+//        everything to FP64), the largest unit, else the group, else the
+//        tile whose sources lie within B/16 of its centre: the FP32
+//        expansion of g's series -s (q/2)(1 - 3q/4 + 5q^2/8), q =
+//        delta^2/z^2, in the distance from the centre, from 8 FP64 moments
+//        rounded once (one reciprocal and 12 FMAs per target for up to 2048
+//        sources);
 //      - beyond the band otherwise: the series pair by pair in FP32;
 //      - a group or tile on one side of all the targets, at least gap away,
 //        whose sources lie within sqrt(gap^2 + delta^2)/16 of its centre:
