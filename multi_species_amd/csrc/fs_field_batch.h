@@ -201,6 +201,7 @@ private:
     size_t o_part_ = 0, o_key_ = 0, o_skey_ = 0, o_perm_ = 0, o_sort_ = 0, o_as_ = 0,
            o_asm1_ = 0, o_ws_ = 0, o_far_ = 0, o_pw_ = 0, o_bsum_ = 0, o_bsa_ = 0, o_tot_ = 0,
            o_gov_ = 0, o_tiles_ = 0, o_groups_ = 0, o_tiles64_ = 0, o_moms_ = 0, o_gexp_ = 0,
+           o_gexp64_ = 0,
            o_gmoms_ = 0;
     int ntiles_ = 0, ngroups_ = 0, nrecs_ = 0;
     int lstart_[2] = {0, 0}, nunits_[2] = {0, 0};  // far-field unit records per level

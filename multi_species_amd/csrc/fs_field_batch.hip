@@ -382,6 +382,7 @@ __global__ __launch_bounds__(kBlock) void group_kernel(int n_total, const int32_
                                                        const TileRec* __restrict__ tiles,
                                                        GroupRec* __restrict__ groups,
                                                        TileRec* __restrict__ gexp,
+                                                       TileRec64* __restrict__ gexp64,
                                                        TileMom* __restrict__ gmoms) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n_total) return;
@@ -402,8 +403,10 @@ __global__ __launch_bounds__(kBlock) void group_kernel(int n_total, const int32_
     // the group's own expansions about its centre (FP32 far, FP64 near)
     TileRec T;
     TileMom N;
-    expand_range(us, ws, i, ge, S, T, N, nullptr);
+    TileRec64 T64;
+    expand_range(us, ws, i, ge, S, T, N, &T64);
     gexp[g] = T;
+    gexp64[g] = T64;
     gmoms[g] = N;
 }
 
@@ -906,6 +909,16 @@ __device__ __forceinline__ uint64_t ladder_band(const SegDev& S, int q) {
     return b >= 0.25 ? 0 : (uint64_t)__builtin_ldexp(b, 64);  // 0: all pairs FP64
 }
 
+// The FP64 reference remainder of an expanded range at rho = H/Z (coefficients
+// read through the pointer, a few at a time, so they do not all sit in
+// registers at once).
+__device__ __forceinline__ double ref_value(const TileRec64* __restrict__ T, double rho) {
+    double p = T->c[kCoef64 - 1];
+#pragma unroll 4
+    for (int c = kCoef64 - 2; c >= 0; --c) p = __builtin_fma(p, rho, T->c[c]);
+    return rho * __builtin_fabs(rho) * p;
+}
+
 __device__ __forceinline__ int probe_samples(int n) {
     const int s = n / 64;
     return min(n, min(kProbeMax, max(16, s)));
@@ -921,6 +934,7 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
                                                        const TileRec64* __restrict__ tiles64,
                                                        const GroupRec* __restrict__ groups,
                                                        const TileRec* __restrict__ gexp,
+                                                       const TileRec64* __restrict__ gexp64,
                                                        uint64_t* __restrict__ gov) {
     __shared__ double red[kGovVals][kBlock];
     const int d = blockIdx.y;
@@ -988,6 +1002,19 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
 #pragma unroll
         for (int q = 0; q < kLadder; ++q) et[q] = 0.0;
         double exg = 0.0;
+        // a group far from the target at band 0 and compact there: at every
+        // band where it is far the pair kernel takes its own expansion (or a
+        // unit), so its exact field and remainder come from its FP64
+        // reference expansion, and its tiles count only at the bands where
+        // they are far and the group is not
+        const bool gfast = qg == 0 && kfg > 0;
+        if (gfast) {
+            const TileRec64* T64 = gexp64 + R.group_base + g;
+            const double Z = __builtin_ldexp((double)(int64_t)(ut - ((uint64_t)GE.hc << 32)), -64);
+            exg = ref_value(T64, H / Z);
+            efield += 0.5 * (__builtin_copysign(1.0, Z) * T64->m0 -
+                             S.C * (Z * T64->m0 + H * T64->m1) + exg);
+        }
         for (int m = g * (kGroupSrc / kTile); m < min(ntile, (g + 1) * (kGroupSrc / kTile)); ++m) {
             const int jb = R.off + m * kTile, je = min(jb + kTile, R.off + R.n);
             const uint64_t vf = us[jb] & ~kLowSort, vl = us[je - 1] | kLowSort;
@@ -1000,6 +1027,7 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
             if (!wide && !anti)
                 while (kf < kLadder && band[kf] != 0 && !cyc_meets(lo + band[kf], len, 2 * band[kf]))
                     ++kf;
+            if (gfast && kf <= kfg) continue;
             const TileRec T = tiles[R.tile_base + m];
             // the tile expands at the bands q >= qx and takes the pair path below
             int qx = 0;
@@ -1014,14 +1042,10 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
             // ~1e-16 d^-2 z^2 relative, far below the FP32 errors being measured)
             double ek, ex;
             if (kf > 0 && qx == 0) {
-                const TileRec64 T64 = tiles64[R.tile_base + m];
+                const TileRec64* T64 = tiles64 + R.tile_base + m;
                 const double Z = __builtin_ldexp((double)(int64_t)(ut - ((uint64_t)T.hc << 32)), -64);
-                const double rho = H / Z;
-                double p = T64.c[kCoef64 - 1];
-#pragma unroll
-                for (int c = kCoef64 - 2; c >= 0; --c) p = __builtin_fma(p, rho, T64.c[c]);
-                ex = rho * __builtin_fabs(rho) * p;
-                ek = __builtin_copysign(1.0, Z) * T64.m0 - S.C * (Z * T64.m0 + H * T64.m1) + ex;
+                ex = ref_value(T64, H / Z);
+                ek = __builtin_copysign(1.0, Z) * T64->m0 - S.C * (Z * T64->m0 + H * T64->m1) + ex;
             } else {
                 ek = 0.0;
                 ex = 0.0;
@@ -1032,8 +1056,10 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
                     ex += ws[j] * (tz - __builtin_copysign(1.0, z));
                 }
             }
-            efield += 0.5 * ek;
-            exg += ex;
+            if (!gfast) {
+                efield += 0.5 * ek;
+                exg += ex;
+            }
             if (kf == 0) continue;
             float acc = 0.0f;
             if (qx > 0)
@@ -1048,7 +1074,7 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
                 }
             const double ep = 0.5 * (__builtin_ldexp((double)acc, 64) - ex);
             const double ee = (qx < kf) ? 0.5 * ((double)tile_value(T, ht, hexp) - ex) : 0.0;
-            for (int q = 0; q < kf; ++q) et[q] += (q < qx) ? ep : ee;
+            for (int q = gfast ? kfg : 0; q < kf; ++q) et[q] += (q < qx) ? ep : ee;
         }
         const double eg = (qg < kfg) ? 0.5 * ((double)tile_value(GE, ht, hexp) - exg) : 0.0;
         for (int q = 0; q < kLadder; ++q) {
@@ -1428,6 +1454,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         o_tiles64_ = take(sizeof(TileRec64) * (size_t)std::max(ntiles_, 1));
         o_moms_ = take(sizeof(TileMom) * (size_t)std::max(ntiles_, 1));
         o_gexp_ = take(sizeof(TileRec) * (size_t)std::max(nrecs_, 1));
+        o_gexp64_ = take(sizeof(TileRec64) * (size_t)std::max(ngroups_, 1));
         o_gmoms_ = take(sizeof(TileMom) * (size_t)std::max(ngroups_, 1));
     }
     FS_TRY(work_.ensure(o, stream));
@@ -1655,7 +1682,8 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
     FS_HIP_CHECK(hipGetLastError());
     group_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
         n_total_, d_seg_of_, d_segs_, skey, ws, tiles, reinterpret_cast<GroupRec*>(wk + o_groups_),
-        reinterpret_cast<TileRec*>(wk + o_gexp_), reinterpret_cast<TileMom*>(wk + o_gmoms_));
+        reinterpret_cast<TileRec*>(wk + o_gexp_), reinterpret_cast<TileRec64*>(wk + o_gexp64_),
+        reinterpret_cast<TileMom*>(wk + o_gmoms_));
     FS_HIP_CHECK(hipGetLastError());
     for (int L = 0; L < kLevels; ++L) {
         if (nunits_[L] == 0) continue;
@@ -1677,7 +1705,7 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
         probe_kernel<<<dim3(kProbeMax, nseg_), kBlock, 0, stream>>>(
             d_segs_, skey, ws, far, tiles, reinterpret_cast<TileRec64*>(wk + o_tiles64_),
             reinterpret_cast<GroupRec*>(wk + o_groups_), reinterpret_cast<TileRec*>(wk + o_gexp_),
-            gov());
+            reinterpret_cast<TileRec64*>(wk + o_gexp64_), gov());
         FS_HIP_CHECK(hipGetLastError());
         governor_kernel<<<nblocks(nseg_), kBlock, 0, stream>>>(nseg_, d_segs_, gov(), gov_target_);
         FS_HIP_CHECK(hipGetLastError());
