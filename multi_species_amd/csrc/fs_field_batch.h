@@ -196,6 +196,8 @@ private:
     WorkItem* d_items_ = nullptr;
     ScanBlock* d_scan_ = nullptr;
     int32_t* d_seg_of_ = nullptr;
+    int32_t* d_tile_seg_ = nullptr;  // mixed: segment of each tile and of each group
+    int32_t* d_grp_seg_ = nullptr;
     int* d_counter_ = nullptr;
     // work space offsets (bytes into work_)
     size_t o_part_ = 0, o_key_ = 0, o_skey_ = 0, o_perm_ = 0, o_sort_ = 0, o_as_ = 0,

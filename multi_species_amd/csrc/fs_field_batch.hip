@@ -352,18 +352,18 @@ __device__ void expand_coeffs(const double* M, uint32_t hc, uint64_t emax, const
     }
 }
 
-__global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t* __restrict__ seg_of,
+// One thread per tile (tile_seg: the segment of each tile).
+__global__ __launch_bounds__(kBlock) void tile_kernel(int ntiles, const int32_t* __restrict__ tile_seg,
                                                       const SegDev* __restrict__ segs,
                                                       const uint64_t* __restrict__ us,
                                                       const double* __restrict__ ws,
                                                       TileRec* __restrict__ tiles,
                                                       TileRec64* __restrict__ tiles64,
                                                       TileMom* __restrict__ moms) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n_total) return;
-    const SegDev& S = segs[seg_of[i]];
-    if (!S.is_src || ((i - S.off) & (kTile - 1)) != 0) return;
-    const int t = S.tile_base + (i - S.off) / kTile;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= ntiles) return;
+    const SegDev& S = segs[tile_seg[t]];
+    const int i = S.off + (t - S.tile_base) * kTile;
     TileRec T;
     TileMom N;
     TileRec64 T64;
@@ -374,8 +374,10 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(int n_total, const int32_t
 }
 
 // Span and largest tile emax of each group of 4 tiles (after tile_kernel),
-// and the group's own expansions.
-__global__ __launch_bounds__(kBlock) void group_kernel(int n_total, const int32_t* __restrict__ seg_of,
+// and the group's own expansions: four adjacent lanes per group, one tile
+// each, combined by butterfly shuffles in a fixed order (grp_seg: the
+// segment of each group).
+__global__ __launch_bounds__(kBlock) void group_kernel(int ngroups, const int32_t* __restrict__ grp_seg,
                                                        const SegDev* __restrict__ segs,
                                                        const uint64_t* __restrict__ us,
                                                        const double* __restrict__ ws,
@@ -384,27 +386,54 @@ __global__ __launch_bounds__(kBlock) void group_kernel(int n_total, const int32_
                                                        TileRec* __restrict__ gexp,
                                                        TileRec64* __restrict__ gexp64,
                                                        TileMom* __restrict__ gmoms) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n_total) return;
-    const SegDev& S = segs[seg_of[i]];
-    if (!S.is_src || ((i - S.off) % kGroupSrc) != 0) return;
+    const int tid = blockIdx.x * kBlock + threadIdx.x;
+    const int g = tid >> 2, sub = tid & 3;
+    if (g >= ngroups) return;  // all four lanes of a group return together
+    const SegDev& S = segs[grp_seg[g]];
+    const int i = S.off + (g - S.group_base) * kGroupSrc;
     const int ge = min(i + kGroupSrc, S.off + S.n);
+    const int jb = i + sub * kTile, je = min(jb + kTile, ge);
+    uint64_t umin = ~uint64_t(0), umax = 0, temax = 0;
+    for (int j = jb; j < je; ++j) {
+        const uint64_t u = us[j];
+        umin = u < umin ? u : umin;
+        umax = u > umax ? u : umax;
+    }
+    if (jb < je) temax = tiles[S.tile_base + (jb - S.off) / kTile].emax;
+#pragma unroll
+    for (int m = 1; m < 4; m <<= 1) {
+        const uint64_t a = __shfl_xor(umin, m, 4), b = __shfl_xor(umax, m, 4);
+        const uint64_t c = __shfl_xor(temax, m, 4);
+        umin = a < umin ? a : umin;
+        umax = b > umax ? b : umax;
+        temax = c > temax ? c : temax;
+    }
+    const uint32_t hc = range_centre(umin, umax);
+    const uint64_t uc = (uint64_t)hc << 32;
+    double M[kMoments64];
+#pragma unroll
+    for (int k = 0; k < kMoments64; ++k) M[k] = 0.0;
+    uint64_t emax = 0;
+    for (int j = jb; j < je; ++j) add_moments(M, emax, uc, us[j], ws[j], S.log2h);
+#pragma unroll
+    for (int m = 1; m < 4; m <<= 1) {
+#pragma unroll
+        for (int k = 0; k < kMoments64; ++k) M[k] += __shfl_xor(M[k], m, 4);
+        const uint64_t e = __shfl_xor(emax, m, 4);
+        emax = e > emax ? e : emax;
+    }
+    if (sub != 0) return;
     GroupRec G;
     G.vf = us[i] & ~kLowSort;
     G.vl = us[ge - 1] | kLowSort;
-    G.emax = 0;
+    G.emax = temax;
     G.pad = 0;
-    for (int t = (i - S.off) / kTile; t * kTile < ge - S.off; ++t) {
-        const uint64_t e = tiles[S.tile_base + t].emax;
-        G.emax = e > G.emax ? e : G.emax;
-    }
-    const int g = S.group_base + (i - S.off) / kGroupSrc;
     groups[g] = G;
     // the group's own expansions about its centre (FP32 far, FP64 near)
     TileRec T;
     TileMom N;
     TileRec64 T64;
-    expand_range(us, ws, i, ge, S, T, N, &T64);
+    expand_coeffs(M, hc, emax, S, T, &N, &T64);
     gexp[g] = T;
     gexp64[g] = T64;
     gmoms[g] = N;
@@ -1259,6 +1288,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     n_total_ = n_total;
     nseg_ = (int)segs.size();
     for (int L = 0; L < kLevels; ++L) lstart_[L] = nunits_[L] = 0;
+    ntiles_ = ngroups_ = nrecs_ = 0;
     // mixed precision: a fixed number of targets per thread, so target
     // blocks are kBlock * kMixedTpt targets from the segment start whatever
     // the launch shape (the near/far choices are made per target block)
@@ -1402,17 +1432,35 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
                                           first, cnt});
         }
     }
+    // segment of each tile and of each group (mixed precision)
+    std::vector<int32_t> tile_seg(std::max(ntiles_, 1), 0), grp_seg(std::max(ngroups_, 1), 0);
+    if (precision == kPrecisionMixed)
+        for (int d = 0; d < nseg_; ++d) {
+            if (!segs_[d].is_src) continue;
+            const int nt = (segs_[d].n + kTile - 1) / kTile;
+            const int ng = (segs_[d].n + kGroupSrc - 1) / kGroupSrc;
+            for (int t = 0; t < nt; ++t) tile_seg[segs_[d].tile_base + t] = d;
+            for (int g = 0; g < ng; ++g) grp_seg[segs_[d].group_base + g] = d;
+        }
     // device tables
     const size_t b_segs = align_up(sizeof(SegDev) * nseg_, 256);
     const size_t b_scan = align_up(sizeof(ScanBlock) * std::max<size_t>(scan_.size(), 1), 256);
     const size_t b_segof = align_up(sizeof(int32_t) * seg_of.size(), 256);
     const size_t b_sym = align_up(sizeof(SymSeg) * std::max<size_t>(symsegs_.size(), 1), 256);
-    FS_TRY(tables_.ensure(b_segs + b_scan + b_segof + b_sym + 256, stream));
+    const size_t b_tseg = align_up(sizeof(int32_t) * tile_seg.size(), 256);
+    const size_t b_gseg = align_up(sizeof(int32_t) * grp_seg.size(), 256);
+    FS_TRY(tables_.ensure(b_segs + b_scan + b_segof + b_sym + b_tseg + b_gseg + 256, stream));
     d_segs_ = tables_.as<SegDev>(0);
     d_scan_ = tables_.as<ScanBlock>(b_segs);
     d_seg_of_ = tables_.as<int32_t>(b_segs + b_scan);
     d_symsegs_ = tables_.as<SymSeg>(b_segs + b_scan + b_segof);
-    d_counter_ = tables_.as<int>(b_segs + b_scan + b_segof + b_sym);
+    d_tile_seg_ = tables_.as<int32_t>(b_segs + b_scan + b_segof + b_sym);
+    d_grp_seg_ = tables_.as<int32_t>(b_segs + b_scan + b_segof + b_sym + b_tseg);
+    d_counter_ = tables_.as<int>(b_segs + b_scan + b_segof + b_sym + b_tseg + b_gseg);
+    FS_HIP_CHECK(hipMemcpyAsync(d_tile_seg_, tile_seg.data(), sizeof(int32_t) * tile_seg.size(),
+                                hipMemcpyHostToDevice, stream));
+    FS_HIP_CHECK(hipMemcpyAsync(d_grp_seg_, grp_seg.data(), sizeof(int32_t) * grp_seg.size(),
+                                hipMemcpyHostToDevice, stream));
     if (!symsegs_.empty())
         FS_HIP_CHECK(hipMemcpyAsync(d_symsegs_, symsegs_.data(), sizeof(SymSeg) * symsegs_.size(),
                                     hipMemcpyHostToDevice, stream));
@@ -1676,12 +1724,15 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
                                                             perm, w, as, asm1, ws, far);
     FS_HIP_CHECK(hipGetLastError());
     TileRec* tiles = reinterpret_cast<TileRec*>(wk + o_tiles_);
-    tile_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
-        n_total_, d_seg_of_, d_segs_, skey, ws, tiles, reinterpret_cast<TileRec64*>(wk + o_tiles64_),
-        reinterpret_cast<TileMom*>(wk + o_moms_));
-    FS_HIP_CHECK(hipGetLastError());
-    group_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
-        n_total_, d_seg_of_, d_segs_, skey, ws, tiles, reinterpret_cast<GroupRec*>(wk + o_groups_),
+    if (ntiles_ > 0) {
+        tile_kernel<<<nblocks(ntiles_), kBlock, 0, stream>>>(
+            ntiles_, d_tile_seg_, d_segs_, skey, ws, tiles,
+            reinterpret_cast<TileRec64*>(wk + o_tiles64_), reinterpret_cast<TileMom*>(wk + o_moms_));
+        FS_HIP_CHECK(hipGetLastError());
+    }
+    if (ngroups_ > 0)
+    group_kernel<<<nblocks(4 * (int64_t)ngroups_), kBlock, 0, stream>>>(
+        ngroups_, d_grp_seg_, d_segs_, skey, ws, tiles, reinterpret_cast<GroupRec*>(wk + o_groups_),
         reinterpret_cast<TileRec*>(wk + o_gexp_), reinterpret_cast<TileRec64*>(wk + o_gexp64_),
         reinterpret_cast<TileMom*>(wk + o_gmoms_));
     FS_HIP_CHECK(hipGetLastError());
