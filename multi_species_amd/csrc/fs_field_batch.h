@@ -198,6 +198,7 @@ private:
     int32_t* d_seg_of_ = nullptr;
     int32_t* d_tile_seg_ = nullptr;  // mixed: segment of each tile and of each group
     int32_t* d_grp_seg_ = nullptr;
+    int32_t* d_unit_seg_ = nullptr;  // mixed: segment of each far-field unit record
     int* d_counter_ = nullptr;
     // work space offsets (bytes into work_)
     size_t o_part_ = 0, o_key_ = 0, o_skey_ = 0, o_perm_ = 0, o_sort_ = 0, o_as_ = 0,

@@ -329,9 +329,9 @@ __device__ void expand_coeffs(const double* M, uint32_t hc, uint64_t emax, const
         T64->m1 = M[1];
     }
     // coefficient of rho^(m+2): n = m + 2 = 2i + k
-    for (int m = 0; m < kCoef64; ++m) {
+    for (int m = 0; m < (T64 ? kCoef64 : kCoef); ++m) {
         double c32 = 0.0, c64 = 0.0;
-        for (int ii = 1; ii <= kTerms64; ++ii) {
+        for (int ii = 1; ii <= (T64 ? kTerms64 : 3); ++ii) {
             const int k = m + 2 - 2 * ii;
             if (k < 0 || k >= kMoments64) continue;
             const double t = a[ii - 1] * kBinom[ii - 1][k] * ((k & 1) ? -M[k] : M[k]);
@@ -439,83 +439,63 @@ __global__ __launch_bounds__(kBlock) void group_kernel(int ngroups, const int32_
     gmoms[g] = N;
 }
 
-// Span and FP32 expansion of each far-field unit of one level (one
-// workgroup per unit): the keys' extremes, then the 13 moments and the
-// largest distance about the centre, summed per thread over a fixed stride
-// and reduced in a fixed order. The record's emax is the unit's own.
-__global__ __launch_bounds__(kBlock) void unit_kernel(int level, int lstart, int nseg,
+// Span and FP32 expansion of each far-field unit of one level: one
+// wavefront per unit (unit_seg: the segment of each unit record), the keys'
+// extremes, then the 13 moments and the largest distance about the centre,
+// summed per lane over a fixed stride and reduced by butterfly shuffles in a
+// fixed order. The record's emax is the unit's own.
+__global__ __launch_bounds__(kBlock) void unit_kernel(int level, int lstart, int nunits,
+                                                      int ngroups,
+                                                      const int32_t* __restrict__ unit_seg,
                                                       const SegDev* __restrict__ segs,
                                                       const uint64_t* __restrict__ us,
                                                       const double* __restrict__ ws,
                                                       GroupRec* __restrict__ groups,
                                                       TileRec* __restrict__ gexp) {
-    __shared__ uint64_t smin[kBlock], smax[kBlock];
-    __shared__ double sm[kMoments64][kBlock];
-    const int ua = lstart + (int)blockIdx.x;  // record index (after the groups)
+    const int tid = blockIdx.x * kBlock + threadIdx.x;
+    const int wu = tid >> 6, lane = tid & 63;
+    if (wu >= nunits) return;  // whole wavefronts return together
+    const int ua = lstart + wu;  // record index (after the groups)
     const int usrc = unit_src(level);
-    int d = -1;
-    for (int k = 0; k < nseg; ++k) {
-        const SegDev& Q = segs[k];
-        const int nu = (Q.n + usrc - 1) / usrc;
-        if (Q.is_src && ua >= Q.unit_base[level] && ua < Q.unit_base[level] + nu) d = k;
-    }
-    if (d < 0) return;
-    const SegDev S = segs[d];
+    const SegDev& S = segs[unit_seg[ua - ngroups]];
     const int b = S.off + (ua - S.unit_base[level]) * usrc;
     const int e = min(b + usrc, S.off + S.n);
     uint64_t umin = ~uint64_t(0), umax = 0;
-    for (int j = b + (int)threadIdx.x; j < e; j += kBlock) {
+    for (int j = b + lane; j < e; j += 64) {
         const uint64_t u = us[j];
         umin = u < umin ? u : umin;
         umax = u > umax ? u : umax;
     }
-    smin[threadIdx.x] = umin;
-    smax[threadIdx.x] = umax;
-    __syncthreads();
-    for (int h = kBlock / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-            const uint64_t a0 = smin[threadIdx.x + h], a1 = smax[threadIdx.x + h];
-            smin[threadIdx.x] = a0 < smin[threadIdx.x] ? a0 : smin[threadIdx.x];
-            smax[threadIdx.x] = a1 > smax[threadIdx.x] ? a1 : smax[threadIdx.x];
-        }
-        __syncthreads();
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const uint64_t a0 = __shfl_xor(umin, m), a1 = __shfl_xor(umax, m);
+        umin = a0 < umin ? a0 : umin;
+        umax = a1 > umax ? a1 : umax;
     }
-    const uint32_t hc = range_centre(smin[0], smax[0]);
+    const uint32_t hc = range_centre(umin, umax);
     const uint64_t uc = (uint64_t)hc << 32;
     double M[kMoments64];
 #pragma unroll
     for (int k = 0; k < kMoments64; ++k) M[k] = 0.0;
     uint64_t emax = 0;
-    for (int j = b + (int)threadIdx.x; j < e; j += kBlock)
-        add_moments(M, emax, uc, us[j], ws[j], S.log2h);
-    __syncthreads();  // smin/smax[0] read by every thread above
+    for (int j = b + lane; j < e; j += 64) add_moments(M, emax, uc, us[j], ws[j], S.log2h);
 #pragma unroll
-    for (int k = 0; k < kMoments64; ++k) sm[k][threadIdx.x] = M[k];
-    smax[threadIdx.x] = emax;
-    __syncthreads();
-    for (int h = kBlock / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
+    for (int m = 1; m < 64; m <<= 1) {
 #pragma unroll
-            for (int k = 0; k < kMoments64; ++k) sm[k][threadIdx.x] += sm[k][threadIdx.x + h];
-            const uint64_t a1 = smax[threadIdx.x + h];
-            smax[threadIdx.x] = a1 > smax[threadIdx.x] ? a1 : smax[threadIdx.x];
-        }
-        __syncthreads();
+        for (int k = 0; k < kMoments64; ++k) M[k] += __shfl_xor(M[k], m);
+        const uint64_t a1 = __shfl_xor(emax, m);
+        emax = a1 > emax ? a1 : emax;
     }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < kMoments64; ++k) M[k] = sm[k][0];
-        TileRec T;
-        expand_coeffs(M, hc, smax[0], S, T, nullptr, nullptr);
-        const int u = ua;
-        GroupRec G;
-        G.vf = us[b] & ~kLowSort;
-        G.vl = us[e - 1] | kLowSort;
-        G.emax = T.emax;
-        G.pad = 0;
-        groups[u] = G;
-        gexp[u] = T;
-    }
+    if (lane != 0) return;
+    TileRec T;
+    expand_coeffs(M, hc, emax, S, T, nullptr, nullptr);
+    GroupRec G;
+    G.vf = us[b] & ~kLowSort;
+    G.vl = us[e - 1] | kLowSort;
+    G.emax = T.emax;
+    G.pad = 0;
+    groups[ua] = G;
+    gexp[ua] = T;
 }
 
 // The FP32 value of one expanded tile at a target (high word ht): the pair
@@ -1434,6 +1414,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     }
     // segment of each tile and of each group (mixed precision)
     std::vector<int32_t> tile_seg(std::max(ntiles_, 1), 0), grp_seg(std::max(ngroups_, 1), 0);
+    std::vector<int32_t> unit_seg(std::max(nrecs_ - ngroups_, 1), 0);
     if (precision == kPrecisionMixed)
         for (int d = 0; d < nseg_; ++d) {
             if (!segs_[d].is_src) continue;
@@ -1441,6 +1422,10 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
             const int ng = (segs_[d].n + kGroupSrc - 1) / kGroupSrc;
             for (int t = 0; t < nt; ++t) tile_seg[segs_[d].tile_base + t] = d;
             for (int g = 0; g < ng; ++g) grp_seg[segs_[d].group_base + g] = d;
+            for (int L = 0; L < kLevels; ++L) {
+                const int nu = (segs_[d].n + unit_src(L) - 1) / unit_src(L);
+                for (int u = 0; u < nu; ++u) unit_seg[segs_[d].unit_base[L] - ngroups_ + u] = d;
+            }
         }
     // device tables
     const size_t b_segs = align_up(sizeof(SegDev) * nseg_, 256);
@@ -1449,14 +1434,19 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     const size_t b_sym = align_up(sizeof(SymSeg) * std::max<size_t>(symsegs_.size(), 1), 256);
     const size_t b_tseg = align_up(sizeof(int32_t) * tile_seg.size(), 256);
     const size_t b_gseg = align_up(sizeof(int32_t) * grp_seg.size(), 256);
-    FS_TRY(tables_.ensure(b_segs + b_scan + b_segof + b_sym + b_tseg + b_gseg + 256, stream));
+    const size_t b_useg = align_up(sizeof(int32_t) * unit_seg.size(), 256);
+    FS_TRY(tables_.ensure(b_segs + b_scan + b_segof + b_sym + b_tseg + b_gseg + b_useg + 256,
+                          stream));
     d_segs_ = tables_.as<SegDev>(0);
     d_scan_ = tables_.as<ScanBlock>(b_segs);
     d_seg_of_ = tables_.as<int32_t>(b_segs + b_scan);
     d_symsegs_ = tables_.as<SymSeg>(b_segs + b_scan + b_segof);
     d_tile_seg_ = tables_.as<int32_t>(b_segs + b_scan + b_segof + b_sym);
     d_grp_seg_ = tables_.as<int32_t>(b_segs + b_scan + b_segof + b_sym + b_tseg);
-    d_counter_ = tables_.as<int>(b_segs + b_scan + b_segof + b_sym + b_tseg + b_gseg);
+    d_unit_seg_ = tables_.as<int32_t>(b_segs + b_scan + b_segof + b_sym + b_tseg + b_gseg);
+    d_counter_ = tables_.as<int>(b_segs + b_scan + b_segof + b_sym + b_tseg + b_gseg + b_useg);
+    FS_HIP_CHECK(hipMemcpyAsync(d_unit_seg_, unit_seg.data(), sizeof(int32_t) * unit_seg.size(),
+                                hipMemcpyHostToDevice, stream));
     FS_HIP_CHECK(hipMemcpyAsync(d_tile_seg_, tile_seg.data(), sizeof(int32_t) * tile_seg.size(),
                                 hipMemcpyHostToDevice, stream));
     FS_HIP_CHECK(hipMemcpyAsync(d_grp_seg_, grp_seg.data(), sizeof(int32_t) * grp_seg.size(),
@@ -1738,9 +1728,9 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
     FS_HIP_CHECK(hipGetLastError());
     for (int L = 0; L < kLevels; ++L) {
         if (nunits_[L] == 0) continue;
-        unit_kernel<<<nunits_[L], kBlock, 0, stream>>>(
-            L, lstart_[L], nseg_, d_segs_, skey, ws, reinterpret_cast<GroupRec*>(wk + o_groups_),
-            reinterpret_cast<TileRec*>(wk + o_gexp_));
+        unit_kernel<<<nblocks(64 * (int64_t)nunits_[L]), kBlock, 0, stream>>>(
+            L, lstart_[L], nunits_[L], ngroups_, d_unit_seg_, d_segs_, skey, ws,
+            reinterpret_cast<GroupRec*>(wk + o_groups_), reinterpret_cast<TileRec*>(wk + o_gexp_));
         FS_HIP_CHECK(hipGetLastError());
     }
     if (!scan_.empty()) {
