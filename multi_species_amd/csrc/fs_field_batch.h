@@ -60,6 +60,7 @@ struct SegDev {
     // record in the group arrays (after every group), and its source's
     int unit_base[2];
     int src_unit_base[2];
+    int levels;       // unit levels the pair kernel uses for this segment's sources (0..2)
 };
 
 // Span of 4 consecutive tiles (128 sorted sources) and their largest emax:

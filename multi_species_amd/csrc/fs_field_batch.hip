@@ -54,7 +54,12 @@ constexpr int kGroupSrc = 4 * kTile;   // mixed: sources per group
 // the largest, so every unit lies inside one chunk
 constexpr int kLevels = 2;
 __device__ __host__ constexpr int unit_src(int level) { return kGroupSrc << (2 * (level + 1)); }
-constexpr int kTopSrc = kGroupSrc << (2 * kLevels);  // the mixed chunk unit
+// Unit levels used for a source segment of n points (from its size alone,
+// so a deck's results do not depend on its batch): small decks keep finer
+// source chunks (more work items) instead of the larger units. The mixed
+// chunk unit is the largest unit in use, so no unit straddles two chunks.
+inline int unit_levels(int64_t n) { return n >= (int64_t(1) << 16) ? 2 : n >= (int64_t(1) << 13) ? 1 : 0; }
+inline int chunk_unit_src(int64_t n) { return kGroupSrc << (2 * unit_levels(n)); }
 // mixed: targets per thread, fixed, since the kernel chooses each source
 // group's path for a whole work item (kBlock * kMixedTpt sorted targets): a
 // tuned value would make results depend on timing
@@ -701,7 +706,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
             for (int L = kLevels - 1; L >= 0; --L) {
               const int rel = gb - S.src_off;
-              if (step != 0 || (rel & (unit_src(L) - 1)) != 0) continue;
+              if (L >= S.levels || step != 0 || (rel & (unit_src(L) - 1)) != 0) continue;
               const int uix = S.src_unit_base[L] + rel / unit_src(L);
               const GroupRec U = groups[uix];
               const uint64_t uspan = U.vl - U.vf;
@@ -997,7 +1002,7 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
             const uint64_t uspan = U.vl - U.vf;
             const uint64_t ulen = tspan + uspan, ulo = uf - U.vl;
             kfu[L] = 0;
-            if (tspan < kHalf && uspan < kHalf && ulen < kHalf &&
+            if (L < S.levels && tspan < kHalf && uspan < kHalf && ulen < kHalf &&
                 !cyc_meets(ulo - kHalf + kAntiMargin, ulen, 2 * kAntiMargin))
                 while (kfu[L] < kLadder && band[kfu[L]] != 0 &&
                        !cyc_meets(ulo + band[kfu[L]], ulen, 2 * band[kfu[L]]))
@@ -1221,13 +1226,13 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 void chunking(int precision, int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun,
               int& splits, int& chunk) {
     const int64_t want_waves = int64_t(64) * num_cus;
-    const int unit = precision == kPrecisionF64 ? kSub : kTopSrc;
+    const int unit = precision == kPrecisionF64 ? kSub : chunk_unit_src(ns);
     const int tpt_auto = (nt >= want_waves * 64 * 2) ? 2 : 1;
     const int64_t waves = (nt + 64 * tpt_auto - 1) / (64 * tpt_auto);
     int64_t sp = tun.source_splits > 0 ? tun.source_splits
                                        : (want_waves + waves - 1) / std::max<int64_t>(waves, 1);
     const int64_t max_splits =
-        std::max<int64_t>(1, ns / (precision == kPrecisionF64 ? kSub : kTopSrc));
+        std::max<int64_t>(1, ns / unit);
     sp = std::min(std::max<int64_t>(sp, 1), max_splits);
     sp = std::min<int64_t>(sp, 65535);
     int64_t ch = (ns + sp - 1) / sp;
@@ -1337,6 +1342,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
             S.all_band0 = S.all_band;
             S.band0 = S.band;
             S.xband = S.band >> kXShift;
+            S.levels = unit_levels(S.src_n);
             S.log2h = (int)std::floor(std::log2(B));
             S.d2s = d2 * two64 * two64;
             const double kq2 = d2 * two64;
