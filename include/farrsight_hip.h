@@ -74,8 +74,9 @@ int fs_emq_treecode_f64(fs_ctx* ctx, double* es, const double* sources, const do
  * later calls on this context. */
 int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits);
 
-/* FP64 direct sum with the targets being the sources (the same array, or the
- * same host pointer, and nt == ns): the kernel is odd, so each pair can be
+/* FP64 direct sum with the targets being the sources (nt == ns and the same
+ * pointer; for the host entry point also two arrays of identical contents,
+ * as the reference passes them): the kernel is odd, so each pair can be
  * evaluated once and given to both points. mode 0 (default) = from 16384
  * points up, 1 = never, 2 = at every size. No counterpart in the reference
  * (its E_MQ_DirectSum evaluates every ordered pair); the result agrees with

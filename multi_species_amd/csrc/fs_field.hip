@@ -22,6 +22,8 @@
 // second kernel in split order, so results are deterministic and
 // independent of co-scheduled work. Accumulation: a plain FP64 sum over each
 // run of kSub sources, folded into a TwoSum (hi, lo) pair per target.
+#include <cstring>
+
 #include "fs_internal.h"
 #include "fs_sym.h"
 
@@ -316,9 +318,11 @@ int fs_emq_periodic_f64(fs_ctx* ctx, double* es, const double* targets, int64_t 
     double* d_e = ctx->io.as<double>(bt);
     double* d_s = ctx->io.as<double>(2 * bt);
     double* d_w = ctx->io.as<double>(2 * bt + bs);
-    // the same host array as targets and sources: one device copy (and the
-    // pair-once form of the sum)
-    const bool same = targets == sources && nt == ns;
+    // targets and sources the same positions (the same array, or two copies
+    // of it, as the reference passes them): one device copy, and the
+    // pair-once form of the sum
+    const bool same = nt == ns && (targets == sources ||
+                                   std::memcmp(targets, sources, sizeof(double) * nt) == 0);
     if (same)
         d_t = d_s;
     else

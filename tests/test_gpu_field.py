@@ -113,6 +113,23 @@ def test_pair_once_ragged(gpu_ctx, n, tpt):
         assert rel(e, ex) <= 1e-14, rel(e, ex)
 
 
+def test_pair_once_with_copied_positions(gpu_ctx):
+    """The reference passes two copies of the same positions as targets and
+    sources: identical contents take the pair-once form like one array."""
+    n = 40000
+    x, w = c2_deck(n, seed=5)
+    e1 = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+    ev1 = gpu_ctx.f64_pair_evals()
+    e2 = gpu_ctx.emq_periodic_f64(x.copy(), x.copy(), w, L0, 0.05)
+    ev2 = gpu_ctx.f64_pair_evals()
+    assert np.array_equal(e1, e2)
+    assert ev1 == ev2 and ev2 < n * n * 0.6
+    y = x.copy()
+    y[123] += 1e-9  # not the same positions: the full sum
+    gpu_ctx.emq_periodic_f64(x, y, w, L0, 0.05)
+    assert gpu_ctx.f64_pair_evals() >= n * n
+
+
 @pytest.mark.parametrize("splits", [0, 1, 3, 1000])
 def test_pair_once_matches_full_sum(gpu_ctx, splits):
     """The pair-once form against the full sum of every ordered pair (same
