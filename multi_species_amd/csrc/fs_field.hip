@@ -116,10 +116,10 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(double* __restrict__ a, in
 // ---- targets = sources: each pair once (fs_sym.h) ----------------------
 
 template <int T>
-__global__ __launch_bounds__(sym::kThreads) void emq_sym_f64_kernel(
+__global__ __launch_bounds__(sym::threads<T>()) void emq_sym_f64_kernel(
     const double* __restrict__ x, const double* __restrict__ w, sym::Plan P, double invL,
     double delta2, double* __restrict__ rowpart, double* __restrict__ colpart) {
-    __shared__ double red[2][sym::kWaves][64];
+    __shared__ double red[2][sym::kMaxWaves][64];
     sym::item<T>(x, w, P, blockIdx.x, invL, delta2, rowpart, colpart, red);
 }
 
@@ -212,15 +212,15 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src, const
                                                     sym_rowpart_bytes(P));
         switch (P.t) {
             case 4:
-                emq_sym_f64_kernel<4><<<P.nitems, sym::kThreads, 0, stream>>>(
+                emq_sym_f64_kernel<4><<<P.nitems, sym::threads<4>(), 0, stream>>>(
                     d_src, d_w, P, invL, delta2, rowpart, colpart);
                 break;
             case 2:
-                emq_sym_f64_kernel<2><<<P.nitems, sym::kThreads, 0, stream>>>(
+                emq_sym_f64_kernel<2><<<P.nitems, sym::threads<2>(), 0, stream>>>(
                     d_src, d_w, P, invL, delta2, rowpart, colpart);
                 break;
             default:
-                emq_sym_f64_kernel<1><<<P.nitems, sym::kThreads, 0, stream>>>(
+                emq_sym_f64_kernel<1><<<P.nitems, sym::threads<1>(), 0, stream>>>(
                     d_src, d_w, P, invL, delta2, rowpart, colpart);
                 break;
         }

@@ -141,10 +141,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
 // Pair-once segments (fs_sym.h) of one targets-per-lane value: workgroup
 // blockIdx.x belongs to the last table entry whose item_base is not above it.
 template <int T>
-__global__ __launch_bounds__(sym::kThreads) void batch_sym_kernel(
+__global__ __launch_bounds__(sym::threads<T>()) void batch_sym_kernel(
     const SymSeg* __restrict__ tab, int first, int count, const double* __restrict__ x,
     const double* __restrict__ w, double* __restrict__ sympart) {
-    __shared__ double red[2][sym::kWaves][64];
+    __shared__ double red[2][sym::kMaxWaves][64];
     int lo = first, hi = first + count - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -1688,13 +1688,13 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
         FS_TRY(launch_pairs(nit, grid_, stream));
         double* sympart = reinterpret_cast<double*>(wk + o_sympart_);
         if (sym_items_[0] > 0)
-            batch_sym_kernel<1><<<sym_items_[0], sym::kThreads, 0, stream>>>(
+            batch_sym_kernel<1><<<sym_items_[0], sym::threads<1>(), 0, stream>>>(
                 d_symsegs_, sym_first_[0], sym_count_[0], x, w, sympart);
         if (sym_items_[1] > 0)
-            batch_sym_kernel<2><<<sym_items_[1], sym::kThreads, 0, stream>>>(
+            batch_sym_kernel<2><<<sym_items_[1], sym::threads<2>(), 0, stream>>>(
                 d_symsegs_, sym_first_[1], sym_count_[1], x, w, sympart);
         if (sym_items_[2] > 0)
-            batch_sym_kernel<4><<<sym_items_[2], sym::kThreads, 0, stream>>>(
+            batch_sym_kernel<4><<<sym_items_[2], sym::threads<4>(), 0, stream>>>(
                 d_symsegs_, sym_first_[2], sym_count_[2], x, w, sympart);
         FS_HIP_CHECK(hipGetLastError());
         finalize_f64_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(

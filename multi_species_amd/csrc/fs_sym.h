@@ -6,7 +6,8 @@
 // rint rounds half to even, which is symmetric), so the pair (i, j) gives
 // w_j K to target i and -w_i K to target j: half the kernel evaluations of
 // the full sum (reference src/FieldStructure.cpp:106-156 evaluates every
-// ordered pair). Points are cut into row blocks of kSymWaves * 64 * T points.
+// ordered pair). Points are cut into row blocks of W * 64 * T points (W
+// wavefronts per workgroup: 4 at one target per lane, else 8).
 // A workgroup takes one row block's targets (T per lane) and a chunk of the
 // source blocks at cyclic block distances d = 1..H_b from it (H = (nb - 1)/2,
 // plus the distance nb/2 for the first half of the blocks when nb is even:
@@ -15,7 +16,7 @@
 // tile each lane holds one source (x_j, w_j and its accumulator) and the three
 // move one lane on per step (DPP wavefront rotation), so after 64 steps every
 // lane has met every source of the tile and each source's accumulator is back
-// in its own lane; the workgroup's 8 wavefronts add their accumulators of a
+// in its own lane; the workgroup's wavefronts add their accumulators of a
 // tile through LDS in a fixed order. Targets keep their sums in registers
 // (plain FP64 over a tile, TwoSum across tiles). Every item writes its row
 // partials and its sources' column partials to their own slots, and the
@@ -32,8 +33,11 @@
 namespace fs {
 namespace sym {
 
-constexpr int kWaves = 8;
-constexpr int kThreads = 64 * kWaves;
+// Wavefronts per workgroup: 4 for one target per lane (the decks below
+// 65536 points, where more, smaller row blocks keep the chip busy), else 8.
+template <int T> constexpr int waves() { return T == 1 ? 4 : 8; }
+template <int T> constexpr int threads() { return 64 * waves<T>(); }
+constexpr int kMaxWaves = 8;
 // Points from which the pair-once form is the default (below it the row
 // blocks are too few to fill the chip and the full sum is faster).
 constexpr int64_t kMinPoints = 16384;
@@ -41,7 +45,7 @@ constexpr int64_t kMinPoints = 16384;
 struct Plan {
     int n;       // points
     int t;       // targets per lane (1, 2, 4)
-    int rb;      // points per row block, kThreads * t
+    int rb;      // points per row block, threads<t>() * t
     int nb;      // row blocks
     int h;       // (nb - 1) / 2
     int even;    // nb even: blocks b < nb/2 also take the distance nb/2
@@ -64,7 +68,7 @@ inline Plan plan(int64_t n, int num_cus, const FieldTuning& tun) {
         P.t = tun.targets_per_thread;
     else
         P.t = n >= (int64_t(1) << 18) ? 4 : n >= (int64_t(1) << 16) ? 2 : 1;
-    P.rb = kThreads * P.t;
+    P.rb = (P.t == 1 ? threads<1>() : P.t == 2 ? threads<2>() : threads<4>()) * P.t;
     P.nb = (int)((n + P.rb - 1) / P.rb);
     P.h = (P.nb - 1) / 2;
     P.even = (P.nb % 2 == 0) ? 1 : 0;
@@ -105,14 +109,15 @@ __device__ __forceinline__ double wave_rotate(double v) {
 
 // One item (row block and source chunk, or a block with itself) of the plan
 // P over the points x[0..n), w[0..n). red: the workgroup's LDS,
-// [2][kWaves][64] doubles. Called by all kThreads threads of a workgroup.
+// [2][kMaxWaves][64] doubles. Called by all threads<T>() threads of a workgroup.
 template <int T>
 __device__ __forceinline__ void item(const double* __restrict__ x, const double* __restrict__ w,
                                      const Plan& P, int id, double invL, double delta2,
                                      double* __restrict__ rowpart, double* __restrict__ colpart,
-                                     double (*red)[kWaves][64]) {
+                                     double (*red)[kMaxWaves][64]) {
+    constexpr int kWaves = waves<T>();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int rb = kThreads * T;
+    const int rb = threads<T>() * T;
     int b, d0, d1;
     const bool self = id >= P.nsym;
     if (!self) {
