@@ -680,16 +680,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const SegDev S = segs[W.seg];
         const double d2 = S.delta2;
         const int hexp = -32 - S.log2h;
-        // positions are sorted by their top bits only: widen the block's range
-        // by one unit of the sort on each side
-        const uint64_t uf = us[W.tb] & ~kLowSort, ul = us[W.te - 1] | kLowSort;
+        // each wavefront takes 64 * TPT consecutive sorted targets of the item
+        // and makes its own near/far choices (uniform over the wavefront: no
+        // divergence, and scalar as long as the wavefront index is known to
+        // be uniform): the narrower the target range, the fewer sources need
+        // pairs or the smaller ranges
+        const int lane = (int)threadIdx.x & 63;
+        const int wtb = W.tb + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) * 64 * TPT;
+        const int wte = min(W.te, wtb + 64 * TPT);
+        if (wtb >= wte) continue;
+        // positions are sorted by their top bits only: widen the range by one
+        // unit of the sort on each side
+        const uint64_t uf = us[wtb] & ~kLowSort, ul = us[wte - 1] | kLowSort;
         const uint64_t tspan = ul - uf;
         uint64_t ui[TPT];
         uint32_t hi[TPT];
         double ai[TPT], sh[TPT], sl[TPT], gx[TPT];
 #pragma unroll
         for (int t = 0; t < TPT; ++t) {
-            const int i = min(W.tb + (int)threadIdx.x + t * kBlock, W.te - 1);
+            const int i = min(wtb + lane + t * 64, wte - 1);
             ui[t] = us[i];
             hi[t] = (uint32_t)(ui[t] >> 32);
             ai[t] = as[i];
@@ -890,8 +899,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
         for (int t = 0; t < TPT; ++t) {
             two_sum_acc(sh[t], sl[t], gx[t]);
-            const int i = W.tb + (int)threadIdx.x + t * kBlock;
-            if (i < W.te) part[S.part_base + (int64_t)W.split * S.n + (i - S.off)] = sh[t] + sl[t];
+            const int i = wtb + lane + t * 64;
+            if (i < wte) part[S.part_base + (int64_t)W.split * S.n + (i - S.off)] = sh[t] + sl[t];
         }
     }
 }
