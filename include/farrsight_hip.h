@@ -69,8 +69,8 @@ int fs_emq_treecode_f64(fs_ctx* ctx, double* es, const double* sources, const do
                         int max_source, int64_t* evals);
 
 /* Tuning knobs of the direct-sum kernels (0 = automatic). targets_per_thread
- * in {1,2,4} (the mixed-precision kernels always take 2: their near/far
- * choices are made per block of 512 sorted targets); source_splits >= 1. Applies to
+ * in {1,2,4} (the mixed-precision kernels always take 4: each wavefront
+ * makes its near/far choices for its 256 consecutive sorted targets); source_splits >= 1. Applies to
  * later calls on this context. */
 int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits);
 

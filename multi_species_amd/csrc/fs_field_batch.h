@@ -148,7 +148,7 @@ public:
     // The last governed run's measurements (nseg * kGovVals values: per
     // segment max|E| and the error at each candidate band, as doubles).
     int get_gov(std::vector<double>& out, hipStream_t stream) const;
-    // Launch shape of the pair kernel: targets per thread (1, 2, 4; always 2
+    // Launch shape of the pair kernel: targets per thread (1, 2, 4; always 4
     // in mixed precision) and persistent workgroups per CU. Does not change
     // any result.
     int set_shape(int tpt, int wgs_per_cu, hipStream_t stream);

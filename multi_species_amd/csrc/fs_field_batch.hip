@@ -4,7 +4,7 @@
 // The field of each deck is the reference's E_MQ_DirectSum
 // (src/FieldStructure.cpp:106-156) over that deck's points. The work of all
 // decks is cut into items of (deck, block of 256*TPT targets, chunk of
-// sources; TPT is 2 in mixed precision); a persistent kernel sized to the device pulls items from an
+// sources; TPT is 4 in mixed precision); a persistent kernel sized to the device pulls items from an
 // atomic counter, largest first, so decks of very different sizes (2k to
 // 500k points) share the chip without a launch per deck. Every item writes
 // its own partial sum; a finalize kernel adds the partials of each target in
@@ -60,10 +60,10 @@ __device__ __host__ constexpr int unit_src(int level) { return kGroupSrc << (2 *
 // chunk unit is the largest unit in use, so no unit straddles two chunks.
 inline int unit_levels(int64_t n) { return n >= (int64_t(1) << 16) ? 2 : n >= (int64_t(1) << 13) ? 1 : 0; }
 inline int chunk_unit_src(int64_t n) { return kGroupSrc << (2 * unit_levels(n)); }
-// mixed: targets per thread, fixed, since the kernel chooses each source
-// group's path for a whole work item (kBlock * kMixedTpt sorted targets): a
-// tuned value would make results depend on timing
-constexpr int kMixedTpt = 2;
+// mixed: targets per thread, fixed, since each wavefront chooses each source
+// range's path for its 64 * kMixedTpt consecutive sorted targets: a tuned
+// value would make results depend on timing
+constexpr int kMixedTpt = 4;
 constexpr int kXShift = 4;             // tiles expand when their sources lie within band/16
 constexpr int kMoments = 8;            // moments of a tile's expansion
 constexpr int kCoef = kMoments + 4;    // its polynomial coefficients (TileRec::c)
@@ -1283,9 +1283,9 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     nseg_ = (int)segs.size();
     for (int L = 0; L < kLevels; ++L) lstart_[L] = nunits_[L] = 0;
     ntiles_ = ngroups_ = nrecs_ = 0;
-    // mixed precision: a fixed number of targets per thread, so target
-    // blocks are kBlock * kMixedTpt targets from the segment start whatever
-    // the launch shape (the near/far choices are made per target block)
+    // mixed precision: a fixed number of targets per thread, so each
+    // wavefront's targets are the same whatever the launch shape (its
+    // near/far choices are made for them)
     tpt_ = precision == kPrecisionMixed ? kMixedTpt
            : (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
               tun.targets_per_thread == 4)
@@ -1615,7 +1615,8 @@ int FieldBatch::autotune(const double* x, const double* w, double* es, hipStream
     int rc = FS_OK;
     // mixed precision: tpt fixed (its near/far choices are made per item)
     const bool mixed = precision_ == kPrecisionMixed;
-    for (int a = mixed ? 1 : 0; a < (mixed ? 2 : 3) && rc == FS_OK; ++a) {
+    const int a0 = mixed ? (kMixedTpt == 1 ? 0 : kMixedTpt == 2 ? 1 : 2) : 0;
+    for (int a = a0; a < (mixed ? a0 + 1 : 3) && rc == FS_OK; ++a) {
         // every item of the real launch (so the grid fills as it will), each
         // cut to a source count that keeps the trial near 2e10 pairs
         build_items(tpts[a], trial);
