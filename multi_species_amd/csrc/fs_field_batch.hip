@@ -656,8 +656,9 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(
     }
 }
 
-// Pair kernel. Per (target block, 32-source tile), one of three paths, the
-// choice uniform over the workgroup:
+// Pair kernel. Per (a wavefront's 256 consecutive sorted targets, range of
+// sources: far-field unit, group or 32-source tile), one path, the choice
+// uniform over the wavefront; for a tile one of:
 //  * near tile (some displacement within B of 0): FP64 from the FP64
 //    positions, z = a_i - a_j with the periodic seam handled per tile (the
 //    targets' a - 1 and the sources' a - 1 are exact where they are used);
@@ -708,7 +709,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
         for (int gb = W.sb; gb < W.se;) {
           // the largest far-field unit (2048 or 512 sources) starting here
-          // that is far from the whole target block and compact: one FP32
+          // that is far from all the wavefront's targets and compact: one FP32
           // expansion for all its sources
           if (tspan < kHalf && !S.all_band) {
             int step = 0;
@@ -746,7 +747,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 !cyc_meets(glo - kHalf + kAntiMargin, glen, 2 * kAntiMargin)) {
                 const uint64_t gemax = gexp[gix].emax;
                 if (!S.all_band && !cyc_meets(glo + S.band, glen, 2 * S.band)) {
-                    // a group far from the whole target block: its own FP32
+                    // a group far from all the wavefront's targets: its own FP32
                     // expansion when compact, else its tiles' when they are
                     if (gemax <= S.xband) {
                         const TileRec T = gexp[gix];
@@ -764,7 +765,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     }
                 } else if (!cyc_meets(glo, glen, 1) &&
                            near_compact(gemax, glo < kHalf ? glo : 0 - (glo + glen), d2)) {
-                    // a group on one side of the whole target block, within
+                    // a group on one side of all the wavefront's targets, within
                     // reach of the band and compact for its distance: its
                     // FP64 expansion
                     const uint64_t uc = (uint64_t)gexp[gix].hc << 32;
@@ -808,7 +809,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             } else if (near && !cyc_meets(lo, len, 1) &&
                        near_compact(tiles[S.src_tile_base + (jb - S.src_off) / kTile].emax,
                                     lo < kHalf ? lo : 0 - (lo + len), d2)) {
-                // near tile on one side of the whole target block, compact for
+                // near tile on one side of all the wavefront's targets, compact for
                 // its distance: its FP64 expansion about the centre (Legendre
                 // generating function; truncation below 16^-12 of sum |w|), and
                 // s = the side
@@ -986,7 +987,7 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const SegDev* __restrict_
     for (int g = threadIdx.x; g < ngroup; g += kBlock) {
         // the group as one: far at the bands q < kfg, its own expansion
         // compact at q >= qg (the pair kernel takes it when the group is far
-        // from its whole target block)
+        // from all the targets of a wavefront)
         const GroupRec G = groups[R.group_base + g];
         const TileRec GE = gexp[R.group_base + g];
         const uint64_t gspan = G.vl - G.vf;

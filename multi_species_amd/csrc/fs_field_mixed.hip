@@ -13,7 +13,8 @@
 //  * G_i = sum_j w_j g(z_ij) carries the rest. g is O(1) only within a few
 //    delta of z = 0 and decays like delta^2/(2 z^2) beyond. The sorted sources
 //    form 32-source tiles, 128-source groups and far-field units of 512 and
-//    2048 sources, and each (block of targets, unit, group or tile) takes
+//    2048 sources, and each (wavefront's 256 consecutive sorted targets,
+//    unit, group or tile) takes
 //    one of:
 //      - beyond the band B = max(2^-6, 16 delta, 4000 delta^2) (B >= 1/4 sends
 //        everything to FP64), the largest unit, else the group, else the
