@@ -112,7 +112,7 @@ int fs_emq_periodic_mixed(fs_ctx* ctx, double* es, const double* targets, int64_
 
 /* Error target of the mixed entry points on this context (no counterpart in
  * the reference). 0 (the default): the minimum band B above. > 0: before
- * its pair sums, each call measures on up to 256 probed targets the E error
+ * its pair sums, each call measures on up to 128 probed targets the E error
  * that each of 8 candidate bands (B/2 to 4 sqrt(2) B) would give against the
  * exact FP64 field, and takes the narrowest one within rel_err * max|E|
  * (every pair in FP64 when none is). At most 1e-3. */
@@ -261,7 +261,7 @@ int fs_sim_deck_info(fs_sim* sim, int deck, int* first_species, int* n_species, 
 int fs_sim_set_options(fs_sim* sim, int precision, int use_graph);
 /* Mixed precision: what the accuracy governor measured in the deck's last
  * full-field evaluation (every mixed evaluation measures its own error
- * before its pair sums: up to 256 probed targets against the exact FP64
+ * before its pair sums: up to 128 probed targets against the exact FP64
  * field). max_e = max|E| over the probed targets; errs[q] = the largest E
  * error there when pairs beyond bands[q] (fraction of the period; >= 1/4
  * means every pair in FP64) take the FP32 path, q = 0..7. The evaluation

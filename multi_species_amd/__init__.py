@@ -179,7 +179,7 @@ class Context:
 
     def set_mixed_target(self, rel_err: float):
         """Error target of the mixed calls on this context: 0 = minimum FP64
-        band, no probe; > 0 = each call first measures, at up to 256 targets
+        band, no probe; > 0 = each call first measures, at up to 128 targets
         per segment against the exact FP64 field, the error of each candidate
         band and takes the narrowest one within rel_err * max|E|."""
         _check(lib().fs_emq_set_mixed_target(self.h, rel_err), "fs_emq_set_mixed_target")

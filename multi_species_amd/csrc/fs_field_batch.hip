@@ -72,7 +72,7 @@ constexpr int kTerms64 = 7;            // terms of the series in d^2/z^2,
 constexpr int kCoef64 = 2 * kTerms64 + kMoments64 - 2;  // and coefficients (TileRec64::c)
 constexpr int kNearTerms = 12;         // near-band FP64 expansion: Legendre terms
 constexpr int kLadder = kGovVals - 1;  // governor: candidate bands
-constexpr int kProbeMax = 256;         // governor: probed targets per segment
+constexpr int kProbeMax = 128;         // governor: probed targets per segment
 constexpr double kGovTarget = 4e-9;    // governor: default error target (relative to max |E|)
 
 __device__ __forceinline__ int pull_item(int* counter) {
@@ -909,7 +909,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // ---- accuracy governor ----------------------------------------------------
 // Before the pair kernel of a governed run, the FP32 far-field error of each
 // evaluated segment is measured on the run's own inputs: a probe takes up to
-// 256 evenly spaced targets of the segment and sums, for each target,
+// 128 evenly spaced targets of the segment and sums, for each target,
 //  * the exact FP64 field (every source, the reference kernel on the exact
 //    64-bit displacement), whose largest magnitude over the probed targets
 //    stands for max|E|;

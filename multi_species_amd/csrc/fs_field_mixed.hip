@@ -35,7 +35,7 @@
 //        FP64;
 //    and per target the pieces go into an FP64 TwoSum (hi, lo) pair.
 //  * With an error target (fs_emq_set_mixed_target) a probe first measures
-//    on up to 256 targets the error each of 8 candidate bands would give
+//    on up to 128 targets the error each of 8 candidate bands would give
 //    against the exact FP64 field, and the band is the narrowest within the
 //    target.
 // Positions are wrapped in FP64 (a = x/L - floor(x/L)) and turned into 64-bit
