@@ -526,15 +526,18 @@ __device__ __forceinline__ int vgpr_zero() {
 }
 
 // The near expansion of a range within emax of its centre, whose
-// displacements to every target of a block lie on one side, at least gap
-// away (both in units of 2^-64 of the period): every target is at r =
-// sqrt(Z^2 + d^2) >= sqrt(gap^2 + d^2) from the centre, and 12 Legendre terms
-// leave a truncation below (emax/r)^12, at most 16^-12 of sum |w| when
-// 16 emax <= sqrt(gap^2 + d^2).
-__device__ __forceinline__ bool near_compact(uint64_t emax, uint64_t gap, double d2) {
-    const double e = __builtin_ldexp((double)emax, -60);
+// displacements to every target lie on one side, at least gap away (both in
+// units of 2^-64 of the period): every target is at r = sqrt(Z^2 + d^2) >=
+// r0 = sqrt(gap^2 + d^2) from the centre, and K Legendre terms leave a
+// truncation below (emax/r)^K of sum |w|. The number of terms that keeps it
+// at most 2^-48 (16^-12): 12 when emax <= r0/16, 10 within r0/32, 8 within
+// r0/64, 6 within r0/256; 0 = not compact enough for the expansion.
+__device__ __forceinline__ int near_terms(uint64_t emax, uint64_t gap, double d2) {
+    const double e = __builtin_ldexp((double)emax, -64);
     const double g = __builtin_ldexp((double)gap, -64);
-    return e * e <= __builtin_fma(g, g, d2);
+    const double r2 = __builtin_fma(g, g, d2), e2 = e * e;
+    return e2 * 65536.0 <= r2 ? 6 : e2 * 4096.0 <= r2 ? 8 : e2 * 1024.0 <= r2 ? 10
+                                                          : e2 * 256.0 <= r2 ? 12 : 0;
 }
 
 // sum_j w_j g(z_j) of a compact range (scaled moments N about the centre uc)
@@ -543,9 +546,21 @@ __device__ __forceinline__ bool near_compact(uint64_t emax, uint64_t gap, double
 // r = sqrt(Z^2 + d^2), x = -Z/r and V_k = P_k(x) r^-(k+1) / gamma_k, the sum is
 // sum_k V_k (Z a_k + b_k), and V obeys V_k+1 = A V_k - c_k B V_k-1 with
 // A = -Z/r^2, B = 1/r^2, c_k = k^2/(4k^2 - 1): three operations per term
-// for the recurrence, two for the sum.
+// for the recurrence, two for the sum. K terms (6, 8, 10 or 12, uniform over
+// the wavefront; the moments are indexed with constants only, so they stay in
+// registers).
+template <int k>
+__device__ __forceinline__ void near_term(const TileMom& N, double Z, double A, double B, double& vm,
+                                          double& v, double& acc) {
+    constexpr double ck = (double)(k * k) / (double)(4 * k * k - 1);
+    const double vn = __builtin_fma(-ck, B * vm, A * v);
+    vm = v;
+    v = vn;
+    acc = __builtin_fma(v, __builtin_fma(Z, N.a[k + 1], N.b[k + 1]), acc);
+}
+
 __device__ __forceinline__ double near_value(const TileMom& N, uint64_t uc, uint64_t ut, double d2,
-                                             double s) {
+                                             double s, int K) {
     const double Z = __builtin_ldexp((double)(int64_t)(ut - uc), -64);
     const double ri = rsq_f64(__builtin_fma(Z, Z, d2));
     const double B = ri * ri;
@@ -553,13 +568,21 @@ __device__ __forceinline__ double near_value(const TileMom& N, uint64_t uc, uint
     double vm = ri, v = A * ri;
     double acc = vm * __builtin_fma(Z, N.a[0], N.b[0]);
     acc = __builtin_fma(v, __builtin_fma(Z, N.a[1], N.b[1]), acc);
-#pragma unroll
-    for (int k = 1; k + 1 < kNearTerms; ++k) {
-        const double ck = (double)(k * k) / (double)(4 * k * k - 1);
-        const double vn = __builtin_fma(-ck, B * vm, A * v);
-        vm = v;
-        v = vn;
-        acc = __builtin_fma(v, __builtin_fma(Z, N.a[k + 1], N.b[k + 1]), acc);
+    near_term<1>(N, Z, A, B, vm, v, acc);
+    near_term<2>(N, Z, A, B, vm, v, acc);
+    near_term<3>(N, Z, A, B, vm, v, acc);
+    near_term<4>(N, Z, A, B, vm, v, acc);
+    if (K > 6) {
+        near_term<5>(N, Z, A, B, vm, v, acc);
+        near_term<6>(N, Z, A, B, vm, v, acc);
+    }
+    if (K > 8) {
+        near_term<7>(N, Z, A, B, vm, v, acc);
+        near_term<8>(N, Z, A, B, vm, v, acc);
+    }
+    if (K > 10) {
+        near_term<9>(N, Z, A, B, vm, v, acc);
+        near_term<10>(N, Z, A, B, vm, v, acc);
     }
     return acc - s * N.a[0];
 }
@@ -707,6 +730,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             sl[t] = 0.0;
             gx[t] = 0.0;
         }
+        int nk = 0;  // terms of a near expansion
         for (int gb = W.sb; gb < W.se;) {
           // the largest far-field unit (2048 or 512 sources) starting here
           // that is far from all the wavefront's targets and compact: one FP32
@@ -764,7 +788,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         continue;
                     }
                 } else if (!cyc_meets(glo, glen, 1) &&
-                           near_compact(gemax, glo < kHalf ? glo : 0 - (glo + glen), d2)) {
+                           (nk = near_terms(gemax, glo < kHalf ? glo : 0 - (glo + glen), d2)) > 0) {
                     // a group on one side of all the wavefront's targets, within
                     // reach of the band and compact for its distance: its
                     // FP64 expansion
@@ -773,7 +797,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     const double s = (glo < kHalf) ? 1.0 : -1.0;
 #pragma unroll
                     for (int t = 0; t < TPT; ++t)
-                        two_sum_acc(sh[t], sl[t], near_value(N, uc, ui[t], d2, s));
+                        two_sum_acc(sh[t], sl[t], near_value(N, uc, ui[t], d2, s, nk));
                     continue;
                 }
             }
@@ -807,8 +831,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], acc[t]);
             } else if (near && !cyc_meets(lo, len, 1) &&
-                       near_compact(tiles[S.src_tile_base + (jb - S.src_off) / kTile].emax,
-                                    lo < kHalf ? lo : 0 - (lo + len), d2)) {
+                       (nk = near_terms(tiles[S.src_tile_base + (jb - S.src_off) / kTile].emax,
+                                        lo < kHalf ? lo : 0 - (lo + len), d2)) > 0) {
                 // near tile on one side of all the wavefront's targets, compact for
                 // its distance: its FP64 expansion about the centre (Legendre
                 // generating function; truncation below 16^-12 of sum |w|), and
@@ -818,7 +842,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const TileMom N = moms[tix + vgpr_zero()];
                 const double s = (lo < kHalf) ? 1.0 : -1.0;
 #pragma unroll
-                for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], near_value(N, uc, ui[t], d2, s));
+                for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], near_value(N, uc, ui[t], d2, s, nk));
             } else if (near) {
                 // no displacement of this tile reaches 1/2, so each pair's
                 // wrap is the tile's: targets above the sources by more than
