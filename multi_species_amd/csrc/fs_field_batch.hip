@@ -456,7 +456,8 @@ __global__ __launch_bounds__(kBlock) void unit_kernel(int level, int lstart, int
                                                       const uint64_t* __restrict__ us,
                                                       const double* __restrict__ ws,
                                                       GroupRec* __restrict__ groups,
-                                                      TileRec* __restrict__ gexp) {
+                                                      TileRec* __restrict__ gexp,
+                                                      TileMom* __restrict__ gmoms) {
     const int tid = blockIdx.x * kBlock + threadIdx.x;
     const int wu = tid >> 6, lane = tid & 63;
     if (wu >= nunits) return;  // whole wavefronts return together
@@ -493,7 +494,8 @@ __global__ __launch_bounds__(kBlock) void unit_kernel(int level, int lstart, int
     }
     if (lane != 0) return;
     TileRec T;
-    expand_coeffs(M, hc, emax, S, T, nullptr, nullptr);
+    TileMom N;
+    expand_coeffs(M, hc, emax, S, T, &N, nullptr);
     GroupRec G;
     G.vf = us[b] & ~kLowSort;
     G.vl = us[e - 1] | kLowSort;
@@ -501,6 +503,7 @@ __global__ __launch_bounds__(kBlock) void unit_kernel(int level, int lstart, int
     G.pad = 0;
     groups[ua] = G;
     gexp[ua] = T;
+    gmoms[ua] = N;
 }
 
 // The FP32 value of one expanded tile at a target (high word ht): the pair
@@ -732,10 +735,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
         int nk = 0;  // terms of a near expansion
         for (int gb = W.sb; gb < W.se;) {
-          // the largest far-field unit (2048 or 512 sources) starting here
-          // that is far from all the wavefront's targets and compact: one FP32
-          // expansion for all its sources
-          if (tspan < kHalf && !S.all_band) {
+          // the largest unit (2048 or 512 sources) starting here that is far
+          // from all the wavefront's targets and compact: one FP32 expansion
+          // for all its sources; else the largest on one side of them and
+          // compact for its distance: one FP64 near expansion
+          if (tspan < kHalf) {
             int step = 0;
 #pragma unroll
             for (int L = kLevels - 1; L >= 0; --L) {
@@ -745,12 +749,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
               const GroupRec U = groups[uix];
               const uint64_t uspan = U.vl - U.vf;
               const uint64_t ulen = tspan + uspan, ulo = uf - U.vl;
-              if (uspan < kHalf && ulen < kHalf && U.emax <= S.xband &&
-                  !cyc_meets(ulo - kHalf + kAntiMargin, ulen, 2 * kAntiMargin) &&
-                  !cyc_meets(ulo + S.band, ulen, 2 * S.band)) {
+              if (uspan >= kHalf || ulen >= kHalf ||
+                  cyc_meets(ulo - kHalf + kAntiMargin, ulen, 2 * kAntiMargin))
+                continue;
+              if (!S.all_band && U.emax <= S.xband && !cyc_meets(ulo + S.band, ulen, 2 * S.band)) {
                 const TileRec T = gexp[uix];
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) gx[t] += (double)tile_value(T, hi[t], hexp);
+                step = unit_src(L);
+              } else if (!cyc_meets(ulo, ulen, 1) &&
+                         (nk = near_terms(U.emax, ulo < kHalf ? ulo : 0 - (ulo + ulen), d2)) > 0) {
+                const uint64_t uc = (uint64_t)gexp[uix].hc << 32;
+                const TileMom N = gmoms[uix + vgpr_zero()];
+                const double s = (ulo < kHalf) ? 1.0 : -1.0;
+#pragma unroll
+                for (int t = 0; t < TPT; ++t)
+                    two_sum_acc(sh[t], sl[t], near_value(N, uc, ui[t], d2, s, nk));
                 step = unit_src(L);
               }
             }
@@ -1533,7 +1547,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         o_moms_ = take(sizeof(TileMom) * (size_t)std::max(ntiles_, 1));
         o_gexp_ = take(sizeof(TileRec) * (size_t)std::max(nrecs_, 1));
         o_gexp64_ = take(sizeof(TileRec64) * (size_t)std::max(ngroups_, 1));
-        o_gmoms_ = take(sizeof(TileMom) * (size_t)std::max(ngroups_, 1));
+        o_gmoms_ = take(sizeof(TileMom) * (size_t)std::max(nrecs_, 1));
     }
     FS_TRY(work_.ensure(o, stream));
     if (precision == kPrecisionMixed) {
@@ -1771,7 +1785,8 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
         if (nunits_[L] == 0) continue;
         unit_kernel<<<nblocks(64 * (int64_t)nunits_[L]), kBlock, 0, stream>>>(
             L, lstart_[L], nunits_[L], ngroups_, d_unit_seg_, d_segs_, skey, ws,
-            reinterpret_cast<GroupRec*>(wk + o_groups_), reinterpret_cast<TileRec*>(wk + o_gexp_));
+            reinterpret_cast<GroupRec*>(wk + o_groups_), reinterpret_cast<TileRec*>(wk + o_gexp_),
+            reinterpret_cast<TileMom*>(wk + o_gmoms_));
         FS_HIP_CHECK(hipGetLastError());
     }
     if (!scan_.empty()) {
