@@ -801,11 +801,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         }
                         continue;
                     }
-                } else if (!cyc_meets(glo, glen, 1) &&
-                           (nk = near_terms(gemax, glo < kHalf ? glo : 0 - (glo + glen), d2)) > 0) {
-                    // a group on one side of all the wavefront's targets, within
-                    // reach of the band and compact for its distance: its
-                    // FP64 expansion
+                }
+                if (!cyc_meets(glo, glen, 1) &&
+                    (nk = near_terms(gemax, glo < kHalf ? glo : 0 - (glo + glen), d2)) > 0) {
+                    // a group on one side of all the wavefront's targets and
+                    // compact for its distance (within reach of the band, or far
+                    // but too spread for the FP32 expansion): its FP64 expansion
                     const uint64_t uc = (uint64_t)gexp[gix].hc << 32;
                     const TileMom N = gmoms[gix + vgpr_zero()];
                     const double s = (glo < kHalf) ? 1.0 : -1.0;
@@ -889,6 +890,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const TileRec T = tiles[S.src_tile_base + (jb - S.src_off) / kTile];
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) gx[t] += (double)tile_value(T, hi[t], hexp);
+            } else if ((nk = near_terms(tiles[S.src_tile_base + (jb - S.src_off) / kTile].emax,
+                                        lo < kHalf ? lo : 0 - (lo + len), d2)) > 0) {
+                // far tile too spread for the FP32 expansion but compact for
+                // its distance: its FP64 expansion (a far tile is on one side)
+                const int tix = S.src_tile_base + (jb - S.src_off) / kTile;
+                const uint64_t uc = (uint64_t)tiles[tix].hc << 32;
+                const TileMom N = moms[tix + vgpr_zero()];
+                const double s = (lo < kHalf) ? 1.0 : -1.0;
+#pragma unroll
+                for (int t = 0; t < TPT; ++t) two_sum_acc(sh[t], sl[t], near_value(N, uc, ui[t], d2, s, nk));
             } else {
                 float acc[TPT];
 #pragma unroll
