@@ -250,10 +250,11 @@ def main():
         "tuned_shape": dict(zip(("targets_per_thread", "wgs_per_cu"), ctx.mixed_shape())),
         "precision": "FP64 wrap + 64-bit fixed-point displacement; FP64 step/neutralizing "
                      "part from sorted prefix sums; FP64 band |z| < max(2^-6, 16 d, 4000 d^2), "
-                     "d = eps/L (12-term Legendre expansions of compact one-sided groups and "
-                     "tiles, exact pairs elsewhere); beyond it FP32: 8-moment expansions of the "
-                     "largest compact unit (2048, 512, 128 or 32 sorted sources), the series "
-                     "pair by pair elsewhere; compensated FP64 accumulation",
+                     "d = eps/L (Legendre expansions of 6 to 12 terms, by distance, of compact "
+                     "one-sided ranges of sorted sources, exact pairs elsewhere); beyond it "
+                     "FP32 8-moment expansions of the largest compact range (2048, 512, 128 or "
+                     "32 sorted sources), else the FP64 expansion where a range is compact "
+                     "enough for it, else FP32 pairs; compensated FP64 accumulation",
     }
     # the same with the error target 4e-9 (the band from each call's own probe)
     ctx.set_mixed_target(4e-9)

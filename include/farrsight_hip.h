@@ -95,11 +95,12 @@ int fs_emq_f64_pair_evals(fs_ctx* ctx, int64_t* evals);
  * neutralizing part of the kernel is summed in FP64 from sorted prefix sums,
  * the regularization remainder in FP64 within B = max(2^-6, 16 d, 4000 d^2)
  * of z = 0 (d = eps/L; every pair in FP64 when B >= 1/4; compact one-sided
- * ranges of sorted sources from a 12-term Legendre expansion, the rest pair
- * by pair) and in FP32 beyond, where the largest range of 2048, 512, 128 or
- * 32 sorted sources that is compact against its distance is summed from an
- * 8-moment expansion about its centre and the others pair by pair;
- * compensated FP64 accumulation per target.
+ * ranges of sorted sources from a Legendre expansion of 6 to 12 terms by
+ * their distance, the rest pair by pair) and in FP32 beyond, where the
+ * largest range of 2048, 512, 128 or 32 sorted sources that is compact
+ * against its distance is summed from an 8-moment expansion about its
+ * centre, a range compact enough for the FP64 expansion from that, and the
+ * others pair by pair; compensated FP64 accumulation per target.
  * Relative error max_i|E_i - E_i^FP64| / max_i|E_i^FP64|: 1e-10 to 1e-8 at
  * that band on the tested decks; with fs_emq_set_mixed_target each call
  * measures the error of candidate bands first and takes the narrowest one
