@@ -20,7 +20,8 @@ ARCH = os.environ.get("FS_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
-    # packed FP32 instructions stay off (see the note in DESIGN.md's kernel list)
+    # packed FP32 instructions (v_pk_*_f32) stay off: some of their forms have
+    # given wrong results on MI355X without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
     "-I", os.path.join(HERE, "..", "include"),
