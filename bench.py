@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--eps", type=float, default=0.05)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["field", "treecode", "c1", "c3", "c4", "c5"],
+    ap.add_argument("--workload", choices=["field", "open", "treecode", "c1", "c3", "c4", "c5"],
                     default="field",
                     help="field: N=2^20 direct-sum evaluation (default); c1: the two-species "
                          "weak-Landau deck, 65x129 points per species, RK4 dt=0.1 with a remesh "
@@ -105,7 +105,9 @@ def main():
                          "with ions and the treecode; c4: 64 independent "
                          "two-species Landau decks (16x32 to 128x256 panels) stepped as one "
                          "batch in mixed precision, against the same decks one by one in FP64; "
-                         "c5: Poisson arrivals of decks served with continuous batching")
+                         "c5: Poisson arrivals of decks served with continuous batching; "
+                         "open: the open-boundary direct sum (E_MQ_DirectSum_openbcs) of the "
+                         "N=2^20 deck")
     ap.add_argument("--precision", choices=["f64", "mixed"], default="f64",
                     help="c1: field precision of the step loop")
     ap.add_argument("--decks", type=int, default=64, help="c4: decks in the batch")
@@ -137,6 +139,8 @@ def main():
         print(json.dumps({"copy_bytes": src.numel() * 8}))
         return
 
+    if args.workload == "open":
+        return bench_open(args)
     if args.workload == "c1":
         return bench_c1(args)
     if args.workload == "treecode":
@@ -276,6 +280,88 @@ def main():
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(x, w, args.eps)
     ctx.close()
+    print(json.dumps(result))
+
+
+FLOP_PER_OPEN_PAIR = 7  # sub, FMA (2), rsqrt (1), product, FMA (2)
+
+
+def bench_open(args):
+    """The open-boundary direct sum (E_MQ_DirectSum_openbcs) of the N=2^20 deck,
+    targets = sources, every ordered pair evaluated."""
+    import torch
+
+    import multi_species_amd as ms
+
+    n = args.n
+    x, w = c2_inputs(n)
+    dev = torch.device("cuda:0")
+    tx = torch.from_numpy(x).to(dev)
+    tw = torch.from_numpy(w).to(dev)
+    te = torch.empty(n, dtype=torch.float64, device=dev)
+    ctx = ms.Context(0)
+    stream = torch.cuda.Stream(device=dev)
+    sptr = stream.cuda_stream
+
+    def step():
+        ctx.emq_open_f64_dev(te.data_ptr(), tx.data_ptr(), n, tx.data_ptr(), tw.data_ptr(), n,
+                             args.eps, stream=sptr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    pairs = float(n) * float(n)
+    achieved = FLOP_PER_OPEN_PAIR * pairs / (kernel_ms * 1e-3) / 1e12
+    result = {
+        "metric": "E-field pair-interactions/s",
+        "value": pairs * args.steps / wall,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "open-boundary regularized E-field direct sum "
+                               "(E_MQ_DirectSum_openbcs), N=2^20 targets=sources of the "
+                               "periodic deck's positions and weights, one evaluation per step",
+                   "n": n, "eps": args.eps},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel_ms": kernel_ms, "kernel_evals_per_launch": pairs,
+                     "flop_per_eval": FLOP_PER_OPEN_PAIR},
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+
+        lib = oracle.ref_lib(required=False)
+        fn = oracle.ref_direct_sum_open if lib is not None else oracle.direct_sum_open
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        fn(x[:256], x, w, args.eps)
+        rate = 256 * n / (time.perf_counter() - t0)
+        m = max(256, int(min(n, rate * 10.0 / n)) // 256 * 256)
+        t0 = time.perf_counter()
+        fn(x[:m], x, w, args.eps)
+        dt = time.perf_counter() - t0
+        result["cpu_baseline"] = {
+            "value": m * n / dt, "unit": "pair-interactions/s", "cores": cores,
+            "kind": "reference" if lib is not None else "port",
+            "sample": f"{m} targets x {n} sources of the same deck, {dt:.1f} s on {cores} "
+                      "host threads"}
     print(json.dumps(result))
 
 

@@ -68,6 +68,34 @@ int fs_emq_treecode_f64(fs_ctx* ctx, double* es, const double* sources, const do
                         int64_t ns, double L, double epsilon, double mac, int degree,
                         int max_source, int64_t* evals);
 
+/* Replaces E_MQ_DirectSum_openbcs::operator() (src/FieldStructure.cpp:166-191):
+ *   es[i] = sum_j q_ws[j] * 0.5 * z / sqrt(z^2 + eps^2) - targets[i],
+ *   z = targets[i] - sources[j]
+ * (open boundaries, no wrap; -targets[i] is the neutralizing background of
+ * unit density). FP64, compensated accumulation; requires epsilon > 0. Host
+ * pointers, synchronous. */
+int fs_emq_open_f64(fs_ctx* ctx, double* es, const double* targets, int64_t nt,
+                    const double* sources, const double* q_ws, int64_t ns, double epsilon);
+
+/* Same operator on device-resident arrays, enqueued on `stream`. */
+int fs_emq_open_f64_dev(fs_ctx* ctx, double* d_es, const double* d_targets, int64_t nt,
+                        const double* d_sources, const double* d_q_ws, int64_t ns,
+                        double epsilon, void* stream);
+
+/* Replaces ExternalTanh / ExternalSine / ExternalLogistic::operator()
+ * (src/FieldStructure.cpp:17-139, constants inc/FieldStructure.hpp:49-84):
+ *   es[i] += a(t) * sin(k * targets[i] - omega * t),
+ * type 1 the sine ramp, 2 the logistic ramp, any other value the tanh drive
+ * (as the deck loader's default branch, AMRSimulation_load.cpp:110-124). es is
+ * read and written. Host pointers, synchronous. */
+int fs_external_field_add(fs_ctx* ctx, double* es, const double* targets, int64_t nt, double t,
+                          int type, double Am, double k, double omega);
+
+/* Same on device-resident arrays, enqueued on `stream`. */
+int fs_external_field_add_dev(fs_ctx* ctx, double* d_es, const double* d_targets, int64_t nt,
+                              double t, int type, double Am, double k, double omega,
+                              void* stream);
+
 /* Tuning knobs of the direct-sum kernels (0 = automatic). targets_per_thread
  * in {1,2,4} (the mixed-precision kernels always take 4: each wavefront
  * makes its near/far choices for its 256 consecutive sorted targets); source_splits >= 1. Applies to

@@ -46,6 +46,11 @@ def lib():
         L.fs_ctx_synchronize.argtypes = [_vp]
         L.fs_emq_periodic_f64.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
         L.fs_emq_periodic_f64_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d, _vp]
+        L.fs_emq_open_f64.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d]
+        L.fs_emq_open_f64_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _vp]
+        L.fs_external_field_add.argtypes = [_vp, _dp, _dp, _i64, _d, ctypes.c_int, _d, _d, _d]
+        L.fs_external_field_add_dev.argtypes = [_vp, _vp, _vp, _i64, _d, ctypes.c_int, _d, _d,
+                                                _d, _vp]
         L.fs_emq_set_tuning.argtypes = [_vp, _int, _int]
         L.fs_emq_set_symmetric.argtypes = [_vp, _int]
         L.fs_emq_f64_pair_evals.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64)]
@@ -164,6 +169,27 @@ class Context:
                                          s.size, L, eps), "fs_emq_periodic_f64")
         return es
 
+    def emq_open_f64(self, targets, sources, q_ws, eps: float) -> np.ndarray:
+        """E_MQ_DirectSum_openbcs::operator() on host arrays (synchronous)."""
+        t, s, w = _f64(targets), _f64(sources), _f64(q_ws)
+        if s.size != w.size:
+            raise ValueError("sources and q_ws differ in length")
+        es = np.empty(t.size)
+        _check(lib().fs_emq_open_f64(self.h, _P(es), _P(t), t.size, _P(s), _P(w), s.size, eps),
+               "fs_emq_open_f64")
+        return es
+
+    def external_field_add(self, es, targets, t: float, type_: int, Am: float, k: float,
+                           omega: float) -> np.ndarray:
+        """ExternalTanh (type 0) / ExternalSine (1) / ExternalLogistic (2)
+        ::operator() on host arrays: returns es + a(t) sin(k x - omega t)."""
+        e, x = _f64(es).copy(), _f64(targets)
+        if e.size != x.size:
+            raise ValueError("es and targets differ in length")
+        _check(lib().fs_external_field_add(self.h, _P(e), _P(x), x.size, t, type_, Am, k, omega),
+               "fs_external_field_add")
+        return e
+
     def emq_periodic_mixed(self, targets, sources, q_ws, L: float, eps: float,
                            same: bool = False) -> np.ndarray:
         """Mixed-precision direct sum on host arrays (synchronous). same=True
@@ -216,6 +242,13 @@ class Context:
         _check(lib().fs_emq_periodic_f64_dev(self.h, es_ptr, t_ptr, nt, s_ptr, w_ptr, ns, L,
                                              eps, stream or None),
                "fs_emq_periodic_f64_dev")
+
+    def emq_open_f64_dev(self, es_ptr: int, t_ptr: int, nt: int, s_ptr: int, w_ptr: int,
+                         ns: int, eps: float, stream: Optional[int] = None):
+        """fs_emq_open_f64 on device pointers, enqueued on `stream`."""
+        _check(lib().fs_emq_open_f64_dev(self.h, es_ptr, t_ptr, nt, s_ptr, w_ptr, ns, eps,
+                                         stream or None),
+               "fs_emq_open_f64_dev")
 
 
 PANEL_INTS = 20

@@ -65,6 +65,11 @@ def lib():
         L.orc_direct_sum.argtypes = [_dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
         L.orc_direct_sum_ext.argtypes = [_dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
         L.orc_treecode.argtypes = [_dp, _dp, _dp, _i64, _d, _d, _d, ctypes.c_int, ctypes.c_int]
+        L.orc_direct_sum_open.argtypes = [_dp, _dp, _i64, _dp, _dp, _i64, _d]
+        L.orc_direct_sum_open_ext.argtypes = [_dp, _dp, _i64, _dp, _dp, _i64, _d]
+        L.orc_external_field_add.argtypes = [_dp, _dp, _i64, _d, ctypes.c_int, _d, _d, _d]
+        L.orc_external_amplitude.argtypes = [ctypes.c_int, _d, _d, _d]
+        L.orc_external_amplitude.restype = _d
         L.orc_treecode_last_stats.argtypes = [ctypes.POINTER(ctypes.c_int64)]
         L.orc_species_create.restype = ctypes.c_void_p
         L.orc_species_create.argtypes = [ctypes.POINTER(SimParams), ctypes.POINTER(SpeciesParams)]
@@ -109,6 +114,9 @@ def ref_lib(required: bool = True):
             return None
         R = ctypes.CDLL(path)
         R.fsref_direct_sum.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _d, _d]
+        R.fsref_direct_sum_open.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _d]
+        R.fsref_external_field_add.argtypes = [_dp, _dp, ctypes.c_int, _d, ctypes.c_int, _d, _d,
+                                               _d]
         R.fsref_treecode.argtypes = [_dp, _dp, _dp, ctypes.c_int, _d, _d, _d, ctypes.c_int,
                                      ctypes.c_int]
         _ref = R
@@ -158,6 +166,48 @@ def ref_direct_sum(targets, sources, q_ws, L: float, eps: float) -> np.ndarray:
     rc = ref_lib().fsref_direct_sum(_P(es), _P(t), t.size, _P(s), _P(w), s.size, L, eps)
     assert rc == 0
     return es
+
+
+def direct_sum_open(targets, sources, q_ws, eps: float) -> np.ndarray:
+    """E_MQ_DirectSum_openbcs::operator() (src/FieldStructure.cpp:166-191)."""
+    t, s, w = _f64(targets), _f64(sources), _f64(q_ws)
+    es = np.empty(t.size)
+    assert lib().orc_direct_sum_open(_P(es), _P(t), t.size, _P(s), _P(w), s.size, eps) == 0
+    return es
+
+
+def direct_sum_open_ext(targets, sources, q_ws, eps: float) -> np.ndarray:
+    t, s, w = _f64(targets), _f64(sources), _f64(q_ws)
+    es = np.empty(t.size)
+    assert lib().orc_direct_sum_open_ext(_P(es), _P(t), t.size, _P(s), _P(w), s.size, eps) == 0
+    return es
+
+
+def external_field_add(es, targets, t: float, type_: int, Am: float, k: float,
+                       omega: float) -> np.ndarray:
+    """ExternalTanh/Sine/Logistic::operator() (src/FieldStructure.cpp:17-139) on a copy of es."""
+    e, x = _f64(es).copy(), _f64(targets)
+    assert lib().orc_external_field_add(_P(e), _P(x), x.size, t, type_, Am, k, omega) == 0
+    return e
+
+
+def external_amplitude(type_: int, t: float, Am: float, k: float) -> float:
+    return float(lib().orc_external_amplitude(type_, t, Am, k))
+
+
+def ref_direct_sum_open(targets, sources, q_ws, eps: float) -> np.ndarray:
+    t, s, w = _f64(targets), _f64(sources), _f64(q_ws)
+    es = np.empty(t.size)
+    assert ref_lib().fsref_direct_sum_open(_P(es), _P(t), t.size, _P(s), _P(w), s.size, eps) == 0
+    return es
+
+
+def ref_external_field_add(es, targets, t: float, type_: int, Am: float, k: float,
+                           omega: float) -> np.ndarray:
+    e, x = _f64(es).copy(), _f64(targets)
+    assert ref_lib().fsref_external_field_add(_P(e), _P(x), x.size, t, type_, Am, k,
+                                              omega) == 0
+    return e
 
 
 def ref_treecode(sources, q_ws, L: float, eps: float, mac: float, degree: int,

@@ -32,6 +32,20 @@ int orc_direct_sum_ext(double* es, const double* targets, int64_t nt,
                        const double* sources, const double* q_ws, int64_t ns,
                        double L, double epsilon);
 
+/* E_MQ_DirectSum_openbcs::operator() (src/FieldStructure.cpp:166-191), same
+ * expression order, OpenMP over targets; and its extended-precision form. */
+int orc_direct_sum_open(double* es, const double* targets, int64_t nt, const double* sources,
+                        const double* q_ws, int64_t ns, double epsilon);
+int orc_direct_sum_open_ext(double* es, const double* targets, int64_t nt, const double* sources,
+                            const double* q_ws, int64_t ns, double epsilon);
+
+/* ExternalTanh / ExternalSine / ExternalLogistic::operator()
+ * (src/FieldStructure.cpp:17-139): es[i] += a(t) sin(k x_i - omega t); type 1
+ * sine, 2 logistic, otherwise tanh. orc_external_amplitude gives a(t). */
+int orc_external_field_add(double* es, const double* targets, int64_t nt, double t, int type,
+                           double Am, double k, double omega);
+double orc_external_amplitude(int type, double t, double Am, double k);
+
 /* E_MQ_Treecode::operator() (src/FieldStructure.cpp:230-1020): evaluates at
  * the sources, writes es[0:ns]. */
 int orc_treecode(double* es, const double* sources, const double* q_ws,

@@ -37,6 +37,61 @@ void direct_sum(double* es, const double* targets, int64_t nt,
     }
 }
 
+// src/FieldStructure.cpp:166-191 (E_MQ_DirectSum_openbcs::operator()): the
+// same expression, q_ws[j] * 0.5 * z / sqrt(z*z + eps^2) summed in j order,
+// then the -x_i background.
+void direct_sum_open(double* es, const double* targets, int64_t nt, const double* sources,
+                     const double* q_ws, int64_t ns, double epsilon) {
+    const double eps_sq = epsilon * epsilon;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nt; ++i) {
+        const double xi = targets[i];
+        double ei = 0.0;
+        for (int64_t j = 0; j < ns; ++j) {
+            const double z = xi - sources[j];
+            ei += q_ws[j] * 0.5 * z / std::sqrt(z * z + eps_sq);
+        }
+        es[i] = ei - xi;
+    }
+}
+
+// The drive amplitude of src/FieldStructure.cpp:17-139 (ExternalTanh's
+// a_t_k, ExternalSine's and ExternalLogistic's a0) with the constants of
+// inc/FieldStructure.hpp:49-84; type as the deck loader's switch
+// (AMRSimulation_load.cpp:110-124: 1 sine, 2 logistic, otherwise tanh).
+double external_amplitude(int type, double t, double Am, double k) {
+    if (type == 1) {
+        const double t1 = 50, t2 = 150, t3 = 200;
+        double a0 = 0;
+        if (t < t1)
+            a0 = Am * std::sin(t * M_PI / 100);
+        else if (t < t2)
+            a0 = Am;
+        else if (t < t3)
+            a0 = Am * std::cos((t - 150) * M_PI / 100);
+        return a0;
+    }
+    if (type == 2) {
+        const double t1 = 60;
+        if (t < t1) return Am / (1.0 + std::exp(-40.0 * (t - 10.0)));
+        return Am * (1.0 - 1.0 / (1.0 + std::exp(-40.0 * (t - 110.0))));
+    }
+    const double t0 = 0, tL = 69, tR = 307, twL = 20, twR = 20;
+    auto drive_fun = [&](double s) {
+        return 0.5 * (std::tanh((s - tL) / twL) - std::tanh((s - tR) / twR));
+    };
+    const double eps = drive_fun(t0);
+    return (drive_fun(t) - eps) / (1 - eps) * Am * k;
+}
+
+// es[i] += a(t) * sin(k * x_i - omega * t) (the three operator()s' loop).
+void external_add(double* es, const double* targets, int64_t nt, double t, int type, double Am,
+                  double k, double omega) {
+    const double a = external_amplitude(type, t, Am, k);
+    const double omt = omega * t;
+    for (int64_t i = 0; i < nt; ++i) es[i] += a * std::sin(k * targets[i] - omt);
+}
+
 namespace {
 
 // Treecode node (src/FieldStructure.hpp:26-40).
@@ -297,6 +352,49 @@ int orc_direct_sum_ext(double* es, const double* targets, int64_t nt,
         es[i] = (double)(s + c);
     }
     return 0;
+}
+
+int orc_direct_sum_open(double* es, const double* targets, int64_t nt, const double* sources,
+                        const double* q_ws, int64_t ns, double epsilon) {
+    if (nt < 0 || ns < 0 || !(epsilon > 0)) return -1;
+    orc::direct_sum_open(es, targets, nt, sources, q_ws, ns, epsilon);
+    return 0;
+}
+
+// Extended-precision evaluation of the open-boundary sum: long double kernel
+// values, Neumaier-compensated long double accumulator.
+int orc_direct_sum_open_ext(double* es, const double* targets, int64_t nt, const double* sources,
+                            const double* q_ws, int64_t ns, double epsilon) {
+    if (nt < 0 || ns < 0 || !(epsilon > 0)) return -1;
+    const long double e2 = (long double)epsilon * (long double)epsilon;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nt; ++i) {
+        long double s = 0.0L, c = 0.0L;
+        const long double xi = targets[i];
+        for (int64_t j = 0; j < ns; ++j) {
+            const long double z = xi - (long double)sources[j];
+            const long double term = (long double)q_ws[j] * 0.5L * z / sqrtl(z * z + e2);
+            const long double tsum = s + term;
+            if (fabsl(s) >= fabsl(term))
+                c += (s - tsum) + term;
+            else
+                c += (term - tsum) + s;
+            s = tsum;
+        }
+        es[i] = (double)((s + c) - xi);
+    }
+    return 0;
+}
+
+int orc_external_field_add(double* es, const double* targets, int64_t nt, double t, int type,
+                           double Am, double k, double omega) {
+    if (nt < 0) return -1;
+    orc::external_add(es, targets, nt, t, type, Am, k, omega);
+    return 0;
+}
+
+double orc_external_amplitude(int type, double t, double Am, double k) {
+    return orc::external_amplitude(type, t, Am, k);
 }
 
 int orc_treecode(double* es, const double* sources, const double* q_ws,

@@ -78,4 +78,36 @@ int fsref_treecode(double* es, const double* sources, const double* q_ws,
     return 0;
 }
 
+// E_MQ_DirectSum_openbcs::operator() (src/FieldStructure.cpp:166-191), run as-is.
+int fsref_direct_sum_open(double* es, const double* targets, int nt, const double* sources,
+                          const double* q_ws, int ns, double epsilon) {
+    if (nt < 0 || ns < 0) return -1;
+    std::vector<double> t(targets, targets + nt), s(sources, sources + ns),
+        w(q_ws, q_ws + ns);
+    E_MQ_DirectSum_openbcs field(epsilon);
+    QuietCout quiet;
+    field(es, t.data(), nt, s.data(), w.data(), ns);
+    return 0;
+}
+
+// ExternalTanh / ExternalSine / ExternalLogistic::operator()
+// (src/FieldStructure.cpp:17-139), chosen as AMRSimulation_load.cpp:110-124
+// does: type 1 sine, 2 logistic, otherwise tanh.
+int fsref_external_field_add(double* es, const double* targets, int nt, double t, int type,
+                             double Am, double k, double omega) {
+    if (nt < 0) return -1;
+    std::vector<double> x(targets, targets + nt);
+    if (type == 1) {
+        ExternalSine f(Am, k, omega);
+        f(es, x.data(), nt, t);
+    } else if (type == 2) {
+        ExternalLogistic f(Am, k, omega);
+        f(es, x.data(), nt, t);
+    } else {
+        ExternalTanh f(Am, k, omega);
+        f(es, x.data(), nt, t);
+    }
+    return 0;
+}
+
 }  // extern "C"
