@@ -72,7 +72,10 @@ int fs_emq_treecode_f64(fs_ctx* ctx, double* es, const double* sources, const do
  *   es[i] = sum_j q_ws[j] * 0.5 * z / sqrt(z^2 + eps^2) - targets[i],
  *   z = targets[i] - sources[j]
  * (open boundaries, no wrap; -targets[i] is the neutralizing background of
- * unit density). FP64, compensated accumulation; requires epsilon > 0. Host
+ * unit density). FP64, compensated accumulation; requires epsilon > 0. With
+ * the targets being the sources (the same pointer, or for this host entry
+ * point two arrays of identical contents) and 16384 points or more, each pair
+ * is evaluated once and given to both points (the kernel is odd). Host
  * pointers, synchronous. */
 int fs_emq_open_f64(fs_ctx* ctx, double* es, const double* targets, int64_t nt,
                     const double* sources, const double* q_ws, int64_t ns, double epsilon);
@@ -112,8 +115,8 @@ int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits);
 int fs_emq_set_symmetric(fs_ctx* ctx, int mode);
 
 /* Kernel evaluations (pairs, padding of the last block or tile included) of
- * the last FP64 direct sum launched on this context: nt * ns for the full
- * sum, about n^2 / 2 for the pair-once form. No counterpart in the reference
+ * the last FP64 direct sum launched on this context (periodic or open):
+ * nt * ns for the full sum, about n^2 / 2 for the pair-once form. No counterpart in the reference
  * (for throughput accounting). */
 int fs_emq_f64_pair_evals(fs_ctx* ctx, int64_t* evals);
 

@@ -86,6 +86,7 @@ struct fs_ctx {
     fs::FieldBatch* api_batch = nullptr;  // plan of the last mixed entry-point call
     double mixed_target = 0;              // > 0: mixed entry points hold this error bound
     int64_t f64_pair_evals = 0;           // kernel evaluations of the last FP64 direct sum
+                                          // (periodic or open)
     // autotuned launch shapes (tpt, workgroups per CU) of the mixed entry
     // points, by problem class
     std::map<uint64_t, std::pair<int, int>> shape_cache;

@@ -7,7 +7,10 @@
 //   are the same for the whole wavefront (scalar loads), each run of 256
 //   sources is summed in FP64 and the runs are added with TwoSum; small target
 //   sets split the sources over workgroups and add the splits in a fixed
-//   order, so the result does not depend on scheduling. Each pair costs a
+//   order, so the result does not depend on scheduling. When the targets are
+//   the sources (one device array of 16384 points or more) the kernel's
+//   oddness gives each pair once to both points (fs_sym.h, as the periodic
+//   sum does): half the evaluations. Each pair costs a
 //   subtraction, an FMA, an FP64 reciprocal square root (v_rsq_f64 plus a
 //   third-order correction), a product and an FMA, all FP64 vector
 //   instructions: the kernel is bound by FP64 VALU issue like the periodic one.
@@ -18,9 +21,11 @@
 //   the kernel adds the drive to every point.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "fs_device.h"
 #include "fs_internal.h"
+#include "fs_sym.h"
 
 namespace fs {
 namespace {
@@ -69,6 +74,21 @@ __global__ __launch_bounds__(kBlock) void open_f64_kernel(const double* __restri
         const int64_t i = i0 + (int64_t)t * kBlock;
         if (i < nt) part[(int64_t)blockIdx.y * nt + i] = hi[t] + lo[t];
     }
+}
+
+template <int T>
+__global__ __launch_bounds__(sym::threads<T>()) void open_sym_kernel(
+    const double* __restrict__ x, const double* __restrict__ w, sym::Plan P, double eps2,
+    double* __restrict__ rowpart, double* __restrict__ colpart) {
+    __shared__ double red[2][sym::kMaxWaves][64];
+    sym::item<T, true>(x, w, P, blockIdx.x, 0.0, eps2, rowpart, colpart, red);
+}
+
+__global__ __launch_bounds__(kBlock) void open_sym_finish_kernel(
+    sym::Plan P, const double* __restrict__ rowpart, const double* __restrict__ colpart,
+    const double* __restrict__ x, double* __restrict__ es) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < P.n) es[p] = sym::point(P, p, rowpart, colpart) - x[p];
 }
 
 // es[i] = 0.5 * (sum of the splits, in split order) - x_i.
@@ -151,6 +171,36 @@ int fs_emq_open_f64_dev(fs_ctx* ctx, double* d_es, const double* d_targets, int6
     if (!d_es || !d_targets || (ns > 0 && (!d_sources || !d_q_ws)))
         return fs::fail(FS_E_INVALID, "null pointer");
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    if (d_targets == d_sources && nt == ns && nt >= fs::sym::kMinPoints && nt <= INT32_MAX) {
+        const fs::sym::Plan P = fs::sym::plan(nt, ctx->num_cus, fs::FieldTuning{});
+        const size_t rb = fs::align_up(sizeof(double) * (size_t)fs::sym::rowpart_elems(P), 256);
+        const size_t cb = fs::align_up(sizeof(double) * (size_t)fs::sym::colpart_elems(P), 256);
+        FS_TRY(ctx->scratch.ensure(rb + cb, st));
+        double* rowpart = ctx->scratch.as<double>(0);
+        double* colpart = ctx->scratch.as<double>(rb);
+        const double eps2 = epsilon * epsilon;
+        switch (P.t) {
+            case 4:
+                fs::open_sym_kernel<4><<<P.nitems, fs::sym::threads<4>(), 0, st>>>(
+                    d_sources, d_q_ws, P, eps2, rowpart, colpart);
+                break;
+            case 2:
+                fs::open_sym_kernel<2><<<P.nitems, fs::sym::threads<2>(), 0, st>>>(
+                    d_sources, d_q_ws, P, eps2, rowpart, colpart);
+                break;
+            default:
+                fs::open_sym_kernel<1><<<P.nitems, fs::sym::threads<1>(), 0, st>>>(
+                    d_sources, d_q_ws, P, eps2, rowpart, colpart);
+                break;
+        }
+        FS_HIP_CHECK(hipGetLastError());
+        fs::open_sym_finish_kernel<<<(unsigned)((nt + fs::kBlock - 1) / fs::kBlock), fs::kBlock, 0,
+                                     st>>>(P, rowpart, colpart, d_sources, d_es);
+        FS_HIP_CHECK(hipGetLastError());
+        ctx->f64_pair_evals = fs::sym::evals(P);
+        return FS_OK;
+    }
+    ctx->f64_pair_evals = nt * ns;
     const int splits = fs::open_splits(nt, ns, ctx->num_cus);
     const int64_t chunk = ns > 0 ? (ns + splits - 1) / splits : 0;
     FS_TRY(ctx->scratch.ensure(sizeof(double) * (size_t)splits * (size_t)nt, st));
@@ -182,8 +232,15 @@ int fs_emq_open_f64(fs_ctx* ctx, double* es, const double* targets, int64_t nt,
     double* d_e = ctx->io.as<double>(bt);
     double* d_s = ctx->io.as<double>(2 * bt);
     double* d_w = ctx->io.as<double>(2 * bt + bs);
-    FS_HIP_CHECK(hipMemcpyAsync(d_t, targets, sizeof(double) * nt, hipMemcpyHostToDevice,
-                                ctx->stream));
+    // targets and sources the same positions (one array, or a copy of it):
+    // one device copy, so the sum takes each pair once
+    const bool same = nt == ns && (targets == sources ||
+                                   std::memcmp(targets, sources, sizeof(double) * nt) == 0);
+    if (same)
+        d_t = d_s;
+    else
+        FS_HIP_CHECK(hipMemcpyAsync(d_t, targets, sizeof(double) * nt, hipMemcpyHostToDevice,
+                                    ctx->stream));
     if (ns > 0) {
         FS_HIP_CHECK(hipMemcpyAsync(d_s, sources, sizeof(double) * ns, hipMemcpyHostToDevice,
                                     ctx->stream));

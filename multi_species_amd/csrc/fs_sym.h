@@ -110,7 +110,10 @@ __device__ __forceinline__ double wave_rotate(double v) {
 // One item (row block and source chunk, or a block with itself) of the plan
 // P over the points x[0..n), w[0..n). red: the workgroup's LDS,
 // [2][kMaxWaves][64] doubles. Called by all threads<T>() threads of a workgroup.
-template <int T>
+// kOpen: the open-boundary kernel z / sqrt(z^2 + delta2) with z = x_i - x_j
+// (E_MQ_DirectSum_openbcs, src/FieldStructure.cpp:166-191; also odd in z;
+// invL unused) instead of the periodic one.
+template <int T, bool kOpen = false>
 __device__ __forceinline__ void item(const double* __restrict__ x, const double* __restrict__ w,
                                      const Plan& P, int id, double invL, double delta2,
                                      double* __restrict__ rowpart, double* __restrict__ colpart,
@@ -142,7 +145,7 @@ __device__ __forceinline__ void item(const double* __restrict__ x, const double*
         hi[t] = 0.0;
         lo[t] = 0.0;
     }
-    const double C = dev::rsq_f64(__builtin_fma(0.5, 0.5, delta2));
+    const double C = kOpen ? 0.0 : dev::rsq_f64(__builtin_fma(0.5, 0.5, delta2));
     int tile = 0;
     for (int d = d0; d < d1; ++d) {
         const int B = b + d < P.nb ? b + d : b + d - P.nb;
@@ -155,10 +158,16 @@ __device__ __forceinline__ void item(const double* __restrict__ x, const double*
             for (int s = 0; s < 64; ++s) {
 #pragma unroll
                 for (int t = 0; t < T; ++t) {
-                    const double u = (xi[t] - xj) * invL;
-                    const double z = u - __builtin_rint(u);
-                    const double y = dev::rsq_f64(__builtin_fma(z, z, delta2));
-                    const double k = z * (y - C);
+                    double k;
+                    if constexpr (kOpen) {
+                        const double z = xi[t] - xj;
+                        k = z * dev::rsq_f64(__builtin_fma(z, z, delta2));
+                    } else {
+                        const double u = (xi[t] - xj) * invL;
+                        const double z = u - __builtin_rint(u);
+                        const double y = dev::rsq_f64(__builtin_fma(z, z, delta2));
+                        k = z * (y - C);
+                    }
                     acc[t] = __builtin_fma(wj, k, acc[t]);
                     accj = __builtin_fma(nwi[t], k, accj);
                 }
