@@ -27,6 +27,12 @@ def test_argument_errors_without_device_work():
     assert b"null" in lib.fs_last_error()
     assert lib.fs_emq_periodic_f64(None, None, None, 0, None, None, 0, 1.0, 0.1) == -1
     assert lib.fs_emq_set_tuning(None, 1, 1) == -1
+    assert lib.fs_emq_open_f64(None, None, None, 0, None, None, 0, 0.1) == -1
+    assert lib.fs_emq_open_f64_dev(None, None, None, 0, None, None, 0, 0.1, None) == -1
+    assert lib.fs_external_field_add(None, None, None, 0, 0.0, 0, 0.4, 0.26, 0.37) == -1
+    assert lib.fs_external_field_add_dev(None, None, None, 0, 0.0, 0, 0.4, 0.26, 0.37,
+                                         None) == -1
+    assert b"null" in lib.fs_last_error()
 
 
 def test_product_refuses_to_run_without_its_library(monkeypatch, tmp_path):
