@@ -93,3 +93,47 @@ def test_external_field_matches_oracle(gpu_ctx, typ):
         ref = oracle.external_field_add(es0, x, t_, typ, 0.4, 0.26, 0.37)
         a = abs(oracle.external_amplitude(typ, t_, 0.4, 0.26))
         assert np.max(np.abs(e - ref)) <= 4.5e-16 * (a + np.max(np.abs(es0))) + 1e-300, t_
+
+
+def test_device_entry_points_match_host(gpu_ctx):
+    """fs_emq_open_f64_dev (targets = sources: the pair-once form) and
+    fs_external_field_add_dev on device arrays, bitwise equal to the host
+    entry points (the same kernels and plans). Device memory comes from the
+    HIP runtime the library itself uses."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    h2d, d2h = 1, 2
+
+    def upload(a):
+        p = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(p), a.nbytes) == 0
+        assert hip.hipMemcpy(p, a.ctypes.data, a.nbytes, h2d) == 0
+        return p
+
+    rng = np.random.default_rng(21)
+    n = 20000
+    x = rng.uniform(0, 10, n)
+    w = rng.normal(size=n) * 1e-4
+    host = gpu_ctx.emq_open_f64(x, x, w, 0.05)
+    dx, dw, de = upload(x), upload(w), upload(np.zeros(n))
+    try:
+        print("open-bc dev entry point, n=20000", flush=True)
+        gpu_ctx.emq_open_f64_dev(de.value, dx.value, n, dx.value, dw.value, n, 0.05)
+        gpu_ctx.synchronize()
+        out = np.empty(n)
+        assert hip.hipMemcpy(out.ctypes.data, de, out.nbytes, d2h) == 0
+        assert np.array_equal(out, host)
+        print("external drive dev entry point", flush=True)
+        want = gpu_ctx.external_field_add(host, x, 61.0, 2, 0.4, 0.26, 0.37)
+        assert ms.lib().fs_external_field_add_dev(gpu_ctx.h, de, dx, n, 61.0, 2, 0.4, 0.26, 0.37,
+                                                  None) == 0
+        gpu_ctx.synchronize()
+        assert hip.hipMemcpy(out.ctypes.data, de, out.nbytes, d2h) == 0
+        assert np.array_equal(out, want)
+    finally:
+        for p in (dx, dw, de):
+            hip.hipFree(p)
