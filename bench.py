@@ -289,7 +289,8 @@ FLOP_PER_OPEN_PAIR = 7  # sub, FMA (2), rsqrt (1), product, FMA (2)
 
 def bench_open(args):
     """The open-boundary direct sum (E_MQ_DirectSum_openbcs) of the N=2^20 deck,
-    targets = sources, every ordered pair evaluated."""
+    targets = sources: each unordered pair evaluated once and given to both
+    points (the kernel is odd); the rate counts n^2 interaction-equivalents."""
     import torch
 
     import multi_species_amd as ms

@@ -109,10 +109,30 @@ int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits);
  * pointer; for the host entry point also two arrays of identical contents,
  * as the reference passes them): the kernel is odd, so each pair can be
  * evaluated once and given to both points. mode 0 (default) = from 16384
- * points up, 1 = never, 2 = at every size. No counterpart in the reference
+ * points up, 1 = never, 2 = at every size. Periodic and open-boundary sums
+ * alike (targets_per_thread and source_splits of fs_emq_set_tuning also
+ * shape the pair-once form of both; the open-boundary full sum has a fixed
+ * launch shape). No counterpart in the reference
  * (its E_MQ_DirectSum evaluates every ordered pair); the result agrees with
  * the full sum to rounding. Applies to later calls on this context. */
 int fs_emq_set_symmetric(fs_ctx* ctx, int mode);
+
+/* Bound on the pair-once form's partial sums (row and column partials and
+ * the per-point state carried between passes), in bytes; 0 (the default) =
+ * 4 GiB. The column partials of a whole evaluation take about n^2 / 512
+ * bytes (2 GiB at n = 2^20, 32 GiB at 2^22); beyond the bound the source
+ * distances are evaluated in passes, each folded into the per-point sums in
+ * the same order, so the result does not depend on the bound. At least one
+ * pass of 16 source blocks is always taken (about 160 B per point). Applies
+ * to later periodic and open-boundary calls on this context. No counterpart
+ * in the reference. */
+int fs_emq_set_pair_once_budget(fs_ctx* ctx, int64_t bytes);
+
+/* Host-only: the pair-once plan of n points (targets = sources) under a
+ * budget (0 = the default): device scratch in bytes, passes, and kernel
+ * evaluations. Any output may be NULL. No counterpart in the reference. */
+int fs_emq_pair_once_plan(int64_t n, int64_t budget_bytes, int64_t* scratch_bytes, int* passes,
+                          int64_t* evals);
 
 /* Kernel evaluations (pairs, padding of the last block or tile included) of
  * the last FP64 direct sum launched on this context (periodic or open):
@@ -272,6 +292,13 @@ typedef struct fs_deck {
     const fs_species_params* species;
     int n_species;
 } fs_deck;
+
+/* Host-only: the device bytes a deck is estimated to take in a batch or a
+ * server (state, RK4 work, field partials -- in FP64 the pair-once partials of
+ * its full field --, sorted copies in mixed precision, panel records); the
+ * server admits decks against its budget with this estimate. Builds the
+ * deck's meshes on the host. No counterpart in the reference. */
+int fs_deck_footprint(const fs_deck* deck, int precision, int64_t* bytes, int64_t* n_points);
 
 /* Creates a batch. Decks share remesh_period; precision FS_PRECISION_F64
  * (the reference's method) or FS_PRECISION_MIXED (fs_emq_periodic_mixed's

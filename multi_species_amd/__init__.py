@@ -53,6 +53,11 @@ def lib():
                                                 _d, _vp]
         L.fs_emq_set_tuning.argtypes = [_vp, _int, _int]
         L.fs_emq_set_symmetric.argtypes = [_vp, _int]
+        L.fs_emq_set_pair_once_budget.argtypes = [_vp, _i64]
+        L.fs_emq_pair_once_plan.argtypes = [_i64, _i64, ctypes.POINTER(_i64),
+                                            ctypes.POINTER(_int), ctypes.POINTER(_i64)]
+        L.fs_deck_footprint.argtypes = [ctypes.POINTER(Deck), _int, ctypes.POINTER(_i64),
+                                        ctypes.POINTER(_i64)]
         L.fs_emq_f64_pair_evals.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64)]
         L.fs_emq_periodic_mixed.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
         L.fs_emq_mixed_shape.argtypes = [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int)]
@@ -153,6 +158,12 @@ class Context:
         """Pair-once FP64 sum when targets are the sources: 0 = from 16384
         points up (default), 1 = never, 2 = always."""
         _check(lib().fs_emq_set_symmetric(self.h, mode), "fs_emq_set_symmetric")
+
+    def set_pair_once_budget(self, nbytes: int = 0):
+        """Bound on the pair-once form's partials in bytes (0 = 4 GiB); the
+        result does not depend on it."""
+        _check(lib().fs_emq_set_pair_once_budget(self.h, int(nbytes)),
+               "fs_emq_set_pair_once_budget")
 
     def set_tuning(self, targets_per_thread: int = 0, source_splits: int = 0):
         _check(lib().fs_emq_set_tuning(self.h, targets_per_thread, source_splits),
@@ -335,6 +346,25 @@ def sim_params(x_min=0.0, x_max=0.0, bcs=0, quadrature=1, greens_epsilon=0.2, us
     p.greens_epsilon, p.use_treecode, p.mac = greens_epsilon, use_treecode, mac
     p.degree, p.max_source, p.dt, p.remesh_period = degree, max_source, dt, remesh_period
     return p
+
+
+def pair_once_plan(n: int, budget_bytes: int = 0):
+    """(scratch bytes, passes, kernel evaluations) of the pair-once FP64 sum
+    of n points under a budget (host-only)."""
+    b, p, e = _i64(), _int(), _i64()
+    _check(lib().fs_emq_pair_once_plan(int(n), int(budget_bytes), ctypes.byref(b),
+                                       ctypes.byref(p), ctypes.byref(e)), "fs_emq_pair_once_plan")
+    return b.value, p.value, e.value
+
+
+def deck_footprint(sim: "SimParams", species, precision: int = 0):
+    """(estimated device bytes, points) of a deck in a batch or server
+    (host-only: builds the deck's meshes on the host)."""
+    d, arr = _deck(sim, species)
+    b, n = _i64(), _i64()
+    _check(lib().fs_deck_footprint(ctypes.byref(d), int(precision), ctypes.byref(b),
+                                   ctypes.byref(n)), "fs_deck_footprint")
+    return b.value, n.value
 
 
 def mesh_build(sim: SimParams, sp: SpeciesParams):

@@ -113,23 +113,6 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(double* __restrict__ a, in
     if (i < n) a[i] = v;
 }
 
-// ---- targets = sources: each pair once (fs_sym.h) ----------------------
-
-template <int T>
-__global__ __launch_bounds__(sym::threads<T>()) void emq_sym_f64_kernel(
-    const double* __restrict__ x, const double* __restrict__ w, sym::Plan P, double invL,
-    double delta2, double* __restrict__ rowpart, double* __restrict__ colpart) {
-    __shared__ double red[2][sym::kMaxWaves][64];
-    sym::item<T>(x, w, P, blockIdx.x, invL, delta2, rowpart, colpart, red);
-}
-
-__global__ __launch_bounds__(kBlock) void sym_finalize_kernel(
-    sym::Plan P, const double* __restrict__ rowpart, const double* __restrict__ colpart,
-    double* __restrict__ es) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p < P.n) es[p] = sym::point(P, p, rowpart, colpart);
-}
-
 struct Plan {
     int tpt;
     int splits;
@@ -161,25 +144,16 @@ Plan plan_direct(int64_t nt, int64_t ns, int num_cus, const FieldTuning& tun) {
     return p;
 }
 
+}  // namespace
+
 bool use_sym(int64_t nt, int64_t ns, bool same, const FieldTuning& tun) {
     if (!same || nt != ns || nt == 0 || tun.symmetric == 1) return false;
     return tun.symmetric == 2 || nt >= sym::kMinPoints;
 }
 
-size_t sym_rowpart_bytes(const sym::Plan& P) {
-    return align_up(sizeof(double) * (size_t)sym::rowpart_elems(P), 256);
-}
-size_t sym_colpart_bytes(const sym::Plan& P) {
-    return align_up(sizeof(double) * (size_t)sym::colpart_elems(P), 256);
-}
-}  // namespace
-
 size_t field_scratch_bytes(int64_t nt, int64_t ns, bool same, int num_cus,
                            const FieldTuning& tun) {
-    if (use_sym(nt, ns, same, tun)) {
-        const sym::Plan P = sym::plan(nt, num_cus, tun);
-        return sym_rowpart_bytes(P) + sym_colpart_bytes(P);
-    }
+    if (use_sym(nt, ns, same, tun)) return sym::scratch_bytes(sym::plan(nt, tun, tun.sym_budget));
     const Plan p = plan_direct(nt, ns, num_cus, tun);
     return p.splits > 1 ? align_up(sizeof(double) * nt * p.splits, 256) : 0;
 }
@@ -206,28 +180,8 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src, const
     const double delta = eps / L;
     const double delta2 = delta * delta;
     if (use_sym(nt, ns, d_tgt == d_src, tun)) {
-        const sym::Plan P = sym::plan(nt, num_cus, tun);
-        double* rowpart = static_cast<double*>(scratch);
-        double* colpart = reinterpret_cast<double*>(static_cast<char*>(scratch) +
-                                                    sym_rowpart_bytes(P));
-        switch (P.t) {
-            case 4:
-                emq_sym_f64_kernel<4><<<P.nitems, sym::threads<4>(), 0, stream>>>(
-                    d_src, d_w, P, invL, delta2, rowpart, colpart);
-                break;
-            case 2:
-                emq_sym_f64_kernel<2><<<P.nitems, sym::threads<2>(), 0, stream>>>(
-                    d_src, d_w, P, invL, delta2, rowpart, colpart);
-                break;
-            default:
-                emq_sym_f64_kernel<1><<<P.nitems, sym::threads<1>(), 0, stream>>>(
-                    d_src, d_w, P, invL, delta2, rowpart, colpart);
-                break;
-        }
-        FS_HIP_CHECK(hipGetLastError());
-        sym_finalize_kernel<<<(nti + kBlock - 1) / kBlock, kBlock, 0, stream>>>(P, rowpart,
-                                                                              colpart, d_es);
-        FS_HIP_CHECK(hipGetLastError());
+        const sym::Plan P = sym::plan(nt, tun, tun.sym_budget);
+        FS_HIP_CHECK(sym::run<false>(d_src, d_w, P, invL, delta2, scratch, d_es, stream));
         if (pair_evals) *pair_evals = sym::evals(P);
         return FS_OK;
     }
@@ -279,6 +233,24 @@ int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits) {
 int fs_emq_f64_pair_evals(fs_ctx* ctx, int64_t* evals) {
     if (!ctx || !evals) return fs::fail(FS_E_INVALID, "null pointer");
     *evals = ctx->f64_pair_evals;
+    return FS_OK;
+}
+
+int fs_emq_set_pair_once_budget(fs_ctx* ctx, int64_t bytes) {
+    if (!ctx) return fs::fail(FS_E_INVALID, "null context");
+    if (bytes < 0) return fs::fail(FS_E_INVALID, "budget must be >= 0");
+    ctx->tuning.sym_budget = bytes;
+    return FS_OK;
+}
+
+int fs_emq_pair_once_plan(int64_t n, int64_t budget_bytes, int64_t* scratch_bytes, int* passes,
+                          int64_t* evals) {
+    if (n < 1 || n > INT32_MAX) return fs::fail(FS_E_INVALID, "n must be in [1, 2^31)");
+    if (budget_bytes < 0) return fs::fail(FS_E_INVALID, "budget must be >= 0");
+    const fs::sym::Plan P = fs::sym::plan(n, fs::FieldTuning{}, budget_bytes);
+    if (scratch_bytes) *scratch_bytes = (int64_t)fs::sym::scratch_bytes(P);
+    if (passes) *passes = P.npass;
+    if (evals) *evals = fs::sym::evals(P);
     return FS_OK;
 }
 

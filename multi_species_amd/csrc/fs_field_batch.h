@@ -124,8 +124,12 @@ struct ScanBlock {
 };
 
 // Pair-once plan of one FP64 segment (its own source; fs_sym.h).
+// A batch runs each such segment in one pass (sym::pass(P, 0) covers every
+// chunk); segments whose single-pass partials exceed sym::kBatchDeckLimit
+// take the full sum instead, a choice from the segment's size alone.
 struct SymSeg {
     sym::Plan P;
+    sym::Pass S;
     int seg, off;
     int item_base;          // first workgroup of the segment in its launch
     int64_t rbase, cbase;   // row and column partials (doubles into the work space)

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(sym::threads<T>()) void batch_sym_kernel(
         else hi = mid - 1;
     }
     const SymSeg E = tab[lo];
-    sym::item<T>(x + E.off, w + E.off, E.P, (int)blockIdx.x - E.item_base, E.invL, E.delta2,
+    sym::item<T>(x + E.off, w + E.off, E.P, E.S, (int)blockIdx.x - E.item_base, E.invL, E.delta2,
                  sympart + E.rbase, sympart + E.cbase, red);
 }
 
@@ -171,7 +171,9 @@ __global__ __launch_bounds__(kBlock) void finalize_f64_kernel(
     }
     if (S.sym >= 0) {
         const SymSeg& E = symsegs[S.sym];
-        es[g] = sym::point(E.P, g - S.off, sympart + E.rbase, sympart + E.cbase);
+        double s = 0.0, lo = 0.0;
+        sym::fold(E.P, E.S, g - S.off, sympart + E.rbase, sympart + E.cbase, s, lo);
+        es[g] = 0.5 * (s + lo);
         return;
     }
     double s = 0.0, lo = 0.0;
@@ -1371,12 +1373,17 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         S.splits = 1;
         S.chunk = std::max(1, S.src_n);
         S.sym = -1;
-        const bool pair_once = precision == kPrecisionF64 && b.eval && b.src == d && b.n > 0 &&
-                               tun.symmetric != 1 &&
-                               (tun.symmetric == 2 || b.n >= sym::kMinPoints);
+        bool pair_once = precision == kPrecisionF64 && b.eval && b.src == d && b.n > 0 &&
+                         use_sym(b.n, b.n, true, tun);
+        sym::Plan SP{};
+        if (pair_once) {
+            SP = sym::plan(b.n, tun, INT64_MAX);  // one pass
+            pair_once = (int64_t)sym::scratch_bytes(SP) <= sym::kBatchDeckLimit;
+        }
         if (pair_once) {
             SymSeg E{};
-            E.P = sym::plan(b.n, num_cus, tun);
+            E.P = SP;
+            E.S = sym::pass(SP, 0);
             E.seg = d;
             E.off = b.off;
             E.invL = S.invL;
@@ -1422,7 +1429,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
                 if (E0.P.t != t) continue;
                 SymSeg E = E0;
                 E.item_base = sym_items_[k];
-                sym_items_[k] += E.P.nitems;
+                sym_items_[k] += E.S.nitems;
                 E.rbase = sym_elems;
                 sym_elems += sym::rowpart_elems(E.P);
                 E.cbase = sym_elems;

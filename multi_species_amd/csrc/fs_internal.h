@@ -54,7 +54,14 @@ struct FieldTuning {
     // FP64 direct sum with targets = sources: 0 = each pair once (K odd)
     // from a size up, 1 = never, 2 = at every size
     int symmetric = 0;
+    // bound on the pair-once form's partials in bytes (0 = the default,
+    // 4 GiB); beyond it the evaluation runs in passes (same result)
+    int64_t sym_budget = 0;
 };
+
+// FP64 direct sum with targets = sources (one array, nt == ns): whether the
+// pair-once form (fs_sym.h) runs, by the size and the context's tuning.
+bool use_sym(int64_t nt, int64_t ns, bool same, const FieldTuning& tun);
 
 // Periodic direct sum on device arrays (FP64, compensated). `scratch` must
 // hold field_scratch_bytes(nt, ns, same, ...) bytes, same = the targets are

@@ -76,21 +76,6 @@ __global__ __launch_bounds__(kBlock) void open_f64_kernel(const double* __restri
     }
 }
 
-template <int T>
-__global__ __launch_bounds__(sym::threads<T>()) void open_sym_kernel(
-    const double* __restrict__ x, const double* __restrict__ w, sym::Plan P, double eps2,
-    double* __restrict__ rowpart, double* __restrict__ colpart) {
-    __shared__ double red[2][sym::kMaxWaves][64];
-    sym::item<T, true>(x, w, P, blockIdx.x, 0.0, eps2, rowpart, colpart, red);
-}
-
-__global__ __launch_bounds__(kBlock) void open_sym_finish_kernel(
-    sym::Plan P, const double* __restrict__ rowpart, const double* __restrict__ colpart,
-    const double* __restrict__ x, double* __restrict__ es) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p < P.n) es[p] = sym::point(P, p, rowpart, colpart) - x[p];
-}
-
 // es[i] = 0.5 * (sum of the splits, in split order) - x_i.
 __global__ __launch_bounds__(kBlock) void open_finish_kernel(const double* __restrict__ part,
                                                              int64_t nt, int splits,
@@ -171,32 +156,11 @@ int fs_emq_open_f64_dev(fs_ctx* ctx, double* d_es, const double* d_targets, int6
     if (!d_es || !d_targets || (ns > 0 && (!d_sources || !d_q_ws)))
         return fs::fail(FS_E_INVALID, "null pointer");
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-    if (d_targets == d_sources && nt == ns && nt >= fs::sym::kMinPoints && nt <= INT32_MAX) {
-        const fs::sym::Plan P = fs::sym::plan(nt, ctx->num_cus, fs::FieldTuning{});
-        const size_t rb = fs::align_up(sizeof(double) * (size_t)fs::sym::rowpart_elems(P), 256);
-        const size_t cb = fs::align_up(sizeof(double) * (size_t)fs::sym::colpart_elems(P), 256);
-        FS_TRY(ctx->scratch.ensure(rb + cb, st));
-        double* rowpart = ctx->scratch.as<double>(0);
-        double* colpart = ctx->scratch.as<double>(rb);
-        const double eps2 = epsilon * epsilon;
-        switch (P.t) {
-            case 4:
-                fs::open_sym_kernel<4><<<P.nitems, fs::sym::threads<4>(), 0, st>>>(
-                    d_sources, d_q_ws, P, eps2, rowpart, colpart);
-                break;
-            case 2:
-                fs::open_sym_kernel<2><<<P.nitems, fs::sym::threads<2>(), 0, st>>>(
-                    d_sources, d_q_ws, P, eps2, rowpart, colpart);
-                break;
-            default:
-                fs::open_sym_kernel<1><<<P.nitems, fs::sym::threads<1>(), 0, st>>>(
-                    d_sources, d_q_ws, P, eps2, rowpart, colpart);
-                break;
-        }
-        FS_HIP_CHECK(hipGetLastError());
-        fs::open_sym_finish_kernel<<<(unsigned)((nt + fs::kBlock - 1) / fs::kBlock), fs::kBlock, 0,
-                                     st>>>(P, rowpart, colpart, d_sources, d_es);
-        FS_HIP_CHECK(hipGetLastError());
+    if (nt <= INT32_MAX && fs::use_sym(nt, ns, d_targets == d_sources, ctx->tuning)) {
+        const fs::sym::Plan P = fs::sym::plan(nt, ctx->tuning, ctx->tuning.sym_budget);
+        FS_TRY(ctx->scratch.ensure(fs::sym::scratch_bytes(P), st));
+        FS_HIP_CHECK(fs::sym::run<true>(d_sources, d_q_ws, P, 0.0, epsilon * epsilon,
+                                        ctx->scratch.ptr, d_es, st));
         ctx->f64_pair_evals = fs::sym::evals(P);
         return FS_OK;
     }

@@ -909,8 +909,12 @@ int64_t deck_points(const Deck* d) {
 size_t deck_footprint(const Deck* d, int precision) {
     // per point: state 5 + RK4 7 doubles, reduced map and species index,
     // field partials (up to 16 splits) and, in mixed precision, the sorted
-    // copies; per panel: record, coefficient, walk and descent records
+    // copies; per panel: record, coefficient, walk and descent records; in
+    // FP64, the pair-once partials of the deck's full field (about n^2 / 512
+    // bytes, fs_sym.h)
     size_t b = 0;
+    if (precision == FS_PRECISION_F64 && d->prm.use_treecode <= 0)
+        b += sym::batch_bytes(deck_points(d), FieldTuning{});
     for (const auto& s : d->sp) {
         const size_t n = (size_t)s->n(), P = (size_t)s->npanels(), g = (size_t)(s->nx * s->np);
         b += n * (12 * 8 + 3 * 4 + 16 * 8 + (precision == FS_PRECISION_MIXED ? 96 : 0));
@@ -936,23 +940,48 @@ int create_empty(fs_ctx* ctx, int precision, fs_sim** out) {
 
 int attach_decks(fs_sim* S, std::vector<Deck*>& decks, std::vector<int>* indices) {
     if (decks.empty()) return FS_OK;
-    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
-    for (Deck*& d : decks) {
-        if (d->prm.remesh_period != 1) {
-            for (Deck*& e : decks) free_deck(e), e = nullptr;
-            decks.clear();
+    auto drop_all = [&]() {
+        for (Deck*& e : decks) free_deck(e), e = nullptr;
+        decks.clear();
+    };
+    // every deck is checked before any joins, so a refused call leaves the
+    // batch as it was
+    for (Deck* d : decks) {
+        if (!d || d->prm.remesh_period != 1) {
+            drop_all();
             return fail(FS_E_INVALID, "decks joining a running batch remesh every step");
         }
+    }
+    if (hipSetDevice(S->ctx->device) != hipSuccess) {
+        drop_all();
+        return fail(FS_E_HIP, "attach: hipSetDevice failed");
+    }
+    const size_t before = S->decks.size();
+    for (Deck*& d : decks) {
         S->decks.emplace_back(d);
-        if (indices) indices->push_back((int)S->decks.size() - 1);
         d = nullptr;
     }
     decks.clear();
     renumber(*S);
     S->tune_pending = true;
-    FS_TRY(layout(*S, true));
-    FS_TRY(field_reduced(*S));
-    FS_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+    int rc = layout(*S, true);
+    if (rc == FS_OK) rc = field_reduced(*S);
+    if (rc == FS_OK && hipStreamSynchronize(S->ctx->stream) != hipSuccess)
+        rc = fail(FS_E_HIP, "attach: stream synchronization failed");
+    if (rc != FS_OK) {
+        // the joining decks leave again; the resident ones keep their state
+        // (layout copies it device to device from wherever it lives now)
+        const std::string why = fs_last_error();
+        for (size_t k = before; k < S->decks.size(); ++k)
+            for (auto& s : S->decks[k]->sp) s->release();
+        S->decks.resize(before);
+        renumber(*S);
+        S->tune_pending = true;
+        (void)layout(*S, true);
+        return fail(rc, "%s", why.c_str());
+    }
+    if (indices)
+        for (size_t k = before; k < S->decks.size(); ++k) indices->push_back((int)k);
     return FS_OK;
 }
 
@@ -1047,15 +1076,25 @@ int fs_sim_add_deck(fs_sim* S, const fs_deck* deck, int* index) {
     FS_HIP_CHECK(hipSetDevice(S->ctx->device));
     std::unique_ptr<sim::Deck> D;
     FS_TRY(sim::build_deck(deck->sim, deck->species, deck->n_species, D));
-    S->decks.push_back(std::move(D));
-    sim::renumber(*S);
-    S->tune_pending = true;
     // resident decks keep their state; the field of the reduced points is
-    // recomputed for all (bitwise the same for the resident ones)
-    FS_TRY(sim::layout(*S, true));
-    FS_TRY(sim::field_reduced(*S));
-    FS_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
-    if (index) *index = (int)S->decks.size() - 1;
+    // recomputed for all (bitwise the same for the resident ones); on failure
+    // the batch is left as it was
+    std::vector<sim::Deck*> one{D.release()};
+    std::vector<int> idx;
+    FS_TRY(sim::attach_decks(S, one, &idx));
+    if (index) *index = idx.empty() ? -1 : idx[0];
+    return FS_OK;
+}
+
+int fs_deck_footprint(const fs_deck* deck, int precision, int64_t* bytes, int64_t* n_points) {
+    if (!deck || !bytes) return fail(FS_E_INVALID, "fs_deck_footprint: null argument");
+    if (precision != FS_PRECISION_F64 && precision != FS_PRECISION_MIXED)
+        return fail(FS_E_INVALID, "unknown precision %d", precision);
+    sim::Deck* D = nullptr;
+    FS_TRY(sim::build_deck_host(deck, &D));
+    *bytes = (int64_t)sim::deck_footprint(D, precision);
+    if (n_points) *n_points = sim::deck_points(D);
+    sim::free_deck(D);
     return FS_OK;
 }
 

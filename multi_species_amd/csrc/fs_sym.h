@@ -22,6 +22,14 @@
 // partials and its sources' column partials to their own slots, and the
 // finalize adds each point's partials in a fixed order, so results are
 // deterministic and independent of scheduling.
+//
+// The column partials take 8 B per point and source block of a chunk, about
+// n^2 / (2 * rb) doubles for a whole evaluation (2 GiB at n = 2^20, 32 GiB at
+// 2^22). To bound them, the chunks of every row block are evaluated in passes
+// of as many chunks as a byte budget holds; after each pass a fold kernel
+// adds the pass's partials to a per-point (s, lo) TwoSum state, chunk by
+// chunk, so the sequence of additions -- and the result -- does not depend on
+// how many passes the budget allows.
 #pragma once
 
 #include <algorithm>
@@ -42,6 +50,20 @@ constexpr int kMaxWaves = 8;
 // blocks are too few to fill the chip and the full sum is faster).
 constexpr int64_t kMinPoints = 16384;
 
+// Reference CU count for the item sizes, so a plan (and with it the
+// summation order) is a function of the point count and the tuning alone.
+constexpr int kPlanCUs = 256;
+// Source blocks per item at most: keeps one chunk's column partials at
+// 8 * 16 B per point, so the passes below can bound the scratch at any size.
+constexpr int kMaxChunkBlocks = 16;
+// Default bound on the partials (row, column, carried state) of one
+// pair-once evaluation.
+constexpr int64_t kDefaultBudget = int64_t(4) << 30;
+// Batched decks run their pair-once evaluation in one pass; a deck whose
+// single-pass partials would exceed this takes the full sum (about 740k
+// points and up).
+constexpr int64_t kBatchDeckLimit = int64_t(1) << 30;
+
 struct Plan {
     int n;       // points
     int t;       // targets per lane (1, 2, 4)
@@ -50,17 +72,47 @@ struct Plan {
     int h;       // (nb - 1) / 2
     int even;    // nb even: blocks b < nb/2 also take the distance nb/2
     int hmax;    // largest distance
-    int chb;     // source blocks per item
-    int nch;     // items per row block
+    int chb;     // source blocks (distances) per chunk
+    int nch;     // chunks per row block
     int nsym;    // nb * nch symmetric items, then nb self items
-    int nitems;
+    int nitems;  // items of the whole evaluation
+    int cpp;     // chunks per pass
+    int npass;   // passes (> 1 when the partials of all chunks exceed the budget)
+};
+
+// Pass k of a plan: the chunks [c0, c0 + cnt) of every row block, and in
+// pass 0 also every block with itself. Item ids of a pass: b * cnt + (c - c0)
+// for the chunk items, then nb * cnt + b for the self items.
+struct Pass {
+    int c0, cnt, self, nitems;
 };
 
 __device__ __host__ __forceinline__ int hb(const Plan& P, int b) {
     return P.h + ((P.even && b < P.nb / 2) ? 1 : 0);
 }
 
-inline Plan plan(int64_t n, int num_cus, const FieldTuning& tun) {
+inline Pass pass(const Plan& P, int k) {
+    Pass S{};
+    S.c0 = k * P.cpp;
+    S.cnt = std::max(0, std::min(P.cpp, P.nch - S.c0));
+    S.self = k == 0 ? 1 : 0;
+    S.nitems = P.nb * S.cnt + (S.self ? P.nb : 0);
+    return S;
+}
+
+// Bytes of each partial array (per pass; reused by the next pass).
+inline int64_t rowpart_elems(const Plan& P) { return ((int64_t)P.nb * P.cpp + P.nb) * P.rb; }
+inline int64_t colpart_elems(const Plan& P) { return (int64_t)P.nb * P.cpp * P.chb * P.rb; }
+// (s, lo) carried per point from one pass to the next
+inline int64_t state_elems(const Plan& P) { return P.npass > 1 ? 2 * (int64_t)P.n : 0; }
+inline size_t al256(int64_t elems) { return ((size_t)elems * sizeof(double) + 255) / 256 * 256; }
+inline size_t scratch_bytes(const Plan& P) {
+    return al256(rowpart_elems(P)) + al256(colpart_elems(P)) + al256(state_elems(P));
+}
+
+// budget <= 0: kDefaultBudget. The chunking depends on n and the tuning only;
+// the budget decides how many chunks share a pass, which changes no result.
+inline Plan plan(int64_t n, const FieldTuning& tun, int64_t budget = 0) {
     Plan P{};
     P.n = (int)n;
     if (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
@@ -76,24 +128,39 @@ inline Plan plan(int64_t n, int num_cus, const FieldTuning& tun) {
     // about 32 items per CU (two resident workgroups per CU, 16 rounds) so
     // the last round's imbalance stays small
     const int64_t pairs = (int64_t)P.nb * P.h + (P.even ? P.nb / 2 : 0);
-    const int64_t want = int64_t(32) * num_cus;
+    const int64_t want = int64_t(32) * kPlanCUs;
     int64_t chb = tun.source_splits > 0 ? (P.hmax + tun.source_splits - 1) / tun.source_splits
                                         : (pairs + want - 1) / want;
+    chb = std::min<int64_t>(chb, kMaxChunkBlocks);
     chb = std::max<int64_t>(1, std::min<int64_t>(chb, std::max(P.hmax, 1)));
     P.chb = (int)chb;
     P.nch = P.hmax > 0 ? (P.hmax + P.chb - 1) / P.chb : 0;
     P.nsym = P.nb * P.nch;
     P.nitems = P.nsym + P.nb;
+    // passes: as many chunks per pass as the budget holds (at least one)
+    const int64_t lim = budget > 0 ? budget : kDefaultBudget;
+    const int64_t blk = (int64_t)P.nb * P.rb * (int64_t)sizeof(double);
+    const int64_t per_chunk = blk * (1 + P.chb);
+    if (P.nch == 0 || blk + per_chunk * P.nch <= lim) {
+        P.cpp = std::max(P.nch, 1);
+    } else {
+        const int64_t avail = lim - blk - 16 * (int64_t)P.n;
+        P.cpp = (int)std::max<int64_t>(1, std::min<int64_t>(P.nch, avail / per_chunk));
+    }
+    P.npass = P.nch > 0 ? (P.nch + P.cpp - 1) / P.cpp : 1;
     return P;
 }
 
-// Partial slots (doubles): one row partial per point of a row block per item,
-// one column partial per point of a source block per item and distance.
-inline int64_t rowpart_elems(const Plan& P) { return (int64_t)P.nitems * P.rb; }
-inline int64_t colpart_elems(const Plan& P) { return (int64_t)P.nsym * P.chb * P.rb; }
+// Partials of a batched deck of n points evaluated in one pass (0 where the
+// deck takes the full sum instead).
+inline size_t batch_bytes(int64_t n, const FieldTuning& tun) {
+    if (n < 1 || n > INT32_MAX || !use_sym(n, n, true, tun)) return 0;
+    const size_t b = scratch_bytes(plan(n, tun, INT64_MAX));
+    return b <= (size_t)kBatchDeckLimit ? b : 0;
+}
 
-// Kernel evaluations of one pass: block pairs at distance >= 1 once, each
-// block with itself in full.
+// Kernel evaluations of a whole evaluation: block pairs at distance >= 1
+// once, each block with itself in full.
 inline int64_t evals(const Plan& P) {
     const int64_t bp = (int64_t)P.nb * P.h + (P.even ? P.nb / 2 : 0) + P.nb;
     return bp * P.rb * P.rb;
@@ -107,29 +174,31 @@ __device__ __forceinline__ double wave_rotate(double v) {
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
 
-// One item (row block and source chunk, or a block with itself) of the plan
-// P over the points x[0..n), w[0..n). red: the workgroup's LDS,
-// [2][kMaxWaves][64] doubles. Called by all threads<T>() threads of a workgroup.
+// One item (row block and source chunk, or a block with itself) of pass S of
+// the plan P over the points x[0..n), w[0..n). id: the item's index within
+// the pass. red: the workgroup's LDS, [2][kMaxWaves][64] doubles. Called by
+// all threads<T>() threads of a workgroup.
 // kOpen: the open-boundary kernel z / sqrt(z^2 + delta2) with z = x_i - x_j
 // (E_MQ_DirectSum_openbcs, src/FieldStructure.cpp:166-191; also odd in z;
 // invL unused) instead of the periodic one.
 template <int T, bool kOpen = false>
 __device__ __forceinline__ void item(const double* __restrict__ x, const double* __restrict__ w,
-                                     const Plan& P, int id, double invL, double delta2,
-                                     double* __restrict__ rowpart, double* __restrict__ colpart,
+                                     const Plan& P, const Pass& S, int id, double invL,
+                                     double delta2, double* __restrict__ rowpart,
+                                     double* __restrict__ colpart,
                                      double (*red)[kMaxWaves][64]) {
     constexpr int kWaves = waves<T>();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int rb = threads<T>() * T;
     int b, d0, d1;
-    const bool self = id >= P.nsym;
+    const bool self = id >= P.nb * S.cnt;
     if (!self) {
-        b = id / P.nch;
-        const int c = id - b * P.nch;
+        b = id / S.cnt;
+        const int c = S.c0 + (id - b * S.cnt);
         d0 = 1 + c * P.chb;
         d1 = min(hb(P, b), (c + 1) * P.chb) + 1;
     } else {
-        b = id - P.nsym;
+        b = id - P.nb * S.cnt;
         d0 = 0;
         d1 = 1;
     }
@@ -197,23 +266,108 @@ __device__ __forceinline__ void item(const double* __restrict__ x, const double*
         rowpart[(int64_t)id * rb + wave * 64 * T + t * 64 + lane] = hi[t] + lo[t];
 }
 
-// E at point p: its self item's row partial, its row block's items in chunk
-// order, then the column partials of the items that reached its block, by
-// increasing distance; TwoSum throughout, halved at the end.
-__device__ __forceinline__ double point(const Plan& P, int p, const double* __restrict__ rowpart,
-                                        const double* __restrict__ colpart) {
+// Adds point p's partials of pass S to (s, lo) in a fixed order: its self
+// item's row partial (pass 0), then chunk by chunk its row block's row
+// partial of the chunk and the column partials of the items of that chunk
+// that reached its block, by increasing distance. TwoSum throughout. The
+// sequence of additions is the same however the chunks are cut into passes,
+// so every budget gives the same result bit for bit.
+__device__ __forceinline__ void fold(const Plan& P, const Pass& S, int p,
+                                     const double* __restrict__ rowpart,
+                                     const double* __restrict__ colpart, double& s, double& lo) {
     const int b = p / P.rb, loc = p - b * P.rb;
+    if (S.self) dev::two_sum_acc(s, lo, rowpart[((int64_t)P.nb * S.cnt + b) * P.rb + loc]);
+    for (int k = 0; k < S.cnt; ++k) {
+        const int c = S.c0 + k;
+        dev::two_sum_acc(s, lo, rowpart[((int64_t)b * S.cnt + k) * P.rb + loc]);
+        const int dl = min((c + 1) * P.chb, P.hmax);
+        for (int d = c * P.chb + 1; d <= dl; ++d) {
+            const int b2 = b - d >= 0 ? b - d : b - d + P.nb;
+            if (d > hb(P, b2)) continue;
+            dev::two_sum_acc(
+                s, lo,
+                colpart[(((int64_t)b2 * S.cnt + k) * P.chb + (d - 1 - c * P.chb)) * P.rb + loc]);
+        }
+    }
+}
+
+// Pass S at point p: (s, lo) from the carried state (zero in pass 0), the
+// pass's partials added, written back to the state unless this is the last
+// pass. Returns (s + lo) / 2, which is E at p after the last pass.
+__device__ __forceinline__ double pass_point(const Plan& P, const Pass& S, int p, bool last,
+                                             const double* __restrict__ rowpart,
+                                             const double* __restrict__ colpart,
+                                             double* __restrict__ state) {
     double s = 0.0, lo = 0.0;
-    dev::two_sum_acc(s, lo, rowpart[(int64_t)(P.nsym + b) * P.rb + loc]);
-    for (int c = 0; c < P.nch; ++c)
-        dev::two_sum_acc(s, lo, rowpart[(int64_t)(b * P.nch + c) * P.rb + loc]);
-    for (int d = 1; d <= P.hmax; ++d) {
-        const int b2 = b - d >= 0 ? b - d : b - d + P.nb;
-        if (d > hb(P, b2)) continue;
-        const int id2 = b2 * P.nch + (d - 1) / P.chb;
-        dev::two_sum_acc(s, lo, colpart[((int64_t)id2 * P.chb + (d - 1) % P.chb) * P.rb + loc]);
+    if (!S.self) {
+        s = state[2 * (int64_t)p];
+        lo = state[2 * (int64_t)p + 1];
+    }
+    fold(P, S, p, rowpart, colpart, s, lo);
+    if (!last) {
+        state[2 * (int64_t)p] = s;
+        state[2 * (int64_t)p + 1] = lo;
     }
     return 0.5 * (s + lo);
+}
+
+template <int T, bool kOpen>
+__global__ __launch_bounds__(threads<T>()) void pass_kernel(
+    const double* __restrict__ x, const double* __restrict__ w, Plan P, Pass S, double invL,
+    double delta2, double* __restrict__ rowpart, double* __restrict__ colpart) {
+    __shared__ double red[2][kMaxWaves][64];
+    item<T, kOpen>(x, w, P, S, blockIdx.x, invL, delta2, rowpart, colpart, red);
+}
+
+// kOpen: E = (s + lo) / 2 - x_p (the open-boundary form's uniform background).
+template <bool kOpen>
+__global__ __launch_bounds__(256) void fold_kernel(Plan P, Pass S, int last,
+                                                   const double* __restrict__ rowpart,
+                                                   const double* __restrict__ colpart,
+                                                   double* __restrict__ state,
+                                                   const double* __restrict__ x,
+                                                   double* __restrict__ es) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= P.n) return;
+    const double e = pass_point(P, S, p, last != 0, rowpart, colpart, state);
+    if (last) es[p] = kOpen ? e - x[p] : e;
+}
+
+// Every pass of plan P over x[0..n), w[0..n) on `stream`, E into es. scratch:
+// scratch_bytes(P) bytes.
+template <bool kOpen>
+inline hipError_t run(const double* x, const double* w, const Plan& P, double invL,
+                      double delta2, void* scratch, double* es, hipStream_t stream) {
+    double* rowpart = static_cast<double*>(scratch);
+    double* colpart = reinterpret_cast<double*>(static_cast<char*>(scratch) +
+                                                al256(rowpart_elems(P)));
+    double* state = reinterpret_cast<double*>(static_cast<char*>(scratch) +
+                                              al256(rowpart_elems(P)) +
+                                              al256(colpart_elems(P)));
+    for (int k = 0; k < P.npass; ++k) {
+        const Pass S = pass(P, k);
+        switch (P.t) {
+            case 4:
+                pass_kernel<4, kOpen><<<S.nitems, threads<4>(), 0, stream>>>(
+                    x, w, P, S, invL, delta2, rowpart, colpart);
+                break;
+            case 2:
+                pass_kernel<2, kOpen><<<S.nitems, threads<2>(), 0, stream>>>(
+                    x, w, P, S, invL, delta2, rowpart, colpart);
+                break;
+            default:
+                pass_kernel<1, kOpen><<<S.nitems, threads<1>(), 0, stream>>>(
+                    x, w, P, S, invL, delta2, rowpart, colpart);
+                break;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        fold_kernel<kOpen><<<(P.n + 255) / 256, 256, 0, stream>>>(
+            P, S, k == P.npass - 1 ? 1 : 0, rowpart, colpart, state, x, es);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace sym

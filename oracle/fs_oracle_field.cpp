@@ -221,8 +221,11 @@ void treecode(double* es, const double* sources, const double* q_ws,
     for (int j = 0; j < PP; ++j) w1i[j] = ((j % 2 == 0) ? 1 : -1) * dj[j];
     const size_t nnode = tree.size();
     std::vector<double> t1(nnode * PP, 0.0), mom(nnode * PP, 0.0);
-    std::vector<double> a1i(PP);
+    // nodes in parallel: each node's moments are summed over its own points
+    // in the reference's order
+#pragma omp parallel for schedule(dynamic, 16)
     for (size_t node = 1; node < nnode; ++node) {
+        std::vector<double> a1i(PP);
         const double x1 = tree[node].x0, x2 = tree[node].x1;
         double* tn = &t1[node * PP];
         double* mn_ = &mom[node * PP];
@@ -261,6 +264,9 @@ void treecode(double* es, const double* sources, const double* q_ws,
         return -1.0 * s * (1 - std::sqrt(deno / (4 * s * s * L * L +
                                                  4 * epsilon * epsilon)));
     };
+    // leaves in parallel: every target's far and near sums keep the
+    // reference's order (each target belongs to one leaf)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : far_evals)
     for (size_t li = 0; li < nleaf; ++li) {
         const TNode& lf = tree[leaves[li]];
         for (size_t ii = lf.m0; ii <= lf.m1; ++ii) {
@@ -282,6 +288,7 @@ void treecode(double* es, const double* sources, const double* q_ws,
             keps[ii] += tempx;
         }
     }
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : near_evals)
     for (size_t li = 0; li < nleaf; ++li) {
         const TNode& lf = tree[leaves[li]];
         for (size_t ii = lf.m0; ii <= lf.m1; ++ii) {

@@ -183,3 +183,50 @@ def test_rsq_refinement_is_full_precision(gpu_ctx):
         ex = oracle.direct_sum_ext(t, np.array([0.0]), np.array([1.0]), L0, eps)
         scale = np.max(np.abs(ex))
         assert np.max(np.abs(e - ex)) <= 8e-16 * scale, eps
+
+
+def test_pair_once_budget_changes_no_bit(gpu_ctx):
+    """The pair-once partials in passes under a small budget (3 and 16 passes
+    at N = 2^20) against one pass: bitwise the same field, periodic and
+    open-boundary alike."""
+    import multi_species_amd as ms
+
+    n = 1 << 20
+    x, w = c2_deck(n, seed=5)
+    assert ms.pair_once_plan(n)[1] == 1
+    assert ms.pair_once_plan(n, 1 << 30)[1] == 3
+    assert ms.pair_once_plan(n, 1)[1] == 16
+    ref = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+    ref_open = gpu_ctx.emq_open_f64(x, x, w, 0.05)
+    try:
+        for budget in (1 << 30, 1):
+            gpu_ctx.set_pair_once_budget(budget)
+            e = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+            assert np.array_equal(e, ref), budget
+            eo = gpu_ctx.emq_open_f64(x, x, w, 0.05)
+            assert np.array_equal(eo, ref_open), budget
+    finally:
+        gpu_ctx.set_pair_once_budget(0)
+
+
+def test_pair_once_at_2p22_within_budget(gpu_ctx):
+    """N = 2^22 targets = sources: one pass would need 32 GiB of partials; the
+    default budget runs it in 10 passes within 4 GiB. Sampled targets against
+    the full-sum kernel over all sources (distinct target array), the uniform
+    lattice's zero field, and the exact oddness in w."""
+    import multi_species_amd as ms
+
+    n = 1 << 22
+    b, passes, evals = ms.pair_once_plan(n)
+    assert passes == 10 and b <= 4 << 30
+    assert evals == n * n // 2 + n * 2048 // 2
+    x, w = c2_deck(n, seed=11)
+    e = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+    assert gpu_ctx.f64_pair_evals() == evals
+    pick = np.sort(np.random.default_rng(2).choice(n, 2048, replace=False))
+    ref = gpu_ctx.emq_periodic_f64(x[pick].copy(), x, w, L0, 0.05)
+    assert np.max(np.abs(e[pick] - ref)) <= TOL * np.max(np.abs(e))
+    assert np.array_equal(gpu_ctx.emq_periodic_f64(x, x, -w, L0, 0.05), -e)
+    lat = np.arange(n) * (L0 / n)
+    e0 = gpu_ctx.emq_periodic_f64(lat, lat, np.full(n, 1.0 / n), L0, 0.05)
+    assert np.max(np.abs(e0)) <= 1e-13

@@ -219,3 +219,29 @@ def test_rejects_bad_eps(gpu_ctx):
     x, w = random_deck(10, 1)
     with pytest.raises(msa.FarrsightError):
         gpu_ctx.emq_periodic_mixed(x, x, w, L0, 0.0, same=True)
+
+
+def test_c2_twenty_seeds_full_field(gpu_ctx):
+    """The c2 error study: N = 2^20 decks of 20 seeds at eps 0.01 and 0.05,
+    the full mixed field (every target, minimum band and error target 4e-9)
+    against the full FP64 field of the same deck (the pair-once FP64 path,
+    pinned to the reference and the extended oracle by test_gpu_field)."""
+    n = 1 << 20
+    worst = {}
+    try:
+        for eps in (0.01, 0.05):
+            for seed in range(20):
+                x, w = random_deck(n, seed)
+                f64 = gpu_ctx.emq_periodic_f64(x, x, w, L0, eps)
+                gpu_ctx.set_mixed_target(0.0)
+                e0 = gpu_ctx.emq_periodic_mixed(x, x, w, L0, eps, same=True)
+                gpu_ctx.set_mixed_target(4e-9)
+                e1 = gpu_ctx.emq_periodic_mixed(x, x, w, L0, eps, same=True)
+                r0, r1 = rel(e0, f64), rel(e1, f64)
+                worst[eps] = max(worst.get(eps, (0, 0))[0], r0), max(worst.get(eps, (0, 0))[1], r1)
+                print(f"c2 seed {seed:2d} eps {eps}: minimum band {r0:.2e}, target 4e-9 {r1:.2e}",
+                      flush=True)
+                assert r0 <= TOL and r1 <= TOL, (seed, eps, r0, r1)
+    finally:
+        gpu_ctx.set_mixed_target(0.0)
+    print("c2 worst over 20 seeds:", {k: f"{a:.2e}/{b:.2e}" for k, (a, b) in worst.items()})

@@ -65,6 +65,32 @@ def test_open_targets_are_sources(gpu_ctx, n):
     _check_open(gpu_ctx, x, x, w, 0.05, f"same n={n}")
 
 
+@pytest.mark.parametrize("n", [1 << 18, 1 << 20])
+def test_open_pair_once_at_benchmark_size(gpu_ctx, n):
+    """The benchmarked configuration (targets = sources, four targets per
+    lane, N = 2^18 and 2^20): sampled targets against the extended oracle
+    over all sources, and the whole field against the full sum over ordered
+    pairs (fs_emq_set_symmetric(1)) within 1e-12 of max|sum|."""
+    rng = np.random.default_rng(n + 3)
+    x = rng.uniform(0, 4 * np.pi, n)
+    w = np.where(np.arange(n) < n // 2, -1.0, 1.0) * np.exp(-rng.uniform(-6, 6, n) ** 2 / 2) * 1e-4
+    print(f"open-bc pair-once n={n}", flush=True)
+    e = gpu_ctx.emq_open_f64(x, x, w, 0.05)
+    assert gpu_ctx.f64_pair_evals() < n * n  # the pair-once form ran
+    pick = np.sort(rng.choice(n, 128, replace=False))
+    ext = oracle.direct_sum_open_ext(x[pick], x, w, 0.05)
+    scale = np.max(np.abs(e + x))
+    slack = 4.5e-16 * np.abs(x[pick])
+    assert np.max(np.maximum(np.abs(e[pick] - ext) - slack, 0.0)) <= 1e-12 * scale
+    gpu_ctx.set_symmetric(1)
+    try:
+        full = gpu_ctx.emq_open_f64(x, x, w, 0.05)
+        assert gpu_ctx.f64_pair_evals() == n * n
+    finally:
+        gpu_ctx.set_symmetric(0)
+    assert np.max(np.maximum(np.abs(e - full) - 4.5e-16 * np.abs(x), 0.0)) <= 1e-12 * scale
+
+
 def test_open_distinct_and_split(gpu_ctx):
     rng = np.random.default_rng(11)
     s = rng.uniform(-10, 20, 200000)

@@ -103,3 +103,80 @@ def test_slots_and_budget():
     ctx.close()
     assert st["completed"] == 5 and st["failed"] == 1
     assert 1 <= max_active <= 2
+
+
+def test_failed_requests_leave_the_others_intact():
+    """Bad decks beside good ones: a deck that remeshes every second step is
+    refused at submit, a deck whose mesh cannot be built (p_max < p_min)
+    fails in the builder; the good decks, submitted before and after them,
+    complete bitwise equal to the same decks run alone."""
+    ctx = ms.Context(0)
+    srv = ms.Server(ctx, hbm_budget_bytes=8e9, max_active=0, precision=ms.PRECISION_F64)
+    good = [(landau(4), 3), (landau(3, amp=0.2, mass=25.0), 4)]
+    bad_period = landau(3)
+    bad_period[0].remesh_period = 2
+    with pytest.raises(ms.FarrsightError):
+        srv.submit(*bad_period, 2)
+    bad_mesh = landau(3)
+    bad_mesh[1][0].p_max = -7.0
+    ids = [srv.submit(*good[0][0], good[0][1]), srv.submit(*bad_mesh, 2),
+           srv.submit(*good[1][0], good[1][1])]
+    srv.wait(ids[0], 120)
+    srv.wait(ids[2], 120)
+    with pytest.raises(ms.FarrsightError):
+        srv.wait(ids[1], 60)
+    assert srv.status(ids[1])[0] == ms.REQ_FAILED
+    results = {rid: [srv.result(rid, k) for k in range(2)] for rid in (ids[0], ids[2])}
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == 2 and st["failed"] == 1
+    for rid, (deck, n) in zip((ids[0], ids[2]), good):
+        ref = alone(ctx, deck, n, ms.PRECISION_F64)
+        for k in range(2):
+            for a in range(5):
+                assert np.array_equal(results[rid][k][a], ref[k][a]), (rid, k, a)
+    ctx.close()
+
+
+def test_refused_join_leaves_the_batch_unchanged():
+    """fs_sim_add_deck refuses a deck that remeshes every second step before
+    anything joins: the batch keeps stepping bitwise as the deck alone."""
+    ctx = ms.Context(0)
+    deck = landau(4)
+    s = ms.Simulation.batch(ctx, [deck], ms.PRECISION_F64)
+    s.step(1)
+    bad = landau(3)
+    bad[0].remesh_period = 2
+    with pytest.raises(ms.FarrsightError):
+        s.add_deck(*bad)
+    assert s.n_decks == 1
+    s.step(2)
+    got = [s.points(k) for k in range(2)]
+    s.close()
+    ref = alone(ctx, deck, 3, ms.PRECISION_F64)
+    for k in range(2):
+        for a in range(5):
+            assert np.array_equal(got[k][a], ref[k][a]), (k, a)
+    ctx.close()
+
+
+def test_fp64_admission_counts_pair_once_partials():
+    """The server's estimate of an FP64 deck includes its pair-once partials:
+    a budget between the estimate without them and with them refuses the
+    deck (it is larger than the whole budget)."""
+    deck = landau(6)
+    f64, n = ms.deck_footprint(*deck, precision=ms.PRECISION_F64)
+    sym = ms.pair_once_plan(n, 1 << 62)[0]
+    assert sym > 0
+    ctx = ms.Context(0)
+    srv = ms.Server(ctx, hbm_budget_bytes=f64 - sym // 2, max_active=0,
+                    precision=ms.PRECISION_F64)
+    rid = srv.submit(*deck, 1)
+    with pytest.raises(ms.FarrsightError):
+        srv.wait(rid, 60)
+    assert srv.status(rid)[0] == ms.REQ_FAILED
+    ok = srv.submit(*landau(4), 1)
+    srv.wait(ok, 60)
+    assert srv.stats()["completed"] == 1
+    srv.close()
+    ctx.close()
