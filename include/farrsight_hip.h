@@ -261,6 +261,7 @@ int fs_mesh_build(const fs_sim_params* sim, const fs_species_params* species,
  * simulation's state stays in device memory. */
 int fs_sim_create(fs_ctx* ctx, const fs_sim_params* sim, const fs_species_params* species,
                   int n_species, fs_sim** out);
+/* Destroy every simulation and server of a context before the context. */
 int fs_sim_destroy(fs_sim* sim);
 /* AMRSimulation::step (AMRSimulation_step.cpp:5-99) without file output:
  * one RK4 step (three field evaluations), then a remesh with the reduced-x

@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes
 import os
 import sys
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -125,9 +126,17 @@ class Context:
         _check(lib().fs_ctx_create(ctypes.byref(h), device), "fs_ctx_create")
         self.h = h
         self.device = device
+        # simulations and servers on this context: closed before it is
+        # destroyed (their C objects use the context until they are)
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, child):
+        self._children.add(child)
 
     def close(self):
         if getattr(self, "h", None):
+            for child in list(getattr(self, "_children", ())):
+                child.close()
             lib().fs_ctx_destroy(self.h)
             self.h = None
 
@@ -388,6 +397,7 @@ class Simulation:
     def __init__(self, ctx: "Context", sim: SimParams = None, species=None, *, _handle=None,
                  _keep=None):
         self.ctx = ctx
+        ctx._adopt(self)
         if _handle is not None:
             self.h = _handle
             self._keep = _keep
@@ -570,6 +580,7 @@ class Server:
                "fs_server_create")
         self.h = h
         self.n_species = {}
+        ctx._adopt(self)
 
     def submit(self, sim: "SimParams", species, n_steps: int) -> int:
         d, arr = _deck(sim, species)

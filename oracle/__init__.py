@@ -98,6 +98,10 @@ def lib():
         L.orc_sim_species.restype = ctypes.c_void_p
         L.orc_sim_get_es.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.orc_sim_diagnostics.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_sim_diagnostics_full.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_diagnostics.argtypes = [_i64, _dp, _dp, _dp, _dp, _dp, _d, _d, _d, _dp]
+        L.orc_species_set_fs.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_species_descend_points.argtypes = [ctypes.c_void_p, _dp, _dp, _i64, _ip]
         L.orc_sim_iter.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
@@ -322,6 +326,20 @@ class OracleSpecies:
         assert xs.size == self.n and ps.size == self.n
         lib().orc_species_set_points(self.h, _P(xs), _P(ps))
 
+    def set_fs(self, fs):
+        fs = _f64(fs)
+        assert fs.size == self.n
+        lib().orc_species_set_fs(self.h, _P(fs))
+
+    def descend_points(self, x, p) -> np.ndarray:
+        """The old-mesh leaf interp_from_mesh uses at each point (0: beyond
+        a p boundary)."""
+        x, p = _f64(x), _f64(p)
+        out = np.empty(x.size, dtype=np.int32)
+        if lib().orc_species_descend_points(self.h, _P(x), _P(p), x.size, _PI(out)) != 0:
+            raise RuntimeError("too many shifts!")
+        return out
+
     def panels(self) -> np.ndarray:
         n = lib().orc_species_num_panels(self.h)
         rec = np.empty((n, self.PANEL_INTS), dtype=np.int32)
@@ -403,6 +421,27 @@ class OracleSim:
         out = np.empty(4)
         lib().orc_sim_diagnostics(self.h, i, _P(out))
         return out
+
+    def diagnostics_full(self, i: int) -> np.ndarray:
+        out = np.empty(DIAG_COUNT)
+        lib().orc_sim_diagnostics_full(self.h, i, _P(out))
+        return out
+
+
+DIAG_COUNT = 14
+DIAG_NAMES = ("charge", "momentum", "kinetic", "electric", "entropy", "l1", "l2", "min_f",
+              "max_f", "negative_area", "num_negative", "frac_negative", "negative_mass",
+              "num_points")
+
+
+def diagnostics(xs, ps, fs, q_ws, es, q: float, qm: float, L: float) -> np.ndarray:
+    """The per-species diagnostic set of diagnostic_plot.py:116-143 on arrays."""
+    a = [_f64(v) for v in (xs, ps, fs, q_ws, es)]
+    n = a[0].size
+    assert all(v.size == n for v in a)
+    out = np.empty(DIAG_COUNT)
+    lib().orc_diagnostics(n, *[_P(v) for v in a], q, qm, L, _P(out))
+    return out
 
 
 def landau_deck(initial_height: int = 5, p_height: int = 1, dt: float = 0.1,

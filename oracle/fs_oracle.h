@@ -117,6 +117,13 @@ int orc_species_remesh(orc_species* s);
 int orc_species_copy_to_old(orc_species* s);
 int orc_species_interp_from_mesh(orc_species* s, const double* x,
                                  const double* p, int64_t n, double* f_out);
+/* Overwrites fs (same count) -- test input for the interpolation pins. */
+int orc_species_set_fs(orc_species* s, const double* fs);
+/* The leaf of the old mesh that interpolate_from_mesh (AMRStructure_interp.cpp:
+ * 1094-1143) interpolates from at each point: periodic pre-shift, root
+ * descent, 0 beyond a p boundary. */
+int orc_species_descend_points(orc_species* s, const double* x, const double* p,
+                               int64_t n, int32_t* leaf_out);
 /* interpolate_to_initial_xps (AMRStructure_interp.cpp:493-793) of the old
  * mesh onto arbitrary base-grid points (n = nx*np); also returns the leaf
  * each sorted point was assigned (0 = beyond boundary) in sorted order and
@@ -140,6 +147,20 @@ void orc_sim_get_es(orc_sim* s, int i, double* es);
 /* Per-species diagnostics (standard_interface/diagnostic_plot.py:116-131):
  * out[0]=charge, [1]=momentum, [2]=kinetic energy, [3]=electric energy. */
 void orc_sim_diagnostics(orc_sim* s, int i, double out[4]);
+/* The whole per-species set of standard_interface/diagnostic_plot.py:116-143
+ * (ORC_DIAG_COUNT values): [0] charge sum q_w, [1] momentum sum p q_w / q,
+ * [2] kinetic energy sum p^2 q_w / (2 q/m), [3] electric energy
+ * 1/2 sum dx_k E_k^2 over the unique x in [0, L) (first occurrence, periodic
+ * midpoint spacing), [4] entropy sum_{f>0} f log10(f) q_w / q, [5] L1
+ * sum |q_w| / |q|, [6] L2 sum q_w f / q, [7] min f, [8] max f, [9] negative
+ * area sum_{f<0} q_w / q, [10] number of f < 0, [11] their fraction,
+ * [12] negative mass |sum_{f<0} q_w| / sum |q_w|, [13] number of points.
+ * orc_diagnostics is the same on plain arrays. */
+#define ORC_DIAG_COUNT 14
+void orc_sim_diagnostics_full(orc_sim* s, int i, double* out);
+void orc_diagnostics(int64_t n, const double* xs, const double* ps, const double* fs,
+                     const double* q_ws, const double* es, double q, double qm, double L,
+                     double* out);
 int orc_sim_iter(const orc_sim* s);
 
 #ifdef __cplusplus

@@ -253,6 +253,27 @@ int orc_species_interp_from_mesh(orc_species* s, const double* x, const double* 
     }
     return 0;
 }
+int orc_species_set_fs(orc_species* s, const double* fs) {
+    std::copy(fs, fs + s->fs.size(), s->fs.begin());
+    return 0;
+}
+int orc_species_descend_points(orc_species* s, const double* x, const double* p, int64_t n,
+                               int32_t* leaf_out) {
+    try {
+        for (int64_t i = 0; i < n; ++i) {
+            // the first half of Species::interpolate_from_mesh
+            std::vector<double> xv(1, x[i]), pv(1, p[i]), sv(1, x[i]);
+            if (s->periodic) s->shift_xs(sv, xv, pv);
+            double sx = sv[0];
+            bool beyond = false;
+            int leaf = s->descend(sx, p[i], beyond, 0);
+            leaf_out[i] = beyond ? 0 : leaf;
+        }
+    } catch (...) {
+        return -1;
+    }
+    return 0;
+}
 int orc_species_interp_to_grid(orc_species* s, const double* xs, const double* ps, int nx,
                                int np, double* f_out, int32_t* leaf_sorted,
                                int32_t* sort_perm) {
@@ -306,31 +327,41 @@ orc_species* orc_sim_species(orc_sim* s, int i) {
 void orc_sim_get_es(orc_sim* s, int i, double* es) { s->species_es(i, es); }
 int orc_sim_iter(const orc_sim* s) { return s->iter; }
 
-// standard_interface/diagnostic_plot.py:116-131 for one species.
-void orc_sim_diagnostics(orc_sim* s, int i, double out[4]) {
-    if (s->need_scatter) s->scatter(false);
-    Species& sp = *s->species[i];
-    const size_t n = sp.xs.size();
-    std::vector<double> e(n);
-    s->species_es(i, e.data());
-    double charge = 0, mom = 0, kin = 0;
+// standard_interface/diagnostic_plot.py:116-143 for one species, in point
+// order (np.dot and np.sum order aside).
+void orc_diagnostics(int64_t n_, const double* xs, const double* ps, const double* fs,
+                     const double* q_ws, const double* e, double q, double qm, double L,
+                     double* out) {
+    const size_t n = (size_t)n_;
+    double charge = 0, mom = 0, kin = 0, ent = 0, l1 = 0, l2 = 0, neg = 0, absw = 0;
+    double fmin = n ? fs[0] : 0.0, fmax = n ? fs[0] : 0.0;
+    int64_t nneg = 0;
     for (size_t k = 0; k < n; ++k) {
-        charge += sp.q_ws[k];
-        mom += sp.ps[k] * sp.q_ws[k];
-        kin += sp.ps[k] * sp.ps[k] * sp.q_ws[k];
+        charge += q_ws[k];
+        mom += ps[k] * q_ws[k];
+        kin += ps[k] * ps[k] * q_ws[k];
+        if (fs[k] > 0) ent += fs[k] * std::log10(fs[k]) * q_ws[k];
+        l1 += std::fabs(q_ws[k]);
+        l2 += q_ws[k] * fs[k];
+        if (fs[k] < 0) {
+            neg += q_ws[k];
+            ++nneg;
+        }
+        absw += std::fabs(q_ws[k]);
+        if (fs[k] < fmin) fmin = fs[k];
+        if (fs[k] > fmax) fmax = fs[k];
     }
     out[0] = charge;
-    out[1] = 1.0 / sp.q * mom;
-    out[2] = 1.0 / 2.0 / sp.qm * kin;
+    out[1] = 1.0 / q * mom;
+    out[2] = 1.0 / 2.0 / qm * kin;
     // electric energy over the unique x in [0, Lx): first occurrence of each x
     std::vector<size_t> idx(n);
     for (size_t k = 0; k < n; ++k) idx[k] = k;
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return sp.xs[a] < sp.xs[b]; });
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return xs[a] < xs[b]; });
     std::vector<double> ux, ue;
-    const double L = s->Lx();
     for (size_t k = 0; k < n; ++k) {
-        const double x = sp.xs[idx[k]];
-        if (k > 0 && x == sp.xs[idx[k - 1]]) continue;  // np.unique keeps the first
+        const double x = xs[idx[k]];
+        if (k > 0 && x == xs[idx[k - 1]]) continue;  // np.unique keeps the first
         if (x >= 0 && x < L) {
             ux.push_back(x);
             ue.push_back(e[idx[k]]);
@@ -352,6 +383,32 @@ void orc_sim_diagnostics(orc_sim* s, int i, double out[4]) {
         pot *= 0.5;
     }
     out[3] = pot;
+    out[4] = 1.0 / q * ent;
+    out[5] = 1.0 / std::fabs(q) * l1;
+    out[6] = 1.0 / q * l2;
+    out[7] = fmin;
+    out[8] = fmax;
+    out[9] = 1.0 / q * neg;
+    out[10] = (double)nneg;
+    out[11] = n ? (double)nneg / (double)n : 0.0;
+    out[12] = absw > 0 ? std::fabs(neg) / absw : 0.0;
+    out[13] = (double)n;
+}
+
+void orc_sim_diagnostics_full(orc_sim* s, int i, double* out) {
+    if (s->need_scatter) s->scatter(false);
+    Species& sp = *s->species[i];
+    const size_t n = sp.xs.size();
+    std::vector<double> e(n);
+    s->species_es(i, e.data());
+    orc_diagnostics((int64_t)n, sp.xs.data(), sp.ps.data(), sp.fs.data(), sp.q_ws.data(),
+                    e.data(), sp.q, sp.qm, s->Lx(), out);
+}
+
+void orc_sim_diagnostics(orc_sim* s, int i, double out[4]) {
+    double full[ORC_DIAG_COUNT];
+    orc_sim_diagnostics_full(s, i, full);
+    for (int k = 0; k < 4; ++k) out[k] = full[k];
 }
 
 }  // extern "C"

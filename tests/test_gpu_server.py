@@ -180,3 +180,17 @@ def test_fp64_admission_counts_pair_once_partials():
     assert srv.stats()["completed"] == 1
     srv.close()
     ctx.close()
+
+
+def test_context_close_closes_its_simulations_and_servers():
+    """A context closed while a simulation and a server built on it are still
+    referenced closes them first (their C objects use the context until they
+    are destroyed); closing them again afterwards does nothing."""
+    ctx = ms.Context(0)
+    s = ms.Simulation.batch(ctx, [landau(3)], ms.PRECISION_F64)
+    s.step(1)
+    srv = ms.Server(ctx, hbm_budget_bytes=1e9, max_active=0, precision=ms.PRECISION_F64)
+    ctx.close()
+    assert s.h is None and srv.h is None
+    s.close()
+    srv.close()

@@ -2,8 +2,8 @@
 
 Bars (per species, after every step): panel tree bit-exact, points back on
 the grid bit-exact after a remesh, E normwise within 1e-12 of the oracle,
-f within 1e-12, charge/momentum/energy/electric energy within 1e-10
-(relative). Decks: two-species weak Landau damping (ions m = 1836) with
+f within 1e-12, charge/momentum/energy/electric energy within 1e-10 of the
+sum of their terms' magnitudes. Decks: two-species weak Landau damping (ions m = 1836) with
 remesh every step and every second step, Simpson quadrature, and an
 adaptively refined two-stream deck.
 """
@@ -35,7 +35,27 @@ def rel(a, b):
     return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
 
 
-def compare(gsim, osim, nsp, step, tol_e=1e-12, tol_f=1e-12, tol_d=1e-10, expect_grid=True):
+def contribution_scale(xs_t, xs, ws, L, eps, m=128):
+    """max_i sum_j |w_j K(x_i - x_j)| over m sampled targets: the magnitude of
+    the terms a field value sums, which bounds the rounding of any order of
+    summation (the field itself can be thousands of times smaller)."""
+    idx = np.linspace(0, xs_t.size - 1, min(m, xs_t.size)).astype(int)
+    d2 = (eps / L) ** 2
+    C = 2 / np.sqrt(1 + 4 * d2)
+    best = 0.0
+    for i in idx:
+        z = (xs_t[i] - xs) / L
+        z -= np.round(z)
+        best = max(best, float(np.sum(np.abs(ws * 0.5 * z * (1 / np.sqrt(z * z + d2) - C)))))
+    return best
+
+
+def compare(gsim, osim, nsp, step, tol_e=1e-12, tol_f=1e-12, tol_d=1e-10, expect_grid=True,
+            e_scale=0.0):
+    """e_scale > 0: E may also differ by 1e-14 e_scale (the contribution scale
+    of contribution_scale) -- where E is small against the terms it sums, two
+    FP64 orders of summation differ by that much (the reference's own FP64 sum
+    is 3e-12 of max|E| from the exact one at the Landau mode's zero crossings)."""
     worst = 0.0
     for k in range(nsp):
         xs, ps, fs, qw, es = gsim.points(k)
@@ -49,11 +69,19 @@ def compare(gsim, osim, nsp, step, tol_e=1e-12, tol_f=1e-12, tol_d=1e-10, expect
             assert rel(xs, oxs) < 1e-13 and rel(ps, ops) < 1e-12
         ef, ee = rel(fs, ofs), rel(es, oes)
         assert ef <= tol_f, f"step {step} species {k}: f {ef}"
-        assert ee <= tol_e, f"step {step} species {k}: E {ee}"
+        bound = tol_e * np.max(np.abs(oes)) + 1e-14 * e_scale
+        assert np.max(np.abs(es - oes)) <= bound, f"step {step} species {k}: E {ee}"
         gd, od = gsim.diagnostics(k), osim.diagnostics(k)
-        dd = np.abs(gd - od) / np.maximum(np.abs(od), 1e-300)
-        # momentum sits at round-off (~1e-16); compare it absolutely against the charge scale
-        dd[1] = abs(gd[1] - od[1]) / max(abs(od[0]), 1.0)
+        # each diagnostic against the sum of its terms' magnitudes (charge
+        # sum |q_w|, momentum sum |p q_w| / |q|, kinetic sum p^2 |q_w| / 2|q/m|,
+        # electric energy itself: its terms are >= 0); a cold species' kinetic
+        # energy comes from f values ~1e-4 of its peak, next to rows whose
+        # rounding is weighted by p^2 up to 36
+        sp = osim.deck.species[k]
+        qm = sp.q / sp.m
+        scale = np.array([np.sum(np.abs(oqw)), np.sum(np.abs(ops * oqw)) / abs(sp.q),
+                          np.sum(ops * ops * np.abs(oqw)) / (2 * abs(qm)), abs(od[3])])
+        dd = np.abs(gd - od) / np.maximum(scale, 1e-300)
         assert np.all(dd <= tol_d), f"step {step} species {k}: diagnostics {dd}"
         worst = max(worst, ee, ef)
     return worst
