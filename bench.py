@@ -16,7 +16,16 @@ evaluation with one accumulation (14 with the second accumulation of the
 pair-once form) times the evaluations the library reports, against
 MI355X's FP64 vector peak. The CPU baseline times the reference's own
 compiled E_MQ_DirectSum (oracle/_ref) on a bounded target sample on this
-host's cores.
+host's cores (all threads, bound close, and one thread).
+
+Beside the field the line carries the step-loop half of the metric:
+"mixed" (the mixed-precision form of the same evaluation, in N^2
+interaction-equivalents/s, with its VALU issue fraction), "plain_step_loop"
+(the Landau deck's FP64 step loop against the oracle's OpenMP step loop on
+the host), "batched" (64 decks stepped as one mixed-precision batch against
+the same decks one by one in FP64: point-steps/s, p50/p99 step latency, E
+error against FP64 at the same points) and "served" (a Poisson request
+stream with continuous batching). --workload runs one of them on its own.
 """
 from __future__ import annotations
 
@@ -26,7 +35,11 @@ import os
 import sys
 import time
 
-import numpy as np
+# host-core baselines: threads bound to consecutive cores (read by the OpenMP
+# runtime when it loads, so before numpy, torch and the oracle)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -43,18 +56,63 @@ FP64_PEAK_TFLOPS = 78.6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "direct_f64_n1048576_traffic.txt")
 
 
-def read_traffic():
+MIXED_VALU_FILE = os.path.join(ROOT, "profiles", "mixed_field_n1048576_valu_counters.md")
+SIMDS = 256 * 4          # MI355X: 256 CUs of 4 SIMDs
+CLOCK_HZ = 2.4e9         # peak engine clock
+
+
+def read_profile_value(path, key):
+    """A `key: value` line of a committed profile summary (None if absent)."""
     try:
-        for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
+        for line in open(path):
+            if line.startswith(key + ":"):
+                return float(line.split(":")[1].split()[0])
     except OSError:
         pass
     return None
 
 
-def cpu_baseline(x, w, eps, budget_s: float = 12.0):
-    """Reference E_MQ_DirectSum on a target sample x all N sources."""
+def read_traffic():
+    return read_profile_value(TRAFFIC_FILE, "traffic_bytes_per_launch")
+
+
+def host_info():
+    """(CPU model, OpenMP threads) of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+class omp_threads:
+    """Sets the OpenMP thread count of the host baselines (libgomp, which the
+    oracle and the reference build link) for a with-block."""
+
+    def __init__(self, n: int):
+        import ctypes
+
+        self.n = n
+        self.lib = ctypes.CDLL("libgomp.so.1")
+        self.lib.omp_get_max_threads.restype = ctypes.c_int
+
+    def __enter__(self):
+        self.prev = self.lib.omp_get_max_threads()
+        self.lib.omp_set_num_threads(self.n)
+        return self
+
+    def __exit__(self, *exc):
+        self.lib.omp_set_num_threads(self.prev)
+
+
+def cpu_baseline(x, w, eps, budget_s: float = 10.0):
+    """Reference E_MQ_DirectSum on a target sample x all N sources, on every
+    host thread and on one."""
     import oracle
 
     kind = "reference"
@@ -63,7 +121,7 @@ def cpu_baseline(x, w, eps, budget_s: float = 12.0):
     if lib is None:
         kind = "port"
     n = x.size
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model, cores = host_info()
     probe = 256
     t0 = time.perf_counter()
     fn(x[:probe], x, w, L0, eps)
@@ -74,13 +132,21 @@ def cpu_baseline(x, w, eps, budget_s: float = 12.0):
     t0 = time.perf_counter()
     fn(x[:m], x, w, L0, eps)
     dt = time.perf_counter() - t0
+    m1 = max(64, (m // cores) // 64 * 64)
+    with omp_threads(1):
+        t0 = time.perf_counter()
+        fn(x[:m1], x, w, L0, eps)
+        dt1 = time.perf_counter() - t0
     return {
         "value": m * n / dt,
         "unit": "pair-interactions/s",
         "cores": cores,
         "kind": kind,
+        "cpu_model": model,
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+        "value_1_thread": m1 * n / dt1,
         "sample": f"{m} targets x {n} sources of the same deck (eps={eps}), "
-                  f"{dt:.1f} s on {cores} host threads",
+                  f"{dt:.1f} s on {cores} host threads; {m1} targets on 1 thread, {dt1:.1f} s",
     }
 
 
@@ -112,6 +178,9 @@ def main():
                          "for profiling the batched step alone)")
     ap.add_argument("--requests", type=int, default=12, help="c5: requests served")
     ap.add_argument("--req-steps", type=int, default=50, help="c5: steps per request")
+    ap.add_argument("--blocks", default="c1,c4,c5",
+                    help="field workload: the step-loop blocks added to the line "
+                         "(comma-separated from c1,c4,c5; 'none' for the field alone)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration "
                          "for the HBM traffic figures) and exit")
@@ -188,7 +257,8 @@ def main():
     traffic = read_traffic()
 
     result = {
-        "metric": "E-field pair-interactions/s",
+        "metric": "E-field pair-interactions/s; batched RK4 point-steps/s + p99 step latency "
+                  "@1e-8 E err",
         "value": value,
         "unit": "pair-interactions/s",
         "n_gpus": 1,
@@ -240,11 +310,24 @@ def main():
     wall_m = time.perf_counter() - t0
     mixed_ms = ev0.elapsed_time(ev1) / args.steps
     err = float((tm - te).abs().max() / te.abs().max())
+    valu = read_profile_value(MIXED_VALU_FILE, "valu_instructions_per_evaluation")
     result["mixed"] = {
+        # the mixed form sums ranges of sources through expansions: its rate is
+        # N^2 interaction-equivalents per evaluation time, not pair evaluations
         "value": pairs * args.steps / wall_m,
-        "unit": "pair-interactions/s",
+        "unit": "pair-interaction-equivalents/s (N^2 per evaluation)",
         "ms_per_step": wall_m / args.steps * 1e3,
         "gpu_ms": mixed_ms,
+        "roofline": {
+            "bound": "valu issue",
+            # wave64 VALU instructions of the whole evaluation (profile) x 4 cycles
+            # each, over the evaluation's HIP-event time on every SIMD at peak clock
+            "achieved": (valu * 4 / (mixed_ms * 1e-3 * SIMDS * CLOCK_HZ)) if valu else None,
+            "peak": 1.0, "unit": "fraction of VALU issue slots",
+            "frac": (valu * 4 / (mixed_ms * 1e-3 * SIMDS * CLOCK_HZ)) if valu else None,
+            "valu_instructions_per_evaluation": valu,
+            "source": os.path.relpath(MIXED_VALU_FILE, ROOT),
+        },
         "speedup_vs_f64": kernel_ms / mixed_ms,
         "rel_err_vs_f64": err,
         "tuned_shape": dict(zip(("targets_per_thread", "wgs_per_cu"), ctx.mixed_shape())),
@@ -275,6 +358,19 @@ def main():
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(x, w, args.eps)
+    del tx, tw, te, tm
+    torch.cuda.empty_cache()
+    # the step-loop half of the metric: the plain FP64 step loop of the Landau
+    # deck against the host cores, the batched mixed step of 64 decks against the
+    # same decks one by one in FP64 (point-steps/s, p50/p99 step latency, E error),
+    # and a served request stream
+    blocks = [b for b in args.blocks.split(",") if b and b != "none"]
+    if "c1" in blocks:
+        result["plain_step_loop"] = c1_block(ctx, 200, 5, "f64", not args.no_cpu_baseline)
+    if "c4" in blocks:
+        result["batched"] = c4_block(ctx, 64, 10, 2)
+    if "c5" in blocks:
+        result["served"] = c5_block(ctx, 12, 10)
     ctx.close()
     print(json.dumps(result))
 
@@ -369,60 +465,88 @@ def bench_open(args):
     print(json.dumps(result))
 
 
-def bench_c1(args):
-    """The whole step loop of the two-species weak-Landau deck on the device."""
+C1_WORKLOAD = ("two-species weak Landau damping deck (electrons q=-1 m=1 amp 0.01 k=0.5; ions "
+               "q=1 m=1836 vth=1/sqrt(1836)), 32x64 quadratic panels per species (65x129 "
+               "points), eps=0.05, direct sum, RK4 dt=0.1, remesh every step")
+
+
+def c1_block(ctx, steps: int, warmup: int, precision: str = "f64", cpu: bool = True,
+             cpu_budget_s: float = 8.0):
+    """The whole step loop of the two-species weak-Landau deck on the device
+    (one step = RK4 with three field evaluations, remesh, reduced-x field),
+    and the oracle's OpenMP restatement of the reference step loop on the host
+    cores beside it."""
     import multi_species_amd as ms
 
-    ctx = ms.Context(0)
     sim_p, sps = c1_deck()
-    prec = ms.PRECISION_MIXED if args.precision == "mixed" else ms.PRECISION_F64
+    prec = ms.PRECISION_MIXED if precision == "mixed" else ms.PRECISION_F64
     sim = ms.Simulation.batch(ctx, [(sim_p, sps)], prec)
     npts = sum(sim.counts(k)[0] for k in range(2))
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         sim.step()
     lat = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         t1 = time.perf_counter()
         sim.step()
         lat.append(time.perf_counter() - t1)
     wall = time.perf_counter() - t0
+    sim.close()
     lat = np.array(lat) * 1e3
-    result = {
-        "metric": "RK4 point-steps/s",
-        "value": npts * args.steps / wall,
-        "unit": "point-steps/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if prec == ms.PRECISION_F64 else "f64+f32", "data": "synthetic",
-        "config": {"workload": "two-species weak Landau damping deck (electrons q=-1 m=1 "
-                               "amp 0.01 k=0.5; ions q=1 m=1836 vth=1/sqrt(1836)), 32x64 "
-                               "quadratic panels per species (65x129 points), eps=0.05, "
-                               "direct sum, RK4 dt=0.1, remesh every step",
-                   "points": npts, "precision": args.precision},
+    out = {
+        "value": npts * steps / wall, "unit": "point-steps/s", "steps": steps,
+        "ms_per_step": wall / steps * 1e3, "points": npts, "precision": precision,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "workload": C1_WORKLOAD,
     }
-    if not args.no_cpu_baseline:
+    if cpu:
         import oracle
 
+        model, cores = host_info()
         deck = oracle.landau_deck(initial_height=5, p_height=1)
         osim = oracle.OracleSim(deck)
-        # steps until about 12 s of host work (at least 3)
         k = 0
         t0 = time.perf_counter()
-        while k < 3 or time.perf_counter() - t0 < 12.0:
+        while k < 3 or time.perf_counter() - t0 < cpu_budget_s:
             osim.step()
             k += 1
         dt = time.perf_counter() - t0
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        result["cpu_baseline"] = {
-            "value": npts * k / dt, "unit": "point-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{k} steps of the same deck with the oracle's OpenMP FP64 restatement "
-                      f"of the reference step loop, {dt:.1f} s on {cores} host threads",
-        }
-    sim.close()
+        with omp_threads(1):
+            t1 = time.perf_counter()
+            osim.step()
+            dt1 = time.perf_counter() - t1
+        cpu_rate = npts * k / dt
+        out["cpu_baseline"] = {
+            "value": cpu_rate, "unit": "point-steps/s", "cores": cores, "kind": "port",
+            "cpu_model": model, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "value_1_thread": npts / dt1,
+            "sample": f"{k} steps of the same deck with the oracle's OpenMP FP64 restatement of "
+                      f"the reference step loop, {dt:.1f} s on {cores} host threads; one more "
+                      f"step on 1 thread, {dt1:.1f} s"}
+        out["speedup_vs_cpu"] = out["value"] / cpu_rate
+    return out
+
+
+def bench_c1(args):
+    """The c1 block as its own bench line."""
+    import multi_species_amd as ms
+
+    ctx = ms.Context(0)
+    blk = c1_block(ctx, args.steps, args.warmup, args.precision, not args.no_cpu_baseline)
     ctx.close()
+    result = {
+        "metric": "RK4 point-steps/s", "value": blk["value"], "unit": "point-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": blk["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "f64" else "f64+f32", "data": "synthetic",
+        "config": {"workload": C1_WORKLOAD, "points": blk["points"],
+                   "precision": args.precision},
+        "latency_ms": blk["latency_ms"],
+    }
+    for k in ("cpu_baseline", "speedup_vs_cpu"):
+        if k in blk:
+            result[k] = blk[k]
     print(json.dumps(result))
 
 
@@ -534,130 +658,122 @@ def bench_c3(args):
     }))
 
 
-def bench_c4(args):
-    """64 decks as one batch (mixed precision, graph-captured step) against the same decks
-    stepped one by one in plain FP64."""
-    import torch  # noqa: F401  (device init through the library)
+C4_WORKLOAD = ("independent two-species Landau decks in one batch (amp log-uniform "
+               "[0.01,0.5], k uniform [0.3,0.6], ion mass ratio in {25,100,1836}, 16x32..128x256 "
+               "panels per species, eps 0.05, dt 0.1, remesh every step), mixed precision, "
+               "graph-captured step")
 
+
+def c4_block(ctx, ndecks: int, steps: int, warmup: int, compare: bool = True,
+             plain_steps: int = 2):
+    """ndecks decks as one batch (mixed precision, graph-captured step) against the
+    same decks stepped one by one in plain FP64; per-step latency of the batch, and
+    the mixed E against the FP64 field of the same points."""
     import multi_species_amd as ms
 
-    ctx = ms.Context(0)
-    decks = c4_decks(args.decks)
+    decks = c4_decks(ndecks)
     b = ms.Simulation.batch(ctx, decks, ms.PRECISION_MIXED)
     npts = sum(b.deck_info(d)[3] for d in range(b.n_decks))
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         b.step()
     lat = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         t1 = time.perf_counter()
         b.step()
         lat.append(time.perf_counter() - t1)
     wall = time.perf_counter() - t0
-    value = npts * args.steps / wall
+    value = npts * steps / wall
     # algorithmic bytes of one remesh of every species (target x, p in and f out;
     # old x, p, f; nine point ids and child/flag words per old panel)
     remesh_bytes = 0
     for k in range(sum(b.deck_info(d)[1] for d in range(b.n_decks))):
         n_k, p_k = b.counts(k)
         remesh_bytes += 24 * n_k + 24 * n_k + 48 * p_k
-    if args.no_compare:
+    lat = np.array(lat) * 1e3
+    out = {
+        "value": value, "unit": "point-steps/s", "steps": steps, "decks": ndecks,
+        "points": int(npts), "ms_per_step": wall / steps * 1e3,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "remesh_algorithmic_bytes_per_step": remesh_bytes,
+        "workload": C4_WORKLOAD,
+    }
+    if not compare:
         b.close()
-        ctx.close()
-        lat = np.array(lat) * 1e3
-        print(json.dumps({
-            "metric": "batched RK4 point-steps/s", "value": value, "unit": "point-steps/s",
-            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
-            "config": {"workload": f"{args.decks} Landau decks, batched step only",
-                       "decks": args.decks, "points": int(npts)},
-            "latency_ms": {"p50": float(np.percentile(lat, 50)),
-                           "p99": float(np.percentile(lat, 99))},
-            "remesh_algorithmic_bytes_per_step": remesh_bytes}))
-        return
+        return out
+
     # E error of the mixed batch against the FP64 field of the same points,
-    # every deck: the served E at the end (remesh field), and the full-field
-    # evaluation at moved points (the positions RK stages see) after three
-    # steps of the same decks with remesh every second step; informational,
-    # the E difference to FP64 runs of the same decks after the same steps
-    # (trajectory drift included)
+    # every deck: the E of the last step (remesh field), and the full-field
+    # evaluation at moved points (the positions RK stages see) after one step
+    # of the same decks with remesh every second step
     def field_err(sim_p, pts):
         x = np.concatenate([p[0] for p in pts])
         w = np.concatenate([p[3] for p in pts])
         es = np.concatenate([p[4] for p in pts])
         ef = ctx.emq_periodic_f64(x, x, w, sim_p.x_max - sim_p.x_min, sim_p.greens_epsilon)
-        return float(np.max(np.abs(es - ef)) / np.max(np.abs(ef))), es
+        return float(np.max(np.abs(es - ef)) / np.max(np.abs(ef)))
 
     err, err_stage = 0.0, 0.0
-    mixed_states = []
     for d, (sim_p, _) in enumerate(decks):
         first, nsp, _, _ = b.deck_info(d)
-        e, es = field_err(sim_p, [b.points(first + k) for k in range(nsp)])
-        err = max(err, e)
-        mixed_states.append(es)
+        err = max(err, field_err(sim_p, [b.points(first + k) for k in range(nsp)]))
     b.close()
-    decks2 = c4_decks(args.decks)
+    decks2 = c4_decks(ndecks)
     for sim_p, _ in decks2:
         sim_p.remesh_period = 2
     v = ms.Simulation.batch(ctx, decks2, ms.PRECISION_MIXED)
-    v.step(3)  # two full-field evaluations governed by measured errors
+    v.step(1)
     for d, (sim_p, _) in enumerate(decks2):
         first, nsp, _, _ = v.deck_info(d)
-        err_stage = max(err_stage, field_err(sim_p, [v.points(first + k) for k in range(nsp)])[0])
+        err_stage = max(err_stage, field_err(sim_p, [v.points(first + k) for k in range(nsp)]))
     v.close()
-    f = ms.Simulation.batch(ctx, decks, ms.PRECISION_F64)
-    f.step(args.warmup + args.steps)
-    drift = 0.0
-    for d in range(f.n_decks):
-        first, nsp, _, _ = f.deck_info(d)
-        ef = np.concatenate([f.points(first + k)[4] for k in range(nsp)])
-        drift = max(drift, float(np.max(np.abs(mixed_states[d] - ef)) / np.max(np.abs(ef))))
-    f.close()
-    # plain FP64, one deck at a time
+    # plain FP64, one deck at a time (one untimed step each)
     t_plain = 0.0
     for sim_p, sps in decks:
         s = ms.Simulation.batch(ctx, [(sim_p, sps)], ms.PRECISION_F64)
         s.step()
         t1 = time.perf_counter()
-        for _ in range(args.steps):
-            s.step()
-        t_plain += time.perf_counter() - t1
+        s.step(plain_steps)
+        t_plain += (time.perf_counter() - t1) / plain_steps
         s.close()
-    plain = npts * args.steps / t_plain
-    lat = np.array(lat) * 1e3
-    result = {
-        "metric": "batched RK4 point-steps/s",
-        "value": value,
-        "unit": "point-steps/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64+f32", "data": "synthetic",
-        "config": {"workload": f"{args.decks} independent two-species Landau decks in one batch "
-                               "(amp log-uniform [0.01,0.5], k uniform [0.3,0.6], ion mass "
-                               "ratio in {25,100,1836}, 16x32..128x256 panels per species, "
-                               "eps 0.05, dt 0.1, remesh every step), mixed precision, "
-                               "graph-captured step",
-                   "decks": args.decks, "points": int(npts)},
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "plain_f64_one_by_one": {"value": plain, "unit": "point-steps/s",
-                                 "speedup_of_batched_mixed": value / plain},
-        "remesh_algorithmic_bytes_per_step": remesh_bytes,
-        "rel_e_err_vs_f64": err,
-        "stage_field_rel_e_err_vs_f64": err_stage,
-        "rel_e_diff_vs_f64_run": drift,
-    }
-    ctx.close()
-    print(json.dumps(result))
+    plain = npts / t_plain
+    out["plain_f64_one_by_one"] = {"value": plain, "unit": "point-steps/s",
+                                   "speedup_of_batched_mixed": value / plain,
+                                   "sample": f"{plain_steps} timed steps of each deck alone"}
+    out["rel_e_err_vs_f64"] = {"remesh_field": err, "full_field_at_moved_points": err_stage,
+                               "bound": 1e-8}
+    return out
 
-def bench_c5(args):
-    """Serving: requests arrive as a Poisson process and are stepped with continuous
-    batching (mixed precision); per-step latency percentiles and aggregate rate."""
+
+def bench_c4(args):
+    """The c4 block as its own bench line."""
     import multi_species_amd as ms
 
-    reqs = c5_requests(args.requests)
     ctx = ms.Context(0)
+    blk = c4_block(ctx, args.decks, args.steps, args.warmup, not args.no_compare)
+    ctx.close()
+    result = {
+        "metric": "batched RK4 point-steps/s", "value": blk["value"], "unit": "point-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": blk["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": f"{args.decks} " + C4_WORKLOAD, "decks": args.decks,
+                   "points": blk["points"]},
+    }
+    result.update({k: v for k, v in blk.items() if k not in ("value", "unit", "steps",
+                                                             "ms_per_step", "workload",
+                                                             "points", "decks")})
+    print(json.dumps(result))
+
+
+def c5_block(ctx, requests: int, req_steps: int, plain_steps: int = 3):
+    """Serving: requests arrive as a Poisson process and are stepped with continuous
+    batching (mixed precision, error governed at 4e-9); per-step latency percentiles
+    and aggregate rate; every request's final E against the FP64 field of the same
+    points; the same requests one by one in plain FP64."""
+    import multi_species_amd as ms
+
+    reqs = c5_requests(requests)
     srv = ms.Server(ctx, hbm_budget_bytes=200e9, max_active=0, precision=ms.PRECISION_MIXED)
     ids = []
     t0 = time.perf_counter()
@@ -665,71 +781,68 @@ def bench_c5(args):
         delay = t_arr - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
-        ids.append(srv.submit(*deck, args.req_steps))
+        ids.append(srv.submit(*deck, req_steps))
     for rid in ids:
         srv.wait(rid, 1200)
     wall = time.perf_counter() - t0
     st = srv.stats()
     finals = {rid: [srv.result(rid, k) for k in range(2)] for rid in ids}
     srv.close()
-    # E error of the served field (mixed, governed) on the final states of the
-    # smallest three decks: the served E against the FP64 field of the same
-    # points; and the one-call mixed entry point at its minimum band and with
-    # the error target 4e-9 on the same points
-    errs, errs_min, errs_chk = [], [], []
-    order = sorted(ids, key=lambda r: sum(finals[r][k][0].size for k in range(2)))
-    for rid in order[:3]:
+    errs = []
+    for rid, (_, _, deck) in zip(ids, reqs):
         x = np.concatenate([finals[rid][k][0] for k in range(2)])
         w = np.concatenate([finals[rid][k][3] for k in range(2)])
         es = np.concatenate([finals[rid][k][4] for k in range(2)])
-        L = reqs[ids.index(rid)][2][0].x_max
-        ef = ctx.emq_periodic_f64(x, x, w, L, 0.05)
-        rel = lambda e: float(np.max(np.abs(e - ef)) / np.max(np.abs(ef)))
-        errs.append(rel(es))
-        ctx.set_mixed_target(0.0)
-        errs_min.append(rel(ctx.emq_periodic_mixed(x, x, w, L, 0.05, same=True)))
-        ctx.set_mixed_target(4e-9)
-        errs_chk.append(rel(ctx.emq_periodic_mixed(x, x, w, L, 0.05, same=True)))
-    ctx.set_mixed_target(0.0)
-    # the same requests in plain FP64, one deck at a time (3 timed steps per deck
-    # after one untimed: the rate at which a one-deck-at-a-time FP64 loop runs them)
+        sim_p = deck[0]
+        ef = ctx.emq_periodic_f64(x, x, w, sim_p.x_max - sim_p.x_min, sim_p.greens_epsilon)
+        errs.append(float(np.max(np.abs(es - ef)) / np.max(np.abs(ef))))
+    sizes = [int(sum(finals[r][k][0].size for k in range(2))) for r in ids]
     plain_pts, plain_s = 0, 0.0
     for _, _, deck in reqs:
         s = ms.Simulation.batch(ctx, [deck], ms.PRECISION_F64)
         n = sum(s.counts(k)[0] for k in range(len(deck[1])))
         s.step()
         t1 = time.perf_counter()
-        s.step(3)
-        plain_s += (time.perf_counter() - t1) / 3
+        s.step(plain_steps)
+        plain_s += (time.perf_counter() - t1) / plain_steps
         plain_pts += n
         s.close()
     plain = plain_pts / plain_s
-    ctx.close()
-    result = {
-        "metric": "served RK4 point-steps/s",
-        "value": st["point_steps_per_s"],
-        "unit": "point-steps/s",
-        "n_gpus": 1, "steps": int(st["steps"]), "warmup": 0,
-        "ms_per_step": st["step_p50_ms"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64+f32", "data": "synthetic",
-        "config": {"workload": f"{args.requests} requests (Poisson arrivals at 20/s) of seeded "
-                               "two-species decks (weak/strong Landau or bump-on-tail), "
-                               "5k..500k base points log-uniform, adaptive refinement up to 2 "
-                               f"levels, {args.req_steps} steps each at dt=0.1, HBM budget "
-                               "200 GB, continuous batching, mixed precision",
-                   "requests": args.requests},
+    return {
+        "value": st["point_steps_per_s"], "unit": "point-steps/s",
         "latency_ms": {"p50": st["step_p50_ms"], "p99": st["step_p99_ms"]},
         "served": {k: st[k] for k in ("submitted", "completed", "failed", "steps",
                                       "point_steps", "busy_s", "span_s")},
-        "wall_s": wall,
-        "rel_e_err_vs_f64": max(errs) if errs else None,
+        "wall_s": wall, "requests": requests, "steps_per_request": req_steps,
+        "points_per_request": {"min": min(sizes), "max": max(sizes)},
+        "kinds": sorted({k for _, k, _ in reqs}),
+        "rel_e_err_vs_f64": {"max_over_requests": max(errs), "bound": 1e-8},
         "plain_f64_one_by_one": {"value": plain, "unit": "point-steps/s",
                                  "speedup_of_served_mixed": st["point_steps_per_s"] / plain,
-                                 "sample": "3 timed steps of each request's deck in plain FP64"},
-        "entry_point_rel_e_err_vs_f64": {"min_band": max(errs_min) if errs_min else None,
-                                         "target_4e-9": max(errs_chk) if errs_chk else None},
+                                 "sample": f"{plain_steps} timed steps of each request's deck"},
+        "workload": (f"{requests} requests (Poisson arrivals at 20/s) of seeded two-species "
+                     "decks (weak/strong Landau or bump-on-tail), 5k..500k base points "
+                     "log-uniform, adaptive refinement up to 2 levels, "
+                     f"{req_steps} steps each at dt=0.1, HBM budget 200 GB, continuous "
+                     "batching, mixed precision"),
     }
+
+
+def bench_c5(args):
+    """The c5 block as its own bench line."""
+    import multi_species_amd as ms
+
+    ctx = ms.Context(0)
+    blk = c5_block(ctx, args.requests, args.req_steps)
+    ctx.close()
+    result = {
+        "metric": "served RK4 point-steps/s", "value": blk["value"], "unit": "point-steps/s",
+        "n_gpus": 1, "steps": int(blk["served"]["steps"]), "warmup": 0,
+        "ms_per_step": blk["latency_ms"]["p50"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": blk["workload"], "requests": args.requests},
+    }
+    result.update({k: v for k, v in blk.items() if k not in ("value", "unit", "workload")})
     print(json.dumps(result))
 
 
