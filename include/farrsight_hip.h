@@ -57,6 +57,20 @@ int fs_emq_periodic_f64_dev(fs_ctx* ctx, double* d_es, const double* d_targets,
                             int64_t nt, const double* d_sources, const double* d_q_ws,
                             int64_t ns, double L, double epsilon, void* stream);
 
+/* Allocation-free form of fs_emq_periodic_f64_dev, with the same arguments
+ * and results (bit for bit: the pair-once partials run in passes within the
+ * work space, which changes no result). No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Its work
+ * space is a fixed 256 MiB array per device, held until the process ends;
+ * the pair-once sum (targets = sources from 16384 points) takes about 160 B
+ * per point, so it accepts at most about 1.6e6 such points and refuses larger
+ * ones with FS_E_RANGE before any launch. Calls overlapping on one device
+ * race for that array. Calls no hipMalloc, hipFree or allocating routine. */
+int fs_emq_periodic_f64_noalloc(fs_ctx* ctx, double* d_es, const double* d_targets, int64_t nt,
+                                const double* d_sources, const double* d_q_ws, int64_t ns,
+                                double L, double epsilon, void* stream);
+
 /* Replaces E_MQ_Treecode::operator() (src/FieldStructure.cpp:230-1050):
  * the periodic barycentric-Lagrange treecode (bisection tree with leaves of
  * fewer than max_source points, MAC theta = mac with the periodic distance,

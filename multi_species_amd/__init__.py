@@ -55,6 +55,8 @@ def lib():
         L.fs_emq_set_tuning.argtypes = [_vp, _int, _int]
         L.fs_emq_set_symmetric.argtypes = [_vp, _int]
         L.fs_emq_set_pair_once_budget.argtypes = [_vp, _i64]
+        L.fs_emq_periodic_f64_noalloc.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d,
+                                                  _vp]
         L.fs_emq_pair_once_plan.argtypes = [_i64, _i64, ctypes.POINTER(_i64),
                                             ctypes.POINTER(_int), ctypes.POINTER(_i64)]
         L.fs_deck_footprint.argtypes = [ctypes.POINTER(Deck), _int, ctypes.POINTER(_i64),
@@ -262,6 +264,16 @@ class Context:
         _check(lib().fs_emq_periodic_f64_dev(self.h, es_ptr, t_ptr, nt, s_ptr, w_ptr, ns, L,
                                              eps, stream or None),
                "fs_emq_periodic_f64_dev")
+
+    def emq_periodic_f64_noalloc(self, es_ptr: int, t_ptr: int, nt: int, s_ptr: int,
+                                 w_ptr: int, ns: int, L: float, eps: float,
+                                 stream: Optional[int] = None):
+        """fs_emq_periodic_f64_dev without any device allocation: the work
+        space is the library's fixed 256 MiB per device (at most about 1.6e6
+        points with targets = sources). Calls overlapping on one device race."""
+        _check(lib().fs_emq_periodic_f64_noalloc(self.h, es_ptr, t_ptr, nt, s_ptr, w_ptr, ns, L,
+                                                 eps, stream or None),
+               "fs_emq_periodic_f64_noalloc")
 
     def emq_open_f64_dev(self, es_ptr: int, t_ptr: int, nt: int, s_ptr: int, w_ptr: int,
                          ns: int, eps: float, stream: Optional[int] = None):

@@ -67,6 +67,9 @@ int fs_ctx_create(fs_ctx** out, int device) {
         delete c;
         return fs::fail(FS_E_HIP, "hipStreamCreateWithFlags failed");
     }
+    // the allocation-free entry point's work space on this device; a failure
+    // is kept for that entry point to report and leaves the others usable
+    if (fs::noalloc_init(device) != FS_OK) (void)hipGetLastError();
     *out = c;
     return FS_OK;
 }

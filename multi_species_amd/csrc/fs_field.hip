@@ -23,6 +23,7 @@
 // independent of co-scheduled work. Accumulation: a plain FP64 sum over each
 // run of kSub sources, folded into a TwoSum (hi, lo) pair per target.
 #include <cstring>
+#include <mutex>
 
 #include "fs_internal.h"
 #include "fs_sym.h"
@@ -215,9 +216,71 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src, const
     return FS_OK;
 }
 
+// ---- the allocation-free form --------------------------------------------
+// The work space of fs_emq_periodic_f64_noalloc: a fixed array in device
+// memory, one per device the library runs on, held until the process ends.
+// Host code never uses the array's name as an address: each device's
+// address comes from hipGetSymbolAddress at context creation.
+static __device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_init(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        return fail(FS_E_INVALID, "no-alloc work space: device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return FS_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        return fail(FS_E_HIP, "hipGetSymbolAddress of the no-alloc work space failed: %s",
+                    hipGetErrorString(e));
+    }
+    g_noalloc_ptr[device] = p;
+    return FS_OK;
+}
+
+void* noalloc_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return g_noalloc_ptr[device];
+}
+
 }  // namespace fs
 
 extern "C" {
+
+int fs_emq_periodic_f64_noalloc(fs_ctx* ctx, double* d_es, const double* d_targets, int64_t nt,
+                                const double* d_sources, const double* d_q_ws, int64_t ns,
+                                double L, double epsilon, void* stream) {
+    if (!ctx) return fs::fail(FS_E_INVALID, "null context");
+    if (nt < 0 || ns < 0 || nt > INT32_MAX || ns > INT32_MAX)
+        return fs::fail(FS_E_INVALID, "direct sum: sizes nt=%lld ns=%lld out of range",
+                        (long long)nt, (long long)ns);
+    void* ws = fs::noalloc_ptr(ctx->device);
+    if (!ws) {  // context creation could not take the address: once more, reporting why
+        FS_HIP_CHECK(hipSetDevice(ctx->device));
+        FS_TRY(fs::noalloc_init(ctx->device));
+        ws = fs::noalloc_ptr(ctx->device);
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    // the pair-once partials in passes within the fixed array
+    fs::FieldTuning tun = ctx->tuning;
+    tun.sym_budget = (int64_t)fs::kNoAllocBytes;
+    const size_t need = fs::field_scratch_bytes(nt, ns, d_targets == d_sources, ctx->num_cus, tun);
+    if (need > fs::kNoAllocBytes)
+        return fs::fail(FS_E_RANGE,
+                        "fs_emq_periodic_f64_noalloc: %lld points need %zu bytes of work space, "
+                        "beyond its fixed %zu (the pair-once sum takes about 160 B per point: "
+                        "at most about 1.6e6 points)",
+                        (long long)nt, need, fs::kNoAllocBytes);
+    return fs::field_direct_f64(d_targets, nt, d_sources, d_q_ws, ns, L, epsilon, d_es, ws,
+                                ctx->num_cus, tun, st, &ctx->f64_pair_evals);
+}
 
 int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits) {
     if (!ctx) return fs::fail(FS_E_INVALID, "null context");

@@ -74,6 +74,13 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src,
                      const FieldTuning& tun, hipStream_t stream,
                      int64_t* pair_evals = nullptr);
 
+// Fixed device work space of fs_emq_periodic_f64_noalloc (per device).
+constexpr size_t kNoAllocBytes = size_t(256) << 20;
+// Takes the current device's address of that work space (once per device;
+// fs_ctx_create calls it with the device current).
+int noalloc_init(int device);
+void* noalloc_ptr(int device);
+
 // Treecode on device arrays (the points are sorted on the device and the
 // tree built on the host). Synchronizes `stream` before returning.
 int treecode_f64(const double* d_x, const double* d_w, int n, double L, double eps, double mac,
