@@ -65,6 +65,32 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+SHIM_SRC = os.path.join(HERE, "shim", "fs_field_shim.cpp")
+SHIM_LIB = os.path.join(HERE, "shim", "libfarrsight_field_shim.so")
+
+
+def build_shim(reference: str, verbose: bool = False) -> str:
+    """The reference-side ElectricField binding (shim/fs_field_shim.cpp),
+    compiled against the reference's own inc/FieldStructure.hpp where it lies
+    and linked to libfarrsight_hip.so. The base class's out-of-line members
+    stay undefined here: the reference's compiled FieldStructure.cpp provides
+    them in the program that loads the binding."""
+    inc = os.path.join(reference, "inc")
+    if not os.path.exists(os.path.join(inc, "FieldStructure.hpp")):
+        raise FileNotFoundError(f"{inc}/FieldStructure.hpp not found")
+    tmp = SHIM_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++14", "-fPIC", "-shared", "-w", "-I", inc,
+           "-I", os.path.join(HERE, "..", "include"), "-o", tmp, SHIM_SRC,
+           "-L", HERE, "-lfarrsight_hip", "-Wl,-rpath,$ORIGIN/.."]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"shim build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, SHIM_LIB)
+    if verbose:
+        print(f"built {SHIM_LIB}")
+    return SHIM_LIB
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)
