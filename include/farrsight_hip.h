@@ -289,8 +289,22 @@ int fs_sim_get_points(fs_sim* sim, int species, double* xs, double* ps, double* 
                       double* q_ws, double* es);
 int fs_sim_get_panels(fs_sim* sim, int species, int32_t* panels);
 /* out[0] charge, [1] momentum, [2] kinetic energy, [3] electric energy
- * (standard_interface/diagnostic_plot.py:116-131). */
+ * (standard_interface/diagnostic_plot.py:116-131); the first four values of
+ * fs_sim_diagnostics_full. */
 int fs_sim_diagnostics(fs_sim* sim, int species, double out[4]);
+
+/* The per-species diagnostics of standard_interface/diagnostic_plot.py:
+ * 116-143, reduced on the device: [0] charge sum q_w, [1] momentum
+ * sum p q_w / q, [2] kinetic energy sum p^2 q_w / (2 q/m), [3] electric
+ * energy 1/2 sum dx_k E_k^2 over the distinct x in [0, L) (E at the first
+ * occurrence, periodic midpoint spacing), [4] entropy
+ * sum_{f>0} f log10(f) q_w / q, [5] L1 sum |q_w| / |q|, [6] L2 sum q_w f / q,
+ * [7] min f, [8] max f, [9] negative area sum_{f<0} q_w / q, [10] number of
+ * points with f < 0, [11] their fraction, [12] negative mass
+ * |sum_{f<0} q_w| / sum |q_w|, [13] number of points. Sums in a fixed order
+ * (they depend on the species' points alone). n_out >= FS_DIAG_COUNT. */
+#define FS_DIAG_COUNT 14
+int fs_sim_diagnostics_full(fs_sim* sim, int species, double* out, int n_out);
 
 /* ---- Batches of independent decks (no counterpart in the reference) -----
  * One fs_sim may hold many decks; each deck's field sees only its own
@@ -315,6 +329,31 @@ typedef struct fs_deck {
  * deck's meshes on the host. No counterpart in the reference. */
 int fs_deck_footprint(const fs_deck* deck, int precision, int64_t* bytes, int64_t* n_points);
 
+/* ---- The reference's on-disk state (no counterpart needed: these are the
+ * reference's own formats, AMRStructure_io.cpp:3-101,
+ * AMRStructure_restart.cpp:33-98, AMRSimulation_field.cpp:81-111) -----------
+ * Per species under <sim_dir>/simulation_output/<species_name>/:
+ * xs/xs_<iter>, ps/ps_<iter>, fs/fs_<iter>, qws/qws_<iter>, es/es_<iter> (raw
+ * doubles, one per point), panels/leaf_point_inds_<iter> (9 int32 per leaf,
+ * depth-first leaf order) and panels/full_tree_<iter> (int64 panel count,
+ * then one 80-byte record per panel: panel_ind, level, parent_ind,
+ * which_child, point_inds[9], left, top, right, bottom, child_inds_start as
+ * int32, then is_left_bdry, is_right_bdry, is_refined_xp, is_refined_p,
+ * needs_refinement as int8 and 3 bytes of padding). Missing directories are
+ * created. */
+
+/* Host-only: one species' files from host arrays (q_ws or es may be NULL:
+ * those files are not written). panels: FS_PANEL_INTS int32 per panel. */
+int fs_write_species_output(const char* sim_dir, const char* species_name, int iter, int64_t n,
+                            const double* xs, const double* ps, const double* fs,
+                            const double* q_ws, const double* es, int64_t n_panels,
+                            const int32_t* panels);
+/* Host-only: reads xs, ps, fs and full_tree of one species back (any output
+ * may be NULL; call with NULL arrays to size them). */
+int fs_read_species_output(const char* sim_dir, const char* species_name, int iter,
+                           int64_t* n, double* xs, double* ps, double* fs, int64_t* n_panels,
+                           int32_t* panels);
+
 /* Creates a batch. Decks share remesh_period; precision FS_PRECISION_F64
  * (the reference's method) or FS_PRECISION_MIXED (fs_emq_periodic_mixed's
  * method for every field evaluation). fs_sim_create is a batch of one deck
@@ -326,6 +365,27 @@ int fs_batch_create(fs_ctx* ctx, const fs_deck* decks, int n_decks, int precisio
  * remesh_period 1 for the batch and the deck. */
 int fs_sim_add_deck(fs_sim* sim, const fs_deck* deck, int* index);
 int fs_sim_remove_deck(fs_sim* sim, int deck);
+/* A batch with no decks yet (remesh every step), for decks that join
+ * through fs_sim_add_deck or fs_sim_restart_deck. */
+int fs_batch_create_empty(fs_ctx* ctx, int precision, fs_sim** out);
+/* AMRSimulation::write_to_file (AMRSimulation_io.cpp:7-17) and the reduced
+ * set of AMRSimulation_field.cpp:81-111 for one deck at the step it reached:
+ * each species' files (names: one per species of the deck), and
+ * <sim_dir>/simulation_output/reduce_xs/xs_<iter> and reduced_es/es_<iter>
+ * (the deck's reduced x, species after species, and E there from the last
+ * reduced evaluation). */
+int fs_sim_write_output(fs_sim* sim, int deck, const char* sim_dir,
+                        const char* const* species_names);
+/* The restart of AMRSimulation.cpp:65-121: a deck joins the batch from the
+ * files of step iter instead of f0 -- its meshes are built from the deck,
+ * then every species' points, f and panel tree replaced by the files, the
+ * weights recomputed from the tree and E evaluated on the reduced set. The
+ * files must hold a state written right after a remesh (points on their
+ * mesh: the weights of moved points are not in the files; refused
+ * otherwise); the batch and the deck remesh every step. Continuing a deck this
+ * way gives bitwise the states of the run that wrote the files. */
+int fs_sim_restart_deck(fs_sim* sim, const fs_deck* deck, const char* sim_dir,
+                        const char* const* species_names, int iter, int* index);
 int fs_sim_num_decks(fs_sim* sim, int* n_decks);
 /* Any output may be NULL. iter counts the steps the deck has taken. */
 int fs_sim_deck_info(fs_sim* sim, int deck, int* first_species, int* n_species, int* iter,
@@ -392,6 +452,16 @@ int fs_server_result(fs_server* server, int64_t request_id, int species, int64_t
                      double* xs, double* ps, double* fs, double* q_ws, double* es,
                      double diag[4]);
 int fs_server_get_stats(fs_server* server, fs_server_stats* out);
+/* The final state of a finished request written in the reference's files
+ * (fs_sim_write_output's per-species files, at the step the deck reached):
+ * the checkpoint of a served deck. species_names: one per species. */
+int fs_server_write_result(fs_server* server, int64_t request_id, const char* sim_dir,
+                           const char* const* species_names);
+/* A request that resumes a deck from such files (fs_sim_restart_deck's
+ * semantics) and takes n_steps more steps. */
+int fs_server_submit_restart(fs_server* server, const fs_deck* deck, const char* sim_dir,
+                             const char* const* species_names, int iter, int n_steps,
+                             int64_t* request_id);
 /* Stops the threads; requests not finished are dropped. */
 int fs_server_destroy(fs_server* server);
 

@@ -87,6 +87,20 @@ def lib():
         L.fs_sim_field_governor.argtypes = [_vp, _int, _dp, _dp, _dp]
         L.fs_sim_get_panels.argtypes = [_vp, _int, _ip]
         L.fs_sim_diagnostics.argtypes = [_vp, _int, _dp]
+        L.fs_sim_diagnostics_full.argtypes = [_vp, _int, _dp, _int]
+        _cpp = ctypes.POINTER(ctypes.c_char_p)
+        L.fs_write_species_output.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _int, _i64, _dp,
+                                              _dp, _dp, _dp, _dp, _i64, _ip]
+        L.fs_read_species_output.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _int,
+                                             ctypes.POINTER(_i64), _dp, _dp, _dp,
+                                             ctypes.POINTER(_i64), _ip]
+        L.fs_batch_create_empty.argtypes = [_vp, _int, ctypes.POINTER(_vp)]
+        L.fs_sim_write_output.argtypes = [_vp, _int, ctypes.c_char_p, _cpp]
+        L.fs_sim_restart_deck.argtypes = [_vp, ctypes.POINTER(Deck), ctypes.c_char_p, _cpp, _int,
+                                          ctypes.POINTER(_int)]
+        L.fs_server_write_result.argtypes = [_vp, _i64, ctypes.c_char_p, _cpp]
+        L.fs_server_submit_restart.argtypes = [_vp, ctypes.POINTER(Deck), ctypes.c_char_p, _cpp,
+                                               _int, _int, ctypes.POINTER(_i64)]
         L.fs_batch_create.argtypes = [_vp, ctypes.POINTER(Deck), _int, _int, ctypes.POINTER(_vp)]
         L.fs_sim_add_deck.argtypes = [_vp, ctypes.POINTER(Deck), ctypes.POINTER(_int)]
         L.fs_sim_remove_deck.argtypes = [_vp, _int]
@@ -315,6 +329,11 @@ class Deck(ctypes.Structure):
 PRECISION_F64 = 0
 PRECISION_MIXED = 1
 
+DIAG_COUNT = 14
+DIAG_NAMES = ("charge", "momentum", "kinetic", "electric", "entropy", "l1", "l2", "min_f",
+              "max_f", "negative_area", "num_negative", "frac_negative", "negative_mass",
+              "num_points")
+
 REQ_QUEUED, REQ_BUILDING, REQ_READY, REQ_ACTIVE, REQ_DONE, REQ_FAILED = 0, 1, 2, 3, 4, -1
 
 
@@ -388,6 +407,38 @@ def deck_footprint(sim: "SimParams", species, precision: int = 0):
     return b.value, n.value
 
 
+def _names(names):
+    arr = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+    return ctypes.cast(arr, ctypes.POINTER(ctypes.c_char_p)), arr
+
+
+def write_species_output(sim_dir: str, name: str, it: int, xs, ps, fs, panels, q_ws=None,
+                         es=None):
+    """One species' files in the reference's formats (host arrays)."""
+    x, p, f = _f64(xs), _f64(ps), _f64(fs)
+    rec = _i32(panels).reshape(-1, PANEL_INTS)
+    w = None if q_ws is None else _f64(q_ws)
+    e = None if es is None else _f64(es)
+    _check(lib().fs_write_species_output(sim_dir.encode(), name.encode(), it, x.size, _P(x),
+                                         _P(p), _P(f), None if w is None else _P(w),
+                                         None if e is None else _P(e), rec.shape[0], _PI(rec)),
+           "fs_write_species_output")
+
+
+def read_species_output(sim_dir: str, name: str, it: int):
+    """(xs, ps, fs, panels) read back from one species' files."""
+    n, P = _i64(), _i64()
+    _check(lib().fs_read_species_output(sim_dir.encode(), name.encode(), it, ctypes.byref(n),
+                                        None, None, None, ctypes.byref(P), None),
+           "fs_read_species_output")
+    xs, ps, fs = (np.empty(n.value) for _ in range(3))
+    rec = np.empty((P.value, PANEL_INTS), dtype=np.int32)
+    _check(lib().fs_read_species_output(sim_dir.encode(), name.encode(), it, None, _P(xs),
+                                        _P(ps), _P(fs), None, _PI(rec)),
+           "fs_read_species_output")
+    return xs, ps, fs, rec
+
+
 def mesh_build(sim: SimParams, sp: SpeciesParams):
     """The initial mesh of one species, built by the library's host code."""
     n, P = _i64(), _i64()
@@ -434,6 +485,29 @@ class Simulation:
         _check(lib().fs_batch_create(ctx.h, darr, len(ds), precision, ctypes.byref(h)),
                "fs_batch_create")
         return cls(ctx, _handle=h, _keep=keep)
+
+    @classmethod
+    def empty(cls, ctx: "Context", precision: int = PRECISION_F64) -> "Simulation":
+        """A batch with no decks (they join with add_deck or restart_deck)."""
+        h = _vp()
+        _check(lib().fs_batch_create_empty(ctx.h, precision, ctypes.byref(h)),
+               "fs_batch_create_empty")
+        return cls(ctx, _handle=h, _keep=[])
+
+    def write_output(self, deck: int, sim_dir: str, names):
+        """The deck's state in the reference's files at the step it reached."""
+        p, keep = _names(names)
+        _check(lib().fs_sim_write_output(self.h, deck, sim_dir.encode(), p),
+               "fs_sim_write_output")
+
+    def restart_deck(self, sim: SimParams, species, sim_dir: str, names, it: int) -> int:
+        """A deck joins from the files of step `it` (written after a remesh)."""
+        d, arr = _deck(sim, species)
+        p, keep = _names(names)
+        idx = _int()
+        _check(lib().fs_sim_restart_deck(self.h, ctypes.byref(d), sim_dir.encode(), p, it,
+                                         ctypes.byref(idx)), "fs_sim_restart_deck")
+        return idx.value
 
     def add_deck(self, sim: SimParams, species) -> int:
         d, arr = _deck(sim, species)
@@ -522,6 +596,13 @@ class Simulation:
     def diagnostics(self, k: int) -> np.ndarray:
         out = np.empty(4)
         _check(lib().fs_sim_diagnostics(self.h, k, _P(out)), "fs_sim_diagnostics")
+        return out
+
+    def diagnostics_full(self, k: int) -> np.ndarray:
+        """The DIAG_COUNT per-species diagnostics (DIAG_NAMES), reduced on the device."""
+        out = np.empty(DIAG_COUNT)
+        _check(lib().fs_sim_diagnostics_full(self.h, k, _P(out), DIAG_COUNT),
+               "fs_sim_diagnostics_full")
         return out
 
 
@@ -621,6 +702,24 @@ class Server:
         _check(lib().fs_server_result(self.h, rid, k, None, *[_P(a) for a in arrs], _P(diag)),
                "fs_server_result")
         return (*arrs, diag)
+
+    def write_result(self, rid: int, sim_dir: str, names):
+        """A finished request's final state in the reference's files."""
+        p, keep = _names(names)
+        _check(lib().fs_server_write_result(self.h, rid, sim_dir.encode(), p),
+               "fs_server_write_result")
+
+    def submit_restart(self, sim: "SimParams", species, sim_dir: str, names, it: int,
+                       n_steps: int) -> int:
+        """A request resuming a deck from the files of step `it`."""
+        d, arr = _deck(sim, species)
+        p, keep = _names(names)
+        rid = _i64()
+        _check(lib().fs_server_submit_restart(self.h, ctypes.byref(d), sim_dir.encode(), p, it,
+                                              int(n_steps), ctypes.byref(rid)),
+               "fs_server_submit_restart")
+        self.n_species[rid.value] = len(species)
+        return rid.value
 
     def stats(self) -> dict:
         st = ServerStats()

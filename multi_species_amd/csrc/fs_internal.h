@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <map>
 #include <utility>
+#include <vector>
 
 #include "../../include/farrsight_hip.h"
 
@@ -73,6 +74,27 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src,
                      double* d_es, void* scratch, int num_cus,
                      const FieldTuning& tun, hipStream_t stream,
                      int64_t* pair_evals = nullptr);
+
+// Per-species diagnostics on device arrays (fs_diag.hip): FS_DIAG_COUNT
+// values into out (host); work holds diag_work_bytes(n) bytes. Synchronizes
+// `stream`.
+namespace diag {
+size_t work_bytes(int n);
+int species(const double* x, const double* p, const double* f, const double* w,
+            const double* e, int n, double q, double qm, double L, void* work, double* out,
+            hipStream_t stream);
+}  // namespace diag
+
+// The reference's on-disk state (fs_io.cpp): one species' files, and the
+// deck's reduced set. Host arrays.
+namespace io {
+int write_species(const char* sim_dir, const char* name, int iter, int64_t n, const double* xs,
+                  const double* ps, const double* fs, const double* q_ws, const double* es,
+                  int64_t n_panels, const int32_t* rec);
+int read_species(const char* sim_dir, const char* name, int iter, std::vector<double>& xs,
+                 std::vector<double>& ps, std::vector<double>& fs, std::vector<int32_t>& rec);
+int write_reduced(const char* sim_dir, int iter, int64_t u, const double* rx, const double* re);
+}  // namespace io
 
 // Fixed device work space of fs_emq_periodic_f64_noalloc (per device).
 constexpr size_t kNoAllocBytes = size_t(256) << 20;

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <numeric>
 #include <stdexcept>
 
@@ -22,8 +23,7 @@ namespace mesh {
 int PanelTree::append_panel(int parent, int which, const int (&pts)[9], int l, int t, int r,
                             int b, bool lb, bool rb) {
     const int id = size();
-    rec.resize(rec.size() + kInts, 0);
-    int32_t* q = &rec[(size_t)id * kInts];
+    int32_t q[kInts] = {};
     q[IND] = id;
     for (int k = 0; k < 9; ++k) q[PT + k] = pts[k];
     q[LEVEL] = parent >= 0 ? at(parent, LEVEL) + 1 : 0;
@@ -35,6 +35,7 @@ int PanelTree::append_panel(int parent, int which, const int (&pts)[9], int l, i
     q[BOTTOM] = b;
     q[FLAGS] = (lb ? LEFT_BDRY : 0) | (rb ? RIGHT_BDRY : 0);
     q[CHILD] = -1;
+    rec.insert(rec.end(), q, q + kInts);
     return id;
 }
 
@@ -128,8 +129,24 @@ const Side kSides[4] = {
 
 }  // namespace
 
+// Room for a pass that splits every panel flagged now into `kids` children
+// with at most `pts` new points each (balance flags may add more panels to
+// the same pass; the vectors then grow as usual).
+void PanelTree::reserve_pass(int kids, int pts, std::vector<double>& nx,
+                             std::vector<double>& np) {
+    size_t flagged = 0;
+    for (int k = min_unrefined; k < size(); ++k) flagged += has(k, NEEDS_REF) ? 1 : 0;
+    rec.reserve(rec.size() + (flagged * kids + 16) * kInts);
+    nx.reserve(flagged * pts + 16);
+    np.reserve(flagged * pts + 16);
+    xs.reserve(xs.size() + flagged * pts + 16);
+    ps.reserve(ps.size() + flagged * pts + 16);
+    fs.reserve(fs.size() + flagged * pts + 16);
+}
+
 void PanelTree::refine_xp(const BatchEval& f) {
     std::vector<double> nx, np;
+    reserve_pass(4, 16, nx, np);
     int nv = (int)xs.size();
     const int before = size();
     for (int jj = min_unrefined; jj < before; ++jj) {
@@ -221,6 +238,7 @@ void PanelTree::refine_xp(const BatchEval& f) {
 
 void PanelTree::refine_p(const BatchEval& f) {
     std::vector<double> nx, np;
+    reserve_pass(2, 6, nx, np);
     int nv = (int)xs.size();
     const int before = size();
     auto refined = [&](int k) { return has(k, REFINED_XP) || has(k, REFINED_P); };
@@ -415,24 +433,56 @@ std::vector<int32_t> grid_rows(const std::vector<double>& xs, const std::vector<
 }
 
 ReducedMap reduced_map(const std::vector<double>& xs) {
+    // unique values by exact equality (as the reference's sort-then-compare
+    // does; +0 and -0 are one value), found with an open-addressed hash in
+    // point order -- a remeshed species has ~1e3 distinct x among ~1e5..1e6
+    // points -- then ranked by sorting only the distinct values
     const size_t n = xs.size();
-    std::vector<int32_t> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return xs[a] < xs[b]; });
     ReducedMap m;
     m.unique_of.assign(n, 0);
-    int u = -1;
-    double last = 0;
-    for (size_t k = 0; k < n; ++k) {
-        const double v = xs[idx[k]];
-        if (u < 0 || v > last) {
-            ++u;
-            last = v;
-            m.ux.push_back(v);
+    size_t cap = 1024;
+    std::vector<int32_t> slot(cap, -1);  // -> index into `first`
+    std::vector<int32_t> first;          // first point of each distinct value
+    auto key = [](double v) {
+        if (v == 0.0) v = 0.0;
+        uint64_t u;
+        std::memcpy(&u, &v, sizeof u);
+        u ^= u >> 33;
+        u *= 0xff51afd7ed558ccdULL;
+        u ^= u >> 33;
+        return u;
+    };
+    std::vector<int32_t> tag(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (2 * (first.size() + 1) > cap) {  // grow, rehash
+            cap *= 2;
+            slot.assign(cap, -1);
+            for (size_t d = 0; d < first.size(); ++d) {
+                size_t h = key(xs[first[d]]) & (cap - 1);
+                while (slot[h] >= 0) h = (h + 1) & (cap - 1);
+                slot[h] = (int32_t)d;
+            }
         }
-        m.unique_of[idx[k]] = u;
+        size_t h = key(xs[i]) & (cap - 1);
+        while (slot[h] >= 0 && !(xs[first[slot[h]]] == xs[i])) h = (h + 1) & (cap - 1);
+        if (slot[h] < 0) {
+            slot[h] = (int32_t)first.size();
+            first.push_back((int32_t)i);
+        }
+        tag[i] = slot[h];
     }
-    const size_t U = m.ux.size();
+    const size_t U = first.size();
+    std::vector<int32_t> order(U);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(),
+              [&](int32_t a, int32_t b) { return xs[first[a]] < xs[first[b]]; });
+    std::vector<int32_t> rank(U);
+    m.ux.resize(U);
+    for (size_t u = 0; u < U; ++u) {
+        rank[order[u]] = (int32_t)u;
+        m.ux[u] = xs[first[order[u]]];
+    }
+    for (size_t i = 0; i < n; ++i) m.unique_of[i] = rank[tag[i]];
     m.offsets.assign(U + 1, 0);
     for (size_t i = 0; i < n; ++i) m.offsets[m.unique_of[i] + 1]++;
     for (size_t k = 0; k < U; ++k) m.offsets[k + 1] += m.offsets[k];

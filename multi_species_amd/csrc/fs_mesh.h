@@ -62,6 +62,7 @@ public:
                       std::vector<int32_t>& leaves) const;
 
 private:
+    void reserve_pass(int kids, int pts, std::vector<double>& nx, std::vector<double>& np);
     int append_panel(int parent, int which, const int (&pts)[9], int l, int t, int r, int b,
                      bool lb, bool rb);
     bool periodic_ = true;

@@ -41,6 +41,10 @@ struct Request {
     fs_sim_params sim{};
     std::vector<fs_species_params> species;
     int n_steps = 0;
+    // a deck resumed from the files of step restart_iter (-1: built from f0)
+    int restart_iter = -1;
+    std::string restart_dir;
+    std::vector<std::string> names;
     int state = FS_REQ_QUEUED;
     int steps_done = 0;
     std::string error;
@@ -93,7 +97,15 @@ void builder_loop(fs_server* S) {
         }
         fs_deck d{&r->sim, r->species.data(), (int)r->species.size()};
         fs::sim::Deck* D = nullptr;
-        const int rc = fs::sim::build_deck_host(&d, &D);
+        int rc;
+        if (r->restart_iter >= 0) {
+            std::vector<const char*> nm;
+            for (const auto& s : r->names) nm.push_back(s.c_str());
+            rc = fs::sim::build_deck_restart_host(&d, r->restart_dir.c_str(), nm.data(),
+                                                  r->restart_iter, &D);
+        } else {
+            rc = fs::sim::build_deck_host(&d, &D);
+        }
         std::string err = rc == FS_OK ? "" : fs_last_error();
         {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -296,6 +308,58 @@ int fs_server_submit(fs_server* S, const fs_deck* deck, int n_steps, int64_t* id
         ++S->submitted;
     }
     S->cv.notify_all();
+    return FS_OK;
+}
+
+int fs_server_submit_restart(fs_server* S, const fs_deck* deck, const char* sim_dir,
+                             const char* const* species_names, int iter, int n_steps,
+                             int64_t* id) {
+    if (!S || !deck || !deck->sim || !deck->species || !id || !species_names)
+        return fail(FS_E_INVALID, "fs_server_submit_restart: null argument");
+    if (n_steps < 1 || deck->n_species < 1 || deck->n_species > 64)
+        return fail(FS_E_INVALID, "fs_server_submit_restart: n_steps >= 1 and 1..64 species");
+    if (deck->sim->remesh_period != 1)
+        return fail(FS_E_INVALID, "fs_server_submit_restart: served decks remesh every step");
+    if (iter < 0) return fail(FS_E_INVALID, "fs_server_submit_restart: negative step");
+    auto r = std::make_unique<Request>();
+    r->sim = *deck->sim;
+    r->species.assign(deck->species, deck->species + deck->n_species);
+    for (int k = 0; k < deck->n_species; ++k) {
+        if (!species_names[k])
+            return fail(FS_E_INVALID, "fs_server_submit_restart: null species name");
+        r->names.emplace_back(species_names[k]);
+    }
+    r->restart_iter = iter;
+    r->restart_dir = sim_dir ? sim_dir : "";
+    r->n_steps = n_steps;
+    r->t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        r->id = S->next_id++;
+        *id = r->id;
+        S->to_build.push_back(r.get());
+        S->all[r->id] = std::move(r);
+        ++S->submitted;
+    }
+    S->cv.notify_all();
+    return FS_OK;
+}
+
+int fs_server_write_result(fs_server* S, int64_t id, const char* sim_dir,
+                           const char* const* species_names) {
+    if (!S || !species_names) return fail(FS_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return fail(FS_E_INVALID, "unknown request %lld", (long long)id);
+    Request* r = it->second.get();
+    if (r->state != FS_REQ_DONE) return fail(FS_E_INVALID, "request %lld not done", (long long)id);
+    for (size_t k = 0; k < r->result.size(); ++k) {
+        if (!species_names[k]) return fail(FS_E_INVALID, "null species name");
+        const sim::SpeciesState& st = r->result[k];
+        FS_TRY(io::write_species(sim_dir, species_names[k], st.iter, (int64_t)st.xs.size(),
+                                 st.xs.data(), st.ps.data(), st.fs.data(), st.qw.data(),
+                                 st.es.data(), (int64_t)(st.rec.size() / 20), st.rec.data()));
+    }
     return FS_OK;
 }
 

@@ -280,6 +280,7 @@ struct fs_sim {
     // concatenated state: xs, ps, qw, es, fs; RK4 work: xk, p2, p3, p4, e2, e3, e4
     DevBuf state, rk, red, status, jobs, pinfo;
     DevBuf tree_work;  // treecode work space (grows to the largest treecode deck)
+    DevBuf diag_work;  // diagnostics work space
     fs::FieldBatch ffull, fred;
     rm::Job* d_jobs = nullptr;            // grid jobs, one per species
     fs::sim::ResetJob* d_reset = nullptr; // reset descriptors, one per species
@@ -894,6 +895,78 @@ int build_deck_host(const fs_deck* deck, Deck** out) {
     return FS_OK;
 }
 
+int load_deck_files(Deck& D, const char* sim_dir, const char* const* names, int iter) {
+    if (!names) return fail(FS_E_INVALID, "restart: null species names");
+    if (iter < 0) return fail(FS_E_INVALID, "restart: negative step");
+    if (iter % D.prm.remesh_period != 0)
+        return fail(FS_E_INVALID,
+                    "restart: step %d is not a remesh step (remesh_period %d): the weights of "
+                    "moved points are not recoverable from the files",
+                    iter, D.prm.remesh_period);
+    for (size_t k = 0; k < D.sp.size(); ++k) {
+        Species& s = *D.sp[k];
+        if (!names[k]) return fail(FS_E_INVALID, "restart: null name of species %zu", k);
+        std::vector<double> xs, ps, fv;
+        std::vector<int32_t> rec;
+        FS_TRY(io::read_species(sim_dir, names[k], iter, xs, ps, fv, rec));
+        if (!s.adaptive && (xs.size() != s.base.xs.size() || rec.size() != s.base.rec.size()))
+            return fail(FS_E_INVALID,
+                        "restart: species %s has %zu points and %zu panels, its deck's mesh "
+                        "%zu and %d",
+                        names[k], xs.size(), rec.size() / mesh::kInts, s.base.xs.size(),
+                        s.base.size());
+        // the points must sit on their mesh (a state written after a remesh):
+        // a static species on its base grid exactly, an adaptive one on every
+        // panel's 3x3 lattice (shared points carry their neighbour's rounding)
+        if (!s.adaptive) {
+            if (xs != s.base.xs || ps != s.base.ps)
+                return fail(FS_E_INVALID,
+                            "restart: species %s's points are not on its grid (not written "
+                            "after a remesh)", names[k]);
+        } else {
+            const double tx = 1e-12 * (D.prm.x_max - D.prm.x_min);
+            const double tp = 1e-12 * (s.prm.p_max - s.prm.p_min);
+            for (size_t r = 0; r < rec.size(); r += mesh::kInts) {
+                const int32_t* q = &rec[r + mesh::PT];
+                for (int c = 0; c < 3; ++c)
+                    if (std::fabs(xs[q[3 * c]] - xs[q[3 * c + 1]]) > tx ||
+                        std::fabs(xs[q[3 * c]] - xs[q[3 * c + 2]]) > tx ||
+                        std::fabs(ps[q[c]] - ps[q[c + 3]]) > tp ||
+                        std::fabs(ps[q[c]] - ps[q[c + 6]]) > tp)
+                        return fail(FS_E_INVALID,
+                                    "restart: species %s's points are off its panels' lattice "
+                                    "(not written after a remesh)", names[k]);
+            }
+        }
+        s.tree.xs = std::move(xs);
+        s.tree.ps = std::move(ps);
+        s.tree.fs = std::move(fv);
+        s.tree.rec = std::move(rec);
+        std::vector<int32_t> leaves;
+        s.tree.leaf_weights(s.q, D.prm.quadrature == 1, s.base_w, leaves);
+        s.red = mesh::reduced_map(s.tree.xs);
+        s.off = -1;
+    }
+    D.iter = iter;
+    D.t = iter * D.prm.dt;
+    return FS_OK;
+}
+
+int build_deck_restart_host(const fs_deck* deck, const char* sim_dir, const char* const* names,
+                            int iter, Deck** out) {
+    if (!out) return fail(FS_E_INVALID, "restart: null argument");
+    *out = nullptr;
+    Deck* D = nullptr;
+    FS_TRY(build_deck_host(deck, &D));
+    const int rc = load_deck_files(*D, sim_dir, names, iter);
+    if (rc != FS_OK) {
+        free_deck(D);
+        return rc;
+    }
+    *out = D;
+    return FS_OK;
+}
+
 void free_deck(Deck* d) {
     if (!d) return;
     for (auto& s : d->sp) s->release();
@@ -1008,6 +1081,8 @@ int detach_decks(fs_sim* S, const std::vector<int>& idx,
             FS_TRY(download_species(*S, g, st.xs.data(), st.ps.data(), st.fs.data(),
                                     st.qw.data(), st.es.data()));
             FS_TRY(fs_sim_diagnostics(S, g, st.diag));
+            st.rec = S->sp[g]->tree.rec;
+            st.iter = S->decks[d]->iter;
             out[q].push_back(std::move(st));
         }
     }
@@ -1161,6 +1236,7 @@ int fs_sim_destroy(fs_sim* S) {
     S->ffull.release();
     S->fred.release();
     S->tree_work.release();
+    S->diag_work.release();
     delete S;
     return FS_OK;
 }
@@ -1263,47 +1339,74 @@ int fs_sim_get_panels(fs_sim* S, int k, int32_t* panels) {
     return FS_OK;
 }
 
-int fs_sim_diagnostics(fs_sim* S, int k, double out[4]) {
+int fs_batch_create_empty(fs_ctx* ctx, int precision, fs_sim** out) {
+    return sim::create_empty(ctx, precision, out);
+}
+
+int fs_sim_write_output(fs_sim* S, int deck, const char* sim_dir, const char* const* names) {
+    if (!S || deck < 0 || deck >= (int)S->decks.size() || !names)
+        return fail(FS_E_INVALID, "fs_sim_write_output: bad argument");
+    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
+    const sim::Deck& D = *S->decks[deck];
+    int first = 0;
+    for (int e = 0; e < deck; ++e) first += (int)S->decks[e]->sp.size();
+    for (size_t k = 0; k < D.sp.size(); ++k) {
+        if (!names[k]) return fail(FS_E_INVALID, "fs_sim_write_output: null species name");
+        const int g = first + (int)k;
+        const int n = S->sp[g]->n();
+        std::vector<double> xs(n), ps(n), fv(n), qw(n), es(n);
+        FS_TRY(sim::download_species(*S, g, xs.data(), ps.data(), fv.data(), qw.data(),
+                                     es.data()));
+        const auto& rec = S->sp[g]->tree.rec;
+        FS_TRY(io::write_species(sim_dir, names[k], D.iter, n, xs.data(), ps.data(), fv.data(),
+                                 qw.data(), es.data(), (int64_t)(rec.size() / msh::kInts),
+                                 rec.data()));
+    }
+    // the deck's reduced set and its field (the last reduced evaluation)
+    std::vector<double> rx(D.u), re(D.u);
+    if (D.u > 0) {
+        FS_HIP_CHECK(hipMemcpyAsync(rx.data(), S->rx + D.uoff, sizeof(double) * D.u,
+                                    hipMemcpyDeviceToHost, S->stream()));
+        FS_HIP_CHECK(hipMemcpyAsync(re.data(), S->re + D.uoff, sizeof(double) * D.u,
+                                    hipMemcpyDeviceToHost, S->stream()));
+        FS_HIP_CHECK(hipStreamSynchronize(S->stream()));
+    }
+    return io::write_reduced(sim_dir, D.iter, D.u, rx.data(), re.data());
+}
+
+int fs_sim_restart_deck(fs_sim* S, const fs_deck* deck, const char* sim_dir,
+                        const char* const* names, int iter, int* index) {
+    if (!S || !deck || !deck->sim) return fail(FS_E_INVALID, "fs_sim_restart_deck: null argument");
+    if (S->remesh_period != 1 || deck->sim->remesh_period != 1)
+        return fail(FS_E_INVALID, "fs_sim_restart_deck: decks joining a running batch remesh "
+                                  "every step");
+    sim::Deck* D = nullptr;
+    FS_TRY(sim::build_deck_restart_host(deck, sim_dir, names, iter, &D));
+    std::vector<sim::Deck*> one{D};
+    std::vector<int> idx;
+    FS_TRY(sim::attach_decks(S, one, &idx));
+    if (index) *index = idx.empty() ? -1 : idx[0];
+    return FS_OK;
+}
+
+int fs_sim_diagnostics_full(fs_sim* S, int k, double* out, int n_out) {
     if (!S || !out || k < 0 || k >= (int)S->sp.size())
         return fail(FS_E_INVALID, "bad species index");
+    if (n_out < FS_DIAG_COUNT) return fail(FS_E_INVALID, "need %d outputs", FS_DIAG_COUNT);
     const sim::Species& s = *S->sp[k];
-    const int n = s.n();
-    std::vector<double> xs(n), ps(n), qw(n), es(n);
-    FS_TRY(sim::download_species(*S, k, xs.data(), ps.data(), nullptr, qw.data(), es.data()));
-    double charge = 0, mom = 0, kin = 0;
-    for (int i = 0; i < n; ++i) {
-        charge += qw[i];
-        mom += ps[i] * qw[i];
-        kin += ps[i] * ps[i] * qw[i];
-    }
-    out[0] = charge;
-    out[1] = 1.0 / s.q * mom;
-    out[2] = 1.0 / 2.0 / s.qm * kin;
-    std::vector<int> idx(n);
-    for (int i = 0; i < n; ++i) idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return xs[a] < xs[b]; });
-    std::vector<double> ux, ue;
-    const double L = S->decks[s.deck]->Lx();
-    for (int j = 0; j < n; ++j) {
-        const double x = xs[idx[j]];
-        if (j > 0 && x == xs[idx[j - 1]]) continue;
-        if (x >= 0 && x < L) {
-            ux.push_back(x);
-            ue.push_back(es[idx[j]]);
-        }
-    }
-    const size_t m = ux.size();
-    double pot = 0;
-    if (m >= 3) {
-        for (size_t j = 0; j < m; ++j) {
-            const double d = (j == 0) ? .5 * (ux[1] + L - ux[m - 1])
-                             : (j == m - 1) ? .5 * (ux[0] + L - ux[m - 2])
-                                            : .5 * (ux[j + 1] - ux[j - 1]);
-            pot += d * ue[j] * ue[j];
-        }
-        pot *= 0.5;
-    }
-    out[3] = pot;
+    const sim::Deck& D = *S->decks[s.deck];
+    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
+    const int n = s.n(), o = s.off;
+    FS_TRY(S->diag_work.ensure(diag::work_bytes(n), S->stream()));
+    return diag::species(S->xs + o, S->ps + o, S->fs_ + o, S->qw + o, S->es + o, n, s.q, s.qm,
+                         D.Lx(), S->diag_work.ptr, out, S->stream());
+}
+
+int fs_sim_diagnostics(fs_sim* S, int k, double out[4]) {
+    if (!out) return fail(FS_E_INVALID, "null output");
+    double full[FS_DIAG_COUNT];
+    FS_TRY(fs_sim_diagnostics_full(S, k, full, FS_DIAG_COUNT));
+    for (int q = 0; q < 4; ++q) out[q] = full[q];
     return FS_OK;
 }
 

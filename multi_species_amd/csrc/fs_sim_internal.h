@@ -19,6 +19,12 @@ int64_t deck_points(const Deck* d);
 // Estimated device bytes the deck holds while resident in a batch.
 size_t deck_footprint(const Deck* d, int precision);
 
+// build_deck_host, then each species' points and tree replaced by the files
+// <sim_dir>/simulation_output/<names[k]>/... of step `iter` (fs_io.cpp): the
+// deck resumes from a state written after a remesh. Host only.
+int build_deck_restart_host(const fs_deck* deck, const char* sim_dir,
+                            const char* const* names, int iter, Deck** out);
+
 // An empty batch (remesh every step) that decks join and leave.
 int create_empty(fs_ctx* ctx, int precision, fs_sim** out);
 // Adds the decks (ownership passes to the batch) with one relayout and
@@ -27,6 +33,8 @@ int attach_decks(fs_sim* S, std::vector<Deck*>& decks, std::vector<int>* indices
 
 struct SpeciesState {
     std::vector<double> xs, ps, fs, qw, es;
+    std::vector<int32_t> rec;  // panel records of the final mesh
+    int iter = 0;              // steps the deck had taken
     double diag[4];
 };
 // Copies the final state of the given decks (every species) to the host and

@@ -50,6 +50,28 @@ def contribution_scale(xs_t, xs, ws, L, eps, m=128):
     return best
 
 
+def check_full_diagnostics(g, o, g4, ofs, oqw, sp, tol, tag):
+    """The device's full diagnostic set against the oracle's: the first four
+    equal to fs_sim_diagnostics; entropy, L1, L2 and the negative parts within
+    tol of the sum of their terms' magnitudes; min/max f within 1e-12 of
+    max|f|; the count of f < 0 within max(2, 1e-3 n) (a value at rounding
+    level can change sign); the number of points exact."""
+    assert np.array_equal(g[:4], g4), tag
+    q = abs(sp.q)
+    ent = np.where(ofs > 0, ofs * np.log10(np.where(ofs > 0, ofs, 1.0)), 0.0)
+    scale = {4: np.sum(np.abs(ent * oqw)) / q, 5: np.sum(np.abs(oqw)) / q,
+             6: np.sum(np.abs(oqw * ofs)) / q, 9: np.sum(np.abs(oqw)) / q,
+             12: 1.0}
+    for i, sc in scale.items():
+        assert abs(g[i] - o[i]) <= tol * max(sc, 1e-300), (tag, i, g[i], o[i])
+    fmax = np.max(np.abs(ofs))
+    assert abs(g[7] - o[7]) <= 1e-12 * fmax and abs(g[8] - o[8]) <= 1e-12 * fmax, tag
+    n = ofs.size
+    assert abs(g[10] - o[10]) <= max(2, 1e-3 * n), (tag, g[10], o[10])
+    assert abs(g[11] - o[11]) <= max(2, 1e-3 * n) / n, tag
+    assert g[13] == o[13] == n, tag
+
+
 def compare(gsim, osim, nsp, step, tol_e=1e-12, tol_f=1e-12, tol_d=1e-10, expect_grid=True,
             e_scale=0.0):
     """e_scale > 0: E may also differ by 1e-14 e_scale (the contribution scale
@@ -83,6 +105,8 @@ def compare(gsim, osim, nsp, step, tol_e=1e-12, tol_f=1e-12, tol_d=1e-10, expect
                           np.sum(ops * ops * np.abs(oqw)) / (2 * abs(qm)), abs(od[3])])
         dd = np.abs(gd - od) / np.maximum(scale, 1e-300)
         assert np.all(dd <= tol_d), f"step {step} species {k}: diagnostics {dd}"
+        check_full_diagnostics(gsim.diagnostics_full(k), osim.diagnostics_full(k), gd, ofs, oqw,
+                               sp, tol_d, f"step {step} species {k}")
         worst = max(worst, ee, ef)
     return worst
 
