@@ -41,6 +41,7 @@ enum : int {
     kWalkBoundaryLeft = 8,
     kWalkLeftBdry = 16,
     kWalkRightBdry = 32,
+    kWalkRefined = 64,  // the panel is refined (the root descent goes on)
 };
 struct WalkRec {
     double xbl, pbl, xtl, ptl, xmid, xbr, pbr, xtr, ptr;
@@ -50,13 +51,20 @@ struct WalkRec {
 };
 
 // One root-descent step's inputs: all nine advected points, the children
-// and the children's neighbours the descent may step to.
+// and the children's neighbours the descent may step to, and which of those
+// are refined themselves. Written for refined panels only: a descent that
+// reaches a leaf tests the leaf's WalkRec (the same corners and boundary
+// flags), so leaves need no descent record.
 enum : int {
     kDescRefinedXP = 1,
     kDescRefinedP = 2,
     kDescLeftBdry = 4,
     kDescRightBdry = 8,
     kDescNbrBoundary = 16,  // << d, d = 0 right, 1 top, 2 bottom, 3 left
+    // << t: step target t is refined; t = 0..3 the children in the order
+    // the descent names them (cs, cs+1, and cs+2, cs+3 or for a p-only split
+    // cs, cs+1), t = 4..11 the panels cn[0..7]
+    kDescTargetRefined = 256,
 };
 struct DescRec {
     double x[9], p[9];

@@ -11,9 +11,11 @@
 //            coefficients and the panel centre.
 //   records  one thread per old panel: packs what one leaf-search step reads
 //            (advected corners, the neighbour to step to in each direction,
-//            boundary flags) into one 96-byte record, and the root-descent
-//            record (all 9 points and the children's neighbours) into 192
-//            bytes, so every search step is a single dependent load.
+//            boundary flags) into one 96-byte record, and for refined panels
+//            the root-descent record (all 9 points, the children's
+//            neighbours, which step targets are refined) into 192 bytes, so
+//            every search step is a single dependent load (a descent ends on
+//            the leaf's walk record).
 //   shift    one thread per new point: the periodic pre-shift of
 //            AMRStructure_interp.cpp:429-491.
 //   rowsort  one workgroup per grid row: rank sort of the row by shifted x.
@@ -21,8 +23,9 @@
 //   rows     one workgroup per row: the walk along the row (:696-722).
 //            The reference walks each line sequentially, every point
 //            starting from its predecessor's leaf. Here every point first
-//            gets a guess (root descent, then a walk), then walks again from
-//            its predecessor's guess, all in parallel; where every guess up
+//            gets a guess (in a row every 4th a root descent and a walk, the
+//            others a walk from that one's guess), then walks again from its
+//            predecessor's guess, all in parallel; where every guess up
 //            to a point equals the leaf the walk from the predecessor's guess
 //            returned, that walk started from the true predecessor leaf and
 //            is exactly the reference's. From the first mismatch on the line
@@ -45,8 +48,11 @@ namespace remesh {
 constexpr int kLineThreads = 256;
 constexpr int kLinePerThread = kMaxLine / kLineThreads;
 constexpr int kMaxWalkSteps = 4096;  // a walk's steps (the reference's has no bound)
-constexpr int kMaxHistory = 32;      // visited panels kept in registers
+constexpr int kMaxHistory = 16;      // visited panels kept in registers
 constexpr int kMaxDescend = 256;
+// line elements per root descent in the guess phase of a row (the first
+// column, one workgroup for the whole line, descends for every element)
+constexpr int kRowAnchor = 4;
 
 __global__ __launch_bounds__(64) void coef_kernel(const Job* __restrict__ jobs) {
     const Job& J = jobs[blockIdx.y];
@@ -194,10 +200,16 @@ __global__ __launch_bounds__(256) void records_kernel(const Job* __restrict__ jo
     }
     if (flags & kFlagLeftBdry) wf |= kWalkLeftBdry;
     if (flags & kFlagRightBdry) wf |= kWalkRightBdry;
+    const bool refined = (flags & (kFlagRefinedXP | kFlagRefinedP)) != 0;
+    if (refined) wf |= kWalkRefined;
     w.flags = wf;
     w.pad[0] = w.pad[1] = w.pad[2] = 0;
     walk[pi] = w;
+    if (!refined) return;  // a leaf: the descent tests its WalkRec
 
+    auto is_refined = [&](int q) {
+        return q >= 0 && (rec[(size_t)q * kPanelInts + 17] & (kFlagRefinedXP | kFlagRefinedP)) != 0;
+    };
     DescRec dr;
     for (int i = 0; i < 9; ++i) {
         dr.x[i] = ox[r[1 + i]];
@@ -228,6 +240,12 @@ __global__ __launch_bounds__(256) void records_kernel(const Job* __restrict__ jo
         dr.cn[5] = c0[13];  // child 0 left
         dr.cn[6] = c2[16];  // child 2 bottom
         dr.cn[7] = c2[15];  // child 2 right
+        const int tgt[4] = {cs, cs + 1, ponly ? cs : cs + 2, ponly ? cs + 1 : cs + 3};
+        for (int t = 0; t < 4; ++t)
+            if (is_refined(tgt[t])) df |= kDescTargetRefined << t;
+        for (int i = 0; i < 8; ++i)
+            if (is_refined(dr.cn[i])) df |= kDescTargetRefined << (4 + i);
+        dr.flags = df;
     }
     desc[pi] = dr;
 }
@@ -317,29 +335,33 @@ __device__ int walk_one(const WalkRec* __restrict__ W, double& tx, double tp, bo
 }
 
 // find_leaf_containing_xp_recursively (AMRStructure_interp.cpp:232-426).
-__device__ int descend_one(const DescRec* __restrict__ D, double& x, double p, bool& beyond,
-                           int pi, double Lx, bool periodic, int& err) {
+// Refined panels step through their DescRec; the leaf reached is tested with
+// its WalkRec (whose corners and boundary bits are the DescRec's).
+__device__ int descend_one(const DescRec* __restrict__ D, const WalkRec* __restrict__ W,
+                           double& x, double p, bool& beyond, int pi, double Lx, bool periodic,
+                           int& err) {
+    bool refined = pi >= 0 && (W[pi].flags & kWalkRefined) != 0;
     for (int it = 0; it < kMaxDescend; ++it) {
         if (pi < 0) {
             err = 1;
             return 0;
         }
+        if (!refined) {
+            const WalkRec& w = W[pi];
+            const int f = w.flags;
+            const bool r = (w.xtr - w.xbr) * (p - w.pbr) > (w.ptr - w.pbr) * (x - (w.xbr));
+            const bool l = (w.xtl - w.xbl) * (p - w.pbl) <= (w.ptl - w.pbl) * (x - w.xbl);
+            const bool t = (w.xtr - w.xtl) * (p - w.ptl) < (w.ptr - w.ptl) * (x - w.xtl);
+            const bool b = (w.xbr - w.xbl) * (p - w.pbl) >= (w.pbr - w.pbl) * (x - w.xbl);
+            if ((!l && (f & kWalkBoundaryLeft)) || (!r && (f & kWalkBoundaryRight)) ||
+                (!t && (f & kWalkBoundaryTop)) || (!b && (f & kWalkBoundaryBottom)))
+                beyond = true;
+            return pi;
+        }
         const DescRec& d = D[pi];
         const int f = d.flags;
         const double* X = d.x;
         const double* P = d.p;
-        if (!(f & (kDescRefinedXP | kDescRefinedP))) {
-            const double x_bl = X[0], p_bl = P[0], x_tl = X[2], p_tl = P[2];
-            const double x_br = X[6], p_br = P[6], x_tr = X[8], p_tr = P[8];
-            const bool r = (x_tr - x_br) * (p - p_br) > (p_tr - p_br) * (x - (x_br));
-            const bool l = (x_tl - x_bl) * (p - p_bl) <= (p_tl - p_bl) * (x - x_bl);
-            const bool t = (x_tr - x_tl) * (p - p_tl) < (p_tr - p_tl) * (x - x_tl);
-            const bool b = (x_br - x_bl) * (p - p_bl) >= (p_br - p_bl) * (x - x_bl);
-            if ((!l && (f & (kDescNbrBoundary << 3))) || (!r && (f & (kDescNbrBoundary << 0))) ||
-                (!t && (f & (kDescNbrBoundary << 1))) || (!b && (f & (kDescNbrBoundary << 2))))
-                beyond = true;
-            return pi;
-        }
         const int cs = d.child_start;
         const bool ponly = (f & kDescRefinedP) != 0;
         const double x_bl = X[0], p_bl = P[0], x_ml = X[1], p_ml = P[1];
@@ -350,19 +372,22 @@ __device__ int descend_one(const DescRec* __restrict__ D, double& x, double p, b
         const bool b1 = (x_mm - x_ml) * (p - p_ml) >= (p_mm - p_ml) * (x - x_ml);
         const bool r1 = (x_tm - x_mm) * (p - p_mm) >= (p_tm - p_mm) * (x - x_mm);
         const bool b3 = (x_mr - x_mm) * (p - p_mm) >= (p_mr - p_mm) * (x - x_mm);
-        int sub;
+        int sub, tgt;
         if (b1 && r1) {
             const bool t1 = (x_tm - x_tl) * (p - p_tl) <= (p_tm - p_tl) * (x - x_tl);
             if (t1 || d.cn[0] < 0) {
                 const bool l1 = (x_tl - x_ml) * (p - p_ml) <= (p_tl - p_ml) * (x - x_ml);
                 if (l1 || d.cn[1] < 0) {
                     sub = cs + 1;
+                    tgt = 1;
                 } else {
                     sub = d.cn[1];
+                    tgt = 4 + 1;
                     if ((f & kDescLeftBdry) && periodic) x += Lx;
                 }
             } else {
                 sub = d.cn[0];
+                tgt = 4 + 0;
             }
         } else if (b3 && !r1) {
             const bool t3 = (x_tr - x_tm) * (p - p_tm) <= (p_tr - p_tm) * (x - x_tm);
@@ -370,12 +395,15 @@ __device__ int descend_one(const DescRec* __restrict__ D, double& x, double p, b
                 const bool r3 = (x_tr - x_mr) * (p - p_mr) <= (p_tr - p_mr) * (x - x_mr);
                 if (!r3 || d.cn[3] < 0) {
                     sub = ponly ? cs + 1 : cs + 3;
+                    tgt = 3;
                 } else {
                     sub = d.cn[3];
+                    tgt = 4 + 3;
                     if ((f & kDescRightBdry) && periodic) x -= Lx;
                 }
             } else {
                 sub = d.cn[2];
+                tgt = 4 + 2;
             }
         } else {
             const bool r0 = (x_mm - x_bm) * (p - p_bm) >= (p_mm - p_bm) * (x - x_bm);
@@ -385,12 +413,15 @@ __device__ int descend_one(const DescRec* __restrict__ D, double& x, double p, b
                     const bool l0 = (x_ml - x_bl) * (p - p_bl) <= (p_ml - p_bl) * (x - x_bl);
                     if (l0 || d.cn[5] < 0) {
                         sub = cs;
+                        tgt = 0;
                     } else {
                         sub = d.cn[5];
+                        tgt = 4 + 5;
                         if ((f & kDescLeftBdry) && periodic) x += Lx;
                     }
                 } else {
                     sub = d.cn[4];
+                    tgt = 4 + 4;
                 }
             } else {
                 const bool b2 = (x_br - x_bm) * (p - p_bm) >= (p_br - p_bm) * (x - x_bm);
@@ -398,15 +429,19 @@ __device__ int descend_one(const DescRec* __restrict__ D, double& x, double p, b
                     const bool r2 = (x_mr - x_br) * (p - p_br) <= (p_mr - p_br) * (x - x_br);
                     if (!r2 || d.cn[7] < 0) {
                         sub = ponly ? cs : cs + 2;
+                        tgt = 2;
                     } else {
                         sub = d.cn[7];
+                        tgt = 4 + 7;
                         if ((f & kDescRightBdry) && periodic) x -= Lx;
                     }
                 } else {
                     sub = d.cn[6];
+                    tgt = 4 + 6;
                 }
             }
         }
+        refined = (f & (kDescTargetRefined << tgt)) != 0;
         pi = sub;
     }
     err = 1;
@@ -482,18 +517,21 @@ struct LineOut {
 };
 
 __device__ void walk_line(const Job& J, size_t base, size_t stride, int len, int first_start,
-                          bool element0_exact_from_start, int* s_guess, int* s_leaf, int* s_first,
-                          int* s_fail, int* ext, LineOut& out, int& err) {
+                          bool element0_exact_from_start, int anchor, int* s_guess, int* s_leaf,
+                          int* s_first, int* s_fail, int* ext, LineOut& out, int& err) {
     const bool periodic = J.periodic != 0;
     const int tid = threadIdx.x;
-    // A: a guess per element from a root descent and a walk (a failed guess
-    // only costs the sequential fallback, so its error flag is dropped);
-    // element 0 of the first column walks exactly from its given start
+    // A: a guess per element (a failed or wrong guess only costs the
+    // sequential fallback of phase C, so its error flag is dropped). Anchors,
+    // every anchor-th element, descend from the root and walk; the elements
+    // between walk from their anchor's guess (a few steps along the line
+    // instead of a descent through every level). Element 0 of the first
+    // column walks exactly from its given start.
     int guess_err = 0;
 #pragma unroll
     for (int q = 0; q < kLinePerThread; ++q) {
         const int k = tid + q * kLineThreads;
-        if (k >= len) continue;
+        if (k >= len || k % anchor != 0) continue;
         double x = J.sorted_x[base + (size_t)k * stride];
         const double p = J.sorted_p[base + (size_t)k * stride];
         bool bb = false;
@@ -501,9 +539,20 @@ __device__ void walk_line(const Job& J, size_t base, size_t stride, int len, int
             s_guess[0] = walk_one(J.walk, x, p, bb, first_start, J.Lx, periodic, err, ext);
             continue;
         }
-        const int d = descend_one(J.desc, x, p, bb, 0, J.Lx, periodic, guess_err);
+        const int d = descend_one(J.desc, J.walk, x, p, bb, 0, J.Lx, periodic, guess_err);
         bb = false;
         s_guess[k] = walk_one(J.walk, x, p, bb, d, J.Lx, periodic, guess_err, nullptr);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kLinePerThread; ++q) {
+        const int k = tid + q * kLineThreads;
+        if (k >= len || k % anchor == 0) continue;
+        double x = J.sorted_x[base + (size_t)k * stride];
+        const double p = J.sorted_p[base + (size_t)k * stride];
+        bool bb = false;
+        s_guess[k] = walk_one(J.walk, x, p, bb, s_guess[k - k % anchor], J.Lx, periodic,
+                              guess_err, nullptr);
     }
     __syncthreads();
     // B: walk every element from its predecessor's guess (element 0: from the
@@ -558,13 +607,13 @@ __global__ __launch_bounds__(kLineThreads) void column_kernel(const Job* __restr
     if (threadIdx.x == 0) {
         double x0 = J.sorted_x[0];
         bool bb = false;
-        s_start = descend_one(J.desc, x0, J.sorted_p[0], bb, 0, J.Lx, periodic, err);
+        s_start = descend_one(J.desc, J.walk, x0, J.sorted_p[0], bb, 0, J.Lx, periodic, err);
         J.sorted_x[0] = x0;
     }
     __syncthreads();
     LineOut out;
-    walk_line(J, 0, (size_t)nx, np, s_start, true, s_guess, s_leaf, &s_first, &s_fail, ext, out,
-              err);
+    walk_line(J, 0, (size_t)nx, np, s_start, true, 1, s_guess, s_leaf, &s_first, &s_fail, ext,
+              out, err);
     const int m = s_first;  // rows 0..m exact
 #pragma unroll
     for (int q = 0; q < kLinePerThread; ++q) {
@@ -604,7 +653,8 @@ __global__ __launch_bounds__(kLineThreads) void rows_kernel(const Job* __restric
     const size_t base = (size_t)r * nx + 1;
     LineOut out;
     const int start = J.first_leaf[r];
-    walk_line(J, base, 1, nx - 1, start, false, s_guess, s_leaf, &s_first, &s_fail, ext, out, err);
+    walk_line(J, base, 1, nx - 1, start, false, kRowAnchor, s_guess, s_leaf, &s_first, &s_fail,
+              ext, out, err);
     const int m = s_first;
 #pragma unroll
     for (int q = 0; q < kLinePerThread; ++q) {
@@ -665,7 +715,7 @@ __global__ __launch_bounds__(256) void points_kernel(const Job* __restrict__ job
     }
     bool beyond = false;
     int err = 0;
-    int leaf = descend_one(J.desc, xt, pt, beyond, 0, J.Lx, J.periodic != 0, err);
+    int leaf = descend_one(J.desc, J.walk, xt, pt, beyond, 0, J.Lx, J.periodic != 0, err);
     if (err) atomicOr(J.status, kStatusSearchFailed);
     if (beyond) leaf = 0;
     double v;
