@@ -75,12 +75,31 @@ int fs_emq_periodic_f64_noalloc(fs_ctx* ctx, double* d_es, const double* d_targe
  * the periodic barycentric-Lagrange treecode (bisection tree with leaves of
  * fewer than max_source points, MAC theta = mac with the periodic distance,
  * Chebyshev degree `degree` <= 15), evaluated at the sources like the
- * reference (es[0:ns]). Tree built on the host, moments and far/near sums on
- * the device. evals (may be NULL) receives {far, near} kernel-evaluation
- * counts. Host pointers, synchronous. */
+ * reference (es[0:ns]). Sort, tree, interaction lists, moments and far/near
+ * sums all on the device. evals (may be NULL) receives {far, near}
+ * kernel-evaluation counts. Host pointers, synchronous. FS_E_RANGE if more
+ * than max_source points share one coordinate (the tree cannot separate
+ * them). */
 int fs_emq_treecode_f64(fs_ctx* ctx, double* es, const double* sources, const double* q_ws,
                         int64_t ns, double L, double epsilon, double mac, int degree,
                         int max_source, int64_t* evals);
+
+/* The tree and interaction lists fs_emq_treecode_f64 builds on the device
+ * for the points `sources` (E_MQ_Treecode's build_tree_init,
+ * build_tree_1D_Recursive and split_2, src/FieldStructure.cpp:506-632, and
+ * build_interaction_list, :663-702), copied out for inspection. stats:
+ * [0] nodes, [1] leaves, [2] far entries, [3] near entries. A node is given
+ * as three int32 (first, last, depth): its member range in tree order and
+ * its depth below the root (0). perm[ns]: the input index of each point in
+ * tree order (leaves in increasing x, each leaf's points in input order);
+ * leaf_nodes[3 * leaves] in increasing x; far_off and near_off[leaves + 1]
+ * are offsets into far_nodes[3 * far entries] and near_nodes[3 * near
+ * entries], each leaf's entries in the reference's list order. Any array may
+ * be NULL (stats alone sizes the others). Host pointers, synchronous. */
+int fs_treecode_tree(fs_ctx* ctx, const double* sources, int64_t ns, double L, double mac,
+                     int degree, int max_source, int64_t stats[4], int32_t* perm,
+                     int32_t* leaf_nodes, int32_t* far_off, int32_t* far_nodes,
+                     int32_t* near_off, int32_t* near_nodes);
 
 /* Replaces E_MQ_DirectSum_openbcs::operator() (src/FieldStructure.cpp:166-191):
  *   es[i] = sum_j q_ws[j] * 0.5 * z / sqrt(z^2 + eps^2) - targets[i],

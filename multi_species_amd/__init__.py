@@ -70,6 +70,8 @@ def lib():
         L.fs_emq_treecode_f64.argtypes = [_vp, _dp, _dp, _dp, _i64, _d, _d, _d, _int, _int,
                                           ctypes.POINTER(_i64)]
         _ip = ctypes.POINTER(ctypes.c_int32)
+        L.fs_treecode_tree.argtypes = [_vp, _dp, _i64, _d, _d, _int, _int, ctypes.POINTER(_i64),
+                                       _ip, _ip, _ip, _ip, _ip, _ip]
         L.fs_remesh_interp_grid.argtypes = [_vp, _ip, _i64, _dp, _dp, _dp, _i64, _dp, _dp, _int,
                                             _int, _d, _d, _int, _d, _dp]
         L.fs_remesh_interp_points.argtypes = [_vp, _ip, _i64, _dp, _dp, _dp, _i64, _dp, _dp,
@@ -271,6 +273,28 @@ class Context:
         _check(lib().fs_emq_treecode_f64(self.h, _P(es), _P(s), _P(w), s.size, L, eps, mac,
                                          degree, max_source, ev), "fs_emq_treecode_f64")
         return (es, (ev[0], ev[1])) if return_evals else es
+
+    def treecode_tree(self, sources, L: float, mac: float, degree: int, max_source: int):
+        """The tree and interaction lists emq_treecode_f64 builds on the device
+        (fs_treecode_tree): stats (nodes, leaves, far entries, near entries),
+        perm, leaf nodes and the far/near lists (CSR), a node as
+        (first, last, depth) in tree order."""
+        s = _f64(sources)
+        st = (_i64 * 4)()
+        _check(lib().fs_treecode_tree(self.h, _P(s), s.size, L, mac, degree, max_source, st,
+                                      None, None, None, None, None, None), "fs_treecode_tree")
+        nn, nl, nf, nr = (int(v) for v in st)
+        out = dict(stats=(nn, nl, nf, nr), perm=np.empty(s.size, np.int32),
+                   leaves=np.empty((nl, 3), np.int32), far_off=np.empty(nl + 1, np.int32),
+                   far=np.empty((max(nf, 1), 3), np.int32), near_off=np.empty(nl + 1, np.int32),
+                   near=np.empty((max(nr, 1), 3), np.int32))
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        _check(lib().fs_treecode_tree(self.h, _P(s), s.size, L, mac, degree, max_source, st,
+                                      ip(out["perm"]), ip(out["leaves"]), ip(out["far_off"]),
+                                      ip(out["far"]), ip(out["near_off"]), ip(out["near"])),
+               "fs_treecode_tree")
+        out["far"], out["near"] = out["far"][:nf], out["near"][:nr]
+        return out
 
     def emq_periodic_f64_dev(self, es_ptr: int, t_ptr: int, nt: int, s_ptr: int, w_ptr: int,
                              ns: int, L: float, eps: float, stream: Optional[int] = None):

@@ -81,6 +81,7 @@ int fs_ctx_destroy(fs_ctx* ctx) {
     ctx->scratch.release();
     ctx->io.release();
     ctx->aux.release();
+    ctx->tree.release();
     if (ctx->api_batch) {
         ctx->api_batch->release();
         delete ctx->api_batch;

@@ -223,5 +223,9 @@ int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, 
 // Stable sort by all 64 key bits (one segment).
 int radix_sort_full(const uint64_t* kin, int n, uint64_t* kout, int32_t* iout, void* tmp,
                     hipStream_t stream);
+// Stable sort by the top key_bytes bytes of the keys (one segment); iout is
+// the permutation (original index of each sorted key).
+int radix_sort_top(const uint64_t* kin, int n, int key_bytes, uint64_t* kout, int32_t* iout,
+                   void* tmp, hipStream_t stream);
 
 }  // namespace fs

@@ -103,10 +103,24 @@ constexpr size_t kNoAllocBytes = size_t(256) << 20;
 int noalloc_init(int device);
 void* noalloc_ptr(int device);
 
-// Treecode on device arrays (the points are sorted on the device and the
-// tree built on the host). Synchronizes `stream` before returning.
+// Work space of the device treecode: the build (sized by the point count
+// and the node capacity) and the interaction lists with the evaluation's
+// arrays (sized after the build). node_cap persists between calls and grows
+// when a build needs more nodes.
+struct TreeWork {
+    DevBuf base, lists;
+    int node_cap = 0;
+    void release() {
+        base.release();
+        lists.release();
+    }
+};
+
+// Treecode on device arrays: sort, tree, lists and sums on the device, one
+// host synchronization (the lists' sizes). evals (optional): far and near
+// kernel evaluations.
 int treecode_f64(const double* d_x, const double* d_w, int n, double L, double eps, double mac,
-                 int degree, int max_source, double* d_es, DevBuf& work, hipStream_t stream,
+                 int degree, int max_source, double* d_es, TreeWork& work, hipStream_t stream,
                  int64_t* evals);
 
 }  // namespace fs
@@ -118,6 +132,7 @@ struct fs_ctx {
     fs::DevBuf scratch;   // kernel work space
     fs::DevBuf io;        // staging for the host-pointer entry points
     fs::DevBuf aux;       // concatenated points of the mixed entry points
+    fs::TreeWork tree;    // the treecode entry points' work space
     fs::FieldTuning tuning;
     fs::FieldBatch* api_batch = nullptr;  // plan of the last mixed entry-point call
     double mixed_target = 0;              // > 0: mixed entry points hold this error bound

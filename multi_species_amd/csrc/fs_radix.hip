@@ -246,4 +246,10 @@ int radix_sort_full(const uint64_t* kin, int n, uint64_t* kout, int32_t* iout, v
     return radix_sort_bits(kin, n, nullptr, 1, 8, kout, iout, tmp, stream);
 }
 
+int radix_sort_top(const uint64_t* kin, int n, int key_bytes, uint64_t* kout, int32_t* iout,
+                   void* tmp, hipStream_t stream) {
+    if (key_bytes < 1 || key_bytes > 8) return fail(FS_E_INVALID, "radix sort: 1..8 key bytes");
+    return radix_sort_bits(kin, n, nullptr, 1, key_bytes, kout, iout, tmp, stream);
+}
+
 }  // namespace fs

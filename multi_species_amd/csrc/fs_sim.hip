@@ -279,7 +279,7 @@ struct fs_sim {
     bool any_adaptive = false, any_treecode = false;
     // concatenated state: xs, ps, qw, es, fs; RK4 work: xk, p2, p3, p4, e2, e3, e4
     DevBuf state, rk, red, status, jobs, pinfo;
-    DevBuf tree_work;  // treecode work space (grows to the largest treecode deck)
+    fs::TreeWork tree_work;  // treecode work space (grows to the largest treecode deck)
     DevBuf diag_work;  // diagnostics work space
     fs::FieldBatch ffull, fred;
     rm::Job* d_jobs = nullptr;            // grid jobs, one per species
@@ -310,7 +310,7 @@ void drop_graphs(Sim& S) {
     S.g_remesh = S.g_plain = nullptr;
 }
 
-// E_MQ_Treecode on one deck's slice (points sorted on the device, tree on the host).
+// E_MQ_Treecode on one deck's slice (tree, lists and sums on the device).
 int treecode_deck(Sim& S, const Deck& D, double* out, const double* x, const double* w, int n) {
     return treecode_f64(x, w, n, D.Lx(), D.prm.greens_epsilon, D.prm.mac, D.prm.degree,
                         D.prm.max_source, out, S.tree_work, S.stream(), nullptr);

@@ -1,8 +1,8 @@
 // Periodic barycentric-Lagrange treecode on the device (gfx950).
 //
 // Replaces E_MQ_Treecode::operator() (reference src/FieldStructure.cpp:
-// 230-1050). The tree, the MAC lists and the Chebyshev points come from the
-// host builder (fs_tree.cpp); the device computes
+// 230-1050). The tree, the MAC lists, the tree order and the Chebyshev points
+// are built on the device (fs_tree_build.hip); then
 //   moments  one workgroup per node: M_k = sum_i lambda_i l_k(x_i) with the
 //            barycentric weights (-1)^k delta_k and the exact-hit rule
 //            (:775-866); per-thread partials reduced in thread order.
@@ -154,134 +154,40 @@ __global__ __launch_bounds__(256) void gather_kernel(const double* __restrict__ 
     dst[k] = do_fmod ? fmod(v, L) : v;
 }
 
-// Order-preserving 64-bit image of fmod(x, L) (-0 taken as +0): the bit
-// pattern with the sign bit set for positive values, inverted for negative.
-__global__ __launch_bounds__(256) void tree_key_kernel(const double* __restrict__ x, int n,
-                                                       double L, uint64_t* __restrict__ key) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    double v = fmod(x[k], L);
-    if (v == 0.0) v = 0.0;
-    const uint64_t b = (uint64_t)__double_as_longlong(v);
-    key[k] = (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
-double key_value(uint64_t key) {
-    uint64_t b = (key >> 63) ? (key ^ 0x8000000000000000ull) : ~key;
-    double v;
-    std::memcpy(&v, &b, 8);
-    return v;
-}
-
 }  // namespace
 
-// Device treecode: d_x, d_w (input order) -> d_es (input order). The points
-// are sorted by fmod(x, L) on the device (a stable sort, so equal
-// coordinates keep input order, as the reference's stable partitions do);
-// the tree is built on the host from the sorted order; everything else runs
-// on `stream`. evals (optional) receives the far and near kernel-evaluation
-// counts.
+// Device treecode: d_x, d_w (input order) -> d_es (input order). The tree,
+// its lists and the tree order are built on the device (fs_tree_build.hip;
+// one host synchronization for the lists' sizes), then the moments, the far
+// and near sums and the scatter back to input order run on `stream`. evals
+// (optional) receives the far and near kernel-evaluation counts.
 int treecode_f64(const double* d_x, const double* d_w, int n, double L, double eps, double mac,
-                 int degree, int max_source, double* d_es, DevBuf& work, hipStream_t stream,
+                 int degree, int max_source, double* d_es, TreeWork& work, hipStream_t stream,
                  int64_t* evals) {
     if (n <= 0) return FS_OK;
     if (degree < 1 || degree + 1 > kMaxPP)
         return fail(FS_E_INVALID, "treecode: degree must be in [1, %d]", kMaxPP - 1);
     if (max_source < 1) return fail(FS_E_INVALID, "treecode: max_source must be >= 1");
     if (!(L > 0)) return fail(FS_E_INVALID, "treecode: L must be > 0");
-    // the points in increasing fmod(x, L) (FieldStructure.cpp:240-247), stable
-    std::vector<int32_t> order(n);
-    std::vector<double> sx(n);
-    {
-        const size_t b_k = align_up(sizeof(uint64_t) * n, 256), b_i = align_up(sizeof(int32_t) * n, 256);
-        FS_TRY(work.ensure(2 * b_k + b_i + radix_sort_bytes(n), stream));
-        char* p = work.as<char>();
-        uint64_t* key = reinterpret_cast<uint64_t*>(p);
-        uint64_t* skey = reinterpret_cast<uint64_t*>(p + b_k);
-        int32_t* ord = reinterpret_cast<int32_t*>(p + 2 * b_k);
-        tree_key_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(d_x, n, L, key);
-        FS_HIP_CHECK(hipGetLastError());
-        FS_TRY(radix_sort_full(key, n, skey, ord, p + 2 * b_k + b_i, stream));
-        std::vector<uint64_t> hk(n);
-        FS_HIP_CHECK(hipMemcpyAsync(hk.data(), skey, sizeof(uint64_t) * n, hipMemcpyDeviceToHost,
-                                    stream));
-        FS_HIP_CHECK(hipMemcpyAsync(order.data(), ord, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
-                                    stream));
-        FS_HIP_CHECK(hipStreamSynchronize(stream));
-        for (int k = 0; k < n; ++k) sx[k] = key_value(hk[k]);
-    }
-    tree::Tree t;
-    if (!tree::build_sorted(order, sx, L, mac, degree, max_source, t))
-        return fail(FS_E_RANGE, "treecode: more than max_source coincident points");
+    tree::Built T;
+    FS_TRY(tree::build_device(d_x, n, L, mac, degree, max_source, work, T, stream));
     const int PP = degree + 1;
-    const int nn = (int)t.nodes.size();
-    const int nleaf = (int)t.leaves.size();
-    std::vector<int2> range(nn);
-    for (int k = 0; k < nn; ++k) range[k] = make_int2(t.nodes[k].m0, t.nodes[k].m1);
-    std::vector<int4> blocks;
-    for (int li = 0; li < nleaf; ++li) {
-        const tree::Node& lf = t.nodes[t.leaves[li]];
-        for (int s = lf.m0; s <= lf.m1; s += 64)
-            blocks.push_back(make_int4(li, s, std::min(lf.m1, s + 63), 0));
-    }
-    // device layout
-    auto al = [](size_t b) { return align_up(b, 256); };
-    const size_t b_x = al(sizeof(double) * n), b_range = al(sizeof(int2) * nn),
-                 b_blocks = al(sizeof(int4) * blocks.size()),
-                 b_foff = al(sizeof(int32_t) * (nleaf + 1)),
-                 b_flist = al(sizeof(int32_t) * std::max<size_t>(1, t.far_list.size())),
-                 b_noff = al(sizeof(int32_t) * (nleaf + 1)),
-                 b_nlist = al(sizeof(int32_t) * std::max<size_t>(1, t.near_list.size())),
-                 b_cheb = al(sizeof(double) * t.cheb.size()), b_perm = al(sizeof(int32_t) * n);
-    const size_t total = 3 * b_x + b_range + b_blocks + b_foff + b_flist + b_noff + b_nlist +
-                         2 * b_cheb + b_perm;
-    FS_TRY(work.ensure(total, stream));
-    char* p = work.as<char>();
-    double* px_t = reinterpret_cast<double*>(p); p += b_x;
-    double* lam_t = reinterpret_cast<double*>(p); p += b_x;
-    double* e_t = reinterpret_cast<double*>(p); p += b_x;
-    int2* d_range = reinterpret_cast<int2*>(p); p += b_range;
-    int4* d_blocks = reinterpret_cast<int4*>(p); p += b_blocks;
-    int32_t* d_foff = reinterpret_cast<int32_t*>(p); p += b_foff;
-    int32_t* d_flist = reinterpret_cast<int32_t*>(p); p += b_flist;
-    int32_t* d_noff = reinterpret_cast<int32_t*>(p); p += b_noff;
-    int32_t* d_nlist = reinterpret_cast<int32_t*>(p); p += b_nlist;
-    double* d_cheb = reinterpret_cast<double*>(p); p += b_cheb;
-    double* d_mom = reinterpret_cast<double*>(p); p += b_cheb;
-    int32_t* d_perm = reinterpret_cast<int32_t*>(p);
-    FS_HIP_CHECK(hipMemcpyAsync(d_range, range.data(), sizeof(int2) * nn, hipMemcpyHostToDevice, stream));
-    FS_HIP_CHECK(hipMemcpyAsync(d_blocks, blocks.data(), sizeof(int4) * blocks.size(),
-                                hipMemcpyHostToDevice, stream));
-    FS_HIP_CHECK(hipMemcpyAsync(d_foff, t.far_off.data(), sizeof(int32_t) * (nleaf + 1),
-                                hipMemcpyHostToDevice, stream));
-    if (!t.far_list.empty())
-        FS_HIP_CHECK(hipMemcpyAsync(d_flist, t.far_list.data(), sizeof(int32_t) * t.far_list.size(),
-                                    hipMemcpyHostToDevice, stream));
-    FS_HIP_CHECK(hipMemcpyAsync(d_noff, t.near_off.data(), sizeof(int32_t) * (nleaf + 1),
-                                hipMemcpyHostToDevice, stream));
-    FS_HIP_CHECK(hipMemcpyAsync(d_nlist, t.near_list.data(), sizeof(int32_t) * t.near_list.size(),
-                                hipMemcpyHostToDevice, stream));
-    FS_HIP_CHECK(hipMemcpyAsync(d_cheb, t.cheb.data(), sizeof(double) * t.cheb.size(),
-                                hipMemcpyHostToDevice, stream));
-    FS_HIP_CHECK(hipMemcpyAsync(d_perm, t.perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice,
-                                stream));
     const unsigned gb = (unsigned)((n + 255) / 256);
-    gather_kernel<<<gb, 256, 0, stream>>>(d_x, d_perm, n, px_t, L, 1);
-    gather_kernel<<<gb, 256, 0, stream>>>(d_w, d_perm, n, lam_t, L, 0);
-    if (nn > 1) moments_kernel<<<nn - 1, 256, 0, stream>>>(px_t, lam_t, d_range, d_cheb, PP, d_mom);
+    gather_kernel<<<gb, 256, 0, stream>>>(d_x, T.perm, n, T.px, L, 1);
+    gather_kernel<<<gb, 256, 0, stream>>>(d_w, T.perm, n, T.lam, L, 0);
+    if (T.c.nnodes > 1)
+        moments_kernel<<<T.c.nnodes - 1, 256, 0, stream>>>(T.px, T.lam, T.range, T.cheb, PP, T.mom);
     const double delta = eps / L, delta2 = delta * delta;
     const double scale = 0.5 * std::sqrt(1 + 4 * delta2);
-    tree_sum_kernel<<<(unsigned)blocks.size(), 64, 0, stream>>>(
-        px_t, lam_t, d_range, d_blocks, d_foff, d_flist, d_noff, d_nlist, d_cheb, d_mom, PP,
-        1.0 / L, delta2, scale, e_t);
-    scatter_perm_kernel<<<gb, 256, 0, stream>>>(e_t, d_perm, n, d_es);
+    tree_sum_kernel<<<(unsigned)T.c.blk_total, 64, 0, stream>>>(
+        T.px, T.lam, T.range, T.blocks, T.far_off, T.far_list, T.near_off, T.near_list, T.cheb,
+        T.mom, PP, 1.0 / L, delta2, scale, T.e_tree);
+    scatter_perm_kernel<<<gb, 256, 0, stream>>>(T.e_tree, T.perm, n, d_es);
     FS_HIP_CHECK(hipGetLastError());
     if (evals) {
-        evals[0] = t.far_evals;
-        evals[1] = t.near_evals;
+        evals[0] = T.c.far_evals;
+        evals[1] = T.c.near_evals;
     }
-    // the host vectors above back the async copies
-    FS_HIP_CHECK(hipStreamSynchronize(stream));
     return FS_OK;
 }
 
@@ -305,9 +211,71 @@ int fs_emq_treecode_f64(fs_ctx* ctx, double* es, const double* sources, const do
     FS_HIP_CHECK(hipMemcpyAsync(d_x, sources, sizeof(double) * ns, hipMemcpyHostToDevice, ctx->stream));
     FS_HIP_CHECK(hipMemcpyAsync(d_w, q_ws, sizeof(double) * ns, hipMemcpyHostToDevice, ctx->stream));
     FS_TRY(fs::treecode_f64(d_x, d_w, (int)ns, L, epsilon, mac, degree, max_source, d_e,
-                            ctx->scratch, ctx->stream, evals));
+                            ctx->tree, ctx->stream, evals));
     FS_HIP_CHECK(hipMemcpyAsync(es, d_e, sizeof(double) * ns, hipMemcpyDeviceToHost, ctx->stream));
     FS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return FS_OK;
+}
+
+int fs_treecode_tree(fs_ctx* ctx, const double* sources, int64_t ns, double L, double mac,
+                     int degree, int max_source, int64_t stats[4], int32_t* perm,
+                     int32_t* leaf_nodes, int32_t* far_off, int32_t* far_nodes,
+                     int32_t* near_off, int32_t* near_nodes) {
+    if (!ctx || !sources || !stats) return fs::fail(FS_E_INVALID, "null pointer");
+    if (ns <= 0 || ns > INT32_MAX) return fs::fail(FS_E_INVALID, "ns out of range");
+    if (degree < 1 || degree + 1 > 16) return fs::fail(FS_E_INVALID, "degree must be in [1, 15]");
+    if (max_source < 1) return fs::fail(FS_E_INVALID, "max_source must be >= 1");
+    if (!(L > 0)) return fs::fail(FS_E_INVALID, "L must be > 0");
+    FS_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    FS_TRY(ctx->io.ensure(fs::align_up(sizeof(double) * ns, 256), st));
+    double* d_x = ctx->io.as<double>(0);
+    FS_HIP_CHECK(hipMemcpyAsync(d_x, sources, sizeof(double) * ns, hipMemcpyHostToDevice, st));
+    fs::tree::Built T;
+    FS_TRY(fs::tree::build_device(d_x, (int)ns, L, mac, degree, max_source, ctx->tree, T, st));
+    const fs::tree::Counts& C = T.c;
+    stats[0] = C.nnodes;
+    stats[1] = C.nleaf;
+    stats[2] = C.far_total;
+    stats[3] = C.near_total;
+    std::vector<fs::tree::Node> nodes(C.nnodes);
+    std::vector<int32_t> leaves(C.nleaf), foff(C.nleaf + 1), noff(C.nleaf + 1),
+        flist(C.far_total), nlist(C.near_total), hperm(ns);
+    FS_HIP_CHECK(hipMemcpyAsync(nodes.data(), T.nodes, sizeof(fs::tree::Node) * C.nnodes,
+                                hipMemcpyDeviceToHost, st));
+    FS_HIP_CHECK(hipMemcpyAsync(leaves.data(), T.leaves, 4 * C.nleaf, hipMemcpyDeviceToHost, st));
+    FS_HIP_CHECK(hipMemcpyAsync(foff.data(), T.far_off, 4 * (C.nleaf + 1), hipMemcpyDeviceToHost, st));
+    FS_HIP_CHECK(hipMemcpyAsync(noff.data(), T.near_off, 4 * (C.nleaf + 1), hipMemcpyDeviceToHost, st));
+    if (C.far_total > 0)
+        FS_HIP_CHECK(hipMemcpyAsync(flist.data(), T.far_list, 4 * C.far_total, hipMemcpyDeviceToHost, st));
+    if (C.near_total > 0)
+        FS_HIP_CHECK(hipMemcpyAsync(nlist.data(), T.near_list, 4 * C.near_total, hipMemcpyDeviceToHost, st));
+    FS_HIP_CHECK(hipMemcpyAsync(hperm.data(), T.perm, 4 * ns, hipMemcpyDeviceToHost, st));
+    FS_HIP_CHECK(hipStreamSynchronize(st));
+    if (perm) std::copy(hperm.begin(), hperm.end(), perm);
+    // leaves in increasing x (their sorted ranges are disjoint and in order)
+    std::vector<int32_t> order(C.nleaf);
+    for (int i = 0; i < C.nleaf; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(),
+              [&](int32_t a, int32_t b) { return nodes[leaves[a]].s0 < nodes[leaves[b]].s0; });
+    auto put = [&](int32_t* dst, int node) {
+        dst[0] = nodes[node].s0;
+        dst[1] = nodes[node].s1 - 1;
+        dst[2] = nodes[node].depth;
+    };
+    int64_t fo = 0, no = 0;
+    for (int i = 0; i < C.nleaf; ++i) {
+        const int li = order[i];
+        if (leaf_nodes) put(leaf_nodes + 3 * (int64_t)i, leaves[li]);
+        if (far_off) far_off[i] = (int32_t)fo;
+        if (near_off) near_off[i] = (int32_t)no;
+        for (int k = foff[li]; k < foff[li + 1]; ++k, ++fo)
+            if (far_nodes) put(far_nodes + 3 * fo, flist[k]);
+        for (int k = noff[li]; k < noff[li + 1]; ++k, ++no)
+            if (near_nodes) put(near_nodes + 3 * no, nlist[k]);
+    }
+    if (far_off) far_off[C.nleaf] = (int32_t)fo;
+    if (near_off) near_off[C.nleaf] = (int32_t)no;
     return FS_OK;
 }
 

@@ -71,6 +71,10 @@ def lib():
         L.orc_external_amplitude.argtypes = [ctypes.c_int, _d, _d, _d]
         L.orc_external_amplitude.restype = _d
         L.orc_treecode_last_stats.argtypes = [ctypes.POINTER(ctypes.c_int64)]
+        _ip = ctypes.POINTER(ctypes.c_int32)
+        L.orc_treecode_tree.argtypes = [_dp, _i64, _d, _d, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int64), _ip, _ip, _ip, _ip, _ip,
+                                        _ip]
         L.orc_species_create.restype = ctypes.c_void_p
         L.orc_species_create.argtypes = [ctypes.POINTER(SimParams), ctypes.POINTER(SpeciesParams)]
         for n in ("destroy", "copy_to_old", "remesh"):
@@ -156,6 +160,31 @@ def treecode(sources, q_ws, L: float, eps: float, mac: float, degree: int,
     rc = lib().orc_treecode(_P(es), _P(s), _P(w), s.size, L, eps, mac, degree, max_source)
     assert rc == 0
     return es
+
+
+def treecode_tree(sources, L: float, mac: float, max_source: int):
+    """E_MQ_Treecode's tree and interaction lists (orc_treecode_tree), in the
+    layout of the product's treecode_tree; None when more than max_source
+    points share a coordinate."""
+    s = _f64(sources)
+    st = (ctypes.c_int64 * 4)()
+    rc = lib().orc_treecode_tree(_P(s), s.size, L, mac, max_source, st, None, None, None, None,
+                                 None, None)
+    if rc == -2:
+        return None
+    assert rc == 0
+    nn, nl, nf, nr = (int(v) for v in st)
+    out = dict(stats=(nn, nl, nf, nr), perm=np.empty(s.size, np.int32),
+               leaves=np.empty((nl, 3), np.int32), far_off=np.empty(nl + 1, np.int32),
+               far=np.empty((max(nf, 1), 3), np.int32), near_off=np.empty(nl + 1, np.int32),
+               near=np.empty((max(nr, 1), 3), np.int32))
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    rc = lib().orc_treecode_tree(_P(s), s.size, L, mac, max_source, st, ip(out["perm"]),
+                                 ip(out["leaves"]), ip(out["far_off"]), ip(out["far"]),
+                                 ip(out["near_off"]), ip(out["near"]))
+    assert rc == 0
+    out["far"], out["near"] = out["far"][:nf], out["near"][:nr]
+    return out
 
 
 def treecode_stats():

@@ -52,6 +52,16 @@ int orc_treecode(double* es, const double* sources, const double* q_ws,
                  int64_t ns, double L, double epsilon, double mac, int degree,
                  int max_source);
 
+/* The tree and interaction lists of E_MQ_Treecode for `sources`
+ * (src/FieldStructure.cpp:506-702) in the layout of the product's
+ * fs_treecode_tree: stats {nodes, leaves, far entries, near entries}; a node
+ * as (first, last, depth) in tree order; perm = input index of each
+ * tree-order point; leaves in depth-first (increasing x) order; CSR lists.
+ * -2 when more than max_source points share a coordinate. */
+int orc_treecode_tree(const double* sources, int64_t ns, double L, double mac, int max_source,
+                      int64_t stats[4], int32_t* perm, int32_t* leaf_nodes, int32_t* far_off,
+                      int32_t* far_nodes, int32_t* near_off, int32_t* near_nodes);
+
 /* Tree statistics of the last orc_treecode call on this thread:
  * out[0]=node count, out[1]=leaf count, out[2]=far kernel evaluations,
  * out[3]=near kernel evaluations. */

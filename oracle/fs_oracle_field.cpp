@@ -94,31 +94,44 @@ void external_add(double* es, const double* targets, int64_t nt, double t, int t
 
 namespace {
 
-// Treecode node (src/FieldStructure.hpp:26-40).
+// Treecode node (src/FieldStructure.hpp:26-40), with its depth below the
+// root for the tree export.
 struct TNode {
     size_t m0 = 0, m1 = 0;
     double x0 = 0, x1 = 0, xc = 0;
+    int depth = 0;
     std::vector<size_t> children;
 };
 
 thread_local int64_t g_tc_stats[4] = {0, 0, 0, 0};
 
-}  // namespace
+// The tree of E_MQ_Treecode in its tree order: px and lambda reordered,
+// old_index the input index of each tree-order point, leaves in depth-first
+// order, far/near the interaction lists of each leaf.
+struct TTree {
+    std::vector<double> px, lambda;
+    std::vector<int> old_index;
+    std::vector<TNode> tree;
+    std::vector<size_t> leaves;
+    std::vector<std::vector<size_t>> far, near;
+    bool ok = true;
+};
 
-// src/FieldStructure.cpp:230-1050 without the heap bookkeeping (and without
-// its double frees). Evaluates at the sources.
-void treecode(double* es, const double* sources, const double* q_ws,
-              int64_t ns_, double L, double epsilon, double mac, int degree,
-              int max_source) {
-    const size_t n = (size_t)ns_;
-    std::vector<double> px(n), lambda(n);
+// src/FieldStructure.cpp:240-247 (fmod), build_tree_init (:506-527) with
+// minval/maxval and their paddings (:1032-1050), build_tree_1D_Recursive and
+// split_2 (:530-632), build_interaction_list (:663-702).
+TTree build_tree(const double* sources, const double* q_ws, size_t n, double L, double mac,
+                 int max_source) {
+    TTree T;
+    std::vector<double>& px = T.px;
+    std::vector<double>& lambda = T.lambda;
+    px.resize(n);
+    lambda.resize(n);
     for (size_t i = 0; i < n; ++i) {            // :240-247
         px[i] = std::fmod(sources[i], L);
-        lambda[i] = q_ws[i];
+        lambda[i] = q_ws ? q_ws[i] : 0.0;
     }
-    const int P = degree, PP = P + 1;            // :249-256
     const double sq_theta = mac * mac;
-    const double deno = L * L + 4 * epsilon * epsilon;
     const size_t N0 = (size_t)max_source;
 
     // minval/maxval with their paddings (:1032-1050)
@@ -130,24 +143,32 @@ void treecode(double* es, const double* sources, const double* q_ws,
     mn -= 0.1234;
     mx += 0.001;
 
-    std::vector<int> old_index(n);
+    std::vector<int>& old_index = T.old_index;
+    old_index.resize(n);
     for (size_t i = 0; i < n; ++i) old_index[i] = (int)i;
 
     // build_tree_init (:506-527)
-    std::vector<TNode> tree(1);
+    std::vector<TNode>& tree = T.tree;
+    tree.resize(1);
     tree[0].m0 = 0;
     tree[0].m1 = n - 1;
     tree[0].x0 = mn;
     tree[0].x1 = mx;
     tree[0].xc = 0.5 * (mn + mx);
-    std::vector<size_t> leaves;
+    std::vector<size_t>& leaves = T.leaves;
 
     // split_2 (:555-632): midpoint bisection, x <= mid goes to child 0, the
     // address-table swaps amount to a stable partition of x, lambda and the
-    // original indices; empty children are dropped.
+    // original indices; empty children are dropped. A node deeper than 200
+    // that must split (more than max_source coincident points) fails.
     std::function<void(size_t)> build = [&](size_t node) {   // :530-545
+        if (!T.ok) return;
         const size_t cnt = tree[node].m1 - tree[node].m0 + 1;
         if (cnt >= N0) {
+            if (tree[node].depth > 200) {
+                T.ok = false;
+                return;
+            }
             const double a = tree[node].x0, b = tree[node].x1;
             const double mid = 0.5 * (a + b);
             const size_t s = tree[node].m0, e = tree[node].m1;
@@ -176,6 +197,7 @@ void treecode(double* es, const double* sources, const double* q_ws,
                 ch.xc = 0.5 * (ch.x0 + ch.x1);
                 ch.m0 = seq;
                 ch.m1 = seq + side[c].size() - 1;
+                ch.depth = tree[node].depth + 1;
                 seq += side[c].size();
                 tree[node].children.push_back(tree.size());
                 tree.push_back(ch);
@@ -187,10 +209,12 @@ void treecode(double* es, const double* sources, const double* q_ws,
         }
     };
     build(0);
+    if (!T.ok) return T;
 
     // build_interaction_list (:663-702)
     const size_t nleaf = leaves.size();
-    std::vector<std::vector<size_t>> far(nleaf), near(nleaf);
+    T.far.assign(nleaf, {});
+    T.near.assign(nleaf, {});
     std::function<void(size_t, size_t)> lists = [&](size_t li, size_t node) {
         const double xcb = tree[leaves[li]].xc, xcp = tree[node].xc;
         const double x_vb = tree[leaves[li]].x1, x_vp = tree[node].x1;
@@ -199,15 +223,39 @@ void treecode(double* es, const double* sources, const double* q_ws,
         const double dfbx = x_vb - xcb, dfpx = x_vp - xcp;
         const double sq_R = dfbpx * dfbpx, sq_rb = dfbx * dfbx, sq_rp = dfpx * dfpx;
         if (tree[node].children.empty()) {
-            near[li].push_back(node);
+            T.near[li].push_back(node);
         } else if ((std::sqrt(sq_rb) + std::sqrt(sq_rp)) / std::sqrt(sq_R) <
                    std::sqrt(sq_theta)) {
-            far[li].push_back(node);
+            T.far[li].push_back(node);
         } else {
             for (size_t c : tree[node].children) lists(li, c);
         }
     };
     for (size_t li = 0; li < nleaf; ++li) lists(li, 0);
+    return T;
+}
+
+}  // namespace
+
+// src/FieldStructure.cpp:230-1050 without the heap bookkeeping (and without
+// its double frees). Evaluates at the sources. False when more than
+// max_source points share a coordinate (the reference recurses without end).
+bool treecode(double* es, const double* sources, const double* q_ws,
+              int64_t ns_, double L, double epsilon, double mac, int degree,
+              int max_source) {
+    const size_t n = (size_t)ns_;
+    const int P = degree, PP = P + 1;            // :249-256
+    const double deno = L * L + 4 * epsilon * epsilon;
+    TTree T = build_tree(sources, q_ws, n, L, mac, max_source);
+    if (!T.ok) return false;
+    const std::vector<double>& px = T.px;
+    const std::vector<double>& lambda = T.lambda;
+    const std::vector<int>& old_index = T.old_index;
+    const std::vector<TNode>& tree = T.tree;
+    const std::vector<size_t>& leaves = T.leaves;
+    const size_t nleaf = leaves.size();
+    const std::vector<std::vector<size_t>>& far = T.far;
+    const std::vector<std::vector<size_t>>& near = T.near;
 
     // alloc_set_cluster_list (:775-866): Chebyshev points of the 2nd kind and
     // barycentric moments for every node but the root.
@@ -316,6 +364,7 @@ void treecode(double* es, const double* sources, const double* q_ws,
     g_tc_stats[1] = (int64_t)nleaf;
     g_tc_stats[2] = far_evals;
     g_tc_stats[3] = near_evals;
+    return true;
 }
 
 }  // namespace orc
@@ -408,7 +457,47 @@ int orc_treecode(double* es, const double* sources, const double* q_ws,
                  int64_t ns, double L, double epsilon, double mac, int degree,
                  int max_source) {
     if (ns <= 0 || degree < 1 || max_source < 1 || !(L > 0)) return -1;
-    orc::treecode(es, sources, q_ws, ns, L, epsilon, mac, degree, max_source);
+    return orc::treecode(es, sources, q_ws, ns, L, epsilon, mac, degree, max_source) ? 0 : -2;
+}
+
+int orc_treecode_tree(const double* sources, int64_t ns, double L, double mac, int max_source,
+                      int64_t stats[4], int32_t* perm, int32_t* leaf_nodes, int32_t* far_off,
+                      int32_t* far_nodes, int32_t* near_off, int32_t* near_nodes) {
+    if (ns <= 0 || max_source < 1 || !(L > 0)) return -1;
+    orc::TTree T = orc::build_tree(sources, nullptr, (size_t)ns, L, mac, max_source);
+    if (!T.ok) return -2;
+    int64_t nfar = 0, nnear = 0;
+    for (size_t li = 0; li < T.leaves.size(); ++li) {
+        nfar += (int64_t)T.far[li].size();
+        nnear += (int64_t)T.near[li].size();
+    }
+    stats[0] = (int64_t)T.tree.size();
+    stats[1] = (int64_t)T.leaves.size();
+    stats[2] = nfar;
+    stats[3] = nnear;
+    auto put = [&](int32_t* dst, size_t node) {
+        dst[0] = (int32_t)T.tree[node].m0;
+        dst[1] = (int32_t)T.tree[node].m1;
+        dst[2] = T.tree[node].depth;
+    };
+    if (perm)
+        for (int64_t i = 0; i < ns; ++i) perm[i] = T.old_index[i];
+    int64_t fo = 0, no = 0;
+    for (size_t li = 0; li < T.leaves.size(); ++li) {
+        if (leaf_nodes) put(leaf_nodes + 3 * li, T.leaves[li]);
+        if (far_off) far_off[li] = (int32_t)fo;
+        if (near_off) near_off[li] = (int32_t)no;
+        for (size_t node : T.far[li]) {
+            if (far_nodes) put(far_nodes + 3 * fo, node);
+            ++fo;
+        }
+        for (size_t node : T.near[li]) {
+            if (near_nodes) put(near_nodes + 3 * no, node);
+            ++no;
+        }
+    }
+    if (far_off) far_off[T.leaves.size()] = (int32_t)fo;
+    if (near_off) near_off[T.leaves.size()] = (int32_t)no;
     return 0;
 }
 

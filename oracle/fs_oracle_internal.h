@@ -97,7 +97,8 @@ void solve9(const double A[81], const double b[9], double c[9]);
 // Direct sum / treecode used by the simulation (same as the C API).
 void direct_sum(double* es, const double* t, int64_t nt, const double* s,
                 const double* w, int64_t ns, double L, double eps);
-void treecode(double* es, const double* s, const double* w, int64_t ns,
+// false when more than max_source points share a coordinate
+bool treecode(double* es, const double* s, const double* w, int64_t ns,
               double L, double eps, double mac, int degree, int max_source);
 void direct_sum_open(double* es, const double* t, int64_t nt, const double* s,
                      const double* w, int64_t ns, double eps);
