@@ -144,8 +144,22 @@ void PanelTree::reserve_pass(int kids, int pts, std::vector<double>& nx,
     fs.reserve(fs.size() + flagged * pts + 16);
 }
 
+// The new points of a pass, kept per thread so that their pages stay mapped
+// from pass to pass.
+struct PassBuffers {
+    std::vector<double> nx, np, nf;
+};
+static PassBuffers& pass_buffers() {
+    static thread_local PassBuffers b;
+    b.nx.clear();
+    b.np.clear();
+    return b;
+}
+
 void PanelTree::refine_xp(const BatchEval& f) {
-    std::vector<double> nx, np;
+    PassBuffers& pb = pass_buffers();
+    std::vector<double>& nx = pb.nx;
+    std::vector<double>& np = pb.np;
     reserve_pass(4, 16, nx, np);
     int nv = (int)xs.size();
     const int before = size();
@@ -229,7 +243,8 @@ void PanelTree::refine_xp(const BatchEval& f) {
         append_panel(jj, 2, c2, base, base + 3, cn[2][3], cn[2][1], false, rb);
         append_panel(jj, 3, c3, base + 1, cn[3][2], cn[3][3], base + 2, false, rb);
     }
-    std::vector<double> nf(nx.size());
+    std::vector<double>& nf = pb.nf;
+    nf.resize(nx.size());
     if (!nx.empty()) f(nx.data(), np.data(), nx.size(), nf.data());
     xs.insert(xs.end(), nx.begin(), nx.end());
     ps.insert(ps.end(), np.begin(), np.end());
@@ -237,7 +252,9 @@ void PanelTree::refine_xp(const BatchEval& f) {
 }
 
 void PanelTree::refine_p(const BatchEval& f) {
-    std::vector<double> nx, np;
+    PassBuffers& pb = pass_buffers();
+    std::vector<double>& nx = pb.nx;
+    std::vector<double>& np = pb.np;
     reserve_pass(2, 6, nx, np);
     int nv = (int)xs.size();
     const int before = size();
@@ -340,7 +357,8 @@ void PanelTree::refine_p(const BatchEval& f) {
         append_panel(jj, 0, c0, c0L, base + 1, c0R, c0B, lb, rb);
         append_panel(jj, 1, c1, c1L, c1T, c1R, base, lb, rb);
     }
-    std::vector<double> nf(nx.size());
+    std::vector<double>& nf = pb.nf;
+    nf.resize(nx.size());
     if (!nx.empty()) f(nx.data(), np.data(), nx.size(), nf.data());
     xs.insert(xs.end(), nx.begin(), nx.end());
     ps.insert(ps.end(), np.begin(), np.end());
@@ -433,12 +451,17 @@ std::vector<int32_t> grid_rows(const std::vector<double>& xs, const std::vector<
 }
 
 ReducedMap reduced_map(const std::vector<double>& xs) {
+    ReducedMap m;
+    reduced_map(xs, m);
+    return m;
+}
+
+void reduced_map(const std::vector<double>& xs, ReducedMap& m) {
     // unique values by exact equality (as the reference's sort-then-compare
     // does; +0 and -0 are one value), found with an open-addressed hash in
     // point order -- a remeshed species has ~1e3 distinct x among ~1e5..1e6
     // points -- then ranked by sorting only the distinct values
     const size_t n = xs.size();
-    ReducedMap m;
     m.unique_of.assign(n, 0);
     size_t cap = 1024;
     std::vector<int32_t> slot(cap, -1);  // -> index into `first`
@@ -452,7 +475,8 @@ ReducedMap reduced_map(const std::vector<double>& xs) {
         u ^= u >> 33;
         return u;
     };
-    std::vector<int32_t> tag(n);
+    static thread_local std::vector<int32_t> tag;
+    tag.resize(n);
     for (size_t i = 0; i < n; ++i) {
         if (2 * (first.size() + 1) > cap) {  // grow, rehash
             cap *= 2;
@@ -489,7 +513,6 @@ ReducedMap reduced_map(const std::vector<double>& xs) {
     m.members.assign(n, 0);
     std::vector<int32_t> fill(m.offsets.begin(), m.offsets.end() - 1);
     for (size_t i = 0; i < n; ++i) m.members[fill[m.unique_of[i]]++] = (int32_t)i;
-    return m;
 }
 
 }  // namespace mesh

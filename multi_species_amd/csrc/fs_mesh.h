@@ -82,6 +82,8 @@ struct ReducedMap {
     std::vector<int32_t> unique_of;  // per point
 };
 ReducedMap reduced_map(const std::vector<double>& xs);
+// The same into m, reusing its vectors' storage.
+void reduced_map(const std::vector<double>& xs, ReducedMap& m);
 
 }  // namespace mesh
 }  // namespace fs

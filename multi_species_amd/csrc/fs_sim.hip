@@ -222,6 +222,7 @@ struct Species {
     bool adaptive = false;
     mesh::PanelTree base;  // the uniform base mesh (structure and grid)
     mesh::PanelTree tree;  // the current mesh
+    mesh::PanelTree next;  // the adaptive remesh's work tree (buffers kept between remeshes)
     int nx = 0, np = 0;
     std::vector<int32_t> rows;
     std::vector<double> base_w;
@@ -667,12 +668,12 @@ int remesh_adaptive(Sim& S) {
     std::vector<Species*> refine;
     for (Species* s : S.sp)
         if (s->adaptive) refine.push_back(s);
-    std::vector<mesh::PanelTree> trees(refine.size());
-    for (size_t q = 0; q < refine.size(); ++q) {
-        Species& s = *refine[q];
-        trees[q] = s.base;
-        FS_HIP_CHECK(hipMemcpyAsync(trees[q].fs.data(), s.d_fnew.ptr,
-                                    sizeof(double) * s.nx * s.np, hipMemcpyDeviceToHost, st));
+    // each refining species rebuilds into its work tree, whose buffers stay
+    // allocated from remesh to remesh (no fresh pages for ~1e5 panels)
+    for (Species* s : refine) {
+        s->next = s->base;
+        FS_HIP_CHECK(hipMemcpyAsync(s->next.fs.data(), s->d_fnew.ptr,
+                                    sizeof(double) * s->nx * s->np, hipMemcpyDeviceToHost, st));
     }
     FS_HIP_CHECK(hipStreamSynchronize(st));
     std::mutex dev_mu;
@@ -723,7 +724,7 @@ int remesh_adaptive(Sim& S) {
             }
             rc = rm::status_to_error(status);
         };
-        trees[q].refine_adaptively(s.spec, eval);
+        s.next.refine_adaptively(s.spec, eval);
         {
             std::lock_guard<std::mutex> lk(dev_mu);
             (void)hipStreamSynchronize(st);
@@ -744,10 +745,10 @@ int remesh_adaptive(Sim& S) {
     for (size_t q = 0; q < refine.size(); ++q) {
         Species& s = *refine[q];
         const Deck& D = *S.decks[s.deck];
-        s.tree = std::move(trees[q]);
+        std::swap(s.tree, s.next);
         std::vector<int32_t> leaves;
         s.tree.leaf_weights(s.q, D.prm.quadrature == 1, s.base_w, leaves);
-        s.red = mesh::reduced_map(s.tree.xs);
+        mesh::reduced_map(s.tree.xs, s.red);
         s.off = -1;  // upload from the host tree
     }
     FS_TRY(layout(S, true));
