@@ -581,7 +581,7 @@ def bench_treecode(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "E_MQ_Treecode of the N=2^20 deck (c2 inputs), theta 0.7, "
-                               "degree 8, max_source 2000, host arrays (tree on the host)",
+                               "degree 8, max_source 2000, host arrays (sort, tree, lists and sums on the device)",
                    "n": n, "eps": args.eps},
         "evals": {"far": int(ev[0]), "near": int(ev[1])},
         "equivalent_pairs_per_s": n * n / dt,
