@@ -664,11 +664,48 @@ C4_WORKLOAD = ("independent two-species Landau decks in one batch (amp log-unifo
                "graph-captured step")
 
 
+DIAG_KEYS = ("charge", "momentum", "kinetic_energy", "electric_energy")
+
+
+def mixed_diag_errors(ctx, ndecks: int, steps: int):
+    """The same decks stepped as one batch in mixed precision and in FP64: after
+    every step, each species' charge, momentum, kinetic and electric energy of the
+    mixed run against the FP64 run, relative to the diagnostic's own scale (the sum
+    of its terms' magnitudes: sum |q_w|, sum |p q_w| / |q|, sum p^2 |q_w| / (2 |q/m|),
+    and the electric energy itself, a sum of positive terms). Max over species and
+    steps; the electric-energy entry covers its whole history."""
+    import multi_species_amd as ms
+
+    decks = c4_decks(ndecks)
+    sims = [ms.Simulation.batch(ctx, c4_decks(ndecks), prec)
+            for prec in (ms.PRECISION_MIXED, ms.PRECISION_F64)]
+    params = []
+    for d, (_, sps) in enumerate(decks):
+        first, nsp, _, _ = sims[1].deck_info(d)
+        params += [(first + j, sps[j].q, sps[j].q / sps[j].m) for j in range(nsp)]
+    worst = dict.fromkeys(DIAG_KEYS, 0.0)
+    for _ in range(steps):
+        for s in sims:
+            s.step()
+        for k, q, qm in params:
+            dm, df = sims[0].diagnostics(k), sims[1].diagnostics(k)
+            _, p, _, qw, _ = sims[1].points(k)
+            scale = (np.sum(np.abs(qw)), np.sum(np.abs(p * qw)) / abs(q),
+                     np.sum(p * p * np.abs(qw)) / (2 * abs(qm)), abs(df[3]))
+            for i, key in enumerate(DIAG_KEYS):
+                if scale[i] > 0:
+                    worst[key] = max(worst[key], float(abs(dm[i] - df[i]) / scale[i]))
+    for s in sims:
+        s.close()
+    return worst
+
+
 def c4_block(ctx, ndecks: int, steps: int, warmup: int, compare: bool = True,
-             plain_steps: int = 2):
+             plain_steps: int = 2, diag_steps: int = 5):
     """ndecks decks as one batch (mixed precision, graph-captured step) against the
-    same decks stepped one by one in plain FP64; per-step latency of the batch, and
-    the mixed E against the FP64 field of the same points."""
+    same decks stepped one by one in plain FP64; per-step latency of the batch, the
+    mixed E against the FP64 field of the same points, and the mixed diagnostics
+    against the same decks stepped in FP64 (mixed_diag_errors)."""
     import multi_species_amd as ms
 
     decks = c4_decks(ndecks)
@@ -742,6 +779,8 @@ def c4_block(ctx, ndecks: int, steps: int, warmup: int, compare: bool = True,
                                    "sample": f"{plain_steps} timed steps of each deck alone"}
     out["rel_e_err_vs_f64"] = {"remesh_field": err, "full_field_at_moved_points": err_stage,
                                "bound": 1e-8}
+    out["rel_diag_err_vs_f64"] = dict(mixed_diag_errors(ctx, ndecks, diag_steps),
+                                      steps=diag_steps)
     return out
 
 

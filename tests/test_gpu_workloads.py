@@ -16,6 +16,9 @@
   bitwise equal to every deck run alone; the mixed full field at moved points
   (remesh every second step, after one step) within 1e-8 of the FP64 field of
   the same points, every deck.
+* The mixed form's error per diagnostic: charge, momentum, kinetic and
+  electric energy of 16 of those decks, mixed against FP64, within 1e-8 of
+  each diagnostic's scale after every step of 4.
 * A served Poisson stream (12 requests at 20/s, 5k..526k points, weak/strong
   Landau and bump-on-tail decks with adaptive refinement, mixed precision):
   every request's final state bitwise that of its deck run alone, and the
@@ -154,6 +157,38 @@ def test_batch_of_64_decks_mixed_field_at_moved_points(gpu_ctx):
         assert r <= 1e-8, (d, r)
     b.close()
     print(f"64 decks, mixed full field at moved points: worst rel E err {worst:.2e}")
+
+
+def test_mixed_diagnostics_against_fp64(gpu_ctx):
+    """The mixed form's error per diagnostic: 16 of the Landau decks stepped 4
+    steps as a mixed and as an FP64 batch; after every step each species'
+    charge, momentum, kinetic and electric energy agree within 1e-8 of the
+    diagnostic's scale (the sum of its terms' magnitudes; the electric energy
+    is a sum of positive terms, so its own value), the electric-energy history
+    included."""
+    ds = decks.landau_batch(16)
+    sims = [ms.Simulation.batch(gpu_ctx, decks.landau_batch(16), p)
+            for p in (ms.PRECISION_MIXED, ms.PRECISION_F64)]
+    params = []
+    for d, (_, sps) in enumerate(ds):
+        first, nsp, _, _ = sims[1].deck_info(d)
+        params += [(first + j, sps[j].q, sps[j].q / sps[j].m) for j in range(nsp)]
+    worst = np.zeros(4)
+    for step in range(4):
+        for s in sims:
+            s.step()
+        for k, q, qm in params:
+            dm, df = sims[0].diagnostics(k), sims[1].diagnostics(k)
+            _, p, _, qw, _ = sims[1].points(k)
+            scale = np.array([np.sum(np.abs(qw)), np.sum(np.abs(p * qw)) / abs(q),
+                              np.sum(p * p * np.abs(qw)) / (2 * abs(qm)), abs(df[3])])
+            err = np.abs(dm - df) / scale
+            worst = np.maximum(worst, err)
+            assert np.all(err <= 1e-8), (step, k, err)
+    for s in sims:
+        s.close()
+    print("mixed vs FP64 diagnostics (charge, momentum, kinetic, electric):",
+          " ".join(f"{v:.2e}" for v in worst))
 
 
 def test_served_poisson_stream_equals_alone(gpu_ctx):
