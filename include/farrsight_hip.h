@@ -266,7 +266,15 @@ typedef struct fs_species_params {
     double beam_frac, beam_v, beam_vth;
 } fs_species_params;
 
-/* The deck's top-level keys (AMRSimulation_load.cpp:14-96). */
+/* The deck's top-level keys (AMRSimulation_load.cpp:14-136). The field
+ * follows make_field_return_ptr (:49-96): periodic decks take the direct sum
+ * or, with use_treecode, the treecode; open decks (bcs 1) the open-boundary
+ * direct sum, or with use_treecode the periodic direct sum (the reference's
+ * selection). Open decks run in FP64 only. The external drive is the deck's
+ * "external_field" block (:99-136): when use_external_field is set,
+ * a(t) sin(k x - omega t) is added to every field evaluation at the
+ * reference's stage times (ExternalSine for external_type 1,
+ * ExternalLogistic for 2, ExternalTanh otherwise). */
 typedef struct fs_sim_params {
     double x_min, x_max;
     int bcs;            /* 0 periodic, 1 open */
@@ -277,6 +285,9 @@ typedef struct fs_sim_params {
     int degree, max_source;
     double dt;
     int remesh_period;
+    int use_external_field;
+    int external_type;
+    double Am, k, omega;
 } fs_sim_params;
 
 typedef struct fs_sim fs_sim;

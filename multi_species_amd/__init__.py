@@ -341,6 +341,8 @@ class SimParams(ctypes.Structure):
         ("x_min", _d), ("x_max", _d), ("bcs", _int), ("quadrature", _int),
         ("greens_epsilon", _d), ("use_treecode", _int), ("mac", _d), ("degree", _int),
         ("max_source", _int), ("dt", _d), ("remesh_period", _int),
+        ("use_external_field", _int), ("external_type", _int), ("Am", _d), ("k", _d),
+        ("omega", _d),
     ]
 
 
@@ -403,12 +405,16 @@ def species_params(q=-1.0, m=1.0, p_min=-1.0, p_max=1.0, initial_height=6, p_hei
 
 def sim_params(x_min=0.0, x_max=0.0, bcs=0, quadrature=1, greens_epsilon=0.2, use_treecode=0,
                mac=-1.0, degree=-1, max_source=2000, dt=0.5, remesh_period=1,
+               use_external_field=0, external_type=0, Am=0.0, k=0.0, omega=0.0,
                **_ignored) -> SimParams:
-    """Top-level deck keys with the reference's defaults (AMRSimulation_load.cpp:14-59)."""
+    """Top-level deck keys with the reference's defaults (AMRSimulation_load.cpp:14-59),
+    and the external drive block (:99-136)."""
     p = SimParams()
     p.x_min, p.x_max, p.bcs, p.quadrature = x_min, x_max, bcs, quadrature
     p.greens_epsilon, p.use_treecode, p.mac = greens_epsilon, use_treecode, mac
     p.degree, p.max_source, p.dt, p.remesh_period = degree, max_source, dt, remesh_period
+    p.use_external_field, p.external_type = use_external_field, external_type
+    p.Am, p.k, p.omega = Am, k, omega
     return p
 
 

@@ -29,6 +29,7 @@ struct BatchSeg {
     int off = 0, n = 0;  // points [off, off + n)
     int src = 0;         // segment whose points are the sources
     bool eval = true;    // evaluate the field at this segment's points
+    bool open = false;   // open boundaries (FP64 only): E_MQ_DirectSum_openbcs
     double L = 1.0, eps = 0.0;
 };
 
@@ -39,6 +40,7 @@ struct SegDev {
     int src_off, src_n;
     int splits, chunk;
     int eval, all_band;
+    int open;         // open boundaries: z = x_i - x_j, delta2 = eps^2, E = sum - x_i
     int64_t part_base;
     double invL, delta2, C;
     // mixed only
@@ -134,6 +136,7 @@ struct SymSeg {
     int item_base;          // first workgroup of the segment in its launch
     int64_t rbase, cbase;   // row and column partials (doubles into the work space)
     double invL, delta2;
+    int open;               // open-boundary kernel (delta2 = eps^2)
 };
 
 // Plan and device tables of one batched field evaluation; the tables live

@@ -86,6 +86,58 @@ __device__ __forceinline__ int pull_item(int* counter) {
 
 // ---- FP64 ------------------------------------------------------------------
 
+// One (deck, target block, source split) item of the FP64 full sum: the
+// periodic kernel z (1/sqrt(z^2 + delta^2) - C), z wrapped, or with kOpen the
+// open-boundary kernel z / sqrt(z^2 + eps^2), z = x_i - x_j; runs of kSub
+// sources summed in FP64, the runs added with TwoSum.
+template <int TPT, bool kOpen>
+__device__ __forceinline__ void f64_item(const WorkItem& W, const SegDev& S,
+                                         const double* __restrict__ x,
+                                         const double* __restrict__ w, double* __restrict__ part) {
+    const double invL = S.invL, delta2 = S.delta2;
+    const double C = kOpen ? 0.0 : rsq_f64(__builtin_fma(0.5, 0.5, delta2));
+    double xi[TPT], hi[TPT], lo[TPT];
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        const int i = W.tb + (int)threadIdx.x + t * kBlock;
+        xi[t] = (i < W.te) ? x[i] : 0.0;
+        hi[t] = 0.0;
+        lo[t] = 0.0;
+    }
+    for (int jb = W.sb; jb < W.se; jb += kSub) {
+        const int je = min(W.se, jb + kSub);
+        double acc[TPT];
+#pragma unroll
+        for (int t = 0; t < TPT; ++t) acc[t] = 0.0;
+#pragma unroll 4
+        for (int j = jb; j < je; ++j) {
+            const double xj = x[j];
+            const double wj = w[j];
+#pragma unroll
+            for (int t = 0; t < TPT; ++t) {
+                if constexpr (kOpen) {
+                    const double z = xi[t] - xj;
+                    const double y = rsq_f64(__builtin_fma(z, z, delta2));
+                    acc[t] = __builtin_fma(wj * z, y, acc[t]);
+                } else {
+                    const double s = (xi[t] - xj) * invL;
+                    const double z = s - __builtin_rint(s);
+                    const double y = rsq_f64(__builtin_fma(z, z, delta2));
+                    acc[t] = __builtin_fma(wj * z, y - C, acc[t]);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < TPT; ++t) two_sum_acc(hi[t], lo[t], acc[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        const int i = W.tb + (int)threadIdx.x + t * kBlock;
+        if (i < W.te)
+            part[S.part_base + (int64_t)W.split * S.n + (i - S.off)] = 0.5 * (hi[t] + lo[t]);
+    }
+}
+
 template <int TPT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void batch_f64_kernel(const WorkItem* __restrict__ items,
                                                            int nitems,
@@ -99,42 +151,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
         if (it >= nitems) break;
         const WorkItem W = items[it];
         const SegDev S = segs[W.seg];
-        const double invL = S.invL, delta2 = S.delta2;
-        const double C = rsq_f64(__builtin_fma(0.5, 0.5, delta2));
-        double xi[TPT], hi[TPT], lo[TPT];
-#pragma unroll
-        for (int t = 0; t < TPT; ++t) {
-            const int i = W.tb + (int)threadIdx.x + t * kBlock;
-            xi[t] = (i < W.te) ? x[i] : 0.0;
-            hi[t] = 0.0;
-            lo[t] = 0.0;
-        }
-        for (int jb = W.sb; jb < W.se; jb += kSub) {
-            const int je = min(W.se, jb + kSub);
-            double acc[TPT];
-#pragma unroll
-            for (int t = 0; t < TPT; ++t) acc[t] = 0.0;
-#pragma unroll 4
-            for (int j = jb; j < je; ++j) {
-                const double xj = x[j];
-                const double wj = w[j];
-#pragma unroll
-                for (int t = 0; t < TPT; ++t) {
-                    const double s = (xi[t] - xj) * invL;
-                    const double z = s - __builtin_rint(s);
-                    const double y = rsq_f64(__builtin_fma(z, z, delta2));
-                    acc[t] = __builtin_fma(wj * z, y - C, acc[t]);
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < TPT; ++t) two_sum_acc(hi[t], lo[t], acc[t]);
-        }
-#pragma unroll
-        for (int t = 0; t < TPT; ++t) {
-            const int i = W.tb + (int)threadIdx.x + t * kBlock;
-            if (i < W.te)
-                part[S.part_base + (int64_t)W.split * S.n + (i - S.off)] = 0.5 * (hi[t] + lo[t]);
-        }
+        if (S.open) f64_item<TPT, true>(W, S, x, w, part);
+        else f64_item<TPT, false>(W, S, x, w, part);
     }
 }
 
@@ -152,34 +170,42 @@ __global__ __launch_bounds__(sym::threads<T>()) void batch_sym_kernel(
         else hi = mid - 1;
     }
     const SymSeg E = tab[lo];
-    sym::item<T>(x + E.off, w + E.off, E.P, E.S, (int)blockIdx.x - E.item_base, E.invL, E.delta2,
-                 sympart + E.rbase, sympart + E.cbase, red);
+    if (E.open)
+        sym::item<T, true>(x + E.off, w + E.off, E.P, E.S, (int)blockIdx.x - E.item_base, E.invL,
+                           E.delta2, sympart + E.rbase, sympart + E.cbase, red);
+    else
+        sym::item<T, false>(x + E.off, w + E.off, E.P, E.S, (int)blockIdx.x - E.item_base, E.invL,
+                            E.delta2, sympart + E.rbase, sympart + E.cbase, red);
 }
 
+// A point's partials in a fixed order; open-boundary segments subtract the
+// point's own x (E_MQ_DirectSum_openbcs's uniform background).
 __global__ __launch_bounds__(kBlock) void finalize_f64_kernel(
     int n_total, const int32_t* __restrict__ seg_of, const SegDev* __restrict__ segs,
     const double* __restrict__ part, const SymSeg* __restrict__ symsegs,
-    const double* __restrict__ sympart, double* __restrict__ es, int* __restrict__ counter) {
+    const double* __restrict__ sympart, const double* __restrict__ x, double* __restrict__ es,
+    int* __restrict__ counter) {
     const int g = blockIdx.x * kBlock + threadIdx.x;
     if (g == 0) *counter = 0;  // the pair kernel has finished: ready for the next run
     if (g >= n_total) return;
     const SegDev& S = segs[seg_of[g]];
     if (!S.eval) return;
     if (S.src_n == 0) {
-        es[g] = 0.0;
+        es[g] = S.open ? -x[g] : 0.0;
         return;
     }
     if (S.sym >= 0) {
         const SymSeg& E = symsegs[S.sym];
         double s = 0.0, lo = 0.0;
         sym::fold(E.P, E.S, g - S.off, sympart + E.rbase, sympart + E.cbase, s, lo);
-        es[g] = 0.5 * (s + lo);
+        const double e = 0.5 * (s + lo);
+        es[g] = S.open ? e - x[g] : e;
         return;
     }
     double s = 0.0, lo = 0.0;
     for (int k = 0; k < S.splits; ++k)
         two_sum_acc(s, lo, part[S.part_base + (int64_t)k * S.n + (g - S.off)]);
-    es[g] = s + lo;
+    es[g] = S.open ? (s + lo) - x[g] : s + lo;
 }
 
 // ---- mixed -----------------------------------------------------------------
@@ -1357,6 +1383,8 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         if (!(b.L > 0)) return fail(FS_E_INVALID, "field batch: segment %d has L <= 0", d);
         if (precision == kPrecisionMixed && b.eval && !(b.eps > 0))
             return fail(FS_E_INVALID, "field batch: mixed precision needs eps > 0");
+        if (precision == kPrecisionMixed && b.eval && b.open)
+            return fail(FS_E_INVALID, "field batch: open boundaries run in FP64 only");
         for (int g = b.off; g < b.off + b.n; ++g) seg_of[g] = d;
         SegDev& S = segs_[d];
         S.off = b.off;
@@ -1365,10 +1393,16 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         S.src_off = segs[b.src].off;
         S.src_n = segs[b.src].n;
         S.eval = b.eval ? 1 : 0;
+        S.open = b.open ? 1 : 0;
         S.invL = 1.0 / b.L;
         const double delta = b.eps / b.L;
         S.delta2 = delta * delta;
         S.C = 2.0 / std::sqrt(1.0 + 4.0 * S.delta2);
+        if (b.open) {  // the open kernel in absolute units
+            S.invL = 1.0;
+            S.delta2 = b.eps * b.eps;
+            S.C = 0.0;
+        }
         S.pw_base = b.off + d;
         S.splits = 1;
         S.chunk = std::max(1, S.src_n);
@@ -1388,6 +1422,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
             E.off = b.off;
             E.invL = S.invL;
             E.delta2 = S.delta2;
+            E.open = S.open;
             S.sym = (int)symsegs_.size();  // renumbered below, grouped by targets per lane
             symsegs_.push_back(E);
             pairs_ += (int64_t)b.n * b.n;
@@ -1765,7 +1800,7 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
                 d_symsegs_, sym_first_[2], sym_count_[2], x, w, sympart);
         FS_HIP_CHECK(hipGetLastError());
         finalize_f64_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
-            n_total_, d_seg_of_, d_segs_, part, d_symsegs_, sympart, es, d_counter_);
+            n_total_, d_seg_of_, d_segs_, part, d_symsegs_, sympart, x, es, d_counter_);
         FS_HIP_CHECK(hipGetLastError());
         return FS_OK;
     }

@@ -96,6 +96,10 @@ int read_species(const char* sim_dir, const char* name, int iter, std::vector<do
 int write_reduced(const char* sim_dir, int iter, int64_t u, const double* rx, const double* re);
 }  // namespace io
 
+// The drive amplitude a(t) of ExternalTanh / ExternalSine / ExternalLogistic
+// (fs_open.hip; the reference's expressions and constants).
+double external_amplitude(int type, double t, double Am, double k);
+
 // Fixed device work space of fs_emq_periodic_f64_noalloc (per device).
 constexpr size_t kNoAllocBytes = size_t(256) << 20;
 // Takes the current device's address of that work space (once per device;

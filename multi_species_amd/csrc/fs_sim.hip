@@ -52,6 +52,28 @@ struct SpeciesInfo {
     double qm, h, dt, dt6;
 };
 
+// The external drive of each species' deck at the two times a step needs:
+// slot 0 the RK4 mid stages (t + dt/2), slot 1 the last stage and the field
+// after the step (t + dt), or the deck's current time outside a step.
+struct DriveDev {
+    int on, pad;
+    double k;
+    double a[2], omt[2];
+};
+
+// (*calculate_e_external) (AMRSimulation_field.cpp:70-72, 169-171):
+// e += a(t) sin(k x - omega t) with the reference's rounding (k x and the
+// product each rounded, no FMA).
+__global__ __launch_bounds__(kBlock) void drive_kernel(int n, const int32_t* __restrict__ sid,
+                                                       const DriveDev* __restrict__ tab, int slot,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ e) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const DriveDev D = tab[sid[i]];
+    if (D.on) e[i] += D.a[slot] * sin(D.k * x[i] - D.omt[slot]);
+}
+
 // k1 -> k2 (AMRSimulation_step.cpp:148-164): ef1 = es q/m, p2 = p + dt/2 ef1,
 // x_k = x + dt/2 p1 (p is the velocity).
 __global__ __launch_bounds__(kBlock) void rk4_stage1(int n, const int32_t* __restrict__ sid,
@@ -247,6 +269,13 @@ struct Species {
     }
 };
 
+// The field of a deck (make_field_return_ptr, AMRSimulation_load.cpp:49-96):
+// the treecode for periodic decks that ask for it, the open-boundary direct
+// sum for open decks without it, the periodic direct sum otherwise (open
+// decks with use_treecode included, :61-73).
+inline bool uses_treecode(const fs_sim_params& p) { return p.use_treecode > 0 && p.bcs == 0; }
+inline bool open_field(const fs_sim_params& p) { return p.bcs != 0 && p.use_treecode <= 0; }
+
 struct Deck {
     fs_sim_params prm{};
     std::vector<std::unique_ptr<Species>> sp;
@@ -291,8 +320,15 @@ struct fs_sim {
            *e3 = nullptr, *e4 = nullptr;
     double *rx = nullptr, *rw = nullptr, *re = nullptr;
     int32_t *r_off = nullptr, *r_mem = nullptr, *r_uof = nullptr;
+    int32_t* rsid = nullptr;  // species of each reduced point
     int32_t* sid = nullptr;
     fs::sim::SpeciesInfo* sinfo = nullptr;
+    // external drives: per-species table, its host copy (kept until the copy
+    // to the device has run: every step synchronizes)
+    bool any_drive = false;
+    DevBuf drive;
+    fs::sim::DriveDev* drive_d = nullptr;
+    std::vector<fs::sim::DriveDev> drive_h;
     int* d_status = nullptr;
     hipGraphExec_t g_remesh = nullptr, g_plain = nullptr;
 
@@ -317,14 +353,47 @@ int treecode_deck(Sim& S, const Deck& D, double* out, const double* x, const dou
                         D.prm.max_source, out, S.tree_work, S.stream(), nullptr);
 }
 
-// E at every point: the batched direct sum over all direct-sum decks, then
-// the treecode decks one by one.
-int field_full(Sim& S, double* out, const double* x) {
+// The drive table for the next field evaluations: in a step, slot 0 at
+// t + dt/2 (AMRSimulation_step.cpp:170, 201) and slot 1 at t + dt (:224, and
+// the field after the step at the advanced t); outside a step, slot 1 at the
+// deck's current t. Amplitudes on the host with the reference's
+// expressions; a replayed graph reads them from device memory.
+int set_drive(Sim& S, bool in_step) {
+    if (!S.any_drive) return FS_OK;
+    for (size_t k = 0; k < S.sp.size(); ++k) {
+        const Deck& D = *S.decks[S.sp[k]->deck];
+        DriveDev v{};
+        if (D.prm.use_external_field > 0) {
+            const fs_sim_params& P = D.prm;
+            const double t0 = D.t + P.dt * 0.5;
+            const double t1 = in_step ? D.t + P.dt : D.t;
+            v.on = 1;
+            v.k = P.k;
+            v.a[0] = external_amplitude(P.external_type, t0, P.Am, P.k);
+            v.a[1] = external_amplitude(P.external_type, t1, P.Am, P.k);
+            v.omt[0] = P.omega * t0;
+            v.omt[1] = P.omega * t1;
+        }
+        S.drive_h[k] = v;
+    }
+    FS_HIP_CHECK(hipMemcpyAsync(S.drive_d, S.drive_h.data(), sizeof(DriveDev) * S.drive_h.size(),
+                                hipMemcpyHostToDevice, S.stream()));
+    return FS_OK;
+}
+
+// E at every point: the batched direct sums over all direct-sum decks
+// (periodic and open), then the treecode decks one by one, then the drives
+// (slot: set_drive).
+int field_full(Sim& S, double* out, const double* x, int slot) {
     FS_TRY(S.ffull.run(x, S.qw, out, S.stream()));
     if (S.any_treecode)
         for (auto& D : S.decks)
-            if (D->prm.use_treecode > 0)
+            if (uses_treecode(D->prm))
                 FS_TRY(treecode_deck(S, *D, out + D->off, x + D->off, S.qw + D->off, D->n));
+    if (S.any_drive && S.N > 0) {
+        drive_kernel<<<blocks(S.N), kBlock, 0, S.stream()>>>(S.N, S.sid, S.drive_d, slot, x, out);
+        FS_HIP_CHECK(hipGetLastError());
+    }
     return FS_OK;
 }
 
@@ -338,8 +407,14 @@ int field_reduced(Sim& S) {
     FS_TRY(S.fred.run(S.rx, S.rw, S.re, st));
     if (S.any_treecode)
         for (auto& D : S.decks)
-            if (D->prm.use_treecode > 0)
+            if (uses_treecode(D->prm))
                 FS_TRY(treecode_deck(S, *D, S.re + D->uoff, S.rx + D->uoff, S.rw + D->uoff, D->u));
+    // the drive at the reduced points (slot 1), before they are scattered
+    // (AMRSimulation_field.cpp:70-72)
+    if (S.any_drive && S.U > 0) {
+        drive_kernel<<<blocks(S.U), kBlock, 0, st>>>(S.U, S.rsid, S.drive_d, 1, S.rx, S.re);
+        FS_HIP_CHECK(hipGetLastError());
+    }
     scatter_reduced<<<blocks(S.N), kBlock, 0, st>>>(S.N, S.r_uof, S.re, S.es);
     FS_HIP_CHECK(hipGetLastError());
     return FS_OK;
@@ -351,13 +426,13 @@ int rk4(Sim& S) {
     const int N = S.N;
     if (N == 0) return FS_OK;
     rk4_stage1<<<blocks(N), kBlock, 0, st>>>(N, S.sid, S.sinfo, S.xs, S.ps, S.es, S.p2, S.xk);
-    FS_TRY(field_full(S, S.e2, S.xk));
+    FS_TRY(field_full(S, S.e2, S.xk, 0));
     rk4_stage2<<<blocks(N), kBlock, 0, st>>>(N, S.sid, S.sinfo, S.xs, S.ps, S.e2, S.p2, S.p3,
                                              S.xk);
-    FS_TRY(field_full(S, S.e3, S.xk));
+    FS_TRY(field_full(S, S.e3, S.xk, 0));
     rk4_stage3<<<blocks(N), kBlock, 0, st>>>(N, S.sid, S.sinfo, S.xs, S.ps, S.e3, S.p3, S.p4,
                                              S.xk);
-    FS_TRY(field_full(S, S.e4, S.xk));
+    FS_TRY(field_full(S, S.e4, S.xk, 1));
     rk4_final<<<blocks(N), kBlock, 0, st>>>(N, S.sid, S.sinfo, S.xs, S.ps, S.es, S.e2, S.e3,
                                             S.e4, S.p2, S.p3, S.p4);
     FS_HIP_CHECK(hipGetLastError());
@@ -445,7 +520,7 @@ int layout(Sim& S, bool keep_resident) {
     hipStream_t st = S.stream();
     drop_graphs(S);
     S.sp.clear();
-    S.any_adaptive = S.any_treecode = false;
+    S.any_adaptive = S.any_treecode = S.any_drive = false;
     int N = 0;
     for (auto& D : S.decks) {
         D->off = N;
@@ -455,7 +530,8 @@ int layout(Sim& S, bool keep_resident) {
         }
         D->n = N - D->off;
         S.any_adaptive = S.any_adaptive || D->adaptive;
-        S.any_treecode = S.any_treecode || D->prm.use_treecode > 0;
+        S.any_treecode = S.any_treecode || uses_treecode(D->prm);
+        S.any_drive = S.any_drive || D->prm.use_external_field > 0;
     }
     S.N = N;
     const int nsp = (int)S.sp.size();
@@ -535,13 +611,14 @@ int layout(Sim& S, bool keep_resident) {
     // reduced map: each deck's species' unique x concatenated (duplicates
     // across species kept, AMRSimulation_field.cpp:9-13)
     std::vector<double> rxh;
-    std::vector<int32_t> off(1, 0), mem, uof(std::max(N, 1));
-    int ubase = 0;
+    std::vector<int32_t> off(1, 0), mem, uof(std::max(N, 1)), rsid;
+    int ubase = 0, kspec = 0;
     for (auto& D : S.decks) {
         D->uoff = ubase;
         for (auto& sp : D->sp) {
             const Species& s = *sp;
             rxh.insert(rxh.end(), s.red.ux.begin(), s.red.ux.end());
+            rsid.insert(rsid.end(), s.red.ux.size(), kspec++);
             for (size_t u = 0; u < s.red.ux.size(); ++u) {
                 for (int m = s.red.offsets[u]; m < s.red.offsets[u + 1]; ++m)
                     mem.push_back(s.off + s.red.members[m]);
@@ -556,15 +633,22 @@ int layout(Sim& S, bool keep_resident) {
     const size_t bU = align_up(sizeof(double) * std::max(S.U, 1), 256);
     const size_t bo = align_up(sizeof(int32_t) * (S.U + 1), 256);
     const size_t bm = align_up(sizeof(int32_t) * std::max(N, 1), 256);
-    FS_TRY(S.red.ensure(3 * bU + bo + 2 * bm, st));
+    FS_TRY(S.red.ensure(3 * bU + bo + 3 * bm, st));
     S.rx = S.red.as<double>(0);
     S.rw = S.red.as<double>(bU);
     S.re = S.red.as<double>(2 * bU);
     S.r_off = S.red.as<int32_t>(3 * bU);
     S.r_mem = S.red.as<int32_t>(3 * bU + bo);
     S.r_uof = S.red.as<int32_t>(3 * bU + bo + bm);
-    if (S.U > 0)
+    S.rsid = S.red.as<int32_t>(3 * bU + bo + 2 * bm);  // U <= N
+    if (S.U > 0) {
         FS_HIP_CHECK(hipMemcpyAsync(S.rx, rxh.data(), sizeof(double) * S.U, hipMemcpyHostToDevice, st));
+        FS_HIP_CHECK(hipMemcpyAsync(S.rsid, rsid.data(), sizeof(int32_t) * S.U,
+                                    hipMemcpyHostToDevice, st));
+    }
+    S.drive_h.assign(std::max(nsp, 1), DriveDev{});
+    FS_TRY(S.drive.ensure(sizeof(DriveDev) * S.drive_h.size(), st));
+    S.drive_d = S.drive.as<DriveDev>();
     FS_HIP_CHECK(hipMemcpyAsync(S.r_off, off.data(), sizeof(int32_t) * (S.U + 1),
                                 hipMemcpyHostToDevice, st));
     if (N > 0) {
@@ -614,7 +698,8 @@ int layout(Sim& S, bool keep_resident) {
         a.off = D.off;
         a.n = D.n;
         a.src = (int)d;
-        a.eval = D.prm.use_treecode <= 0;
+        a.eval = !uses_treecode(D.prm);
+        a.open = open_field(D.prm);
         a.L = D.Lx();
         a.eps = D.prm.greens_epsilon;
         full.push_back(a);
@@ -805,7 +890,7 @@ int check_deck(const fs_sim_params* prm, const fs_species_params* species, int n
     if (n_species < 1 || n_species > 64) return fail(FS_E_INVALID, "deck: 1..64 species");
     if (!(prm->x_max > prm->x_min) || !(prm->dt > 0) || prm->remesh_period < 1)
         return fail(FS_E_INVALID, "deck: bad domain, dt or remesh_period");
-    if (prm->bcs != 0) return fail(FS_E_INVALID, "deck: only periodic decks (bcs 0)");
+    if (prm->bcs != 0 && prm->bcs != 1) return fail(FS_E_INVALID, "deck: bcs is 0 (periodic) or 1 (open)");
     for (int k = 0; k < n_species; ++k) {
         const fs_species_params& p = species[k];
         if (p.initial_height < 1 || p.initial_height > 12 || p.p_height < 0 ||
@@ -859,7 +944,7 @@ int download_species(Sim& S, int k, double* xs, double* ps, double* fs, double* 
 int step_body(Sim& S, bool remesh_now) {
     FS_TRY(rk4(S));
     if (remesh_now) return remesh_static(S);
-    return field_full(S, S.es, S.xs);
+    return field_full(S, S.es, S.xs, 1);
 }
 
 int capture(Sim& S, bool remesh_now, hipGraphExec_t& exec) {
@@ -987,7 +1072,7 @@ size_t deck_footprint(const Deck* d, int precision) {
     // FP64, the pair-once partials of the deck's full field (about n^2 / 512
     // bytes, fs_sym.h)
     size_t b = 0;
-    if (precision == FS_PRECISION_F64 && d->prm.use_treecode <= 0)
+    if (precision == FS_PRECISION_F64 && !uses_treecode(d->prm))
         b += sym::batch_bytes(deck_points(d), FieldTuning{});
     for (const auto& s : d->sp) {
         const size_t n = (size_t)s->n(), P = (size_t)s->npanels(), g = (size_t)(s->nx * s->np);
@@ -1025,6 +1110,10 @@ int attach_decks(fs_sim* S, std::vector<Deck*>& decks, std::vector<int>* indices
             drop_all();
             return fail(FS_E_INVALID, "decks joining a running batch remesh every step");
         }
+        if (S->precision == FS_PRECISION_MIXED && open_field(d->prm)) {
+            drop_all();
+            return fail(FS_E_INVALID, "open-boundary decks run in FP64 only");
+        }
     }
     if (hipSetDevice(S->ctx->device) != hipSuccess) {
         drop_all();
@@ -1039,6 +1128,7 @@ int attach_decks(fs_sim* S, std::vector<Deck*>& decks, std::vector<int>* indices
     renumber(*S);
     S->tune_pending = true;
     int rc = layout(*S, true);
+    if (rc == FS_OK) rc = set_drive(*S, false);
     if (rc == FS_OK) rc = field_reduced(*S);
     if (rc == FS_OK && hipStreamSynchronize(S->ctx->stream) != hipSuccess)
         rc = fail(FS_E_HIP, "attach: stream synchronization failed");
@@ -1131,6 +1221,7 @@ int fs_batch_create(fs_ctx* ctx, const fs_deck* decks, int n_decks, int precisio
     S->remesh_period = S->decks[0]->prm.remesh_period;
     sim::renumber(*S);
     FS_TRY(sim::layout(*S, false));
+    FS_TRY(sim::set_drive(*S, false));
     FS_TRY(sim::field_reduced(*S));  // AMRSimulation.cpp:84-89
     FS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     *out = S.release();
@@ -1213,9 +1304,17 @@ int fs_sim_set_options(fs_sim* S, int precision, int use_graph) {
     FS_HIP_CHECK(hipSetDevice(S->ctx->device));
     S->use_graph = use_graph != 0;
     if (precision != S->precision) {
+        const int before = S->precision;
         S->precision = precision;
         S->tune_pending = true;
-        FS_TRY(sim::layout(*S, true));
+        const int rc = sim::layout(*S, true);
+        if (rc != FS_OK) {  // e.g. open-boundary decks asked to run mixed
+            const std::string why = fs_last_error();
+            S->precision = before;
+            S->tune_pending = true;
+            (void)sim::layout(*S, true);
+            return fail(rc, "%s", why.c_str());
+        }
     } else {
         sim::drop_graphs(*S);
     }
@@ -1237,6 +1336,7 @@ int fs_sim_destroy(fs_sim* S) {
     S->ffull.release();
     S->fred.release();
     S->tree_work.release();
+    S->drive.release();
     S->diag_work.release();
     delete S;
     return FS_OK;
@@ -1247,6 +1347,7 @@ int fs_sim_step(fs_sim* S) {
     FS_HIP_CHECK(hipSetDevice(S->ctx->device));
     const bool remesh_now = ((S->iter + 1) % S->remesh_period) == 0;
     const bool eager = S->any_adaptive || S->any_treecode || !S->use_graph;
+    FS_TRY(sim::set_drive(*S, true));
     if (eager) {
         FS_TRY(sim::rk4(*S));
         if (remesh_now) {
@@ -1256,7 +1357,7 @@ int fs_sim_step(fs_sim* S) {
                 FS_TRY(sim::remesh_static(*S));
             }
         } else {
-            FS_TRY(sim::field_full(*S, S->es, S->xs));
+            FS_TRY(sim::field_full(*S, S->es, S->xs, 1));
         }
     } else {
         hipGraphExec_t& g = remesh_now ? S->g_remesh : S->g_plain;

@@ -51,6 +51,8 @@ class SimParams(ctypes.Structure):
         ("quadrature", ctypes.c_int), ("greens_epsilon", _d),
         ("use_treecode", ctypes.c_int), ("mac", _d), ("degree", ctypes.c_int),
         ("max_source", ctypes.c_int), ("dt", _d), ("remesh_period", ctypes.c_int),
+        ("use_external_field", ctypes.c_int), ("external_type", ctypes.c_int), ("Am", _d),
+        ("k", _d), ("omega", _d),
     ]
 
 
@@ -306,6 +308,11 @@ class Deck:
     max_source: int = 2000
     dt: float = 0.1
     remesh_period: int = 1
+    use_external_field: int = 0
+    external_type: int = 0
+    Am: float = 0.0
+    k: float = 0.0
+    omega: float = 0.0
     species: List[Species] = field(default_factory=list)
 
     def to_c(self) -> SimParams:
@@ -314,6 +321,8 @@ class Deck:
         p.greens_epsilon, p.use_treecode = self.greens_epsilon, self.use_treecode
         p.mac, p.degree, p.max_source = self.mac, self.degree, self.max_source
         p.dt, p.remesh_period = self.dt, self.remesh_period
+        p.use_external_field, p.external_type = self.use_external_field, self.external_type
+        p.Am, p.k, p.omega = self.Am, self.k, self.omega
         return p
 
 

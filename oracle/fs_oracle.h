@@ -91,6 +91,9 @@ typedef struct orc_sim_params {
     int degree, max_source;
     double dt;
     int remesh_period;
+    int use_external_field;     /* AMRSimulation_load.cpp:99-136 */
+    int external_type;          /* 1 sine, 2 logistic, else tanh */
+    double Am, k, omega;
 } orc_sim_params;
 
 /* ---- one species' mesh (part B) ------------------------------------------ */

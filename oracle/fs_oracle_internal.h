@@ -103,6 +103,8 @@ bool treecode(double* es, const double* s, const double* w, int64_t ns,
 void direct_sum_open(double* es, const double* t, int64_t nt, const double* s,
                      const double* w, int64_t ns, double eps);
 double external_amplitude(int type, double t, double Am, double k);
+void external_add(double* es, const double* targets, int64_t nt, double t, int type, double Am,
+                  double k, double omega);
 void external_add(double* es, const double* x, int64_t nt, double t, int type, double Am,
                   double k, double omega);
 

@@ -1,8 +1,7 @@
 // Test infrastructure only: FP64 CPU restatement of the FARRSIGHT step loop
 // (src/simulation_files/AMRSimulation_step.cpp, _field.cpp, _remesh.cpp,
-// _gather_scatter.cpp) and the C API of the oracle. External drive fields and
-// file output are not part of it (disabled in every deck this repository
-// uses; the reduced-x output directories are taken to exist, so
+// _gather_scatter.cpp) and the C API of the oracle. File output is not part
+// of it (the reduced-x output directories are taken to exist, so
 // evaluate_field_uniform_grid never takes its early return,
 // AMRSimulation_field.cpp:91-94).
 #include <algorithm>
@@ -28,17 +27,40 @@ struct Sim {
 
     double Lx() const { return prm.x_max - prm.x_min; }
 
-    // AMRSimulation_field.cpp:164-173 with (*calculate_e) from
-    // make_field_return_ptr (AMRSimulation_load.cpp:49-96, periodic decks).
-    void field(std::vector<double>& e, const std::vector<double>& x,
-               const std::vector<double>& w) {
+    // (*calculate_e) from make_field_return_ptr (AMRSimulation_load.cpp:49-96):
+    // periodic decks the direct sum or the treecode; open decks the
+    // open-boundary direct sum, or the periodic direct sum when use_treecode
+    // is set (:61-73).
+    void field_only(std::vector<double>& e, const std::vector<double>& x,
+                    const std::vector<double>& w) {
         std::vector<double> xc(x), xc2(x);
-        if (prm.use_treecode > 0)
-            treecode(e.data(), xc2.data(), w.data(), (int64_t)xc.size(), Lx(),
-                     prm.greens_epsilon, prm.mac, prm.degree, prm.max_source);
-        else
+        const bool open = prm.bcs != 0;
+        if (prm.use_treecode > 0 && !open) {
+            if (!treecode(e.data(), xc2.data(), w.data(), (int64_t)xc.size(), Lx(),
+                          prm.greens_epsilon, prm.mac, prm.degree, prm.max_source))
+                throw std::runtime_error("treecode: more than max_source coincident points");
+        } else if (open && prm.use_treecode <= 0) {
+            direct_sum_open(e.data(), xc.data(), (int64_t)e.size(), xc2.data(), w.data(),
+                            (int64_t)xc.size(), prm.greens_epsilon);
+        } else {
             direct_sum(e.data(), xc.data(), (int64_t)e.size(), xc2.data(), w.data(),
                        (int64_t)xc.size(), Lx(), prm.greens_epsilon);
+        }
+    }
+
+    // (*calculate_e_external) at time tt (AMRSimulation_field.cpp:70-72,
+    // 169-171; make_external_field, AMRSimulation_load.cpp:99-136)
+    void drive(std::vector<double>& e, const std::vector<double>& x, double tt) {
+        if (prm.use_external_field > 0)
+            external_add(e.data(), x.data(), (int64_t)e.size(), tt, prm.external_type, prm.Am,
+                         prm.k, prm.omega);
+    }
+
+    // AMRSimulation_field.cpp:164-173
+    void field(std::vector<double>& e, const std::vector<double>& x,
+               const std::vector<double>& w, double tt) {
+        field_only(e, x, w);
+        drive(e, x, tt);
     }
 
     // AMRSimulation_field.cpp:5-162
@@ -50,7 +72,7 @@ struct Sim {
             rw.insert(rw.end(), s->reduced_ws.begin(), s->reduced_ws.end());
         }
         std::vector<double> re(rw.size());
-        field(re, rx, rw);
+        field(re, rx, rw, t);
         size_t off = 0;
         for (auto& s : species) {
             s->get_reduced_es(re.data() + off);
@@ -110,21 +132,22 @@ struct Sim {
                 p2[i] = ps[i] + 0.5 * dt * ef1[i];
                 xk[i] = xs[i] + 0.5 * dt * p1[i];
             }
-        field(ef2, xk, q_ws);
+        const double t2 = t + dt * 0.5, t3 = t2, t4 = t + dt;   // :170, 201, 224
+        field(ef2, xk, q_ws, t2);
         for (size_t sp = 0; sp < species.size(); ++sp)
             for (size_t i = start[sp]; i < end[sp]; ++i) {
                 ef2[i] *= qms[sp];
                 p3[i] = ps[i] + 0.5 * dt * ef2[i];
                 xk[i] = xs[i] + 0.5 * dt * p2[i];
             }
-        field(ef3, xk, q_ws);
+        field(ef3, xk, q_ws, t3);
         for (size_t sp = 0; sp < species.size(); ++sp)
             for (size_t i = start[sp]; i < end[sp]; ++i) {
                 ef3[i] *= qms[sp];
                 p4[i] = ps[i] + dt * ef3[i];
                 xk[i] = xs[i] + dt * p3[i];
             }
-        field(ef4, xk, q_ws);
+        field(ef4, xk, q_ws, t4);
         for (size_t sp = 0; sp < species.size(); ++sp)
             for (size_t i = start[sp]; i < end[sp]; ++i) ef4[i] *= qms[sp];
         for (size_t i = 0; i < N; ++i) {
@@ -146,7 +169,7 @@ struct Sim {
             field_uniform_grid();
             need_gather = true;
         } else {
-            field(es, xs, q_ws);
+            field(es, xs, q_ws, t);
         }
     }
 

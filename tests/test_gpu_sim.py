@@ -23,7 +23,10 @@ def to_ms(deck: oracle.Deck):
                         quadrature=deck.quadrature, greens_epsilon=deck.greens_epsilon,
                         use_treecode=deck.use_treecode, mac=deck.mac, degree=deck.degree,
                         max_source=deck.max_source, dt=deck.dt,
-                        remesh_period=deck.remesh_period)
+                        remesh_period=deck.remesh_period,
+                        use_external_field=deck.use_external_field,
+                        external_type=deck.external_type, Am=deck.Am, k=deck.k,
+                        omega=deck.omega)
     sps = [ms.species_params(**{k: getattr(s, k) for k in (
         "q", "m", "p_min", "p_max", "initial_height", "p_height", "max_height",
         "do_adaptively_refine", "amr_epsilons", "ics_type", "normalized_wavenumber", "amp",
