@@ -57,6 +57,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "direct_f64_n1048576_traffic.txt")
 
 
 MIXED_VALU_FILE = os.path.join(ROOT, "profiles", "mixed_field_n1048576_valu_counters.md")
+REMESH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c4_remesh_hbm.md")
+HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E
 SIMDS = 256 * 4          # MI355X: 256 CUs of 4 SIMDs
 CLOCK_HZ = 2.4e9         # peak engine clock
 
@@ -728,11 +730,29 @@ def c4_block(ctx, ndecks: int, steps: int, warmup: int, compare: bool = True,
         n_k, p_k = b.counts(k)
         remesh_bytes += 24 * n_k + 24 * n_k + 48 * p_k
     lat = np.array(lat) * 1e3
+    # the remesh's device work timed live with HIP events (three eager steps
+    # after the timed ones), against its algorithmic bytes
+    b.set_timing(True)
+    rms = []
+    for _ in range(3):
+        b.step()
+        rms.append(b.remesh_ms())
+    b.set_timing(False)
+    remesh_ms = float(np.median(rms))
+    traffic = read_profile_value(REMESH_TRAFFIC_FILE, "remesh_counted_traffic_bytes_per_step")
     out = {
         "value": value, "unit": "point-steps/s", "steps": steps, "decks": ndecks,
         "points": int(npts), "ms_per_step": wall / steps * 1e3,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "remesh_algorithmic_bytes_per_step": remesh_bytes,
+        "remesh_roofline": {
+            "bound": "hbm", "achieved": remesh_bytes / (remesh_ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": remesh_bytes / (remesh_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+            "traffic": traffic, "ms": remesh_ms,
+            "source": "HIP events around the remesh kernels of 3 eager steps; traffic from "
+                      + os.path.relpath(REMESH_TRAFFIC_FILE, ROOT),
+        },
         "workload": C4_WORKLOAD,
     }
     if not compare:

@@ -352,6 +352,15 @@ typedef struct fs_deck {
     int n_species;
 } fs_deck;
 
+/* Times the remesh's device work (the grid interpolation of every species
+ * and the reset of the points to the grid: shift, row sort, walks,
+ * coefficients, evaluation; not the reduced-x field) with HIP events on the
+ * step's stream. on = 1 runs later steps eagerly (no graph replay) and
+ * records the events; fs_sim_remesh_ms gives the last step's figure when
+ * that step remeshed. No counterpart in the reference. */
+int fs_sim_set_timing(fs_sim* sim, int on);
+int fs_sim_remesh_ms(fs_sim* sim, double* ms);
+
 /* Host-only: the device bytes a deck is estimated to take in a batch or a
  * server (state, RK4 work, field partials -- in FP64 the pair-once partials of
  * its full field --, sorted copies in mixed precision, panel records); the

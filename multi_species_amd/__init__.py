@@ -67,6 +67,8 @@ def lib():
         L.fs_emq_set_mixed_target.argtypes = [_vp, _d]
         L.fs_emq_periodic_mixed_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d,
                                                 _vp]
+        L.fs_sim_set_timing.argtypes = [_vp, _int]
+        L.fs_sim_remesh_ms.argtypes = [_vp, ctypes.POINTER(ctypes.c_double)]
         L.fs_emq_treecode_f64.argtypes = [_vp, _dp, _dp, _dp, _i64, _d, _d, _d, _int, _int,
                                           ctypes.POINTER(_i64)]
         _ip = ctypes.POINTER(ctypes.c_int32)
@@ -561,6 +563,16 @@ class Simulation:
         _check(lib().fs_sim_deck_info(self.h, deck, ctypes.byref(a), ctypes.byref(b),
                                       ctypes.byref(c), ctypes.byref(n)), "fs_sim_deck_info")
         return a.value, b.value, c.value, n.value
+
+    def set_timing(self, on: bool = True):
+        """Eager steps with HIP events around the remesh's device work."""
+        _check(lib().fs_sim_set_timing(self.h, int(bool(on))), "fs_sim_set_timing")
+
+    def remesh_ms(self) -> float:
+        """Device time of the last step's remesh (set_timing first)."""
+        out = ctypes.c_double()
+        _check(lib().fs_sim_remesh_ms(self.h, ctypes.byref(out)), "fs_sim_remesh_ms")
+        return out.value
 
     def set_options(self, precision: int = PRECISION_F64, use_graph: bool = True):
         _check(lib().fs_sim_set_options(self.h, precision, int(bool(use_graph))),

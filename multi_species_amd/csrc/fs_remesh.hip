@@ -6,16 +6,14 @@
 // (AMRStructure_interp.cpp:1094-1143). Kernels:
 //   coef     one thread per old panel: the 9x9 monomial system of
 //            AMRStructure_interp.cpp:923-935 solved by LU with partial
-//            pivoting (per-thread matrix in LDS, lane-interleaved so the 64
-//            lanes of a wavefront hit 64 different banks); stores the 9
-//            coefficients and the panel centre.
+//            pivoting (the matrix in registers: every index static); stores
+//            the 9 coefficients and the panel centre.
 //   records  one thread per old panel: packs what one leaf-search step reads
 //            (advected corners, the neighbour to step to in each direction,
-//            boundary flags) into one 96-byte record, and for refined panels
-//            the root-descent record (all 9 points, the children's
-//            neighbours, which step targets are refined) into 192 bytes, so
-//            every search step is a single dependent load (a descent ends on
-//            the leaf's walk record).
+//            boundary flags) into one walk record, and for refined panels
+//            the root-descent record (all 9 points, the children, which
+//            step targets are refined), so every search step is a single
+//            dependent load (a descent ends on the leaf's walk record).
 //   shift    one thread per new point: the periodic pre-shift of
 //            AMRStructure_interp.cpp:429-491.
 //   rowsort  one workgroup per grid row: rank sort of the row by shifted x.
@@ -53,6 +51,23 @@ constexpr int kMaxDescend = 256;
 // line elements per root descent in the guess phase of a row (the first
 // column, one workgroup for the whole line, descends for every element)
 constexpr int kRowAnchor = 4;
+
+// Workgroups go to the chip's 8 XCDs (each with its own L2) round-robin by
+// workgroup id. With the grid's x size a multiple of 8 (the launches of the
+// row walks and the grid evaluation pad it), block x of a job runs on XCD
+// x % 8; this maps a job's n blocks so that each XCD takes a contiguous
+// range of them -- neighbouring rows and points, which read the same walk
+// and coefficient records, share an L2 -- and the XCDs stay balanced within
+// every job. -1 for the padding blocks (the whole workgroup returns). (The
+// per-panel kernels ran slower mapped this way: their neighbour reads are
+// not local in the panel order.)
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int x, int n) {
+    const int xcd = x % kXcds, k = x / kXcds;
+    const int q = n / kXcds, r = n % kXcds;
+    if (k >= q + (xcd < r ? 1 : 0)) return -1;
+    return xcd * q + min(xcd, r) + k;
+}
 
 __global__ __launch_bounds__(64) void coef_kernel(const Job* __restrict__ jobs) {
     const Job& J = jobs[blockIdx.y];
@@ -644,8 +659,8 @@ __global__ __launch_bounds__(kLineThreads) void rows_kernel(const Job* __restric
     __shared__ int s_guess[kMaxLine], s_leaf[kMaxLine], s_first, s_fail;
     __shared__ int ext[kMaxWalkSteps];
     const Job& J = jobs[blockIdx.y];
-    const int r = blockIdx.x;
-    if (r >= J.np) return;
+    const int r = xcd_block(blockIdx.x, J.np);
+    if (r < 0) return;
     const bool periodic = J.periodic != 0;
     const int nx = J.nx;
     int err = 0;
@@ -682,7 +697,9 @@ __global__ __launch_bounds__(kLineThreads) void rows_kernel(const Job* __restric
 __global__ __launch_bounds__(256) void eval_grid_kernel(const Job* __restrict__ jobs) {
     const Job& J = jobs[blockIdx.y];
     const int n = J.nx * J.np;
-    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int bx = xcd_block(blockIdx.x, (n + 255) / 256);
+    if (bx < 0) return;
+    const int k = bx * 256 + threadIdx.x;
     if (k >= n) return;
     const int leaf = J.leaf_of[k];
     double v;
@@ -776,6 +793,8 @@ void carve_grid(Job& j, void* base, int n, int np) {
 }
 
 static unsigned cdiv(int a, int b) { return (unsigned)((a + b - 1) / b); }
+// grid x sizes of the XCD-mapped kernels: a multiple of the XCD count
+static unsigned pad8(unsigned a) { return (a + kXcds - 1) / kXcds * kXcds; }
 
 int clear_status(const Job* d_jobs, const Launch& l, hipStream_t stream) {
     clear_status_kernel<<<cdiv(l.njobs, 64), 64, 0, stream>>>(d_jobs, l.njobs);
@@ -799,8 +818,8 @@ int interp_grids(const Job* d_jobs, const Launch& l, hipStream_t stream) {
     shift_kernel<<<dim3(cdiv(l.max_points, 256), nj), 256, 0, stream>>>(d_jobs);
     rowsort_kernel<<<dim3((unsigned)l.max_rows, nj), 256, 0, stream>>>(d_jobs);
     column_kernel<<<dim3(1, nj), kLineThreads, 0, stream>>>(d_jobs);
-    rows_kernel<<<dim3((unsigned)l.max_rows, nj), kLineThreads, 0, stream>>>(d_jobs);
-    eval_grid_kernel<<<dim3(cdiv(l.max_points, 256), nj), 256, 0, stream>>>(d_jobs);
+    rows_kernel<<<dim3(pad8((unsigned)l.max_rows), nj), kLineThreads, 0, stream>>>(d_jobs);
+    eval_grid_kernel<<<dim3(pad8(cdiv(l.max_points, 256)), nj), 256, 0, stream>>>(d_jobs);
     FS_HIP_CHECK(hipGetLastError());
     return FS_OK;
 }

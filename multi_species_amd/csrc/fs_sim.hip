@@ -331,6 +331,11 @@ struct fs_sim {
     std::vector<fs::sim::DriveDev> drive_h;
     int* d_status = nullptr;
     hipGraphExec_t g_remesh = nullptr, g_plain = nullptr;
+    // remesh timing (fs_sim_set_timing): events around the remesh's device
+    // work of eager steps
+    bool timing = false;
+    bool timed = false;  // the events of the last step enclose a remesh
+    hipEvent_t ev_r0 = nullptr, ev_r1 = nullptr;
 
     hipStream_t stream() const { return ctx->stream; }
 };
@@ -457,12 +462,17 @@ int read_status(Sim& S) {
 int remesh_static(Sim& S) {
     hipStream_t st = S.stream();
     if (S.sp.empty()) return FS_OK;
+    if (S.timing) FS_HIP_CHECK(hipEventRecord(S.ev_r0, st));
     FS_TRY(rm::clear_status(S.d_jobs, S.launch, st));
     FS_TRY(rm::prepare_old_meshes(S.d_jobs, S.launch, st));
     FS_TRY(rm::interp_grids(S.d_jobs, S.launch, st));
     reset_points_jobs<<<dim3(blocks(S.launch.max_points), (unsigned)S.sp.size()), kBlock, 0, st>>>(
         S.d_reset);
     FS_HIP_CHECK(hipGetLastError());
+    if (S.timing) {
+        FS_HIP_CHECK(hipEventRecord(S.ev_r1, st));
+        S.timed = true;
+    }
     return field_reduced(S);
 }
 
@@ -742,6 +752,7 @@ int layout(Sim& S, bool keep_resident) {
 // threads take turns on the stream), and is uploaded again.
 int remesh_adaptive(Sim& S) {
     hipStream_t st = S.stream();
+    if (S.timing) FS_HIP_CHECK(hipEventRecord(S.ev_r0, st));
     FS_TRY(rm::clear_status(S.d_jobs, S.launch, st));
     FS_TRY(rm::prepare_old_meshes(S.d_jobs, S.launch, st));
     FS_TRY(rm::interp_grids(S.d_jobs, S.launch, st));
@@ -749,6 +760,10 @@ int remesh_adaptive(Sim& S) {
     reset_points_jobs<<<dim3(blocks(S.launch.max_points), (unsigned)S.sp.size()), kBlock, 0, st>>>(
         S.d_reset);
     FS_HIP_CHECK(hipGetLastError());
+    if (S.timing) {
+        FS_HIP_CHECK(hipEventRecord(S.ev_r1, st));
+        S.timed = true;
+    }
     FS_TRY(read_status(S));
     std::vector<Species*> refine;
     for (Species* s : S.sp)
@@ -1321,6 +1336,30 @@ int fs_sim_set_options(fs_sim* S, int precision, int use_graph) {
     return FS_OK;
 }
 
+int fs_sim_set_timing(fs_sim* S, int on) {
+    if (!S) return fail(FS_E_INVALID, "null simulation");
+    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
+    if (on && !S->ev_r0) {
+        FS_HIP_CHECK(hipEventCreate(&S->ev_r0));
+        FS_HIP_CHECK(hipEventCreate(&S->ev_r1));
+    }
+    S->timing = on != 0;
+    S->timed = false;
+    return FS_OK;
+}
+
+int fs_sim_remesh_ms(fs_sim* S, double* ms) {
+    if (!S || !ms) return fail(FS_E_INVALID, "null argument");
+    if (!S->timing || !S->timed)
+        return fail(FS_E_INVALID, "fs_sim_remesh_ms: no timed remesh (fs_sim_set_timing, then a "
+                                  "remeshing step)");
+    FS_HIP_CHECK(hipEventSynchronize(S->ev_r1));
+    float t = 0.0f;
+    FS_HIP_CHECK(hipEventElapsedTime(&t, S->ev_r0, S->ev_r1));
+    *ms = t;
+    return FS_OK;
+}
+
 int fs_sim_destroy(fs_sim* S) {
     if (!S) return FS_OK;
     (void)hipStreamSynchronize(S->ctx->stream);
@@ -1337,6 +1376,8 @@ int fs_sim_destroy(fs_sim* S) {
     S->fred.release();
     S->tree_work.release();
     S->drive.release();
+    if (S->ev_r0) (void)hipEventDestroy(S->ev_r0);
+    if (S->ev_r1) (void)hipEventDestroy(S->ev_r1);
     S->diag_work.release();
     delete S;
     return FS_OK;
@@ -1346,7 +1387,8 @@ int fs_sim_step(fs_sim* S) {
     if (!S) return fail(FS_E_INVALID, "null simulation");
     FS_HIP_CHECK(hipSetDevice(S->ctx->device));
     const bool remesh_now = ((S->iter + 1) % S->remesh_period) == 0;
-    const bool eager = S->any_adaptive || S->any_treecode || !S->use_graph;
+    const bool eager = S->any_adaptive || S->any_treecode || !S->use_graph || S->timing;
+    S->timed = false;
     FS_TRY(sim::set_drive(*S, true));
     if (eager) {
         FS_TRY(sim::rk4(*S));
