@@ -11,6 +11,10 @@
 namespace fs {
 namespace tree {
 
+// Member points per moment chunk: a node's moments are summed chunk by
+// chunk (one workgroup each), then the chunks in order.
+constexpr int kMomChunk = 2048;
+
 // A node covers the sorted points [s0, s1) and the interval [x0, x1] (xc its
 // centre). Nodes are numbered level by level, the root 0; child c* and parent
 // are -1 when absent, leaf is the node's index in the leaf array or -1.
@@ -26,7 +30,8 @@ struct Node {
 // Sizes the build leaves in device memory (err: 0 ok, 1 a node of more than
 // max_source coincident points, 2 node capacity exceeded).
 struct Counts {
-    int nnodes, nleaf, err, pad;
+    int nnodes, nleaf, err;
+    int mom_items;  // moment chunks of kMomChunk member points over all non-root nodes
     long long far_total, near_total, blk_total;
     long long far_evals, near_evals;
 };
@@ -46,12 +51,14 @@ struct Built {
     const int32_t* near_off;
     const int32_t* near_list;
     const int4* blocks;            // [c.blk_total] (leaf, first, last, 0)
+    const int32_t* mom_off;        // [c.nnodes + 1] first moment chunk of each node
     // the evaluation's work arrays: tree-order positions fmod(x, L) and
     // weights, E in tree order, moments [c.nnodes][degree + 1]
     double* px;
     double* lam;
     double* e_tree;
     double* mom;
+    double* mompart;               // [c.mom_items][degree + 1] chunk partial moments
 };
 
 // Builds the tree of the n points d_x (input order) on `stream`: sort, tree,

@@ -111,6 +111,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(const uint64_t* __
                                                               int n, int max_source, int cap,
                                                               Node* __restrict__ nodes,
                                                               int32_t* __restrict__ leaves,
+                                                              int32_t* __restrict__ mom_off,
                                                               Counts* __restrict__ cnt) {
     __shared__ int s_wave[kWaves];
     const int t = threadIdx.x;
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(const uint64_t* __
         nodes[0] = make_node(a, b, 0, n, -1, 0);
     }
     __syncthreads();
-    int lb = 0, le = 1, nl = 0, err = 0;
+    int lb = 0, le = 1, nl = 0, err = 0, mom_run = 0;
     for (int depth = 0; lb < le && err == 0; ++depth) {
         int next = le;
         for (int c0 = lb; c0 < le; c0 += kBuildThreads) {
@@ -146,9 +147,14 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(const uint64_t* __
                     isleaf = 1;
                 }
             }
-            int tot_ch = 0, tot_lf = 0;
+            int tot_ch = 0, tot_lf = 0, tot_mc = 0;
             const int ex_ch = block_scan(nch, s_wave, tot_ch);
             const int ex_lf = block_scan(isleaf, s_wave, tot_lf);
+            // moment chunks of every node but the root, numbered in node order
+            const int nmc = (mine && node > 0) ? (nd.s1 - nd.s0 + kMomChunk - 1) / kMomChunk : 0;
+            const int ex_mc = block_scan(nmc, s_wave, tot_mc);
+            if (mine) mom_off[node] = mom_run + ex_mc;
+            mom_run += tot_mc;
             if (tot_ch > 0 && depth > kMaxDepth) {
                 err = 1;
                 break;
@@ -189,6 +195,8 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(const uint64_t* __
         cnt->nnodes = le;
         cnt->nleaf = nl;
         cnt->err = err;
+        cnt->mom_items = mom_run;
+        if (err == 0) mom_off[le] = mom_run;
     }
 }
 
@@ -375,7 +383,7 @@ int build_device(const double* d_x, int n, double L, double mac, int degree, int
     Counts* d_cnt = nullptr;
     uint64_t *key = nullptr, *skey = nullptr;
     int32_t *ord = nullptr, *perm = nullptr, *leaves = nullptr, *far_cnt = nullptr,
-            *near_cnt = nullptr, *blk_cnt = nullptr;
+            *near_cnt = nullptr, *blk_cnt = nullptr, *mom_off = nullptr;
     long long *fev = nullptr, *nev = nullptr;
     Node* nodes = nullptr;
     int2* range = nullptr;
@@ -384,7 +392,7 @@ int build_device(const double* d_x, int n, double L, double mac, int degree, int
     for (;;) {
         const size_t c = (size_t)cap;
         const size_t total = al(sizeof(Counts)) + 2 * b_k + 2 * b_i + b_sort + al(sizeof(Node) * c) +
-                             al(4 * c) + 3 * al(4 * (c + 1)) + 2 * al(8 * c) + al(8 * c) +
+                             al(4 * c) + 4 * al(4 * (c + 1)) + 2 * al(8 * c) + al(8 * c) +
                              al(8 * c * PP);
         FS_TRY(work.base.ensure(total, stream));
         char* p = work.base.as<char>();
@@ -399,6 +407,7 @@ int build_device(const double* d_x, int n, double L, double mac, int degree, int
         far_cnt = reinterpret_cast<int32_t*>(p); p += al(4 * (c + 1));
         near_cnt = reinterpret_cast<int32_t*>(p); p += al(4 * (c + 1));
         blk_cnt = reinterpret_cast<int32_t*>(p); p += al(4 * (c + 1));
+        mom_off = reinterpret_cast<int32_t*>(p); p += al(4 * (c + 1));
         fev = reinterpret_cast<long long*>(p); p += al(8 * c);
         nev = reinterpret_cast<long long*>(p); p += al(8 * c);
         range = reinterpret_cast<int2*>(p); p += al(8 * c);
@@ -408,7 +417,7 @@ int build_device(const double* d_x, int n, double L, double mac, int degree, int
         FS_HIP_CHECK(hipGetLastError());
         FS_TRY(radix_sort_full(key, n, skey, ord, rtmp, stream));
         build_kernel<<<1, kBuildThreads, 0, stream>>>(skey, n, max_source, (int)cap, nodes, leaves,
-                                                      d_cnt);
+                                                      mom_off, d_cnt);
         FS_HIP_CHECK(hipGetLastError());
         lists_kernel<false><<<grid_for(cap), kBlock, 0, stream>>>(
             nodes, leaves, d_cnt, L, std::sqrt(mac * mac), PP, far_cnt, near_cnt, blk_cnt, fev,
@@ -435,8 +444,9 @@ int build_device(const double* d_x, int n, double L, double mac, int degree, int
     const size_t b_far = al(4 * std::max<long long>(1, C.far_total)),
                  b_near = al(4 * std::max<long long>(1, C.near_total)),
                  b_blk = al(sizeof(int4) * std::max<long long>(1, C.blk_total)),
-                 b_mom = al(8 * (size_t)C.nnodes * PP), b_x = al(8 * (size_t)n);
-    FS_TRY(work.lists.ensure(b_far + b_near + b_blk + b_mom + 3 * b_x, stream));
+                 b_mom = al(8 * (size_t)C.nnodes * PP), b_x = al(8 * (size_t)n),
+                 b_mp = al(8 * (size_t)std::max(C.mom_items, 1) * PP);
+    FS_TRY(work.lists.ensure(b_far + b_near + b_blk + b_mom + 3 * b_x + b_mp, stream));
     char* q = work.lists.as<char>();
     int32_t* far_list = reinterpret_cast<int32_t*>(q); q += b_far;
     int32_t* near_list = reinterpret_cast<int32_t*>(q); q += b_near;
@@ -444,7 +454,8 @@ int build_device(const double* d_x, int n, double L, double mac, int degree, int
     out.mom = reinterpret_cast<double*>(q); q += b_mom;
     out.px = reinterpret_cast<double*>(q); q += b_x;
     out.lam = reinterpret_cast<double*>(q); q += b_x;
-    out.e_tree = reinterpret_cast<double*>(q);
+    out.e_tree = reinterpret_cast<double*>(q); q += b_x;
+    out.mompart = reinterpret_cast<double*>(q);
 
     lists_kernel<true><<<grid_for(C.nleaf), kBlock, 0, stream>>>(
         nodes, leaves, d_cnt, L, std::sqrt(mac * mac), PP, far_cnt, near_cnt, blk_cnt, fev, nev,
@@ -475,6 +486,7 @@ int build_device(const double* d_x, int n, double L, double mac, int degree, int
     out.near_off = near_cnt;
     out.near_list = near_list;
     out.blocks = blocks;
+    out.mom_off = mom_off;
     return FS_OK;
 }
 

@@ -3,9 +3,10 @@
 // Replaces E_MQ_Treecode::operator() (reference src/FieldStructure.cpp:
 // 230-1050). The tree, the MAC lists, the tree order and the Chebyshev points
 // are built on the device (fs_tree_build.hip); then
-//   moments  one workgroup per node: M_k = sum_i lambda_i l_k(x_i) with the
-//            barycentric weights (-1)^k delta_k and the exact-hit rule
-//            (:775-866); per-thread partials reduced in thread order.
+//   moments  one workgroup per chunk of a node's members: M_k =
+//            sum_i lambda_i l_k(x_i) with the barycentric weights
+//            (-1)^k delta_k and the exact-hit rule (:775-866); per-thread
+//            partials reduced in thread order, chunks in chunk order.
 //   sums     one wavefront per (leaf, 64 targets): far clusters
 //            (Call_BL, :904-961) then near leaves (Call_Ds, :963-1020);
 //            cluster points/moments and near sources are uniform across the
@@ -49,14 +50,31 @@ __device__ __forceinline__ double kern(double xi, double xj, double invL, double
     return z * (y - C);
 }
 
+// Modified Chebyshev moments (:775-866), one workgroup per chunk of
+// kMomChunk member points of a node (the node found by binary search over
+// the nodes' first chunks): M_k = sum_i lambda_i l_k(x_i) with the
+// barycentric weights (-1)^k delta_k and the exact-hit rule; per-thread
+// partials reduced in thread order into the chunk's partial moments. The
+// big nodes near the root spread over many workgroups.
 __global__ __launch_bounds__(256) void moments_kernel(const double* __restrict__ px,
                                                       const double* __restrict__ lam,
                                                       const int2* __restrict__ range,
-                                                      const double* __restrict__ cheb, int PP,
-                                                      double* __restrict__ mom) {
+                                                      const int32_t* __restrict__ mom_off,
+                                                      int nnodes, const double* __restrict__ cheb,
+                                                      int PP, double* __restrict__ mompart) {
     __shared__ double part[kMaxPP][256];
-    const int node = blockIdx.x + 1;  // the root has no moments
+    const int item = blockIdx.x;
+    // the node whose chunks contain this item: the last with mom_off <= item
+    int lo = 1, hi = nnodes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (mom_off[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    const int node = lo;
     const int2 r = range[node];
+    const int k0 = r.x + (item - mom_off[node]) * tree::kMomChunk;
+    const int k1 = min(r.y + 1, k0 + tree::kMomChunk);
     const double* tk = cheb + (size_t)node * PP;
     double acc[kMaxPP];
     double w1[kMaxPP];
@@ -67,7 +85,7 @@ __global__ __launch_bounds__(256) void moments_kernel(const double* __restrict__
         w1[j] = (j < PP) ? (((j % 2 == 0) ? 1.0 : -1.0) * ((j == 0 || j == PP - 1) ? 0.5 : 1.0))
                          : 0.0;
     }
-    for (int k = r.x + (int)threadIdx.x; k <= r.y; k += 256) {
+    for (int k = k0 + (int)threadIdx.x; k < k1; k += 256) {
         const double x = px[k];
         double a[kMaxPP];
         double sum = 0.0;
@@ -101,8 +119,21 @@ __global__ __launch_bounds__(256) void moments_kernel(const double* __restrict__
     if ((int)threadIdx.x < PP) {
         double s = 0.0;
         for (int t = 0; t < 256; ++t) s += part[threadIdx.x][t];
-        mom[(size_t)node * PP + threadIdx.x] = s;
+        mompart[(size_t)item * PP + threadIdx.x] = s;
     }
+}
+
+// A node's moments: its chunks' partial moments added in chunk order.
+__global__ __launch_bounds__(256) void moments_reduce_kernel(const int32_t* __restrict__ mom_off,
+                                                             int nnodes, int PP,
+                                                             const double* __restrict__ mompart,
+                                                             double* __restrict__ mom) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const int node = 1 + g / PP, j = g % PP;
+    if (node >= nnodes) return;
+    double s = 0.0;
+    for (int c = mom_off[node]; c < mom_off[node + 1]; ++c) s += mompart[(size_t)c * PP + j];
+    mom[(size_t)node * PP + j] = s;
 }
 
 // One wavefront per (leaf, up to 64 of its targets): blocks[b] = (leaf, first
@@ -175,8 +206,12 @@ int treecode_f64(const double* d_x, const double* d_w, int n, double L, double e
     const unsigned gb = (unsigned)((n + 255) / 256);
     gather_kernel<<<gb, 256, 0, stream>>>(d_x, T.perm, n, T.px, L, 1);
     gather_kernel<<<gb, 256, 0, stream>>>(d_w, T.perm, n, T.lam, L, 0);
-    if (T.c.nnodes > 1)
-        moments_kernel<<<T.c.nnodes - 1, 256, 0, stream>>>(T.px, T.lam, T.range, T.cheb, PP, T.mom);
+    if (T.c.nnodes > 1) {
+        moments_kernel<<<T.c.mom_items, 256, 0, stream>>>(T.px, T.lam, T.range, T.mom_off,
+                                                           T.c.nnodes, T.cheb, PP, T.mompart);
+        moments_reduce_kernel<<<(unsigned)(((int64_t)(T.c.nnodes - 1) * PP + 255) / 256), 256, 0,
+                                stream>>>(T.mom_off, T.c.nnodes, PP, T.mompart, T.mom);
+    }
     const double delta = eps / L, delta2 = delta * delta;
     const double scale = 0.5 * std::sqrt(1 + 4 * delta2);
     tree_sum_kernel<<<(unsigned)T.c.blk_total, 64, 0, stream>>>(
