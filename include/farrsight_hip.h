@@ -393,18 +393,21 @@ int fs_read_species_output(const char* sim_dir, const char* species_name, int it
                            int64_t* n, double* xs, double* ps, double* fs, int64_t* n_panels,
                            int32_t* panels);
 
-/* Creates a batch. Decks share remesh_period; precision FS_PRECISION_F64
+/* Creates a batch. Each deck remeshes at its own remesh_period (counted from
+ * its own first step): a step where only some decks remesh evaluates the
+ * full field for every deck and the remesh and reduced-x field for those;
+ * precision FS_PRECISION_F64
  * (the reference's method) or FS_PRECISION_MIXED (fs_emq_periodic_mixed's
  * method for every field evaluation). fs_sim_create is a batch of one deck
  * in FP64. */
 int fs_batch_create(fs_ctx* ctx, const fs_deck* decks, int n_decks, int precision,
                     fs_sim** out);
 /* Continuous batching: a deck joins (built on the host, uploaded; the others
- * keep their device state) or leaves between steps. Joining needs
- * remesh_period 1 for the batch and the deck. */
+ * keep their device state and field) or leaves between steps, whatever the
+ * decks' remesh periods. */
 int fs_sim_add_deck(fs_sim* sim, const fs_deck* deck, int* index);
 int fs_sim_remove_deck(fs_sim* sim, int deck);
-/* A batch with no decks yet (remesh every step), for decks that join
+/* A batch with no decks yet, for decks that join
  * through fs_sim_add_deck or fs_sim_restart_deck. */
 int fs_batch_create_empty(fs_ctx* ctx, int precision, fs_sim** out);
 /* AMRSimulation::write_to_file (AMRSimulation_io.cpp:7-17) and the reduced
@@ -421,7 +424,7 @@ int fs_sim_write_output(fs_sim* sim, int deck, const char* sim_dir,
  * weights recomputed from the tree and E evaluated on the reduced set. The
  * files must hold a state written right after a remesh (points on their
  * mesh: the weights of moved points are not in the files; refused
- * otherwise); the batch and the deck remesh every step. Continuing a deck this
+ * otherwise). Continuing a deck this
  * way gives bitwise the states of the run that wrote the files. */
 int fs_sim_restart_deck(fs_sim* sim, const fs_deck* deck, const char* sim_dir,
                         const char* const* species_names, int iter, int* index);
@@ -450,7 +453,7 @@ int fs_sim_field_governor(fs_sim* sim, int deck, double* max_e, double* errs, do
  * the estimated device footprint of the resident decks stays within
  * hbm_budget_bytes, steps all resident decks together, and retires finished
  * ones with their final state copied to the host. While a server runs, its
- * context must not be used by other calls. Decks must remesh every step. */
+ * context must not be used by other calls. */
 enum {
     FS_REQ_QUEUED = 0,
     FS_REQ_BUILDING = 1,

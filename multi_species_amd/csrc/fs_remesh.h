@@ -113,6 +113,9 @@ struct Job {
     // output
     double* f;
     int* status;
+    // when set, the job runs only while *on != 0 (a batch whose decks
+    // remesh at different periods switches its species on and off here)
+    const int* on;
 };
 
 // Sizes of the per-job work arrays.

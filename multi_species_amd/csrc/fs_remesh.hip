@@ -71,6 +71,7 @@ __device__ __forceinline__ int xcd_block(int x, int n) {
 
 __global__ __launch_bounds__(64) void coef_kernel(const Job* __restrict__ jobs) {
     const Job& J = jobs[blockIdx.y];
+    if (J.on && *J.on == 0) return;
     const int pi = blockIdx.x * 64 + threadIdx.x;
     if (pi >= J.npanels) return;
     const int32_t* __restrict__ rec = J.rec;
@@ -173,6 +174,7 @@ __global__ __launch_bounds__(64) void coef_kernel(const Job* __restrict__ jobs) 
 
 __global__ __launch_bounds__(256) void records_kernel(const Job* __restrict__ jobs) {
     const Job& J = jobs[blockIdx.y];
+    if (J.on && *J.on == 0) return;
     const int pi = blockIdx.x * 256 + threadIdx.x;
     if (pi >= J.npanels) return;
     const int32_t* __restrict__ rec = J.rec;
@@ -483,6 +485,7 @@ __device__ __forceinline__ RootCorners load_corners(const double* __restrict__ o
 
 __global__ __launch_bounds__(256) void shift_kernel(const Job* __restrict__ jobs) {
     const Job& J = jobs[blockIdx.y];
+    if (J.on && *J.on == 0) return;
     const int n = J.nx * J.np;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -497,6 +500,7 @@ __global__ __launch_bounds__(256) void shift_kernel(const Job* __restrict__ jobs
 __global__ __launch_bounds__(256) void rowsort_kernel(const Job* __restrict__ jobs) {
     __shared__ double keys[kMaxLine];
     const Job& J = jobs[blockIdx.y];
+    if (J.on && *J.on == 0) return;
     const int row = blockIdx.x;
     if (row >= J.np) return;
     const int nx = J.nx;
@@ -614,6 +618,7 @@ __global__ __launch_bounds__(kLineThreads) void column_kernel(const Job* __restr
     __shared__ int s_guess[kMaxLine], s_leaf[kMaxLine], s_first, s_fail;
     __shared__ int ext[kMaxWalkSteps];
     const Job& J = jobs[blockIdx.y];
+    if (J.on && *J.on == 0) return;
     const bool periodic = J.periodic != 0;
     const int np = J.np, nx = J.nx;
     int err = 0;
@@ -659,6 +664,7 @@ __global__ __launch_bounds__(kLineThreads) void rows_kernel(const Job* __restric
     __shared__ int s_guess[kMaxLine], s_leaf[kMaxLine], s_first, s_fail;
     __shared__ int ext[kMaxWalkSteps];
     const Job& J = jobs[blockIdx.y];
+    if (J.on && *J.on == 0) return;
     const int r = xcd_block(blockIdx.x, J.np);
     if (r < 0) return;
     const bool periodic = J.periodic != 0;
@@ -696,6 +702,7 @@ __global__ __launch_bounds__(kLineThreads) void rows_kernel(const Job* __restric
 // monomials times coefficients, summed in order.
 __global__ __launch_bounds__(256) void eval_grid_kernel(const Job* __restrict__ jobs) {
     const Job& J = jobs[blockIdx.y];
+    if (J.on && *J.on == 0) return;
     const int n = J.nx * J.np;
     const int bx = xcd_block(blockIdx.x, (n + 255) / 256);
     if (bx < 0) return;
@@ -720,6 +727,7 @@ __global__ __launch_bounds__(256) void eval_grid_kernel(const Job* __restrict__ 
 // interpolate_from_mesh (AMRStructure_interp.cpp:1094-1143) at n = nx points.
 __global__ __launch_bounds__(256) void points_kernel(const Job* __restrict__ jobs) {
     const Job& J = jobs[blockIdx.y];
+    if (J.on && *J.on == 0) return;
     const int n = J.nx;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;

@@ -292,8 +292,6 @@ int fs_server_submit(fs_server* S, const fs_deck* deck, int n_steps, int64_t* id
         return fail(FS_E_INVALID, "fs_server_submit: null argument");
     if (n_steps < 1 || deck->n_species < 1 || deck->n_species > 64)
         return fail(FS_E_INVALID, "fs_server_submit: n_steps >= 1 and 1..64 species");
-    if (deck->sim->remesh_period != 1)
-        return fail(FS_E_INVALID, "fs_server_submit: served decks remesh every step");
     auto r = std::make_unique<Request>();
     r->sim = *deck->sim;
     r->species.assign(deck->species, deck->species + deck->n_species);
@@ -318,8 +316,6 @@ int fs_server_submit_restart(fs_server* S, const fs_deck* deck, const char* sim_
         return fail(FS_E_INVALID, "fs_server_submit_restart: null argument");
     if (n_steps < 1 || deck->n_species < 1 || deck->n_species > 64)
         return fail(FS_E_INVALID, "fs_server_submit_restart: n_steps >= 1 and 1..64 species");
-    if (deck->sim->remesh_period != 1)
-        return fail(FS_E_INVALID, "fs_server_submit_restart: served decks remesh every step");
     if (iter < 0) return fail(FS_E_INVALID, "fs_server_submit_restart: negative step");
     auto r = std::make_unique<Request>();
     r->sim = *deck->sim;

@@ -153,10 +153,12 @@ struct ResetJob {
     double* ps;
     double* fs;
     double* qw;
+    const int* on;  // the species remeshes in this step (its deck's period)
 };
 
 __global__ __launch_bounds__(kBlock) void reset_points_jobs(const ResetJob* __restrict__ jobs) {
     const ResetJob& J = jobs[blockIdx.y];
+    if (*J.on == 0) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= J.n) return;
     const double f = J.fnew[i];
@@ -192,13 +194,16 @@ __global__ __launch_bounds__(64) void reduced_sum(int U, const int32_t* __restri
     if (threadIdx.x == 0) rw[u] = s;
 }
 
-// es[i] = reduced_es[unique(i)] (AMRStructure.cpp:191-195).
+// es[i] = reduced_es[unique(i)] (AMRStructure.cpp:191-195), at the points of
+// the species that remeshed (on[species]).
 __global__ __launch_bounds__(kBlock) void scatter_reduced(int n,
                                                           const int32_t* __restrict__ unique_of,
+                                                          const int32_t* __restrict__ sid,
+                                                          const int* __restrict__ on,
                                                           const double* __restrict__ re,
                                                           double* __restrict__ es) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || on[sid[i]] == 0) return;
     es[i] = re[unique_of[i]];
 }
 
@@ -301,7 +306,6 @@ struct fs_sim {
     int N = 0;
     int U = 0;
     int iter = 0;
-    int remesh_period = 1;
     int precision = FS_PRECISION_F64;
     bool use_graph = true;
     bool tune_pending = true;  // autotune the field launch shape at the next layout
@@ -331,6 +335,15 @@ struct fs_sim {
     std::vector<fs::sim::DriveDev> drive_h;
     int* d_status = nullptr;
     hipGraphExec_t g_remesh = nullptr, g_plain = nullptr;
+    // species that remesh in the current step (decks of different
+    // remesh_period share a batch): device flags read by the remesh jobs,
+    // the reset and the scatter of the reduced field; the host copy last
+    // written to the device
+    DevBuf onbuf;
+    int* d_on = nullptr;
+    std::vector<int> on_dev;
+    std::vector<char> deck_on;  // the decks that remesh in this step (empty: all)
+    hipGraphExec_t g_mixed = nullptr;
     // remesh timing (fs_sim_set_timing): events around the remesh's device
     // work of eager steps
     bool timing = false;
@@ -349,7 +362,22 @@ using Sim = ::fs_sim;
 void drop_graphs(Sim& S) {
     if (S.g_remesh) (void)hipGraphExecDestroy(S.g_remesh);
     if (S.g_plain) (void)hipGraphExecDestroy(S.g_plain);
-    S.g_remesh = S.g_plain = nullptr;
+    if (S.g_mixed) (void)hipGraphExecDestroy(S.g_mixed);
+    S.g_remesh = S.g_plain = S.g_mixed = nullptr;
+}
+
+// Switches the species of the decks with deck_on[d] on for the remesh and
+// the reduced field (all on when deck_on is empty); copies the flags to the
+// device when they changed.
+int set_active(Sim& S, const std::vector<char>& deck_on) {
+    std::vector<int> h(std::max<size_t>(S.sp.size(), 1), 1);
+    for (size_t k = 0; k < S.sp.size(); ++k)
+        h[k] = deck_on.empty() ? 1 : (deck_on[S.sp[k]->deck] ? 1 : 0);
+    if (h == S.on_dev) return FS_OK;
+    S.on_dev = h;
+    FS_HIP_CHECK(hipMemcpyAsync(S.d_on, S.on_dev.data(), sizeof(int) * S.on_dev.size(),
+                                hipMemcpyHostToDevice, S.stream()));
+    return FS_OK;
 }
 
 // E_MQ_Treecode on one deck's slice (tree, lists and sums on the device).
@@ -420,7 +448,7 @@ int field_reduced(Sim& S) {
         drive_kernel<<<blocks(S.U), kBlock, 0, st>>>(S.U, S.rsid, S.drive_d, 1, S.rx, S.re);
         FS_HIP_CHECK(hipGetLastError());
     }
-    scatter_reduced<<<blocks(S.N), kBlock, 0, st>>>(S.N, S.r_uof, S.re, S.es);
+    scatter_reduced<<<blocks(S.N), kBlock, 0, st>>>(S.N, S.r_uof, S.sid, S.d_on, S.re, S.es);
     FS_HIP_CHECK(hipGetLastError());
     return FS_OK;
 }
@@ -669,6 +697,10 @@ int layout(Sim& S, bool keep_resident) {
         for (auto& sp : D->sp) FS_TRY(upload_species(S, *D, *sp, sp->d_grid.ptr == nullptr || !keep_resident));
     FS_TRY(S.status.ensure(sizeof(int) * std::max(nsp, 1), st));
     S.d_status = S.status.as<int>();
+    FS_TRY(S.onbuf.ensure(sizeof(int) * std::max(nsp, 1), st));
+    S.d_on = S.onbuf.as<int>();
+    S.on_dev.clear();
+    FS_TRY(set_active(S, S.deck_on));
     zero_ints<<<blocks(std::max(nsp, 1)), kBlock, 0, st>>>(S.d_status, std::max(nsp, 1));
     FS_HIP_CHECK(hipGetLastError());
     // job descriptors: the old mesh of species k is its own slice of the state
@@ -684,9 +716,10 @@ int layout(Sim& S, bool keep_resident) {
         j.op = S.ps + o2;
         j.of = S.fs_ + o2;
         j.status = S.d_status + k;
+        j.on = S.d_on + k;
         jobs[k] = j;
         resets[k] = ResetJob{s.nx * s.np, s.gx, s.gp, j.f, s.d_bw.as<double>(), S.xs + o2,
-                             S.ps + o2, S.fs_ + o2, S.qw + o2};
+                             S.ps + o2, S.fs_ + o2, S.qw + o2, S.d_on + k};
         S.launch.max_panels = std::max(S.launch.max_panels, s.npanels());
         S.launch.max_points = std::max(S.launch.max_points, s.nx * s.np);
         S.launch.max_rows = std::max(S.launch.max_rows, s.np);
@@ -767,7 +800,7 @@ int remesh_adaptive(Sim& S) {
     FS_TRY(read_status(S));
     std::vector<Species*> refine;
     for (Species* s : S.sp)
-        if (s->adaptive) refine.push_back(s);
+        if (s->adaptive && (S.deck_on.empty() || S.deck_on[s->deck])) refine.push_back(s);
     // each refining species rebuilds into its work tree, whose buffers stay
     // allocated from remesh to remesh (no fresh pages for ~1e5 panels)
     for (Species* s : refine) {
@@ -956,16 +989,25 @@ int download_species(Sim& S, int k, double* xs, double* ps, double* fs, double* 
 }
 
 // The device part of one step (everything but the adaptive host work).
-int step_body(Sim& S, bool remesh_now) {
+// Step modes: no deck remeshes after the RK4 step (the full field at the
+// moved points), every deck does (remesh and the reduced-x field), or some do
+// (decks of different remesh_period): the full field for every deck, then
+// the remesh and the reduced field of the remeshing decks' species, chosen
+// by the device flags (set_active) so one graph serves every such step.
+enum StepMode { kNoRemesh = 0, kAllRemesh = 1, kSomeRemesh = 2 };
+
+int step_body(Sim& S, int mode) {
     FS_TRY(rk4(S));
-    if (remesh_now) return remesh_static(S);
-    return field_full(S, S.es, S.xs, 1);
+    if (mode == kAllRemesh) return remesh_static(S);
+    FS_TRY(field_full(S, S.es, S.xs, 1));
+    if (mode == kSomeRemesh) return remesh_static(S);
+    return FS_OK;
 }
 
-int capture(Sim& S, bool remesh_now, hipGraphExec_t& exec) {
+int capture(Sim& S, int mode, hipGraphExec_t& exec) {
     hipStream_t st = S.stream();
     FS_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    const int rc = step_body(S, remesh_now);
+    const int rc = step_body(S, mode);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(st, &g);
     if (rc != FS_OK) {
@@ -1106,7 +1148,6 @@ int create_empty(fs_ctx* ctx, int precision, fs_sim** out) {
     auto S = std::make_unique<fs_sim>();
     S->ctx = ctx;
     S->precision = precision;
-    S->remesh_period = 1;
     FS_TRY(layout(*S, true));
     *out = S.release();
     return FS_OK;
@@ -1121,9 +1162,9 @@ int attach_decks(fs_sim* S, std::vector<Deck*>& decks, std::vector<int>* indices
     // every deck is checked before any joins, so a refused call leaves the
     // batch as it was
     for (Deck* d : decks) {
-        if (!d || d->prm.remesh_period != 1) {
+        if (!d) {
             drop_all();
-            return fail(FS_E_INVALID, "decks joining a running batch remesh every step");
+            return fail(FS_E_INVALID, "attach: null deck");
         }
         if (S->precision == FS_PRECISION_MIXED && open_field(d->prm)) {
             drop_all();
@@ -1143,8 +1184,16 @@ int attach_decks(fs_sim* S, std::vector<Deck*>& decks, std::vector<int>* indices
     renumber(*S);
     S->tune_pending = true;
     int rc = layout(*S, true);
+    // the joining decks' initial field (AMRSimulation.cpp:84-89); the resident
+    // decks keep theirs (mid-period ones hold the full field at moved points)
+    if (rc == FS_OK) {
+        S->deck_on.assign(S->decks.size(), 0);
+        for (size_t k = before; k < S->decks.size(); ++k) S->deck_on[k] = 1;
+        rc = set_active(*S, S->deck_on);
+    }
     if (rc == FS_OK) rc = set_drive(*S, false);
     if (rc == FS_OK) rc = field_reduced(*S);
+    S->deck_on.clear();
     if (rc == FS_OK && hipStreamSynchronize(S->ctx->stream) != hipSuccess)
         rc = fail(FS_E_HIP, "attach: stream synchronization failed");
     if (rc != FS_OK) {
@@ -1220,10 +1269,8 @@ int fs_batch_create(fs_ctx* ctx, const fs_deck* decks, int n_decks, int precisio
     if (n_decks < 1) return fail(FS_E_INVALID, "fs_batch_create: no decks");
     if (precision != FS_PRECISION_F64 && precision != FS_PRECISION_MIXED)
         return fail(FS_E_INVALID, "fs_batch_create: unknown precision %d", precision);
-    for (int d = 1; d < n_decks; ++d)
-        if (!decks[d].sim || !decks[0].sim ||
-            decks[d].sim->remesh_period != decks[0].sim->remesh_period)
-            return fail(FS_E_INVALID, "fs_batch_create: decks of a batch share remesh_period");
+    for (int d = 0; d < n_decks; ++d)
+        if (!decks[d].sim) return fail(FS_E_INVALID, "fs_batch_create: null deck");
     FS_HIP_CHECK(hipSetDevice(ctx->device));
     auto S = std::make_unique<fs_sim>();
     S->ctx = ctx;
@@ -1233,7 +1280,6 @@ int fs_batch_create(fs_ctx* ctx, const fs_deck* decks, int n_decks, int precisio
         FS_TRY(sim::build_deck(decks[d].sim, decks[d].species, decks[d].n_species, D));
         S->decks.push_back(std::move(D));
     }
-    S->remesh_period = S->decks[0]->prm.remesh_period;
     sim::renumber(*S);
     FS_TRY(sim::layout(*S, false));
     FS_TRY(sim::set_drive(*S, false));
@@ -1253,8 +1299,6 @@ int fs_sim_create(fs_ctx* ctx, const fs_sim_params* prm, const fs_species_params
 int fs_sim_add_deck(fs_sim* S, const fs_deck* deck, int* index) {
     if (!S || !deck) return fail(FS_E_INVALID, "fs_sim_add_deck: null argument");
     if (!deck->sim) return fail(FS_E_INVALID, "fs_sim_add_deck: null deck");
-    if (S->remesh_period != 1 || deck->sim->remesh_period != 1)
-        return fail(FS_E_INVALID, "fs_sim_add_deck: decks joining a running batch remesh every step");
     FS_HIP_CHECK(hipSetDevice(S->ctx->device));
     std::unique_ptr<sim::Deck> D;
     FS_TRY(sim::build_deck(deck->sim, deck->species, deck->n_species, D));
@@ -1376,6 +1420,7 @@ int fs_sim_destroy(fs_sim* S) {
     S->fred.release();
     S->tree_work.release();
     S->drive.release();
+    S->onbuf.release();
     if (S->ev_r0) (void)hipEventDestroy(S->ev_r0);
     if (S->ev_r1) (void)hipEventDestroy(S->ev_r1);
     S->diag_work.release();
@@ -1386,24 +1431,36 @@ int fs_sim_destroy(fs_sim* S) {
 int fs_sim_step(fs_sim* S) {
     if (!S) return fail(FS_E_INVALID, "null simulation");
     FS_HIP_CHECK(hipSetDevice(S->ctx->device));
-    const bool remesh_now = ((S->iter + 1) % S->remesh_period) == 0;
+    // which decks remesh after this step (each by its own remesh_period and
+    // iteration count, AMRSimulation_step.cpp:74-90)
+    std::vector<char> on(S->decks.size(), 0);
+    size_t nrem = 0;
+    for (size_t d = 0; d < S->decks.size(); ++d) {
+        const sim::Deck& D = *S->decks[d];
+        on[d] = ((D.iter + 1) % D.prm.remesh_period) == 0 ? 1 : 0;
+        nrem += on[d];
+    }
+    const int mode = nrem == 0 ? sim::kNoRemesh
+                     : nrem == S->decks.size() ? sim::kAllRemesh : sim::kSomeRemesh;
+    S->deck_on = mode == sim::kSomeRemesh ? on : std::vector<char>{};
+    FS_TRY(sim::set_active(*S, S->deck_on));
     const bool eager = S->any_adaptive || S->any_treecode || !S->use_graph || S->timing;
     S->timed = false;
     FS_TRY(sim::set_drive(*S, true));
     if (eager) {
         FS_TRY(sim::rk4(*S));
-        if (remesh_now) {
+        if (mode != sim::kAllRemesh) FS_TRY(sim::field_full(*S, S->es, S->xs, 1));
+        if (mode != sim::kNoRemesh) {
             if (S->any_adaptive) {
                 FS_TRY(sim::remesh_adaptive(*S));
             } else {
                 FS_TRY(sim::remesh_static(*S));
             }
-        } else {
-            FS_TRY(sim::field_full(*S, S->es, S->xs, 1));
         }
     } else {
-        hipGraphExec_t& g = remesh_now ? S->g_remesh : S->g_plain;
-        if (!g) FS_TRY(sim::capture(*S, remesh_now, g));
+        hipGraphExec_t& g = mode == sim::kAllRemesh ? S->g_remesh
+                            : mode == sim::kNoRemesh ? S->g_plain : S->g_mixed;
+        if (!g) FS_TRY(sim::capture(*S, mode, g));
         FS_HIP_CHECK(hipGraphLaunch(g, S->ctx->stream));
     }
     S->iter += 1;
@@ -1411,7 +1468,8 @@ int fs_sim_step(fs_sim* S) {
         D->iter += 1;
         D->t += D->prm.dt;
     }
-    if (remesh_now) return sim::read_status(*S);
+    S->deck_on.clear();
+    if (mode != sim::kNoRemesh) return sim::read_status(*S);
     FS_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
     return FS_OK;
 }
@@ -1521,9 +1579,6 @@ int fs_sim_write_output(fs_sim* S, int deck, const char* sim_dir, const char* co
 int fs_sim_restart_deck(fs_sim* S, const fs_deck* deck, const char* sim_dir,
                         const char* const* names, int iter, int* index) {
     if (!S || !deck || !deck->sim) return fail(FS_E_INVALID, "fs_sim_restart_deck: null argument");
-    if (S->remesh_period != 1 || deck->sim->remesh_period != 1)
-        return fail(FS_E_INVALID, "fs_sim_restart_deck: decks joining a running batch remesh "
-                                  "every step");
     sim::Deck* D = nullptr;
     FS_TRY(sim::build_deck_restart_host(deck, sim_dir, names, iter, &D));
     std::vector<sim::Deck*> one{D};

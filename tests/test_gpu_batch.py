@@ -169,9 +169,56 @@ def test_continuous_batching(gpu_ctx, precision):
     b.close()
 
 
-def test_batch_rejects_mixed_periods(gpu_ctx):
-    s1, sp1 = landau(3, 1)
-    s2, sp2 = landau(3, 1)
-    s2.remesh_period = 2
-    with pytest.raises(ms.FarrsightError):
-        ms.Simulation.batch(gpu_ctx, [(s1, sp1), (s2, sp2)])
+def _with_period(deck, period):
+    sim, sps = deck
+    sim = ms.sim_params(**{f: getattr(sim, f) for f, _ in ms.SimParams._fields_})
+    sim.remesh_period = period
+    return sim, sps
+
+
+@pytest.mark.parametrize("precision", [ms.PRECISION_F64, ms.PRECISION_MIXED])
+def test_mixed_remesh_periods_equal_each_alone(gpu_ctx, precision):
+    """Decks that remesh every step, every second and every third step in one
+    batch (graph replays for the steps where all, none and some of them
+    remesh, and eagerly): after every step each deck's state equals the deck
+    run alone, bit for bit (AMRSimulation_step.cpp:74-90 per deck)."""
+    decks = [_with_period(DECKS[0], 1), _with_period(DECKS[1], 2), _with_period(DECKS[2], 3),
+             _with_period(DECKS[3], 2)]
+    g = ms.Simulation.batch(gpu_ctx, decks, precision)
+    e = ms.Simulation.batch(gpu_ctx, decks, precision)
+    e.set_options(precision, use_graph=False)
+    alone = [ms.Simulation.batch(gpu_ctx, [d], precision) for d in decks]
+    for step in range(1, 7):
+        g.step()
+        e.step()
+        for d, deck in enumerate(decks):
+            alone[d].step()
+            first, nsp, it, _ = g.deck_info(d)
+            assert it == step
+            ref = state(alone[d], 0, nsp)
+            for k in range(nsp):
+                assert np.array_equal(state(g, first, nsp)[k], ref[k]), (precision, step, d, k)
+                assert np.array_equal(state(e, first, nsp)[k], ref[k]), (precision, step, d, k)
+    for s in [g, e] + alone:
+        s.close()
+
+
+def test_deck_with_other_period_joins_mid_run(gpu_ctx):
+    """A deck that remeshes every second step joins a batch after 3 steps of
+    a deck that remeshes every step (and one that remeshes every third): it
+    keeps its own iteration count, and after 4 more steps every deck equals
+    its run alone."""
+    a, c = _with_period(DECKS[0], 1), _with_period(DECKS[2], 3)
+    b = _with_period(DECKS[1], 2)
+    s = ms.Simulation.batch(gpu_ctx, [a, c], ms.PRECISION_F64)
+    s.step(3)
+    s.add_deck(*b)
+    s.step(4)
+    for d, (deck, steps) in enumerate([(a, 7), (c, 7), (b, 4)]):
+        first, nsp, it, _ = s.deck_info(d)
+        assert it == steps
+        ref = run_alone(gpu_ctx, deck, steps, ms.PRECISION_F64)
+        got = state(s, first, nsp)
+        for k in range(nsp):
+            assert np.array_equal(got[k], ref[k]), (d, k)
+    s.close()

@@ -106,17 +106,17 @@ def test_slots_and_budget():
 
 
 def test_failed_requests_leave_the_others_intact():
-    """Bad decks beside good ones: a deck that remeshes every second step is
-    refused at submit, a deck whose mesh cannot be built (p_max < p_min)
-    fails in the builder; the good decks, submitted before and after them,
-    complete bitwise equal to the same decks run alone."""
+    """Bad decks beside good ones: a request of zero steps is refused at
+    submit, a deck whose mesh cannot be built (p_max < p_min) fails in the
+    builder; the good decks (one of them remeshing every second step),
+    submitted before and after them, complete bitwise equal to the same decks
+    run alone."""
     ctx = ms.Context(0)
     srv = ms.Server(ctx, hbm_budget_bytes=8e9, max_active=0, precision=ms.PRECISION_F64)
-    good = [(landau(4), 3), (landau(3, amp=0.2, mass=25.0), 4)]
-    bad_period = landau(3)
-    bad_period[0].remesh_period = 2
+    good = [(landau(4), 3), (landau(3, amp=0.2, mass=25.0), 5)]
+    good[1][0][0].remesh_period = 2
     with pytest.raises(ms.FarrsightError):
-        srv.submit(*bad_period, 2)
+        srv.submit(*landau(3), 0)
     bad_mesh = landau(3)
     bad_mesh[1][0].p_max = -7.0
     ids = [srv.submit(*good[0][0], good[0][1]), srv.submit(*bad_mesh, 2),
@@ -139,21 +139,22 @@ def test_failed_requests_leave_the_others_intact():
 
 
 def test_refused_join_leaves_the_batch_unchanged():
-    """fs_sim_add_deck refuses a deck that remeshes every second step before
-    anything joins: the batch keeps stepping bitwise as the deck alone."""
+    """fs_sim_add_deck refuses an open-boundary deck in a mixed-precision batch
+    before anything joins: the batch keeps stepping bitwise as the deck
+    alone."""
     ctx = ms.Context(0)
     deck = landau(4)
-    s = ms.Simulation.batch(ctx, [deck], ms.PRECISION_F64)
+    s = ms.Simulation.batch(ctx, [deck], ms.PRECISION_MIXED)
     s.step(1)
     bad = landau(3)
-    bad[0].remesh_period = 2
-    with pytest.raises(ms.FarrsightError):
+    bad[0].bcs = 1
+    with pytest.raises(ms.FarrsightError, match="FP64 only"):
         s.add_deck(*bad)
     assert s.n_decks == 1
     s.step(2)
     got = [s.points(k) for k in range(2)]
     s.close()
-    ref = alone(ctx, deck, 3, ms.PRECISION_F64)
+    ref = alone(ctx, deck, 3, ms.PRECISION_MIXED)
     for k in range(2):
         for a in range(5):
             assert np.array_equal(got[k][a], ref[k][a]), (k, a)
