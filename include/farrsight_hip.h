@@ -436,9 +436,10 @@ int fs_sim_deck_info(fs_sim* sim, int deck, int* first_species, int* n_species, 
  * treecode decks always do). */
 int fs_sim_set_options(fs_sim* sim, int precision, int use_graph);
 /* Mixed precision: what the accuracy governor measured in the deck's last
- * full-field evaluation (every mixed evaluation measures its own error
- * before its pair sums: up to 128 probed targets against the exact FP64
- * field). max_e = max|E| over the probed targets; errs[q] = the largest E
+ * measured full-field evaluation (RK4 stages 2 and 4 measure their own error
+ * before their pair sums -- up to 128 probed targets against the exact FP64
+ * field; stage 3 and the full field after a step without remesh keep the
+ * band of the stage before; every reduced-x evaluation measures its own). max_e = max|E| over the probed targets; errs[q] = the largest E
  * error there when pairs beyond bands[q] (fraction of the period; >= 1/4
  * means every pair in FP64) take the FP32 path, q = 0..7. The evaluation
  * took the narrowest band with errs[q] <= 4e-9 max_e. Any output may be

@@ -152,6 +152,9 @@ public:
     // FP64 band would give (probed targets against the exact FP64 field) and
     // takes the narrowest band within target * max|E| (target <= 0: 4e-9).
     void govern(bool on, double target = 0);
+    // Governed runs: whether the next runs measure (probe and governor) or
+    // keep every segment's band from the last run that did (on by default).
+    void set_probe(bool on) { probe_ = on; }
     // The last governed run's measurements (nseg * kGovVals values: per
     // segment max|E| and the error at each candidate band, as doubles).
     int get_gov(std::vector<double>& out, hipStream_t stream) const;
@@ -183,6 +186,7 @@ private:
 
     int precision_ = kPrecisionF64;
     bool governed_ = false;
+    bool probe_ = true;
     double gov_target_ = 4e-9;
     uint64_t* gov() const;
     int n_total_ = 0, nseg_ = 0, tpt_ = 2, grid_ = 0, wgs_per_cu_ = 8, num_cus_ = 256;

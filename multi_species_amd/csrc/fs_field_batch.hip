@@ -1268,7 +1268,19 @@ __device__ __forceinline__ double field_at(int k, int d, const SegDev* __restric
     const SegDev& R = segs[S.src];
     const uint64_t u = us[k];
     const double W = pw[R.pw_base + R.n], A = totals[S.src];
-    const double P0 = w_le(us, ws, pw, R.off, R.n, R.pw_base, u);
+    double P0;
+    if (S.src == d) {
+        // u is this segment's own sorted key at k: the run of its top bits
+        // starts at or just before k (the search w_le makes, found locally)
+        const uint64_t kt = u >> kSortShift;
+        int lo = k - S.off;
+        while (lo > 0 && (us[S.off + lo - 1] >> kSortShift) == kt) --lo;
+        P0 = pw[R.pw_base + lo];
+        for (int j = lo; j < R.n && (us[R.off + j] >> kSortShift) == kt; ++j)
+            if (us[R.off + j] <= u) P0 += ws[R.off + j];
+    } else {
+        P0 = w_le(us, ws, pw, R.off, R.n, R.pw_base, u);
+    }
     const double Pm = (u >= kHalf) ? w_le(us, ws, pw, R.off, R.n, R.pw_base, u - kHalf) : 0.0;
     const double Pp = (u < kHalf) ? w_le(us, ws, pw, R.off, R.n, R.pw_base, u + kHalf) : W;
     const double sum_s = 2.0 * ((P0 - Pm) - Pp) + W;
@@ -1849,7 +1861,7 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
                                                                    pw, tot);
         FS_HIP_CHECK(hipGetLastError());
     }
-    if (governed_) {
+    if (governed_ && probe_) {
         zero_u64_kernel<<<nblocks(nseg_ * kGovVals), kBlock, 0, stream>>>(gov(), nseg_ * kGovVals);
         FS_HIP_CHECK(hipGetLastError());
         probe_kernel<<<dim3(kProbeMax, nseg_), kBlock, 0, stream>>>(

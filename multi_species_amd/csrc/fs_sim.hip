@@ -416,9 +416,16 @@ int set_drive(Sim& S, bool in_step) {
 
 // E at every point: the batched direct sums over all direct-sum decks
 // (periodic and open), then the treecode decks one by one, then the drives
-// (slot: set_drive).
-int field_full(Sim& S, double* out, const double* x, int slot) {
-    FS_TRY(S.ffull.run(x, S.qw, out, S.stream()));
+// (slot: set_drive). Mixed precision: probe = false keeps each deck's FP64
+// band from the last evaluation that measured it: RK4 stages 2 and 4
+// measure, stage 3 (positions half a stage from stage 2's) and the full
+// field after a step without remesh (a sixth of a stage from stage 4's)
+// reuse theirs.
+int field_full(Sim& S, double* out, const double* x, int slot, bool probe = true) {
+    S.ffull.set_probe(probe);
+    const int rc = S.ffull.run(x, S.qw, out, S.stream());
+    S.ffull.set_probe(true);
+    FS_TRY(rc);
     if (S.any_treecode)
         for (auto& D : S.decks)
             if (uses_treecode(D->prm))
@@ -462,7 +469,7 @@ int rk4(Sim& S) {
     FS_TRY(field_full(S, S.e2, S.xk, 0));
     rk4_stage2<<<blocks(N), kBlock, 0, st>>>(N, S.sid, S.sinfo, S.xs, S.ps, S.e2, S.p2, S.p3,
                                              S.xk);
-    FS_TRY(field_full(S, S.e3, S.xk, 0));
+    FS_TRY(field_full(S, S.e3, S.xk, 0, false));
     rk4_stage3<<<blocks(N), kBlock, 0, st>>>(N, S.sid, S.sinfo, S.xs, S.ps, S.e3, S.p3, S.p4,
                                              S.xk);
     FS_TRY(field_full(S, S.e4, S.xk, 1));
@@ -999,7 +1006,7 @@ enum StepMode { kNoRemesh = 0, kAllRemesh = 1, kSomeRemesh = 2 };
 int step_body(Sim& S, int mode) {
     FS_TRY(rk4(S));
     if (mode == kAllRemesh) return remesh_static(S);
-    FS_TRY(field_full(S, S.es, S.xs, 1));
+    FS_TRY(field_full(S, S.es, S.xs, 1, false));
     if (mode == kSomeRemesh) return remesh_static(S);
     return FS_OK;
 }
@@ -1449,7 +1456,7 @@ int fs_sim_step(fs_sim* S) {
     FS_TRY(sim::set_drive(*S, true));
     if (eager) {
         FS_TRY(sim::rk4(*S));
-        if (mode != sim::kAllRemesh) FS_TRY(sim::field_full(*S, S->es, S->xs, 1));
+        if (mode != sim::kAllRemesh) FS_TRY(sim::field_full(*S, S->es, S->xs, 1, false));
         if (mode != sim::kNoRemesh) {
             if (S->any_adaptive) {
                 FS_TRY(sim::remesh_adaptive(*S));
