@@ -162,7 +162,23 @@ __global__ __launch_bounds__(64) void tree_sum_kernel(
     double near_acc = 0.0;
     for (int q = near_off[leaf]; q < near_off[leaf + 1]; ++q) {
         const int2 nr = range[near_list[q]];
-        for (int j = nr.x; j <= nr.y; ++j)
+        int j = nr.x;
+        // eight sources per round: their scalar loads are issued together,
+        // and the terms still add in source order
+        for (; j + 7 <= nr.y; j += 8) {
+            double xs[8], ls[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                xs[u] = px[j + u];
+                ls[u] = lam[j + u];
+            }
+            double kv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) kv[u] = kern(xi, xs[u], invL, delta2, C);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) near_acc = __builtin_fma(ls[u], kv[u], near_acc);
+        }
+        for (; j <= nr.y; ++j)
             near_acc = __builtin_fma(lam[j], kern(xi, px[j], invL, delta2, C), near_acc);
     }
     if (active) e_tree[i] = far_acc * scale + near_acc * scale;
