@@ -361,6 +361,17 @@ typedef struct fs_deck {
 int fs_sim_set_timing(fs_sim* sim, int on);
 int fs_sim_remesh_ms(fs_sim* sim, double* ms);
 
+/* Where adaptive remeshes refine their trees (the adaptive loop of
+ * generate_mesh, AMRStructure_amr.cpp:901-941, with refine_panels,
+ * test_panel and set_leaves_weights): mode 0 on the device (default; the
+ * passes run in parallel and give the sequential order's tree bit for bit),
+ * 1 on the host (the sequential builder; also what a device remesh falls
+ * back to if its passes meet a neighbour link they do not reproduce).
+ * fs_sim_refinement_counts gives the remeshes refined each way. No
+ * counterpart in the reference. */
+int fs_sim_set_refinement(fs_sim* sim, int mode);
+int fs_sim_refinement_counts(fs_sim* sim, int64_t* on_device, int64_t* on_host);
+
 /* Host-only: the device bytes a deck is estimated to take in a batch or a
  * server (state, RK4 work, field partials -- in FP64 the pair-once partials of
  * its full field --, sorted copies in mixed precision, panel records); the

@@ -69,6 +69,8 @@ def lib():
                                                 _vp]
         L.fs_sim_set_timing.argtypes = [_vp, _int]
         L.fs_sim_remesh_ms.argtypes = [_vp, ctypes.POINTER(ctypes.c_double)]
+        L.fs_sim_set_refinement.argtypes = [_vp, _int]
+        L.fs_sim_refinement_counts.argtypes = [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]
         L.fs_emq_treecode_f64.argtypes = [_vp, _dp, _dp, _dp, _i64, _d, _d, _d, _int, _int,
                                           ctypes.POINTER(_i64)]
         _ip = ctypes.POINTER(ctypes.c_int32)
@@ -573,6 +575,18 @@ class Simulation:
         out = ctypes.c_double()
         _check(lib().fs_sim_remesh_ms(self.h, ctypes.byref(out)), "fs_sim_remesh_ms")
         return out.value
+
+    def set_refinement(self, on_device: bool = True):
+        """Adaptive remeshes refine on the device (default) or on the host."""
+        _check(lib().fs_sim_set_refinement(self.h, 0 if on_device else 1),
+               "fs_sim_set_refinement")
+
+    def refinement_counts(self) -> tuple:
+        """(remeshes refined on the device, on the host)."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().fs_sim_refinement_counts(self.h, ctypes.byref(a), ctypes.byref(b)),
+               "fs_sim_refinement_counts")
+        return a.value, b.value
 
     def set_options(self, precision: int = PRECISION_F64, use_graph: bool = True):
         _check(lib().fs_sim_set_options(self.h, precision, int(bool(use_graph))),

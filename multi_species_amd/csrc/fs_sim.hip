@@ -36,6 +36,7 @@
 #include <thread>
 #include <vector>
 
+#include "fs_amr.h"
 #include "fs_field_batch.h"
 #include "fs_mesh.h"
 #include "fs_remesh.h"
@@ -207,6 +208,32 @@ __global__ __launch_bounds__(kBlock) void scatter_reduced(int n,
     es[i] = re[unique_of[i]];
 }
 
+// A tree refined on the device joins the batch arrays: q_ws = weight * f.
+__global__ __launch_bounds__(kBlock) void qw_kernel(int n, const double* __restrict__ bw,
+                                                    const double* __restrict__ f,
+                                                    double* __restrict__ qw) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) qw[i] = bw[i] * f[i];
+}
+
+// ... and its reduced-x map, placed at the species' offsets: points from o,
+// distinct x from ub.
+__global__ __launch_bounds__(kBlock) void place_map_kernel(
+    int n, int U, int o, int ub, const int32_t* __restrict__ roff,
+    const int32_t* __restrict__ members, const int32_t* __restrict__ uof,
+    const double* __restrict__ ux, int32_t* __restrict__ r_off, int32_t* __restrict__ r_mem,
+    int32_t* __restrict__ r_uof, double* __restrict__ rx) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        r_mem[o + i] = o + members[i];
+        r_uof[o + i] = ub + uof[i];
+    }
+    if (i < U) {
+        r_off[ub + i] = o + roff[i];
+        rx[ub + i] = ux[i];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void zero_ints(int* a, int n) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) a[i] = 0;
@@ -262,8 +289,20 @@ struct Species {
     int32_t* rows_d = nullptr;
     remesh::Job job{};  // this species' grid interpolation (device pointers)
 
-    int n() const { return (int)tree.xs.size(); }
-    int npanels() const { return tree.size(); }
+    // Adaptive species refined on the device: the current tree lives in
+    // `aw` (records, and until the next layout its points), with its weights
+    // and reduced-x map; `tree` on the host is stale until host_tree()
+    // downloads it. The base mesh's records stay on the device.
+    amr::Work aw;
+    DevBuf d_base_rec;
+    bool dev_tree = false;   // the current tree is aw's
+    bool host_valid = true;  // `tree` holds the current tree
+    bool from_amr = false;   // the state is in aw, not yet in the batch arrays
+    amr::Output amr_out;
+
+    int n() const { return dev_tree ? amr_out.V : (int)tree.xs.size(); }
+    int npanels() const { return dev_tree ? amr_out.P : tree.size(); }
+    int nunique() const { return dev_tree ? amr_out.U : (int)red.ux.size(); }
     void release() {
         d_rec.release();
         d_grid.release();
@@ -271,6 +310,8 @@ struct Species {
         d_fnew.release();
         d_old.release();
         d_work.release();
+        aw.release();
+        d_base_rec.release();
     }
 };
 
@@ -349,6 +390,12 @@ struct fs_sim {
     bool timing = false;
     bool timed = false;  // the events of the last step enclose a remesh
     hipEvent_t ev_r0 = nullptr, ev_r1 = nullptr;
+    // adaptive refinement: 0 on the device, 1 on the host (the host
+    // builder, also taken when the device passes meet a link they do not
+    // reproduce); remeshes done each way
+    int amr_mode = 0;
+    int64_t amr_device = 0, amr_host = 0;
+    DevBuf amr_jobs;
 
     hipStream_t stream() const { return ctx->stream; }
 };
@@ -491,6 +538,26 @@ int read_status(Sim& S) {
     return rm::status_to_error(all);
 }
 
+int clear_status_all(Sim& S) {
+    const int n = (int)S.sp.size();
+    if (n == 0) return FS_OK;
+    zero_ints<<<blocks(n), kBlock, 0, S.stream()>>>(S.d_status, n);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+// The host copy of a device-refined tree's panel records.
+int host_tree(Sim& S, Species& s) {
+    if (!s.dev_tree || s.host_valid) return FS_OK;
+    s.tree.rec.resize((size_t)s.npanels() * msh::kInts);
+    FS_HIP_CHECK(hipMemcpyAsync(s.tree.rec.data(), s.amr_out.rec,
+                                sizeof(int32_t) * s.tree.rec.size(), hipMemcpyDeviceToHost,
+                                S.stream()));
+    FS_HIP_CHECK(hipStreamSynchronize(S.stream()));
+    s.host_valid = true;
+    return FS_OK;
+}
+
 // AMRSimulation::remesh (AMRSimulation_remesh.cpp:3-48), device-only path:
 // every species' grid interpolation in one launch per kernel, then the
 // points reset to the grid and the reduced-x field.
@@ -516,12 +583,20 @@ int remesh_static(Sim& S) {
 int upload_species(Sim& S, const Deck& D, Species& s, bool grid_too) {
     hipStream_t st = S.stream();
     const int n = s.n(), nb = s.nx * s.np, P = s.npanels();
-    FS_TRY(s.d_rec.ensure(sizeof(int32_t) * s.tree.rec.size(), st));
-    FS_HIP_CHECK(hipMemcpyAsync(s.d_rec.ptr, s.tree.rec.data(), sizeof(int32_t) * s.tree.rec.size(),
-                                hipMemcpyHostToDevice, st));
-    FS_TRY(s.d_bw.ensure(sizeof(double) * n, st));
-    FS_HIP_CHECK(hipMemcpyAsync(s.d_bw.ptr, s.base_w.data(), sizeof(double) * n,
-                                hipMemcpyHostToDevice, st));
+    // a tree refined on the device is already there (records, weights)
+    if (!s.dev_tree) {
+        FS_TRY(s.d_rec.ensure(sizeof(int32_t) * s.tree.rec.size(), st));
+        FS_HIP_CHECK(hipMemcpyAsync(s.d_rec.ptr, s.tree.rec.data(),
+                                    sizeof(int32_t) * s.tree.rec.size(), hipMemcpyHostToDevice, st));
+        FS_TRY(s.d_bw.ensure(sizeof(double) * n, st));
+        FS_HIP_CHECK(hipMemcpyAsync(s.d_bw.ptr, s.base_w.data(), sizeof(double) * n,
+                                    hipMemcpyHostToDevice, st));
+    }
+    if (s.adaptive && !s.d_base_rec.ptr) {
+        FS_TRY(s.d_base_rec.ensure(sizeof(int32_t) * s.base.rec.size(), st));
+        FS_HIP_CHECK(hipMemcpyAsync(s.d_base_rec.ptr, s.base.rec.data(),
+                                    sizeof(int32_t) * s.base.rec.size(), hipMemcpyHostToDevice, st));
+    }
     if (grid_too) {
         const size_t bg = align_up(sizeof(double) * nb, 256);
         FS_TRY(s.d_grid.ensure(2 * bg + sizeof(int32_t) * nb, st));
@@ -542,7 +617,7 @@ int upload_species(Sim& S, const Deck& D, Species& s, bool grid_too) {
     j = rm::Job{};
     rm::carve_mesh(j, s.d_old.ptr, P);
     rm::carve_grid(j, s.d_work.ptr, nb, s.np);
-    j.rec = s.d_rec.as<int32_t>();
+    j.rec = s.dev_tree ? s.amr_out.rec : s.d_rec.as<int32_t>();
     j.npanels = P;
     j.x = s.gx;
     j.p = s.gp;
@@ -593,7 +668,14 @@ int layout(Sim& S, bool keep_resident) {
     for (int k = 0; k < nsp; ++k) {
         Species& s = *S.sp[k];
         const int n = s.n();
-        if (keep_resident && s.off >= 0) {
+        if (s.from_amr) {
+            // refined on the device: points from its tree (q_ws below)
+            const double* src[3] = {s.amr_out.x, s.amr_out.p, s.amr_out.f};
+            double* dst[3] = {xs, ps, fsv};
+            for (int a = 0; a < 3; ++a)
+                FS_HIP_CHECK(hipMemcpyAsync(dst[a] + o, src[a], sizeof(double) * n,
+                                            hipMemcpyDeviceToDevice, st));
+        } else if (keep_resident && s.off >= 0) {
             const double* src[5] = {S.xs, S.ps, S.qw, S.es, S.fs_};
             double* dst[5] = {xs, ps, qw, es, fsv};
             for (int a = 0; a < 5; ++a)
@@ -625,8 +707,14 @@ int layout(Sim& S, bool keep_resident) {
     S.fs_ = fsv;
     o = 0;
     for (int k = 0; k < nsp; ++k) {
-        S.sp[k]->off = o;
-        o += S.sp[k]->n();
+        Species& s = *S.sp[k];
+        s.off = o;
+        if (s.from_amr && s.n() > 0) {
+            qw_kernel<<<blocks(s.n()), kBlock, 0, st>>>(s.n(), s.amr_out.bw, fsv + o, qw + o);
+            FS_HIP_CHECK(hipGetLastError());
+        }
+        s.from_amr = false;
+        o += s.n();
     }
     FS_TRY(S.rk.ensure(7 * bN, st));
     S.xk = S.rk.as<double>(0);
@@ -658,10 +746,22 @@ int layout(Sim& S, bool keep_resident) {
     std::vector<double> rxh;
     std::vector<int32_t> off(1, 0), mem, uof(std::max(N, 1)), rsid;
     int ubase = 0, kspec = 0;
+    std::vector<std::pair<const Species*, int>> placed;  // device maps and their bases
     for (auto& D : S.decks) {
         D->uoff = ubase;
         for (auto& sp : D->sp) {
             const Species& s = *sp;
+            if (s.dev_tree) {
+                // placeholders, written by place_map_kernel below
+                const int U = s.amr_out.U;
+                rxh.resize(rxh.size() + U, 0.0);
+                rsid.insert(rsid.end(), U, kspec++);
+                off.resize(off.size() + U, s.off + s.n());
+                mem.resize(mem.size() + s.n(), 0);
+                placed.emplace_back(&s, ubase);
+                ubase += U;
+                continue;
+            }
             rxh.insert(rxh.end(), s.red.ux.begin(), s.red.ux.end());
             rsid.insert(rsid.end(), s.red.ux.size(), kspec++);
             for (size_t u = 0; u < s.red.ux.size(); ++u) {
@@ -700,6 +800,16 @@ int layout(Sim& S, bool keep_resident) {
         FS_HIP_CHECK(hipMemcpyAsync(S.r_mem, mem.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
         FS_HIP_CHECK(hipMemcpyAsync(S.r_uof, uof.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
     }
+    for (const auto& pl : placed) {
+        const Species& s = *pl.first;
+        const amr::Output& a = s.amr_out;
+        const int m = std::max(s.n(), a.U);
+        if (m == 0) continue;
+        place_map_kernel<<<blocks(m), kBlock, 0, st>>>(s.n(), a.U, s.off, pl.second, a.roff,
+                                                       a.members, a.uof, a.ux, S.r_off, S.r_mem,
+                                                       S.r_uof, S.rx);
+        FS_HIP_CHECK(hipGetLastError());
+    }
     for (auto& D : S.decks)
         for (auto& sp : D->sp) FS_TRY(upload_species(S, *D, *sp, sp->d_grid.ptr == nullptr || !keep_resident));
     FS_TRY(S.status.ensure(sizeof(int) * std::max(nsp, 1), st));
@@ -725,7 +835,9 @@ int layout(Sim& S, bool keep_resident) {
         j.status = S.d_status + k;
         j.on = S.d_on + k;
         jobs[k] = j;
-        resets[k] = ResetJob{s.nx * s.np, s.gx, s.gp, j.f, s.d_bw.as<double>(), S.xs + o2,
+        // (a refining species' reset is replaced by its refined tree)
+        const double* bw = s.dev_tree ? s.amr_out.bw : s.d_bw.as<double>();
+        resets[k] = ResetJob{s.nx * s.np, s.gx, s.gp, j.f, bw, S.xs + o2,
                              S.ps + o2, S.fs_ + o2, S.qw + o2, S.d_on + k};
         S.launch.max_panels = std::max(S.launch.max_panels, s.npanels());
         S.launch.max_points = std::max(S.launch.max_points, s.nx * s.np);
@@ -784,6 +896,43 @@ int layout(Sim& S, bool keep_resident) {
     return FS_OK;
 }
 
+// The refining species' trees rebuilt on the device from their base meshes
+// and grid f (fs_amr.hip); their points, weights and reduced-x maps join the
+// batch arrays at the next layout.
+int refine_on_device(Sim& S, const std::vector<Species*>& refine) {
+    std::vector<amr::Input> in;
+    std::vector<amr::Output> out;
+    for (Species* s : refine) {
+        const Deck& D = *S.decks[s->deck];
+        amr::Input I;
+        I.base_rec = s->d_base_rec.as<int32_t>();
+        I.base_panels = s->base.size();
+        I.mu = s->base.min_unrefined;
+        I.gx = s->gx;
+        I.gp = s->gp;
+        I.gf = s->d_fnew.as<double>();
+        I.nb = s->nx * s->np;
+        I.old = s->job;
+        I.max_height = s->spec.max_height;
+        I.periodic = s->spec.periodic;
+        I.eps = s->spec.amr_eps;
+        I.q = s->q;
+        I.simpson = D.prm.quadrature == 1;
+        I.w = &s->aw;
+        in.push_back(std::move(I));
+    }
+    FS_TRY(amr::refine(in, out, S.amr_jobs, S.stream()));
+    FS_TRY(read_status(S));
+    for (size_t k = 0; k < refine.size(); ++k) {
+        Species& s = *refine[k];
+        s.amr_out = out[k];
+        s.dev_tree = true;
+        s.host_valid = false;
+        s.from_amr = true;
+    }
+    return FS_OK;
+}
+
 // Remesh with adaptive refinement: grid interpolation of every species on
 // the device; species without refinement are reset to their grid there and
 // stay resident; each refining species rebuilds its tree on a host thread of
@@ -808,6 +957,19 @@ int remesh_adaptive(Sim& S) {
     std::vector<Species*> refine;
     for (Species* s : S.sp)
         if (s->adaptive && (S.deck_on.empty() || S.deck_on[s->deck])) refine.push_back(s);
+    if (S.amr_mode == 0 && !refine.empty()) {
+        if (refine_on_device(S, refine) == FS_OK) {
+            S.amr_device += 1;
+            FS_TRY(layout(S, true));
+            return field_reduced(S);
+        }
+        // a link structure the device passes do not reproduce (or a device
+        // error): the host builder from the same base and grid f
+        (void)hipGetLastError();
+        FS_HIP_CHECK(hipStreamSynchronize(st));
+        FS_TRY(clear_status_all(S));
+    }
+    if (!refine.empty()) S.amr_host += 1;
     // each refining species rebuilds into its work tree, whose buffers stay
     // allocated from remesh to remesh (no fresh pages for ~1e5 panels)
     for (Species* s : refine) {
@@ -890,6 +1052,9 @@ int remesh_adaptive(Sim& S) {
         s.tree.leaf_weights(s.q, D.prm.quadrature == 1, s.base_w, leaves);
         mesh::reduced_map(s.tree.xs, s.red);
         s.off = -1;  // upload from the host tree
+        s.dev_tree = false;
+        s.host_valid = true;
+        s.from_amr = false;
     }
     FS_TRY(layout(S, true));
     return field_reduced(S);
@@ -1143,6 +1308,9 @@ size_t deck_footprint(const Deck* d, int precision) {
         b += n * (12 * 8 + 3 * 4 + 16 * 8 + (precision == FS_PRECISION_MIXED ? 96 : 0));
         b += P * (80 + 96 + 104 + 184);
         b += g * (4 * 8 + 4) + rm::grid_bytes((int)g, s->np);
+        // device refinement: tree, pass state and weights/map work per point
+        // and panel, the base records
+        if (s->adaptive) b += n * 200 + P * 170 + (size_t)s->base.size() * 80;
     }
     return b + (size_t(1) << 20);
 }
@@ -1243,6 +1411,7 @@ int detach_decks(fs_sim* S, const std::vector<int>& idx,
             FS_TRY(download_species(*S, g, st.xs.data(), st.ps.data(), st.fs.data(),
                                     st.qw.data(), st.es.data()));
             FS_TRY(fs_sim_diagnostics(S, g, st.diag));
+            FS_TRY(host_tree(*S, *S->sp[g]));
             st.rec = S->sp[g]->tree.rec;
             st.iter = S->decks[d]->iter;
             out[q].push_back(std::move(st));
@@ -1387,6 +1556,20 @@ int fs_sim_set_options(fs_sim* S, int precision, int use_graph) {
     return FS_OK;
 }
 
+int fs_sim_set_refinement(fs_sim* S, int mode) {
+    if (!S) return fail(FS_E_INVALID, "null simulation");
+    if (mode != 0 && mode != 1) return fail(FS_E_INVALID, "refinement mode is 0 (device) or 1 (host)");
+    S->amr_mode = mode;
+    return FS_OK;
+}
+
+int fs_sim_refinement_counts(fs_sim* S, int64_t* on_device, int64_t* on_host) {
+    if (!S) return fail(FS_E_INVALID, "null simulation");
+    if (on_device) *on_device = S->amr_device;
+    if (on_host) *on_host = S->amr_host;
+    return FS_OK;
+}
+
 int fs_sim_set_timing(fs_sim* S, int on) {
     if (!S) return fail(FS_E_INVALID, "null simulation");
     FS_HIP_CHECK(hipSetDevice(S->ctx->device));
@@ -1431,6 +1614,7 @@ int fs_sim_destroy(fs_sim* S) {
     if (S->ev_r0) (void)hipEventDestroy(S->ev_r0);
     if (S->ev_r1) (void)hipEventDestroy(S->ev_r1);
     S->diag_work.release();
+    S->amr_jobs.release();
     delete S;
     return FS_OK;
 }
@@ -1543,6 +1727,8 @@ int fs_sim_field_governor(fs_sim* S, int deck, double* max_e, double* errs, doub
 int fs_sim_get_panels(fs_sim* S, int k, int32_t* panels) {
     if (!S || !panels || k < 0 || k >= (int)S->sp.size())
         return fail(FS_E_INVALID, "bad species index");
+    FS_HIP_CHECK(hipSetDevice(S->ctx->device));
+    FS_TRY(sim::host_tree(*S, *S->sp[k]));
     const auto& r = S->sp[k]->tree.rec;
     std::copy(r.begin(), r.end(), panels);
     return FS_OK;
@@ -1566,6 +1752,7 @@ int fs_sim_write_output(fs_sim* S, int deck, const char* sim_dir, const char* co
         std::vector<double> xs(n), ps(n), fv(n), qw(n), es(n);
         FS_TRY(sim::download_species(*S, g, xs.data(), ps.data(), fv.data(), qw.data(),
                                      es.data()));
+        FS_TRY(sim::host_tree(*S, *S->sp[g]));
         const auto& rec = S->sp[g]->tree.rec;
         FS_TRY(io::write_species(sim_dir, names[k], D.iter, n, xs.data(), ps.data(), fv.data(),
                                  qw.data(), es.data(), (int64_t)(rec.size() / msh::kInts),
