@@ -160,6 +160,8 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--eps", type=float, default=0.05)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--refinement", choices=["device", "host"], default="device",
+                    help="where adaptive remeshes refine their trees (c3)")
     ap.add_argument("--workload", choices=["field", "open", "treecode", "c1", "c3", "c4", "c5"],
                     default="field",
                     help="field: N=2^20 direct-sum evaluation (default); c1: the two-species "
@@ -618,6 +620,7 @@ def bench_c3(args):
 
     ctx = ms.Context(0)
     s = ms.Simulation(ctx, *c3_deck())
+    s.set_refinement(on_device=args.refinement == "device")
     counts0 = [s.counts(k)[0] for k in range(2)]
     for _ in range(args.warmup):
         s.step()
@@ -657,6 +660,7 @@ def bench_c3(args):
                    "points_start": counts0, "points_end": counts},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "treecode_rel_e_err_vs_direct_f64": err,
+        "refinement": args.refinement,
     }))
 
 

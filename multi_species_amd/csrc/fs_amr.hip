@@ -210,10 +210,6 @@ __global__ __launch_bounds__(kB) void leaf_init_kernel(const Job* __restrict__ j
     leaf_init_one(jobs[blockIdx.y], tid());
 }
 
-__global__ __launch_bounds__(kB) void leaf_up_kernel(const Job* __restrict__ jobs) {
-    leaf_up_one(jobs[blockIdx.y], tid(), DevOps{});
-}
-
 __global__ __launch_bounds__(kB) void leaf_terms_kernel(const Job* __restrict__ jobs) {
     leaf_terms_one(jobs[blockIdx.y], tid(), DevOps{});
 }
@@ -281,15 +277,13 @@ void set_views(Job& j, Work& w) {
 }
 
 // Pass temporaries for p panels (plans of up to p members, scan block sums
-// of p or v items, subtree counts).
+// of p or v items).
 int ensure_tmp(Work& w, Job& j, int p, int v, hipStream_t st) {
     const size_t bp = align_up(sizeof(int32_t) * kPlanInts * (size_t)p, 256);
     const size_t bs = align_up(sizeof(int2) * (cdiv(std::max(p, v), kTile) + 1), 256);
-    const size_t bsub = align_up(sizeof(int32_t) * (size_t)p, 256);
-    FS_TRY(w.tmp.ensure(bp + bs + bsub, st));
+    FS_TRY(w.tmp.ensure(bp + bs, st));
     j.plan = w.tmp.as<int32_t>(0);
     j.bsum = w.tmp.as<int2>(bp);
-    j.sub = w.tmp.as<int32_t>(bp + bs);
     return FS_OK;
 }
 
@@ -297,15 +291,15 @@ int ensure_tmp(Work& w, Job& j, int p, int v, hipStream_t st) {
 int ensure_finish(Work& w, Job& j, int v, hipStream_t st) {
     const size_t n = (size_t)std::max(v, 1);
     const size_t b4 = align_up(4 * n, 256), b8 = align_up(8 * n, 256);
-    const size_t bsl4 = align_up(4 * n * kSlots, 256), bsl8 = align_up(8 * n * kSlots, 256);
+    const size_t bsl8 = align_up(8 * n * kSlots, 256);
     const size_t b2 = align_up(sizeof(int2) * n, 256);
     const size_t rs = radix_sort_bytes((int)n);
-    FS_TRY(w.fin.ensure(b4 + bsl4 + bsl8 + 2 * b8 + b4 + 2 * b2 + rs, st));
+    FS_TRY(w.fin.ensure(b4 + 2 * bsl8 + 2 * b8 + b4 + 2 * b2 + rs, st));
     char* c = w.fin.as<char>();
     j.nslot = reinterpret_cast<int*>(c);
     c += b4;
-    j.srank = reinterpret_cast<int32_t*>(c);
-    c += bsl4;
+    j.spath = reinterpret_cast<uint64_t*>(c);
+    c += bsl8;
     j.sval = reinterpret_cast<double*>(c);
     c += bsl8;
     j.key = reinterpret_cast<uint64_t*>(c);
@@ -468,11 +462,9 @@ int refine(std::vector<Input>& in, std::vector<Output>& out, DevBuf& jobs_buf, h
         maxv = std::max(maxv, hc[k].V);
     }
     FS_TRY(upload());
-    const dim3 gpv(cdiv(std::max(std::max(maxp, maxv), 1), kB), (unsigned)nj);
     const dim3 gp(cdiv(std::max(maxp, 1), kB), (unsigned)nj);
     const dim3 gv(cdiv(std::max(maxv, 1), kB), (unsigned)nj);
-    leaf_init_kernel<<<gpv, kB, 0, st>>>(dj);
-    leaf_up_kernel<<<gp, kB, 0, st>>>(dj);
+    leaf_init_kernel<<<gv, kB, 0, st>>>(dj);
     leaf_terms_kernel<<<gp, kB, 0, st>>>(dj);
     weight_kernel<<<gv, kB, 0, st>>>(dj);
     key_kernel<<<gv, kB, 0, st>>>(dj);

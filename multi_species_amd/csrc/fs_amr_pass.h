@@ -35,7 +35,7 @@ enum : int {
 };
 constexpr int kPlanInts = 12;  // member, flags, 4 side kinds, 4 side targets, 2 spare
 constexpr int kSlots = 6;      // leaves whose nine points hold one point (<= 4)
-constexpr int kMaxDepth = 64;
+constexpr int kMaxDepth = 32;  // levels a depth-first path key holds (2 bits each)
 constexpr int kChainStack = 24;
 
 // Side plans of a member (what the link step does on that side).
@@ -80,10 +80,9 @@ struct Job {
     // pass temporaries
     int32_t* plan;  // by rank
     int2* bsum;
-    int32_t* sub;   // subtree leaf counts (finish)
     // finishing, by point
     int* nslot;
-    int32_t* srank;
+    uint64_t* spath;  // depth-first path key of each slot's leaf
     double* sval;
     uint64_t* key;
     uint64_t* skey;
@@ -475,35 +474,22 @@ FS_HD void commit_one(const Job& J) {
 FS_HD bool is_leaf(int flags) { return (flags & (RXP | RP)) == 0; }
 
 FS_HD void leaf_init_one(const Job& J, int k) {
-    if (k < J.ctr->P) J.sub[k] = is_leaf(J.rec[(size_t)k * kI + FLAGS]) ? 1 : 0;
     if (k < J.ctr->V) J.nslot[k] = 0;
 }
 
-template <class Ops>
-FS_HD void leaf_up_one(const Job& J, int k, const Ops& ops) {
-    const int P = J.ctr->P;
-    if (k >= P || !is_leaf(J.rec[(size_t)k * kI + FLAGS])) return;
-    int a = k, depth = 0;
-    while (a != 0) {
-        const int b = J.rec[(size_t)a * kI + PARENT];
-        if (b < 0 || b >= P || ++depth > kMaxDepth) {
-            ops.or_(&J.ctr->status, kStatDepth);
-            return;
-        }
-        ops.add(&J.sub[b], 1);
-        a = b;
-    }
-}
-
-// Leaf k's depth-first rank (children in index order) and its nine
-// quadrature terms q dx dp / 4 {1,2,1,2,4,2,1,2,1} or / 9 {1,4,1,4,16,4,1,4,1}.
+// Leaf k's nine quadrature terms q dx dp / 4 {1,2,1,2,4,2,1,2,1} or / 9
+// {1,4,1,4,16,4,1,4,1}, each with the leaf's depth-first path key: the child
+// index at every level from the root, 2 bits a level, left-aligned. No leaf's
+// path is a prefix of another's, so the keys order the leaves as the
+// depth-first walk of set_leaves_weights does (children in index order).
 template <class Ops>
 FS_HD void leaf_terms_one(const Job& J, int k, const Ops& ops) {
     const int P = J.ctr->P, V = J.ctr->V;
     if (k >= P) return;
     const int32_t* rk = J.rec + (size_t)k * kI;
     if (!is_leaf(rk[FLAGS])) return;
-    int rank = 0, a = k, depth = 0;
+    uint64_t path = 0;
+    int a = k, depth = 0;
     while (a != 0) {
         const int b = J.rec[(size_t)a * kI + PARENT];
         const int cs = b >= 0 && b < P ? J.rec[(size_t)b * kI + CHILD] : -1;
@@ -511,7 +497,7 @@ FS_HD void leaf_terms_one(const Job& J, int k, const Ops& ops) {
             ops.or_(&J.ctr->status, kStatDepth);
             return;
         }
-        for (int c = cs; c < a; ++c) rank += J.sub[c];
+        path = (path >> 2) | ((uint64_t)(a - cs) << 62);
         a = b;
     }
     bool ok = true;
@@ -532,7 +518,7 @@ FS_HD void leaf_terms_one(const Job& J, int k, const Ops& ops) {
             ops.or_(&J.ctr->status, kStatSlots);
             continue;
         }
-        J.srank[(size_t)pt * kSlots + s] = rank;
+        J.spath[(size_t)pt * kSlots + s] = path;
         J.sval[(size_t)pt * kSlots + s] = w * kt;
     }
 }
@@ -541,14 +527,14 @@ FS_HD void leaf_terms_one(const Job& J, int k, const Ops& ops) {
 FS_HD void weight_one(const Job& J, int i) {
     if (i >= J.ctr->V) return;
     const int n = J.nslot[i] < kSlots ? J.nslot[i] : kSlots;
-    int rk[kSlots];
+    uint64_t rk[kSlots];
     double v[kSlots];
     for (int s = 0; s < n; ++s) {
-        rk[s] = J.srank[(size_t)i * kSlots + s];
+        rk[s] = J.spath[(size_t)i * kSlots + s];
         v[s] = J.sval[(size_t)i * kSlots + s];
     }
     for (int s = 1; s < n; ++s) {
-        const int r = rk[s];
+        const uint64_t r = rk[s];
         const double t = v[s];
         int u = s - 1;
         while (u >= 0 && rk[u] > r) {

@@ -68,7 +68,7 @@ const int kGarbage = 0x5a5a5a5a;
 Tree replay(const mesh::PanelTree& base, const mesh::Spec& s, double q, bool simpson, const F& fn) {
     const double nan = std::nan("");
     std::vector<int32_t> rec(base.rec.size(), kGarbage), chain(base.size(), 7),
-        plan((size_t)base.size() * kPlanInts, 3), sub(base.size(), kGarbage);
+        plan((size_t)base.size() * kPlanInts, 3);
     std::vector<int2> cnt(base.size(), make_int2(1, 9)), off(base.size(), make_int2(5, 5));
     std::vector<double> x(base.xs.size(), nan), p(base.xs.size(), nan), f(base.xs.size(), nan);
     std::copy(base.rec.begin(), base.rec.end(), rec.begin());
@@ -99,7 +99,6 @@ Tree replay(const mesh::PanelTree& base, const mesh::Spec& s, double q, bool sim
         J.cnt = cnt.data();
         J.off = off.data();
         J.plan = plan.data();
-        J.sub = sub.data();
         J.cap_p = (int)chain.size();
         J.cap_v = (int)x.size();
     };
@@ -127,7 +126,6 @@ Tree replay(const mesh::PanelTree& base, const mesh::Spec& s, double q, bool sim
             rec.resize(np_ * kI, kGarbage);
             chain.resize(np_, 7);
             plan.resize(np_ * kPlanInts, 3);
-            sub.resize(np_, kGarbage);
             cnt.resize(np_, make_int2(1, 9));
             off.resize(np_, make_int2(5, 5));
         }
@@ -146,14 +144,13 @@ Tree replay(const mesh::PanelTree& base, const mesh::Spec& s, double q, bool sim
     // weights
     const int V = C.V, P = C.P;
     std::vector<int> nslot(V, 3);
-    std::vector<int32_t> srank((size_t)V * kSlots, kGarbage);
+    std::vector<uint64_t> spath((size_t)V * kSlots, 0x5a5a5a5a5a5a5a5aull);
     std::vector<double> sval((size_t)V * kSlots, nan), bw(V, nan);
     J.nslot = nslot.data();
-    J.srank = srank.data();
+    J.spath = spath.data();
     J.sval = sval.data();
     J.bw = bw.data();
-    launch(0, std::max(P, V), [&](int k) { leaf_init_one(J, k); });
-    launch(0, P, [&](int k) { leaf_up_one(J, k, ops); });
+    launch(0, V, [&](int k) { leaf_init_one(J, k); });
     launch(0, P, [&](int k) { leaf_terms_one(J, k, ops); });
     launch(0, V, [&](int k) { weight_one(J, k); });
     // reduced-x map: keys, a stable sort, run heads, their scan, the map
