@@ -180,7 +180,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="c4: only the batched steps (no FP64 comparison runs, no error checks; "
                          "for profiling the batched step alone)")
-    ap.add_argument("--requests", type=int, default=12, help="c5: requests served")
+    ap.add_argument("--requests", type=int, default=40, help="c5: requests served")
     ap.add_argument("--req-steps", type=int, default=50, help="c5: steps per request")
     ap.add_argument("--blocks", default="c1,c4,c5",
                     help="field workload: the step-loop blocks added to the line "
@@ -374,7 +374,7 @@ def main():
     if "c4" in blocks:
         result["batched"] = c4_block(ctx, 64, 10, 2)
     if "c5" in blocks:
-        result["served"] = c5_block(ctx, 12, 10)
+        result["served"] = c5_block(ctx, 40, 20)
     ctx.close()
     print(json.dumps(result))
 

@@ -63,8 +63,9 @@ int fs_emq_periodic_f64_dev(fs_ctx* ctx, double* d_es, const double* d_targets,
  * is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc). Its work
  * space is a fixed 256 MiB array per device, held until the process ends;
- * the pair-once sum (targets = sources from 16384 points) takes about 160 B
- * per point, so it accepts at most about 1.6e6 such points and refuses larger
+ * the pair-once sum (targets = sources from 16384 points) takes about 150-210 B
+ * per point (its partials and the sort of the points), so it accepts at most
+ * about 1.6e6 such points and refuses larger
  * ones with FS_E_RANGE before any launch. Calls overlapping on one device
  * race for that array. Calls no hipMalloc, hipFree or allocating routine. */
 int fs_emq_periodic_f64_noalloc(fs_ctx* ctx, double* d_es, const double* d_targets, int64_t nt,
@@ -133,9 +134,10 @@ int fs_external_field_add_dev(fs_ctx* ctx, double* d_es, const double* d_targets
                               void* stream);
 
 /* Tuning knobs of the direct-sum kernels (0 = automatic). targets_per_thread
- * in {1,2,4} (the mixed-precision kernels always take 4: each wavefront
- * makes its near/far choices for its 256 consecutive sorted targets); source_splits >= 1. Applies to
- * later calls on this context. */
+ * in {1,2,4,8} (8: the FP64 pair-once form only, its default from 2^19 points;
+ * the mixed-precision kernels always take 4: each wavefront makes its
+ * near/far choices for its 256 consecutive sorted targets); source_splits >=
+ * 1. Applies to later calls on this context. */
 int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits);
 
 /* FP64 direct sum with the targets being the sources (nt == ns and the same

@@ -154,7 +154,8 @@ bool use_sym(int64_t nt, int64_t ns, bool same, const FieldTuning& tun) {
 
 size_t field_scratch_bytes(int64_t nt, int64_t ns, bool same, int num_cus,
                            const FieldTuning& tun) {
-    if (use_sym(nt, ns, same, tun)) return sym::scratch_bytes(sym::plan(nt, tun, tun.sym_budget));
+    if (use_sym(nt, ns, same, tun))
+        return sym::scratch_bytes(sym::plan(nt, tun, tun.sym_budget)) + sym::sort_bytes(nt);
     const Plan p = plan_direct(nt, ns, num_cus, tun);
     return p.splits > 1 ? align_up(sizeof(double) * nt * p.splits, 256) : 0;
 }
@@ -182,7 +183,7 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src, const
     const double delta2 = delta * delta;
     if (use_sym(nt, ns, d_tgt == d_src, tun)) {
         const sym::Plan P = sym::plan(nt, tun, tun.sym_budget);
-        FS_HIP_CHECK(sym::run<false>(d_src, d_w, P, invL, delta2, scratch, d_es, stream));
+        FS_TRY(sym::run_sorted(d_src, d_w, P, L, invL, delta2, scratch, d_es, stream));
         if (pair_evals) *pair_evals = sym::evals(P);
         return FS_OK;
     }
@@ -268,15 +269,18 @@ int fs_emq_periodic_f64_noalloc(fs_ctx* ctx, double* d_es, const double* d_targe
         ws = fs::noalloc_ptr(ctx->device);
     }
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-    // the pair-once partials in passes within the fixed array
+    // the pair-once partials in passes within the fixed array, beside the
+    // sort of the points
     fs::FieldTuning tun = ctx->tuning;
-    tun.sym_budget = (int64_t)fs::kNoAllocBytes;
+    tun.sym_budget = std::max<int64_t>(
+        1, (int64_t)fs::kNoAllocBytes - (int64_t)fs::sym::sort_bytes(std::max<int64_t>(nt, 1)));
     const size_t need = fs::field_scratch_bytes(nt, ns, d_targets == d_sources, ctx->num_cus, tun);
     if (need > fs::kNoAllocBytes)
         return fs::fail(FS_E_RANGE,
                         "fs_emq_periodic_f64_noalloc: %lld points need %zu bytes of work space, "
-                        "beyond its fixed %zu (the pair-once sum takes about 160 B per point: "
-                        "at most about 1.6e6 points)",
+                        "beyond its fixed %zu (the pair-once sum, its partials and the sort of "
+                        "the points, takes about 150-210 B per point: at most about 1.6e6 "
+                        "points)",
                         (long long)nt, need, fs::kNoAllocBytes);
     return fs::field_direct_f64(d_targets, nt, d_sources, d_q_ws, ns, L, epsilon, d_es, ws,
                                 ctx->num_cus, tun, st, &ctx->f64_pair_evals);
@@ -285,8 +289,8 @@ int fs_emq_periodic_f64_noalloc(fs_ctx* ctx, double* d_es, const double* d_targe
 int fs_emq_set_tuning(fs_ctx* ctx, int targets_per_thread, int source_splits) {
     if (!ctx) return fs::fail(FS_E_INVALID, "null context");
     if (targets_per_thread != 0 && targets_per_thread != 1 && targets_per_thread != 2 &&
-        targets_per_thread != 4)
-        return fs::fail(FS_E_INVALID, "targets_per_thread must be 0, 1, 2 or 4");
+        targets_per_thread != 4 && targets_per_thread != 8)
+        return fs::fail(FS_E_INVALID, "targets_per_thread must be 0, 1, 2, 4 or 8");
     if (source_splits < 0) return fs::fail(FS_E_INVALID, "source_splits must be >= 0");
     ctx->tuning.targets_per_thread = targets_per_thread;
     ctx->tuning.source_splits = source_splits;

@@ -196,7 +196,7 @@ private:
     // FP64 segments that are their own source and large enough: each pair
     // once (fs_sym.h), one launch per targets-per-lane value
     std::vector<SymSeg> symsegs_;
-    int sym_first_[3] = {0, 0, 0}, sym_count_[3] = {0, 0, 0}, sym_items_[3] = {0, 0, 0};
+    int sym_first_[4] = {0, 0, 0, 0}, sym_count_[4] = {0, 0, 0, 0}, sym_items_[4] = {0, 0, 0, 0};
     SymSeg* d_symsegs_ = nullptr;
     size_t o_sympart_ = 0;
     std::vector<SegDev> segs_;
@@ -222,17 +222,5 @@ private:
     int lstart_[2] = {0, 0}, nunits_[2] = {0, 0};  // far-field unit records per level
 };
 
-// Stable radix sort by (segment of the original index, top 24 key bits)
-// (fs_radix.hip). kin must not alias kout.
-size_t radix_sort_bytes(int n);
-int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, uint64_t* kout,
-                   int32_t* iout, void* tmp, hipStream_t stream);
-// Stable sort by all 64 key bits (one segment).
-int radix_sort_full(const uint64_t* kin, int n, uint64_t* kout, int32_t* iout, void* tmp,
-                    hipStream_t stream);
-// Stable sort by the top key_bytes bytes of the keys (one segment); iout is
-// the permutation (original index of each sorted key).
-int radix_sort_top(const uint64_t* kin, int n, int key_bytes, uint64_t* kout, int32_t* iout,
-                   void* tmp, hipStream_t stream);
 
 }  // namespace fs

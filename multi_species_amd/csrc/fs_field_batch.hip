@@ -171,11 +171,11 @@ __global__ __launch_bounds__(sym::threads<T>()) void batch_sym_kernel(
     }
     const SymSeg E = tab[lo];
     if (E.open)
-        sym::item<T, true>(x + E.off, w + E.off, E.P, E.S, (int)blockIdx.x - E.item_base, E.invL,
-                           E.delta2, sympart + E.rbase, sympart + E.cbase, red);
+        sym::item<T, true>(x + E.off, w + E.off, E.P, E.S, (int)blockIdx.x - E.item_base, 0.0,
+                           E.invL, E.delta2, sympart + E.rbase, sympart + E.cbase, red);
     else
-        sym::item<T, false>(x + E.off, w + E.off, E.P, E.S, (int)blockIdx.x - E.item_base, E.invL,
-                            E.delta2, sympart + E.rbase, sympart + E.cbase, red);
+        sym::item<T, false>(x + E.off, w + E.off, E.P, E.S, (int)blockIdx.x - E.item_base, 0.0,
+                            E.invL, E.delta2, sympart + E.rbase, sympart + E.cbase, red);
 }
 
 // A point's partials in a fixed order; open-boundary segments subtract the
@@ -1468,7 +1468,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     int64_t sym_elems = 0;
     {
         std::vector<SymSeg> grouped;
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 4; ++k) {
             const int t = 1 << k;
             sym_first_[k] = (int)grouped.size();
             sym_items_[k] = 0;
@@ -1810,6 +1810,9 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
         if (sym_items_[2] > 0)
             batch_sym_kernel<4><<<sym_items_[2], sym::threads<4>(), 0, stream>>>(
                 d_symsegs_, sym_first_[2], sym_count_[2], x, w, sympart);
+        if (sym_items_[3] > 0)
+            batch_sym_kernel<8><<<sym_items_[3], sym::threads<8>(), 0, stream>>>(
+                d_symsegs_, sym_first_[3], sym_count_[3], x, w, sympart);
         FS_HIP_CHECK(hipGetLastError());
         finalize_f64_kernel<<<nblocks(n_total_), kBlock, 0, stream>>>(
             n_total_, d_seg_of_, d_segs_, part, d_symsegs_, sympart, x, es, d_counter_);

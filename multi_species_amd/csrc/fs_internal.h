@@ -75,6 +75,19 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src,
                      const FieldTuning& tun, hipStream_t stream,
                      int64_t* pair_evals = nullptr);
 
+// Stable radix sort by (segment of the original index, top 24 key bits)
+// (fs_radix.hip). kin must not alias kout.
+size_t radix_sort_bytes(int n);
+int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, uint64_t* kout,
+                   int32_t* iout, void* tmp, hipStream_t stream);
+// Stable sort by all 64 key bits (one segment).
+int radix_sort_full(const uint64_t* kin, int n, uint64_t* kout, int32_t* iout, void* tmp,
+                    hipStream_t stream);
+// Stable sort by the top key_bytes bytes of the keys (one segment); iout is
+// the permutation (original index of each sorted key).
+int radix_sort_top(const uint64_t* kin, int n, int key_bytes, uint64_t* kout, int32_t* iout,
+                   void* tmp, hipStream_t stream);
+
 // Per-species diagnostics on device arrays (fs_diag.hip): FS_DIAG_COUNT
 // values into out (host); work holds diag_work_bytes(n) bytes. Synchronizes
 // `stream`.

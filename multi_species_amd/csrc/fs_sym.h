@@ -42,9 +42,12 @@ namespace fs {
 namespace sym {
 
 // Wavefronts per workgroup: 4 for one target per lane (the decks below
-// 65536 points, where more, smaller row blocks keep the chip busy), else 8.
-template <int T> constexpr int waves() { return T == 1 ? 4 : 8; }
+// 65536 points, where more, smaller row blocks keep the chip busy) and for
+// eight (a tuning choice: fewer lane moves per evaluation, 110 VGPRs, the
+// same 2048-point row blocks as four), else 8.
+template <int T> constexpr int waves() { return T == 1 || T == 8 ? 4 : 8; }
 template <int T> constexpr int threads() { return 64 * waves<T>(); }
+inline int threads_of(int t) { return t == 1 || t == 8 ? threads<1>() : threads<2>(); }
 constexpr int kMaxWaves = 8;
 // Points from which the pair-once form is the default (below it the row
 // blocks are too few to fill the chip and the full sum is faster).
@@ -116,11 +119,11 @@ inline Plan plan(int64_t n, const FieldTuning& tun, int64_t budget = 0) {
     Plan P{};
     P.n = (int)n;
     if (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
-        tun.targets_per_thread == 4)
+        tun.targets_per_thread == 4 || tun.targets_per_thread == 8)
         P.t = tun.targets_per_thread;
     else
         P.t = n >= (int64_t(1) << 18) ? 4 : n >= (int64_t(1) << 16) ? 2 : 1;
-    P.rb = (P.t == 1 ? threads<1>() : P.t == 2 ? threads<2>() : threads<4>()) * P.t;
+    P.rb = threads_of(P.t) * P.t;
     P.nb = (int)((n + P.rb - 1) / P.rb);
     P.h = (P.nb - 1) / 2;
     P.even = (P.nb % 2 == 0) ? 1 : 0;
@@ -181,9 +184,18 @@ __device__ __forceinline__ double wave_rotate(double v) {
 // kOpen: the open-boundary kernel z / sqrt(z^2 + delta2) with z = x_i - x_j
 // (E_MQ_DirectSum_openbcs, src/FieldStructure.cpp:166-191; also odd in z;
 // invL unused) instead of the periodic one.
-template <int T, bool kOpen = false>
+// kSorted: x is sorted ascending. A row block and a 64-source tile then span
+// the displacements [x_first - x'_last, x_last - x'_first]; when that range
+// sits inside one period's window (|d| < L/2, or d - L, or d + L, with a
+// margin), every pair of the tile has the same wrap, and the tile computes
+// z = (x_i - x_j) / L with the tile's sources moved by one period beforehand
+// (forward where d > L/2, back where d < -L/2): no rint per pair. Other tiles (those
+// straddling the half-period) take the rint form. For |d| < L/2 both forms
+// round alike; K is odd and vanishes at z = +-1/2, so a tile whose range
+// touches the half-period within the margin loses nothing to either choice.
+template <int T, bool kOpen = false, bool kSorted = false>
 __device__ __forceinline__ void item(const double* __restrict__ x, const double* __restrict__ w,
-                                     const Plan& P, const Pass& S, int id, double invL,
+                                     const Plan& P, const Pass& S, int id, double L, double invL,
                                      double delta2, double* __restrict__ rowpart,
                                      double* __restrict__ colpart,
                                      double (*red)[kMaxWaves][64]) {
@@ -215,6 +227,13 @@ __device__ __forceinline__ void item(const double* __restrict__ x, const double*
         lo[t] = 0.0;
     }
     const double C = kOpen ? 0.0 : dev::rsq_f64(__builtin_fma(0.5, 0.5, delta2));
+    // the row block's first and last points (kSorted)
+    double r0 = 0.0, r1 = 0.0;
+    if constexpr (kSorted) {
+        const int pa = b * rb;
+        r0 = x[pa];
+        r1 = x[min(pa + rb, n) - 1];
+    }
     int tile = 0;
     for (int d = d0; d < d1; ++d) {
         const int B = b + d < P.nb ? b + d : b + d - P.nb;
@@ -223,26 +242,61 @@ __device__ __forceinline__ void item(const double* __restrict__ x, const double*
             double xj = q < n ? x[q] : 0.0;
             double wj = q < n ? w[q] : 0.0;
             double accj = 0.0;
-#pragma unroll 2
-            for (int s = 0; s < 64; ++s) {
-#pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    double k;
-                    if constexpr (kOpen) {
-                        const double z = xi[t] - xj;
-                        k = z * dev::rsq_f64(__builtin_fma(z, z, delta2));
-                    } else {
-                        const double u = (xi[t] - xj) * invL;
-                        const double z = u - __builtin_rint(u);
-                        const double y = dev::rsq_f64(__builtin_fma(z, z, delta2));
-                        k = z * (y - C);
-                    }
-                    acc[t] = __builtin_fma(wj, k, acc[t]);
-                    accj = __builtin_fma(nwi[t], k, accj);
+            // the tile's wrap: 0 (none), +1 (targets back one period), -1
+            // (sources back one period), 2 (per pair)
+            int wrap = 2;
+            if constexpr (kSorted && !kOpen) {
+                const int q0 = B * rb + jt * 64;
+                if (q0 < n) {
+                    const double lo = (r0 - x[min(q0 + 63, n - 1)]) * invL;
+                    const double hi = (r1 - x[q0]) * invL;
+                    constexpr double m = 1e-9;
+                    if (hi < 0.5 - m && lo > -0.5 + m) wrap = 0;
+                    else if (lo > 0.5 + m && hi < 1.5 - m) wrap = 1;
+                    else if (hi < -0.5 - m && lo > -1.5 + m) wrap = -1;
                 }
-                xj = wave_rotate(xj);
-                wj = wave_rotate(wj);
-                accj = wave_rotate(accj);
+            }
+            if (wrap != 2) {
+                // the tile's sources one period on (targets beyond L/2 ahead of
+                // them) or back
+                if (wrap == 1) xj = xj + L;
+                if (wrap == -1) xj = xj - L;
+#pragma unroll(T > 4 ? 1 : 2)
+                for (int s = 0; s < 64; ++s) {
+#pragma unroll
+                    for (int t = 0; t < T; ++t) {
+                        const double z = (xi[t] - xj) * invL;
+                        const double y = dev::rsq_f64(__builtin_fma(z, z, delta2));
+                        const double k = z * (y - C);
+                        acc[t] = __builtin_fma(wj, k, acc[t]);
+                        accj = __builtin_fma(nwi[t], k, accj);
+                    }
+                    xj = wave_rotate(xj);
+                    wj = wave_rotate(wj);
+                    accj = wave_rotate(accj);
+                }
+            } else {
+#pragma unroll 2
+                for (int s = 0; s < 64; ++s) {
+#pragma unroll
+                    for (int t = 0; t < T; ++t) {
+                        double k;
+                        if constexpr (kOpen) {
+                            const double z = xi[t] - xj;
+                            k = z * dev::rsq_f64(__builtin_fma(z, z, delta2));
+                        } else {
+                            const double u = (xi[t] - xj) * invL;
+                            const double z = u - __builtin_rint(u);
+                            const double y = dev::rsq_f64(__builtin_fma(z, z, delta2));
+                            k = z * (y - C);
+                        }
+                        acc[t] = __builtin_fma(wj, k, acc[t]);
+                        accj = __builtin_fma(nwi[t], k, accj);
+                    }
+                    xj = wave_rotate(xj);
+                    wj = wave_rotate(wj);
+                    accj = wave_rotate(accj);
+                }
             }
 #pragma unroll
             for (int t = 0; t < T; ++t) {
@@ -311,33 +365,34 @@ __device__ __forceinline__ double pass_point(const Plan& P, const Pass& S, int p
     return 0.5 * (s + lo);
 }
 
-template <int T, bool kOpen>
+template <int T, bool kOpen, bool kSorted>
 __global__ __launch_bounds__(threads<T>()) void pass_kernel(
-    const double* __restrict__ x, const double* __restrict__ w, Plan P, Pass S, double invL,
-    double delta2, double* __restrict__ rowpart, double* __restrict__ colpart) {
+    const double* __restrict__ x, const double* __restrict__ w, Plan P, Pass S, double L,
+    double invL, double delta2, double* __restrict__ rowpart, double* __restrict__ colpart) {
     __shared__ double red[2][kMaxWaves][64];
-    item<T, kOpen>(x, w, P, S, blockIdx.x, invL, delta2, rowpart, colpart, red);
+    item<T, kOpen, kSorted>(x, w, P, S, blockIdx.x, L, invL, delta2, rowpart, colpart, red);
 }
 
 // kOpen: E = (s + lo) / 2 - x_p (the open-boundary form's uniform background).
+// perm (sorted points): E of sorted point p goes to es[perm[p]].
 template <bool kOpen>
 __global__ __launch_bounds__(256) void fold_kernel(Plan P, Pass S, int last,
                                                    const double* __restrict__ rowpart,
                                                    const double* __restrict__ colpart,
                                                    double* __restrict__ state,
                                                    const double* __restrict__ x,
+                                                   const int32_t* __restrict__ perm,
                                                    double* __restrict__ es) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= P.n) return;
     const double e = pass_point(P, S, p, last != 0, rowpart, colpart, state);
-    if (last) es[p] = kOpen ? e - x[p] : e;
+    if (last) es[perm ? perm[p] : p] = kOpen ? e - x[p] : e;
 }
 
-// Every pass of plan P over x[0..n), w[0..n) on `stream`, E into es. scratch:
-// scratch_bytes(P) bytes.
-template <bool kOpen>
-inline hipError_t run(const double* x, const double* w, const Plan& P, double invL,
-                      double delta2, void* scratch, double* es, hipStream_t stream) {
+template <bool kOpen, bool kSorted>
+inline hipError_t launch_passes(const double* x, const double* w, const Plan& P, double L,
+                                double invL, double delta2, void* scratch,
+                                const int32_t* perm, double* es, hipStream_t stream) {
     double* rowpart = static_cast<double*>(scratch);
     double* colpart = reinterpret_cast<double*>(static_cast<char*>(scratch) +
                                                 al256(rowpart_elems(P)));
@@ -347,27 +402,107 @@ inline hipError_t run(const double* x, const double* w, const Plan& P, double in
     for (int k = 0; k < P.npass; ++k) {
         const Pass S = pass(P, k);
         switch (P.t) {
+            case 8:
+                pass_kernel<8, kOpen, kSorted><<<S.nitems, threads<8>(), 0, stream>>>(
+                    x, w, P, S, L, invL, delta2, rowpart, colpart);
+                break;
             case 4:
-                pass_kernel<4, kOpen><<<S.nitems, threads<4>(), 0, stream>>>(
-                    x, w, P, S, invL, delta2, rowpart, colpart);
+                pass_kernel<4, kOpen, kSorted><<<S.nitems, threads<4>(), 0, stream>>>(
+                    x, w, P, S, L, invL, delta2, rowpart, colpart);
                 break;
             case 2:
-                pass_kernel<2, kOpen><<<S.nitems, threads<2>(), 0, stream>>>(
-                    x, w, P, S, invL, delta2, rowpart, colpart);
+                pass_kernel<2, kOpen, kSorted><<<S.nitems, threads<2>(), 0, stream>>>(
+                    x, w, P, S, L, invL, delta2, rowpart, colpart);
                 break;
             default:
-                pass_kernel<1, kOpen><<<S.nitems, threads<1>(), 0, stream>>>(
-                    x, w, P, S, invL, delta2, rowpart, colpart);
+                pass_kernel<1, kOpen, kSorted><<<S.nitems, threads<1>(), 0, stream>>>(
+                    x, w, P, S, L, invL, delta2, rowpart, colpart);
                 break;
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         fold_kernel<kOpen><<<(P.n + 255) / 256, 256, 0, stream>>>(
-            P, S, k == P.npass - 1 ? 1 : 0, rowpart, colpart, state, x, es);
+            P, S, k == P.npass - 1 ? 1 : 0, rowpart, colpart, state, x, perm, es);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// Every pass of plan P over x[0..n), w[0..n) on `stream`, E into es, in the
+// points' own order. scratch: scratch_bytes(P) bytes.
+template <bool kOpen>
+inline hipError_t run(const double* x, const double* w, const Plan& P, double L, double invL,
+                      double delta2, void* scratch, double* es, hipStream_t stream) {
+    return launch_passes<kOpen, false>(x, w, P, L, invL, delta2, scratch, nullptr, es, stream);
+}
+
+// ---- the periodic sum on points sorted by x -------------------------------
+
+// Work space of the sort in front of the passes: keys, sorted keys, the
+// permutation, sorted x and w, and the radix sort's own.
+inline size_t sort_bytes(int64_t n) {
+    return 4 * al256(n) + (((size_t)n * 4 + 255) / 256 * 256) + radix_sort_bytes((int)n);
+}
+
+// Order-preserving image of a double (+0 and -0 one value).
+__device__ __forceinline__ uint64_t order_key(double v) {
+    if (v == 0.0) v = 0.0;
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+namespace {  // one copy per translation unit that launches them
+
+__global__ __launch_bounds__(256) void key_kernel(const double* __restrict__ x, int n,
+                                                  uint64_t* __restrict__ key) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) key[i] = order_key(x[i]);
+}
+
+__global__ __launch_bounds__(256) void gather_kernel(const double* __restrict__ x,
+                                                     const double* __restrict__ w,
+                                                     const int32_t* __restrict__ perm, int n,
+                                                     double* __restrict__ xs,
+                                                     double* __restrict__ ws) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int i = perm[k];
+    xs[k] = x[i];
+    ws[k] = w[i];
+}
+
+}  // namespace
+
+// The periodic pair-once sum with the points sorted by x first (stable: equal
+// x keep their order), so that nearly every tile pair wraps as a whole (see
+// item); E is written back in the points' own order. scratch: scratch_bytes(P)
+// + sort_bytes(n) bytes.
+inline int run_sorted(const double* x, const double* w, const Plan& P, double L, double invL,
+                      double delta2, void* scratch, double* es, hipStream_t stream) {
+    const int n = P.n;
+    char* c = static_cast<char*>(scratch) + scratch_bytes(P);
+    uint64_t* key = reinterpret_cast<uint64_t*>(c);
+    c += al256(n);
+    uint64_t* skey = reinterpret_cast<uint64_t*>(c);
+    c += al256(n);
+    double* xs = reinterpret_cast<double*>(c);
+    c += al256(n);
+    double* ws = reinterpret_cast<double*>(c);
+    c += al256(n);
+    int32_t* perm = reinterpret_cast<int32_t*>(c);
+    c += ((size_t)n * 4 + 255) / 256 * 256;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    key_kernel<<<g, 256, 0, stream>>>(x, n, key);
+    if (hipGetLastError() != hipSuccess) return fail(FS_E_HIP, "pair-once sort: key launch failed");
+    FS_TRY(radix_sort_full(key, n, skey, perm, c, stream));
+    gather_kernel<<<g, 256, 0, stream>>>(x, w, perm, n, xs, ws);
+    if (hipGetLastError() != hipSuccess)
+        return fail(FS_E_HIP, "pair-once sort: gather launch failed");
+    const hipError_t e = launch_passes<false, true>(xs, ws, P, L, invL, delta2, scratch, perm, es,
+                                                    stream);
+    if (e != hipSuccess) return fail(FS_E_HIP, "pair-once sum: %s", hipGetErrorString(e));
+    return FS_OK;
 }
 
 }  // namespace sym

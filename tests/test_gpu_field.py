@@ -87,9 +87,10 @@ def test_launch_shapes_agree(gpu_ctx, tpt, splits):
     assert rel(e1, ex) <= 1e-14
 
 
-@pytest.mark.parametrize("n,tpt", [(1, 1), (2, 1), (63, 1), (64, 2), (65, 4), (511, 1),
-                                   (512, 1), (513, 1), (1025, 1), (1536, 2), (2049, 1),
-                                   (4097, 4), (16770, 0), (40000, 0)])
+@pytest.mark.parametrize("n,tpt", [(1, 1), (2, 1), (63, 1), (64, 2), (65, 4), (65, 8),
+                                   (511, 1), (512, 1), (513, 1), (1025, 1), (1536, 2),
+                                   (2049, 1), (4097, 4), (4097, 8), (8193, 8), (16770, 0),
+                                   (40000, 0)])
 def test_pair_once_ragged(gpu_ctx, n, tpt):
     """Targets = sources, each pair evaluated once and given to both points:
     ragged sizes around the 64-point tile and the row block (512, 1024, 2048
@@ -111,6 +112,34 @@ def test_pair_once_ragged(gpu_ctx, n, tpt):
         assert e[0] == 0.0
     else:
         assert rel(e, ex) <= 1e-14, rel(e, ex)
+
+
+@pytest.mark.parametrize("tpt", [2, 8])
+def test_pair_once_positions_beyond_the_period(gpu_ctx, tpt):
+    """The reference does not wrap positions (the RK4 step moves them out of
+    [0, L)): half the points spread over [-1.3L, 2.2L), a cluster straddling
+    the periodic seam at L and one at L/2 (tiles whose displacements reach
+    +-L/2 and +-3L/2). The pair-once sum sorts the points and gives each tile
+    of sources one wrap where its displacement range allows, per pair
+    otherwise; against the extended oracle. A mistake this catches: a tile
+    given one wrap while its pairs need two (its far pairs then enter with
+    the wrong image, an O(1) error)."""
+    rng = np.random.default_rng(3)
+    n = 40000
+    x = np.concatenate([rng.uniform(-1.3 * L0, 2.2 * L0, n // 2),
+                        L0 + rng.normal(0.0, 0.01, n // 4),
+                        0.5 * L0 + rng.normal(0.0, 0.01, n - n // 2 - n // 4)])
+    rng.shuffle(x)
+    w = rng.normal(size=n) * 1e-3
+    gpu_ctx.set_symmetric(2)
+    gpu_ctx.set_tuning(tpt, 0)
+    try:
+        e = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+    finally:
+        gpu_ctx.set_symmetric(0)
+        gpu_ctx.set_tuning(0, 0)
+    ex = oracle.direct_sum_ext(x, x, w, L0, 0.05)
+    assert rel(e, ex) <= 1e-13, rel(e, ex)
 
 
 def test_pair_once_with_copied_positions(gpu_ctx):

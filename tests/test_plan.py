@@ -32,7 +32,7 @@ def test_pair_once_scratch_within_budget(n):
 def test_pair_once_evaluation_count_closed_form():
     # n a multiple of the 2048-point row block: every unordered block pair
     # once plus each block with itself in full = n^2/2 + n*2048/2
-    for n in (1 << 20, 1 << 22):
+    for n in (1 << 18, 1 << 20, 1 << 22):
         assert ms.pair_once_plan(n)[2] == n * n // 2 + n * 1024
     # one pass at 2^22 would need 32 GiB; the default budget takes 10 passes
     assert ms.pair_once_plan(1 << 22, 1 << 62)[0] >= 32 * GIB
