@@ -80,10 +80,16 @@ int shape(fs_ctx* ctx, FieldBatch& fb, int64_t nt, int64_t ns, bool same, const 
     if (ctx->tuning.targets_per_thread != 0) return FS_OK;  // explicit tuning wins
     auto lg = [](int64_t v) { uint64_t b = 0; while ((int64_t(1) << b) < v && b < 62) ++b; return b; };
     const uint64_t key = (lg(nt) << 8) | (lg(ns) << 1) | (same ? 1 : 0);
-    auto it = ctx->shape_cache.find(key);
-    if (it != ctx->shape_cache.end()) return fb.set_shape(it->second.first, it->second.second, st);
+    std::pair<int, int> known{0, 0};
+    {
+        std::lock_guard<std::mutex> lk(ctx->cache_mu);
+        auto it = ctx->shape_cache.find(key);
+        if (it != ctx->shape_cache.end()) known = it->second;
+    }
+    if (known.first) return fb.set_shape(known.first, known.second, st);
     int t = 2, g = 8;
     FS_TRY(fb.autotune(x, w, es, st, &t, &g));
+    std::lock_guard<std::mutex> lk(ctx->cache_mu);
     ctx->shape_cache[key] = std::make_pair(t, g);
     return FS_OK;
 }

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -158,4 +159,8 @@ struct fs_ctx {
     // autotuned launch shapes (tpt, workgroups per CU) of the mixed entry
     // points, by problem class
     std::map<uint64_t, std::pair<int, int>> shape_cache;
+    // the same for the batches of the step loop and the server (by
+    // precision, total points in half-octaves and deck count in octaves)
+    std::map<uint64_t, std::pair<int, int>> batch_shape_cache;
+    std::mutex cache_mu;  // the serving loop's worker thread shares the caches
 };

@@ -882,11 +882,33 @@ int layout(Sim& S, bool keep_resident) {
     // and sets each deck's FP64 band from it
     S.ffull.govern(true);
     S.fred.govern(true);
-    // launch shape of the full field: timed on this batch when its membership
-    // or precision changed, reused when an adaptive remesh only resized it
+    // launch shape of the full field (no result depends on it): timed on this
+    // batch when its membership or precision changed into a size class the
+    // context has not timed yet (total points in half-octaves, decks in
+    // octaves), reused otherwise and when an adaptive remesh only resized it
     FS_HIP_CHECK(hipStreamSynchronize(st));
     if (S.tune_pending && N > 0 && S.ctx->tuning.targets_per_thread == 0) {
-        FS_TRY(S.ffull.autotune(S.xs, S.qw, S.e2, st, &S.tuned_tpt, &S.tuned_wgs));
+        const int nd = (int)S.decks.size();
+        const uint64_t cls = ((uint64_t)S.precision << 40) |
+                             ((uint64_t)(2.0 * std::log2((double)N)) << 8) |
+                             (uint64_t)std::ilogb((double)std::max(nd, 1));
+        bool known = false;
+        {
+            std::lock_guard<std::mutex> lk(S.ctx->cache_mu);
+            auto it = S.ctx->batch_shape_cache.find(cls);
+            if (it != S.ctx->batch_shape_cache.end()) {
+                known = true;
+                S.tuned_tpt = it->second.first;
+                S.tuned_wgs = it->second.second;
+            }
+        }
+        if (known) {
+            FS_TRY(S.ffull.set_shape(S.tuned_tpt, S.tuned_wgs, st));
+        } else {
+            FS_TRY(S.ffull.autotune(S.xs, S.qw, S.e2, st, &S.tuned_tpt, &S.tuned_wgs));
+            std::lock_guard<std::mutex> lk(S.ctx->cache_mu);
+            S.ctx->batch_shape_cache[cls] = {S.tuned_tpt, S.tuned_wgs};
+        }
         S.tune_pending = false;
     } else if (S.ctx->tuning.targets_per_thread == 0) {
         FS_TRY(S.ffull.set_shape(S.tuned_tpt, S.tuned_wgs, st));
