@@ -374,6 +374,12 @@ int fs_sim_remesh_ms(fs_sim* sim, double* ms);
 int fs_sim_set_refinement(fs_sim* sim, int mode);
 int fs_sim_refinement_counts(fs_sim* sim, int64_t* on_device, int64_t* on_host);
 
+/* Steps taken by replaying a captured HIP graph and steps run eagerly (a step
+ * in which an adaptive deck remeshes, any step of a batch with a treecode
+ * deck, and every step with use_graph 0 or timing on). No counterpart in the
+ * reference. */
+int fs_sim_step_counts(fs_sim* sim, int64_t* replayed, int64_t* eager);
+
 /* Host-only: the device bytes a deck is estimated to take in a batch or a
  * server (state, RK4 work, field partials -- in FP64 the pair-once partials of
  * its full field --, sorted copies in mixed precision, panel records); the
@@ -445,8 +451,8 @@ int fs_sim_num_decks(fs_sim* sim, int* n_decks);
 /* Any output may be NULL. iter counts the steps the deck has taken. */
 int fs_sim_deck_info(fs_sim* sim, int deck, int* first_species, int* n_species, int* iter,
                      int64_t* n_points);
-/* precision as above; use_graph 0 runs every step eagerly (adaptive and
- * treecode decks always do). */
+/* precision as above; use_graph 0 runs every step eagerly (steps in which an
+ * adaptive deck remeshes, and batches with a treecode deck, always do). */
 int fs_sim_set_options(fs_sim* sim, int precision, int use_graph);
 /* Mixed precision: what the accuracy governor measured in the deck's last
  * measured full-field evaluation (RK4 stages 2 and 4 measure their own error

@@ -70,6 +70,7 @@ def lib():
         L.fs_sim_set_timing.argtypes = [_vp, _int]
         L.fs_sim_remesh_ms.argtypes = [_vp, ctypes.POINTER(ctypes.c_double)]
         L.fs_sim_set_refinement.argtypes = [_vp, _int]
+        L.fs_sim_step_counts.argtypes = [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]
         L.fs_sim_refinement_counts.argtypes = [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]
         L.fs_emq_treecode_f64.argtypes = [_vp, _dp, _dp, _dp, _i64, _d, _d, _d, _int, _int,
                                           ctypes.POINTER(_i64)]
@@ -586,6 +587,13 @@ class Simulation:
         a, b = ctypes.c_int64(), ctypes.c_int64()
         _check(lib().fs_sim_refinement_counts(self.h, ctypes.byref(a), ctypes.byref(b)),
                "fs_sim_refinement_counts")
+        return a.value, b.value
+
+    def step_counts(self) -> tuple:
+        """(steps replayed from a captured graph, steps run eagerly)."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().fs_sim_step_counts(self.h, ctypes.byref(a), ctypes.byref(b)),
+               "fs_sim_step_counts")
         return a.value, b.value
 
     def set_options(self, precision: int = PRECISION_F64, use_graph: bool = True):

@@ -396,6 +396,7 @@ struct fs_sim {
     int amr_mode = 0;
     int64_t amr_device = 0, amr_host = 0;
     DevBuf amr_jobs;
+    int64_t steps_replayed = 0, steps_eager = 0;  // graph replays, eager steps
 
     hipStream_t stream() const { return ctx->stream; }
 };
@@ -1592,6 +1593,13 @@ int fs_sim_refinement_counts(fs_sim* S, int64_t* on_device, int64_t* on_host) {
     return FS_OK;
 }
 
+int fs_sim_step_counts(fs_sim* S, int64_t* replayed, int64_t* eager) {
+    if (!S) return fail(FS_E_INVALID, "null simulation");
+    if (replayed) *replayed = S->steps_replayed;
+    if (eager) *eager = S->steps_eager;
+    return FS_OK;
+}
+
 int fs_sim_set_timing(fs_sim* S, int on) {
     if (!S) return fail(FS_E_INVALID, "null simulation");
     FS_HIP_CHECK(hipSetDevice(S->ctx->device));
@@ -1655,26 +1663,34 @@ int fs_sim_step(fs_sim* S) {
     }
     const int mode = nrem == 0 ? sim::kNoRemesh
                      : nrem == S->decks.size() ? sim::kAllRemesh : sim::kSomeRemesh;
+    // an adaptive deck remeshing now changes the arrays' sizes (its tree is
+    // rebuilt): that step runs eagerly; every other step (no remesh, or only
+    // static decks remeshing) replays a captured graph
+    bool adaptive_now = false;
+    for (size_t d = 0; d < S->decks.size(); ++d)
+        adaptive_now = adaptive_now || (on[d] && S->decks[d]->adaptive);
     S->deck_on = mode == sim::kSomeRemesh ? on : std::vector<char>{};
     FS_TRY(sim::set_active(*S, S->deck_on));
-    const bool eager = S->any_adaptive || S->any_treecode || !S->use_graph || S->timing;
+    const bool eager = adaptive_now || S->any_treecode || !S->use_graph || S->timing;
     S->timed = false;
     FS_TRY(sim::set_drive(*S, true));
     if (eager) {
         FS_TRY(sim::rk4(*S));
         if (mode != sim::kAllRemesh) FS_TRY(sim::field_full(*S, S->es, S->xs, 1, false));
         if (mode != sim::kNoRemesh) {
-            if (S->any_adaptive) {
+            if (adaptive_now) {
                 FS_TRY(sim::remesh_adaptive(*S));
             } else {
                 FS_TRY(sim::remesh_static(*S));
             }
         }
+        S->steps_eager += 1;
     } else {
         hipGraphExec_t& g = mode == sim::kAllRemesh ? S->g_remesh
                             : mode == sim::kNoRemesh ? S->g_plain : S->g_mixed;
         if (!g) FS_TRY(sim::capture(*S, mode, g));
         FS_HIP_CHECK(hipGraphLaunch(g, S->ctx->stream));
+        S->steps_replayed += 1;
     }
     S->iter += 1;
     for (auto& D : S->decks) {

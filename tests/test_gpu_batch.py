@@ -222,3 +222,42 @@ def test_deck_with_other_period_joins_mid_run(gpu_ctx):
         for k in range(nsp):
             assert np.array_equal(got[k], ref[k]), (d, k)
     s.close()
+
+
+def adaptive_landau(h, period, amp=0.3):
+    sim, sps = landau(h, 1, amp=amp)
+    sim.remesh_period = period
+    sps[0].max_height = h + 3
+    sps[0].do_adaptively_refine = True
+    sps[0].amr_epsilons[0] = 2e-3
+    sps[0].n_amr_epsilons = 1
+    return sim, sps
+
+
+@pytest.mark.parametrize("precision", [ms.PRECISION_F64, ms.PRECISION_MIXED])
+def test_adaptive_batch_replays_graphs_between_its_remeshes(gpu_ctx, precision):
+    """Adaptive decks remeshing every third step beside static decks remeshing
+    every step: the steps in which no adaptive deck remeshes (static remeshes
+    only, or none) replay captured graphs; every state bitwise equal to the
+    same batch stepped eagerly, with host reads and eager work between
+    replays."""
+    decks = [adaptive_landau(4, 3), DECKS[0], adaptive_landau(3, 3, amp=0.5), DECKS[1]]
+    g = ms.Simulation.batch(gpu_ctx, decks, precision)
+    e = ms.Simulation.batch(gpu_ctx, decks, precision)
+    e.set_options(precision, use_graph=False)
+    nsp = g.n_species
+    for step in range(7):
+        g.step()
+        e.step()
+        _ = gpu_ctx.emq_periodic_f64(np.linspace(0, 1, 100), np.linspace(0, 1, 50),
+                                     np.ones(50), 1.0, 0.1)
+        for k in range(nsp):
+            assert np.array_equal(np.concatenate(g.points(k)), np.concatenate(e.points(k))), \
+                (step, k)
+            assert np.array_equal(g.panels(k), e.panels(k)), (step, k)
+    # steps 3 and 6 remesh the adaptive decks (eager); the other five replay
+    assert g.step_counts() == (5, 2), g.step_counts()
+    assert e.step_counts() == (0, 7)
+    assert g.refinement_counts()[0] == 2
+    g.close()
+    e.close()
