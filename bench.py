@@ -875,7 +875,11 @@ def c5_block(ctx, requests: int, req_steps: int, plain_steps: int = 3):
         "value": st["point_steps_per_s"], "unit": "point-steps/s",
         "latency_ms": {"p50": st["step_p50_ms"], "p99": st["step_p99_ms"]},
         "served": {k: st[k] for k in ("submitted", "completed", "failed", "steps",
-                                      "point_steps", "busy_s", "span_s")},
+                                      "steps_replayed", "point_steps", "busy_s", "span_s")},
+        "graph_replays": ("steps in which no resident deck remeshes adaptively replay a captured "
+                          "graph; every request here remeshes adaptively at every step (its "
+                          "tree, and with it the arrays' sizes, changes), so those steps run "
+                          "eagerly"),
         "wall_s": wall, "requests": requests, "steps_per_request": req_steps,
         "points_per_request": {"min": min(sizes), "max": max(sizes)},
         "kinds": sorted({k for _, k, _ in reqs}),

@@ -499,6 +499,8 @@ typedef struct fs_server_stats {
     double point_steps_per_s;  /* point_steps / span_s */
     double step_p50_ms, step_p99_ms;  /* per-step latency of the resident decks */
     int64_t hbm_in_use;      /* estimated bytes of the resident decks */
+    int64_t steps_replayed;  /* steps taken by a captured-graph replay (steps in
+                                which no adaptive deck remeshed) */
 } fs_server_stats;
 
 typedef struct fs_server fs_server;

@@ -379,7 +379,7 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", _i64), ("completed", _i64), ("failed", _i64), ("active", _i64),
                 ("queued", _i64), ("steps", _i64), ("point_steps", _i64), ("busy_s", _d),
                 ("span_s", _d), ("point_steps_per_s", _d), ("step_p50_ms", _d),
-                ("step_p99_ms", _d), ("hbm_in_use", _i64)]
+                ("step_p99_ms", _d), ("hbm_in_use", _i64), ("steps_replayed", _i64)]
 
 
 def _deck(sim: SimParams, species):

@@ -74,6 +74,7 @@ struct fs_server {
     // statistics
     std::vector<double> step_ms;
     int64_t point_steps = 0;
+    int64_t steps_replayed = 0;  // steps taken by a graph replay
     double busy_s = 0;
     double t_first = -1, t_last = -1;
     size_t hbm_in_use = 0;
@@ -210,8 +211,11 @@ void worker_loop(fs_server* S) {
         }
         std::vector<int> fin;
         int64_t pts = 0;
+        int64_t replayed = 0;
+        (void)fs_sim_step_counts(S->sim, &replayed, nullptr);
         {
             std::lock_guard<std::mutex> lk(S->mu);
+            S->steps_replayed = replayed;
             S->step_ms.push_back((t1 - t0) * 1e3);
             S->busy_s += t1 - t0;
             ++S->steps;
@@ -425,6 +429,7 @@ int fs_server_get_stats(fs_server* S, fs_server_stats* out) {
     st.step_p50_ms = percentile(S->step_ms, 50);
     st.step_p99_ms = percentile(S->step_ms, 99);
     st.hbm_in_use = (int64_t)S->hbm_in_use;
+    st.steps_replayed = S->steps_replayed;
     *out = st;
     return FS_OK;
 }
