@@ -106,15 +106,25 @@ __device__ __forceinline__ int lower(const uint64_t* __restrict__ a, int lo, int
     return lo;
 }
 
+// The in-range keys [key(0), key(L)) of the sorted array: ab = their first
+// and one-past-last positions.
+__global__ void bounds_kernel(const uint64_t* __restrict__ skey, int n, uint64_t k0, uint64_t kL,
+                              int* __restrict__ ab) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    ab[0] = lower(skey, 0, n, k0);
+    ab[1] = lower(skey, 0, n, kL);
+}
+
 // Per sorted position: dx E^2 of a run start in [a, b) (0 elsewhere) and the
 // run-start count, as the two sums of a Part (the other slots zero).
 __global__ __launch_bounds__(kBlock) void energy_kernel(const uint64_t* __restrict__ skey,
                                                         const int32_t* __restrict__ perm,
                                                         const double* __restrict__ x,
                                                         const double* __restrict__ e, int n,
-                                                        int a, int b, double L,
+                                                        const int* __restrict__ ab, double L,
                                                         Part* __restrict__ parts) {
     __shared__ double sh[kBlock];
+    const int a = ab[0], b = ab[1];
     double term = 0.0, runs = 0.0;
     const int base = blockIdx.x * kChunk + threadIdx.x;
     for (int j = 0; j < kPer; ++j) {
@@ -200,29 +210,26 @@ int species(const double* x, const double* p, const double* f, const double* w,
     key_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(x, n, key);
     FS_HIP_CHECK(hipGetLastError());
     FS_TRY(radix_sort_full(key, n, skey, perm, rtmp, stream));
-    // the in-range keys [key(0), key(L)) of the sorted array
-    std::vector<uint64_t> hk(n);
-    FS_HIP_CHECK(hipMemcpyAsync(hk.data(), skey, 8 * (size_t)n, hipMemcpyDeviceToHost, stream));
-    double s[2 * kSums];
-    FS_HIP_CHECK(hipStreamSynchronize(stream));
+    // the in-range keys [key(0), key(L)) of the sorted array, found on the
+    // device; no in-range key leaves every term of the energy zero
     auto host_key = [](double v) {
         if (v == 0.0) v = 0.0;
         uint64_t b;
         std::memcpy(&b, &v, 8);
         return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
     };
-    const int a = (int)(std::lower_bound(hk.begin(), hk.end(), host_key(0.0)) - hk.begin());
-    const int b = (int)(std::lower_bound(hk.begin(), hk.end(), host_key(L)) - hk.begin());
-    if (b > a) {
-        energy_kernel<<<nb, kBlock, 0, stream>>>(skey, perm, x, e, n, a, b, L, eparts);
-        FS_HIP_CHECK(hipGetLastError());
-        final_kernel<<<1, 64, 0, stream>>>(eparts, nb, dres + kSums);
-        FS_HIP_CHECK(hipGetLastError());
-    }
-    FS_HIP_CHECK(hipMemcpyAsync(s, dres, sizeof(double) * (b > a ? 2 : 1) * kSums,
-                                hipMemcpyDeviceToHost, stream));
+    int* ab = reinterpret_cast<int*>(dres + 2 * kSums);
+    bounds_kernel<<<1, 64, 0, stream>>>(skey, n, host_key(0.0), host_key(L), ab);
+    FS_HIP_CHECK(hipGetLastError());
+    energy_kernel<<<nb, kBlock, 0, stream>>>(skey, perm, x, e, n, ab, L, eparts);
+    FS_HIP_CHECK(hipGetLastError());
+    final_kernel<<<1, 64, 0, stream>>>(eparts, nb, dres + kSums);
+    FS_HIP_CHECK(hipGetLastError());
+    double s[2 * kSums];
+    FS_HIP_CHECK(hipMemcpyAsync(s, dres, sizeof(double) * 2 * kSums, hipMemcpyDeviceToHost,
+                                stream));
     FS_HIP_CHECK(hipStreamSynchronize(stream));
-    const double pot = (b > a && s[kSums + 1] >= 3) ? 0.5 * s[kSums] : 0.0;
+    const double pot = s[kSums + 1] >= 3 ? 0.5 * s[kSums] : 0.0;
     out[0] = s[0];
     out[1] = 1.0 / q * s[1];
     out[2] = 1.0 / 2.0 / qm * s[2];
