@@ -21,10 +21,14 @@
 // That sequence is captured once in a HIP graph and replayed every step
 // (the plans it uses are built at layout time; nothing in it allocates,
 // clears with memset or synchronizes).
-// Adaptive decks rebuild the refined tree on the host after the grid
-// interpolation, with every refinement pass' new-point f evaluated by one
-// device launch (interpolate_from_mesh), then re-upload; treecode decks build
-// their tree on the host. Both run eagerly, without the graph.
+// Adaptive decks refine their tree on the device after the grid
+// interpolation (fs_amr.hip: one pass per refinement level, a few counters
+// read back per pass); the sequential host builder stays available as the
+// selectable second mode and as the fallback when a device pass reports a
+// status. A step whose remesh refines adaptively runs eagerly; every other
+// step (no remesh, or a remesh in which only static decks reset to their
+// grids) replays the graph. Treecode decks read their list sizes back and
+// run eagerly.
 #pragma clang fp contract(off)
 
 #include <algorithm>
@@ -958,10 +962,13 @@ int refine_on_device(Sim& S, const std::vector<Species*>& refine) {
 
 // Remesh with adaptive refinement: grid interpolation of every species on
 // the device; species without refinement are reset to their grid there and
-// stay resident; each refining species rebuilds its tree on a host thread of
-// its own (the trees are independent), evaluating each refinement pass' new
-// points with one device launch (interpolate_from_mesh; device calls from the
-// threads take turns on the stream), and is uploaded again.
+// stay resident; the refining species are refined together on the device
+// (amr::refine, every species' pass in the same launches). In the host mode,
+// or when a device pass reports a status, each refining species rebuilds its
+// tree on a host thread of its own (the trees are independent), evaluating
+// each refinement pass' new points with one device launch
+// (interpolate_from_mesh; device calls from the threads take turns on the
+// stream), and is uploaded again.
 int remesh_adaptive(Sim& S) {
     hipStream_t st = S.stream();
     if (S.timing) FS_HIP_CHECK(hipEventRecord(S.ev_r0, st));
