@@ -155,7 +155,7 @@ bool use_sym(int64_t nt, int64_t ns, bool same, const FieldTuning& tun) {
 size_t field_scratch_bytes(int64_t nt, int64_t ns, bool same, int num_cus,
                            const FieldTuning& tun) {
     if (use_sym(nt, ns, same, tun))
-        return sym::scratch_bytes(sym::plan(nt, tun, tun.sym_budget)) + sym::sort_bytes(nt);
+        return sym::scratch_bytes(sym::plan(nt, tun, tun.sym_budget, true)) + sym::sort_bytes(nt);
     const Plan p = plan_direct(nt, ns, num_cus, tun);
     return p.splits > 1 ? align_up(sizeof(double) * nt * p.splits, 256) : 0;
 }
@@ -182,7 +182,7 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src, const
     const double delta = eps / L;
     const double delta2 = delta * delta;
     if (use_sym(nt, ns, d_tgt == d_src, tun)) {
-        const sym::Plan P = sym::plan(nt, tun, tun.sym_budget);
+        const sym::Plan P = sym::plan(nt, tun, tun.sym_budget, true);
         FS_TRY(sym::run_sorted(d_src, d_w, P, L, invL, delta2, scratch, d_es, stream));
         if (pair_evals) *pair_evals = sym::evals(P);
         return FS_OK;

@@ -163,6 +163,7 @@ __global__ __launch_bounds__(sym::threads<T>()) void batch_sym_kernel(
     const SymSeg* __restrict__ tab, int first, int count, const double* __restrict__ x,
     const double* __restrict__ w, double* __restrict__ sympart) {
     __shared__ double red[2][sym::kMaxWaves][64];
+    __shared__ double2 stage[sym::kMaxWaves][128];
     int lo = first, hi = first + count - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -172,10 +173,10 @@ __global__ __launch_bounds__(sym::threads<T>()) void batch_sym_kernel(
     const SymSeg E = tab[lo];
     if (E.open)
         sym::item<T, true>(x + E.off, w + E.off, E.P, E.S, (int)blockIdx.x - E.item_base, 0.0,
-                           E.invL, E.delta2, sympart + E.rbase, sympart + E.cbase, red);
+                           E.invL, E.delta2, sympart + E.rbase, sympart + E.cbase, red, stage);
     else
         sym::item<T, false>(x + E.off, w + E.off, E.P, E.S, (int)blockIdx.x - E.item_base, 0.0,
-                            E.invL, E.delta2, sympart + E.rbase, sympart + E.cbase, red);
+                            E.invL, E.delta2, sympart + E.rbase, sympart + E.cbase, red, stage);
 }
 
 // A point's partials in a fixed order; open-boundary segments subtract the

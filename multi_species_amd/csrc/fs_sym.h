@@ -13,11 +13,13 @@
 // plus the distance nb/2 for the first half of the blocks when nb is even:
 // every unordered pair of blocks exactly once); a block's pairs with itself
 // are one more item, summed in full without the symmetry. Within a 64-source
-// tile each lane holds one source (x_j, w_j and its accumulator) and the three
-// move one lane on per step (DPP wavefront rotation), so after 64 steps every
-// lane has met every source of the tile and each source's accumulator is back
-// in its own lane; the workgroup's wavefronts add their accumulators of a
-// tile through LDS in a fixed order. Targets keep their sums in registers
+// tile each lane holds one source's accumulator, which moves one lane on per
+// step (DPP wavefront rotation), and at step s reads source (lane - s) mod 64
+// from the wavefront's LDS copy of the tile (written twice over, so the
+// source sits at slot lane + 64 - s): after 64 steps every lane has met every
+// source of the tile and each source's accumulator is back in its own lane;
+// the workgroup's wavefronts add their accumulators of a tile through LDS in
+// a fixed order. Targets keep their sums in registers
 // (plain FP64 over a tile, TwoSum across tiles). Every item writes its row
 // partials and its sources' column partials to their own slots, and the
 // finalize adds each point's partials in a fixed order, so results are
@@ -43,8 +45,7 @@ namespace sym {
 
 // Wavefronts per workgroup: 4 for one target per lane (the decks below
 // 65536 points, where more, smaller row blocks keep the chip busy) and for
-// eight (a tuning choice: fewer lane moves per evaluation, 110 VGPRs, the
-// same 2048-point row blocks as four), else 8.
+// eight (the same 2048-point row blocks as four), else 8.
 template <int T> constexpr int waves() { return T == 1 || T == 8 ? 4 : 8; }
 template <int T> constexpr int threads() { return 64 * waves<T>(); }
 inline int threads_of(int t) { return t == 1 || t == 8 ? threads<1>() : threads<2>(); }
@@ -115,14 +116,16 @@ inline size_t scratch_bytes(const Plan& P) {
 
 // budget <= 0: kDefaultBudget. The chunking depends on n and the tuning only;
 // the budget decides how many chunks share a pass, which changes no result.
-inline Plan plan(int64_t n, const FieldTuning& tun, int64_t budget = 0) {
+// sorted: the plan of the sum on sorted points, whose kernel takes eight
+// targets per lane from 2^18 points (the same 2048-point row blocks as four).
+inline Plan plan(int64_t n, const FieldTuning& tun, int64_t budget = 0, bool sorted = false) {
     Plan P{};
     P.n = (int)n;
     if (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
         tun.targets_per_thread == 4 || tun.targets_per_thread == 8)
         P.t = tun.targets_per_thread;
     else
-        P.t = n >= (int64_t(1) << 18) ? 4 : n >= (int64_t(1) << 16) ? 2 : 1;
+        P.t = n >= (int64_t(1) << 18) ? (sorted ? 8 : 4) : n >= (int64_t(1) << 16) ? 2 : 1;
     P.rb = threads_of(P.t) * P.t;
     P.nb = (int)((n + P.rb - 1) / P.rb);
     P.h = (P.nb - 1) / 2;
@@ -169,6 +172,13 @@ inline int64_t evals(const Plan& P) {
     return bp * P.rb * P.rb;
 }
 
+// x W - M with the product rounded on its own (not fused), as the tiles' sums
+// round theirs: a lone point's linear part is then exactly zero.
+__device__ __forceinline__ double lin_part(double x, double W, double M) {
+#pragma clang fp contract(off)
+    return x * W - M;
+}
+
 // one lane on across the wavefront (DPP wave_ror:1), 64-bit
 __device__ __forceinline__ double wave_rotate(double v) {
     const long long u = __builtin_bit_cast(long long, v);
@@ -179,7 +189,8 @@ __device__ __forceinline__ double wave_rotate(double v) {
 
 // One item (row block and source chunk, or a block with itself) of pass S of
 // the plan P over the points x[0..n), w[0..n). id: the item's index within
-// the pass. red: the workgroup's LDS, [2][kMaxWaves][64] doubles. Called by
+// the pass. red, stage: the workgroup's LDS, [2][kMaxWaves][64] doubles and
+// [kMaxWaves][128] (x, w) pairs. tsum: the tiles' sums (kSorted). Called by
 // all threads<T>() threads of a workgroup.
 // kOpen: the open-boundary kernel z / sqrt(z^2 + delta2) with z = x_i - x_j
 // (E_MQ_DirectSum_openbcs, src/FieldStructure.cpp:166-191; also odd in z;
@@ -193,12 +204,20 @@ __device__ __forceinline__ double wave_rotate(double v) {
 // straddling the half-period) take the rint form. For |d| < L/2 both forms
 // round alike; K is odd and vanishes at z = +-1/2, so a tile whose range
 // touches the half-period within the margin loses nothing to either choice.
+// Such a whole-wrap tile is evaluated in x units, K = d / sqrt(d^2 + eps^2) -
+// C_x d with d = x_i - x_j (shifted source), eps = delta L, C_x = C / L (the
+// same K: z / sqrt(z^2 + delta^2) = d / sqrt(d^2 + eps^2)), and its linear part
+// for all 64 sources at once: sum_j w_j C_x d_ij = C_x (x_i W - M) from the
+// tile's sums W = sum w_j, M = sum w_j x_j (tsum, per 64-point tile of the
+// sorted points), and for the sources C_x (M_r - x_j W_r) from the row block's
+// sums.
 template <int T, bool kOpen = false, bool kSorted = false>
 __device__ __forceinline__ void item(const double* __restrict__ x, const double* __restrict__ w,
                                      const Plan& P, const Pass& S, int id, double L, double invL,
                                      double delta2, double* __restrict__ rowpart,
                                      double* __restrict__ colpart,
-                                     double (*red)[kMaxWaves][64]) {
+                                     double (*red)[kMaxWaves][64], double2 (*stage)[128],
+                                     const double2* __restrict__ tsum = nullptr) {
     constexpr int kWaves = waves<T>();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int rb = threads<T>() * T;
@@ -227,21 +246,32 @@ __device__ __forceinline__ void item(const double* __restrict__ x, const double*
         lo[t] = 0.0;
     }
     const double C = kOpen ? 0.0 : dev::rsq_f64(__builtin_fma(0.5, 0.5, delta2));
-    // the row block's first and last points (kSorted)
-    double r0 = 0.0, r1 = 0.0;
+    // the row block's first and last points, and (whole-wrap tiles) eps^2,
+    // C_x and the row block's sums W_r, M_r (kSorted)
+    double r0 = 0.0, r1 = 0.0, eps2 = 0.0, Cx = 0.0, Wr = 0.0, Mr = 0.0;
     if constexpr (kSorted) {
         const int pa = b * rb;
         r0 = x[pa];
         r1 = x[min(pa + rb, n) - 1];
+        if constexpr (!kOpen) {
+            eps2 = delta2 * (L * L);
+            Cx = dev::rsq_f64(__builtin_fma(0.5 * L, 0.5 * L, eps2));
+            for (int k = 0; k < rb / 64 && pa + 64 * k < n; ++k) {
+                const double2 s = tsum[pa / 64 + k];
+                Wr += s.x;
+                Mr += s.y;
+            }
+        }
     }
     int tile = 0;
     for (int d = d0; d < d1; ++d) {
         const int B = b + d < P.nb ? b + d : b + d - P.nb;
         for (int jt = 0; jt < rb / 64; ++jt, ++tile) {
             const int q = B * rb + jt * 64 + lane;
-            double xj = q < n ? x[q] : 0.0;
-            double wj = q < n ? w[q] : 0.0;
+            const double xj = q < n ? x[q] : 0.0;
+            const double wj = q < n ? w[q] : 0.0;
             double accj = 0.0;
+            double src_lin = 0.0;  // the sources' linear part (whole-wrap tiles)
             // the tile's wrap: 0 (none), +1 (targets back one period), -1
             // (sources back one period), 2 (per pair)
             int wrap = 2;
@@ -256,46 +286,67 @@ __device__ __forceinline__ void item(const double* __restrict__ x, const double*
                     else if (hi < -0.5 - m && lo > -1.5 + m) wrap = -1;
                 }
             }
+            // the tile's sources (one period on or back for a whole-wrap tile)
+            // in the wavefront's LDS copy
+            const double sh = wrap == 1 ? L : (wrap == -1 ? -L : 0.0);
+            const double xs = xj + sh;
+            stage[wave][lane] = make_double2(xs, wj);
+            stage[wave][lane + 64] = make_double2(xs, wj);
+            __syncthreads();
             if (wrap != 2) {
-                // the tile's sources one period on (targets beyond L/2 ahead of
-                // them) or back
-                if (wrap == 1) xj = xj + L;
-                if (wrap == -1) xj = xj - L;
-#pragma unroll(T > 4 ? 1 : 2)
-                for (int s = 0; s < 64; ++s) {
+                if constexpr (kSorted && !kOpen) {
+#pragma unroll 1
+                    for (int s0 = 0; s0 < 64; s0 += 4) {
+                        double2 src[4];
 #pragma unroll
-                    for (int t = 0; t < T; ++t) {
-                        const double z = (xi[t] - xj) * invL;
-                        const double y = dev::rsq_f64(__builtin_fma(z, z, delta2));
-                        const double k = z * (y - C);
-                        acc[t] = __builtin_fma(wj, k, acc[t]);
-                        accj = __builtin_fma(nwi[t], k, accj);
+                        for (int u = 0; u < 4; ++u) src[u] = stage[wave][lane + 64 - (s0 + u)];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const double2 sj = src[u];
+#pragma unroll
+                            for (int t = 0; t < T; ++t) {
+                                const double dd = xi[t] - sj.x;
+                                const double k = dd * dev::rsq_f64(__builtin_fma(dd, dd, eps2));
+                                acc[t] = __builtin_fma(sj.y, k, acc[t]);
+                                accj = __builtin_fma(nwi[t], k, accj);
+                            }
+                            accj = wave_rotate(accj);
+                        }
                     }
-                    xj = wave_rotate(xj);
-                    wj = wave_rotate(wj);
-                    accj = wave_rotate(accj);
+                    // the linear part -C_x d of the tile's pairs
+                    const double2 ts = tsum[(B * rb) / 64 + jt];
+                    const double Mt = __builtin_fma(sh, ts.x, ts.y);
+#pragma unroll
+                    for (int t = 0; t < T; ++t)
+                        acc[t] = __builtin_fma(-Cx, lin_part(xi[t], ts.x, Mt), acc[t]);
+                    src_lin = -Cx * lin_part(xs, Wr, Mr);
                 }
             } else {
-#pragma unroll 2
-                for (int s = 0; s < 64; ++s) {
+#pragma unroll 1
+                for (int s0 = 0; s0 < 64; s0 += 4) {
+                    double2 src[4];
 #pragma unroll
-                    for (int t = 0; t < T; ++t) {
-                        double k;
-                        if constexpr (kOpen) {
-                            const double z = xi[t] - xj;
-                            k = z * dev::rsq_f64(__builtin_fma(z, z, delta2));
-                        } else {
-                            const double u = (xi[t] - xj) * invL;
-                            const double z = u - __builtin_rint(u);
-                            const double y = dev::rsq_f64(__builtin_fma(z, z, delta2));
-                            k = z * (y - C);
+                    for (int u = 0; u < 4; ++u) src[u] = stage[wave][lane + 64 - (s0 + u)];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const double2 sj = src[u];
+#pragma unroll
+                        for (int t = 0; t < T; ++t) {
+                            double k;
+                            if constexpr (kOpen) {
+                                const double z = xi[t] - sj.x;
+                                k = z * dev::rsq_f64(__builtin_fma(z, z, delta2));
+                            } else {
+                                const double v = (xi[t] - sj.x) * invL;
+                                const double z = v - __builtin_rint(v);
+                                const double y = dev::rsq_f64(__builtin_fma(z, z, delta2));
+                                k = z * (y - C);
+                            }
+                            acc[t] = __builtin_fma(sj.y, k, acc[t]);
+                            accj = __builtin_fma(nwi[t], k, accj);
                         }
-                        acc[t] = __builtin_fma(wj, k, acc[t]);
-                        accj = __builtin_fma(nwi[t], k, accj);
+                        accj = wave_rotate(accj);
                     }
-                    xj = wave_rotate(xj);
-                    wj = wave_rotate(wj);
-                    accj = wave_rotate(accj);
                 }
             }
 #pragma unroll
@@ -310,7 +361,8 @@ __device__ __forceinline__ void item(const double* __restrict__ x, const double*
                     double sum = red[tile & 1][0][lane];
 #pragma unroll
                     for (int v = 1; v < kWaves; ++v) sum += red[tile & 1][v][lane];
-                    colpart[((int64_t)id * P.chb + (d - d0)) * rb + jt * 64 + lane] = sum;
+                    colpart[((int64_t)id * P.chb + (d - d0)) * rb + jt * 64 + lane] =
+                        sum + src_lin;
                 }
             }
         }
@@ -366,11 +418,14 @@ __device__ __forceinline__ double pass_point(const Plan& P, const Pass& S, int p
 }
 
 template <int T, bool kOpen, bool kSorted>
-__global__ __launch_bounds__(threads<T>()) void pass_kernel(
+__global__ __launch_bounds__(threads<T>(), T == 8 ? 4 : 1) void pass_kernel(
     const double* __restrict__ x, const double* __restrict__ w, Plan P, Pass S, double L,
-    double invL, double delta2, double* __restrict__ rowpart, double* __restrict__ colpart) {
+    double invL, double delta2, double* __restrict__ rowpart, double* __restrict__ colpart,
+    const double2* __restrict__ tsum) {
     __shared__ double red[2][kMaxWaves][64];
-    item<T, kOpen, kSorted>(x, w, P, S, blockIdx.x, L, invL, delta2, rowpart, colpart, red);
+    __shared__ double2 stage[kMaxWaves][128];
+    item<T, kOpen, kSorted>(x, w, P, S, blockIdx.x, L, invL, delta2, rowpart, colpart, red, stage,
+                            tsum);
 }
 
 // kOpen: E = (s + lo) / 2 - x_p (the open-boundary form's uniform background).
@@ -392,7 +447,8 @@ __global__ __launch_bounds__(256) void fold_kernel(Plan P, Pass S, int last,
 template <bool kOpen, bool kSorted>
 inline hipError_t launch_passes(const double* x, const double* w, const Plan& P, double L,
                                 double invL, double delta2, void* scratch,
-                                const int32_t* perm, double* es, hipStream_t stream) {
+                                const int32_t* perm, const double2* tsum, double* es,
+                                hipStream_t stream) {
     double* rowpart = static_cast<double*>(scratch);
     double* colpart = reinterpret_cast<double*>(static_cast<char*>(scratch) +
                                                 al256(rowpart_elems(P)));
@@ -404,19 +460,19 @@ inline hipError_t launch_passes(const double* x, const double* w, const Plan& P,
         switch (P.t) {
             case 8:
                 pass_kernel<8, kOpen, kSorted><<<S.nitems, threads<8>(), 0, stream>>>(
-                    x, w, P, S, L, invL, delta2, rowpart, colpart);
+                    x, w, P, S, L, invL, delta2, rowpart, colpart, tsum);
                 break;
             case 4:
                 pass_kernel<4, kOpen, kSorted><<<S.nitems, threads<4>(), 0, stream>>>(
-                    x, w, P, S, L, invL, delta2, rowpart, colpart);
+                    x, w, P, S, L, invL, delta2, rowpart, colpart, tsum);
                 break;
             case 2:
                 pass_kernel<2, kOpen, kSorted><<<S.nitems, threads<2>(), 0, stream>>>(
-                    x, w, P, S, L, invL, delta2, rowpart, colpart);
+                    x, w, P, S, L, invL, delta2, rowpart, colpart, tsum);
                 break;
             default:
                 pass_kernel<1, kOpen, kSorted><<<S.nitems, threads<1>(), 0, stream>>>(
-                    x, w, P, S, L, invL, delta2, rowpart, colpart);
+                    x, w, P, S, L, invL, delta2, rowpart, colpart, tsum);
                 break;
         }
         hipError_t e = hipGetLastError();
@@ -434,15 +490,20 @@ inline hipError_t launch_passes(const double* x, const double* w, const Plan& P,
 template <bool kOpen>
 inline hipError_t run(const double* x, const double* w, const Plan& P, double L, double invL,
                       double delta2, void* scratch, double* es, hipStream_t stream) {
-    return launch_passes<kOpen, false>(x, w, P, L, invL, delta2, scratch, nullptr, es, stream);
+    return launch_passes<kOpen, false>(x, w, P, L, invL, delta2, scratch, nullptr, nullptr, es,
+                                       stream);
 }
 
 // ---- the periodic sum on points sorted by x -------------------------------
 
+// 64-point tiles of n sorted points.
+__host__ __device__ inline int64_t tiles(int64_t n) { return (n + 63) / 64; }
+
 // Work space of the sort in front of the passes: keys, sorted keys, the
-// permutation, sorted x and w, and the radix sort's own.
+// permutation, sorted x and w, the tiles' sums, and the radix sort's own.
 inline size_t sort_bytes(int64_t n) {
-    return 4 * al256(n) + (((size_t)n * 4 + 255) / 256 * 256) + radix_sort_bytes((int)n);
+    return 4 * al256(n) + (((size_t)n * 4 + 255) / 256 * 256) + al256(2 * tiles(n)) +
+           radix_sort_bytes((int)n);
 }
 
 // Order-preserving image of a double (+0 and -0 one value).
@@ -472,6 +533,22 @@ __global__ __launch_bounds__(256) void gather_kernel(const double* __restrict__ 
     ws[k] = w[i];
 }
 
+// (sum w, sum w x) of each 64-point tile of the sorted points, in point order.
+__global__ __launch_bounds__(256) void tile_sums_kernel(const double* __restrict__ xs,
+                                                        const double* __restrict__ ws, int n,
+                                                        double2* __restrict__ tsum) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if ((int64_t)k >= tiles(n)) return;
+    double W = 0.0, M = 0.0;
+    const int e = min(64 * k + 64, n);
+    for (int p = 64 * k; p < e; ++p) {
+        W += ws[p];
+        M += ws[p] * xs[p];
+    }
+    tsum[k] = make_double2(W, M);
+}
+
 }  // namespace
 
 // The periodic pair-once sum with the points sorted by x first (stable: equal
@@ -492,6 +569,8 @@ inline int run_sorted(const double* x, const double* w, const Plan& P, double L,
     c += al256(n);
     int32_t* perm = reinterpret_cast<int32_t*>(c);
     c += ((size_t)n * 4 + 255) / 256 * 256;
+    double2* tsum = reinterpret_cast<double2*>(c);
+    c += al256(2 * tiles(n));
     const unsigned g = (unsigned)((n + 255) / 256);
     key_kernel<<<g, 256, 0, stream>>>(x, n, key);
     if (hipGetLastError() != hipSuccess) return fail(FS_E_HIP, "pair-once sort: key launch failed");
@@ -499,8 +578,11 @@ inline int run_sorted(const double* x, const double* w, const Plan& P, double L,
     gather_kernel<<<g, 256, 0, stream>>>(x, w, perm, n, xs, ws);
     if (hipGetLastError() != hipSuccess)
         return fail(FS_E_HIP, "pair-once sort: gather launch failed");
-    const hipError_t e = launch_passes<false, true>(xs, ws, P, L, invL, delta2, scratch, perm, es,
-                                                    stream);
+    tile_sums_kernel<<<(unsigned)((tiles(n) + 255) / 256), 256, 0, stream>>>(xs, ws, n, tsum);
+    if (hipGetLastError() != hipSuccess)
+        return fail(FS_E_HIP, "pair-once sort: tile sums launch failed");
+    const hipError_t e = launch_passes<false, true>(xs, ws, P, L, invL, delta2, scratch, perm,
+                                                    tsum, es, stream);
     if (e != hipSuccess) return fail(FS_E_HIP, "pair-once sum: %s", hipGetErrorString(e));
     return FS_OK;
 }
