@@ -157,7 +157,7 @@ int fs_emq_open_f64_dev(fs_ctx* ctx, double* d_es, const double* d_targets, int6
         return fs::fail(FS_E_INVALID, "null pointer");
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     if (nt <= INT32_MAX && fs::use_sym(nt, ns, d_targets == d_sources, ctx->tuning)) {
-        const fs::sym::Plan P = fs::sym::plan(nt, ctx->tuning, ctx->tuning.sym_budget);
+        const fs::sym::Plan P = fs::sym::plan(nt, ctx->tuning, ctx->tuning.sym_budget, true);
         FS_TRY(ctx->scratch.ensure(fs::sym::scratch_bytes(P), st));
         FS_HIP_CHECK(fs::sym::run<true>(d_sources, d_q_ws, P, 0.0, 0.0, epsilon * epsilon,
                                         ctx->scratch.ptr, d_es, st));

@@ -116,16 +116,16 @@ inline size_t scratch_bytes(const Plan& P) {
 
 // budget <= 0: kDefaultBudget. The chunking depends on n and the tuning only;
 // the budget decides how many chunks share a pass, which changes no result.
-// sorted: the plan of the sum on sorted points, whose kernel takes eight
-// targets per lane from 2^18 points (the same 2048-point row blocks as four).
-inline Plan plan(int64_t n, const FieldTuning& tun, int64_t budget = 0, bool sorted = false) {
+// wide: eight targets per lane by default from 2^18 points (the same
+// 2048-point row blocks as four; the standalone sums take it).
+inline Plan plan(int64_t n, const FieldTuning& tun, int64_t budget = 0, bool wide = false) {
     Plan P{};
     P.n = (int)n;
     if (tun.targets_per_thread == 1 || tun.targets_per_thread == 2 ||
         tun.targets_per_thread == 4 || tun.targets_per_thread == 8)
         P.t = tun.targets_per_thread;
     else
-        P.t = n >= (int64_t(1) << 18) ? (sorted ? 8 : 4) : n >= (int64_t(1) << 16) ? 2 : 1;
+        P.t = n >= (int64_t(1) << 18) ? (wide ? 8 : 4) : n >= (int64_t(1) << 16) ? 2 : 1;
     P.rb = threads_of(P.t) * P.t;
     P.nb = (int)((n + P.rb - 1) / P.rb);
     P.h = (P.nb - 1) / 2;
