@@ -25,7 +25,9 @@ interaction-equivalents/s, with its VALU issue fraction), "plain_step_loop"
 the host), "batched" (64 decks stepped as one mixed-precision batch against
 the same decks one by one in FP64: point-steps/s, p50/p99 step latency, E
 error against FP64 at the same points) and "served" (a Poisson request
-stream with continuous batching). --workload runs one of them on its own.
+stream with continuous batching). --workload runs one of them on its own;
+--workload remesh times the batched remesh at 2^24 grid points (64 species)
+in algorithmic GB/s.
 """
 from __future__ import annotations
 
@@ -58,6 +60,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "direct_f64_n1048576_traffic.txt")
 
 MIXED_VALU_FILE = os.path.join(ROOT, "profiles", "mixed_field_n1048576_valu_counters.md")
 REMESH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c4_remesh_hbm.md")
+REMESH24_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "remesh_2p24_hbm.md")
 HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E
 SIMDS = 256 * 4          # MI355X: 256 CUs of 4 SIMDs
 CLOCK_HZ = 2.4e9         # peak engine clock
@@ -162,7 +165,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--refinement", choices=["device", "host"], default="device",
                     help="where adaptive remeshes refine their trees (c3)")
-    ap.add_argument("--workload", choices=["field", "open", "treecode", "c1", "c3", "c4", "c5"],
+    ap.add_argument("--workload", choices=["field", "open", "treecode", "c1", "c3", "c4", "c5",
+                                           "remesh"],
                     default="field",
                     help="field: N=2^20 direct-sum evaluation (default); c1: the two-species "
                          "weak-Landau deck, 65x129 points per species, RK4 dt=0.1 with a remesh "
@@ -173,7 +177,8 @@ def main():
                          "batch in mixed precision, against the same decks one by one in FP64; "
                          "c5: Poisson arrivals of decks served with continuous batching; "
                          "open: the open-boundary direct sum (E_MQ_DirectSum_openbcs) of the "
-                         "N=2^20 deck")
+                         "N=2^20 deck; remesh: the batched remesh at 2^24 target points "
+                         "(32 decks, 513x513 points per species)")
     ap.add_argument("--precision", choices=["f64", "mixed"], default="f64",
                     help="c1: field precision of the step loop")
     ap.add_argument("--decks", type=int, default=64, help="c4: decks in the batch")
@@ -220,6 +225,8 @@ def main():
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
+    if args.workload == "remesh":
+        return bench_remesh(args)
 
     n = args.n
     x, w = c2_inputs(n)
@@ -806,6 +813,82 @@ def c4_block(ctx, ndecks: int, steps: int, warmup: int, compare: bool = True,
     out["rel_diag_err_vs_f64"] = dict(mixed_diag_errors(ctx, ndecks, diag_steps),
                                       steps=diag_steps)
     return out
+
+
+REMESH24_WORKLOAD = ("remesh of 2^24 grid points in one batched launch per kernel: 32 "
+                     "two-species Landau decks (amp log-uniform [0.01,0.5], ion mass ratio in "
+                     "{25,100,1836}), 256x256 quadratic panels (513x513 points) per species, "
+                     "64 species, 1.68e7 target points; mixed-precision field, remesh every "
+                     "step")
+
+
+def remesh24_decks(n: int = 32, seed: int = 6):
+    """The decks of the 2^24-point remesh: every species on a 513x513 grid (grid
+    lines stay within the 1024 points one workgroup of the leaf search takes)."""
+    import multi_species_amd as ms
+
+    rng = np.random.default_rng(seed)
+    decks = []
+    for _ in range(n):
+        amp = float(np.exp(rng.uniform(np.log(0.01), np.log(0.5))))
+        mass = float(rng.choice([25.0, 100.0, 1836.0]))
+        sim = ms.sim_params(x_min=0.0, x_max=L0, bcs=0, quadrature=0, greens_epsilon=0.05,
+                            use_treecode=0, dt=0.1, remesh_period=1)
+        e = ms.species_params(q=-1.0, m=1.0, p_min=-6.0, p_max=6.0, initial_height=8,
+                              p_height=0, ics_type=1, normalized_wavenumber=1.0, amp=amp,
+                              pth=1.0)
+        i = ms.species_params(q=1.0, m=mass, p_min=-6.0, p_max=6.0, initial_height=8,
+                              p_height=0, ics_type=1, normalized_wavenumber=1.0, amp=0.0,
+                              pth=1.0 / np.sqrt(mass))
+        decks.append((sim, [e, i]))
+    return decks
+
+
+def bench_remesh(args):
+    """The remesh at 2^24 target points (32 decks, 64 species, one launch per
+    kernel): the remesh kernels' time by HIP events per step, against the
+    algorithmic bytes of one remesh of every species."""
+    import multi_species_amd as ms
+
+    ctx = ms.Context(0)
+    decks = remesh24_decks()
+    b = ms.Simulation.batch(ctx, decks, ms.PRECISION_MIXED)
+    npts = sum(b.deck_info(d)[3] for d in range(b.n_decks))
+    nsp = sum(b.deck_info(d)[1] for d in range(b.n_decks))
+    remesh_bytes = 0
+    for k in range(nsp):
+        n_k, p_k = b.counts(k)
+        remesh_bytes += 24 * n_k + 24 * n_k + 48 * p_k
+    for _ in range(args.warmup):
+        b.step()
+    b.set_timing(True)
+    rms, lat = [], []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        b.step()
+        lat.append(time.perf_counter() - t1)
+        rms.append(b.remesh_ms())
+    b.set_timing(False)
+    b.close()
+    ctx.close()
+    remesh_ms = float(np.median(rms))
+    gbps = remesh_bytes / (remesh_ms * 1e-3) / 1e9
+    traffic = read_profile_value(REMESH24_TRAFFIC_FILE, "remesh_counted_traffic_bytes_per_step")
+    result = {
+        "metric": "remesh algorithmic GB/s", "value": gbps, "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": remesh_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": REMESH24_WORKLOAD, "points": int(npts), "species": int(nsp)},
+        "remesh_algorithmic_bytes_per_step": remesh_bytes,
+        "roofline": {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": gbps / HBM_PEAK_GBPS, "traffic": traffic,
+                     "source": "HIP events around the remesh kernels of each timed step; "
+                               "traffic from " + os.path.relpath(REMESH24_TRAFFIC_FILE, ROOT)},
+        "remesh_ms": {"min": float(np.min(rms)), "median": remesh_ms, "max": float(np.max(rms))},
+        "step_ms_with_event_timing": float(np.median(lat) * 1e3),
+    }
+    print(json.dumps(result))
 
 
 def bench_c4(args):

@@ -16,7 +16,9 @@
 //            dependent load (a descent ends on the leaf's walk record).
 //   shift    one thread per new point: the periodic pre-shift of
 //            AMRStructure_interp.cpp:429-491.
-//   rowsort  one workgroup per grid row: rank sort of the row by shifted x.
+//   rowsort  one workgroup per grid row: stable sort of the row by shifted x
+//            (a merge of two ascending runs by binary search, else a bitonic
+//            network in LDS).
 //   column   one workgroup: the walk down the first column (:610-650).
 //   rows     one workgroup per row: the walk along the row (:696-722).
 //            The reference walks each line sequentially, every point
@@ -495,30 +497,98 @@ __global__ __launch_bounds__(256) void shift_kernel(const Job* __restrict__ jobs
     J.shifted[i] = xt;
 }
 
-// One workgroup per row: each point's rank by (shifted x, position in the
-// row list) is its sorted slot.
+// One workgroup per row: the row sorted by (shifted x, position in the row
+// list), a stable sort by shifted x. A grid row is ascending in x and the
+// periodic pre-shift moves a run at one end by a period, so a row is nearly
+// always at most two ascending runs: then each point's rank is its index in
+// its run plus a binary search in the other (points of the first run before
+// equal keys of the second). Other rows take a bitonic network over the row
+// padded to a power of two (+inf keys past the row sort last). Rows are
+// mapped to XCDs like the row walks: neighbouring rows read the same lines of
+// the strided point arrays, and each XCD's L2 holds the rows it walks next.
 __global__ __launch_bounds__(256) void rowsort_kernel(const Job* __restrict__ jobs) {
     __shared__ double keys[kMaxLine];
+    __shared__ int pos[kMaxLine];
+    __shared__ int s_desc, s_at;
     const Job& J = jobs[blockIdx.y];
     if (J.on && *J.on == 0) return;
-    const int row = blockIdx.x;
-    if (row >= J.np) return;
+    const int row = xcd_block(blockIdx.x, J.np);
+    if (row < 0) return;
     const int nx = J.nx;
     const int32_t* rp = J.rows + (size_t)row * nx;
+    if (threadIdx.x == 0) {
+        s_desc = 0;
+        s_at = nx;
+    }
     for (int k = threadIdx.x; k < nx; k += blockDim.x) keys[k] = J.shifted[rp[k]];
     __syncthreads();
-    for (int k = threadIdx.x; k < nx; k += blockDim.x) {
-        const double kk = keys[k];
-        int rank = 0;
-        for (int m = 0; m < nx; ++m) {
-            const double km = keys[m];
-            rank += (km < kk) || (km == kk && m < k);
+    for (int k = threadIdx.x + 1; k < nx; k += blockDim.x)
+        if (keys[k] < keys[k - 1]) {
+            atomicAdd(&s_desc, 1);
+            atomicMin(&s_at, k);
         }
-        const int32_t idx = rp[k];
-        const size_t o = (size_t)row * nx + rank;
-        J.sorted_x[o] = kk;
-        J.sorted_p[o] = J.p[idx];
-        J.sorted_idx[o] = idx;
+    __syncthreads();
+    const size_t ob = (size_t)row * nx;
+    if (s_desc <= 1) {
+        const int d = s_at;  // runs [0, d) and [d, nx)
+        for (int k = threadIdx.x; k < nx; k += blockDim.x) {
+            const double kk = keys[k];
+            int lo, hi, rank;
+            if (k < d) {
+                lo = d;  // first of the second run with key >= kk
+                hi = nx;
+                while (lo < hi) {
+                    const int m = (lo + hi) >> 1;
+                    if (keys[m] < kk) lo = m + 1;
+                    else hi = m;
+                }
+                rank = k + (lo - d);
+            } else {
+                lo = 0;  // first of the first run with key > kk
+                hi = d;
+                while (lo < hi) {
+                    const int m = (lo + hi) >> 1;
+                    if (keys[m] <= kk) lo = m + 1;
+                    else hi = m;
+                }
+                rank = (k - d) + lo;
+            }
+            const int32_t idx = rp[k];
+            J.sorted_x[ob + rank] = kk;
+            J.sorted_p[ob + rank] = J.p[idx];
+            J.sorted_idx[ob + rank] = idx;
+        }
+        return;
+    }
+    int n2 = 1;
+    while (n2 < nx) n2 <<= 1;
+    for (int k = threadIdx.x; k < n2; k += blockDim.x) {
+        if (k >= nx) keys[k] = __builtin_inf();
+        pos[k] = k;
+    }
+    __syncthreads();
+    for (int size = 2; size <= n2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < n2 / 2; t += blockDim.x) {
+                const int i = 2 * t - (t & (stride - 1)), j = i + stride;
+                const double ki = keys[i], kj = keys[j];
+                const int pi = pos[i], pj = pos[j];
+                const bool greater = ki > kj || (ki == kj && pi > pj);
+                if (greater == ((i & size) == 0)) {
+                    keys[i] = kj;
+                    keys[j] = ki;
+                    pos[i] = pj;
+                    pos[j] = pi;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = threadIdx.x; k < nx; k += blockDim.x) {
+        const int32_t idx = rp[pos[k]];
+        J.sorted_x[ob + k] = keys[k];
+        J.sorted_p[ob + k] = J.p[idx];
+        J.sorted_idx[ob + k] = idx;
     }
 }
 
@@ -824,7 +894,7 @@ int interp_grids(const Job* d_jobs, const Launch& l, hipStream_t stream) {
         return fail(FS_E_INVALID, "remesh: grid lines longer than %d points", kMaxLine);
     const unsigned nj = (unsigned)l.njobs;
     shift_kernel<<<dim3(cdiv(l.max_points, 256), nj), 256, 0, stream>>>(d_jobs);
-    rowsort_kernel<<<dim3((unsigned)l.max_rows, nj), 256, 0, stream>>>(d_jobs);
+    rowsort_kernel<<<dim3(pad8((unsigned)l.max_rows), nj), 256, 0, stream>>>(d_jobs);
     column_kernel<<<dim3(1, nj), kLineThreads, 0, stream>>>(d_jobs);
     rows_kernel<<<dim3(pad8((unsigned)l.max_rows), nj), kLineThreads, 0, stream>>>(d_jobs);
     eval_grid_kernel<<<dim3(pad8(cdiv(l.max_points, 256)), nj), 256, 0, stream>>>(d_jobs);
