@@ -321,12 +321,13 @@ __device__ __forceinline__ void add_moments(double* M, uint64_t& emax, uint64_t 
 __device__ void expand_coeffs(const double* M, uint32_t hc, uint64_t emax, const SegDev& S,
                               TileRec& T, TileMom* N, TileRec64* T64);
 
-__device__ void expand_range(const uint64_t* __restrict__ us, const double* __restrict__ ws,
-                             int b, int e, const SegDev& S, TileRec& T, TileMom& N,
-                             TileRec64* T64) {
-    uint64_t umin = us[b], umax = us[b];
+// The sources of [b, e) in order through src(j) -> (key, weight).
+template <class Src>
+__device__ __forceinline__ void expand_range(Src src, int b, int e, const SegDev& S, TileRec& T,
+                                             TileMom& N, TileRec64* T64) {
+    uint64_t umin = src.u(b), umax = src.u(b);
     for (int j = b + 1; j < e; ++j) {
-        const uint64_t u = us[j];
+        const uint64_t u = src.u(j);
         umin = u < umin ? u : umin;
         umax = u > umax ? u : umax;
     }
@@ -336,8 +337,49 @@ __device__ void expand_range(const uint64_t* __restrict__ us, const double* __re
     double M[kMoments64];
 #pragma unroll
     for (int k = 0; k < kMoments64; ++k) M[k] = 0.0;
-    for (int j = b; j < e; ++j) add_moments(M, emax, uc, us[j], ws[j], S.log2h);
+    for (int j = b; j < e; ++j) add_moments(M, emax, uc, src.u(j), src.w(j), S.log2h);
     expand_coeffs(M, hc, emax, S, T, &N, T64);
+}
+
+// Sources read from the sorted arrays.
+struct GlobalSrc {
+    const uint64_t* __restrict__ us;
+    const double* __restrict__ ws;
+    __device__ uint64_t u(int j) const { return us[j]; }
+    __device__ double w(int j) const { return ws[j]; }
+};
+
+// The expansion kernels give each thread (or each four lanes) whole tiles
+// of consecutive sources, which a wavefront's loads would touch 32 sources
+// apart; a workgroup of them first copies its tiles' sources [b0, b0 + len)
+// (len <= kStageSrc, coalesced) into LDS, one pad slot after every 32 so
+// that the lanes' tiles fall in different banks, and reads them from there
+// in the same order as from memory.
+constexpr int kStageThreads = 64;                    // tiles per workgroup
+constexpr int kStageSrc = kStageThreads * kTile;     // 2048 sources
+constexpr int kStageSlots = kStageSrc + kStageSrc / kTile;
+__device__ __forceinline__ int stage_slot(int j) { return j + (j >> 5); }
+struct StagedSrc {
+    const uint64_t* su;
+    const double* sw;
+    int b0;
+    __device__ uint64_t u(int j) const { return su[stage_slot(j - b0)]; }
+    __device__ double w(int j) const { return sw[stage_slot(j - b0)]; }
+};
+
+// Copies the sources [b0, e1) of a workgroup's tiles into LDS when they fit;
+// returns whether they did (the same for every thread of the workgroup).
+__device__ __forceinline__ bool stage_sources(const uint64_t* __restrict__ us,
+                                              const double* __restrict__ ws, int b0, int e1,
+                                              uint64_t* su, double* sw) {
+    const bool fits = e1 > b0 && e1 - b0 <= kStageSrc;
+    if (fits)
+        for (int j = threadIdx.x; j < e1 - b0; j += kStageThreads) {
+            su[stage_slot(j)] = us[b0 + j];
+            sw[stage_slot(j)] = ws[b0 + j];
+        }
+    __syncthreads();
+    return fits;
 }
 
 // Coefficients of the FP32 far expansion (and, when asked, the FP64 near
@@ -386,50 +428,71 @@ __device__ void expand_coeffs(const double* M, uint32_t hc, uint64_t emax, const
     }
 }
 
-// One thread per tile (tile_seg: the segment of each tile).
-__global__ __launch_bounds__(kBlock) void tile_kernel(int ntiles, const int32_t* __restrict__ tile_seg,
+// The sources [b, e) of tile t (tile_seg: the segment of each tile).
+__device__ __forceinline__ int2 tile_range(int t, const int32_t* __restrict__ tile_seg,
+                                           const SegDev* __restrict__ segs) {
+    const SegDev& S = segs[tile_seg[t]];
+    const int b = S.off + (t - S.tile_base) * kTile;
+    return make_int2(b, min(b + kTile, S.off + S.n));
+}
+
+// One thread per tile, kStageThreads tiles per workgroup, the tiles' sources
+// staged in LDS (from memory where a workgroup's tiles span more than
+// kStageSrc sources).
+__global__ __launch_bounds__(kStageThreads) void tile_kernel(int ntiles, const int32_t* __restrict__ tile_seg,
                                                       const SegDev* __restrict__ segs,
                                                       const uint64_t* __restrict__ us,
                                                       const double* __restrict__ ws,
                                                       TileRec* __restrict__ tiles,
                                                       TileRec64* __restrict__ tiles64,
                                                       TileMom* __restrict__ moms) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
+    __shared__ uint64_t su[kStageSlots];
+    __shared__ double sw[kStageSlots];
+    const int t0 = blockIdx.x * kStageThreads;
+    const int t1 = min(t0 + kStageThreads, ntiles) - 1;
+    const int b0 = tile_range(t0, tile_seg, segs).x, e1 = tile_range(t1, tile_seg, segs).y;
+    const bool staged = stage_sources(us, ws, b0, e1, su, sw);
+    const int t = t0 + (int)threadIdx.x;
     if (t >= ntiles) return;
     const SegDev& S = segs[tile_seg[t]];
-    const int i = S.off + (t - S.tile_base) * kTile;
+    const int2 r = tile_range(t, tile_seg, segs);
     TileRec T;
     TileMom N;
     TileRec64 T64;
-    expand_range(us, ws, i, min(i + kTile, S.off + S.n), S, T, N, &T64);
+    if (staged && r.x >= b0 && r.y <= e1)
+        expand_range(StagedSrc{su, sw, b0}, r.x, r.y, S, T, N, &T64);
+    else
+        expand_range(GlobalSrc{us, ws}, r.x, r.y, S, T, N, &T64);
     tiles[t] = T;
     tiles64[t] = T64;
     moms[t] = N;
 }
 
-// Span and largest tile emax of each group of 4 tiles (after tile_kernel),
-// and the group's own expansions: four adjacent lanes per group, one tile
-// each, combined by butterfly shuffles in a fixed order (grp_seg: the
-// segment of each group).
-__global__ __launch_bounds__(kBlock) void group_kernel(int ngroups, const int32_t* __restrict__ grp_seg,
-                                                       const SegDev* __restrict__ segs,
-                                                       const uint64_t* __restrict__ us,
-                                                       const double* __restrict__ ws,
-                                                       const TileRec* __restrict__ tiles,
-                                                       GroupRec* __restrict__ groups,
-                                                       TileRec* __restrict__ gexp,
-                                                       TileRec64* __restrict__ gexp64,
-                                                       TileMom* __restrict__ gmoms) {
-    const int tid = blockIdx.x * kBlock + threadIdx.x;
-    const int g = tid >> 2, sub = tid & 3;
-    if (g >= ngroups) return;  // all four lanes of a group return together
+// The sources [b, e) of group g (grp_seg: the segment of each group).
+__device__ __forceinline__ int2 group_range(int g, const int32_t* __restrict__ grp_seg,
+                                            const SegDev* __restrict__ segs) {
     const SegDev& S = segs[grp_seg[g]];
+    const int b = S.off + (g - S.group_base) * kGroupSrc;
+    return make_int2(b, min(b + kGroupSrc, S.off + S.n));
+}
+
+// Span and largest tile emax of group g of 4 tiles (after tile_kernel), and
+// the group's own expansions: four adjacent lanes, one tile each (sub),
+// combined by butterfly shuffles in a fixed order.
+template <class Src>
+__device__ __forceinline__ void group_one(int g, int sub, const SegDev& S, Src src,
+                                          const uint64_t* __restrict__ us,
+                                          const TileRec* __restrict__ tiles,
+                                          GroupRec* __restrict__ groups,
+                                          TileRec* __restrict__ gexp,
+                                          TileRec64* __restrict__ gexp64,
+                                          TileMom* __restrict__ gmoms) {
     const int i = S.off + (g - S.group_base) * kGroupSrc;
     const int ge = min(i + kGroupSrc, S.off + S.n);
     const int jb = i + sub * kTile, je = min(jb + kTile, ge);
     uint64_t umin = ~uint64_t(0), umax = 0, temax = 0;
     for (int j = jb; j < je; ++j) {
-        const uint64_t u = us[j];
+        const uint64_t u = src.u(j);
         umin = u < umin ? u : umin;
         umax = u > umax ? u : umax;
     }
@@ -448,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void group_kernel(int ngroups, const int32_
 #pragma unroll
     for (int k = 0; k < kMoments64; ++k) M[k] = 0.0;
     uint64_t emax = 0;
-    for (int j = jb; j < je; ++j) add_moments(M, emax, uc, us[j], ws[j], S.log2h);
+    for (int j = jb; j < je; ++j) add_moments(M, emax, uc, src.u(j), src.w(j), S.log2h);
 #pragma unroll
     for (int m = 1; m < 4; m <<= 1) {
 #pragma unroll
@@ -471,6 +534,35 @@ __global__ __launch_bounds__(kBlock) void group_kernel(int ngroups, const int32_
     gexp[g] = T;
     gexp64[g] = T64;
     gmoms[g] = N;
+}
+
+// Four adjacent lanes per group, kStageThreads / 4 groups per workgroup, the
+// groups' sources staged in LDS as in tile_kernel.
+__global__ __launch_bounds__(kStageThreads) void group_kernel(int ngroups, const int32_t* __restrict__ grp_seg,
+                                                       const SegDev* __restrict__ segs,
+                                                       const uint64_t* __restrict__ us,
+                                                       const double* __restrict__ ws,
+                                                       const TileRec* __restrict__ tiles,
+                                                       GroupRec* __restrict__ groups,
+                                                       TileRec* __restrict__ gexp,
+                                                       TileRec64* __restrict__ gexp64,
+                                                       TileMom* __restrict__ gmoms) {
+    __shared__ uint64_t su[kStageSlots];
+    __shared__ double sw[kStageSlots];
+    constexpr int kGroups = kStageThreads / 4;
+    const int g0 = blockIdx.x * kGroups;
+    const int g1 = min(g0 + kGroups, ngroups) - 1;
+    const int b0 = group_range(g0, grp_seg, segs).x, e1 = group_range(g1, grp_seg, segs).y;
+    const bool staged = stage_sources(us, ws, b0, e1, su, sw);
+    const int g = g0 + ((int)threadIdx.x >> 2), sub = (int)threadIdx.x & 3;
+    if (g >= ngroups) return;  // all four lanes of a group return together
+    const SegDev& S = segs[grp_seg[g]];
+    const int2 r = group_range(g, grp_seg, segs);
+    // the four lanes of a group take the same path
+    if (staged && r.x >= b0 && r.y <= e1)
+        group_one(g, sub, S, StagedSrc{su, sw, b0}, us, tiles, groups, gexp, gexp64, gmoms);
+    else
+        group_one(g, sub, S, GlobalSrc{us, ws}, us, tiles, groups, gexp, gexp64, gmoms);
 }
 
 // Span and FP32 expansion of each far-field unit of one level: one
@@ -1839,13 +1931,15 @@ int FieldBatch::run(const double* x, const double* w, double* es, hipStream_t st
     FS_HIP_CHECK(hipGetLastError());
     TileRec* tiles = reinterpret_cast<TileRec*>(wk + o_tiles_);
     if (ntiles_ > 0) {
-        tile_kernel<<<nblocks(ntiles_), kBlock, 0, stream>>>(
+        tile_kernel<<<(unsigned)((ntiles_ + kStageThreads - 1) / kStageThreads), kStageThreads, 0,
+                      stream>>>(
             ntiles_, d_tile_seg_, d_segs_, skey, ws, tiles,
             reinterpret_cast<TileRec64*>(wk + o_tiles64_), reinterpret_cast<TileMom*>(wk + o_moms_));
         FS_HIP_CHECK(hipGetLastError());
     }
     if (ngroups_ > 0)
-    group_kernel<<<nblocks(4 * (int64_t)ngroups_), kBlock, 0, stream>>>(
+    group_kernel<<<(unsigned)((4 * (int64_t)ngroups_ + kStageThreads - 1) / kStageThreads),
+                   kStageThreads, 0, stream>>>(
         ngroups_, d_grp_seg_, d_segs_, skey, ws, tiles, reinterpret_cast<GroupRec*>(wk + o_groups_),
         reinterpret_cast<TileRec*>(wk + o_gexp_), reinterpret_cast<TileRec64*>(wk + o_gexp64_),
         reinterpret_cast<TileMom*>(wk + o_gmoms_));
