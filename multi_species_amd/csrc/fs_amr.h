@@ -42,13 +42,9 @@ struct Work {
     DevBuf tmp;                // pass temporaries by panel
     DevBuf fin;                // finishing work space by point
     DevBuf out;                // weights and the reduced-x map
-    DevBuf ctr;                // Counters
-    void* rtmp = nullptr;      // sort work space inside fin
     int cap_p = 0, cap_v = 0;
     void release() {
-        for (DevBuf* b : {&rec, &x, &p, &f, &chain, &cnt, &off, &tmp, &fin, &out, &ctr})
-            b->release();
-        rtmp = nullptr;
+        for (DevBuf* b : {&rec, &x, &p, &f, &chain, &cnt, &off, &tmp, &fin, &out}) b->release();
         cap_p = cap_v = 0;
     }
 };
@@ -90,9 +86,10 @@ struct Output {
 // for all of them). One host synchronization per refinement pass and one at
 // the end; the grid interpolation of the base points must be enqueued on
 // `stream` before. On failure the species' Work holds no valid tree.
-// jobs_buf holds the launches' job tables.
+// jobs_buf holds the launches' job tables and counters, sort_buf the
+// finishing sort of all species at once.
 int refine(std::vector<Input>& in, std::vector<Output>& out, DevBuf& jobs_buf,
-           hipStream_t stream);
+           DevBuf& sort_buf, hipStream_t stream);
 
 }  // namespace amr
 }  // namespace fs

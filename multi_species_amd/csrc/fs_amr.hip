@@ -230,6 +230,49 @@ __global__ __launch_bounds__(kB) void map_kernel(const Job* __restrict__ jobs) {
     map_one(jobs[blockIdx.y], tid());
 }
 
+// The base tree and grid points of every species copied into its work
+// buffers (one launch for all of them).
+struct BaseCopy {
+    const int32_t* rec;
+    const double* x;
+    const double* p;
+    const double* f;
+    int32_t* rec_to;
+    double* x_to;
+    double* p_to;
+    double* f_to;
+    int nrec, nb;  // int32 words of the records, points
+};
+
+__global__ __launch_bounds__(kB) void base_copy_kernel(const BaseCopy* __restrict__ bc) {
+    const BaseCopy& B = bc[blockIdx.y];
+    const int i = tid();
+    if (i < B.nrec) B.rec_to[i] = B.rec[i];
+    if (i < B.nb) {
+        B.x_to[i] = B.x[i];
+        B.p_to[i] = B.p[i];
+        B.f_to[i] = B.f[i];
+    }
+}
+
+// The finishing sort of all species at once: species k's keys sit at
+// base[k] of the concatenated arrays, its segment number k beside them, and
+// its sorted slice comes back at the same place (the segment is the most
+// significant digit); the permutation is then made species-local.
+__global__ __launch_bounds__(kB) void seg_fill_kernel(const Job* __restrict__ jobs,
+                                                      const int* __restrict__ base,
+                                                      int32_t* __restrict__ seg_of) {
+    const int i = tid();
+    if (i < jobs[blockIdx.y].ctr->V) seg_of[base[blockIdx.y] + i] = (int32_t)blockIdx.y;
+}
+
+__global__ __launch_bounds__(kB) void perm_local_kernel(const Job* __restrict__ jobs,
+                                                        const int* __restrict__ base) {
+    const Job& J = jobs[blockIdx.y];
+    const int i = tid();
+    if (i < J.ctr->V) J.perm[i] -= base[blockIdx.y];
+}
+
 // ---- host side --------------------------------------------------------------
 
 unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
@@ -270,7 +313,6 @@ void set_views(Job& j, Work& w) {
     j.f = w.f.as<double>();
     j.cap_p = w.cap_p;
     j.cap_v = w.cap_v;
-    j.ctr = w.ctr.as<Counters>();
     j.chain = w.chain.as<int32_t>();
     j.cnt = w.cnt.as<int2>();
     j.off = w.off.as<int2>();
@@ -287,14 +329,14 @@ int ensure_tmp(Work& w, Job& j, int p, int v, hipStream_t st) {
     return FS_OK;
 }
 
-// Finishing work space and outputs for v points.
+// Finishing work space and outputs for v points (the keys, sorted keys and
+// permutation are slices of the shared sort buffer).
 int ensure_finish(Work& w, Job& j, int v, hipStream_t st) {
     const size_t n = (size_t)std::max(v, 1);
     const size_t b4 = align_up(4 * n, 256), b8 = align_up(8 * n, 256);
     const size_t bsl8 = align_up(8 * n * kSlots, 256);
     const size_t b2 = align_up(sizeof(int2) * n, 256);
-    const size_t rs = radix_sort_bytes((int)n);
-    FS_TRY(w.fin.ensure(b4 + 2 * bsl8 + 2 * b8 + b4 + 2 * b2 + rs, st));
+    FS_TRY(w.fin.ensure(b4 + 2 * bsl8 + 2 * b2, st));
     char* c = w.fin.as<char>();
     j.nslot = reinterpret_cast<int*>(c);
     c += b4;
@@ -302,17 +344,9 @@ int ensure_finish(Work& w, Job& j, int v, hipStream_t st) {
     c += bsl8;
     j.sval = reinterpret_cast<double*>(c);
     c += bsl8;
-    j.key = reinterpret_cast<uint64_t*>(c);
-    c += b8;
-    j.skey = reinterpret_cast<uint64_t*>(c);
-    c += b8;
-    j.perm = reinterpret_cast<int32_t*>(c);
-    c += b4;
     j.vcnt = reinterpret_cast<int2*>(c);
     c += b2;
     j.voff = reinterpret_cast<int2*>(c);
-    c += b2;
-    w.rtmp = c;
     const size_t bo = align_up(4 * (n + 1), 256);
     FS_TRY(w.out.ensure(2 * b8 + bo + 2 * b4, st));
     c = w.out.as<char>();
@@ -341,35 +375,43 @@ int status_error(int st) {
 
 }  // namespace
 
-int refine(std::vector<Input>& in, std::vector<Output>& out, DevBuf& jobs_buf, hipStream_t st) {
+int refine(std::vector<Input>& in, std::vector<Output>& out, DevBuf& jobs_buf,
+           DevBuf& sort_buf, hipStream_t st) {
     const int nj = (int)in.size();
     out.assign(nj, Output{});
     if (nj == 0) return FS_OK;
+    if (nj > 65536) return fail(FS_E_INVALID, "adaptive refinement: more than 65536 species");
     std::vector<Job> hj(nj);
     std::vector<Counters> hc(nj);
     std::vector<remesh::Job> hp(nj);
+    std::vector<BaseCopy> hb(nj);
+    // job tables, the counters of every species (one copy each way), the
+    // base copies
     const size_t bjobs = align_up(sizeof(Job) * nj, 256);
-    FS_TRY(jobs_buf.ensure(bjobs + sizeof(remesh::Job) * nj, st));
+    const size_t bpj = align_up(sizeof(remesh::Job) * nj, 256);
+    const size_t bctr = align_up(sizeof(Counters) * nj, 256);
+    FS_TRY(jobs_buf.ensure(bjobs + bpj + bctr + sizeof(BaseCopy) * nj, st));
     Job* dj = jobs_buf.as<Job>(0);
     remesh::Job* dpj = jobs_buf.as<remesh::Job>(bjobs);
+    Counters* dctr = jobs_buf.as<Counters>(bjobs + bpj);
+    BaseCopy* dbc = jobs_buf.as<BaseCopy>(bjobs + bpj + bctr);
 
     // the base tree with the grid interpolation's f
+    int max_base = 0;
     for (int k = 0; k < nj; ++k) {
         Input& I = in[k];
         Work& w = *I.w;
         if (I.eps.size() > 5 || I.mu < 1 || I.mu > I.base_panels)
             return fail(FS_E_INVALID, "adaptive refinement: bad species input");
-        FS_TRY(w.ctr.ensure(sizeof(Counters), st));
         FS_TRY(ensure_tree(w, 0, 0, I.base_panels, I.nb, st));
-        FS_HIP_CHECK(hipMemcpyAsync(w.rec.ptr, I.base_rec, sizeof(int32_t) * kI * I.base_panels,
-                                    hipMemcpyDeviceToDevice, st));
-        FS_HIP_CHECK(hipMemcpyAsync(w.x.ptr, I.gx, sizeof(double) * I.nb, hipMemcpyDeviceToDevice, st));
-        FS_HIP_CHECK(hipMemcpyAsync(w.p.ptr, I.gp, sizeof(double) * I.nb, hipMemcpyDeviceToDevice, st));
-        FS_HIP_CHECK(hipMemcpyAsync(w.f.ptr, I.gf, sizeof(double) * I.nb, hipMemcpyDeviceToDevice, st));
+        hb[k] = BaseCopy{I.base_rec, I.gx, I.gp, I.gf, w.rec.as<int32_t>(), w.x.as<double>(),
+                         w.p.as<double>(), w.f.as<double>(), kI * I.base_panels, I.nb};
+        max_base = std::max(max_base, std::max(kI * I.base_panels, I.nb));
         hc[k] = Counters{I.base_panels, I.nb, 0, 0, 0, 0, I.mu, 0};
         Job& j = hj[k];
         j = Job{};
         set_views(j, w);
+        j.ctr = dctr + k;
         j.mu = I.mu;
         j.max_height = I.max_height;
         j.periodic = I.periodic ? 1 : 0;
@@ -381,8 +423,11 @@ int refine(std::vector<Input>& in, std::vector<Output>& out, DevBuf& jobs_buf, h
         FS_TRY(ensure_tmp(w, j, I.base_panels, I.nb, st));
         hp[k] = I.old;
         hp[k].on = nullptr;
-        FS_HIP_CHECK(hipMemcpyAsync(j.ctr, &hc[k], sizeof(Counters), hipMemcpyHostToDevice, st));
     }
+    FS_HIP_CHECK(hipMemcpyAsync(dbc, hb.data(), sizeof(BaseCopy) * nj, hipMemcpyHostToDevice, st));
+    base_copy_kernel<<<dim3(cdiv(std::max(max_base, 1), kB), (unsigned)nj), kB, 0, st>>>(dbc);
+    FS_HIP_CHECK(hipGetLastError());
+    FS_HIP_CHECK(hipMemcpyAsync(dctr, hc.data(), sizeof(Counters) * nj, hipMemcpyHostToDevice, st));
     FS_HIP_CHECK(hipMemcpyAsync(dpj, hp.data(), sizeof(remesh::Job) * nj, hipMemcpyHostToDevice, st));
 
     auto upload = [&]() -> int {
@@ -390,8 +435,7 @@ int refine(std::vector<Input>& in, std::vector<Output>& out, DevBuf& jobs_buf, h
         return FS_OK;
     };
     auto readback = [&]() -> int {
-        for (int k = 0; k < nj; ++k)
-            FS_HIP_CHECK(hipMemcpyAsync(&hc[k], hj[k].ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+        FS_HIP_CHECK(hipMemcpyAsync(hc.data(), dctr, sizeof(Counters) * nj, hipMemcpyDeviceToHost, st));
         FS_HIP_CHECK(hipStreamSynchronize(st));
         int all = 0;
         for (int k = 0; k < nj; ++k) all |= hc[k].status;
@@ -454,13 +498,35 @@ int refine(std::vector<Input>& in, std::vector<Output>& out, DevBuf& jobs_buf, h
 
     // weights in depth-first leaf order, and the reduced-x map
     int maxp = 0, maxv = 0;
+    std::vector<int> base(nj);
+    int64_t total_v = 0;
     for (int k = 0; k < nj; ++k) {
         Work& w = *in[k].w;
         FS_TRY(ensure_tmp(w, hj[k], hc[k].P, hc[k].V, st));
         FS_TRY(ensure_finish(w, hj[k], hc[k].V, st));
         maxp = std::max(maxp, hc[k].P);
         maxv = std::max(maxv, hc[k].V);
+        base[k] = (int)total_v;
+        total_v += hc[k].V;
     }
+    if (total_v > INT32_MAX / 8) return fail(FS_E_RANGE, "adaptive refinement: too many points");
+    const int nv = (int)std::max<int64_t>(total_v, 1);
+    const size_t s8 = align_up(8 * (size_t)nv, 256), s4 = align_up(4 * (size_t)nv, 256);
+    const size_t sb = align_up(sizeof(int) * nj, 256);
+    FS_TRY(sort_buf.ensure(2 * s8 + 2 * s4 + sb + radix_sort_bytes(nv), st));
+    char* sc = sort_buf.as<char>();
+    uint64_t* cat_key = reinterpret_cast<uint64_t*>(sc);
+    uint64_t* cat_skey = reinterpret_cast<uint64_t*>(sc + s8);
+    int32_t* cat_perm = reinterpret_cast<int32_t*>(sc + 2 * s8);
+    int32_t* seg_of = reinterpret_cast<int32_t*>(sc + 2 * s8 + s4);
+    int* dbase = reinterpret_cast<int*>(sc + 2 * s8 + 2 * s4);
+    void* sort_tmp = sc + 2 * s8 + 2 * s4 + sb;
+    for (int k = 0; k < nj; ++k) {
+        hj[k].key = cat_key + base[k];
+        hj[k].skey = cat_skey + base[k];
+        hj[k].perm = cat_perm + base[k];
+    }
+    FS_HIP_CHECK(hipMemcpyAsync(dbase, base.data(), sizeof(int) * nj, hipMemcpyHostToDevice, st));
     FS_TRY(upload());
     const dim3 gp(cdiv(std::max(maxp, 1), kB), (unsigned)nj);
     const dim3 gv(cdiv(std::max(maxv, 1), kB), (unsigned)nj);
@@ -468,9 +534,12 @@ int refine(std::vector<Input>& in, std::vector<Output>& out, DevBuf& jobs_buf, h
     leaf_terms_kernel<<<gp, kB, 0, st>>>(dj);
     weight_kernel<<<gv, kB, 0, st>>>(dj);
     key_kernel<<<gv, kB, 0, st>>>(dj);
+    seg_fill_kernel<<<gv, kB, 0, st>>>(dj, dbase, seg_of);
     FS_HIP_CHECK(hipGetLastError());
-    for (int k = 0; k < nj; ++k)
-        FS_TRY(radix_sort_full(hj[k].key, hc[k].V, hj[k].skey, hj[k].perm, in[k].w->rtmp, st));
+    if (total_v > 0)
+        FS_TRY(radix_sort_seg_full(cat_key, (int)total_v, seg_of, nj, cat_skey, cat_perm, sort_tmp,
+                                   st));
+    perm_local_kernel<<<gv, kB, 0, st>>>(dj, dbase);
     head_kernel<<<gv, kB, 0, st>>>(dj);
     const dim3 gs(cdiv(std::max(maxv, 1), kTile), (unsigned)nj);
     scan_tile_kernel<<<gs, kB, 0, st>>>(dj, 1);

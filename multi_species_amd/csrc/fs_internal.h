@@ -81,6 +81,9 @@ int field_direct_f64(const double* d_tgt, int64_t nt, const double* d_src,
 size_t radix_sort_bytes(int n);
 int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, uint64_t* kout,
                    int32_t* iout, void* tmp, hipStream_t stream);
+// Stable sort by (segment of the original index, all 64 key bits).
+int radix_sort_seg_full(const uint64_t* kin, int n, const int32_t* seg_of, int nseg,
+                        uint64_t* kout, int32_t* iout, void* tmp, hipStream_t stream);
 // Stable sort by all 64 key bits (one segment).
 int radix_sort_full(const uint64_t* kin, int n, uint64_t* kout, int32_t* iout, void* tmp,
                     hipStream_t stream);

@@ -241,6 +241,11 @@ int radix_sort_seg(const uint64_t* kin, int n, const int32_t* seg_of, int nseg, 
     return radix_sort_bits(kin, n, seg_of, nseg, kKeyPasses, kout, iout, tmp, stream);
 }
 
+int radix_sort_seg_full(const uint64_t* kin, int n, const int32_t* seg_of, int nseg,
+                        uint64_t* kout, int32_t* iout, void* tmp, hipStream_t stream) {
+    return radix_sort_bits(kin, n, seg_of, nseg, 8, kout, iout, tmp, stream);
+}
+
 int radix_sort_full(const uint64_t* kin, int n, uint64_t* kout, int32_t* iout, void* tmp,
                     hipStream_t stream) {
     return radix_sort_bits(kin, n, nullptr, 1, 8, kout, iout, tmp, stream);

@@ -399,7 +399,7 @@ struct fs_sim {
     // reproduce); remeshes done each way
     int amr_mode = 0;
     int64_t amr_device = 0, amr_host = 0;
-    DevBuf amr_jobs;
+    DevBuf amr_jobs, amr_sort;
     int64_t steps_replayed = 0, steps_eager = 0;  // graph replays, eager steps
 
     hipStream_t stream() const { return ctx->stream; }
@@ -948,7 +948,7 @@ int refine_on_device(Sim& S, const std::vector<Species*>& refine) {
         I.w = &s->aw;
         in.push_back(std::move(I));
     }
-    FS_TRY(amr::refine(in, out, S.amr_jobs, S.stream()));
+    FS_TRY(amr::refine(in, out, S.amr_jobs, S.amr_sort, S.stream()));
     FS_TRY(read_status(S));
     for (size_t k = 0; k < refine.size(); ++k) {
         Species& s = *refine[k];
@@ -1652,6 +1652,7 @@ int fs_sim_destroy(fs_sim* S) {
     if (S->ev_r1) (void)hipEventDestroy(S->ev_r1);
     S->diag_work.release();
     S->amr_jobs.release();
+    S->amr_sort.release();
     delete S;
     return FS_OK;
 }
