@@ -43,8 +43,21 @@ struct Work {
     DevBuf fin;                // finishing work space by point
     DevBuf out;                // weights and the reduced-x map
     int cap_p = 0, cap_v = 0;
-    void release() {
-        for (DevBuf* b : {&rec, &x, &p, &f, &chain, &cnt, &off, &tmp, &fin, &out}) b->release();
+    // blocks come from and go back to this pool when set (the buffers are
+    // used on the pool's stream only)
+    BlockPool* pool = nullptr;
+    int ensure(DevBuf& b, size_t bytes, hipStream_t st) {
+        return pool ? pool->ensure(b, bytes, st) : b.ensure(bytes, st);
+    }
+    void put(DevBuf& b) {
+        if (pool) pool->put(b);
+        else b.release();
+    }
+    void release(BlockPool* to) {
+        for (DevBuf* b : {&rec, &x, &p, &f, &chain, &cnt, &off, &tmp, &fin, &out}) {
+            if (to) to->put(*b);
+            else b->release();
+        }
         cap_p = cap_v = 0;
     }
 };

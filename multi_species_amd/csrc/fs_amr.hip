@@ -277,14 +277,16 @@ __global__ __launch_bounds__(kB) void perm_local_kernel(const Job* __restrict__ 
 
 unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
-// Grows a buffer to `need` bytes keeping its first `used` bytes.
-int grow_keep(DevBuf& b, size_t used, size_t need, hipStream_t st) {
+// Grows a buffer of w to `need` bytes keeping its first `used` bytes (the
+// old block is let go after the copy is enqueued: a pooled block is reused
+// only by later work on the same stream, a freed one after the device's).
+int grow_keep(Work& w, DevBuf& b, size_t used, size_t need, hipStream_t st) {
     if (need <= b.cap) return FS_OK;
     DevBuf nb;
-    FS_TRY(nb.ensure(std::max(need, b.cap + b.cap / 2), st));
+    FS_TRY(w.ensure(nb, std::max(need, b.cap + b.cap / 2), st));
     if (used > 0) FS_HIP_CHECK(hipMemcpyAsync(nb.ptr, b.ptr, used, hipMemcpyDeviceToDevice, st));
-    FS_HIP_CHECK(hipStreamSynchronize(st));
-    b.release();
+    if (!w.pool) FS_HIP_CHECK(hipStreamSynchronize(st));
+    w.put(b);
     b = nb;
     nb.ptr = nullptr;
     nb.cap = 0;
@@ -293,12 +295,13 @@ int grow_keep(DevBuf& b, size_t used, size_t need, hipStream_t st) {
 
 // Tree and pass-state capacity for p panels and v points (contents kept).
 int ensure_tree(Work& w, int used_p, int used_v, int p, int v, hipStream_t st) {
-    FS_TRY(grow_keep(w.rec, sizeof(int32_t) * kI * (size_t)used_p, sizeof(int32_t) * kI * (size_t)p, st));
-    FS_TRY(grow_keep(w.chain, sizeof(int32_t) * (size_t)used_p, sizeof(int32_t) * (size_t)p, st));
-    FS_TRY(grow_keep(w.cnt, sizeof(int2) * (size_t)used_p, sizeof(int2) * (size_t)p, st));
-    FS_TRY(grow_keep(w.off, sizeof(int2) * (size_t)used_p, sizeof(int2) * (size_t)p, st));
+    FS_TRY(grow_keep(w, w.rec, sizeof(int32_t) * kI * (size_t)used_p,
+                     sizeof(int32_t) * kI * (size_t)p, st));
+    FS_TRY(grow_keep(w, w.chain, sizeof(int32_t) * (size_t)used_p, sizeof(int32_t) * (size_t)p, st));
+    FS_TRY(grow_keep(w, w.cnt, sizeof(int2) * (size_t)used_p, sizeof(int2) * (size_t)p, st));
+    FS_TRY(grow_keep(w, w.off, sizeof(int2) * (size_t)used_p, sizeof(int2) * (size_t)p, st));
     for (DevBuf* b : {&w.x, &w.p, &w.f})
-        FS_TRY(grow_keep(*b, sizeof(double) * (size_t)used_v, sizeof(double) * (size_t)v, st));
+        FS_TRY(grow_keep(w, *b, sizeof(double) * (size_t)used_v, sizeof(double) * (size_t)v, st));
     w.cap_p = (int)std::min<size_t>(w.rec.cap / (sizeof(int32_t) * kI),
                                     std::min(w.chain.cap / sizeof(int32_t), w.cnt.cap / sizeof(int2)));
     w.cap_p = (int)std::min<size_t>(w.cap_p, w.off.cap / sizeof(int2));
@@ -323,7 +326,7 @@ void set_views(Job& j, Work& w) {
 int ensure_tmp(Work& w, Job& j, int p, int v, hipStream_t st) {
     const size_t bp = align_up(sizeof(int32_t) * kPlanInts * (size_t)p, 256);
     const size_t bs = align_up(sizeof(int2) * (cdiv(std::max(p, v), kTile) + 1), 256);
-    FS_TRY(w.tmp.ensure(bp + bs, st));
+    FS_TRY(w.ensure(w.tmp, bp + bs, st));
     j.plan = w.tmp.as<int32_t>(0);
     j.bsum = w.tmp.as<int2>(bp);
     return FS_OK;
@@ -336,7 +339,7 @@ int ensure_finish(Work& w, Job& j, int v, hipStream_t st) {
     const size_t b4 = align_up(4 * n, 256), b8 = align_up(8 * n, 256);
     const size_t bsl8 = align_up(8 * n * kSlots, 256);
     const size_t b2 = align_up(sizeof(int2) * n, 256);
-    FS_TRY(w.fin.ensure(b4 + 2 * bsl8 + 2 * b2, st));
+    FS_TRY(w.ensure(w.fin, b4 + 2 * bsl8 + 2 * b2, st));
     char* c = w.fin.as<char>();
     j.nslot = reinterpret_cast<int*>(c);
     c += b4;
@@ -348,7 +351,7 @@ int ensure_finish(Work& w, Job& j, int v, hipStream_t st) {
     c += b2;
     j.voff = reinterpret_cast<int2*>(c);
     const size_t bo = align_up(4 * (n + 1), 256);
-    FS_TRY(w.out.ensure(2 * b8 + bo + 2 * b4, st));
+    FS_TRY(w.ensure(w.out, 2 * b8 + bo + 2 * b4, st));
     c = w.out.as<char>();
     j.bw = reinterpret_cast<double*>(c);
     c += b8;

@@ -44,6 +44,73 @@ void DevBuf::release() {
     cap = 0;
 }
 
+// Pooled bytes beyond which released blocks are freed.
+constexpr size_t kPoolLimit = size_t(16) << 30;
+
+int BlockPool::ensure(DevBuf& b, size_t bytes, hipStream_t stream) {
+    (void)stream;  // every block here is used on the pool's stream only
+    if (bytes <= b.cap) return FS_OK;
+    const size_t want = align_up(bytes, size_t(2) << 20);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (b.ptr) {
+            blocks.emplace(b.cap, b.ptr);
+            held += b.cap;
+            b.ptr = nullptr;
+            b.cap = 0;
+        }
+        auto it = blocks.lower_bound(want);
+        if (it != blocks.end() && it->first <= 2 * want) {
+            b.ptr = it->second;
+            b.cap = it->first;
+            held -= it->first;
+            blocks.erase(it);
+            return FS_OK;
+        }
+    }
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (hipMalloc(&b.ptr, want) == hipSuccess) {
+            b.cap = want;
+            return FS_OK;
+        }
+        b.ptr = nullptr;
+        (void)hipGetLastError();
+        if (attempt == 0) {
+            // give the pooled blocks back and try once more
+            FS_HIP_CHECK(hipStreamSynchronize(stream));
+            clear();
+        }
+    }
+    return fail(FS_E_NOMEM, "hipMalloc of %zu bytes failed", want);
+}
+
+void BlockPool::put(DevBuf& b) {
+    if (!b.ptr) return;
+    std::vector<void*> drop;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        blocks.emplace(b.cap, b.ptr);
+        held += b.cap;
+        // beyond the limit the largest blocks are freed
+        while (held > kPoolLimit && !blocks.empty()) {
+            auto it = std::prev(blocks.end());
+            held -= it->first;
+            drop.push_back(it->second);
+            blocks.erase(it);
+        }
+    }
+    b.ptr = nullptr;
+    b.cap = 0;
+    for (void* q : drop) (void)hipFree(q);
+}
+
+void BlockPool::clear() {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& kv : blocks) (void)hipFree(kv.second);
+    blocks.clear();
+    held = 0;
+}
+
 }  // namespace fs
 
 extern "C" {
@@ -78,6 +145,7 @@ int fs_ctx_destroy(fs_ctx* ctx) {
     if (!ctx) return FS_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx->pool.clear();
     ctx->scratch.release();
     ctx->io.release();
     ctx->aux.release();

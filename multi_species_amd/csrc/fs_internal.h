@@ -49,6 +49,25 @@ struct DevBuf {
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Device blocks released by buffers used on one stream only, kept for reuse
+// by buffers used on the same stream (every use of a released block is
+// enqueued before any use of its next owner): decks joining and leaving a
+// batch, and layouts that resize it, then allocate and free nothing, and no
+// hipFree synchronizes the device. Thread-safe.
+struct BlockPool {
+    std::mutex mu;
+    std::multimap<size_t, void*> blocks;  // capacity -> block
+    size_t held = 0;
+    // Grows b to `bytes` like DevBuf::ensure (contents not kept), its old
+    // block going to the pool and a pooled block of at most twice the need
+    // reused when there is one.
+    int ensure(DevBuf& b, size_t bytes, hipStream_t stream);
+    // b's block into the pool (b emptied).
+    void put(DevBuf& b);
+    // Frees every pooled block (the stream's work on them must be done).
+    void clear();
+};
+
 // Field-solver launch state kept per context.
 struct FieldTuning {
     int targets_per_thread = 0;  // 0 = automatic
@@ -155,6 +174,7 @@ struct fs_ctx {
     fs::DevBuf aux;       // concatenated points of the mixed entry points
     fs::TreeWork tree;    // the treecode entry points' work space
     fs::FieldTuning tuning;
+    fs::BlockPool pool;   // blocks of the simulations' buffers (all on `stream`)
     fs::FieldBatch* api_batch = nullptr;  // plan of the last mixed entry-point call
     double mixed_target = 0;              // > 0: mixed entry points hold this error bound
     int64_t f64_pair_evals = 0;           // kernel evaluations of the last FP64 direct sum

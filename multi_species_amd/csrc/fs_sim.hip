@@ -307,15 +307,17 @@ struct Species {
     int n() const { return dev_tree ? amr_out.V : (int)tree.xs.size(); }
     int npanels() const { return dev_tree ? amr_out.P : tree.size(); }
     int nunique() const { return dev_tree ? amr_out.U : (int)red.ux.size(); }
+    // the device blocks back to the context's pool (every use of them is on
+    // the context's stream)
+    void release(BlockPool& pool) {
+        for (DevBuf* b : {&d_rec, &d_grid, &d_bw, &d_fnew, &d_old, &d_work, &d_base_rec})
+            pool.put(*b);
+        aw.release(&pool);
+    }
     void release() {
-        d_rec.release();
-        d_grid.release();
-        d_bw.release();
-        d_fnew.release();
-        d_old.release();
-        d_work.release();
-        aw.release();
-        d_base_rec.release();
+        for (DevBuf* b : {&d_rec, &d_grid, &d_bw, &d_fnew, &d_old, &d_work, &d_base_rec})
+            b->release();
+        aw.release(nullptr);
     }
 };
 
@@ -588,23 +590,25 @@ int remesh_static(Sim& S) {
 int upload_species(Sim& S, const Deck& D, Species& s, bool grid_too) {
     hipStream_t st = S.stream();
     const int n = s.n(), nb = s.nx * s.np, P = s.npanels();
-    // a tree refined on the device is already there (records, weights)
-    if (!s.dev_tree) {
-        FS_TRY(s.d_rec.ensure(sizeof(int32_t) * s.tree.rec.size(), st));
+    s.aw.pool = &S.ctx->pool;
+    // a tree refined on the device is already there (records, weights); a
+    // static species' tree and weights never change once uploaded
+    if (!s.dev_tree && (grid_too || s.adaptive)) {
+        FS_TRY(S.ctx->pool.ensure(s.d_rec, sizeof(int32_t) * s.tree.rec.size(), st));
         FS_HIP_CHECK(hipMemcpyAsync(s.d_rec.ptr, s.tree.rec.data(),
                                     sizeof(int32_t) * s.tree.rec.size(), hipMemcpyHostToDevice, st));
-        FS_TRY(s.d_bw.ensure(sizeof(double) * n, st));
+        FS_TRY(S.ctx->pool.ensure(s.d_bw, sizeof(double) * n, st));
         FS_HIP_CHECK(hipMemcpyAsync(s.d_bw.ptr, s.base_w.data(), sizeof(double) * n,
                                     hipMemcpyHostToDevice, st));
     }
     if (s.adaptive && !s.d_base_rec.ptr) {
-        FS_TRY(s.d_base_rec.ensure(sizeof(int32_t) * s.base.rec.size(), st));
+        FS_TRY(S.ctx->pool.ensure(s.d_base_rec, sizeof(int32_t) * s.base.rec.size(), st));
         FS_HIP_CHECK(hipMemcpyAsync(s.d_base_rec.ptr, s.base.rec.data(),
                                     sizeof(int32_t) * s.base.rec.size(), hipMemcpyHostToDevice, st));
     }
     if (grid_too) {
         const size_t bg = align_up(sizeof(double) * nb, 256);
-        FS_TRY(s.d_grid.ensure(2 * bg + sizeof(int32_t) * nb, st));
+        FS_TRY(S.ctx->pool.ensure(s.d_grid, 2 * bg + sizeof(int32_t) * nb, st));
         s.gx = s.d_grid.as<double>(0);
         s.gp = s.d_grid.as<double>(bg);
         s.rows_d = s.d_grid.as<int32_t>(2 * bg);
@@ -615,9 +619,9 @@ int upload_species(Sim& S, const Deck& D, Species& s, bool grid_too) {
         FS_HIP_CHECK(hipMemcpyAsync(s.rows_d, s.rows.data(), sizeof(int32_t) * nb,
                                     hipMemcpyHostToDevice, st));
     }
-    FS_TRY(s.d_fnew.ensure(sizeof(double) * std::max(n, nb), st));
-    FS_TRY(s.d_old.ensure(rm::mesh_bytes(P), st));
-    FS_TRY(s.d_work.ensure(rm::grid_bytes(nb, s.np), st));
+    FS_TRY(S.ctx->pool.ensure(s.d_fnew, sizeof(double) * std::max(n, nb), st));
+    FS_TRY(S.ctx->pool.ensure(s.d_old, rm::mesh_bytes(P), st));
+    FS_TRY(S.ctx->pool.ensure(s.d_work, rm::grid_bytes(nb, s.np), st));
     rm::Job& j = s.job;
     j = rm::Job{};
     rm::carve_mesh(j, s.d_old.ptr, P);
@@ -662,7 +666,7 @@ int layout(Sim& S, bool keep_resident) {
     const int nsp = (int)S.sp.size();
     const size_t bN = align_up(sizeof(double) * std::max(N, 1), 256);
     DevBuf state;
-    FS_TRY(state.ensure(5 * bN, st));
+    FS_TRY(S.ctx->pool.ensure(state, 5 * bN, st));
     double* xs = state.as<double>(0);
     double* ps = state.as<double>(bN);
     double* qw = state.as<double>(2 * bN);
@@ -704,7 +708,7 @@ int layout(Sim& S, bool keep_resident) {
     // the previous arrays must outlive the copies from them
     FS_HIP_CHECK(hipStreamSynchronize(st));
     std::swap(S.state, state);
-    state.release();
+    S.ctx->pool.put(state);
     S.xs = xs;
     S.ps = ps;
     S.qw = qw;
@@ -721,7 +725,7 @@ int layout(Sim& S, bool keep_resident) {
         s.from_amr = false;
         o += s.n();
     }
-    FS_TRY(S.rk.ensure(7 * bN, st));
+    FS_TRY(S.ctx->pool.ensure(S.rk, 7 * bN, st));
     S.xk = S.rk.as<double>(0);
     S.p2 = S.rk.as<double>(bN);
     S.p3 = S.rk.as<double>(2 * bN);
@@ -739,7 +743,7 @@ int layout(Sim& S, bool keep_resident) {
         info[k] = SpeciesInfo{s.qm, 0.5 * dt, dt, dt / 6.0};
     }
     const size_t bsid = align_up(sizeof(int32_t) * sidh.size(), 256);
-    FS_TRY(S.pinfo.ensure(bsid + sizeof(SpeciesInfo) * info.size(), st));
+    FS_TRY(S.ctx->pool.ensure(S.pinfo, bsid + sizeof(SpeciesInfo) * info.size(), st));
     S.sid = S.pinfo.as<int32_t>(0);
     S.sinfo = S.pinfo.as<SpeciesInfo>(bsid);
     FS_HIP_CHECK(hipMemcpyAsync(S.sid, sidh.data(), sizeof(int32_t) * sidh.size(),
@@ -783,7 +787,7 @@ int layout(Sim& S, bool keep_resident) {
     const size_t bU = align_up(sizeof(double) * std::max(S.U, 1), 256);
     const size_t bo = align_up(sizeof(int32_t) * (S.U + 1), 256);
     const size_t bm = align_up(sizeof(int32_t) * std::max(N, 1), 256);
-    FS_TRY(S.red.ensure(3 * bU + bo + 3 * bm, st));
+    FS_TRY(S.ctx->pool.ensure(S.red, 3 * bU + bo + 3 * bm, st));
     S.rx = S.red.as<double>(0);
     S.rw = S.red.as<double>(bU);
     S.re = S.red.as<double>(2 * bU);
@@ -1406,7 +1410,7 @@ int attach_decks(fs_sim* S, std::vector<Deck*>& decks, std::vector<int>* indices
         // (layout copies it device to device from wherever it lives now)
         const std::string why = fs_last_error();
         for (size_t k = before; k < S->decks.size(); ++k)
-            for (auto& s : S->decks[k]->sp) s->release();
+            for (auto& s : S->decks[k]->sp) s->release(S->ctx->pool);
         S->decks.resize(before);
         renumber(*S);
         S->tune_pending = true;
@@ -1450,7 +1454,7 @@ int detach_decks(fs_sim* S, const std::vector<int>& idx,
     std::vector<std::unique_ptr<Deck>> keep;
     for (size_t d = 0; d < S->decks.size(); ++d) {
         if (drop[d]) {
-            for (auto& s : S->decks[d]->sp) s->release();
+            for (auto& s : S->decks[d]->sp) s->release(S->ctx->pool);
         } else {
             keep.push_back(std::move(S->decks[d]));
         }
@@ -1535,7 +1539,7 @@ int fs_sim_remove_deck(fs_sim* S, int deck) {
         return fail(FS_E_INVALID, "fs_sim_remove_deck: bad deck index");
     FS_HIP_CHECK(hipSetDevice(S->ctx->device));
     FS_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
-    for (auto& s : S->decks[deck]->sp) s->release();
+    for (auto& s : S->decks[deck]->sp) s->release(S->ctx->pool);
     S->decks.erase(S->decks.begin() + deck);
     sim::renumber(*S);
     S->tune_pending = true;
@@ -1636,7 +1640,7 @@ int fs_sim_destroy(fs_sim* S) {
     (void)hipStreamSynchronize(S->ctx->stream);
     sim::drop_graphs(*S);
     for (auto& D : S->decks)
-        for (auto& s : D->sp) s->release();
+        for (auto& s : D->sp) s->release(S->ctx->pool);
     S->state.release();
     S->rk.release();
     S->red.release();
