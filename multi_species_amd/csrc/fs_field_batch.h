@@ -170,6 +170,9 @@ public:
     // the evaluated segments (others untouched).
     int run(const double* x, const double* w, double* es, hipStream_t stream);
     void release();
+    // buffers from (and, at release, back to) a block pool when every use
+    // of this batch is on the pool's stream (a simulation's)
+    void set_pool(BlockPool* p) { pool_ = p; }
     int precision() const { return precision_; }
     int tpt() const { return tpt_; }
     int wgs_per_cu() const { return wgs_per_cu_; }
@@ -202,7 +205,11 @@ private:
     std::vector<SegDev> segs_;
     std::vector<WorkItem> items_;
     std::vector<ScanBlock> scan_;
+    BlockPool* pool_ = nullptr;
     DevBuf tables_, work_, items_buf_;
+    int grow(DevBuf& b, size_t bytes, hipStream_t stream) {
+        return pool_ ? pool_->ensure(b, bytes, stream) : b.ensure(bytes, stream);
+    }
     // device views
     SegDev* d_segs_ = nullptr;
     WorkItem* d_items_ = nullptr;

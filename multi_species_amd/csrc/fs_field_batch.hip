@@ -1404,6 +1404,21 @@ __global__ __launch_bounds__(kBlock) void finalize_mixed_kernel(
 
 __global__ void zero_int_kernel(int* p) { *p = 0; }
 
+// The segment of every point for segments in ascending order: the last one
+// starting at or before it, 0 where none holds it (as the host fill leaves).
+__global__ __launch_bounds__(kBlock) void seg_of_kernel(const SegDev* __restrict__ segs, int nseg,
+                                                        int n, int32_t* __restrict__ seg_of) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (segs[m].off <= i) lo = m;
+        else hi = m - 1;
+    }
+    seg_of[i] = (segs[lo].off <= i && i < segs[lo].off + segs[lo].n) ? lo : 0;
+}
+
 __global__ void zero_u64_kernel(uint64_t* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -1479,7 +1494,14 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     scan_.clear();
     pairs_ = evals_ = 0;
     symsegs_.clear();
-    std::vector<int32_t> seg_of(std::max(n_total, 1), 0);
+    // segments in ascending order of their points (the step loop's are):
+    // each point's segment is found on the device; else written here
+    bool ascending = true;
+    for (int d = 0, end = 0; d < nseg_; ++d) {
+        if (segs[d].off < end) ascending = false;
+        end = std::max(end, segs[d].off + segs[d].n);
+    }
+    std::vector<int32_t> seg_of(ascending ? 0 : std::max(n_total, 1), 0);
     int64_t part_total = 0;
     for (int d = 0; d < nseg_; ++d) {
         const BatchSeg& b = segs[d];
@@ -1490,7 +1512,8 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
             return fail(FS_E_INVALID, "field batch: mixed precision needs eps > 0");
         if (precision == kPrecisionMixed && b.eval && b.open)
             return fail(FS_E_INVALID, "field batch: open boundaries run in FP64 only");
-        for (int g = b.off; g < b.off + b.n; ++g) seg_of[g] = d;
+        if (!ascending)
+            for (int g = b.off; g < b.off + b.n; ++g) seg_of[g] = d;
         SegDev& S = segs_[d];
         S.off = b.off;
         S.n = b.n;
@@ -1642,13 +1665,13 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     // device tables
     const size_t b_segs = align_up(sizeof(SegDev) * nseg_, 256);
     const size_t b_scan = align_up(sizeof(ScanBlock) * std::max<size_t>(scan_.size(), 1), 256);
-    const size_t b_segof = align_up(sizeof(int32_t) * seg_of.size(), 256);
+    const size_t b_segof = align_up(sizeof(int32_t) * std::max(n_total, 1), 256);
     const size_t b_sym = align_up(sizeof(SymSeg) * std::max<size_t>(symsegs_.size(), 1), 256);
     const size_t b_tseg = align_up(sizeof(int32_t) * tile_seg.size(), 256);
     const size_t b_gseg = align_up(sizeof(int32_t) * grp_seg.size(), 256);
     const size_t b_useg = align_up(sizeof(int32_t) * unit_seg.size(), 256);
-    FS_TRY(tables_.ensure(b_segs + b_scan + b_segof + b_sym + b_tseg + b_gseg + b_useg + 256,
-                          stream));
+    FS_TRY(grow(tables_, b_segs + b_scan + b_segof + b_sym + b_tseg + b_gseg + b_useg + 256,
+                stream));
     d_segs_ = tables_.as<SegDev>(0);
     d_scan_ = tables_.as<ScanBlock>(b_segs);
     d_seg_of_ = tables_.as<int32_t>(b_segs + b_scan);
@@ -1671,8 +1694,13 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
     if (!scan_.empty())
         FS_HIP_CHECK(hipMemcpyAsync(d_scan_, scan_.data(), sizeof(ScanBlock) * scan_.size(),
                                     hipMemcpyHostToDevice, stream));
-    FS_HIP_CHECK(hipMemcpyAsync(d_seg_of_, seg_of.data(), sizeof(int32_t) * seg_of.size(),
-                                hipMemcpyHostToDevice, stream));
+    if (!ascending) {
+        FS_HIP_CHECK(hipMemcpyAsync(d_seg_of_, seg_of.data(), sizeof(int32_t) * seg_of.size(),
+                                    hipMemcpyHostToDevice, stream));
+    } else if (n_total > 0) {
+        seg_of_kernel<<<nblocks(n_total), kBlock, 0, stream>>>(d_segs_, nseg_, n_total, d_seg_of_);
+        FS_HIP_CHECK(hipGetLastError());
+    }
     zero_int_kernel<<<1, 1, 0, stream>>>(d_counter_);
     FS_HIP_CHECK(hipGetLastError());
     // work space
@@ -1707,7 +1735,7 @@ int FieldBatch::plan(const std::vector<BatchSeg>& segs, int n_total, int precisi
         o_gexp64_ = take(sizeof(TileRec64) * (size_t)std::max(ngroups_, 1));
         o_gmoms_ = take(sizeof(TileMom) * (size_t)std::max(nrecs_, 1));
     }
-    FS_TRY(work_.ensure(o, stream));
+    FS_TRY(grow(work_, o, stream));
     if (precision == kPrecisionMixed) {
         zero_u64_kernel<<<nblocks(nseg_ * kGovVals), kBlock, 0, stream>>>(gov(), nseg_ * kGovVals);
         FS_HIP_CHECK(hipGetLastError());
@@ -1739,7 +1767,7 @@ void FieldBatch::build_items(int tpt, std::vector<WorkItem>& items) const {
 }
 
 int FieldBatch::upload_items(const std::vector<WorkItem>& items, hipStream_t stream) {
-    FS_TRY(items_buf_.ensure(sizeof(WorkItem) * std::max<size_t>(items.size(), 1), stream));
+    FS_TRY(grow(items_buf_, sizeof(WorkItem) * std::max<size_t>(items.size(), 1), stream));
     d_items_ = items_buf_.as<WorkItem>();
     if (!items.empty())
         FS_HIP_CHECK(hipMemcpyAsync(d_items_, items.data(), sizeof(WorkItem) * items.size(),

@@ -223,10 +223,10 @@ __global__ __launch_bounds__(kBlock) void qw_kernel(int n, const double* __restr
 // ... and its reduced-x map, placed at the species' offsets: points from o,
 // distinct x from ub.
 __global__ __launch_bounds__(kBlock) void place_map_kernel(
-    int n, int U, int o, int ub, const int32_t* __restrict__ roff,
+    int n, int U, int o, int ub, int k, const int32_t* __restrict__ roff,
     const int32_t* __restrict__ members, const int32_t* __restrict__ uof,
     const double* __restrict__ ux, int32_t* __restrict__ r_off, int32_t* __restrict__ r_mem,
-    int32_t* __restrict__ r_uof, double* __restrict__ rx) {
+    int32_t* __restrict__ r_uof, double* __restrict__ rx, int32_t* __restrict__ rsid) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
         r_mem[o + i] = o + members[i];
@@ -235,7 +235,25 @@ __global__ __launch_bounds__(kBlock) void place_map_kernel(
     if (i < U) {
         r_off[ub + i] = o + roff[i];
         rx[ub + i] = ux[i];
+        rsid[ub + i] = k;
     }
+}
+
+__global__ void set_int_kernel(int32_t* a, int v) { *a = v; }
+
+// The species of every point: the last species whose range starts at or
+// before it (ranges ascending and contiguous).
+__global__ __launch_bounds__(kBlock) void sid_kernel(int n, const int2* __restrict__ range,
+                                                     int nsp, int32_t* __restrict__ sid) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = nsp - 1;
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (range[m].x <= i) lo = m;
+        else hi = m - 1;
+    }
+    sid[i] = lo;
 }
 
 __global__ __launch_bounds__(kBlock) void zero_ints(int* a, int n) {
@@ -289,6 +307,7 @@ struct Species {
     int off = -1;          // offset in the concatenated arrays (-1: not resident)
 
     DevBuf d_rec, d_grid, d_bw, d_fnew, d_old, d_work;
+    DevBuf d_red;  // the reduced-x map (ux, offsets, members, unique_of) of a host tree
     double *gx = nullptr, *gp = nullptr;
     int32_t* rows_d = nullptr;
     remesh::Job job{};  // this species' grid interpolation (device pointers)
@@ -310,12 +329,12 @@ struct Species {
     // the device blocks back to the context's pool (every use of them is on
     // the context's stream)
     void release(BlockPool& pool) {
-        for (DevBuf* b : {&d_rec, &d_grid, &d_bw, &d_fnew, &d_old, &d_work, &d_base_rec})
+        for (DevBuf* b : {&d_rec, &d_grid, &d_bw, &d_fnew, &d_old, &d_work, &d_base_rec, &d_red})
             pool.put(*b);
         aw.release(&pool);
     }
     void release() {
-        for (DevBuf* b : {&d_rec, &d_grid, &d_bw, &d_fnew, &d_old, &d_work, &d_base_rec})
+        for (DevBuf* b : {&d_rec, &d_grid, &d_bw, &d_fnew, &d_old, &d_work, &d_base_rec, &d_red})
             b->release();
         aw.release(nullptr);
     }
@@ -600,6 +619,26 @@ int upload_species(Sim& S, const Deck& D, Species& s, bool grid_too) {
         FS_TRY(S.ctx->pool.ensure(s.d_bw, sizeof(double) * n, st));
         FS_HIP_CHECK(hipMemcpyAsync(s.d_bw.ptr, s.base_w.data(), sizeof(double) * n,
                                     hipMemcpyHostToDevice, st));
+        const mesh::ReducedMap& r = s.red;
+        const size_t U = r.ux.size();
+        const size_t bu = align_up(sizeof(double) * U, 256);
+        const size_t bo = align_up(sizeof(int32_t) * (U + 1), 256);
+        const size_t bm = align_up(sizeof(int32_t) * r.members.size(), 256);
+        FS_TRY(S.ctx->pool.ensure(s.d_red, bu + bo + bm + sizeof(int32_t) * r.unique_of.size() + 256,
+                                  st));
+        if (U > 0)
+            FS_HIP_CHECK(hipMemcpyAsync(s.d_red.ptr, r.ux.data(), sizeof(double) * U,
+                                        hipMemcpyHostToDevice, st));
+        FS_HIP_CHECK(hipMemcpyAsync(s.d_red.as<int32_t>(bu), r.offsets.data(),
+                                    sizeof(int32_t) * r.offsets.size(), hipMemcpyHostToDevice, st));
+        if (!r.members.empty())
+            FS_HIP_CHECK(hipMemcpyAsync(s.d_red.as<int32_t>(bu + bo), r.members.data(),
+                                        sizeof(int32_t) * r.members.size(), hipMemcpyHostToDevice,
+                                        st));
+        if (!r.unique_of.empty())
+            FS_HIP_CHECK(hipMemcpyAsync(s.d_red.as<int32_t>(bu + bo + bm), r.unique_of.data(),
+                                        sizeof(int32_t) * r.unique_of.size(),
+                                        hipMemcpyHostToDevice, st));
     }
     if (s.adaptive && !s.d_base_rec.ptr) {
         FS_TRY(S.ctx->pool.ensure(s.d_base_rec, sizeof(int32_t) * s.base.rec.size(), st));
@@ -733,55 +772,49 @@ int layout(Sim& S, bool keep_resident) {
     S.e2 = S.rk.as<double>(4 * bN);
     S.e3 = S.rk.as<double>(5 * bN);
     S.e4 = S.rk.as<double>(6 * bN);
-    // per-point species index and per-species constants
-    std::vector<int32_t> sidh(std::max(N, 1), 0);
+    // per-species constants and ranges; the per-point species index on the
+    // device
     std::vector<SpeciesInfo> info(std::max(nsp, 1));
+    std::vector<int2> range(std::max(nsp, 1), make_int2(0, 0));
     for (int k = 0; k < nsp; ++k) {
         const Species& s = *S.sp[k];
-        for (int i = 0; i < s.n(); ++i) sidh[s.off + i] = k;
         const double dt = S.decks[s.deck]->prm.dt;
         info[k] = SpeciesInfo{s.qm, 0.5 * dt, dt, dt / 6.0};
+        range[k] = make_int2(s.off, s.n());
     }
-    const size_t bsid = align_up(sizeof(int32_t) * sidh.size(), 256);
-    FS_TRY(S.ctx->pool.ensure(S.pinfo, bsid + sizeof(SpeciesInfo) * info.size(), st));
+    const size_t bsid = align_up(sizeof(int32_t) * std::max(N, 1), 256);
+    const size_t binfo = align_up(sizeof(SpeciesInfo) * info.size(), 256);
+    FS_TRY(S.ctx->pool.ensure(S.pinfo, bsid + binfo + sizeof(int2) * range.size(), st));
     S.sid = S.pinfo.as<int32_t>(0);
     S.sinfo = S.pinfo.as<SpeciesInfo>(bsid);
-    FS_HIP_CHECK(hipMemcpyAsync(S.sid, sidh.data(), sizeof(int32_t) * sidh.size(),
-                                hipMemcpyHostToDevice, st));
+    int2* d_range = S.pinfo.as<int2>(bsid + binfo);
     FS_HIP_CHECK(hipMemcpyAsync(S.sinfo, info.data(), sizeof(SpeciesInfo) * info.size(),
                                 hipMemcpyHostToDevice, st));
+    FS_HIP_CHECK(hipMemcpyAsync(d_range, range.data(), sizeof(int2) * range.size(),
+                                hipMemcpyHostToDevice, st));
+    if (N > 0 && nsp > 0) {
+        sid_kernel<<<blocks(N), kBlock, 0, st>>>(N, d_range, nsp, S.sid);
+        FS_HIP_CHECK(hipGetLastError());
+    }
+    // per-species device copies of what a remesh reads (and the host trees'
+    // reduced-x maps)
+    for (auto& D : S.decks)
+        for (auto& sp : D->sp) FS_TRY(upload_species(S, *D, *sp, sp->d_grid.ptr == nullptr || !keep_resident));
     // reduced map: each deck's species' unique x concatenated (duplicates
-    // across species kept, AMRSimulation_field.cpp:9-13)
-    std::vector<double> rxh;
-    std::vector<int32_t> off(1, 0), mem, uof(std::max(N, 1)), rsid;
-    int ubase = 0, kspec = 0;
-    std::vector<std::pair<const Species*, int>> placed;  // device maps and their bases
-    for (auto& D : S.decks) {
-        D->uoff = ubase;
-        for (auto& sp : D->sp) {
-            const Species& s = *sp;
-            if (s.dev_tree) {
-                // placeholders, written by place_map_kernel below
-                const int U = s.amr_out.U;
-                rxh.resize(rxh.size() + U, 0.0);
-                rsid.insert(rsid.end(), U, kspec++);
-                off.resize(off.size() + U, s.off + s.n());
-                mem.resize(mem.size() + s.n(), 0);
-                placed.emplace_back(&s, ubase);
-                ubase += U;
-                continue;
+    // across species kept, AMRSimulation_field.cpp:9-13), each species' map
+    // placed on the device at its offsets
+    int ubase = 0;
+    std::vector<int> ub(std::max(nsp, 1), 0);
+    {
+        int k = 0;
+        for (auto& D : S.decks) {
+            D->uoff = ubase;
+            for (auto& sp : D->sp) {
+                ub[k++] = ubase;
+                ubase += sp->nunique();
             }
-            rxh.insert(rxh.end(), s.red.ux.begin(), s.red.ux.end());
-            rsid.insert(rsid.end(), s.red.ux.size(), kspec++);
-            for (size_t u = 0; u < s.red.ux.size(); ++u) {
-                for (int m = s.red.offsets[u]; m < s.red.offsets[u + 1]; ++m)
-                    mem.push_back(s.off + s.red.members[m]);
-                off.push_back((int32_t)mem.size());
-            }
-            for (int i = 0; i < s.n(); ++i) uof[s.off + i] = ubase + s.red.unique_of[i];
-            ubase += (int)s.red.ux.size();
+            D->u = ubase - D->uoff;
         }
-        D->u = ubase - D->uoff;
     }
     S.U = ubase;
     const size_t bU = align_up(sizeof(double) * std::max(S.U, 1), 256);
@@ -795,32 +828,38 @@ int layout(Sim& S, bool keep_resident) {
     S.r_mem = S.red.as<int32_t>(3 * bU + bo);
     S.r_uof = S.red.as<int32_t>(3 * bU + bo + bm);
     S.rsid = S.red.as<int32_t>(3 * bU + bo + 2 * bm);  // U <= N
-    if (S.U > 0) {
-        FS_HIP_CHECK(hipMemcpyAsync(S.rx, rxh.data(), sizeof(double) * S.U, hipMemcpyHostToDevice, st));
-        FS_HIP_CHECK(hipMemcpyAsync(S.rsid, rsid.data(), sizeof(int32_t) * S.U,
-                                    hipMemcpyHostToDevice, st));
-    }
     S.drive_h.assign(std::max(nsp, 1), DriveDev{});
     FS_TRY(S.drive.ensure(sizeof(DriveDev) * S.drive_h.size(), st));
     S.drive_d = S.drive.as<DriveDev>();
-    FS_HIP_CHECK(hipMemcpyAsync(S.r_off, off.data(), sizeof(int32_t) * (S.U + 1),
-                                hipMemcpyHostToDevice, st));
-    if (N > 0) {
-        FS_HIP_CHECK(hipMemcpyAsync(S.r_mem, mem.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
-        FS_HIP_CHECK(hipMemcpyAsync(S.r_uof, uof.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
-    }
-    for (const auto& pl : placed) {
-        const Species& s = *pl.first;
-        const amr::Output& a = s.amr_out;
-        const int m = std::max(s.n(), a.U);
+    for (int k = 0; k < nsp; ++k) {
+        const Species& s = *S.sp[k];
+        const int n = s.n(), U = s.nunique();
+        const int m = std::max(n, U);
         if (m == 0) continue;
-        place_map_kernel<<<blocks(m), kBlock, 0, st>>>(s.n(), a.U, s.off, pl.second, a.roff,
-                                                       a.members, a.uof, a.ux, S.r_off, S.r_mem,
-                                                       S.r_uof, S.rx);
+        const int32_t *roff, *members, *uof;
+        const double* ux;
+        if (s.dev_tree) {
+            const amr::Output& a = s.amr_out;
+            roff = a.roff;
+            members = a.members;
+            uof = a.uof;
+            ux = a.ux;
+        } else {
+            const size_t cu = align_up(sizeof(double) * U, 256);
+            const size_t co = align_up(sizeof(int32_t) * (U + 1), 256);
+            const size_t cm = align_up(sizeof(int32_t) * n, 256);
+            ux = s.d_red.as<double>(0);
+            roff = s.d_red.as<int32_t>(cu);
+            members = s.d_red.as<int32_t>(cu + co);
+            uof = s.d_red.as<int32_t>(cu + co + cm);
+        }
+        place_map_kernel<<<blocks(m), kBlock, 0, st>>>(n, U, s.off, ub[k], k, roff, members, uof,
+                                                       ux, S.r_off, S.r_mem, S.r_uof, S.rx,
+                                                       S.rsid);
         FS_HIP_CHECK(hipGetLastError());
     }
-    for (auto& D : S.decks)
-        for (auto& sp : D->sp) FS_TRY(upload_species(S, *D, *sp, sp->d_grid.ptr == nullptr || !keep_resident));
+    set_int_kernel<<<1, 1, 0, st>>>(S.r_off + S.U, N);
+    FS_HIP_CHECK(hipGetLastError());
     FS_TRY(S.status.ensure(sizeof(int) * std::max(nsp, 1), st));
     S.d_status = S.status.as<int>();
     FS_TRY(S.onbuf.ensure(sizeof(int) * std::max(nsp, 1), st));
@@ -884,6 +923,8 @@ int layout(Sim& S, bool keep_resident) {
         full.push_back(none);
         redsg.push_back(none);
     }
+    S.ffull.set_pool(&S.ctx->pool);
+    S.fred.set_pool(&S.ctx->pool);
     FS_TRY(S.ffull.plan(full, N, S.precision, S.ctx->num_cus, S.ctx->tuning, st));
     FS_TRY(S.fred.plan(redsg, S.U, S.precision, S.ctx->num_cus, S.ctx->tuning, st));
     FS_TRY(S.fred.set_shape(1, 8, st));
