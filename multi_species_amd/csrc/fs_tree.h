@@ -59,6 +59,11 @@ struct Built {
     double* e_tree;
     double* mom;
     double* mompart;               // [c.mom_items][degree + 1] chunk partial moments
+    // per node: (sum m_k, sum m_k t_k, min t_k, max t_k) of its moments m_k
+    // at its Chebyshev points t_k; per leaf (at its node): (sum lambda,
+    // sum lambda x, min x, max x) of its points
+    double4* far_sums;
+    double4* leaf_sums;
 };
 
 // Builds the tree of the n points d_x (input order) on `stream`: sort, tree,

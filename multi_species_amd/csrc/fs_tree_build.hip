@@ -445,8 +445,9 @@ int build_device(const double* d_x, int n, double L, double mac, int degree, int
                  b_near = al(4 * std::max<long long>(1, C.near_total)),
                  b_blk = al(sizeof(int4) * std::max<long long>(1, C.blk_total)),
                  b_mom = al(8 * (size_t)C.nnodes * PP), b_x = al(8 * (size_t)n),
-                 b_mp = al(8 * (size_t)std::max(C.mom_items, 1) * PP);
-    FS_TRY(work.lists.ensure(b_far + b_near + b_blk + b_mom + 3 * b_x + b_mp, stream));
+                 b_mp = al(8 * (size_t)std::max(C.mom_items, 1) * PP),
+                 b_ns = al(sizeof(double4) * (size_t)std::max(C.nnodes, 1));
+    FS_TRY(work.lists.ensure(b_far + b_near + b_blk + b_mom + 3 * b_x + b_mp + 2 * b_ns, stream));
     char* q = work.lists.as<char>();
     int32_t* far_list = reinterpret_cast<int32_t*>(q); q += b_far;
     int32_t* near_list = reinterpret_cast<int32_t*>(q); q += b_near;
@@ -455,7 +456,9 @@ int build_device(const double* d_x, int n, double L, double mac, int degree, int
     out.px = reinterpret_cast<double*>(q); q += b_x;
     out.lam = reinterpret_cast<double*>(q); q += b_x;
     out.e_tree = reinterpret_cast<double*>(q); q += b_x;
-    out.mompart = reinterpret_cast<double*>(q);
+    out.mompart = reinterpret_cast<double*>(q); q += b_mp;
+    out.far_sums = reinterpret_cast<double4*>(q); q += b_ns;
+    out.leaf_sums = reinterpret_cast<double4*>(q);
 
     lists_kernel<true><<<grid_for(C.nleaf), kBlock, 0, stream>>>(
         nodes, leaves, d_cnt, L, std::sqrt(mac * mac), PP, far_cnt, near_cnt, blk_cnt, fev, nev,

@@ -16,8 +16,10 @@
 // The treecode kernel -s (1 - sqrt((L^2+4eps^2)/(4 s^2 L^2 + 4 eps^2))) is
 // sqrt(1 + 4 delta^2) times the direct-sum kernel (delta = eps/L), so it is
 // evaluated as z (1/sqrt(z^2+delta^2) - C) with the sums scaled once by
-// sqrt(1 + 4 delta^2)/2 at the end. z = 0 and z = +-1/2 give exactly 0, as the
-// reference's skips do.
+// sqrt(1 + 4 delta^2)/2 at the end (groups that share one wrap take it in x
+// units with their linear part summed once, see tree_sum_kernel). On the
+// per-pair form z = 0 and z = +-1/2 give exactly 0, as the reference's skips
+// do; on the grouped form to rounding.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -48,6 +50,78 @@ __device__ __forceinline__ double kern(double xi, double xj, double invL, double
     const double z = s - __builtin_rint(s);
     const double y = rsq_f64(__builtin_fma(z, z, delta2));
     return z * (y - C);
+}
+
+// x W - M with the product rounded on its own (not fused), as the sums below
+// round theirs.
+__device__ __forceinline__ double lin_part(double x, double W, double M) {
+#pragma clang fp contract(off)
+    return x * W - M;
+}
+
+// The wrap shared by every pair of a group of targets in [t0, t1] and a
+// group of sources in [s0, s1]: 0 (none), 1 (targets one period back), -1
+// (one period on), 2 (no single wrap: per pair). A margin keeps ranges that
+// touch the half period on the per-pair form.
+__device__ __forceinline__ int group_wrap(double t0, double t1, double s0, double s1,
+                                          double invL) {
+    const double lo = (t0 - s1) * invL, hi = (t1 - s0) * invL;
+    constexpr double m = 1e-9;
+    if (hi < 0.5 - m && lo > -0.5 + m) return 0;
+    if (lo > 0.5 + m && hi < 1.5 - m) return 1;
+    if (hi < -0.5 - m && lo > -1.5 + m) return -1;
+    return 2;
+}
+
+// Per node: (sum m_k, sum m_k t_k, min t_k, max t_k) of its moments m_k at
+// its Chebyshev points t_k, k in order.
+__global__ __launch_bounds__(256) void far_sums_kernel(int nnodes, int PP,
+                                                       const double* __restrict__ cheb,
+                                                       const double* __restrict__ mom,
+                                                       double4* __restrict__ fs) {
+#pragma clang fp contract(off)
+    const int node = blockIdx.x * 256 + threadIdx.x;
+    if (node >= nnodes) return;
+    const double* t = cheb + (size_t)node * PP;
+    const double* m = mom + (size_t)node * PP;
+    double a = 0.0, b = 0.0, lo = t[0], hi = t[0];
+    for (int k = 0; k < PP; ++k) {
+        a += m[k];
+        b += m[k] * t[k];
+        lo = fmin(lo, t[k]);
+        hi = fmax(hi, t[k]);
+    }
+    fs[node] = make_double4(a, b, lo, hi);
+}
+
+// Per leaf, by its first block of targets: (sum lambda, sum lambda x, min x,
+// max x) of its points; 64 lanes over strided points, then a fixed-order
+// tree of shuffles (lane 0's result).
+__global__ __launch_bounds__(64) void leaf_sums_kernel(const int4* __restrict__ blocks,
+                                                       const int32_t* __restrict__ leaves,
+                                                       const int2* __restrict__ range,
+                                                       const double* __restrict__ px,
+                                                       const double* __restrict__ lam,
+                                                       double4* __restrict__ ls) {
+#pragma clang fp contract(off)
+    const int4 b = blocks[blockIdx.x];
+    const int node = leaves[b.x];
+    const int2 r = range[node];
+    if (b.y != r.x) return;
+    double a = 0.0, m = 0.0, lo = __builtin_inf(), hi = -__builtin_inf();
+    for (int j = r.x + (int)threadIdx.x; j <= r.y; j += 64) {
+        a += lam[j];
+        m += lam[j] * px[j];
+        lo = fmin(lo, px[j]);
+        hi = fmax(hi, px[j]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off);
+        m += __shfl_down(m, off);
+        lo = fmin(lo, __shfl_down(lo, off));
+        hi = fmax(hi, __shfl_down(hi, off));
+    }
+    if (threadIdx.x == 0) ls[node] = make_double4(a, m, lo, hi);
 }
 
 // Modified Chebyshev moments (:775-866), one workgroup per chunk of
@@ -138,45 +212,97 @@ __global__ __launch_bounds__(256) void moments_reduce_kernel(const int32_t* __re
 
 // One wavefront per (leaf, up to 64 of its targets): blocks[b] = (leaf, first
 // target, last target) in tree order.
+// A far cluster or near leaf whose x range gives every pair with the
+// block's targets the same wrap is evaluated in x units, K = d/sqrt(d^2 +
+// eps^2) - C_x d with d = x_i - x_j (the targets moved by the period where the
+// wrap asks), eps = delta L, C_x = C/L -- the same K as z/sqrt(z^2 + delta^2)
+// - C z -- and its linear part for the whole group at once, C_x (x_i W - M)
+// from the group's sums (far_sums, leaf_sums): no rint and no C per pair.
+// Groups that straddle the half period take the per-pair form.
 __global__ __launch_bounds__(64) void tree_sum_kernel(
     const double* __restrict__ px, const double* __restrict__ lam,
     const int2* __restrict__ range, const int4* __restrict__ blocks,
+    const int32_t* __restrict__ leaves,
     const int32_t* __restrict__ far_off, const int32_t* __restrict__ far_list,
     const int32_t* __restrict__ near_off, const int32_t* __restrict__ near_list,
-    const double* __restrict__ cheb, const double* __restrict__ mom, int PP, double invL,
-    double delta2, double scale, double* __restrict__ e_tree) {
+    const double* __restrict__ cheb, const double* __restrict__ mom,
+    const double4* __restrict__ far_sums, const double4* __restrict__ leaf_sums, int PP,
+    double L, double invL, double delta2, double scale, double* __restrict__ e_tree) {
     const int4 b = blocks[blockIdx.x];
     const int leaf = b.x;
     const int i = b.y + (int)threadIdx.x;
     const bool active = i <= b.z;
     const double C = rsq_f64(__builtin_fma(0.5, 0.5, delta2));
+    const double eps2 = delta2 * (L * L);
+    const double Cx = rsq_f64(__builtin_fma(0.5 * L, 0.5 * L, eps2));
     const double xi = px[active ? i : b.y];
+    // the block's targets lie in its leaf's x range
+    const double4 tb = leaf_sums[leaves[leaf]];
     double far_acc = 0.0;
     for (int f = far_off[leaf]; f < far_off[leaf + 1]; ++f) {
         const int node = far_list[f];
         const double* tk = cheb + (size_t)node * PP;
         const double* mk = mom + (size_t)node * PP;
-        for (int k = 0; k < PP; ++k)
-            far_acc = __builtin_fma(mk[k], kern(xi, tk[k], invL, delta2, C), far_acc);
+        const double4 fs = far_sums[node];
+        const int wrap = group_wrap(tb.z, tb.w, fs.z, fs.w, invL);
+        if (wrap != 2) {
+            const double xs = wrap == 0 ? xi : (wrap == 1 ? xi - L : xi + L);
+            for (int k = 0; k < PP; ++k) {
+                const double d = xs - tk[k];
+                far_acc = __builtin_fma(mk[k], d * rsq_f64(__builtin_fma(d, d, eps2)), far_acc);
+            }
+            far_acc = __builtin_fma(-Cx, lin_part(xs, fs.x, fs.y), far_acc);
+        } else {
+            for (int k = 0; k < PP; ++k)
+                far_acc = __builtin_fma(mk[k], kern(xi, tk[k], invL, delta2, C), far_acc);
+        }
     }
     double near_acc = 0.0;
     for (int q = near_off[leaf]; q < near_off[leaf + 1]; ++q) {
-        const int2 nr = range[near_list[q]];
+        const int node = near_list[q];
+        const int2 nr = range[node];
+        const double4 ls = leaf_sums[node];
+        const int wrap = group_wrap(tb.z, tb.w, ls.z, ls.w, invL);
         int j = nr.x;
-        // eight sources per round: their scalar loads are issued together,
-        // and the terms still add in source order
+        if (wrap != 2) {
+            const double xs = wrap == 0 ? xi : (wrap == 1 ? xi - L : xi + L);
+            // eight sources per round: their scalar loads are issued
+            // together, and the terms still add in source order
+            for (; j + 7 <= nr.y; j += 8) {
+                double xv[8], lv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    xv[u] = px[j + u];
+                    lv[u] = lam[j + u];
+                }
+                double kv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const double d = xs - xv[u];
+                    kv[u] = d * rsq_f64(__builtin_fma(d, d, eps2));
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) near_acc = __builtin_fma(lv[u], kv[u], near_acc);
+            }
+            for (; j <= nr.y; ++j) {
+                const double d = xs - px[j];
+                near_acc = __builtin_fma(lam[j], d * rsq_f64(__builtin_fma(d, d, eps2)), near_acc);
+            }
+            near_acc = __builtin_fma(-Cx, lin_part(xs, ls.x, ls.y), near_acc);
+            continue;
+        }
         for (; j + 7 <= nr.y; j += 8) {
-            double xs[8], ls[8];
+            double xv[8], lv[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                xs[u] = px[j + u];
-                ls[u] = lam[j + u];
+                xv[u] = px[j + u];
+                lv[u] = lam[j + u];
             }
             double kv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) kv[u] = kern(xi, xs[u], invL, delta2, C);
+            for (int u = 0; u < 8; ++u) kv[u] = kern(xi, xv[u], invL, delta2, C);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) near_acc = __builtin_fma(ls[u], kv[u], near_acc);
+            for (int u = 0; u < 8; ++u) near_acc = __builtin_fma(lv[u], kv[u], near_acc);
         }
         for (; j <= nr.y; ++j)
             near_acc = __builtin_fma(lam[j], kern(xi, px[j], invL, delta2, C), near_acc);
@@ -230,9 +356,15 @@ int treecode_f64(const double* d_x, const double* d_w, int n, double L, double e
     }
     const double delta = eps / L, delta2 = delta * delta;
     const double scale = 0.5 * std::sqrt(1 + 4 * delta2);
+    if (T.c.nnodes > 1)
+        far_sums_kernel<<<(unsigned)((T.c.nnodes + 255) / 256), 256, 0, stream>>>(
+            T.c.nnodes, PP, T.cheb, T.mom, T.far_sums);
+    leaf_sums_kernel<<<(unsigned)T.c.blk_total, 64, 0, stream>>>(T.blocks, T.leaves, T.range,
+                                                                  T.px, T.lam, T.leaf_sums);
     tree_sum_kernel<<<(unsigned)T.c.blk_total, 64, 0, stream>>>(
-        T.px, T.lam, T.range, T.blocks, T.far_off, T.far_list, T.near_off, T.near_list, T.cheb,
-        T.mom, PP, 1.0 / L, delta2, scale, T.e_tree);
+        T.px, T.lam, T.range, T.blocks, T.leaves, T.far_off, T.far_list, T.near_off,
+        T.near_list, T.cheb, T.mom, T.far_sums, T.leaf_sums, PP, L, 1.0 / L, delta2, scale,
+        T.e_tree);
     scatter_perm_kernel<<<gb, 256, 0, stream>>>(T.e_tree, T.perm, n, d_es);
     FS_HIP_CHECK(hipGetLastError());
     if (evals) {
