@@ -205,8 +205,20 @@ int fs_emq_periodic_mixed(fs_ctx* ctx, double* es, const double* targets, int64_
  * (every pair in FP64 when none is). At most 1e-3. */
 int fs_emq_set_mixed_target(fs_ctx* ctx, double rel_err);
 
+/* Size below which the mixed entry points return the plain FP64 sum (no
+ * counterpart in the reference). A call with nt * ns < min_pairs evaluates
+ * through fs_emq_periodic_f64(_dev) and returns its result bit for bit: at
+ * such sizes the mixed form's fixed costs (sort, expansions, prefix sums,
+ * finalize: some 20 short launches) exceed the whole FP64 sum. The rule
+ * depends on the call's own sizes only. 0 sends every call through the
+ * mixed kernels; the default is FS_MIXED_MIN_PAIRS_DEFAULT. Step loops and
+ * servers keep their own precision (fs_batch_create, fs_sim_set_options). */
+#define FS_MIXED_MIN_PAIRS_DEFAULT (INT64_C(1) << 29)
+int fs_emq_set_mixed_min_pairs(fs_ctx* ctx, int64_t min_pairs);
+
 /* Launch shape (targets per thread, persistent workgroups per CU) that the
- * last mixed call used, as chosen by its autotuner. Informational. */
+ * last call through the mixed kernels used, as chosen by its autotuner.
+ * Informational. */
 int fs_emq_mixed_shape(fs_ctx* ctx, int* targets_per_thread, int* wgs_per_cu);
 
 /* Same on device-resident arrays, enqueued on `stream`. */

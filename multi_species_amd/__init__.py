@@ -65,6 +65,7 @@ def lib():
         L.fs_emq_periodic_mixed.argtypes = [_vp, _dp, _dp, _i64, _dp, _dp, _i64, _d, _d]
         L.fs_emq_mixed_shape.argtypes = [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int)]
         L.fs_emq_set_mixed_target.argtypes = [_vp, _d]
+        L.fs_emq_set_mixed_min_pairs.argtypes = [_vp, _i64]
         L.fs_emq_periodic_mixed_dev.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _d, _d,
                                                 _vp]
         L.fs_sim_set_timing.argtypes = [_vp, _int]
@@ -253,6 +254,13 @@ class Context:
         band and takes the narrowest one within rel_err * max|E|."""
         _check(lib().fs_emq_set_mixed_target(self.h, rel_err), "fs_emq_set_mixed_target")
 
+    def set_mixed_min_pairs(self, min_pairs: int):
+        """Mixed calls with nt * ns below min_pairs return the FP64 sum bit
+        for bit (faster at such sizes); 0 = every call through the mixed
+        kernels. Default MIXED_MIN_PAIRS_DEFAULT."""
+        _check(lib().fs_emq_set_mixed_min_pairs(self.h, int(min_pairs)),
+               "fs_emq_set_mixed_min_pairs")
+
     def mixed_shape(self):
         """(targets per thread, workgroups per CU) of the last mixed call."""
         t, g = _int(), _int()
@@ -359,6 +367,7 @@ class Deck(ctypes.Structure):
 
 PRECISION_F64 = 0
 PRECISION_MIXED = 1
+MIXED_MIN_PAIRS_DEFAULT = 1 << 29
 
 DIAG_COUNT = 14
 DIAG_NAMES = ("charge", "momentum", "kinetic", "electric", "entropy", "l1", "l2", "min_f",

@@ -61,6 +61,7 @@
 // scheduling.
 //
 // This file holds the C ABI of the single-call form.
+#include <cstring>
 #include <vector>
 
 #include "fs_field_batch.h"
@@ -111,6 +112,11 @@ int fs_emq_periodic_mixed_dev(fs_ctx* ctx, double* d_es, const double* d_targets
     if (nt == 0) return FS_OK;
     if (!d_es || !d_targets || (ns > 0 && (!d_sources || !d_q_ws)))
         return fs::fail(FS_E_INVALID, "mixed direct sum: null pointer");
+    // small calls: the FP64 sum is both exact and faster than the mixed
+    // form's fixed costs
+    if ((double)nt * (double)ns < (double)ctx->mixed_min_pairs)
+        return fs_emq_periodic_f64_dev(ctx, d_es, d_targets, nt, d_sources, d_q_ws, ns, L,
+                                       epsilon, stream);
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     fs::FieldBatch& fb = fs::api_batch(ctx);
     const bool same = (d_targets == d_sources) && (nt == ns);
@@ -170,6 +176,15 @@ int fs_emq_set_mixed_target(fs_ctx* ctx, double rel_err) {
     return FS_OK;
 }
 
+int fs_emq_set_mixed_min_pairs(fs_ctx* ctx, int64_t min_pairs) {
+    if (!ctx) return fs::fail(FS_E_INVALID, "null context");
+    if (min_pairs < 0)
+        return fs::fail(FS_E_INVALID, "mixed min_pairs must be >= 0, got %lld",
+                        (long long)min_pairs);
+    ctx->mixed_min_pairs = min_pairs;
+    return FS_OK;
+}
+
 int fs_emq_mixed_shape(fs_ctx* ctx, int* targets_per_thread, int* wgs_per_cu) {
     if (!ctx) return fs::fail(FS_E_INVALID, "null context");
     fs::FieldBatch& fb = fs::api_batch(ctx);
@@ -187,7 +202,10 @@ int fs_emq_periodic_mixed(fs_ctx* ctx, double* es, const double* targets, int64_
     if (!es || !targets || (ns > 0 && (!sources || !q_ws)))
         return fs::fail(FS_E_INVALID, "null pointer");
     FS_HIP_CHECK(hipSetDevice(ctx->device));
-    const bool same = (targets == sources) && (nt == ns);
+    // targets and sources the same positions (one array or two copies): one
+    // device copy and one shared sort, as in fs_emq_periodic_f64
+    const bool same = nt == ns && (targets == sources ||
+                                   std::memcmp(targets, sources, sizeof(double) * nt) == 0);
     const size_t bt = fs::align_up(sizeof(double) * nt, 256);
     const size_t bs = fs::align_up(sizeof(double) * ns, 256);
     FS_TRY(ctx->io.ensure(2 * bt + 2 * bs, ctx->stream));

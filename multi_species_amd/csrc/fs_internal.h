@@ -177,6 +177,7 @@ struct fs_ctx {
     fs::BlockPool pool;   // blocks of the simulations' buffers (all on `stream`)
     fs::FieldBatch* api_batch = nullptr;  // plan of the last mixed entry-point call
     double mixed_target = 0;              // > 0: mixed entry points hold this error bound
+    int64_t mixed_min_pairs = FS_MIXED_MIN_PAIRS_DEFAULT;  // smaller calls: FP64 sum
     int64_t f64_pair_evals = 0;           // kernel evaluations of the last FP64 direct sum
                                           // (periodic or open)
     // autotuned launch shapes (tpt, workgroups per CU) of the mixed entry

@@ -33,6 +33,15 @@ def test_argument_errors_without_device_work():
     assert lib.fs_external_field_add_dev(None, None, None, 0, 0.0, 0, 0.4, 0.26, 0.37,
                                          None) == -1
     assert b"null" in lib.fs_last_error()
+    assert lib.fs_emq_set_mixed_min_pairs(None, 0) == -1
+
+
+def test_mixed_min_pairs_default_matches_the_header():
+    import re
+    with open(ms.HEADER) as f:
+        m = re.search(r"#define FS_MIXED_MIN_PAIRS_DEFAULT \(INT64_C\(1\) << (\d+)\)", f.read())
+    assert m is not None
+    assert ms.MIXED_MIN_PAIRS_DEFAULT == 1 << int(m.group(1))
 
 
 def test_product_refuses_to_run_without_its_library(monkeypatch, tmp_path):

@@ -17,6 +17,16 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True)
+def mixed_kernels_at_every_size(gpu_ctx):
+    """The cases below check the mixed kernels themselves, also at sizes
+    where the entry points would otherwise return the FP64 sum."""
+    import multi_species_amd as ms
+    gpu_ctx.set_mixed_min_pairs(0)
+    yield
+    gpu_ctx.set_mixed_min_pairs(ms.MIXED_MIN_PAIRS_DEFAULT)
+
 L0 = 4 * np.pi
 TOL = 1e-8
 
@@ -245,3 +255,23 @@ def test_c2_twenty_seeds_full_field(gpu_ctx):
     finally:
         gpu_ctx.set_mixed_target(0.0)
     print("c2 worst over 20 seeds:", {k: f"{a:.2e}/{b:.2e}" for k, (a, b) in worst.items()})
+
+
+@pytest.mark.parametrize("n", [1, 3000, 16770])
+def test_small_calls_return_the_fp64_sum(gpu_ctx, n):
+    """Below the size threshold the mixed entry point returns the FP64 sum bit
+    for bit (targets = sources and separate targets); at threshold 0 the same
+    call goes through the mixed kernels and stays within the mixed bar. A
+    threshold test with the comparison reversed (< against <=) or a fallback
+    to a different FP64 kernel would fail the bitwise check."""
+    x, w = random_deck(n, 7)
+    t = np.mod(x[::-1] + 0.3, L0)
+    e64_same = gpu_ctx.emq_periodic_f64(x, x, w, L0, 0.05)
+    e64_sep = gpu_ctx.emq_periodic_f64(t, x, w, L0, 0.05)
+    gpu_ctx.set_mixed_min_pairs(n * n + 1)
+    assert np.array_equal(gpu_ctx.emq_periodic_mixed(x, x, w, L0, 0.05, same=True), e64_same)
+    assert np.array_equal(gpu_ctx.emq_periodic_mixed(t, x, w, L0, 0.05), e64_sep)
+    gpu_ctx.set_mixed_min_pairs(n * n)  # nt * ns == min_pairs: mixed kernels
+    em = gpu_ctx.emq_periodic_mixed(x, x, w, L0, 0.05, same=True)
+    assert rel(em, e64_same) <= TOL
+    gpu_ctx.set_mixed_min_pairs(0)
